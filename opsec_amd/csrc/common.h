@@ -1,0 +1,149 @@
+// Internal declarations shared by the HIP translation units of libopsec_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "opsec_hip.h"
+
+namespace opsec {
+
+// ---- xi function descriptors ---------------------------------------------
+enum XiKind : int {
+    XI_TABLE = 0,        // QSpline table (reference tables)
+    XI_BAND = 1,         // closed-form top-hat band
+    XI_TEST_EXP = 2,     // A exp(-b r)
+    XI_TEST_GAUSS = 3,   // A exp(-b r^2)
+    XI_TEST_DGAUSS = 4   // A r exp(-b r^2)
+};
+
+struct XiDesc {
+    int kind;
+    int table;           // row of the table array (XI_TABLE)
+    double a, b;         // band [klo, khi] or test (A, b)
+};
+
+// QSpline geometry of the reference tables (src/RealModel.cpp:98-127):
+// 512 linear nodes over [0,200) then 256 log nodes over [200,20000].
+constexpr int QS_N1 = 512;
+constexpr int QS_N2 = 256;
+constexpr int QS_N = QS_N1 + QS_N2;
+constexpr double QS_X1 = 200.0;
+constexpr double QS_X2 = 20000.0;
+
+// Device view of the model: descriptors + tables.
+struct XiModelDev {
+    const XiDesc* desc;     // nf descriptors
+    const double* y;        // ntables x QS_N values
+    const double* xgrid;    // QS_N r nodes (first QS_N1+1 are the linear part)
+    const double* ugrid;    // QS_N2 log-r nodes
+    double x2, uhi;         // xgrid[QS_N-1], ugrid[QS_N2-1]
+};
+
+// Survey geometry for pair integrals on the device.
+struct SurveyDev {
+    int coordsys;
+    int n1, n2, n3;
+    double min1, max1, min2, max2, min3, max3;
+    double L;
+};
+
+// ---- context -------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct MlpState;  // surrogate network (mlp.hip)
+struct ProjState; // projection results (project.hip)
+
+}  // namespace opsec
+
+struct opsec_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int num_cus = 0;
+    std::string arch;
+
+    // model
+    int model_kind = -1;     // 0 real, 1 test
+    int nparams = 0;
+    std::vector<opsec::XiDesc> funcs;     // nparams + 1
+    std::vector<double> params;           // prior p_n
+    opsec::XiDesc* d_desc = nullptr;
+    double* d_tables = nullptr;           // ntables x QS_N
+    double* d_xgrid = nullptr;
+    double* d_ugrid = nullptr;
+    std::vector<double> h_xgrid, h_ugrid;
+    int ntables = 0;
+    double pk_kmin = 0, pk_kmax = 0;      // pkfile range (for band params)
+    double* d_pk = nullptr;               // pk spline: X,Y,b,c,d (5 x npk)
+    int npk = 0;
+    std::vector<double> band_lo, band_hi;
+
+    // survey (device-resident)
+    opsec::SurveyDev sv{};
+    int ncells = 0;
+    int nkeys = 0;
+    int* d_cellidx = nullptr;             // ncells x 3 grid indices
+    double* d_nbar = nullptr;             // ncells Nbar
+    std::vector<opsec_cell> h_cells;
+    double* d_Q = nullptr;                // (nparams+1) x nkeys pair tables
+    std::vector<char> Q_valid;
+    int64_t st_integrals = 0, st_nodes = 0, st_surrogate = 0, st_spot = 0, st_spot_fail = 0;
+
+    // modes
+    int nmodes = 0;
+    double* d_B = nullptr;                // ncells x nmodes, column-major
+    opsec::ProjState* proj = nullptr;
+    opsec::MlpState* mlp = nullptr;
+
+    std::vector<void*> user_bufs;
+};
+
+namespace opsec {
+
+int set_error(opsec_ctx* ctx, int code, const char* fmt, ...);
+
+#define OPSEC_HIP_TRY(ctx, call)                                                        \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return ::opsec::set_error((ctx), OPSEC_EHIP, "%s failed: %s (%s:%d)", #call, \
+                                      hipGetErrorString(e_), __FILE__, __LINE__);       \
+    } while (0)
+
+#define OPSEC_CHECK_ARG(ctx, cond, ...)                                         \
+    do {                                                                        \
+        if (!(cond)) return ::opsec::set_error((ctx), OPSEC_EINVAL, __VA_ARGS__); \
+    } while (0)
+
+// Allocate (or re-allocate to at least `bytes`) a device buffer owned by ctx.
+int dev_realloc(opsec_ctx* ctx, void** p, size_t* cap, size_t bytes);
+void dev_free(void** p);
+
+// Model view for kernels.
+XiModelDev model_view(const opsec_ctx* ctx);
+
+// Gauss rules (host): n-point Gauss-Legendre on [-1,1]; n-point Gauss rule for
+// weight (1-t) on [0,1].
+void gauss_legendre(int n, double* x, double* w);
+void gauss_jacobi10(int n, double* t, double* w);
+
+// Survey checks / conversions.
+int survey_from_abi(opsec_ctx* ctx, const opsec_survey* in, SurveyDev* out);
+
+// Pair-table builder shared by sig_block and pair_tables: fills d_Q rows for
+// the functions in funcs (K1).
+int build_pair_tables(opsec_ctx* ctx, int nf, const int* funcs, const opsec_quad* q);
+
+// Rows of the packed upper triangle: column j starts at j(j+1)/2.
+inline int64_t packed_size(int64_t m) { return m * (m + 1) / 2; }
+
+}  // namespace opsec

@@ -1,0 +1,331 @@
+// K4: Fisher matrix, bias and quadratic estimates from the packed projected
+// matrices, then the estimate post-processing (src/estimate-mpi.cpp:480-654).
+//
+// Over the packed upper triangle k = j(j+1)/2 + i (i <= j), with
+// C = diag(1 + lambda) and w_k = (i == j ? 1/2 : 1) / (C_i C_j):
+//   F_mn = sum_k w_k D_m[k] D_n[k]
+//   f_n  = sum_k w_k D_n[k] R[k],      R[k] = delta_ij C_i - sum_n p_n D_n[k]
+//   q_n  = sum_k w_k D_n[k] y_i y_j
+// i.e. one Gram product G = X D^T with X = [w D_0 .. w D_{nb-1}, w R, w Y]
+// (nb + 2 rows, padded to 48) and D^T (padded to 48 columns), run on MFMA
+// (FP64 16x16x4, or FP32 16x16x4 for the FP32 traces) with workgroup partial
+// sums reduced by atomics.  The 48 x 48 result is tiny; the inverse / matrix
+// square root / window normalisation on nb x nb runs on the host in FP64.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+#include "gemm.h"
+
+namespace opsec {
+
+constexpr int FK_TILE = 128;   // packed entries per workgroup iteration
+constexpr int FG = 48;         // padded Gram size
+
+template <typename T>
+struct FMfma;
+template <>
+struct FMfma<double> {
+    typedef d4_t acc_t;
+    __device__ static acc_t mma(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+};
+template <>
+struct FMfma<float> {
+    typedef f4_t acc_t;
+    __device__ static acc_t mma(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+};
+
+// column j of packed index k (k = j(j+1)/2 + i, 0 <= i <= j)
+__device__ __forceinline__ int packed_col(long long k) {
+    long long j = (long long)((sqrt(8.0 * (double)k + 1.0) - 1.0) * 0.5);
+    while (j * (j + 1) / 2 > k) j--;
+    while ((j + 1) * (j + 2) / 2 <= k) j++;
+    return (int)j;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void fisher_gram_kernel(const double* D, long long K, int nb, const double* lambda,
+                                                          const double* y, const double* prior, double* G) {
+    typedef typename FMfma<T>::acc_t acc_t;
+    __shared__ T X[FG][FK_TILE + 4];     // scaled rows
+    __shared__ T Dt[FG][FK_TILE + 4];    // plain rows
+    __shared__ double wk[FK_TILE], rk[FK_TILE], yk[FK_TILE];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    acc_t acc[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) acc[i][j] = acc_t{0, 0, 0, 0};
+    for (long long k0 = (long long)blockIdx.x * FK_TILE; k0 < K; k0 += (long long)gridDim.x * FK_TILE) {
+        __syncthreads();
+        // weights, R diagonal part and y_i y_j per packed entry
+        for (int t = threadIdx.x; t < FK_TILE; t += blockDim.x) {
+            long long k = k0 + t;
+            double w = 0, r = 0, yy = 0;
+            if (k < K) {
+                int j = packed_col(k);
+                int i = (int)(k - (long long)j * (j + 1) / 2);
+                double ci = 1.0 + lambda[i], cj = 1.0 + lambda[j];
+                w = (i == j ? 0.5 : 1.0) / (ci * cj);
+                r = (i == j) ? ci : 0.0;
+                yy = y[i] * y[j];
+            }
+            wk[t] = w;
+            rk[t] = r;
+            yk[t] = yy;
+        }
+        // load D rows (coalesced along k)
+        for (int e = threadIdx.x; e < nb * FK_TILE; e += blockDim.x) {
+            int row = e / FK_TILE, t = e % FK_TILE;
+            long long k = k0 + t;
+            double v = (k < K) ? D[(size_t)row * K + k] : 0.0;
+            Dt[row][t] = (T)v;
+        }
+        for (int e = threadIdx.x; e < (FG - nb) * FK_TILE; e += blockDim.x) {
+            int row = nb + e / FK_TILE, t = e % FK_TILE;
+            Dt[row][t] = (T)0;
+        }
+        __syncthreads();
+        // residual R = delta_ij C_i - sum_n p_n D_n (in FP64 from the FP64 inputs)
+        for (int t = threadIdx.x; t < FK_TILE; t += blockDim.x) {
+            long long k = k0 + t;
+            double r = rk[t];
+            if (k < K)
+                for (int n = 0; n < nb; n++) r -= prior[n] * D[(size_t)n * K + k];
+            rk[t] = r;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < FG * FK_TILE; e += blockDim.x) {
+            int row = e / FK_TILE, t = e % FK_TILE;
+            double v;
+            if (row < nb) v = wk[t] * (double)Dt[row][t];
+            else if (row == nb) v = wk[t] * rk[t];
+            else if (row == nb + 1) v = wk[t] * yk[t];
+            else v = 0.0;
+            X[row][t] = (T)v;
+        }
+        __syncthreads();
+        // each wave takes 32 entries = 8 k-steps of 4
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const int kk = wave * 32 + s * 4 + (lane >> 4);
+            T a[3], b[3];
+            for (int i = 0; i < 3; i++) a[i] = X[i * 16 + (lane & 15)][kk];
+            for (int j = 0; j < 3; j++) b[j] = Dt[j * 16 + (lane & 15)][kk];
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) acc[i][j] = FMfma<T>::mma(a[i], b[j], acc[i][j]);
+        }
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int r = 0; r < 4; r++) {
+                int row = i * 16 + 4 * (lane >> 4) + r, col = j * 16 + (lane & 15);
+                atomicAdd(&G[row * FG + col], (double)acc[i][j][r]);
+            }
+}
+
+// ---- host FP64 dense helpers for nb x nb ----------------------------------
+
+// Inverse by LU with partial pivoting (the reference uses LAPACK GETRF/GETRI,
+// src/MyMatrix.hpp:531-569).  Returns false if singular.
+static bool lu_inverse(int n, const std::vector<double>& A, std::vector<double>& Ainv) {
+    std::vector<double> LU(A);
+    std::vector<int> piv(n);
+    for (int c = 0; c < n; c++) {
+        int p = c;
+        for (int r = c + 1; r < n; r++)
+            if (std::fabs(LU[r * n + c]) > std::fabs(LU[p * n + c])) p = r;
+        piv[c] = p;
+        if (LU[p * n + c] == 0.0) return false;
+        if (p != c)
+            for (int j = 0; j < n; j++) std::swap(LU[c * n + j], LU[p * n + j]);
+        for (int r = c + 1; r < n; r++) {
+            LU[r * n + c] /= LU[c * n + c];
+            double l = LU[r * n + c];
+            for (int j = c + 1; j < n; j++) LU[r * n + j] -= l * LU[c * n + j];
+        }
+    }
+    Ainv.assign((size_t)n * n, 0.0);
+    for (int col = 0; col < n; col++) {
+        std::vector<double> b(n, 0.0);
+        b[col] = 1.0;
+        for (int c = 0; c < n; c++) std::swap(b[c], b[piv[c]]);
+        for (int r = 0; r < n; r++)
+            for (int j = 0; j < r; j++) b[r] -= LU[r * n + j] * b[j];
+        for (int r = n - 1; r >= 0; r--) {
+            for (int j = r + 1; j < n; j++) b[r] -= LU[r * n + j] * b[j];
+            b[r] /= LU[r * n + r];
+        }
+        for (int r = 0; r < n; r++) Ainv[r * n + col] = b[r];
+    }
+    return true;
+}
+
+// Principal square root of a symmetric matrix through its eigenbasis
+// (src/MyMatrix.hpp:478-527), eigenvectors by cyclic Jacobi.
+static void sym_sqrtm(int n, const std::vector<double>& A0, std::vector<double>& out) {
+    std::vector<double> A(A0), V((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) V[i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0, tot = 0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                tot += A[i * n + j] * A[i * n + j];
+                if (i != j) off += A[i * n + j] * A[i * n + j];
+            }
+        if (off <= 1e-32 * tot) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                double apq = A[p * n + q];
+                if (std::fabs(apq) < 1e-300) continue;
+                double th = (A[q * n + q] - A[p * n + p]) / (2 * apq);
+                double t = (th >= 0 ? 1 : -1) / (std::fabs(th) + std::sqrt(th * th + 1));
+                double c = 1 / std::sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < n; k++) {
+                    double akp = A[k * n + p], akq = A[k * n + q];
+                    A[k * n + p] = c * akp - s * akq;
+                    A[k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {
+                    double apk = A[p * n + k], aqk = A[q * n + k];
+                    A[p * n + k] = c * apk - s * aqk;
+                    A[q * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++) {
+                    double vkp = V[k * n + p], vkq = V[k * n + q];
+                    V[k * n + p] = c * vkp - s * vkq;
+                    V[k * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    out.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            double s = 0;
+            for (int k = 0; k < n; k++) s += V[i * n + k] * std::sqrt(A[k * n + k]) * V[j * n + k];
+            out[i * n + j] = s;
+        }
+}
+
+static int post_process(opsec_ctx* ctx, int nb, const std::vector<double>& G, int mixing, opsec_estimate* out) {
+    std::vector<double> F((size_t)nb * nb), f(nb), q(nb);
+    for (int m = 0; m < nb; m++)
+        for (int n = 0; n < nb; n++) F[m * nb + n] = 0.5 * (G[m * FG + n] + G[n * FG + m]);
+    for (int n = 0; n < nb; n++) {
+        f[n] = G[nb * FG + n];
+        q[n] = G[(nb + 1) * FG + n];
+    }
+    std::vector<double> M((size_t)nb * nb, 0.0), W((size_t)nb * nb);
+    if (mixing == OPSEC_MIX_INVERSE || mixing == OPSEC_MIX_INVERSE_SQRT) {
+        if (!lu_inverse(nb, F, M)) return set_error(ctx, OPSEC_ECONV, "Fisher matrix is singular");
+        if (mixing == OPSEC_MIX_INVERSE_SQRT) {
+            std::vector<double> Ms;
+            sym_sqrtm(nb, M, Ms);
+            M.swap(Ms);
+        }
+    } else {
+        for (int i = 0; i < nb; i++) M[i * nb + i] = 1.0;
+    }
+    auto mul = [&](const std::vector<double>& A, const std::vector<double>& B, std::vector<double>& Cm) {
+        Cm.assign((size_t)nb * nb, 0.0);
+        for (int i = 0; i < nb; i++)
+            for (int k = 0; k < nb; k++)
+                for (int j = 0; j < nb; j++) Cm[i * nb + j] += A[i * nb + k] * B[k * nb + j];
+    };
+    mul(M, F, W);
+    for (int m = 0; m < nb; m++) {
+        double rs = 0;
+        for (int n = 0; n < nb; n++) rs += W[m * nb + n];
+        for (int n = 0; n < nb; n++) M[m * nb + n] /= rs;
+    }
+    mul(M, F, W);
+    std::vector<double> phat(nb, 0.0), cov((size_t)nb * nb, 0.0);
+    for (int i = 0; i < nb; i++)
+        for (int k = 0; k < nb; k++) phat[i] += M[i * nb + k] * (q[k] - f[k]);
+    for (int i = 0; i < nb; i++)
+        for (int j = 0; j < nb; j++) {
+            double s = 0;
+            for (int k = 0; k < nb; k++) s += W[i * nb + k] * M[j * nb + k];
+            cov[i * nb + j] = s;
+        }
+    auto cp = [](double* dst, const std::vector<double>& v) {
+        if (dst) std::memcpy(dst, v.data(), v.size() * 8);
+    };
+    cp(out->phat, phat);
+    cp(out->cov, cov);
+    cp(out->F, F);
+    cp(out->W, W);
+    cp(out->M, M);
+    cp(out->q, q);
+    cp(out->f, f);
+    return OPSEC_OK;
+}
+
+static int run_fisher(opsec_ctx* ctx, const double* dD, int M, int nb, const double* lambda, const double* y,
+                      const double* prior, int mixing, int prec, opsec_estimate* out) {
+    OPSEC_CHECK_ARG(ctx, nb >= 1 && nb + 2 <= FG, "Nparams must be 1..%d", FG - 2);
+    OPSEC_CHECK_ARG(ctx, M >= 1, "Nmodes must be positive");
+    OPSEC_CHECK_ARG(ctx, mixing >= 0 && mixing <= 2, "unknown mixing mode %d", mixing);
+    OPSEC_CHECK_ARG(ctx, prec == OPSEC_PREC_FP64 || prec == OPSEC_PREC_FP32 || prec == OPSEC_PREC_BF16X3,
+                    "unknown precision %d", prec);
+    OPSEC_CHECK_ARG(ctx, lambda && y && prior && out, "NULL input");
+    long long K = (long long)M * (M + 1) / 2;
+    double *d_aux = nullptr, *d_G = nullptr;
+    OPSEC_HIP_TRY(ctx, hipMalloc(&d_aux, ((size_t)2 * M + nb) * 8));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&d_G, FG * FG * 8));
+    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_aux, lambda, M * 8, hipMemcpyHostToDevice, ctx->stream));
+    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_aux + M, y, M * 8, hipMemcpyHostToDevice, ctx->stream));
+    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_aux + 2 * M, prior, nb * 8, hipMemcpyHostToDevice, ctx->stream));
+    OPSEC_HIP_TRY(ctx, hipMemsetAsync(d_G, 0, FG * FG * 8, ctx->stream));
+    long long tiles = (K + FK_TILE - 1) / FK_TILE;
+    int grid = (int)std::min<long long>(tiles, (long long)ctx->num_cus * 8);
+    if (prec == OPSEC_PREC_FP64)
+        hipLaunchKernelGGL(fisher_gram_kernel<double>, dim3(grid), dim3(256), 0, ctx->stream, dD, K, nb, d_aux,
+                           d_aux + M, d_aux + 2 * M, d_G);
+    else
+        hipLaunchKernelGGL(fisher_gram_kernel<float>, dim3(grid), dim3(256), 0, ctx->stream, dD, K, nb, d_aux,
+                           d_aux + M, d_aux + 2 * M, d_G);
+    OPSEC_HIP_TRY(ctx, hipGetLastError());
+    std::vector<double> G(FG * FG);
+    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(G.data(), d_G, FG * FG * 8, hipMemcpyDeviceToHost, ctx->stream));
+    OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    hipFree(d_aux);
+    hipFree(d_G);
+    return post_process(ctx, nb, G, mixing, out);
+}
+
+void* proj_device_results(opsec_ctx* ctx, int* nb, int* M);
+
+}  // namespace opsec
+
+using namespace opsec;
+
+extern "C" {
+
+int opsec_estimate_run(opsec_ctx* ctx, int nmodes, int nparams, const double* lambda, const double* D, const double* y,
+                       const double* prior, int mixing, int prec, opsec_estimate* out) {
+    if (!ctx) return OPSEC_EINVAL;
+    OPSEC_CHECK_ARG(ctx, D, "NULL covariance derivatives");
+    OPSEC_CHECK_ARG(ctx, nmodes >= 1 && nparams >= 1, "bad sizes");
+    OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    size_t bytes = (size_t)nparams * packed_size(nmodes) * 8;
+    double* dD = nullptr;
+    OPSEC_HIP_TRY(ctx, hipMalloc(&dD, bytes));
+    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dD, D, bytes, hipMemcpyHostToDevice, ctx->stream));
+    int rc = run_fisher(ctx, dD, nmodes, nparams, lambda, y, prior, mixing, prec, out);
+    hipStreamSynchronize(ctx->stream);
+    hipFree(dD);
+    return rc;
+}
+
+int opsec_estimate_device(opsec_ctx* ctx, const double* lambda, const double* y, const double* prior, int mixing,
+                          int prec, opsec_estimate* out) {
+    if (!ctx) return OPSEC_EINVAL;
+    int nb = 0, M = 0;
+    void* dP = proj_device_results(ctx, &nb, &M);
+    if (!dP) return set_error(ctx, OPSEC_ESTATE, "no projection results on the device");
+    OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    return run_fisher(ctx, (const double*)dP, M, nb, lambda, y, prior, mixing, prec, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+// MFMA GEMM building blocks for gfx950 (wave64).
+//
+//   C = A * B,  A: M x K, B: K x N, with the A operand either loaded from a
+//   dense matrix or generated on the fly from the pair table of a box survey
+//   (A[a][b] = Q[key(a,b)], the signal matrix without its sqrt(Nbar) factors),
+//   so that C = T B' never reads an N x N matrix from HBM.
+//
+// Workgroup tile 128 x 128 x BK, 256 threads = 4 waves, each wave owning a
+// 64 x 64 sub-tile as 4 x 4 MFMA fragments of 16 x 16.  Operand tiles are
+// staged through LDS k-major (As[k][m], Bs[k][n]) so that the 16 lanes of a
+// fragment row read consecutive words; the next tile's global loads are issued
+// before the current tile's MFMAs (register double buffering).
+//
+// Fragment layouts (verified by the MFMA layout probe on gfx950):
+//   f64 16x16x4 / f32 16x16x4:  A lane l -> A[l%16][l/16],  B lane l -> B[l/16][l%16],
+//                               D lane l, item r -> D[4(l/16)+r][l%16]
+//   bf16 16x16x32:              A lane l -> A[l%16][8(l/16)+e], B lane l -> B[8(l/16)+e][l%16],
+//                               D as above.
+#pragma once
+
+#include "common.h"
+
+namespace opsec {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+
+constexpr int GT_M = 128, GT_N = 128, GT_THREADS = 256;
+
+// ---- operand sources -------------------------------------------------------
+// Each source exposes  T get(m_or_n, k)  with bounds handled by the caller,
+// and a flag telling which index is contiguous in memory.
+
+struct SrcDenseRowMajor {      // X[i][k] = p[i*ld + k]   (k contiguous)
+    const double* p;
+    size_t ld;
+    static constexpr bool k_contig = true;
+    __device__ __forceinline__ double get(int i, int k) const { return p[(size_t)i * ld + k]; }
+};
+
+struct SrcDenseColMajor {      // X[i][k] = p[k*ld + i]   (i contiguous)
+    const double* p;
+    size_t ld;
+    static constexpr bool k_contig = false;
+    __device__ __forceinline__ double get(int i, int k) const { return p[(size_t)k * ld + i]; }
+};
+
+struct SrcKColMajor {          // X[i][k] = p[i*ld + k], i.e. a column-major K x N operand read as [n][k]
+    const double* p;
+    size_t ld;
+    static constexpr bool k_contig = true;
+    __device__ __forceinline__ double get(int i, int k) const { return p[(size_t)i * ld + k]; }
+};
+
+// Signal matrix of a box survey generated from its pair table.
+struct SrcPairTable {
+    const double* Q;           // pair table of one function
+    const int* cellidx;        // ncells x 3 grid indices
+    int Ny, Nz;
+    static constexpr bool k_contig = true;
+    __device__ __forceinline__ double get(int a, int b) const {
+        const int* ca = cellidx + 3 * a;
+        const int* cb = cellidx + 3 * b;
+        int key = (abs(ca[0] - cb[0]) * Ny + abs(ca[1] - cb[1])) * Nz + abs(ca[2] - cb[2]);
+        return Q[key];
+    }
+};
+
+}  // namespace opsec

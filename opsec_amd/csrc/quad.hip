@@ -1,0 +1,995 @@
+// K1 pair quadrature and K2 signal-matrix assembly.
+//
+// Pair integrals Q_ab = mean of xi(|x - y|) over x in cell a, y in cell b are
+// cached by the reference's translation key (src/sig.cpp:94-123):
+//   box:   key = (|dx_a-dx_b| Ny + |dy_a-dy_b|) Nz + |dz_a-dz_b|
+// and each distinct key is integrated once, on the first (row, col) pair that
+// carries it in row-major order, with the lower cell index first
+// (src/sig.cpp:114-118).  The first pairs are found on the device with one
+// atomicMin per (row, col).
+//
+// Methods:
+//  GM     one wavefront per integral: the Genz-Malik degree-7/5 embedded rule
+//         (Genz & Malik 1980, 149 nodes in 6-D) with h-adaptive bisection of
+//         the worst regions, max-error binary heap and Gladwell's batch
+//         selection, run with the reference's arguments (epsrel lands in the
+//         absolute slot, epsabs in the relative slot: src/sig.cpp:427-430 vs
+//         src/cubature.h:70-73).  The 149 values are summed by one lane in the
+//         reference's node order so that equal xi values give equal sums.
+//  GAUSS  exact tent-kernel reduction of the 6-D cube-pair mean to 3-D:
+//           Q = int xi(|mi(D + s)|) prod_i T(s_i) ds,  T(s) = (1-|s|/h)/h,
+//         integrated by an n/2-point Gauss rule for weight (1-t) on each half
+//         of every axis (n^3 nodes), one thread per integral; pairs with
+//         max|dd| < near use a composite rule (4 panels x 8 Gauss-Legendre
+//         nodes per half axis, 64^3 nodes) reduced over a workgroup.
+//
+// This file is compiled with -ffp-contract=off: the reference's arithmetic is
+// plain IEEE multiply/add (x86-64 baseline code has no FMA).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "common.h"
+#include "xi_dev.h"
+
+namespace opsec {
+
+// ---- Genz-Malik constants (Genz & Malik, SIAM J. Numer. Anal. 20, 1983) -----
+// lambda2^2 = 9/70, lambda4^2 = 9/10, lambda5^2 = 9/19 (correctly rounded).
+#define GM_L2 0.35856858280031809199
+#define GM_L4 0.94868329805051379960
+#define GM_L5 0.68824720161168529772
+constexpr int GM_NPTS = 149;   // 1 + 2*6 + 2*6 + 2*6*5 + 2^6
+
+struct GMJob {
+    int f;
+    int key;
+    double lo[6], hi[6];   // integrand bounds in unit-cube coordinate order
+};
+
+struct GaussJob {
+    int f;
+    int key;
+    double D[3];           // centre offset, cell 2 minus cell 1
+    double h[3];           // cell widths (equal for both cells)
+};
+
+struct GMParams {
+    double absTol, relTol;
+    unsigned maxEval;
+    int rmax;              // region pool size per wave
+    int coordsys;
+    double L;
+};
+
+struct TentRule {
+    int n;                 // nodes per axis
+    double s[64];          // unit offsets in [-1,1] (times h)
+    double w[64];          // weights, sum 1
+};
+
+// ---- GM: one wavefront per integral ----------------------------------------
+
+struct GMSlot {
+    double* c;       // 6 x R centres
+    double* hw;      // 6 x R half-widths
+    double* val;     // R
+    double* err;     // R
+    int* split;      // R
+    int* heap;       // R (region indices)
+    int* list;       // R
+};
+
+__device__ __forceinline__ GMSlot gm_slot(double* dpool, int* ipool, int slot, int R) {
+    GMSlot s;
+    double* d = dpool + (size_t)slot * 14 * R;
+    int* q = ipool + (size_t)slot * 3 * R;
+    s.c = d;
+    s.hw = d + 6 * R;
+    s.val = d + 12 * R;
+    s.err = d + 13 * R;
+    s.split = q;
+    s.heap = q + R;
+    s.list = q + 2 * R;
+    return s;
+}
+
+__device__ __forceinline__ double gm_integrand(const XiModelDev& m, const XiDesc& xd, const GMJob& J,
+                                               const GMParams& P, const double* u) {
+    if (P.coordsys == OPSEC_COORD_CARTESIAN) {
+        // src/sig.cpp:388-414
+        double x1 = (1 - u[0]) * J.lo[0] + u[0] * J.hi[0];
+        double y1 = (1 - u[1]) * J.lo[1] + u[1] * J.hi[1];
+        double z1 = (1 - u[2]) * J.lo[2] + u[2] * J.hi[2];
+        double x2 = (1 - u[3]) * J.lo[3] + u[3] * J.hi[3];
+        double y2 = (1 - u[4]) * J.lo[4] + u[4] * J.hi[4];
+        double z2 = (1 - u[5]) * J.lo[5] + u[5] * J.hi[5];
+        return xi_eval(m, xd, box_r(x1, y1, z1, x2, y2, z2, P.L));
+    }
+    // src/sig.cpp:436-462: lo/hi = {r1^3, r2^3, mu1, mu2, phi1, phi2}
+    double r1 = cbrt((1 - u[0]) * J.lo[0] + u[0] * J.hi[0]);
+    double r2 = cbrt((1 - u[1]) * J.lo[1] + u[1] * J.hi[1]);
+    double mu1 = (1 - u[2]) * J.lo[2] + u[2] * J.hi[2];
+    double mu2 = (1 - u[3]) * J.lo[3] + u[3] * J.hi[3];
+    double ph1 = (1 - u[4]) * J.lo[4] + u[4] * J.hi[4];
+    double ph2 = (1 - u[5]) * J.lo[5] + u[5] * J.hi[5];
+    return xi_eval(m, xd, sph_r(r1, mu1, ph1, r2, mu2, ph2));
+}
+
+// Node p of the rule in the reference's node order (src/cubature.c:347-443).
+__device__ __forceinline__ void gm_node(int p, const double* c, const double* hw, double* u) {
+    for (int d = 0; d < 6; d++) u[d] = c[d];
+    if (p == 0) return;
+    if (p <= 24) {
+        int q = p - 1, i = q >> 2, s = q & 3;
+        double r = hw[i] * (s < 2 ? GM_L2 : GM_L4);
+        u[i] = (s & 1) ? c[i] + r : c[i] - r;
+        return;
+    }
+    if (p <= 84) {
+        int q = p - 25, pair = q >> 2, s = q & 3;
+        int i = 0, j = 1, k = pair;
+        for (i = 0; i < 5; i++) {
+            int cnt = 5 - i;
+            if (k < cnt) { j = i + 1 + k; break; }
+            k -= cnt;
+        }
+        double ri = hw[i] * GM_L4, rj = hw[j] * GM_L4;
+        // order: (i-, j-), (i+, j-), (i+, j+), (i-, j+)
+        bool ip = (s == 1 || s == 2), jp = (s >= 2);
+        u[i] = ip ? c[i] + ri : c[i] - ri;
+        u[j] = jp ? c[j] + rj : c[j] - rj;
+        return;
+    }
+    int g = p - 85;
+    int signs = g ^ (g >> 1);   // Gray-code walk starting from all '+'
+    for (int d = 0; d < 6; d++) {
+        double r = hw[d] * GM_L5;
+        u[d] = ((signs >> d) & 1) ? c[d] - r : c[d] + r;
+    }
+}
+
+// Evaluate region ri: all lanes compute nodes, lane 0 sums in node order.
+__device__ void gm_eval_region(const XiModelDev& m, const XiDesc& xd, const GMJob& J, const GMParams& P,
+                               GMSlot& S, int R, int ri, double* vals) {
+    const int lane = threadIdx.x;
+    double c[6], hw[6];
+    for (int d = 0; d < 6; d++) {
+        c[d] = S.c[d * R + ri];
+        hw[d] = S.hw[d * R + ri];
+    }
+    for (int p = lane; p < GM_NPTS; p += 64) {
+        double u[6];
+        gm_node(p, c, hw, u);
+        vals[p] = gm_integrand(m, xd, J, P, u);
+    }
+    __syncthreads();
+    if (lane == 0) {
+        const double ratio = (GM_L2 * GM_L2) / (GM_L4 * GM_L4);
+        const double w1 = (double)(12824 - 9120 * 6 + 400 * 36) / 19683.0;
+        const double w2 = 980. / 6561.;
+        const double w3 = (double)(1820 - 400 * 6) / 19683.0;
+        const double w4 = 200. / 19683.;
+        const double w5 = 6859.0 / 19683.0 / 64.0;
+        const double e1 = (double)(729 - 950 * 6 + 50 * 36) / 729.0;
+        const double e2 = 245. / 486.;
+        const double e3 = (double)(265 - 100 * 6) / 1458.0;
+        const double e4 = 25. / 729.;
+        double val0 = vals[0], sum2 = 0, sum3 = 0, sum4 = 0, sum5 = 0;
+        double maxdiff = 0;
+        int split = 0;
+        for (int k = 0; k < 6; k++) {
+            double v0 = vals[1 + 4 * k], v1 = vals[2 + 4 * k], v2 = vals[3 + 4 * k], v3 = vals[4 + 4 * k];
+            sum2 += v0 + v1;
+            sum3 += v2 + v3;
+            double diff = 0.0 + fabs(v0 + v1 - 2 * val0 - ratio * (v2 + v3 - 2 * val0));
+            if (diff > maxdiff) {
+                maxdiff = diff;
+                split = k;
+            }
+        }
+        for (int k = 0; k < 60; k++) sum4 += vals[25 + k];
+        for (int k = 0; k < 64; k++) sum5 += vals[85 + k];
+        double vol = 1;
+        for (int d = 0; d < 6; d++) vol *= 2 * hw[d];
+        double result = vol * (w1 * val0 + w2 * sum2 + w3 * sum3 + w4 * sum4 + w5 * sum5);
+        double res5th = vol * (e1 * val0 + e2 * sum2 + e3 * sum3 + e4 * sum4);
+        S.val[ri] = result;
+        S.err[ri] = fabs(res5th - result);
+        S.split[ri] = split;
+    }
+    __syncthreads();
+}
+
+// Binary max-heap on region errors (Cormen-Leiserson-Rivest), lane 0 only.
+__device__ void heap_push(GMSlot& S, int& n, int ri, double& tv, double& te) {
+    tv += S.val[ri];
+    te += S.err[ri];
+    int insert = n++;
+    double key = S.err[ri];
+    while (insert) {
+        int parent = (insert - 1) / 2;
+        if (key <= S.err[S.heap[parent]]) break;
+        S.heap[insert] = S.heap[parent];
+        insert = parent;
+    }
+    S.heap[insert] = ri;
+}
+
+__device__ int heap_pop(GMSlot& S, int& n, double& tv, double& te) {
+    int ret = S.heap[0];
+    int i = 0, child;
+    S.heap[0] = S.heap[--n];
+    while ((child = i * 2 + 1) < n) {
+        int largest = (S.err[S.heap[child]] <= S.err[S.heap[i]]) ? i : child;
+        if (++child < n && S.err[S.heap[largest]] < S.err[S.heap[child]]) largest = child;
+        if (largest == i) break;
+        int t = S.heap[i];
+        S.heap[i] = S.heap[largest];
+        S.heap[largest] = t;
+        i = largest;
+    }
+    tv -= S.val[ret];
+    te -= S.err[ret];
+    return ret;
+}
+
+__device__ __forceinline__ bool gm_converged(double val, double err, const GMParams& P) {
+    double rel = (val == 0.0) ? HUGE_VAL : fabs(err / val);
+    return err <= P.absTol || rel <= P.relTol;
+}
+
+__global__ __launch_bounds__(64) void gm_kernel(XiModelDev m, const GMJob* jobs, int njobs, int* counter,
+                                                double* dpool, int* ipool, GMParams P, double* Q, size_t qstride,
+                                                double* Qerr, int* status, unsigned long long* nevals) {
+    __shared__ double vals[GM_NPTS];
+    __shared__ int sh_job, sh_nR, sh_done;
+    const int lane = threadIdx.x;
+    const int R = P.rmax;
+    GMSlot S = gm_slot(dpool, ipool, blockIdx.x, R);
+    for (;;) {
+        if (lane == 0) sh_job = atomicAdd(counter, 1);
+        __syncthreads();
+        int jb = sh_job;
+        __syncthreads();
+        if (jb >= njobs) break;
+        const GMJob J = jobs[jb];
+        const XiDesc xd = m.desc[J.f];
+        // region 0 = [0,1]^6
+        if (lane < 6) {
+            S.c[lane * R] = 0.5;
+            S.hw[lane * R] = 0.5;
+        }
+        __syncthreads();
+        gm_eval_region(m, xd, J, P, S, R, 0, vals);
+        // lane-0 state (heap size, totals, evaluation count)
+        int nheap = 0, nregions = 1;
+        double tv = 0, te = 0;
+        unsigned numEval = GM_NPTS;
+        if (lane == 0) heap_push(S, nheap, 0, tv, te);
+        for (;;) {
+            if (lane == 0) {
+                int done = 0, nR = 0;
+                if (!(numEval < P.maxEval || !P.maxEval)) done = 1;
+                else if (gm_converged(tv, te, P)) done = 1;
+                else {
+                    double eev = tv, eee = te;
+                    do {
+                        if (nregions + 1 > R) {   // pool exhausted
+                            done = 2;
+                            break;
+                        }
+                        int ri = heap_pop(S, nheap, tv, te);
+                        eee -= S.err[ri];
+                        // cut along the split dimension (src/cubature.c:210-224)
+                        int d = S.split[ri];
+                        int r2 = nregions++;
+                        for (int k = 0; k < 6; k++) {
+                            S.c[k * R + r2] = S.c[k * R + ri];
+                            S.hw[k * R + r2] = S.hw[k * R + ri];
+                        }
+                        double h = S.hw[d * R + ri] * 0.5;
+                        S.hw[d * R + ri] = h;
+                        S.hw[d * R + r2] = h;
+                        S.c[d * R + ri] -= h;
+                        S.c[d * R + r2] += h;
+                        numEval += GM_NPTS * 2;
+                        S.list[nR++] = ri;
+                        S.list[nR++] = r2;
+                        if (gm_converged(eev, eee, P)) break;
+                    } while (nheap > 0 && (numEval < P.maxEval || !P.maxEval));
+                }
+                sh_nR = nR;
+                sh_done = done;
+            }
+            __syncthreads();
+            int nR = sh_nR, done = sh_done;
+            __syncthreads();
+            if (done == 2) {
+                if (lane == 0) atomicOr(status, 1);
+                break;
+            }
+            if (done) break;
+            __threadfence_block();
+            for (int t = 0; t < nR; t++) gm_eval_region(m, xd, J, P, S, R, S.list[t], vals);
+            if (lane == 0)
+                for (int t = 0; t < nR; t++) heap_push(S, nheap, S.list[t], tv, te);
+        }
+        if (lane == 0) {
+            double val = 0, err = 0;
+            for (int i = 0; i < nheap; i++) {
+                val += S.val[S.heap[i]];
+                err += S.err[S.heap[i]];
+            }
+            Q[(size_t)J.f * qstride + J.key] = val;
+            if (Qerr) Qerr[(size_t)J.f * qstride + J.key] = err;
+            atomicAdd(nevals, (unsigned long long)numEval);
+        }
+        __syncthreads();
+    }
+}
+
+// ---- GAUSS: one thread per integral ----------------------------------------
+
+__global__ __launch_bounds__(256) void tent_kernel(XiModelDev m, const GaussJob* jobs, int njobs, TentRule rule,
+                                                   double L, double* Q, size_t qstride) {
+    __shared__ double s_s[64], s_w[64];
+    int n = rule.n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        s_s[i] = rule.s[i];
+        s_w[i] = rule.w[i];
+    }
+    __syncthreads();
+    int jb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (jb >= njobs) return;
+    const GaussJob J = jobs[jb];
+    const XiDesc xd = m.desc[J.f];
+    double acc = 0;
+    for (int a = 0; a < n; a++) {
+        double rx = min_image(J.D[0] + s_s[a] * J.h[0], L);
+        double accy = 0;
+        for (int b = 0; b < n; b++) {
+            double ry = min_image(J.D[1] + s_s[b] * J.h[1], L);
+            double rxy2 = rx * rx + ry * ry;
+            double accz = 0;
+            for (int c = 0; c < n; c++) {
+                double rz = min_image(J.D[2] + s_s[c] * J.h[2], L);
+                accz += s_w[c] * xi_eval(m, xd, sqrt(rxy2 + rz * rz));
+            }
+            accy += s_w[b] * accz;
+        }
+        acc += s_w[a] * accy;
+    }
+    Q[(size_t)J.f * qstride + J.key] = acc;
+}
+
+// Near pairs: composite rule, one 256-thread workgroup per integral.
+__global__ __launch_bounds__(256) void tent_near_kernel(XiModelDev m, const GaussJob* jobs, TentRule rule, double L,
+                                                        double* Q, size_t qstride) {
+    __shared__ double s_s[64], s_w[64];
+    __shared__ double red[256];
+    int n = rule.n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        s_s[i] = rule.s[i];
+        s_w[i] = rule.w[i];
+    }
+    __syncthreads();
+    const GaussJob J = jobs[blockIdx.x];
+    const XiDesc xd = m.desc[J.f];
+    double acc = 0;
+    // thread owns (a, b) pairs; inner c loop
+    for (int ab = threadIdx.x; ab < n * n; ab += blockDim.x) {
+        int a = ab / n, b = ab % n;
+        double rx = min_image(J.D[0] + s_s[a] * J.h[0], L);
+        double ry = min_image(J.D[1] + s_s[b] * J.h[1], L);
+        double rxy2 = rx * rx + ry * ry;
+        double accz = 0;
+        for (int c = 0; c < n; c++) {
+            double rz = min_image(J.D[2] + s_s[c] * J.h[2], L);
+            accz += s_w[c] * xi_eval(m, xd, sqrt(rxy2 + rz * rz));
+        }
+        acc += s_w[a] * s_w[b] * accz;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) Q[(size_t)J.f * qstride + J.key] = red[0];
+}
+
+// ---- first-pair search and job construction --------------------------------
+
+struct CellGeo {
+    int d1, d2, d3;
+    double Nbar;
+};
+
+__device__ __forceinline__ int box_key(int ax, int ay, int az, int bx, int by, int bz, int Ny, int Nz) {
+    return (abs(ax - bx) * Ny + abs(ay - by)) * Nz + abs(az - bz);
+}
+
+__global__ void first_pair_kernel(const int* cellidx, const int* rows, int nrows, const int* cols, int ncols,
+                                  int Ny, int Nz, int row0, unsigned long long* first) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    if (j >= ncols || i >= nrows) return;
+    int a = rows[i], b = cols[j];
+    int key = box_key(cellidx[3 * a], cellidx[3 * a + 1], cellidx[3 * a + 2], cellidx[3 * b], cellidx[3 * b + 1],
+                      cellidx[3 * b + 2], Ny, Nz);
+    atomicMin(&first[key], (unsigned long long)(i + row0) * (unsigned long long)ncols + (unsigned long long)j);
+}
+
+__global__ void fill_u64_kernel(unsigned long long* p, size_t n, unsigned long long v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// K2: s[i*ld + j] = sqrt(Nbar_a Nbar_b) Q[key(a,b)] (src/sig.cpp:120).
+// Tile of 8 rows x 512 columns per 256-thread block; column geometry in LDS.
+__global__ __launch_bounds__(256) void assemble_kernel(const int* cellidx, const double* nbar, const int* rows,
+                                                       int nrows, const int* cols, int ncols, int Ny, int Nz,
+                                                       const double* Qf, double* s, size_t ld) {
+    __shared__ int cx[512], cy[512], cz[512];
+    __shared__ double cn[512];
+    const int j0 = blockIdx.x * 512, i0 = blockIdx.y * 8;
+    for (int t = threadIdx.x; t < 512; t += 256) {
+        int j = j0 + t;
+        if (j < ncols) {
+            int b = cols ? cols[j] : j;
+            cx[t] = cellidx[3 * b];
+            cy[t] = cellidx[3 * b + 1];
+            cz[t] = cellidx[3 * b + 2];
+            cn[t] = nbar[b];
+        }
+    }
+    __syncthreads();
+    for (int r = 0; r < 8; r++) {
+        int i = i0 + r;
+        if (i >= nrows) break;
+        int a = rows ? rows[i] : i;
+        int ax = cellidx[3 * a], ay = cellidx[3 * a + 1], az = cellidx[3 * a + 2];
+        double na = nbar[a];
+        for (int t = threadIdx.x; t < 512; t += 256) {
+            int j = j0 + t;
+            if (j >= ncols) break;
+            int key = box_key(ax, ay, az, cx[t], cy[t], cz[t], Ny, Nz);
+            s[(size_t)i * ld + j] = sqrt(na * cn[t]) * Qf[key];
+        }
+    }
+}
+
+// ---- host side ------------------------------------------------------------
+
+struct Scratch {
+    void* p = nullptr;
+    size_t cap = 0;
+    ~Scratch() { dev_free(&p); }
+};
+
+static int make_tent_rule(opsec_ctx* ctx, int nodes, TentRule* r) {
+    OPSEC_CHECK_ARG(ctx, nodes >= 2 && nodes <= 32 && nodes % 2 == 0, "gauss nodes per axis must be even, 2..32");
+    int m = nodes / 2;
+    double t[16], w[16];
+    gauss_jacobi10(m, t, w);
+    r->n = nodes;
+    // negative half (mirror, reversed) then positive half
+    for (int k = 0; k < m; k++) {
+        r->s[k] = -t[m - 1 - k];
+        r->w[k] = w[m - 1 - k];
+        r->s[m + k] = t[k];
+        r->w[m + k] = w[k];
+    }
+    return OPSEC_OK;
+}
+
+static void make_near_rule(TentRule* r) {
+    const int panels = 4, q = 8;
+    double x[8], w[8];
+    gauss_legendre(q, x, w);
+    int m = panels * q;   // nodes per half
+    double t[32], wt[32];
+    for (int p = 0; p < panels; p++)
+        for (int k = 0; k < q; k++) {
+            double a = (double)p / panels, b = (double)(p + 1) / panels;
+            double tt = 0.5 * (a + b) + 0.5 * (b - a) * x[k];
+            t[p * q + k] = tt;
+            wt[p * q + k] = 0.5 * (b - a) * w[k] * (1.0 - tt);
+        }
+    r->n = 2 * m;
+    for (int k = 0; k < m; k++) {
+        r->s[k] = -t[m - 1 - k];
+        r->w[k] = wt[m - 1 - k];
+        r->s[m + k] = t[k];
+        r->w[m + k] = wt[k];
+    }
+}
+
+// Geometry of cell d in the survey grid, the way Survey::CreateEmptyCell does
+// it (src/Survey.cpp:71-91).
+static void grid_cell(const SurveyDev& sv, int d1, int d2, int d3, double* lo, double* hi) {
+    lo[0] = sv.min1 + d1 * (sv.max1 - sv.min1) / sv.n1;
+    hi[0] = sv.min1 + (d1 + 1) * (sv.max1 - sv.min1) / sv.n1;
+    lo[1] = sv.min2 + d2 * (sv.max2 - sv.min2) / sv.n2;
+    hi[1] = sv.min2 + (d2 + 1) * (sv.max2 - sv.min2) / sv.n2;
+    lo[2] = sv.min3 + d3 * (sv.max3 - sv.min3) / sv.n3;
+    hi[2] = sv.min3 + (d3 + 1) * (sv.max3 - sv.min3) / sv.n3;
+}
+
+struct PairGeo {
+    double lo1[3], hi1[3], lo2[3], hi2[3];
+};
+
+// Run method q->method on the given key pairs for functions funcs; results to
+// dQ[f * nkeys + key].  keys[i] with geometry geo[i].
+static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::vector<int>& keys,
+                          const std::vector<PairGeo>& geo, const std::vector<int>& nearflag, const opsec_quad* q,
+                          double* dQ, size_t nkeys, double* dQerr = nullptr) {
+    XiModelDev mv = model_view(ctx);
+    const size_t npairs = keys.size();
+    if (npairs == 0 || nf == 0) return OPSEC_OK;
+    int method = q->method;
+    if (method == OPSEC_QUAD_GM) {
+        std::vector<GMJob> jobs;
+        jobs.reserve(npairs * nf);
+        // functions with the most adaptive work (the prior) first
+        for (int t = 0; t < nf; t++)
+            for (size_t i = 0; i < npairs; i++) {
+                GMJob J;
+                J.f = funcs[t];
+                J.key = keys[i];
+                const PairGeo& g = geo[i];
+                if (ctx->sv.coordsys == OPSEC_COORD_CARTESIAN) {
+                    for (int d = 0; d < 3; d++) {
+                        J.lo[d] = g.lo1[d];
+                        J.hi[d] = g.hi1[d];
+                        J.lo[3 + d] = g.lo2[d];
+                        J.hi[3 + d] = g.hi2[d];
+                    }
+                } else {
+                    J.lo[0] = g.lo1[0] * g.lo1[0] * g.lo1[0];
+                    J.hi[0] = g.hi1[0] * g.hi1[0] * g.hi1[0];
+                    J.lo[1] = g.lo2[0] * g.lo2[0] * g.lo2[0];
+                    J.hi[1] = g.hi2[0] * g.hi2[0] * g.hi2[0];
+                    J.lo[2] = g.lo1[1];
+                    J.hi[2] = g.hi1[1];
+                    J.lo[3] = g.lo2[1];
+                    J.hi[3] = g.hi2[1];
+                    J.lo[4] = g.lo1[2];
+                    J.hi[4] = g.hi1[2];
+                    J.lo[5] = g.lo2[2];
+                    J.hi[5] = g.hi2[2];
+                }
+                jobs.push_back(J);
+            }
+        int njobs = (int)jobs.size();
+        GMParams P;
+        P.absTol = q->epsrel;   // reference argument order, src/sig.cpp:427-430
+        P.relTol = q->epsabs;
+        P.maxEval = (unsigned)q->maxeval;
+        P.rmax = 4096;
+        P.coordsys = ctx->sv.coordsys;
+        P.L = ctx->sv.L;
+        int nslots = std::min(njobs, ctx->num_cus * 8);
+        Scratch sj, sd, si, sc;
+        int rc;
+        if ((rc = dev_realloc(ctx, &sj.p, &sj.cap, sizeof(GMJob) * njobs))) return rc;
+        if ((rc = dev_realloc(ctx, &sd.p, &sd.cap, sizeof(double) * 14 * (size_t)P.rmax * nslots))) return rc;
+        if ((rc = dev_realloc(ctx, &si.p, &si.cap, sizeof(int) * 3 * (size_t)P.rmax * nslots))) return rc;
+        if ((rc = dev_realloc(ctx, &sc.p, &sc.cap, 64))) return rc;
+        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(sj.p, jobs.data(), sizeof(GMJob) * njobs, hipMemcpyHostToDevice, ctx->stream));
+        OPSEC_HIP_TRY(ctx, hipMemsetAsync(sc.p, 0, 64, ctx->stream));
+        int* d_counter = (int*)sc.p;
+        int* d_status = d_counter + 1;
+        unsigned long long* d_nev = (unsigned long long*)((char*)sc.p + 8);
+        hipLaunchKernelGGL(gm_kernel, dim3(nslots), dim3(64), 0, ctx->stream, mv, (const GMJob*)sj.p, njobs, d_counter,
+                           (double*)sd.p, (int*)si.p, P, dQ, nkeys, dQerr, d_status, d_nev);
+        OPSEC_HIP_TRY(ctx, hipGetLastError());
+        int status = 0;
+        unsigned long long nev = 0;
+        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&status, d_status, 4, hipMemcpyDeviceToHost, ctx->stream));
+        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nev, d_nev, 8, hipMemcpyDeviceToHost, ctx->stream));
+        OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        if (status) return set_error(ctx, OPSEC_ECONV, "adaptive cubature exceeded %d regions", P.rmax);
+        ctx->st_integrals += njobs;
+        ctx->st_nodes += (int64_t)nev;
+        return OPSEC_OK;
+    }
+    if (method == OPSEC_QUAD_GAUSS) {
+        OPSEC_CHECK_ARG(ctx, ctx->sv.coordsys == OPSEC_COORD_CARTESIAN, "GAUSS quadrature needs a box survey");
+        TentRule rule, nrule;
+        int rc = make_tent_rule(ctx, q->nodes, &rule);
+        if (rc) return rc;
+        make_near_rule(&nrule);
+        std::vector<GaussJob> far, nearj;
+        for (int t = 0; t < nf; t++)
+            for (size_t i = 0; i < npairs; i++) {
+                const PairGeo& g = geo[i];
+                GaussJob J;
+                J.f = funcs[t];
+                J.key = keys[i];
+                for (int d = 0; d < 3; d++) {
+                    double h1 = g.hi1[d] - g.lo1[d], h2 = g.hi2[d] - g.lo2[d];
+                    if (std::fabs(h1 - h2) > 1e-9 * std::fabs(h1))
+                        return set_error(ctx, OPSEC_EINVAL, "GAUSS quadrature needs equal cell widths");
+                    J.h[d] = h1;
+                    J.D[d] = 0.5 * (g.lo2[d] + g.hi2[d]) - 0.5 * (g.lo1[d] + g.hi1[d]);
+                }
+                (nearflag[i] ? nearj : far).push_back(J);
+            }
+        Scratch sj;
+        size_t nj = far.size() + nearj.size();
+        if ((rc = dev_realloc(ctx, &sj.p, &sj.cap, sizeof(GaussJob) * nj))) return rc;
+        GaussJob* dj = (GaussJob*)sj.p;
+        if (!far.empty())
+            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dj, far.data(), sizeof(GaussJob) * far.size(), hipMemcpyHostToDevice, ctx->stream));
+        if (!nearj.empty())
+            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dj + far.size(), nearj.data(), sizeof(GaussJob) * nearj.size(),
+                                              hipMemcpyHostToDevice, ctx->stream));
+        if (!far.empty()) {
+            hipLaunchKernelGGL(tent_kernel, dim3((unsigned)((far.size() + 255) / 256)), dim3(256), 0, ctx->stream, mv, dj,
+                               (int)far.size(), rule, ctx->sv.L, dQ, nkeys);
+            OPSEC_HIP_TRY(ctx, hipGetLastError());
+        }
+        if (!nearj.empty()) {
+            hipLaunchKernelGGL(tent_near_kernel, dim3((unsigned)nearj.size()), dim3(256), 0, ctx->stream, mv,
+                               dj + far.size(), nrule, ctx->sv.L, dQ, nkeys);
+            OPSEC_HIP_TRY(ctx, hipGetLastError());
+        }
+        OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->st_integrals += (int64_t)nj;
+        ctx->st_nodes += (int64_t)far.size() * rule.n * rule.n * rule.n +
+                         (int64_t)nearj.size() * nrule.n * nrule.n * nrule.n;
+        return OPSEC_OK;
+    }
+    return set_error(ctx, OPSEC_EINVAL, "unknown quadrature method %d", method);
+}
+
+static int check_model(opsec_ctx* ctx, int f) {
+    if (ctx->model_kind < 0) return set_error(ctx, OPSEC_ESTATE, "no model loaded");
+    OPSEC_CHECK_ARG(ctx, f >= 0 && f < (int)ctx->funcs.size(), "function index %d out of range", f);
+    return OPSEC_OK;
+}
+
+static int check_quad(opsec_ctx* ctx, const opsec_quad* q) {
+    OPSEC_CHECK_ARG(ctx, q, "quadrature options are NULL");
+    OPSEC_CHECK_ARG(ctx, q->method >= 0 && q->method <= 2, "unknown quadrature method %d", q->method);
+    if (q->method == OPSEC_QUAD_GM) OPSEC_CHECK_ARG(ctx, q->maxeval >= 0, "maxeval must be >= 0");
+    return OPSEC_OK;
+}
+
+int survey_cells_upload(opsec_ctx* ctx, const opsec_cell* cells, int ncells, const opsec_survey* sv);
+int surrogate_tables(opsec_ctx* ctx, int nf, const int* funcs, const opsec_quad* q, const std::vector<int>& keys,
+                     const std::vector<PairGeo>& geo, const std::vector<int>& nearflag);
+
+// Pair geometry for a key of the box grid from an explicit cell pair.
+static PairGeo pair_geo(const opsec_cell& c1, const opsec_cell& c2) {
+    PairGeo g;
+    g.lo1[0] = c1.min1; g.hi1[0] = c1.max1;
+    g.lo1[1] = c1.min2; g.hi1[1] = c1.max2;
+    g.lo1[2] = c1.min3; g.hi1[2] = c1.max3;
+    g.lo2[0] = c2.min1; g.hi2[0] = c2.max1;
+    g.lo2[1] = c2.min2; g.hi2[1] = c2.max2;
+    g.lo2[2] = c2.min3; g.hi2[2] = c2.max3;
+    return g;
+}
+
+static int near_of(const opsec_cell& a, const opsec_cell& b, int near) {
+    int m = std::max(std::abs(a.d1 - b.d1), std::max(std::abs(a.d2 - b.d2), std::abs(a.d3 - b.d3)));
+    return m < near ? 1 : 0;
+}
+
+int build_pair_tables(opsec_ctx* ctx, int nf, const int* funcs, const opsec_quad* q) {
+    // Keys of the full survey matrix: first occurrences along row 0, then the
+    // grid-canonical pair for keys that row 0 does not carry.
+    const SurveyDev& sv = ctx->sv;
+    const int nk = ctx->nkeys;
+    const int N = ctx->ncells;
+    std::vector<int> owner(nk, -1);
+    const opsec_cell& c0 = ctx->h_cells[0];
+    for (int b = 0; b < N; b++) {
+        const opsec_cell& cb = ctx->h_cells[b];
+        int key = (std::abs(c0.d1 - cb.d1) * sv.n2 + std::abs(c0.d2 - cb.d2)) * sv.n3 + std::abs(c0.d3 - cb.d3);
+        if (owner[key] < 0) owner[key] = b;
+    }
+    std::vector<int> keys;
+    std::vector<PairGeo> geo;
+    std::vector<int> nearflag;
+    keys.reserve(nk);
+    for (int key = 0; key < nk; key++) {
+        int kz = key % sv.n3, ky = (key / sv.n3) % sv.n2, kx = key / (sv.n3 * sv.n2);
+        PairGeo g;
+        int nflag;
+        if (owner[key] >= 0) {
+            const opsec_cell& cb = ctx->h_cells[owner[key]];
+            g = pair_geo(c0, cb);   // row 0 is the lower index
+            nflag = near_of(c0, cb, q->near);
+        } else {
+            grid_cell(sv, 0, 0, 0, g.lo1, g.hi1);
+            grid_cell(sv, kx, ky, kz, g.lo2, g.hi2);
+            nflag = std::max(kx, std::max(ky, kz)) < q->near;
+        }
+        keys.push_back(key);
+        geo.push_back(g);
+        nearflag.push_back(nflag);
+    }
+    if (q->method == OPSEC_QUAD_SURROGATE) return surrogate_tables(ctx, nf, funcs, q, keys, geo, nearflag);
+    return run_quadrature(ctx, nf, funcs, keys, geo, nearflag, q, ctx->d_Q, nk);
+}
+
+int run_quadrature_public(opsec_ctx* ctx, int nf, const int* funcs, const std::vector<int>& keys,
+                          const std::vector<PairGeo>& geo, const std::vector<int>& nearflag, const opsec_quad* q,
+                          double* dQ, size_t nkeys) {
+    return run_quadrature(ctx, nf, funcs, keys, geo, nearflag, q, dQ, nkeys);
+}
+
+}  // namespace opsec
+
+using namespace opsec;
+
+extern "C" {
+
+int opsec_quad_defaults(int method, opsec_quad* q) {
+    if (!q) return OPSEC_EINVAL;
+    std::memset(q, 0, sizeof *q);
+    q->method = method;
+    q->nodes = 6;
+    q->near = 1;
+    q->maxeval = 50000000;   // SIGNAL_MAXEVAL, src/sig.cpp:35-37
+    q->epsrel = 1e-5;        // src/sig.h:24
+    q->epsabs = 1e-10;
+    q->spot_frac = 0.01;
+    q->spot_tol = 1e-3;
+    q->spot_seed = 11;
+    if (method == OPSEC_QUAD_SURROGATE) q->near = 2;
+    return (method >= 0 && method <= 2) ? OPSEC_OK : OPSEC_EINVAL;
+}
+
+int opsec_cells_upload(opsec_ctx* ctx, const opsec_cell* cells, int ncells, const opsec_survey* sv) {
+    if (!ctx) return OPSEC_EINVAL;
+    OPSEC_CHECK_ARG(ctx, cells && ncells > 0, "no cells");
+    SurveyDev s;
+    int rc = survey_from_abi(ctx, sv, &s);
+    if (rc) return rc;
+    OPSEC_CHECK_ARG(ctx, s.coordsys == OPSEC_COORD_CARTESIAN, "device-resident tables support box surveys");
+    for (int a = 0; a < ncells; a++) {
+        const opsec_cell& c = cells[a];
+        OPSEC_CHECK_ARG(ctx, c.d1 >= 0 && c.d1 < s.n1 && c.d2 >= 0 && c.d2 < s.n2 && c.d3 >= 0 && c.d3 < s.n3,
+                        "cell %d has grid index outside the survey grid", a);
+    }
+    OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    ctx->sv = s;
+    ctx->ncells = ncells;
+    ctx->nkeys = s.n1 * s.n2 * s.n3;
+    ctx->h_cells.assign(cells, cells + ncells);
+    std::vector<int> idx(3 * (size_t)ncells);
+    std::vector<double> nb(ncells);
+    for (int a = 0; a < ncells; a++) {
+        idx[3 * a] = cells[a].d1;
+        idx[3 * a + 1] = cells[a].d2;
+        idx[3 * a + 2] = cells[a].d3;
+        nb[a] = cells[a].Nbar;
+    }
+    size_t cap = 0;
+    dev_free((void**)&ctx->d_cellidx);
+    dev_free((void**)&ctx->d_nbar);
+    dev_free((void**)&ctx->d_Q);
+    if ((rc = dev_realloc(ctx, (void**)&ctx->d_cellidx, &cap, idx.size() * 4))) return rc;
+    cap = 0;
+    if ((rc = dev_realloc(ctx, (void**)&ctx->d_nbar, &cap, nb.size() * 8))) return rc;
+    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_cellidx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_nbar, nb.data(), nb.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->Q_valid.assign(ctx->funcs.size(), 0);
+    return OPSEC_OK;
+}
+
+int opsec_pair_nkeys(opsec_ctx* ctx, int* nkeys) {
+    if (!ctx || !nkeys) return OPSEC_EINVAL;
+    if (ctx->ncells == 0) return set_error(ctx, OPSEC_ESTATE, "no survey uploaded");
+    *nkeys = ctx->nkeys;
+    return OPSEC_OK;
+}
+
+int opsec_pair_tables(opsec_ctx* ctx, int nf, const int* funcs, const opsec_quad* q) {
+    if (!ctx) return OPSEC_EINVAL;
+    if (ctx->ncells == 0) return set_error(ctx, OPSEC_ESTATE, "no survey uploaded");
+    if (ctx->model_kind < 0) return set_error(ctx, OPSEC_ESTATE, "no model loaded");
+    int rc = check_quad(ctx, q);
+    if (rc) return rc;
+    OPSEC_CHECK_ARG(ctx, nf >= 1 && funcs, "no functions");
+    for (int t = 0; t < nf; t++)
+        if ((rc = check_model(ctx, funcs[t]))) return rc;
+    OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    size_t need = (size_t)ctx->funcs.size() * ctx->nkeys * 8;
+    if (!ctx->d_Q) {
+        size_t cap = 0;
+        if ((rc = dev_realloc(ctx, (void**)&ctx->d_Q, &cap, need))) return rc;
+    }
+    ctx->st_integrals = ctx->st_nodes = ctx->st_surrogate = ctx->st_spot = ctx->st_spot_fail = 0;
+    rc = build_pair_tables(ctx, nf, funcs, q);
+    if (rc) return rc;
+    for (int t = 0; t < nf; t++) ctx->Q_valid[funcs[t]] = 1;
+    return OPSEC_OK;
+}
+
+int opsec_pair_table_get(opsec_ctx* ctx, int f, double* Q) {
+    if (!ctx || !Q) return OPSEC_EINVAL;
+    int rc = check_model(ctx, f);
+    if (rc) return rc;
+    if (!ctx->d_Q || !ctx->Q_valid[f]) return set_error(ctx, OPSEC_ESTATE, "pair table %d not built", f);
+    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(Q, ctx->d_Q + (size_t)f * ctx->nkeys, (size_t)ctx->nkeys * 8,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return OPSEC_OK;
+}
+
+int opsec_pair_stats(opsec_ctx* ctx, int64_t* integrals, int64_t* nodes, int64_t* surrogate, int64_t* spot,
+                     int64_t* spot_fail) {
+    if (!ctx) return OPSEC_EINVAL;
+    if (integrals) *integrals = ctx->st_integrals;
+    if (nodes) *nodes = ctx->st_nodes;
+    if (surrogate) *surrogate = ctx->st_surrogate;
+    if (spot) *spot = ctx->st_spot;
+    if (spot_fail) *spot_fail = ctx->st_spot_fail;
+    return OPSEC_OK;
+}
+
+int opsec_assemble_device(opsec_ctx* ctx, int f, double* dS, int ld) {
+    if (!ctx || !dS) return OPSEC_EINVAL;
+    int rc = check_model(ctx, f);
+    if (rc) return rc;
+    if (!ctx->d_Q || !ctx->Q_valid[f]) return set_error(ctx, OPSEC_ESTATE, "pair table %d not built", f);
+    int N = ctx->ncells;
+    if (ld <= 0) ld = N;
+    OPSEC_CHECK_ARG(ctx, ld >= N, "leading dimension %d < Ncells %d", ld, N);
+    hipLaunchKernelGGL(assemble_kernel, dim3((N + 511) / 512, (N + 7) / 8), dim3(256), 0, ctx->stream, ctx->d_cellidx,
+                       ctx->d_nbar, (const int*)nullptr, N, (const int*)nullptr, N, ctx->sv.n2, ctx->sv.n3,
+                       ctx->d_Q + (size_t)f * ctx->nkeys, dS, (size_t)ld);
+    OPSEC_HIP_TRY(ctx, hipGetLastError());
+    return OPSEC_OK;
+}
+
+// Drop-in for ComputeSignalMatrixBlock (box surveys).
+int opsec_sig_block(opsec_ctx* ctx, const opsec_cell* cells, int ncells, const int* rows, int nrows, const int* cols,
+                    int ncols, int f, const opsec_survey* sv, const opsec_quad* q, double* s, int lld) {
+    if (!ctx) return OPSEC_EINVAL;
+    int rc = check_model(ctx, f);
+    if (rc) return rc;
+    if ((rc = check_quad(ctx, q))) return rc;
+    OPSEC_CHECK_ARG(ctx, cells && ncells > 0, "no cells");
+    OPSEC_CHECK_ARG(ctx, nrows >= 0 && ncols >= 0, "negative block size");
+    if (lld <= 0) lld = ncols;
+    OPSEC_CHECK_ARG(ctx, lld >= ncols, "ComputeSignalMatrixBlock: invalid array stride (%d,%d,%d)", lld, nrows, ncols);
+    if (nrows == 0 || ncols == 0) return OPSEC_OK;
+    OPSEC_CHECK_ARG(ctx, rows && cols && s, "NULL rows/cols/output");
+    SurveyDev svd;
+    if ((rc = survey_from_abi(ctx, sv, &svd))) return rc;
+    OPSEC_CHECK_ARG(ctx, svd.coordsys == OPSEC_COORD_CARTESIAN, "opsec_sig_block: box surveys only");
+    for (int i = 0; i < nrows; i++) OPSEC_CHECK_ARG(ctx, rows[i] >= 0 && rows[i] < ncells, "row index out of range");
+    for (int j = 0; j < ncols; j++) OPSEC_CHECK_ARG(ctx, cols[j] >= 0 && cols[j] < ncells, "column index out of range");
+    for (int a = 0; a < ncells; a++) {
+        const opsec_cell& c = cells[a];
+        OPSEC_CHECK_ARG(ctx, c.d1 >= 0 && c.d1 < svd.n1 && c.d2 >= 0 && c.d2 < svd.n2 && c.d3 >= 0 && c.d3 < svd.n3,
+                        "cell %d has grid index outside the survey grid", a);
+    }
+    OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    SurveyDev saved = ctx->sv;
+    ctx->sv = svd;
+    const int nk = svd.n1 * svd.n2 * svd.n3;
+
+    std::vector<int> idx(3 * (size_t)ncells);
+    std::vector<double> nb(ncells);
+    for (int a = 0; a < ncells; a++) {
+        idx[3 * a] = cells[a].d1;
+        idx[3 * a + 1] = cells[a].d2;
+        idx[3 * a + 2] = cells[a].d3;
+        nb[a] = cells[a].Nbar;
+    }
+    Scratch s_idx, s_nb, s_rows, s_cols, s_first, s_Q, s_out;
+    if ((rc = dev_realloc(ctx, &s_idx.p, &s_idx.cap, idx.size() * 4))) goto done;
+    if ((rc = dev_realloc(ctx, &s_nb.p, &s_nb.cap, nb.size() * 8))) goto done;
+    if ((rc = dev_realloc(ctx, &s_rows.p, &s_rows.cap, (size_t)nrows * 4))) goto done;
+    if ((rc = dev_realloc(ctx, &s_cols.p, &s_cols.cap, (size_t)ncols * 4))) goto done;
+    if ((rc = dev_realloc(ctx, &s_first.p, &s_first.cap, (size_t)nk * 8))) goto done;
+    if ((rc = dev_realloc(ctx, &s_Q.p, &s_Q.cap, (size_t)nk * ctx->funcs.size() * 8))) goto done;
+    if ((rc = dev_realloc(ctx, &s_out.p, &s_out.cap, (size_t)nrows * lld * 8))) goto done;
+    {
+        hipStream_t st = ctx->stream;
+#define TRY(call)                                                                                       \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess) {                                                                         \
+            rc = set_error(ctx, OPSEC_EHIP, "%s failed: %s", #call, hipGetErrorString(e_));             \
+            goto done;                                                                                  \
+        }                                                                                               \
+    } while (0)
+        TRY(hipMemcpyAsync(s_idx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(s_nb.p, nb.data(), nb.size() * 8, hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(s_rows.p, rows, (size_t)nrows * 4, hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(s_cols.p, cols, (size_t)ncols * 4, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(fill_u64_kernel, dim3((nk + 255) / 256), dim3(256), 0, st, (unsigned long long*)s_first.p,
+                           (size_t)nk, ~0ull);
+        for (int r0 = 0; r0 < nrows; r0 += 32768) {
+            int nr = std::min(32768, nrows - r0);
+            hipLaunchKernelGGL(first_pair_kernel, dim3((ncols + 255) / 256, nr), dim3(256), 0, st, (const int*)s_idx.p,
+                               (const int*)s_rows.p + r0, nr, (const int*)s_cols.p, ncols, svd.n2, svd.n3, r0,
+                               (unsigned long long*)s_first.p);
+        }
+        TRY(hipGetLastError());
+        std::vector<unsigned long long> first(nk);
+        TRY(hipMemcpyAsync(first.data(), s_first.p, (size_t)nk * 8, hipMemcpyDeviceToHost, st));
+        TRY(hipStreamSynchronize(st));
+        std::vector<int> keys;
+        std::vector<PairGeo> geo;
+        std::vector<int> nearflag;
+        for (int key = 0; key < nk; key++) {
+            if (first[key] == ~0ull) continue;
+            int i = (int)(first[key] / (unsigned long long)ncols), j = (int)(first[key] % (unsigned long long)ncols);
+            int a = rows[i], b = cols[j];
+            const opsec_cell& ca = cells[std::min(a, b)];
+            const opsec_cell& cb = cells[std::max(a, b)];
+            keys.push_back(key);
+            geo.push_back(pair_geo(ca, cb));
+            nearflag.push_back(near_of(ca, cb, q->near));
+        }
+        int fs[1] = {f};
+        if (q->method == OPSEC_QUAD_SURROGATE) {
+            rc = set_error(ctx, OPSEC_EINVAL, "opsec_sig_block: use the device-resident path for the surrogate");
+            goto done;
+        }
+        rc = run_quadrature(ctx, 1, fs, keys, geo, nearflag, q, (double*)s_Q.p, (size_t)nk);
+        if (rc) goto done;
+        hipLaunchKernelGGL(assemble_kernel, dim3((ncols + 511) / 512, (nrows + 7) / 8), dim3(256), 0, st,
+                           (const int*)s_idx.p, (const double*)s_nb.p, (const int*)s_rows.p, nrows,
+                           (const int*)s_cols.p, ncols, svd.n2, svd.n3, (const double*)s_Q.p + (size_t)f * nk,
+                           (double*)s_out.p, (size_t)lld);
+        TRY(hipGetLastError());
+        // copy rows back, only the ncols entries of each row (the rest of s is untouched)
+        TRY(hipMemcpy2DAsync(s, (size_t)lld * 8, s_out.p, (size_t)lld * 8, (size_t)ncols * 8, nrows,
+                             hipMemcpyDeviceToHost, st));
+        TRY(hipStreamSynchronize(st));
+#undef TRY
+    }
+done:
+    ctx->sv = saved;
+    return rc;
+}
+
+int opsec_pair_integral(opsec_ctx* ctx, const opsec_cell* c1, const opsec_cell* c2, int f, const opsec_survey* sv,
+                        const opsec_quad* q, double* Q, double* err) {
+    if (!ctx || !c1 || !c2 || !Q) return OPSEC_EINVAL;
+    int rc = check_model(ctx, f);
+    if (rc) return rc;
+    if ((rc = check_quad(ctx, q))) return rc;
+    OPSEC_CHECK_ARG(ctx, q->method != OPSEC_QUAD_SURROGATE, "single integrals use GM or GAUSS");
+    SurveyDev svd;
+    if ((rc = survey_from_abi(ctx, sv, &svd))) return rc;
+    OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    SurveyDev saved = ctx->sv;
+    ctx->sv = svd;
+    std::vector<int> keys{0};
+    std::vector<PairGeo> geo{pair_geo(*c1, *c2)};
+    std::vector<int> nearflag{near_of(*c1, *c2, q->near)};
+    Scratch sQ;
+    size_t nfun = ctx->funcs.size();
+    if ((rc = dev_realloc(ctx, &sQ.p, &sQ.cap, 2 * nfun * 8))) {
+        ctx->sv = saved;
+        return rc;
+    }
+    OPSEC_HIP_TRY(ctx, hipMemsetAsync(sQ.p, 0, 2 * nfun * 8, ctx->stream));
+    int fs[1] = {f};
+    rc = run_quadrature(ctx, 1, fs, keys, geo, nearflag, q, (double*)sQ.p, 1, (double*)sQ.p + nfun);
+    ctx->sv = saved;
+    if (rc) return rc;
+    double qe[2];
+    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&qe[0], (double*)sQ.p + f, 8, hipMemcpyDeviceToHost, ctx->stream));
+    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&qe[1], (double*)sQ.p + nfun + f, 8, hipMemcpyDeviceToHost, ctx->stream));
+    OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    *Q = qe[0];
+    if (err) *err = qe[1];
+    return OPSEC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,53 @@
+"""Synthetic survey configurations (box surveys of the benchmark configs).
+
+Cells are laid out the way the reference's `basis` program lays them out for a
+box survey (src/basis.cpp:170-190, src/Survey.cpp:71-91): grid order
+(d1, d2, d3) with d3 fastest, bounds Min + d (Max - Min)/N, Veff = volume and
+Nbar = nbar * volume (constant selection, src/BoxSurvey.cpp:61-75).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .lib import CELL_DTYPE
+
+
+def box_cells(n, lo=0.0, hi=400.0, nbar=3e-4):
+    d = np.arange(n)
+    g1, g2, g3 = np.meshgrid(d, d, d, indexing="ij")
+    cells = np.zeros(n ** 3, dtype=CELL_DTYPE)
+    cells["a"] = np.arange(n ** 3)
+    cells["d1"], cells["d2"], cells["d3"] = g1.ravel(), g2.ravel(), g3.ravel()
+    for k, g in (("1", g1), ("2", g2), ("3", g3)):
+        cells["min" + k] = lo + g.ravel() * (hi - lo) / n
+        cells["max" + k] = lo + (g.ravel() + 1) * (hi - lo) / n
+    V = (cells["max1"] - cells["min1"]) * (cells["max2"] - cells["min2"]) * (cells["max3"] - cells["min3"])
+    cells["Veff"] = V
+    cells["Nbar"] = nbar * V
+    return cells
+
+
+def fiducial_pk(k):
+    """P(k) = 2e4 (k/0.02) / (1 + (k/0.02)^2)^1.5  (Mpc/h)^3."""
+    x = np.asarray(k, dtype=np.float64) / 0.02
+    return 2e4 * x / (1 + x * x) ** 1.5
+
+
+def pk_table(npts=2000, kmax=10.5):
+    """pkfile samples: k = 0 then log-spaced over [1e-4, kmax] (spans [0,10])."""
+    k = np.concatenate([[0.0], np.logspace(-4, np.log10(kmax), npts)])
+    return k, fiducial_pk(k)
+
+
+def linear_bands(nb, kmin, kmax):
+    """RealModel regular bands (src/RealModel.cpp:60-71)."""
+    lo = np.array([kmin + n * (kmax - kmin) / nb for n in range(nb)])
+    hi = np.array([kmin + (n + 1) * (kmax - kmin) / nb for n in range(nb)])
+    return lo, hi
+
+
+# The benchmark configurations (box part).
+CONFIGS = {
+    "C1": dict(n=12, nbands=20, kmin=0.01, kmax=0.20, nmodes=500, nodes=8, nbar=3e-4),
+    "C2": dict(n=32, nbands=40, kmin=0.005, kmax=0.25, nmodes=4000, nodes=6, nbar=3e-4),
+}

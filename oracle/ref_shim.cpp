@@ -1,0 +1,186 @@
+/* ref_shim.cpp -- C entry points over the reference's own serial sources.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is compiled together with the
+ * reference's pair-quadrature slice (src/sig.cpp, src/cubature.c,
+ * src/Model.cpp, src/RealModel.cpp, src/TestModel.cpp, src/QSpline.cpp,
+ * src/Spline.cpp, src/BoxSurvey.cpp, src/Survey.cpp, src/SeparationFunc.cpp,
+ * src/SelectionFunc.cpp, src/XiFunc.cpp, src/Cell.cpp, src/cfg.c, src/abn.c,
+ * src/opsec.c) by oracle/Makefile into oracle/_ref/libopsec_ref.so.  Only the
+ * tests and the CPU-baseline leg of bench.py load it; nothing in the product
+ * links it.
+ *
+ * The reference code is called exactly as its own programs call it:
+ *   - models are built the way InitializeModel does (src/Model.cpp:26-51),
+ *     from a "<ModelName>." sub-config;
+ *   - the box survey is built from a "survey." sub-config (src/Survey.cpp:140-165,
+ *     src/BoxSurvey.cpp:85-91);
+ *   - matrix blocks come from ComputeSignalMatrixBlock (src/sig.cpp:50-70).
+ */
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "BoxSurvey.h"
+#include "Cell.h"
+#include "Model.h"
+#include "RealModel.h"
+#include "SeparationFunc.h"
+#include "Spline.h"
+#include "Survey.h"
+#include "TestModel.h"
+#include "XiFunc.h"
+#include "abn.h"
+#include "cfg.h"
+#include "sig.h"
+
+#define REF_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+struct RefModel {
+    Model* model;
+    std::vector<XiFunc> xis;   // index 0 = prior xi, 1+p = derivative p
+};
+
+Config config_from_text(const char* text) {
+    Config cfg = cfg_new();
+    std::string s(text ? text : "");
+    size_t pos = 0;
+    while (pos <= s.size()) {
+        size_t nl = s.find('\n', pos);
+        if (nl == std::string::npos) nl = s.size();
+        std::string line = s.substr(pos, nl - pos);
+        if (!line.empty()) cfg_read_line(cfg, line.c_str());
+        pos = nl + 1;
+    }
+    return cfg;
+}
+
+}  // namespace
+
+/* Build a model from "key = value" lines, as InitializeModel does. */
+REF_API void* ref_model_new(const char* cfgtext) {
+    Config cfg = config_from_text(cfgtext);
+    const char* name = cfg_get(cfg, "model");
+    Model* m = NULL;
+    if (strcmp(name, "RealModel") == 0) {
+        Config sub = cfg_new_sub(cfg, "RealModel.");
+        m = new RealModel(sub);
+        cfg_destroy(sub);
+    } else if (strcmp(name, "TestModel") == 0) {
+        Config sub = cfg_new_sub(cfg, "TestModel.");
+        m = new TestModel(sub);
+        cfg_destroy(sub);
+    }
+    cfg_destroy(cfg);
+    if (!m) return NULL;
+    RefModel* rm = new RefModel;
+    rm->model = m;
+    rm->xis.push_back(m->GetXi());
+    for (int p = 0; p < m->NumParams(); p++) rm->xis.push_back(m->GetXiDeriv(p));
+    return rm;
+}
+
+REF_API void ref_model_free(void* h) {
+    RefModel* rm = (RefModel*)h;
+    if (!rm) return;
+    rm->xis.clear();
+    delete rm->model;
+    delete rm;
+}
+
+REF_API int ref_model_nparams(void* h) { return ((RefModel*)h)->model->NumParams(); }
+
+REF_API double ref_model_param(void* h, int n) { return ((RefModel*)h)->model->GetParam(n); }
+
+/* xi(r) of function f (0 = prior, 1+p = derivative p) at separations r[i]. */
+REF_API void ref_model_xi(void* h, int f, int n, const double* r, double* out) {
+    RefModel* rm = (RefModel*)h;
+    BoxSurvey box("none", 1.0, 1e30);
+    SeparationFunc sep = box.GetSeparationFunction();
+    for (int i = 0; i < n; i++) {
+        Point p1 = {{0.0}, {0.0}, {0.0}};
+        Point p2 = {{r[i]}, {0.0}, {0.0}};
+        out[i] = rm->xis[f](p1, p2, sep);
+    }
+}
+
+/* The reference's ComputeXiLM on a cubic spline through (k, P) samples. */
+REF_API void ref_xilm(int l, int m, int nk_tab, const double* ktab, const double* ptab,
+                      int nr, const double* r, double* xi, int Nk, double kmin,
+                      double kmax, double kdamp) {
+    Spline P = CubicSpline(nk_tab, ktab, ptab);
+    ComputeXiLM(l, m, P, nr, r, xi, Nk, kmin, kmax, kdamp);
+}
+
+/* Cubic spline (netlib fmm) evaluation, src/Spline.cpp:258-359. */
+REF_API void ref_cubic_spline(int n, const double* x, const double* y, int m,
+                              const double* xe, double* ye) {
+    Spline S = CubicSpline(n, x, y);
+    for (int i = 0; i < m; i++) ye[i] = S(xe[i]);
+}
+
+static Survey* make_box(const char* survey_cfgtext) {
+    Config cfg = config_from_text(survey_cfgtext);
+    Config sub = cfg_new_sub(cfg, "survey.");
+    Survey* s = new BoxSurvey(sub);
+    cfg_destroy(sub);
+    cfg_destroy(cfg);
+    return s;
+}
+
+/* Minimum-image distance of the box survey (src/BoxSurvey.cpp:26-37). */
+REF_API void ref_box_sep(const char* survey_cfgtext, int n, const double* p1,
+                         const double* p2, double* r) {
+    Survey* s = make_box(survey_cfgtext);
+    SeparationFunc sep = s->GetSeparationFunction();
+    for (int i = 0; i < n; i++) {
+        Point a = {{p1[3 * i]}, {p1[3 * i + 1]}, {p1[3 * i + 2]}};
+        Point b = {{p2[3 * i]}, {p2[3 * i + 1]}, {p2[3 * i + 2]}};
+        r[i] = sep.r(a, b);
+    }
+    delete s;
+}
+
+/* One cell-pair integral, src/sig.cpp:416-434. cells are 80-byte records. */
+REF_API double ref_signal_c(void* h, int f, const char* survey_cfgtext, const void* c1,
+                            const void* c2, double epsrel, double epsabs, double* err) {
+    RefModel* rm = (RefModel*)h;
+    Survey* s = make_box(survey_cfgtext);
+    SeparationFunc sep = s->GetSeparationFunction();
+    double e = 0;
+    double q = ComputeSignalC(*(const Cell*)c1, *(const Cell*)c2, rm->xis[f], sep, epsrel,
+                              epsabs, &e);
+    if (err) *err = e;
+    delete s;
+    return q;
+}
+
+/* A block of S (f = 0) or C_{,p} (f = 1+p), src/sig.cpp:50-123. */
+REF_API int ref_sig_block(void* h, int f, const char* survey_cfgtext, const void* cells,
+                          int ncells, const int* rows, int nrows, const int* cols,
+                          int ncols, double* s, int lld, double epsrel, double epsabs) {
+    RefModel* rm = (RefModel*)h;
+    Survey* sv = make_box(survey_cfgtext);
+    std::vector<int> R(rows, rows + nrows), C(cols, cols + ncols);
+    ComputeSignalMatrixBlock(ncells, R, C, s, lld, rm->xis[f], sv, (const Cell*)cells,
+                             epsrel, epsabs);
+    delete sv;
+    return 0;
+}
+
+/* Read a cellfile with the reference's reader (src/Cell.cpp:16-80). */
+REF_API int ref_read_cells(const char* path, void* out, int maxcells) {
+    int n = 0;
+    Cell* c = ReadCells(path, &n);
+    if (!c) return -1;
+    if (n <= maxcells) memcpy(out, c, sizeof(Cell) * n);
+    free(c);
+    return n;
+}
+
+REF_API int ref_sizeof_cell(void) { return (int)sizeof(Cell); }

@@ -1,0 +1,155 @@
+"""ctypes bindings of the two checkers: oracle/liboracle.so (the FP64 C
+restatement) and oracle/_ref/libopsec_ref.so (the reference's own serial
+slice, built from its sources by oracle/Makefile when /root/reference is
+present).  Used only by the tests, smoke() and bench.py's cpu_baseline leg.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libopsec_ref.so")
+
+_P, _I, _D = C.c_void_p, C.c_int, C.c_double
+
+
+class OrcXi(C.Structure):
+    _fields_ = [("kind", C.c_int), ("r768", _P), ("y768", _P), ("a", _D), ("b", _D)]
+
+
+_ORC = {
+    "orc_fmm_spline": ([_I, _P, _P, _P, _P, _P], None),
+    "orc_cubic_eval": ([_I, _P, _P, _P, _P, _P, _I, _P, _P], None),
+    "orc_xilm": ([_I, _I, _P, _P, _I, _P, _P, _I, _D, _D, _D], None),
+    "orc_qspline_grid": ([_P], None),
+    "orc_qspline_eval": ([_P, _P, _D], _D),
+    "orc_band_xi": ([_D, _D, _D], _D),
+    "orc_band_param": ([_I, _P, _P, _D, _D], _D),
+    "orc_xi_eval": ([C.POINTER(OrcXi), _D], _D),
+    "orc_box_r": ([_P, _P, _D], _D),
+    "orc_gm_integrate": ([C.POINTER(OrcXi), _I, _D, _P, _P, _D, _D, C.c_uint, C.POINTER(_D),
+                          C.POINTER(C.c_int64)], _D),
+    "orc_tent_gauss": ([C.POINTER(OrcXi), _D, _P, _P, _I], _D),
+    "orc_tent_composite": ([C.POINTER(OrcXi), _D, _P, _P, _I, _I], _D),
+    "orc_brute6": ([C.POINTER(OrcXi), _D, _P, _P, _I], _D),
+    "orc_gauss_legendre": ([_I, _P, _P], None),
+    "orc_gauss_jacobi10": ([_I, _P, _P], None),
+    "orc_sig_block": ([C.POINTER(OrcXi), _P, _I, _I, _I, _I, _D, _P, _I, _P, _I, _P, _I, _I, _D, _D, C.c_uint, _I,
+                       _I], _I),
+    "orc_project": ([_I, _P, _I, _P, _P], None),
+    "orc_estimate": ([_I, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P], _I),
+    "orc_inverse": ([_I, _P, _P], _I),
+    "orc_sqrtm": ([_I, _P, _P], None),
+    "orc_sym_eigen": ([_I, _P, _P, _P], None),
+    "orc_mlp_nweights": ([], _I),
+    "orc_mlp_forward": ([_P, _I, _P, _P], None),
+    "orc_mlp_loss_grad": ([_P, _I, _P, _P, _P], _D),
+}
+
+_REF = {
+    "ref_model_new": ([C.c_char_p], _P),
+    "ref_model_free": ([_P], None),
+    "ref_model_nparams": ([_P], _I),
+    "ref_model_param": ([_P, _I], _D),
+    "ref_model_xi": ([_P, _I, _I, _P, _P], None),
+    "ref_xilm": ([_I, _I, _I, _P, _P, _I, _P, _P, _I, _D, _D, _D], None),
+    "ref_cubic_spline": ([_I, _P, _P, _I, _P, _P], None),
+    "ref_box_sep": ([C.c_char_p, _I, _P, _P, _P], None),
+    "ref_signal_c": ([_P, _I, C.c_char_p, _P, _P, _D, _D, C.POINTER(_D)], _D),
+    "ref_sig_block": ([_P, _I, C.c_char_p, _P, _I, _P, _I, _P, _I, _P, _I, _D, _D], _I),
+    "ref_read_cells": ([C.c_char_p, _P, _I], _I),
+    "ref_sizeof_cell": ([], _I),
+}
+
+
+def _load(path, sigs):
+    lib = C.CDLL(path)
+    for name, (a, r) in sigs.items():
+        fn = getattr(lib, name)
+        fn.argtypes = a
+        fn.restype = r
+    return lib
+
+
+_orc = None
+_ref = None
+
+
+def oracle():
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle/liboracle.so missing: run __graft_entry__.build()")
+        _orc = _load(ORACLE_SO, _ORC)
+    return _orc
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        _ref = _load(REF_SO, _REF)
+    return _ref
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+# ---- convenience wrappers -----------------------------------------------------
+
+class XiHandle:
+    """An oracle xi function; keeps the table arrays alive."""
+
+    def __init__(self, kind, r768=None, y768=None, a=0.0, b=0.0):
+        self.r = None if r768 is None else np.ascontiguousarray(r768, dtype=np.float64)
+        self.y = None if y768 is None else np.ascontiguousarray(y768, dtype=np.float64)
+        self.s = OrcXi(kind, ptr(self.r), ptr(self.y), a, b)
+
+    def __call__(self, r):
+        o = oracle()
+        return np.array([o.orc_xi_eval(C.byref(self.s), float(x)) for x in np.atleast_1d(r)])
+
+
+def oracle_tables(pk_k, pk_P, band_lo, band_hi):
+    """Reference tables (prior + bands) computed by the oracle."""
+    o = oracle()
+    r = np.zeros(768)
+    o.orc_qspline_grid(ptr(r))
+    k, P = np.ascontiguousarray(pk_k, float), np.ascontiguousarray(pk_P, float)
+    tabs = []
+    y = np.zeros(768)
+    o.orc_xilm(0, len(k), ptr(k), ptr(P), 768, ptr(r), ptr(y), 32768, 0.0, 10.0, 1.0)
+    tabs.append(y.copy())
+    for lo, hi in zip(band_lo, band_hi):
+        o.orc_xilm(1, 0, None, None, 768, ptr(r), ptr(y), 8192, float(lo), float(hi), 1.0)
+        tabs.append(y.copy())
+    return r, tabs
+
+
+def survey_text(n, lo=0.0, hi=400.0, L=None, nbar=3e-4):
+    lo, hi = float(lo), float(hi)
+    L = (hi - lo) if L is None else float(L)
+    nbar = float(nbar)
+    s = "".join(f"survey.{a}Min = {lo!r}\nsurvey.{a}Max = {hi!r}\n" for a in "XYZ")
+    s += f"survey.Nx = {n}\nsurvey.Ny = {n}\nsurvey.Nz = {n}\nsurvey.nbar = {nbar!r}\nsurvey.L = {L!r}\n"
+    return s.encode()
+
+
+def write_pkfile(path, k, P):
+    np.savetxt(path, np.c_[k, P], fmt="%.17g")
+
+
+def ref_real_model(pkfile, band_lo, band_hi):
+    """Reference RealModel with explicit linear bands (Nbands, kmin, kmax)."""
+    nb = len(band_lo)
+    txt = (f"model = RealModel\nRealModel.pkfile = {pkfile}\nRealModel.Nbands = {nb}\n"
+           f"RealModel.kmin = {float(band_lo[0])!r}\nRealModel.kmax = {float(band_hi[-1])!r}\n")
+    return ref().ref_model_new(txt.encode())

@@ -1,0 +1,158 @@
+"""GPU parity of the projection (K3) and the Fisher / estimate step (K4)
+against the FP64 oracle."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from opsec_amd import lib, synth
+from tests import oracles as O
+
+pytestmark = pytest.mark.gpu
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.npz"))
+
+
+def orc_project(S, B):
+    n, m = S.shape[0], B.shape[0]
+    out = np.zeros(m * (m + 1) // 2)
+    S = np.ascontiguousarray(S)
+    B = np.ascontiguousarray(B)
+    O.oracle().orc_project(n, O.ptr(S), m, O.ptr(B), O.ptr(out))
+    return out
+
+
+def rand_sym(n, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.normal(size=(n, n))
+    return A + A.T
+
+
+@pytest.mark.parametrize("n,m", [(300, 70), (128, 128), (257, 129)])
+def test_project_dense_fp64(ctx, n, m):
+    S = rand_sym(n, n)
+    B = np.random.default_rng(m).normal(size=(m, n))
+    got = ctx.project_dense(S, B, lib.OPSEC_PREC_FP64)
+    ref = orc_project(S, B)
+    np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("prec,tol", [(lib.OPSEC_PREC_FP32, 2e-5), (lib.OPSEC_PREC_BF16X3, 5e-5)])
+def test_project_dense_mixed(ctx, prec, tol):
+    n, m = 400, 96
+    S = rand_sym(n, 1)
+    B = np.random.default_rng(2).normal(size=(m, n))
+    got = ctx.project_dense(S, B, prec)
+    ref = orc_project(S, B)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print("mixed precision", prec, "max rel err", err)
+    assert err < tol
+
+
+def _box_setup(ctx, n=8, nb=4, lo=0.0, hi=200.0):
+    cells = synth.box_cells(n, lo, hi)
+    sv = lib.survey_box(n, lo, hi)
+    k, P = GOLD["pk_k"], GOLD["pk_P"]
+    blo, bhi = synth.linear_bands(nb, 0.02, 0.2)
+    ctx.model_real(k, P, blo, bhi, lib.OPSEC_XI_TABLE)
+    ctx.cells_upload(cells, sv)
+    return cells, (k, P, blo, bhi)
+
+
+def test_project_pair_tables_fp64_vs_oracle(ctx):
+    """Fused S_p generation + projection vs oracle assembly + oracle projection."""
+    n, nb = 8, 4
+    cells, (k, P, blo, bhi) = _box_setup(ctx, n, nb)
+    N = len(cells)
+    q = ctx.quad(lib.OPSEC_QUAD_GM)
+    funcs = list(range(nb + 1))
+    ctx.pair_tables(funcs, q)
+    rng = np.random.default_rng(9)
+    m = 100
+    B = np.linalg.qr(rng.normal(size=(N, m)))[0].T.copy()
+    ctx.modes_upload(B)
+    got = ctx.project(funcs, lib.OPSEC_PREC_FP64)
+    r768, tabs = O.oracle_tables(k, P, blo, bhi)
+    rows = np.arange(N, dtype=np.int32)
+    for t, f in enumerate(funcs):
+        xi = O.XiHandle(0, r768, tabs[f])
+        S = np.zeros((N, N))
+        O.oracle().orc_sig_block(C.byref(xi.s), cells.ctypes.data, N, n, n, n, 200.0, O.ptr(rows), N, O.ptr(rows), N,
+                                 O.ptr(S), N, 0, 1e-5, 1e-10, 50000000, 0, 0)
+        ref = orc_project(S, B)
+        np.testing.assert_allclose(got[t], ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max(), err_msg=f"f={f}")
+    # mixed precisions against the FP64 result
+    for prec in (lib.OPSEC_PREC_FP32, lib.OPSEC_PREC_BF16X3):
+        mp = ctx.project(funcs, prec)
+        err = np.abs(mp - got).max(axis=1) / np.abs(got).max(axis=1)
+        print("pair-table projection, precision", prec, "max rel err per function", err)
+        assert err.max() < 5e-5
+
+
+def _packed(Mfull):
+    m = Mfull.shape[0]
+    return np.array([Mfull[i, j] for j in range(m) for i in range(j + 1)])
+
+
+def _orc_estimate(lam, D, y, prior, mixing):
+    npar, M = len(prior), len(lam)
+    out = {k: np.zeros(npar) for k in ("phat", "q", "f")}
+    out.update({k: np.zeros((npar, npar)) for k in ("cov", "F", "W", "M")})
+    lam, D, y, prior = [np.ascontiguousarray(a, dtype=float) for a in (lam, D, y, prior)]
+    rc = O.oracle().orc_estimate(M, npar, O.ptr(lam), O.ptr(D), O.ptr(y), O.ptr(prior), mixing,
+                                 *[O.ptr(out[k]) for k in ("phat", "cov", "F", "W", "M", "q", "f")])
+    assert rc == 0
+    return out
+
+
+@pytest.mark.parametrize("mixing", [0, 1, 2])
+def test_estimate_fp64_vs_oracle(ctx, mixing):
+    rng = np.random.default_rng(11 + mixing)
+    M, npar = 150, 7
+    lam = np.sort(rng.uniform(0.05, 5, M))[::-1].copy()
+    D = []
+    for p in range(npar):
+        A = rng.normal(size=(M, M)) * 0.1
+        A = A + A.T + np.diag(rng.uniform(0.5, 1.5, M))
+        D.append(_packed(A))
+    D = np.array(D)
+    y = np.sqrt(1 + lam) * rng.normal(size=M)
+    prior = rng.uniform(0.5, 2.0, npar)
+    got = ctx.estimate(lam, D, y, prior, mixing, lib.OPSEC_PREC_FP64)
+    ref = _orc_estimate(lam, D, y, prior, mixing)
+    for k in ("F", "q", "f", "M", "W", "phat", "cov"):
+        np.testing.assert_allclose(got[k], ref[k], rtol=1e-10, atol=1e-10 * np.abs(ref[k]).max(), err_msg=k)
+    np.testing.assert_allclose(got["W"].sum(1), 1.0, rtol=1e-12)
+
+
+def test_estimate_fp32_traces(ctx):
+    rng = np.random.default_rng(5)
+    M, npar = 300, 5
+    lam = rng.uniform(0.05, 5, M)
+    D = np.array([_packed((lambda A: A + A.T)(rng.normal(size=(M, M)))) for _ in range(npar)])
+    y = np.sqrt(1 + lam) * rng.normal(size=M)
+    prior = rng.uniform(0.5, 2, npar)
+    got = ctx.estimate(lam, D, y, prior, 0, lib.OPSEC_PREC_FP32)
+    ref = _orc_estimate(lam, D, y, prior, 0)
+    errF = np.abs(got["F"] - ref["F"]).max() / np.abs(ref["F"]).max()
+    print("FP32 trace F rel err", errF)
+    assert errF < 1e-5
+
+
+def test_estimate_device_after_projection(ctx):
+    n, nb = 6, 3
+    cells, _ = _box_setup(ctx, n, nb, 0.0, 150.0)
+    N = len(cells)
+    ctx.pair_tables(list(range(nb + 1)), ctx.quad(lib.OPSEC_QUAD_GM))
+    m = 50
+    B = np.linalg.qr(np.random.default_rng(3).normal(size=(N, m)))[0].T.copy()
+    ctx.modes_upload(B)
+    D = ctx.project(list(range(1, nb + 1)), lib.OPSEC_PREC_FP64)
+    rng = np.random.default_rng(4)
+    lam = rng.uniform(0.1, 3, m)
+    y = rng.normal(size=m)
+    prior = ctx.params()
+    a = ctx.estimate_device(lam, y, prior)
+    b = ctx.estimate(lam, D, y, prior)
+    for k in ("F", "q", "f", "phat"):
+        np.testing.assert_allclose(a[k], b[k], rtol=1e-12, atol=1e-14 * np.abs(b[k]).max())

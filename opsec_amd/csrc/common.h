@@ -143,6 +143,19 @@ int survey_from_abi(opsec_ctx* ctx, const opsec_survey* in, SurveyDev* out);
 // the functions in funcs (K1).
 int build_pair_tables(opsec_ctx* ctx, int nf, const int* funcs, const opsec_quad* q);
 
+// Bounds of the two cells of a pair integral (cell 1 then cell 2).
+struct PairGeo {
+    double lo1[3], hi1[3], lo2[3], hi2[3];
+};
+
+// Quadrature of the given keys x functions into dQ[f * nkeys + key].
+int run_quadrature_public(opsec_ctx* ctx, int nf, const int* funcs, const std::vector<int>& keys,
+                          const std::vector<PairGeo>& geo, const std::vector<int>& nearflag, const opsec_quad* q,
+                          double* dQ, size_t nkeys);
+// Geometry of key (kx, ky, kz) of the device survey's grid: the grid cell at
+// the origin and the cell at offset k, as Survey::CreateEmptyCell lays them out.
+PairGeo grid_pair_geo(const SurveyDev& sv, int kx, int ky, int kz);
+
 // Rows of the packed upper triangle: column j starts at j(j+1)/2.
 inline int64_t packed_size(int64_t m) { return m * (m + 1) / 2; }
 

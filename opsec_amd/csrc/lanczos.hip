@@ -1,0 +1,835 @@
+// K5: KL eigenmodes by block Lanczos (replaces ParpackSolver,
+// src/ParpackSolver.cpp:68-227, and the klt solve, src/klt.cpp:160-248).
+//
+// The reference runs PARPACK's implicitly restarted Lanczos (which = LA,
+// tol = 1e-5, ncv = 2 nev) over a pdgemm matvec.  On one MI355X the whole
+// Krylov basis stays in HBM, so this solver never restarts:
+//   * block Lanczos, block size b, operator = the device-resident signal
+//     matrix (dense FP64, streamed from HBM by the MFMA GEMM);
+//   * full re-orthogonalisation against every previous block, twice (CGS2),
+//     as FP64 MFMA GEMMs;
+//   * block orthonormalisation by SVQB (eigen-decomposition of the b x b Gram
+//     matrix), which replaces numerically dependent directions by fresh random
+//     ones -- the periodic box's signal matrix has cubic-symmetry multiplets
+//     that single-vector Lanczos cannot resolve;
+//   * Rayleigh-Ritz on the block-tridiagonal T: GPU Householder
+//     tridiagonalisation (symmetric rank-2 updates), host implicit QL for the
+//     eigenvalues, host inverse iteration (with re-orthogonalisation inside
+//     clusters) for the wanted eigenvectors, GPU back-transformation;
+//   * convergence: |B_j z_last| <= tol |theta_max| for each wanted Ritz pair.
+// Output: eigenvalues descending, eigenvectors mode-major (modefile layout).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <vector>
+
+#include "common.h"
+#include "gemm.h"
+
+namespace opsec {
+
+// ---- FP64 GEMM variants used by the solver ------------------------------------
+
+struct LA_RowMajor : SrcDenseRowMajor {
+    __device__ __forceinline__ void p_shift(int, size_t) {}
+};
+struct LA_ColMajor : SrcDenseColMajor {
+    __device__ __forceinline__ void p_shift(int, size_t) {}
+};
+struct LA_KCol : SrcKColMajor {   // operand X with X[i][k] = p[i*ld + k]
+    __device__ __forceinline__ void p_shift(int, size_t) {}
+};
+
+template <int MODE>   // 0 store, 1 subtract
+struct LPut {
+    double* p;
+    size_t ld;
+    __device__ __forceinline__ void put(int, int m, int n, int M, int N, double v) const {
+        if (m < M && n < N) {
+            if (MODE == 0) p[(size_t)n * ld + m] = v;
+            else p[(size_t)n * ld + m] -= v;
+        }
+    }
+};
+
+template <typename SrcA, typename SrcB, typename Out>
+__global__ __launch_bounds__(GT_THREADS) void lgemm_kernel(SrcA A, SrcB B, Out O, int M, int N, int K) {
+    __shared__ double As[16][GT_M + 4];
+    __shared__ double Bs[16][GT_N + 4];
+    const int m0 = blockIdx.x * GT_M, n0 = blockIdx.y * GT_N;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    // split-K: blockIdx.z owns k range [kz0, kz1)
+    const int kchunk = ((K + gridDim.z - 1) / gridDim.z + 15) / 16 * 16;
+    const int kz0 = blockIdx.z * kchunk, kz1 = min(K, kz0 + kchunk);
+    d4_t acc[4][4];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) acc[i][j] = d4_t{0, 0, 0, 0};
+    const int t = threadIdx.x;
+    for (int k0 = kz0; k0 < kz1; k0 += 16) {
+        __syncthreads();
+        if (SrcA::k_contig) {
+            int i = t >> 1, kb = (t & 1) * 8;
+            for (int e = 0; e < 8; e++) {
+                int m = m0 + i, k = k0 + kb + e;
+                As[kb + e][i] = (m < M && k < kz1) ? A.get(m, k) : 0.0;
+            }
+        } else {
+            int k = t >> 4, ib = (t & 15) * 8;
+            for (int e = 0; e < 8; e++) {
+                int m = m0 + ib + e, kk = k0 + k;
+                As[k][ib + e] = (m < M && kk < kz1) ? A.get(m, kk) : 0.0;
+            }
+        }
+        {
+            int i = t >> 1, kb = (t & 1) * 8;
+            for (int e = 0; e < 8; e++) {
+                int n = n0 + i, k = k0 + kb + e;
+                Bs[kb + e][i] = (n < N && k < kz1) ? B.get(n, k) : 0.0;
+            }
+        }
+        __syncthreads();
+        for (int kk = 0; kk < 4; kk++) {
+            double af[4], bf[4];
+            const int kr = kk * 4 + (lane >> 4);
+            for (int i = 0; i < 4; i++) af[i] = As[kr][wm + i * 16 + (lane & 15)];
+            for (int j = 0; j < 4; j++) bf[j] = Bs[kr][wn + j * 16 + (lane & 15)];
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            const int n = n0 + wn + j * 16 + (lane & 15);
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+                if (gridDim.z == 1) O.put(0, m, n, M, N, acc[i][j][r]);
+                else if (m < M && n < N) atomicAdd(&O.p[(size_t)n * O.ld + m], acc[i][j][r]);
+            }
+        }
+}
+
+__global__ void fill_kernel(double* p, size_t n, double v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+struct Dev {
+    opsec_ctx* ctx;
+    hipStream_t s;
+    int err = 0;
+    void fill(double* p, size_t n, double v) {
+        if (n == 0) return;
+        hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    }
+    // H (p x q, ld p) = X^T Y, X: N x p col-major (ld ldx), Y: N x q col-major
+    void gemm_tn(const double* X, size_t ldx, const double* Y, size_t ldy, int N, int p, int q, double* H) {
+        LA_KCol A;
+        A.p = X;
+        A.ld = ldx;
+        LA_KCol B;
+        B.p = Y;
+        B.ld = ldy;
+        int gx = (p + GT_M - 1) / GT_M, gy = (q + GT_N - 1) / GT_N;
+        int splits = std::max(1, std::min(64, (2 * ctx->num_cus) / std::max(1, gx * gy)));
+        splits = std::min(splits, std::max(1, N / 512));
+        if (splits > 1) fill(H, (size_t)p * q, 0.0);
+        hipLaunchKernelGGL((lgemm_kernel<LA_KCol, LA_KCol, LPut<0>>), dim3(gx, gy, splits), dim3(GT_THREADS), 0, s, A,
+                           B, LPut<0>{H, (size_t)p}, p, q, N);
+    }
+    // Y (N x q) op= X (N x p) H (p x q); op: 0 store, 1 subtract
+    void gemm_nn(int mode, double* Y, size_t ldy, const double* X, size_t ldx, const double* H, size_t ldh, int N,
+                 int p, int q) {
+        LA_ColMajor A;
+        A.p = X;
+        A.ld = ldx;
+        LA_KCol B;
+        B.p = H;
+        B.ld = ldh;
+        dim3 g((N + GT_M - 1) / GT_M, (q + GT_N - 1) / GT_N, 1);
+        if (mode == 0)
+            hipLaunchKernelGGL((lgemm_kernel<LA_ColMajor, LA_KCol, LPut<0>>), g, dim3(GT_THREADS), 0, s, A, B,
+                               LPut<0>{Y, ldy}, N, q, p);
+        else
+            hipLaunchKernelGGL((lgemm_kernel<LA_ColMajor, LA_KCol, LPut<1>>), g, dim3(GT_THREADS), 0, s, A, B,
+                               LPut<1>{Y, ldy}, N, q, p);
+    }
+    // Y (N x q) = S X, S dense row-major N x N (symmetric)
+    void apply(const double* S, int N, const double* X, int q, double* Y) {
+        LA_RowMajor A;
+        A.p = S;
+        A.ld = N;
+        LA_KCol B;
+        B.p = X;
+        B.ld = N;
+        dim3 g((N + GT_M - 1) / GT_M, (q + GT_N - 1) / GT_N, 1);
+        hipLaunchKernelGGL((lgemm_kernel<LA_RowMajor, LA_KCol, LPut<0>>), g, dim3(GT_THREADS), 0, s, A, B,
+                           LPut<0>{Y, (size_t)N}, N, q, N);
+    }
+    int sync() {
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return set_error(ctx, OPSEC_EHIP, "eigen-solver: %s", hipGetErrorString(e));
+        return OPSEC_OK;
+    }
+};
+
+// ---- Householder tridiagonalisation of a dense symmetric K x K (col-major) --
+
+// One workgroup: Householder vector of column k (rows k+1..K-1), in place.
+__global__ __launch_bounds__(1024) void hh_vector_kernel(double* A, int K, int k, double* tau, double* e) {
+    __shared__ double red[1024];
+    __shared__ double sh_scale, sh_tau, sh_beta;
+    const int m = K - k - 1;
+    double* x = A + (size_t)k * K + (k + 1);
+    double s = 0;
+    for (int i = 1 + threadIdx.x; i < m; i += blockDim.x) s += x[i] * x[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double sigma = red[0], alpha = x[0];
+        if (sigma == 0.0) {
+            sh_tau = 0.0;
+            sh_beta = alpha;
+            sh_scale = 0.0;
+        } else {
+            double nrm = sqrt(alpha * alpha + sigma);
+            double beta = alpha <= 0 ? nrm : -nrm;
+            sh_tau = (beta - alpha) / beta;
+            sh_scale = 1.0 / (alpha - beta);
+            sh_beta = beta;
+        }
+        tau[k] = sh_tau;
+        e[k] = sh_beta;
+    }
+    __syncthreads();
+    double sc = sh_scale;
+    for (int i = 1 + threadIdx.x; i < m; i += blockDim.x) x[i] *= sc;
+    if (threadIdx.x == 0) x[0] = 1.0;
+}
+
+// p[i] += sum_{j in tile} A22[i][j] v[j]   (A22 = A[k+1:, k+1:], v = A[k+1:, k])
+__global__ __launch_bounds__(256) void hh_symv_kernel(const double* A, int K, int k, double* p) {
+    __shared__ double vs[256];
+    const int m = K - k - 1;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int j0 = blockIdx.y * 256;
+    const double* v = A + (size_t)k * K + (k + 1);
+    vs[threadIdx.x] = (j0 + threadIdx.x < m) ? v[j0 + threadIdx.x] : 0.0;
+    __syncthreads();
+    if (i >= m) return;
+    double acc = 0;
+    const int jn = min(256, m - j0);
+    const double* col = A + (size_t)(k + 1 + j0) * K + (k + 1) + i;
+    for (int jj = 0; jj < jn; jj++) acc += col[(size_t)jj * K] * vs[jj];
+    atomicAdd(&p[i], acc);
+}
+
+// w = tau p - (tau/2)(tau p . v) v  (one workgroup)
+__global__ __launch_bounds__(1024) void hh_w_kernel(const double* A, int K, int k, const double* tau, double* p) {
+    __shared__ double red[1024];
+    const int m = K - k - 1;
+    const double* v = A + (size_t)k * K + (k + 1);
+    const double t = tau[k];
+    double s = 0;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) s += t * p[i] * v[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double kap = 0.5 * t * red[0];
+    for (int i = threadIdx.x; i < m; i += blockDim.x) p[i] = t * p[i] - kap * v[i];
+}
+
+// A22 -= v w^T + w v^T
+__global__ __launch_bounds__(256) void hh_rank2_kernel(double* A, int K, int k, const double* w) {
+    const int m = K - k - 1;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= m || j >= m) return;
+    const double* v = A + (size_t)k * K + (k + 1);
+    double* a = A + (size_t)(k + 1 + j) * K + (k + 1) + i;
+    *a -= v[i] * w[j] + w[i] * v[j];
+}
+
+// Back-transformation: Z[k+1:, :] -= tau_k v_k (v_k^T Z[k+1:, :])
+__global__ __launch_bounds__(256) void hh_apply_dot_kernel(const double* A, int K, int k, const double* Z, int nz,
+                                                           double* r) {
+    __shared__ double red[256];
+    const int m = K - k - 1;
+    const int c = blockIdx.x;
+    const double* v = A + (size_t)k * K + (k + 1);
+    const double* z = Z + (size_t)c * K + (k + 1);
+    double s = 0;
+    for (int i = threadIdx.x; i < m; i += 256) s += v[i] * z[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) r[c] = red[0];
+}
+
+__global__ __launch_bounds__(256) void hh_apply_upd_kernel(const double* A, int K, int k, const double* tau, double* Z,
+                                                           int nz, const double* r) {
+    const int m = K - k - 1;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= m) return;
+    const double* v = A + (size_t)k * K + (k + 1);
+    Z[(size_t)c * K + (k + 1) + i] -= tau[k] * r[c] * v[i];
+}
+
+__global__ void diag_kernel(const double* A, int K, double* d, double* elast) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < K) d[i] = A[(size_t)i * K + i];
+    if (i == 0 && K >= 2) *elast = A[(size_t)(K - 2) * K + (K - 1)];
+}
+
+// ---- host: symmetric tridiagonal eigenpairs --------------------------------------
+
+// Eigenvalues of the unreduced tridiagonal (d, e) by implicit QL with
+// Wilkinson-type shifts (no vectors).  e[i] couples i and i+1.
+static void tql_values(int n, std::vector<double>& d, std::vector<double> e) {
+    if (n <= 1) return;
+    e.push_back(0.0);
+    for (int l = 0; l < n; l++) {
+        int iter = 0, m;
+        do {
+            for (m = l; m < n - 1; m++) {
+                double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+                if (std::fabs(e[m]) <= 1e-300 + 2.2e-16 * dd) break;
+            }
+            if (m != l) {
+                if (iter++ == 200) break;
+                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = std::hypot(g, 1.0);
+                g = d[m] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
+                double s = 1.0, c = 1.0, p = 0.0;
+                int i;
+                for (i = m - 1; i >= l; i--) {
+                    double f = s * e[i], b = c * e[i];
+                    e[i + 1] = (r = std::hypot(f, g));
+                    if (r == 0.0) {
+                        d[i + 1] -= p;
+                        e[m] = 0.0;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0 * c * b;
+                    d[i + 1] = g + (p = s * r);
+                    g = c * r - b;
+                }
+                if (r == 0.0 && i >= l) continue;
+                d[l] -= p;
+                e[l] = g;
+                e[m] = 0.0;
+            }
+        } while (m != l);
+    }
+}
+
+// Solve (T - lam I) x = b for the symmetric tridiagonal T (n, d, e) by
+// Gaussian elimination with partial pivoting (row interchanges as in LAPACK's
+// dgttrf/dgttrs); zero pivots are replaced by `tiny`.
+static void tri_solve(int n, const double* d, const double* e, double lam, double tiny, double* x) {
+    if (n == 1) {
+        double p = d[0] - lam;
+        x[0] /= (p == 0.0 ? tiny : p);
+        return;
+    }
+    std::vector<double> dl(e, e + n - 1), dd(n), du(e, e + n - 1), du2(std::max(0, n - 2), 0.0);
+    std::vector<char> swp(n, 0);
+    for (int i = 0; i < n; i++) dd[i] = d[i] - lam;
+    for (int i = 0; i < n - 1; i++) {
+        if (std::fabs(dd[i]) >= std::fabs(dl[i])) {
+            if (dd[i] == 0.0) dd[i] = tiny;
+            double fact = dl[i] / dd[i];
+            dl[i] = fact;
+            dd[i + 1] -= fact * du[i];
+        } else {
+            double fact = dd[i] / dl[i];
+            dd[i] = dl[i];
+            dl[i] = fact;
+            double temp = dd[i + 1];
+            dd[i + 1] = du[i] - fact * temp;
+            if (i < n - 2) {
+                du2[i] = du[i + 1];
+                du[i + 1] = -fact * du2[i];
+            }
+            du[i] = temp;
+            swp[i] = 1;
+        }
+    }
+    if (dd[n - 1] == 0.0) dd[n - 1] = tiny;
+    for (int i = 0; i < n - 1; i++) {
+        if (swp[i]) std::swap(x[i], x[i + 1]);
+        x[i + 1] -= dl[i] * x[i];
+    }
+    x[n - 1] /= dd[n - 1];
+    x[n - 2] = (x[n - 2] - du[n - 2] * x[n - 1]) / dd[n - 2];
+    for (int i = n - 3; i >= 0; i--) x[i] = (x[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) / dd[i];
+}
+
+struct TriBlock {
+    int off, n;
+};
+
+// Top-nev eigenpairs of the symmetric tridiagonal (d, e), vectors as columns of
+// Z (n x nev, column-major), eigenvalues descending.
+static void tridiag_top(int n, const std::vector<double>& d, const std::vector<double>& e, int nev,
+                        std::vector<double>& evals, std::vector<double>& Z) {
+    double tnorm = 0;
+    for (int i = 0; i < n; i++)
+        tnorm = std::max(tnorm, std::fabs(d[i]) + (i > 0 ? std::fabs(e[i - 1]) : 0) + (i < n - 1 ? std::fabs(e[i]) : 0));
+    if (tnorm == 0) tnorm = 1;
+    // split into unreduced blocks
+    std::vector<TriBlock> blocks;
+    int start = 0;
+    for (int i = 0; i < n - 1; i++)
+        if (std::fabs(e[i]) <= 2.2e-16 * (std::fabs(d[i]) + std::fabs(d[i + 1])) + 1e-300) {
+            blocks.push_back({start, i + 1 - start});
+            start = i + 1;
+        }
+    blocks.push_back({start, n - start});
+    struct EV {
+        double v;
+        int blk;
+    };
+    std::vector<EV> all;
+    all.reserve(n);
+    for (size_t bi = 0; bi < blocks.size(); bi++) {
+        const TriBlock& B = blocks[bi];
+        std::vector<double> dd(d.begin() + B.off, d.begin() + B.off + B.n);
+        std::vector<double> ee(e.begin() + B.off, e.begin() + B.off + std::max(0, B.n - 1));
+        tql_values(B.n, dd, ee);
+        for (double v : dd) all.push_back({v, (int)bi});
+    }
+    std::sort(all.begin(), all.end(), [](const EV& a, const EV& b) { return a.v > b.v; });
+    nev = std::min(nev, n);
+    evals.resize(nev);
+    Z.assign((size_t)n * nev, 0.0);
+    // group wanted eigenvalues by block, ascending order inside each block
+    std::vector<std::vector<int>> per_block(blocks.size());
+    for (int t = 0; t < nev; t++) {
+        evals[t] = all[t].v;
+        per_block[all[t].blk].push_back(t);
+    }
+    const double ortol = 1e-7 * tnorm, eps = 2.2e-16;
+#pragma omp parallel for schedule(dynamic)
+    for (int bi = 0; bi < (int)blocks.size(); bi++) {
+        const TriBlock& B = blocks[bi];
+        std::vector<int>& idx = per_block[bi];
+        if (idx.empty()) continue;
+        std::sort(idx.begin(), idx.end(), [&](int a, int b) { return evals[a] > evals[b]; });
+        const double* db = &d[B.off];
+        const double* eb = B.n > 1 ? &e[B.off] : nullptr;
+        std::vector<double> x(B.n);
+        std::mt19937_64 rng(1234567 + bi);
+        std::normal_distribution<double> nd;
+        double prev = 0;
+        int cstart = 0;
+        for (size_t t = 0; t < idx.size(); t++) {
+            int col = idx[t];
+            double lam = evals[col];
+            if (t > 0 && std::fabs(prev - lam) > ortol) cstart = (int)t;
+            // separate coincident shifts slightly (as LAPACK's dstein does)
+            if (t > 0 && std::fabs(prev - lam) < 10 * eps * tnorm) lam = prev - 10 * eps * tnorm;
+            prev = lam;
+            if (B.n == 1) {
+                Z[(size_t)col * n + B.off] = 1.0;
+                continue;
+            }
+            for (int i = 0; i < B.n; i++) x[i] = nd(rng);
+            for (int it = 0; it < 5; it++) {
+                tri_solve(B.n, db, eb, lam, eps * tnorm, x.data());
+                // orthogonalise against earlier vectors of the cluster
+                for (int s2 = cstart; s2 < (int)t; s2++) {
+                    const double* z = &Z[(size_t)idx[s2] * n + B.off];
+                    double dot = 0;
+                    for (int i = 0; i < B.n; i++) dot += z[i] * x[i];
+                    for (int i = 0; i < B.n; i++) x[i] -= dot * z[i];
+                }
+                double nrm = 0;
+                for (int i = 0; i < B.n; i++) nrm += x[i] * x[i];
+                nrm = std::sqrt(nrm);
+                if (nrm == 0) {
+                    for (int i = 0; i < B.n; i++) x[i] = nd(rng);
+                    continue;
+                }
+                for (int i = 0; i < B.n; i++) x[i] /= nrm;
+            }
+            double* z = &Z[(size_t)col * n + B.off];
+            for (int i = 0; i < B.n; i++) z[i] = x[i];
+        }
+    }
+}
+
+// Top-nev eigenpairs of the dense symmetric K x K device matrix dA (destroyed).
+// evals (host, descending), dZ (device K x nev col-major).
+static int dense_eigen_top(Dev& D, double* dA, int K, int nev, std::vector<double>& evals, double* dZ) {
+    opsec_ctx* ctx = D.ctx;
+    nev = std::min(nev, K);
+    std::vector<double> d(K), e(std::max(1, K - 1), 0.0);
+    double *dtau = nullptr, *de = nullptr, *dp = nullptr, *dd = nullptr;
+    OPSEC_HIP_TRY(ctx, hipMalloc(&dtau, (size_t)K * 8 + 8));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&de, (size_t)K * 8 + 8));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&dp, (size_t)std::max(K, nev) * 8 + 8));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&dd, (size_t)K * 8 + 8));
+    for (int k = 0; k + 2 < K; k++) {
+        const int m = K - k - 1;
+        hipLaunchKernelGGL(hh_vector_kernel, dim3(1), dim3(1024), 0, D.s, dA, K, k, dtau, de);
+        D.fill(dp, m, 0.0);
+        hipLaunchKernelGGL(hh_symv_kernel, dim3((m + 255) / 256, (m + 255) / 256), dim3(256), 0, D.s, dA, K, k, dp);
+        hipLaunchKernelGGL(hh_w_kernel, dim3(1), dim3(1024), 0, D.s, dA, K, k, dtau, dp);
+        hipLaunchKernelGGL(hh_rank2_kernel, dim3((m + 255) / 256, m), dim3(256), 0, D.s, dA, K, k, dp);
+    }
+    hipLaunchKernelGGL(diag_kernel, dim3((K + 255) / 256), dim3(256), 0, D.s, dA, K, dd, de + (K >= 2 ? K - 2 : 0));
+    int rc = D.sync();
+    if (rc) return rc;
+    OPSEC_HIP_TRY(ctx, hipMemcpy(d.data(), dd, (size_t)K * 8, hipMemcpyDeviceToHost));
+    if (K >= 2) OPSEC_HIP_TRY(ctx, hipMemcpy(e.data(), de, (size_t)(K - 1) * 8, hipMemcpyDeviceToHost));
+    std::vector<double> Z;
+    tridiag_top(K, d, e, nev, evals, Z);
+    OPSEC_HIP_TRY(ctx, hipMemcpy(dZ, Z.data(), (size_t)K * nev * 8, hipMemcpyHostToDevice));
+    for (int k = K - 3; k >= 0; k--) {
+        const int m = K - k - 1;
+        hipLaunchKernelGGL(hh_apply_dot_kernel, dim3(nev), dim3(256), 0, D.s, dA, K, k, dZ, nev, dp);
+        hipLaunchKernelGGL(hh_apply_upd_kernel, dim3((m + 255) / 256, nev), dim3(256), 0, D.s, dA, K, k, dtau, dZ,
+                           nev, dp);
+    }
+    rc = D.sync();
+    hipFree(dtau);
+    hipFree(de);
+    hipFree(dp);
+    hipFree(dd);
+    return rc;
+}
+
+// ---- block orthonormalisation (SVQB) -------------------------------------------
+
+static void host_sym_eigen(int n, std::vector<double> A, std::vector<double>& w, std::vector<double>& V) {
+    V.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) V[i * n + i] = 1;
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0, tot = 0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                tot += A[i * n + j] * A[i * n + j];
+                if (i != j) off += A[i * n + j] * A[i * n + j];
+            }
+        if (off <= 1e-32 * tot || tot == 0) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                double apq = A[p * n + q];
+                if (std::fabs(apq) < 1e-300) continue;
+                double th = (A[q * n + q] - A[p * n + p]) / (2 * apq);
+                double t = (th >= 0 ? 1 : -1) / (std::fabs(th) + std::sqrt(th * th + 1));
+                double c = 1 / std::sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < n; k++) {
+                    double akp = A[k * n + p], akq = A[k * n + q];
+                    A[k * n + p] = c * akp - s * akq;
+                    A[k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {
+                    double apk = A[p * n + k], aqk = A[q * n + k];
+                    A[p * n + k] = c * apk - s * aqk;
+                    A[q * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++) {
+                    double vkp = V[k * n + p], vkq = V[k * n + q];
+                    V[k * n + p] = c * vkp - s * vkq;
+                    V[k * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    w.resize(n);
+    for (int i = 0; i < n; i++) w[i] = A[i * n + i];
+}
+
+struct LanczosWork {
+    double* V = nullptr;      // N x Kmax
+    double* W = nullptr;      // N x b
+    double* H = nullptr;      // Kmax x b
+    double* R = nullptr;      // N x b random
+    double* small = nullptr;  // b x b
+};
+
+// Orthonormalise the N x b block X against V[:, 0:kprev] (twice) and itself
+// (SVQB, twice).  On return X = X_orth * Bmat (b x b, row-major host) for the
+// part of X inside the new directions; dependent directions are replaced by
+// random orthonormal ones with zero coefficient.
+static int orthonormalise_block(Dev& D, LanczosWork& L, int N, int b, int kprev, double* X, std::vector<double>& Bmat,
+                                std::mt19937_64& rng) {
+    opsec_ctx* ctx = D.ctx;
+    std::normal_distribution<double> nd;
+    Bmat.assign((size_t)b * b, 0.0);
+    for (int i = 0; i < b; i++) Bmat[i * b + i] = 1.0;
+    for (int round = 0; round < 3; round++) {
+        // project out previous basis, twice
+        for (int pass = 0; pass < 2 && kprev > 0; pass++) {
+            D.gemm_tn(L.V, N, X, N, N, kprev, b, L.H);
+            D.gemm_nn(1, X, N, L.V, N, L.H, kprev, N, kprev, b);
+        }
+        int rc = D.sync();
+        if (rc) return rc;
+        // SVQB twice
+        bool replaced = false;
+        for (int pass = 0; pass < 2; pass++) {
+            D.gemm_tn(X, N, X, N, N, b, b, L.small);
+            if ((rc = D.sync())) return rc;
+            std::vector<double> G((size_t)b * b), w, U;
+            OPSEC_HIP_TRY(ctx, hipMemcpy(G.data(), L.small, (size_t)b * b * 8, hipMemcpyDeviceToHost));
+            for (int i = 0; i < b; i++)
+                for (int j = 0; j < i; j++) G[i * b + j] = G[j * b + i] = 0.5 * (G[i * b + j] + G[j * b + i]);
+            host_sym_eigen(b, G, w, U);   // G = U diag(w) U^T, U column i = eigvec i (U[k*b+i])
+            double wmax = *std::max_element(w.begin(), w.end());
+            const double thr = std::max(wmax, 1e-300) * 1e-20;
+            // X <- X U diag(1/sqrt(w)); coefficients Bnew = diag(sqrt(w)) U^T
+            std::vector<double> Mh((size_t)b * b, 0.0), Bnew((size_t)b * b, 0.0);
+            std::vector<int> bad;
+            for (int i = 0; i < b; i++) {
+                bool ok = w[i] > thr;
+                if (!ok) bad.push_back(i);
+                double sw = ok ? std::sqrt(w[i]) : 0.0;
+                for (int k = 0; k < b; k++) {
+                    Mh[(size_t)i * b + k] = ok ? U[k * b + i] / sw : 0.0;   // column-major b x b: (k, i)
+                    Bnew[i * b + k] = sw * U[k * b + i];
+                }
+            }
+            OPSEC_HIP_TRY(ctx, hipMemcpy(L.small, Mh.data(), (size_t)b * b * 8, hipMemcpyHostToDevice));
+            OPSEC_HIP_TRY(ctx, hipMemcpy(L.W, X, (size_t)N * b * 8, hipMemcpyDeviceToDevice));
+            D.gemm_nn(0, X, N, L.W, N, L.small, b, N, b, b);
+            if ((rc = D.sync())) return rc;
+            // Bmat <- Bnew * Bmat
+            std::vector<double> T((size_t)b * b, 0.0);
+            for (int i = 0; i < b; i++)
+                for (int k = 0; k < b; k++)
+                    for (int j = 0; j < b; j++) T[i * b + j] += Bnew[i * b + k] * Bmat[k * b + j];
+            Bmat.swap(T);
+            if (!bad.empty()) {
+                // replace dependent directions with random vectors (zero coefficient rows)
+                std::vector<double> r((size_t)N);
+                for (int c : bad) {
+                    for (int i = 0; i < N; i++) r[i] = nd(rng);
+                    OPSEC_HIP_TRY(ctx, hipMemcpy(X + (size_t)c * N, r.data(), (size_t)N * 8, hipMemcpyHostToDevice));
+                    for (int j = 0; j < b; j++) Bmat[c * b + j] = 0.0;
+                }
+                replaced = true;
+                break;   // redo projection against the basis
+            }
+        }
+        if (!replaced) return OPSEC_OK;
+    }
+    return OPSEC_OK;
+}
+
+// Block Lanczos on the dense symmetric device matrix S (N x N row-major).
+static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double tol, int b, int maxdim, double* evals,
+                         double* evecs, int* nconv) {
+    Dev D{ctx, ctx->stream};
+    nev = std::min(nev, N);
+    if (b <= 0) b = std::min(128, std::max(1, N / 4));
+    b = std::max(1, std::min(b, N));
+    if (maxdim <= 0) maxdim = std::max(3 * nev + 4 * b, 8 * b);
+    int Kmax = std::min(N, ((maxdim + b - 1) / b) * b);
+    if (Kmax < nev) Kmax = std::min(N, ((nev + b - 1) / b) * b);
+    Kmax = std::max(b, (Kmax / b) * b);
+    LanczosWork L;
+    auto cleanup = [&]() {
+        hipFree(L.V);
+        hipFree(L.W);
+        hipFree(L.H);
+        hipFree(L.small);
+    };
+    size_t ncols = (size_t)Kmax + b;
+    if (hipMalloc(&L.V, (size_t)N * ncols * 8) != hipSuccess || hipMalloc(&L.W, (size_t)N * b * 8) != hipSuccess ||
+        hipMalloc(&L.H, ncols * b * 8) != hipSuccess || hipMalloc(&L.small, (size_t)b * b * 8 + 8) != hipSuccess) {
+        cleanup();
+        return set_error(ctx, OPSEC_ENOMEM, "Lanczos workspace (%d x %zu) does not fit", N, ncols);
+    }
+    std::mt19937_64 rng(42);
+    std::normal_distribution<double> nd;
+    // starting block
+    {
+        std::vector<double> r((size_t)N * b);
+        for (auto& x : r) x = nd(rng);
+        OPSEC_HIP_TRY(ctx, hipMemcpy(L.V, r.data(), r.size() * 8, hipMemcpyHostToDevice));
+        std::vector<double> Bm;
+        int rc = orthonormalise_block(D, L, N, b, 0, L.V, Bm, rng);
+        if (rc) {
+            cleanup();
+            return rc;
+        }
+    }
+    std::vector<double> T((size_t)Kmax * Kmax, 0.0);   // column-major block tridiagonal
+    int K = 0, j = 0;
+    int check_at = std::min(Kmax, std::max(((nev * 3 / 2 + b - 1) / b) * b, 2 * b));
+    std::vector<double> ritz;
+    double* dA = nullptr;
+    double* dZ = nullptr;
+    int converged = 0, Kr = 0;
+    double* Xn = nullptr;
+    int rc = OPSEC_OK;
+    while (true) {
+        double* Vj = L.V + (size_t)j * b * N;
+        K = (j + 1) * b;
+        Xn = L.V + (size_t)K * N;   // next block slot (ncols = Kmax + b)
+        // Xn = S Vj
+        D.apply(S, N, Vj, b, Xn);
+        // A_j = Vj^T Xn
+        D.gemm_tn(Vj, N, Xn, N, N, b, b, L.small);
+        if ((rc = D.sync())) break;
+        std::vector<double> Aj((size_t)b * b);
+        OPSEC_HIP_TRY(ctx, hipMemcpy(Aj.data(), L.small, (size_t)b * b * 8, hipMemcpyDeviceToHost));
+        for (int c = 0; c < b; c++)
+            for (int r = 0; r < b; r++)
+                T[(size_t)(j * b + c) * Kmax + j * b + r] = 0.5 * (Aj[(size_t)c * b + r] + Aj[(size_t)r * b + c]);
+        if (K >= Kmax) break;
+        std::vector<double> Bm;
+        if ((rc = orthonormalise_block(D, L, N, b, K, Xn, Bm, rng))) break;
+        // T block (j+1, j) = Bm (Xn_old = Xn_new Bm): T[(j+1)b + r][jb + c] = Bm[r][c]
+        for (int r = 0; r < b; r++)
+            for (int c = 0; c < b; c++) {
+                double v = Bm[(size_t)r * b + c];
+                T[(size_t)(j * b + c) * Kmax + (j + 1) * b + r] = v;
+                T[(size_t)((j + 1) * b + r) * Kmax + j * b + c] = v;
+            }
+        j++;
+        if ((j + 1) * b >= check_at || (j + 2) * b > Kmax) {
+            // Rayleigh-Ritz on the completed leading blocks 0..j-1 (K = j b);
+            // Bm couples block j-1 to the next block and gives the residuals
+            K = j * b;
+            double* dT = nullptr;
+            if (hipMalloc(&dT, (size_t)K * K * 8) != hipSuccess) {
+                rc = set_error(ctx, OPSEC_ENOMEM, "Ritz workspace");
+                break;
+            }
+            std::vector<double> Tk((size_t)K * K);
+            for (int c = 0; c < K; c++)
+                for (int r = 0; r < K; r++) Tk[(size_t)c * K + r] = T[(size_t)c * Kmax + r];
+            OPSEC_HIP_TRY(ctx, hipMemcpy(dT, Tk.data(), Tk.size() * 8, hipMemcpyHostToDevice));
+            hipFree(dZ);
+            dZ = nullptr;
+            int want = std::min(nev, K);
+            if (hipMalloc(&dZ, (size_t)K * want * 8) != hipSuccess) {
+                hipFree(dT);
+                rc = set_error(ctx, OPSEC_ENOMEM, "Ritz vectors");
+                break;
+            }
+            rc = dense_eigen_top(D, dT, K, want, ritz, dZ);
+            hipFree(dT);
+            if (rc) break;
+            Kr = K;
+            // residuals |Bm z_last| with z_last = rows (j-1)b..jb-1 of Z
+            std::vector<double> Zl((size_t)b * want);
+            for (int c = 0; c < want; c++)
+                OPSEC_HIP_TRY(ctx, hipMemcpy(&Zl[(size_t)c * b], dZ + (size_t)c * K + (K - b), (size_t)b * 8,
+                                             hipMemcpyDeviceToHost));
+            double tmax = 0;
+            for (double t : ritz) tmax = std::max(tmax, std::fabs(t));
+            converged = 0;
+            for (int c = 0; c < want; c++) {
+                double s = 0;
+                for (int r = 0; r < b; r++) {
+                    double acc = 0;
+                    for (int q = 0; q < b; q++) acc += Bm[(size_t)r * b + q] * Zl[(size_t)c * b + q];
+                    s += acc * acc;
+                }
+                if (std::sqrt(s) <= tol * tmax) converged++;
+                else break;   // count the leading run of converged pairs
+            }
+            if (converged >= nev || (j + 2) * b > Kmax) break;
+            check_at = std::min(Kmax, ((K * 5 / 4 + b - 1) / b) * b + b);
+        }
+    }
+    if (!rc) {
+        if (!dZ) {
+            rc = set_error(ctx, OPSEC_ECONV, "Lanczos stopped before a Rayleigh-Ritz step");
+        } else {
+            int want = std::min(nev, Kr);
+            // eigenvectors Y = V[:, 0:Kr] Z  (N x want, column-major = mode-major)
+            double* dY = nullptr;
+            if (hipMalloc(&dY, (size_t)N * want * 8) != hipSuccess) {
+                rc = set_error(ctx, OPSEC_ENOMEM, "Ritz vector output");
+            } else {
+                D.gemm_nn(0, dY, N, L.V, N, dZ, Kr, N, Kr, want);
+                rc = D.sync();
+                if (!rc) {
+                    hipError_t e = hipMemcpy(evecs, dY, (size_t)N * want * 8, hipMemcpyDeviceToHost);
+                    if (e != hipSuccess) rc = set_error(ctx, OPSEC_EHIP, "copy eigenvectors: %s", hipGetErrorString(e));
+                }
+                hipFree(dY);
+            }
+            for (int c = 0; c < want; c++) evals[c] = ritz[c];
+            for (int c = want; c < nev; c++) evals[c] = 0.0;
+            *nconv = std::min(converged, nev);
+        }
+    }
+    hipFree(dZ);
+    cleanup();
+    return rc;
+}
+
+}  // namespace opsec
+
+using namespace opsec;
+
+extern "C" {
+
+int opsec_lanczos_dense(opsec_ctx* ctx, int n, const double* A, int nev, double tol, int block, int maxdim,
+                        double* evals, double* evecs, int* nconv) {
+    if (!ctx) return OPSEC_EINVAL;
+    OPSEC_CHECK_ARG(ctx, n >= 1 && A && evals && evecs && nconv, "bad arguments");
+    OPSEC_CHECK_ARG(ctx, nev >= 1 && nev <= n, "requesting %d modes from a %d-dimensional space", nev, n);
+    OPSEC_CHECK_ARG(ctx, tol > 0, "tolerance must be positive");
+    OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    double* dS = nullptr;
+    OPSEC_HIP_TRY(ctx, hipMalloc(&dS, (size_t)n * n * 8));
+    OPSEC_HIP_TRY(ctx, hipMemcpy(dS, A, (size_t)n * n * 8, hipMemcpyHostToDevice));
+    int rc = block_lanczos(ctx, dS, n, nev, tol, block, maxdim, evals, evecs, nconv);
+    hipFree(dS);
+    return rc;
+}
+
+int opsec_klt(opsec_ctx* ctx, int nev, double tol, int block, int maxdim, double* evals, double* evecs, int* nconv) {
+    if (!ctx) return OPSEC_EINVAL;
+    if (ctx->ncells == 0) return set_error(ctx, OPSEC_ESTATE, "no survey uploaded");
+    if (ctx->Q_valid.empty() || !ctx->Q_valid[0]) return set_error(ctx, OPSEC_ESTATE, "prior pair table not built");
+    const int N = ctx->ncells;
+    OPSEC_CHECK_ARG(ctx, nev >= 1 && nev <= N, "klt: requesting %d modes from a %d-dimensional space", nev, N);
+    OPSEC_CHECK_ARG(ctx, evals && evecs && nconv, "NULL outputs");
+    OPSEC_CHECK_ARG(ctx, tol > 0, "tolerance must be positive");
+    OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    double* dS = nullptr;
+    OPSEC_HIP_TRY(ctx, hipMalloc(&dS, (size_t)N * N * 8));
+    int rc = opsec_assemble_device(ctx, 0, dS, N);
+    if (!rc) rc = block_lanczos(ctx, dS, N, nev, tol, block, maxdim, evals, evecs, nconv);
+    hipFree(dS);
+    return rc;
+}
+
+// Host tridiagonal eigen-solver, exposed for CPU unit tests (not in the public header).
+int opsec_internal_tridiag_top(int n, const double* d, const double* e, int nev, double* evals, double* Z) {
+    if (n < 1 || nev < 1 || nev > n) return OPSEC_EINVAL;
+    std::vector<double> dd(d, d + n), ee(e, e + std::max(0, n - 1)), ev, Zv;
+    if (ee.empty()) ee.push_back(0.0);
+    tridiag_top(n, dd, ee, nev, ev, Zv);
+    std::copy(ev.begin(), ev.end(), evals);
+    std::copy(Zv.begin(), Zv.end(), Z);
+    return OPSEC_OK;
+}
+
+}  // extern "C"

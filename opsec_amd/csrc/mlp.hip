@@ -1,0 +1,690 @@
+// K6: learned pair-integral surrogate (no counterpart in the reference).
+//
+// Network: 8 -> 128 -> 128 -> 128 -> 128 -> 1, SiLU on hidden layers.
+// Weights: FP32 master copy (Adam state in FP32) and BF16 copies used by the
+// MFMA kernels; every product runs on v_mfma_f32_16x16x32_bf16 with FP32
+// accumulation.  Weight layout (shared with the oracle's FP64 MLP): for each
+// layer, W (out x in, row-major) then b (out).
+//
+// Features of a box pair (key = |dd| per axis in cells, cell width h) and a
+// band [klo, khi]:
+//   x0..x2  sorted |dd| / 32
+//   x3      |D| h^-1 / 32           (centre separation in cells)
+//   x4, x5  klo h / pi, khi h / pi
+//   x6      log(|D|/h) / 4
+//   x7      0.5 (klo + khi) |D| / (8 pi)
+// Target: Q / s,  s = (khi^3 - klo^3) / (6 pi^2) = xi_band(0).
+// Pairs with |D| < near cell widths always use quadrature; a seeded fraction
+// of the other pairs is spot-checked against quadrature and any pair whose
+// error exceeds spot_tol * s is replaced by its quadrature value.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "common.h"
+#include "gemm.h"
+
+namespace opsec {
+
+constexpr int MLP_IN = 8, MLP_H = 128, MLP_LAYERS = 5;
+constexpr int MLP_NW = MLP_IN * MLP_H + MLP_H + 3 * (MLP_H * MLP_H + MLP_H) + MLP_H + 1;
+__host__ __device__ constexpr int mlp_w_off(int l) {
+    return l == 0 ? 0 : (l == 1 ? MLP_IN * MLP_H + MLP_H : MLP_IN * MLP_H + MLP_H + (l - 1) * (MLP_H * MLP_H + MLP_H));
+}
+__host__ __device__ constexpr int mlp_in_dim(int l) { return l == 0 ? MLP_IN : MLP_H; }
+__host__ __device__ constexpr int mlp_out_dim(int l) { return l == 4 ? 1 : MLP_H; }
+
+struct MlpState {
+    float* w = nullptr;       // FP32 master weights
+    __bf16* wb = nullptr;     // BF16 copies
+    float* g = nullptr;       // gradient
+    float* m = nullptr;       // Adam moments
+    float* v = nullptr;
+    int step = 0;
+    bool trained = false;
+    // training activations (batch-sized)
+    float* z = nullptr;       // 4 x B x 128 pre-activations
+    __bf16* a = nullptr;      // 5 x B x 128 activations (a0 = padded input)
+    float* delta = nullptr;   // B x 128
+    float* delta2 = nullptr;  // B x 128
+    float* out = nullptr;     // B
+    int cap_batch = 0;
+};
+
+__device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
+__device__ __forceinline__ float dsilu(float z) {
+    float s = 1.f / (1.f + __expf(-z));
+    return s * (1.f + z * (1.f - s));
+}
+
+// ---- fused forward over a tile of 64 samples ------------------------------------
+// act layout in LDS: [sample][k] bf16, k padded to 128 + 8.
+constexpr int FT = 64;
+constexpr int AP = MLP_H + 8;
+
+template <bool STORE>
+__global__ __launch_bounds__(256) void mlp_forward_kernel(const __bf16* __restrict__ wb, const float* __restrict__ w32,
+                                                          const float* __restrict__ x, int n, float* __restrict__ out,
+                                                          float* __restrict__ zs, __bf16* __restrict__ as, int B) {
+    __shared__ __bf16 act[2][FT][AP];
+    const int s0 = blockIdx.x * FT;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // input layer activations (features, zero-padded to 32)
+    for (int e = threadIdx.x; e < FT * 32; e += 256) {
+        int s = e / 32, k = e % 32;
+        float v = (k < MLP_IN && s0 + s < n) ? x[(size_t)(s0 + s) * MLP_IN + k] : 0.f;
+        act[0][s][k] = (__bf16)v;
+        if (STORE && s0 + s < n) as[(size_t)(s0 + s) * MLP_H + k] = (__bf16)v;   // a0 padded to 128 (first 32 used)
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int l = 0; l < 4; l++) {
+        const int kin = (l == 0) ? 32 : MLP_H;   // padded input width
+        const int din = mlp_in_dim(l);
+        const __bf16* W = wb + mlp_w_off(l);
+        const float* bias = w32 + mlp_w_off(l) + din * MLP_H;
+        // wave owns output rows [32 wave, 32 wave + 32) for all 64 samples: 2 x 4 fragments
+        f4_t acc[2][4];
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 4; j++) acc[i][j] = f4_t{0, 0, 0, 0};
+        for (int k0 = 0; k0 < kin; k0 += 32) {
+            const int kq = k0 + 8 * (lane >> 4);
+            bf8_t af[2], bfr[4];
+            for (int i = 0; i < 2; i++) {
+                const int o = wave * 32 + i * 16 + (lane & 15);
+                bf8_t t;
+                for (int e = 0; e < 8; e++) t[e] = (kq + e < din) ? W[(size_t)o * din + kq + e] : (__bf16)0.f;
+                af[i] = t;
+            }
+            for (int j = 0; j < 4; j++) bfr[j] = *(const bf8_t*)&act[cur][j * 16 + (lane & 15)][kq];
+            for (int i = 0; i < 2; i++)
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 4; j++)
+                for (int r = 0; r < 4; r++) {
+                    const int o = wave * 32 + i * 16 + 4 * (lane >> 4) + r;
+                    const int s = j * 16 + (lane & 15);
+                    float zv = acc[i][j][r] + bias[o];
+                    float av = silu(zv);
+                    act[cur ^ 1][s][o] = (__bf16)av;
+                    if (STORE && s0 + s < n) {
+                        zs[((size_t)l * B + s0 + s) * MLP_H + o] = zv;
+                        as[((size_t)(l + 1) * B + s0 + s) * MLP_H + o] = (__bf16)av;
+                    }
+                }
+        __syncthreads();
+        cur ^= 1;
+    }
+    // output layer (FP32 dot products)
+    if (threadIdx.x < FT) {
+        const int s = threadIdx.x;
+        const float* W4 = w32 + mlp_w_off(4);
+        float accv = W4[MLP_H];
+        for (int i = 0; i < MLP_H; i++) accv += W4[i] * (float)act[cur][s][i];
+        if (s0 + s < n) out[s0 + s] = accv;
+    }
+}
+
+// ---- generic BF16 MFMA GEMM for training: C (M x N, fp32) (+)= A (M x K) B (K x N)
+// Element (i, k) of A at a[i*ars + k*acs] (fp32 or bf16), same for B.
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void bgemm_kernel(const TA* a, long long ars, long long acs, const TB* b, long long brs,
+                                                    long long bcs, float* c, int ldc, int M, int N, int K, int accumulate) {
+    __shared__ __bf16 As[64][40];
+    __shared__ __bf16 Bs[64][40];   // [n][k]
+    const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+    const int kchunk = ((K + gridDim.z - 1) / gridDim.z + 31) / 32 * 32;
+    const int kz0 = blockIdx.z * kchunk, kz1 = min(K, kz0 + kchunk);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    f4_t acc[2][2];
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) acc[i][j] = f4_t{0, 0, 0, 0};
+    for (int k0 = kz0; k0 < kz1; k0 += 32) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+            int i = e / 32, k = e % 32;
+            int gi = m0 + i, gk = k0 + k;
+            As[i][k] = (gi < M && gk < kz1) ? (__bf16)(float)a[gi * ars + gk * acs] : (__bf16)0.f;
+            int gn = n0 + i;
+            Bs[i][k] = (gn < N && gk < kz1) ? (__bf16)(float)b[gk * brs + gn * bcs] : (__bf16)0.f;
+        }
+        __syncthreads();
+        const int kq = 8 * (lane >> 4);
+        bf8_t af[2], bfr[2];
+        for (int i = 0; i < 2; i++) af[i] = *(const bf8_t*)&As[wm + i * 16 + (lane & 15)][kq];
+        for (int j = 0; j < 2; j++) bfr[j] = *(const bf8_t*)&Bs[wn + j * 16 + (lane & 15)][kq];
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++)
+            for (int r = 0; r < 4; r++) {
+                int m = m0 + wm + i * 16 + 4 * (lane >> 4) + r, nn = n0 + wn + j * 16 + (lane & 15);
+                if (m < M && nn < N) {
+                    float v = acc[i][j][r];
+                    if (gridDim.z > 1 || accumulate) atomicAdd(&c[(size_t)m * ldc + nn], v);
+                    else c[(size_t)m * ldc + nn] = v;
+                }
+            }
+}
+
+// delta4 = (out - t)/B; loss partial sums
+__global__ void mlp_out_delta_kernel(const float* out, const float* t, int n, float invB, float* delta, double* loss) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double l = 0;
+    if (s < n) {
+        float e = out[s] - t[s];
+        delta[s] = e * invB;
+        l = 0.5 * (double)e * (double)e;
+    }
+    // block reduction of the loss
+    __shared__ double red[256];
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+}
+
+// last layer: dW4[i] += sum_s d4[s] a4[s][i], db4 += sum d4; d3[s][i] = W4[i] d4[s] silu'(z3[s][i])
+__global__ void mlp_last_back_kernel(const float* d4, const __bf16* a4, const float* z3, const float* w32, int n,
+                                     float* g, float* d3) {
+    const int i = threadIdx.x;   // 128 threads
+    const float* W4 = w32 + mlp_w_off(4);
+    float gw = 0, gb = 0;
+    for (int s = blockIdx.x; s < n; s += gridDim.x) {
+        float d = d4[s];
+        gw += d * (float)a4[(size_t)s * MLP_H + i];
+        if (i == 0) gb += d;
+        d3[(size_t)s * MLP_H + i] = W4[i] * d * dsilu(z3[(size_t)s * MLP_H + i]);
+    }
+    atomicAdd(&g[mlp_w_off(4) + i], gw);
+    if (i == 0) atomicAdd(&g[mlp_w_off(4) + MLP_H], gb);
+}
+
+// bias gradient: gb[o] += sum_s d[s][o]
+__global__ void mlp_bias_grad_kernel(const float* d, int n, float* gb) {
+    const int o = threadIdx.x;
+    float acc = 0;
+    for (int s = blockIdx.x; s < n; s += gridDim.x) acc += d[(size_t)s * MLP_H + o];
+    atomicAdd(&gb[o], acc);
+}
+
+// dprev[s][i] *= silu'(z[s][i])
+__global__ void mlp_dsilu_kernel(float* dprev, const float* z, size_t tot) {
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < tot) dprev[e] *= dsilu(z[e]);
+}
+
+__global__ void adam_kernel(float* w, __bf16* wb, const float* g, float* m, float* v, int n, float lr, float b1, float b2,
+                            float eps, float c1, float c2) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gi = g[i];
+    float mi = b1 * m[i] + (1 - b1) * gi;
+    float vi = b2 * v[i] + (1 - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    wb[i] = (__bf16)wi;
+}
+
+__global__ void to_bf16_kernel(const float* w, __bf16* wb, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) wb[i] = (__bf16)w[i];
+}
+
+__global__ void zero_f_kernel(float* p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+// ---- host ---------------------------------------------------------------------
+
+static int ensure_mlp(opsec_ctx* ctx) {
+    if (ctx->mlp) return OPSEC_OK;
+    MlpState* s = new MlpState;
+    ctx->mlp = s;
+    OPSEC_HIP_TRY(ctx, hipMalloc(&s->w, MLP_NW * 4));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&s->wb, MLP_NW * 2));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&s->g, MLP_NW * 4));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&s->m, MLP_NW * 4));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&s->v, MLP_NW * 4));
+    // He-style initialisation, fixed seed
+    std::vector<float> w(MLP_NW, 0.f);
+    std::mt19937 rng(7);
+    for (int l = 0; l < MLP_LAYERS; l++) {
+        int din = mlp_in_dim(l), dout = mlp_out_dim(l);
+        std::normal_distribution<float> nd(0.f, std::sqrt(2.f / din));
+        for (int i = 0; i < din * dout; i++) w[mlp_w_off(l) + i] = nd(rng) * (l == 4 ? 0.1f : 1.f);
+    }
+    OPSEC_HIP_TRY(ctx, hipMemcpy(s->w, w.data(), MLP_NW * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(to_bf16_kernel, dim3((MLP_NW + 255) / 256), dim3(256), 0, ctx->stream, s->w, s->wb, MLP_NW);
+    hipLaunchKernelGGL(zero_f_kernel, dim3((MLP_NW + 255) / 256), dim3(256), 0, ctx->stream, s->m, (size_t)MLP_NW);
+    hipLaunchKernelGGL(zero_f_kernel, dim3((MLP_NW + 255) / 256), dim3(256), 0, ctx->stream, s->v, (size_t)MLP_NW);
+    OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return OPSEC_OK;
+}
+
+static int ensure_batch(opsec_ctx* ctx, int B) {
+    MlpState* s = ctx->mlp;
+    if (s->cap_batch >= B) return OPSEC_OK;
+    hipFree(s->z);
+    hipFree(s->a);
+    hipFree(s->delta);
+    hipFree(s->delta2);
+    hipFree(s->out);
+    OPSEC_HIP_TRY(ctx, hipMalloc(&s->z, (size_t)4 * B * MLP_H * 4));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&s->a, (size_t)5 * B * MLP_H * 2));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&s->delta, (size_t)B * MLP_H * 4));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&s->delta2, (size_t)B * MLP_H * 4));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&s->out, (size_t)B * 4));
+    s->cap_batch = B;
+    return OPSEC_OK;
+}
+
+// Forward + backward on n device samples (x: n x 8, t: n); gradient in s->g
+// (zeroed first); returns the mean loss through *loss_out (device double).
+static void loss_grad_device(opsec_ctx* ctx, const float* dx, const float* dt, int n, double* dloss) {
+    MlpState* s = ctx->mlp;
+    hipStream_t st = ctx->stream;
+    const int B = n;
+    hipLaunchKernelGGL(zero_f_kernel, dim3((MLP_NW + 255) / 256), dim3(256), 0, st, s->g, (size_t)MLP_NW);
+    hipLaunchKernelGGL(mlp_forward_kernel<true>, dim3((n + FT - 1) / FT), dim3(256), 0, st, s->wb, s->w, dx, n, s->out,
+                       s->z, s->a, B);
+    hipLaunchKernelGGL(mlp_out_delta_kernel, dim3((n + 255) / 256), dim3(256), 0, st, s->out, dt, n, 1.f / n, s->out,
+                       dloss);
+    // last layer -> delta for layer 3
+    hipLaunchKernelGGL(mlp_last_back_kernel, dim3(std::min(n, 1024)), dim3(MLP_H), 0, st, s->out,
+                       s->a + (size_t)4 * B * MLP_H, s->z + (size_t)3 * B * MLP_H, s->w, n, s->g, s->delta);
+    float* d = s->delta;
+    float* dn = s->delta2;
+    for (int l = 3; l >= 0; l--) {
+        const int din = mlp_in_dim(l);
+        // dW_l (128 x din) = d^T (128 x n) . a_l (n x din); a_l row stride 128
+        const __bf16* al = s->a + (size_t)l * B * MLP_H;
+        int splits = std::max(1, std::min(64, n / 1024));
+        hipLaunchKernelGGL((bgemm_kernel<float, __bf16>), dim3(2, (din + 63) / 64, splits), dim3(256), 0, st, d, 1LL,
+                           (long long)MLP_H, al, (long long)MLP_H, 1LL, s->g + mlp_w_off(l), din, MLP_H, din, n, 1);
+        hipLaunchKernelGGL(mlp_bias_grad_kernel, dim3(std::min(n, 1024)), dim3(MLP_H), 0, st, d, n,
+                           s->g + mlp_w_off(l) + din * MLP_H);
+        if (l > 0) {
+            // dprev (n x 128) = d (n x 128) . W_l (128 x 128)
+            hipLaunchKernelGGL((bgemm_kernel<float, __bf16>), dim3((n + 63) / 64, 2, 1), dim3(256), 0, st, d,
+                               (long long)MLP_H, 1LL, s->wb + mlp_w_off(l), (long long)MLP_H, 1LL, dn, MLP_H, n, MLP_H,
+                               MLP_H, 0);
+            size_t tot = (size_t)n * MLP_H;
+            hipLaunchKernelGGL(mlp_dsilu_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, dn,
+                               s->z + (size_t)(l - 1) * B * MLP_H, tot);
+            std::swap(d, dn);
+        }
+    }
+}
+
+// Features of pairs (host, one row per (pair, band)).
+static void pair_features(double kx, double ky, double kz, double h, double klo, double khi, float* x) {
+    double a[3] = {kx, ky, kz};
+    std::sort(a, a + 3, [](double p, double q) { return p > q; });
+    double D = std::sqrt(kx * kx + ky * ky + kz * kz);
+    x[0] = (float)(a[0] / 32);
+    x[1] = (float)(a[1] / 32);
+    x[2] = (float)(a[2] / 32);
+    x[3] = (float)(D / 32);
+    x[4] = (float)(klo * h / M_PI);
+    x[5] = (float)(khi * h / M_PI);
+    x[6] = (float)(std::log(std::max(D, 1e-3)) / 4);
+    x[7] = (float)(0.5 * (klo + khi) * D * h / (8 * M_PI));
+}
+
+static double band_scale(double klo, double khi) { return (khi * khi * khi - klo * klo * klo) / (6 * M_PI * M_PI); }
+
+// minimum-image cell offset along one axis of a periodic box
+static int mi_cells(int k, int n, double L, double extent) {
+    if (L + 1e-9 * extent >= extent && L <= extent * (1 + 1e-12)) return std::min(k, n - k);
+    return k;
+}
+
+}  // namespace opsec
+
+using namespace opsec;
+
+namespace opsec {
+
+int surrogate_tables(opsec_ctx* ctx, int nf, const int* funcs, const opsec_quad* q, const std::vector<int>& keys,
+                     const std::vector<PairGeo>& geo, const std::vector<int>& nearflag_unused) {
+    (void)nearflag_unused;
+    if (!ctx->mlp || !ctx->mlp->trained) return set_error(ctx, OPSEC_ESTATE, "surrogate network is not trained");
+    const SurveyDev& sv = ctx->sv;
+    const double h = (sv.max1 - sv.min1) / sv.n1;
+    // near pairs (|D| < near cells, minimum image) and a seeded spot-check subset use quadrature
+    std::vector<int> qkeys, skeys;
+    std::vector<PairGeo> qgeo;
+    std::vector<int> qnear, far;
+    std::mt19937_64 rng(q->spot_seed);
+    std::uniform_real_distribution<double> ud(0, 1);
+    std::vector<double> Dk(keys.size());
+    for (size_t i = 0; i < keys.size(); i++) {
+        int key = keys[i];
+        int kz = key % sv.n3, ky = (key / sv.n3) % sv.n2, kx = key / (sv.n3 * sv.n2);
+        double ex = mi_cells(kx, sv.n1, sv.L, sv.max1 - sv.min1), ey = mi_cells(ky, sv.n2, sv.L, sv.max2 - sv.min2),
+               ez = mi_cells(kz, sv.n3, sv.L, sv.max3 - sv.min3);
+        double D = std::sqrt(ex * ex + ey * ey + ez * ez);
+        Dk[i] = D;
+        if (D < q->near) {
+            qkeys.push_back(key);
+            qgeo.push_back(geo[i]);
+            qnear.push_back(1);
+        } else {
+            far.push_back((int)i);
+            if (ud(rng) < q->spot_frac) {
+                qkeys.push_back(key);
+                qgeo.push_back(geo[i]);
+                qnear.push_back(0);
+                skeys.push_back(key);
+            }
+        }
+    }
+    // quadrature for near + spot-check pairs (GAUSS rule)
+    opsec_quad qq = *q;
+    qq.method = OPSEC_QUAD_GAUSS;
+    qq.near = 1;
+    std::vector<int> nearflag_q(qkeys.size());
+    for (size_t i = 0; i < qkeys.size(); i++) nearflag_q[i] = (qkeys[i] == 0);
+    int rc = run_quadrature_public(ctx, nf, funcs, qkeys, qgeo, nearflag_q, &qq, ctx->d_Q, ctx->nkeys);
+    if (rc) return rc;
+    // keep the quadrature values of the spot-check pairs
+    std::vector<double> Qq((size_t)ctx->funcs.size() * ctx->nkeys);
+    OPSEC_HIP_TRY(ctx, hipMemcpy(Qq.data(), ctx->d_Q, Qq.size() * 8, hipMemcpyDeviceToHost));
+    // surrogate inference for all far pairs of every band function
+    size_t nrows = far.size() * (size_t)nf;
+    std::vector<float> X(nrows * MLP_IN);
+    for (int t = 0; t < nf; t++) {
+        const XiDesc& d = ctx->funcs[funcs[t]];
+        for (size_t r = 0; r < far.size(); r++) {
+            int key = keys[far[r]];
+            int kz = key % sv.n3, ky = (key / sv.n3) % sv.n2, kx = key / (sv.n3 * sv.n2);
+            double ex = mi_cells(kx, sv.n1, sv.L, sv.max1 - sv.min1), ey = mi_cells(ky, sv.n2, sv.L, sv.max2 - sv.min2),
+                   ez = mi_cells(kz, sv.n3, sv.L, sv.max3 - sv.min3);
+            pair_features(ex, ey, ez, h, d.a, d.b, &X[((size_t)t * far.size() + r) * MLP_IN]);
+        }
+    }
+    float *dx = nullptr, *dout = nullptr;
+    OPSEC_HIP_TRY(ctx, hipMalloc(&dx, X.size() * 4 + 4));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&dout, nrows * 4 + 4));
+    OPSEC_HIP_TRY(ctx, hipMemcpy(dx, X.data(), X.size() * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(mlp_forward_kernel<false>, dim3((unsigned)((nrows + FT - 1) / FT)), dim3(256), 0, ctx->stream,
+                       ctx->mlp->wb, ctx->mlp->w, dx, (int)nrows, dout, (float*)nullptr, (__bf16*)nullptr, 0);
+    OPSEC_HIP_TRY(ctx, hipGetLastError());
+    std::vector<float> Y(nrows);
+    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(Y.data(), dout, nrows * 4, hipMemcpyDeviceToHost, ctx->stream));
+    OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    hipFree(dx);
+    hipFree(dout);
+    std::vector<char> is_spot(ctx->nkeys, 0);
+    for (int k : skeys) is_spot[k] = 1;
+    int64_t spot = 0, fail = 0;
+    for (int t = 0; t < nf; t++) {
+        const int f = funcs[t];
+        const XiDesc& d = ctx->funcs[f];
+        const double sc = band_scale(d.a, d.b);
+        double* Qf = &Qq[(size_t)f * ctx->nkeys];
+        for (size_t r = 0; r < far.size(); r++) {
+            int key = keys[far[r]];
+            double pred = (double)Y[(size_t)t * far.size() + r] * sc;
+            if (is_spot[key]) {
+                spot++;
+                if (std::fabs(pred - Qf[key]) > q->spot_tol * sc) fail++;   // keep quadrature value
+                continue;
+            }
+            Qf[key] = pred;
+        }
+    }
+    OPSEC_HIP_TRY(ctx, hipMemcpy(ctx->d_Q, Qq.data(), Qq.size() * 8, hipMemcpyHostToDevice));
+    ctx->st_surrogate += (int64_t)(far.size() * nf) - (int64_t)skeys.size() * nf;
+    ctx->st_spot += spot;
+    ctx->st_spot_fail += fail;
+    return OPSEC_OK;
+}
+
+}  // namespace opsec
+
+extern "C" {
+
+void opsec_internal_free_mlp(opsec_ctx* ctx) {
+    MlpState* s = ctx->mlp;
+    if (!s) return;
+    hipFree(s->w);
+    hipFree(s->wb);
+    hipFree(s->g);
+    hipFree(s->m);
+    hipFree(s->v);
+    hipFree(s->z);
+    hipFree(s->a);
+    hipFree(s->delta);
+    hipFree(s->delta2);
+    hipFree(s->out);
+    delete s;
+    ctx->mlp = nullptr;
+}
+
+int opsec_mlp_defaults(opsec_mlp_train_opts* o) {
+    if (!o) return OPSEC_EINVAL;
+    o->nsamples = 200000;
+    o->epochs = 60;
+    o->batch = 4096;
+    o->seed = 7;
+    o->lr = 2e-3;
+    o->near = 2.0;
+    return OPSEC_OK;
+}
+
+int opsec_mlp_nweights(opsec_ctx* ctx, int* n) {
+    if (!ctx || !n) return OPSEC_EINVAL;
+    *n = MLP_NW;
+    return OPSEC_OK;
+}
+
+int opsec_mlp_get_weights(opsec_ctx* ctx, float* w) {
+    if (!ctx || !w) return OPSEC_EINVAL;
+    int rc = ensure_mlp(ctx);
+    if (rc) return rc;
+    OPSEC_HIP_TRY(ctx, hipMemcpy(w, ctx->mlp->w, MLP_NW * 4, hipMemcpyDeviceToHost));
+    return OPSEC_OK;
+}
+
+int opsec_mlp_set_weights(opsec_ctx* ctx, const float* w) {
+    if (!ctx || !w) return OPSEC_EINVAL;
+    int rc = ensure_mlp(ctx);
+    if (rc) return rc;
+    OPSEC_HIP_TRY(ctx, hipMemcpy(ctx->mlp->w, w, MLP_NW * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(to_bf16_kernel, dim3((MLP_NW + 255) / 256), dim3(256), 0, ctx->stream, ctx->mlp->w, ctx->mlp->wb,
+                       MLP_NW);
+    OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->mlp->trained = true;
+    return OPSEC_OK;
+}
+
+int opsec_mlp_forward(opsec_ctx* ctx, int n, const float* x, float* out) {
+    if (!ctx || n < 0 || (n > 0 && (!x || !out))) return OPSEC_EINVAL;
+    int rc = ensure_mlp(ctx);
+    if (rc || n == 0) return rc;
+    float *dx = nullptr, *dout = nullptr;
+    OPSEC_HIP_TRY(ctx, hipMalloc(&dx, (size_t)n * MLP_IN * 4));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&dout, (size_t)n * 4));
+    OPSEC_HIP_TRY(ctx, hipMemcpy(dx, x, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(mlp_forward_kernel<false>, dim3((n + FT - 1) / FT), dim3(256), 0, ctx->stream, ctx->mlp->wb,
+                       ctx->mlp->w, dx, n, dout, (float*)nullptr, (__bf16*)nullptr, 0);
+    OPSEC_HIP_TRY(ctx, hipGetLastError());
+    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(out, dout, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    hipFree(dx);
+    hipFree(dout);
+    return OPSEC_OK;
+}
+
+int opsec_mlp_loss_grad(opsec_ctx* ctx, int n, const float* x, const float* t, double* loss, float* grad) {
+    if (!ctx || n <= 0 || !x || !t || !loss || !grad) return OPSEC_EINVAL;
+    int rc = ensure_mlp(ctx);
+    if (rc) return rc;
+    if ((rc = ensure_batch(ctx, n))) return rc;
+    float *dx = nullptr, *dt = nullptr;
+    double* dl = nullptr;
+    OPSEC_HIP_TRY(ctx, hipMalloc(&dx, (size_t)n * MLP_IN * 4));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&dt, (size_t)n * 4));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&dl, 8));
+    OPSEC_HIP_TRY(ctx, hipMemcpy(dx, x, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice));
+    OPSEC_HIP_TRY(ctx, hipMemcpy(dt, t, (size_t)n * 4, hipMemcpyHostToDevice));
+    OPSEC_HIP_TRY(ctx, hipMemset(dl, 0, 8));
+    loss_grad_device(ctx, dx, dt, n, dl);
+    OPSEC_HIP_TRY(ctx, hipGetLastError());
+    OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    double l = 0;
+    OPSEC_HIP_TRY(ctx, hipMemcpy(&l, dl, 8, hipMemcpyDeviceToHost));
+    OPSEC_HIP_TRY(ctx, hipMemcpy(grad, ctx->mlp->g, MLP_NW * 4, hipMemcpyDeviceToHost));
+    *loss = l / n;
+    hipFree(dx);
+    hipFree(dt);
+    hipFree(dl);
+    return OPSEC_OK;
+}
+
+// Train on exact integrals of seeded random far pairs of the device survey.
+int opsec_mlp_train(opsec_ctx* ctx, const opsec_mlp_train_opts* o, const opsec_quad* q, double* final_loss) {
+    if (!ctx || !o || !q) return OPSEC_EINVAL;
+    if (ctx->ncells == 0) return set_error(ctx, OPSEC_ESTATE, "no survey uploaded");
+    if (ctx->model_kind != 0) return set_error(ctx, OPSEC_ESTATE, "the surrogate needs a RealModel");
+    OPSEC_CHECK_ARG(ctx, o->nsamples >= 256 && o->batch >= 64 && o->epochs >= 1 && o->lr > 0, "bad training options");
+    int rc = ensure_mlp(ctx);
+    if (rc) return rc;
+    const SurveyDev& sv = ctx->sv;
+    const double h = (sv.max1 - sv.min1) / sv.n1;
+    // seeded training pairs: uniform in separation over [near, Dmax] and in band
+    const int nb = ctx->nparams;
+    std::vector<std::pair<double, int>> far;   // (minimum-image separation in cells, key)
+    for (int key = 0; key < ctx->nkeys; key++) {
+        int kz = key % sv.n3, ky = (key / sv.n3) % sv.n2, kx = key / (sv.n3 * sv.n2);
+        double ex = mi_cells(kx, sv.n1, sv.L, sv.max1 - sv.min1), ey = mi_cells(ky, sv.n2, sv.L, sv.max2 - sv.min2),
+               ez = mi_cells(kz, sv.n3, sv.L, sv.max3 - sv.min3);
+        double D = std::sqrt(ex * ex + ey * ey + ez * ez);
+        if (D >= o->near) far.push_back({D, key});
+    }
+    if (far.empty()) return set_error(ctx, OPSEC_EINVAL, "no pairs beyond %g cell widths to train on", o->near);
+    std::sort(far.begin(), far.end());
+    std::mt19937_64 rng((uint64_t)o->seed);
+    std::uniform_real_distribution<double> uD(far.front().first, far.back().first);
+    std::uniform_int_distribution<int> bb(1, nb);
+    std::vector<std::vector<int>> keys_of(nb + 1);
+    std::vector<int> tkeys, tf;
+    std::vector<float> X;
+    for (int i = 0; i < o->nsamples; i++) {
+        double D = uD(rng);
+        auto it = std::lower_bound(far.begin(), far.end(), std::make_pair(D, -1));
+        if (it == far.end()) it = far.end() - 1;
+        // random key among those at this separation
+        auto lo = std::lower_bound(far.begin(), far.end(), std::make_pair(it->first, -1));
+        auto hi = std::upper_bound(far.begin(), far.end(), std::make_pair(it->first, 1 << 30));
+        std::uniform_int_distribution<long> pick(0, (long)(hi - lo) - 1);
+        int key = (lo + pick(rng))->second;
+        int f = bb(rng);
+        int kz = key % sv.n3, ky = (key / sv.n3) % sv.n2, kx = key / (sv.n3 * sv.n2);
+        double ex = mi_cells(kx, sv.n1, sv.L, sv.max1 - sv.min1), ey = mi_cells(ky, sv.n2, sv.L, sv.max2 - sv.min2),
+               ez = mi_cells(kz, sv.n3, sv.L, sv.max3 - sv.min3);
+        float x[MLP_IN];
+        pair_features(ex, ey, ez, h, ctx->funcs[f].a, ctx->funcs[f].b, x);
+        X.insert(X.end(), x, x + MLP_IN);
+        tkeys.push_back(key);
+        tf.push_back(f);
+        keys_of[f].push_back(key);
+    }
+    const int ns = (int)tkeys.size();
+    // exact integrals of exactly the sampled (key, band) pairs
+    opsec_quad qq = *q;
+    if (qq.method == OPSEC_QUAD_SURROGATE) qq.method = OPSEC_QUAD_GAUSS;
+    double* dQs = nullptr;
+    OPSEC_HIP_TRY(ctx, hipMalloc(&dQs, (size_t)ctx->funcs.size() * ctx->nkeys * 8));
+    int64_t integrals = 0;
+    for (int f = 1; f <= nb; f++) {
+        std::vector<int>& ks = keys_of[f];
+        std::sort(ks.begin(), ks.end());
+        ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
+        if (ks.empty()) continue;
+        std::vector<PairGeo> geo;
+        std::vector<int> nearflag(ks.size(), 0);
+        for (int key : ks) {
+            int kz = key % sv.n3, ky = (key / sv.n3) % sv.n2, kx = key / (sv.n3 * sv.n2);
+            geo.push_back(grid_pair_geo(sv, kx, ky, kz));
+        }
+        if ((rc = run_quadrature_public(ctx, 1, &f, ks, geo, nearflag, &qq, dQs, ctx->nkeys))) {
+            hipFree(dQs);
+            return rc;
+        }
+        integrals += (int64_t)ks.size();
+    }
+    std::vector<double> Qall((size_t)ctx->funcs.size() * ctx->nkeys);
+    OPSEC_HIP_TRY(ctx, hipMemcpy(Qall.data(), dQs, Qall.size() * 8, hipMemcpyDeviceToHost));
+    hipFree(dQs);
+    std::vector<float> T(ns);
+    for (int i = 0; i < ns; i++) {
+        const XiDesc& d = ctx->funcs[tf[i]];
+        T[i] = (float)(Qall[(size_t)tf[i] * ctx->nkeys + tkeys[i]] / band_scale(d.a, d.b));
+    }
+    ctx->st_integrals = integrals;
+    // device copies
+    float *dX = nullptr, *dT = nullptr;
+    double* dl = nullptr;
+    OPSEC_HIP_TRY(ctx, hipMalloc(&dX, (size_t)ns * MLP_IN * 4));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&dT, (size_t)ns * 4));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&dl, 8));
+    const int B = std::min(o->batch, ns);
+    if ((rc = ensure_batch(ctx, B))) return rc;
+    MlpState* s = ctx->mlp;
+    const int nbatch = ns / B;
+    std::vector<int> perm(ns);
+    for (int i = 0; i < ns; i++) perm[i] = i;
+    std::vector<float> Xb((size_t)ns * MLP_IN), Tb(ns);
+    double last = 0;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const int total_steps = o->epochs * nbatch;
+    for (int ep = 0; ep < o->epochs; ep++) {
+        std::shuffle(perm.begin(), perm.end(), rng);
+        for (int i = 0; i < ns; i++) {
+            std::memcpy(&Xb[(size_t)i * MLP_IN], &X[(size_t)perm[i] * MLP_IN], MLP_IN * 4);
+            Tb[i] = T[perm[i]];
+        }
+        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dX, Xb.data(), Xb.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dT, Tb.data(), Tb.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+        OPSEC_HIP_TRY(ctx, hipMemsetAsync(dl, 0, 8, ctx->stream));
+        for (int bi = 0; bi < nbatch; bi++) {
+            loss_grad_device(ctx, dX + (size_t)bi * B * MLP_IN, dT + (size_t)bi * B, B, dl);
+            s->step++;
+            // cosine learning-rate decay
+            double frac = (double)(ep * nbatch + bi) / std::max(1, total_steps);
+            float lr = (float)(o->lr * 0.5 * (1 + std::cos(M_PI * frac)));
+            float c1 = 1.f - std::pow(b1, (float)s->step), c2 = 1.f - std::pow(b2, (float)s->step);
+            hipLaunchKernelGGL(adam_kernel, dim3((MLP_NW + 255) / 256), dim3(256), 0, ctx->stream, s->w, s->wb, s->g,
+                               s->m, s->v, MLP_NW, lr, b1, b2, eps, c1, c2);
+        }
+        OPSEC_HIP_TRY(ctx, hipGetLastError());
+        double l = 0;
+        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&l, dl, 8, hipMemcpyDeviceToHost, ctx->stream));
+        OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        last = l / ((double)nbatch * B);
+    }
+    hipFree(dX);
+    hipFree(dT);
+    hipFree(dl);
+    s->trained = true;
+    if (final_loss) *final_loss = last;
+    return OPSEC_OK;
+}
+
+}  // extern "C"

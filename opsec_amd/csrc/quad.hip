@@ -517,9 +517,12 @@ static void grid_cell(const SurveyDev& sv, int d1, int d2, int d3, double* lo, d
     hi[2] = sv.min3 + (d3 + 1) * (sv.max3 - sv.min3) / sv.n3;
 }
 
-struct PairGeo {
-    double lo1[3], hi1[3], lo2[3], hi2[3];
-};
+PairGeo grid_pair_geo(const SurveyDev& sv, int kx, int ky, int kz) {
+    PairGeo g;
+    grid_cell(sv, 0, 0, 0, g.lo1, g.hi1);
+    grid_cell(sv, kx, ky, kz, g.lo2, g.hi2);
+    return g;
+}
 
 // Run method q->method on the given key pairs for functions funcs; results to
 // dQ[f * nkeys + key].  keys[i] with geometry geo[i].
@@ -659,7 +662,6 @@ static int check_quad(opsec_ctx* ctx, const opsec_quad* q) {
     return OPSEC_OK;
 }
 
-int survey_cells_upload(opsec_ctx* ctx, const opsec_cell* cells, int ncells, const opsec_survey* sv);
 int surrogate_tables(opsec_ctx* ctx, int nf, const int* funcs, const opsec_quad* q, const std::vector<int>& keys,
                      const std::vector<PairGeo>& geo, const std::vector<int>& nearflag);
 

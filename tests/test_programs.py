@@ -1,0 +1,135 @@
+"""The klt / comma / estimate drop-ins: command line, config and file-format
+behaviour (CPU), and the end-to-end pipeline against the oracle (GPU)."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from opsec_amd import formats, lib, synth
+from tests import oracles as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "opsec_amd", "bin")
+
+
+def run(prog, *args, cwd=None):
+    return subprocess.run([os.path.join(BIN, prog), *args], capture_output=True, text=True, cwd=cwd, timeout=600)
+
+
+@pytest.mark.parametrize("prog", ["klt", "comma", "estimate"])
+def test_help_exits_zero(prog):
+    r = run(prog, "-h")
+    assert r.returncode == 0 and "Usage:" in r.stdout
+
+
+@pytest.mark.parametrize("prog,keys", [("klt", "cellfile"), ("comma", "covfile"), ("estimate", "pixfile")])
+def test_missing_keys_exit_one(prog, keys):
+    r = run(prog, "Nmodes=3")
+    assert r.returncode == 1
+    assert "missing configuration options" in r.stderr
+
+
+def test_unreadable_cellfile_aborts(tmp_path):
+    r = run("klt", f"cellfile={tmp_path}/nope.abn", "Nmodes=1", f"modefile={tmp_path}/m", f"evalfile={tmp_path}/e")
+    assert r.returncode < 0 or r.returncode == 134
+    assert "ReadCells: could not open" in r.stderr
+
+
+def test_klt_rejects_too_many_modes(tmp_path):
+    cells = synth.box_cells(3)
+    cf = tmp_path / "cells.abn"
+    formats.write_cells(cf, cells, {"coordsys": "cartesian"})
+    r = run("klt", f"cellfile={cf}", "Nmodes=30", f"modefile={tmp_path}/m", f"evalfile={tmp_path}/e")
+    assert r.returncode == 1
+    assert "requesting 30 modes from a 27-dimensional space" in r.stderr
+
+
+def test_cellfile_roundtrip_with_reference_reader(tmp_path):
+    if not O.ref_available():
+        pytest.skip("reference build not present")
+    cells = synth.box_cells(4, 0.0, 100.0)
+    cf = str(tmp_path / "cells.abn")
+    formats.write_cells(cf, cells, {"coordsys": "cartesian", "Nx": 4})
+    out = np.zeros(len(cells), dtype=lib.CELL_DTYPE)
+    n = O.ref().ref_read_cells(cf.encode(), out.ctypes.data, len(out))
+    assert n == 64 and out.tobytes() == cells.tobytes()
+
+
+def _write_inputs(tmp_path, n=6, nb=3, lo=0.0, hi=150.0):
+    cells = synth.box_cells(n, lo, hi)
+    cf = tmp_path / "cells.abn"
+    formats.write_cells(cf, cells, {"coordsys": "cartesian"})
+    k, P = synth.pk_table()
+    pk = tmp_path / "pk.dat"
+    formats.write_pkfile(pk, k, P)
+    cfg = {
+        "cellfile": cf, "model": "RealModel", "RealModel.pkfile": pk, "RealModel.Nbands": nb,
+        "RealModel.kmin": 0.02, "RealModel.kmax": 0.14, "survey.type": "BoxSurvey",
+        "survey.XMin": lo, "survey.XMax": hi, "survey.YMin": lo, "survey.YMax": hi, "survey.ZMin": lo,
+        "survey.ZMax": hi, "survey.Nx": n, "survey.Ny": n, "survey.Nz": n, "survey.nbar": 3e-4,
+        "survey.L": hi - lo, "survey.galfile": "none", "solver": "parpack",
+    }
+    cfile = tmp_path / "opsec.cfg"
+    formats.write_config(cfile, cfg)
+    return cells, cfile, np.loadtxt(pk)
+
+
+@pytest.mark.gpu
+def test_pipeline_end_to_end_vs_oracle(tmp_path):
+    n, nb, M = 6, 3, 40
+    cells, cfile, kP = _write_inputs(tmp_path, n, nb)
+    N = len(cells)
+    r = run("klt", "-c", str(cfile), f"Nmodes={M}", f"modefile={tmp_path}/modes.abn", f"evalfile={tmp_path}/evals.abn",
+            "hip.tol=1e-10")
+    assert r.returncode == 0, r.stderr
+    eopts, (lam,) = formats.read_doubles(tmp_path / "evals.abn")
+    assert eopts["Nmodes"] == str(M) and eopts["Ncells"] == str(N) and int(eopts["nconv"]) == M
+    _, (modes,) = formats.read_doubles(tmp_path / "modes.abn")
+    B = modes.reshape(M, N)
+    # oracle S and its spectrum
+    lo, hi = synth.linear_bands(nb, 0.02, 0.14)
+    r768, tabs = O.oracle_tables(kP[:, 0], kP[:, 1], lo, hi)
+    rows = np.arange(N, dtype=np.int32)
+    mats = []
+    for f in range(nb + 1):
+        xi = O.XiHandle(0, r768, tabs[f])
+        S = np.zeros((N, N))
+        O.oracle().orc_sig_block(C.byref(xi.s), cells.ctypes.data, N, n, n, n, 150.0, O.ptr(rows), N, O.ptr(rows), N,
+                                 O.ptr(S), N, 0, 1e-5, 1e-10, 50000000, 0, 0)
+        mats.append(S)
+    w = np.linalg.eigvalsh(mats[0])[::-1][:M]
+    np.testing.assert_allclose(lam, w, atol=1e-9 * w[0])
+    np.testing.assert_allclose(B @ mats[0] @ B.T, np.diag(lam), atol=1e-9 * w[0])
+
+    r = run("comma", "-c", str(cfile), f"Nmodes={M}", f"modefile={tmp_path}/modes.abn", f"covfile={tmp_path}/cov.abn")
+    assert r.returncode == 0, r.stderr
+    copts, (D,) = formats.read_doubles(tmp_path / "cov.abn")
+    assert copts == {"Nmodes": str(M), "Nparams": str(nb), "packed": "1"}
+    K = M * (M + 1) // 2
+    D = D.reshape(nb, K)
+    for p in range(nb):
+        ref = np.zeros(K)
+        O.oracle().orc_project(N, O.ptr(mats[p + 1]), M, O.ptr(np.ascontiguousarray(B)), O.ptr(ref))
+        np.testing.assert_allclose(D[p], ref, rtol=1e-10, atol=1e-11 * np.abs(ref).max())
+
+    y = np.sqrt(1 + lam) * np.random.default_rng(42).normal(size=M)
+    formats.write(tmp_path / "pix.abn", y, "d", comments=("Pixel values",))
+    r = run("estimate", "-c", str(cfile), f"Nmodes={M}", f"evalfile={tmp_path}/evals.abn",
+            f"covfile={tmp_path}/cov.abn", f"pixfile={tmp_path}/pix.abn", f"estfile={tmp_path}/est.abn")
+    assert r.returncode == 0, r.stderr
+    assert "p_n          error" in r.stdout
+    eo, blocks = formats.read_doubles(tmp_path / "est.abn")
+    assert eo == {"Nparams": str(nb)} and len(blocks) == 7
+    phat, cov, F, W, Mm, q, f = blocks
+    prior = np.array([O.oracle().orc_band_param(len(kP), O.ptr(np.ascontiguousarray(kP[:, 0])),
+                                                O.ptr(np.ascontiguousarray(kP[:, 1])), a, b) for a, b in zip(lo, hi)])
+    out = {k2: np.zeros(nb) for k2 in ("phat", "q", "f")}
+    out.update({k2: np.zeros((nb, nb)) for k2 in ("cov", "F", "W", "M")})
+    lam_c, D_c, y_c = np.ascontiguousarray(lam), np.ascontiguousarray(D), np.ascontiguousarray(y)
+    O.oracle().orc_estimate(M, nb, O.ptr(lam_c), O.ptr(D_c), O.ptr(y_c), O.ptr(prior), 0,
+                            *[O.ptr(out[k2]) for k2 in ("phat", "cov", "F", "W", "M", "q", "f")])
+    for name, got in (("F", F), ("q", q), ("f", f), ("phat", phat)):
+        ref = out[name].ravel()
+        np.testing.assert_allclose(got, ref, rtol=1e-9, atol=1e-10 * np.abs(ref).max(), err_msg=name)

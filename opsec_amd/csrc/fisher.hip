@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void fisher_gram_kernel(const double* D, long 
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++)
             for (int r = 0; r < 4; r++) {
-                int row = i * 16 + 4 * (lane >> 4) + r, col = j * 16 + (lane & 15);
+                int row = i * 16 + acc_row<T>(lane, r), col = j * 16 + (lane & 15);
                 atomicAdd(&G[row * FG + col], (double)acc[i][j][r]);
             }
 }

@@ -11,11 +11,12 @@
 // fragment row read consecutive words; the next tile's global loads are issued
 // before the current tile's MFMAs (register double buffering).
 //
-// Fragment layouts (verified by the MFMA layout probe on gfx950):
-//   f64 16x16x4 / f32 16x16x4:  A lane l -> A[l%16][l/16],  B lane l -> B[l/16][l%16],
-//                               D lane l, item r -> D[4(l/16)+r][l%16]
-//   bf16 16x16x32:              A lane l -> A[l%16][8(l/16)+e], B lane l -> B[8(l/16)+e][l%16],
-//                               D as above.
+// Fragment layouts (measured with an in-bounds probe on gfx950):
+//   f64 16x16x4:    A lane l -> A[l%16][l/16],  B lane l -> B[l/16][l%16],
+//                   D lane l, item r -> D[l/16 + 4r][l%16]
+//   f32 16x16x4:    A, B as f64;                D lane l, item r -> D[4(l/16)+r][l%16]
+//   bf16 16x16x32:  A lane l -> A[l%16][8(l/16)+e], B lane l -> B[8(l/16)+e][l%16],
+//                   D as f32.
 #pragma once
 
 #include "common.h"
@@ -27,6 +28,16 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 
 constexpr int GT_M = 128, GT_N = 128, GT_THREADS = 256;
+
+// Row (within a 16 x 16 output fragment) of accumulator item r of `lane`.
+template <typename T>
+__device__ __forceinline__ int acc_row(int lane, int r) {
+    return 4 * (lane >> 4) + r;
+}
+template <>
+__device__ __forceinline__ int acc_row<double>(int lane, int r) {
+    return (lane >> 4) + 4 * r;
+}
 
 // ---- operand sources -------------------------------------------------------
 // Each source exposes  T get(m_or_n, k)  with bounds handled by the caller,

@@ -105,7 +105,7 @@ __global__ __launch_bounds__(GT_THREADS) void lgemm_kernel(SrcA A, SrcB B, Out O
         for (int j = 0; j < 4; j++) {
             const int n = n0 + wn + j * 16 + (lane & 15);
             for (int r = 0; r < 4; r++) {
-                const int m = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+                const int m = m0 + wm + i * 16 + acc_row<double>(lane, r);
                 if (gridDim.z == 1) O.put(0, m, n, M, N, acc[i][j][r]);
                 else if (m < M && n < N) atomicAdd(&O.p[(size_t)n * O.ld + m], acc[i][j][r]);
             }
@@ -644,6 +644,11 @@ static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double
     if (b <= 0) b = std::min(128, std::max(1, N / 4));
     b = std::max(1, std::min(b, N));
     if (maxdim <= 0) maxdim = std::max(3 * nev + 4 * b, 8 * b);
+    if (maxdim >= N) {
+        // the whole space fits: use a block size that divides N
+        while (N % b) b--;
+        maxdim = N;
+    }
     int Kmax = std::min(N, ((maxdim + b - 1) / b) * b);
     if (Kmax < nev) Kmax = std::min(N, ((nev + b - 1) / b) * b);
     Kmax = std::max(b, (Kmax / b) * b);
@@ -675,85 +680,77 @@ static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double
         }
     }
     std::vector<double> T((size_t)Kmax * Kmax, 0.0);   // column-major block tridiagonal
-    int K = 0, j = 0;
     int check_at = std::min(Kmax, std::max(((nev * 3 / 2 + b - 1) / b) * b, 2 * b));
     std::vector<double> ritz;
-    double* dA = nullptr;
     double* dZ = nullptr;
     int converged = 0, Kr = 0;
-    double* Xn = nullptr;
     int rc = OPSEC_OK;
-    while (true) {
+    for (int j = 0;; j++) {
         double* Vj = L.V + (size_t)j * b * N;
-        K = (j + 1) * b;
-        Xn = L.V + (size_t)K * N;   // next block slot (ncols = Kmax + b)
-        // Xn = S Vj
-        D.apply(S, N, Vj, b, Xn);
-        // A_j = Vj^T Xn
-        D.gemm_tn(Vj, N, Xn, N, N, b, b, L.small);
+        const int K = (j + 1) * b;              // basis size including block j
+        double* Xn = L.V + (size_t)K * N;       // next block slot (ncols = Kmax + b)
+        D.apply(S, N, Vj, b, Xn);               // Xn = S V_j
+        D.gemm_tn(Vj, N, Xn, N, N, b, b, L.small);   // A_j = V_j^T S V_j
         if ((rc = D.sync())) break;
         std::vector<double> Aj((size_t)b * b);
         OPSEC_HIP_TRY(ctx, hipMemcpy(Aj.data(), L.small, (size_t)b * b * 8, hipMemcpyDeviceToHost));
         for (int c = 0; c < b; c++)
             for (int r = 0; r < b; r++)
                 T[(size_t)(j * b + c) * Kmax + j * b + r] = 0.5 * (Aj[(size_t)c * b + r] + Aj[(size_t)r * b + c]);
-        if (K >= Kmax) break;
-        std::vector<double> Bm;
-        if ((rc = orthonormalise_block(D, L, N, b, K, Xn, Bm, rng))) break;
-        // T block (j+1, j) = Bm (Xn_old = Xn_new Bm): T[(j+1)b + r][jb + c] = Bm[r][c]
-        for (int r = 0; r < b; r++)
-            for (int c = 0; c < b; c++) {
-                double v = Bm[(size_t)r * b + c];
-                T[(size_t)(j * b + c) * Kmax + (j + 1) * b + r] = v;
-                T[(size_t)((j + 1) * b + r) * Kmax + j * b + c] = v;
-            }
-        j++;
-        if ((j + 1) * b >= check_at || (j + 2) * b > Kmax) {
-            // Rayleigh-Ritz on the completed leading blocks 0..j-1 (K = j b);
-            // Bm couples block j-1 to the next block and gives the residuals
-            K = j * b;
-            double* dT = nullptr;
-            if (hipMalloc(&dT, (size_t)K * K * 8) != hipSuccess) {
-                rc = set_error(ctx, OPSEC_ENOMEM, "Ritz workspace");
-                break;
-            }
-            std::vector<double> Tk((size_t)K * K);
-            for (int c = 0; c < K; c++)
-                for (int r = 0; r < K; r++) Tk[(size_t)c * K + r] = T[(size_t)c * Kmax + r];
-            OPSEC_HIP_TRY(ctx, hipMemcpy(dT, Tk.data(), Tk.size() * 8, hipMemcpyHostToDevice));
-            hipFree(dZ);
-            dZ = nullptr;
-            int want = std::min(nev, K);
-            if (hipMalloc(&dZ, (size_t)K * want * 8) != hipSuccess) {
-                hipFree(dT);
-                rc = set_error(ctx, OPSEC_ENOMEM, "Ritz vectors");
-                break;
-            }
-            rc = dense_eigen_top(D, dT, K, want, ritz, dZ);
-            hipFree(dT);
-            if (rc) break;
-            Kr = K;
-            // residuals |Bm z_last| with z_last = rows (j-1)b..jb-1 of Z
-            std::vector<double> Zl((size_t)b * want);
-            for (int c = 0; c < want; c++)
-                OPSEC_HIP_TRY(ctx, hipMemcpy(&Zl[(size_t)c * b], dZ + (size_t)c * K + (K - b), (size_t)b * 8,
-                                             hipMemcpyDeviceToHost));
-            double tmax = 0;
-            for (double t : ritz) tmax = std::max(tmax, std::fabs(t));
-            converged = 0;
-            for (int c = 0; c < want; c++) {
-                double s = 0;
-                for (int r = 0; r < b; r++) {
-                    double acc = 0;
-                    for (int q = 0; q < b; q++) acc += Bm[(size_t)r * b + q] * Zl[(size_t)c * b + q];
-                    s += acc * acc;
-                }
-                if (std::sqrt(s) <= tol * tmax) converged++;
-                else break;   // count the leading run of converged pairs
-            }
-            if (converged >= nev || (j + 2) * b > Kmax) break;
-            check_at = std::min(Kmax, ((K * 5 / 4 + b - 1) / b) * b + b);
+        const bool last = K >= Kmax;
+        std::vector<double> Bm((size_t)b * b, 0.0);   // coupling: S V_j = ... + V_{j+1} Bm
+        if (K < N) {
+            if ((rc = orthonormalise_block(D, L, N, b, K, Xn, Bm, rng))) break;
+            if (!last)
+                for (int r = 0; r < b; r++)
+                    for (int c = 0; c < b; c++) {
+                        double v = Bm[(size_t)r * b + c];
+                        T[(size_t)(j * b + c) * Kmax + (j + 1) * b + r] = v;
+                        T[(size_t)((j + 1) * b + r) * Kmax + j * b + c] = v;
+                    }
         }
+        if (!last && K < check_at) continue;
+        // Rayleigh-Ritz on T[0:K, 0:K]; residual of Ritz pair z: |Bm z[K-b:K]|
+        double* dT = nullptr;
+        if (hipMalloc(&dT, (size_t)K * K * 8) != hipSuccess) {
+            rc = set_error(ctx, OPSEC_ENOMEM, "Ritz workspace");
+            break;
+        }
+        std::vector<double> Tk((size_t)K * K);
+        for (int c = 0; c < K; c++)
+            for (int r = 0; r < K; r++) Tk[(size_t)c * K + r] = T[(size_t)c * Kmax + r];
+        OPSEC_HIP_TRY(ctx, hipMemcpy(dT, Tk.data(), Tk.size() * 8, hipMemcpyHostToDevice));
+        hipFree(dZ);
+        dZ = nullptr;
+        const int want = std::min(nev, K);
+        if (hipMalloc(&dZ, (size_t)K * want * 8) != hipSuccess) {
+            hipFree(dT);
+            rc = set_error(ctx, OPSEC_ENOMEM, "Ritz vectors");
+            break;
+        }
+        rc = dense_eigen_top(D, dT, K, want, ritz, dZ);
+        hipFree(dT);
+        if (rc) break;
+        Kr = K;
+        std::vector<double> Zl((size_t)b * want);
+        for (int c = 0; c < want; c++)
+            OPSEC_HIP_TRY(ctx, hipMemcpy(&Zl[(size_t)c * b], dZ + (size_t)c * K + (K - b), (size_t)b * 8,
+                                         hipMemcpyDeviceToHost));
+        double tmax = 0;
+        for (double t : ritz) tmax = std::max(tmax, std::fabs(t));
+        converged = 0;
+        for (int c = 0; c < want; c++) {
+            double s2 = 0;
+            for (int r = 0; r < b; r++) {
+                double acc = 0;
+                for (int q2 = 0; q2 < b; q2++) acc += Bm[(size_t)r * b + q2] * Zl[(size_t)c * b + q2];
+                s2 += acc * acc;
+            }
+            if (std::sqrt(s2) <= tol * tmax) converged++;
+            else break;   // the leading run of converged pairs
+        }
+        if (converged >= nev || last) break;
+        check_at = std::min(Kmax, ((K * 5 / 4 + b - 1) / b) * b);
     }
     if (!rc) {
         if (!dZ) {

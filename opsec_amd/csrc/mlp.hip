@@ -359,10 +359,27 @@ using namespace opsec;
 
 namespace opsec {
 
-int surrogate_tables(opsec_ctx* ctx, int nf, const int* funcs, const opsec_quad* q, const std::vector<int>& keys,
-                     const std::vector<PairGeo>& geo, const std::vector<int>& nearflag_unused) {
-    (void)nearflag_unused;
+int surrogate_tables(opsec_ctx* ctx, int nf_all, const int* funcs_all, const opsec_quad* q,
+                     const std::vector<int>& keys, const std::vector<PairGeo>& geo,
+                     const std::vector<int>& nearflag) {
     if (!ctx->mlp || !ctx->mlp->trained) return set_error(ctx, OPSEC_ESTATE, "surrogate network is not trained");
+    // functions without band edges (the prior) are integrated by quadrature
+    std::vector<int> band_funcs, other;
+    for (int t = 0; t < nf_all; t++) {
+        const XiDesc& d = ctx->funcs[funcs_all[t]];
+        (d.b > d.a && d.b > 0 ? band_funcs : other).push_back(funcs_all[t]);
+    }
+    if (!other.empty()) {
+        opsec_quad qq = *q;
+        qq.method = OPSEC_QUAD_GAUSS;
+        qq.near = 1;
+        int rc = run_quadrature_public(ctx, (int)other.size(), other.data(), keys, geo, nearflag, &qq, ctx->d_Q,
+                                       ctx->nkeys);
+        if (rc) return rc;
+    }
+    if (band_funcs.empty()) return OPSEC_OK;
+    const int nf = (int)band_funcs.size();
+    const int* funcs = band_funcs.data();
     const SurveyDev& sv = ctx->sv;
     const double h = (sv.max1 - sv.min1) / sv.n1;
     // near pairs (|D| < near cells, minimum image) and a seeded spot-check subset use quadrature

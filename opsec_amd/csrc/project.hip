@@ -148,7 +148,7 @@ __global__ __launch_bounds__(GT_THREADS) void gemm_kernel(SrcA A, size_t a_batch
             const int n = n0 + wn + j * 16 + (lane & 15);
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int m = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+                const int m = m0 + wm + i * 16 + acc_row<T>(lane, r);
                 O.put(blockIdx.z, m, n, M, N, (double)acc[i][j][r]);
             }
         }

@@ -241,7 +241,8 @@ __device__ __forceinline__ bool gm_converged(double val, double err, const GMPar
 
 __global__ __launch_bounds__(64) void gm_kernel(XiModelDev m, const GMJob* jobs, int njobs, int* counter,
                                                 double* dpool, int* ipool, GMParams P, double* Q, size_t qstride,
-                                                double* Qerr, int* status, unsigned long long* nevals) {
+                                                double* Qerr, int* jobfail, unsigned long long* nevals,
+                                                double* trace) {
     __shared__ double vals[GM_NPTS];
     __shared__ int sh_job, sh_nR, sh_done;
     const int lane = threadIdx.x;
@@ -267,7 +268,29 @@ __global__ __launch_bounds__(64) void gm_kernel(XiModelDev m, const GMJob* jobs,
         double tv = 0, te = 0;
         unsigned numEval = GM_NPTS;
         if (lane == 0) heap_push(S, nheap, 0, tv, te);
+        const bool tr = trace && jb == 0;
+        int iter = 0, status = 0;
+        if (tr && lane == 0) {
+            for (int p = 0; p < GM_NPTS; p++) trace[p] = vals[p];
+            trace[149] = S.val[0];
+            trace[150] = S.err[0];
+            trace[151] = S.split[0];
+            trace[152] = P.absTol;
+            trace[153] = P.relTol;
+            trace[154] = P.maxEval;
+            for (int d = 0; d < 6; d++) trace[155 + d] = J.lo[d], trace[161 + d] = J.hi[d];
+        }
         for (;;) {
+            if (tr && lane == 0 && iter < 20) {
+                double* t = trace + 170 + 6 * iter;
+                t[0] = tv;
+                t[1] = te;
+                t[2] = nheap;
+                t[3] = nregions;
+                t[4] = numEval;
+                t[5] = S.err[S.heap[0]];
+            }
+            iter++;
             if (lane == 0) {
                 int done = 0, nR = 0;
                 if (!(numEval < P.maxEval || !P.maxEval)) done = 1;
@@ -305,25 +328,29 @@ __global__ __launch_bounds__(64) void gm_kernel(XiModelDev m, const GMJob* jobs,
             __syncthreads();
             int nR = sh_nR, done = sh_done;
             __syncthreads();
-            if (done == 2) {
-                if (lane == 0) atomicOr(status, 1);
+            if (done) {   // 2: region pool exhausted, the host reruns this job with a larger pool
+                status = done;
                 break;
             }
-            if (done) break;
             __threadfence_block();
             for (int t = 0; t < nR; t++) gm_eval_region(m, xd, J, P, S, R, S.list[t], vals);
             if (lane == 0)
                 for (int t = 0; t < nR; t++) heap_push(S, nheap, S.list[t], tv, te);
         }
         if (lane == 0) {
-            double val = 0, err = 0;
-            for (int i = 0; i < nheap; i++) {
-                val += S.val[S.heap[i]];
-                err += S.err[S.heap[i]];
+            if (status == 2) {
+                jobfail[jb] = 1;
+            } else {
+                double val = 0, err = 0;
+                for (int i = 0; i < nheap; i++) {
+                    val += S.val[S.heap[i]];
+                    err += S.err[S.heap[i]];
+                }
+                Q[(size_t)J.f * qstride + J.key] = val;
+                if (Qerr) Qerr[(size_t)J.f * qstride + J.key] = err;
+                jobfail[jb] = 0;
+                atomicAdd(nevals, (unsigned long long)numEval);
             }
-            Q[(size_t)J.f * qstride + J.key] = val;
-            if (Qerr) Qerr[(size_t)J.f * qstride + J.key] = err;
-            atomicAdd(nevals, (unsigned long long)numEval);
         }
         __syncthreads();
     }
@@ -462,6 +489,11 @@ __global__ __launch_bounds__(256) void assemble_kernel(const int* cellidx, const
 
 // ---- host side ------------------------------------------------------------
 
+
+// Diagnostic: when set, the GM kernel records its first job's node values and
+// per-iteration heap state here (opsec_internal_gm_trace).
+static double* g_gm_trace = nullptr;
+
 struct Scratch {
     void* p = nullptr;
     size_t cap = 0;
@@ -566,35 +598,57 @@ static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::v
                 }
                 jobs.push_back(J);
             }
-        int njobs = (int)jobs.size();
         GMParams P;
         P.absTol = q->epsrel;   // reference argument order, src/sig.cpp:427-430
         P.relTol = q->epsabs;
         P.maxEval = (unsigned)q->maxeval;
-        P.rmax = 4096;
         P.coordsys = ctx->sv.coordsys;
         P.L = ctx->sv.L;
-        int nslots = std::min(njobs, ctx->num_cus * 8);
-        Scratch sj, sd, si, sc;
-        int rc;
-        if ((rc = dev_realloc(ctx, &sj.p, &sj.cap, sizeof(GMJob) * njobs))) return rc;
-        if ((rc = dev_realloc(ctx, &sd.p, &sd.cap, sizeof(double) * 14 * (size_t)P.rmax * nslots))) return rc;
-        if ((rc = dev_realloc(ctx, &si.p, &si.cap, sizeof(int) * 3 * (size_t)P.rmax * nslots))) return rc;
-        if ((rc = dev_realloc(ctx, &sc.p, &sc.cap, 64))) return rc;
-        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(sj.p, jobs.data(), sizeof(GMJob) * njobs, hipMemcpyHostToDevice, ctx->stream));
-        OPSEC_HIP_TRY(ctx, hipMemsetAsync(sc.p, 0, 64, ctx->stream));
-        int* d_counter = (int*)sc.p;
-        int* d_status = d_counter + 1;
-        unsigned long long* d_nev = (unsigned long long*)((char*)sc.p + 8);
-        hipLaunchKernelGGL(gm_kernel, dim3(nslots), dim3(64), 0, ctx->stream, mv, (const GMJob*)sj.p, njobs, d_counter,
-                           (double*)sd.p, (int*)si.p, P, dQ, nkeys, dQerr, d_status, d_nev);
-        OPSEC_HIP_TRY(ctx, hipGetLastError());
-        int status = 0;
-        unsigned long long nev = 0;
-        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&status, d_status, 4, hipMemcpyDeviceToHost, ctx->stream));
-        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nev, d_nev, 8, hipMemcpyDeviceToHost, ctx->stream));
-        OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        if (status) return set_error(ctx, OPSEC_ECONV, "adaptive cubature exceeded %d regions", P.rmax);
+        // Pass 1: every job with a 4096-region pool per wavefront slot.  Pass 2:
+        // jobs that exhausted it, with pools that hold every region the
+        // evaluation cap allows (maxeval / 149 + 2), on fewer slots.
+        unsigned long long nev_total = 0;
+        const int njobs_all = (int)jobs.size();
+        for (int pass = 0; pass < 2 && !jobs.empty(); pass++) {
+            int njobs = (int)jobs.size();
+            long long cap_regions = P.maxEval ? (long long)P.maxEval / GM_NPTS + 2 : (1LL << 20);
+            P.rmax = pass == 0 ? 4096 : (int)std::min<long long>(cap_regions, 1LL << 22);
+            int nslots = pass == 0 ? std::min(njobs, ctx->num_cus * 8) : std::min(njobs, 64);
+            if (pass == 1) {
+                size_t per_slot = (size_t)P.rmax * (14 * 8 + 3 * 4);
+                size_t budget = (size_t)16 << 30;
+                nslots = (int)std::max<size_t>(1, std::min<size_t>(nslots, budget / per_slot));
+            }
+            Scratch sj, sd, si, sc, sf;
+            int rc;
+            if ((rc = dev_realloc(ctx, &sj.p, &sj.cap, sizeof(GMJob) * njobs))) return rc;
+            if ((rc = dev_realloc(ctx, &sd.p, &sd.cap, sizeof(double) * 14 * (size_t)P.rmax * nslots))) return rc;
+            if ((rc = dev_realloc(ctx, &si.p, &si.cap, sizeof(int) * 3 * (size_t)P.rmax * nslots))) return rc;
+            if ((rc = dev_realloc(ctx, &sc.p, &sc.cap, 64))) return rc;
+            if ((rc = dev_realloc(ctx, &sf.p, &sf.cap, sizeof(int) * njobs))) return rc;
+            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(sj.p, jobs.data(), sizeof(GMJob) * njobs, hipMemcpyHostToDevice, ctx->stream));
+            OPSEC_HIP_TRY(ctx, hipMemsetAsync(sc.p, 0, 64, ctx->stream));
+            int* d_counter = (int*)sc.p;
+            unsigned long long* d_nev = (unsigned long long*)((char*)sc.p + 8);
+            hipLaunchKernelGGL(gm_kernel, dim3(nslots), dim3(64), 0, ctx->stream, mv, (const GMJob*)sj.p, njobs,
+                               d_counter, (double*)sd.p, (int*)si.p, P, dQ, nkeys, dQerr, (int*)sf.p, d_nev,
+                               g_gm_trace);
+            OPSEC_HIP_TRY(ctx, hipGetLastError());
+            unsigned long long nev = 0;
+            std::vector<int> fail(njobs);
+            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nev, d_nev, 8, hipMemcpyDeviceToHost, ctx->stream));
+            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(fail.data(), sf.p, sizeof(int) * njobs, hipMemcpyDeviceToHost, ctx->stream));
+            OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            nev_total += nev;
+            std::vector<GMJob> again;
+            for (int i = 0; i < njobs; i++)
+                if (fail[i]) again.push_back(jobs[i]);
+            if (!again.empty() && pass == 1)
+                return set_error(ctx, OPSEC_ECONV, "adaptive cubature exceeded %d regions", P.rmax);
+            jobs.swap(again);
+        }
+        const int njobs = njobs_all;
+        const unsigned long long nev = nev_total;
         ctx->st_integrals += njobs;
         ctx->st_nodes += (int64_t)nev;
         return OPSEC_OK;
@@ -956,6 +1010,25 @@ int opsec_sig_block(opsec_ctx* ctx, const opsec_cell* cells, int ncells, const i
     }
 done:
     ctx->sv = saved;
+    return rc;
+}
+
+// Diagnostic entry (not in the public header): one GM pair integral with the
+// kernel's trace (290 doubles) copied to `trace`.
+int opsec_pair_integral(opsec_ctx* ctx, const opsec_cell* c1, const opsec_cell* c2, int f, const opsec_survey* sv,
+                        const opsec_quad* q, double* Q, double* err);
+int opsec_internal_gm_trace(opsec_ctx* ctx, const opsec_cell* c1, const opsec_cell* c2, int f, const opsec_survey* sv,
+                            const opsec_quad* q, double* Q, double* trace) {
+    if (!ctx || !trace) return OPSEC_EINVAL;
+    double* dtr = nullptr;
+    OPSEC_HIP_TRY(ctx, hipMalloc(&dtr, 300 * 8));
+    OPSEC_HIP_TRY(ctx, hipMemset(dtr, 0, 300 * 8));
+    g_gm_trace = dtr;
+    double err = 0;
+    int rc = opsec_pair_integral(ctx, c1, c2, f, sv, q, Q, &err);
+    g_gm_trace = nullptr;
+    hipMemcpy(trace, dtr, 300 * 8, hipMemcpyDeviceToHost);
+    hipFree(dtr);
     return rc;
 }
 

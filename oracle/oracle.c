@@ -984,3 +984,20 @@ double orc_mlp_loss_grad(const double* w, int n, const double* x, const double* 
     }
     return loss / n;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Row block of the first projection product, C = S[rows, :] B (CPU baseline
+ * sample of src/comma-scalapack.cpp:249).  S: R x n row-major, B: n x m
+ * column-major, C: R x m column-major (ld R). */
+void orc_gemm_rows(int R, int n, const double* S, int m, const double* B, double* C) {
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < m; j++) {
+        const double* bj = B + (size_t)j * n;
+        for (int a = 0; a < R; a++) {
+            const double* srow = S + (size_t)a * n;
+            double acc = 0;
+            for (int b = 0; b < n; b++) acc += srow[b] * bj[b];
+            C[(size_t)j * R + a] = acc;
+        }
+    }
+}

@@ -88,6 +88,8 @@ int orc_sig_block(const orc_xi* f, const orc_cell* cells, int ncells, int Nx, in
 /* P = B^T S B, S n x n row-major, B n x m column-major; P packed upper
  * column-major (covfile layout). */
 void orc_project(int n, const double* S, int m, const double* B, double* Ppacked);
+/* C = S[rows,:] B for a row block (R x n row-major S), C column-major R x m */
+void orc_gemm_rows(int R, int n, const double* S, int m, const double* B, double* C);
 /* estimate-mpi math (src/estimate-mpi.cpp:480-654).  D: nparams packed
  * matrices. Outputs row-major nparams^2 (cov, F, W, M), vectors phat, q, f. */
 int orc_estimate(int M, int np, const double* lambda, const double* D, const double* y, const double* prior,

@@ -41,6 +41,7 @@ _ORC = {
     "orc_sig_block": ([C.POINTER(OrcXi), _P, _I, _I, _I, _I, _D, _P, _I, _P, _I, _P, _I, _I, _D, _D, C.c_uint, _I,
                        _I], _I),
     "orc_project": ([_I, _P, _I, _P, _P], None),
+    "orc_gemm_rows": ([_I, _I, _P, _I, _P, _P], None),
     "orc_estimate": ([_I, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P], _I),
     "orc_inverse": ([_I, _P, _P], _I),
     "orc_sqrtm": ([_I, _P, _P], None),

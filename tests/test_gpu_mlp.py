@@ -1,0 +1,118 @@
+"""Learned pair-integral surrogate (K6): BF16-MFMA forward and backward
+kernels against the FP64 oracle MLP, training, and the ablation against the
+quadrature-only method."""
+import os
+
+import numpy as np
+import pytest
+
+from opsec_amd import lib, synth
+from tests import oracles as O
+
+pytestmark = pytest.mark.gpu
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.npz"))
+
+
+def rand_weights(seed, scale=1.0):
+    orc = O.oracle()
+    nw = orc.orc_mlp_nweights()
+    rng = np.random.default_rng(seed)
+    w = np.zeros(nw)
+    off = 0
+    for din, dout in [(8, 128), (128, 128), (128, 128), (128, 128), (128, 1)]:
+        w[off:off + din * dout] = rng.normal(scale=scale * np.sqrt(2.0 / din), size=din * dout)
+        off += din * dout
+        w[off:off + dout] = rng.normal(scale=0.1, size=dout)
+        off += dout
+    return w
+
+
+def test_forward_vs_oracle(ctx):
+    w = rand_weights(1)
+    ctx.mlp_set_weights(w.astype(np.float32))
+    # the network sees BF16 weights: compare against the oracle on the rounded weights
+    wb = np.asarray(ctx.mlp_weights(), dtype=np.float64)
+    x = np.random.default_rng(2).uniform(-1, 1, size=(1000, 8))
+    got = ctx.mlp_forward(x.astype(np.float32))
+    ref = np.zeros(1000)
+    O.oracle().orc_mlp_forward(O.ptr(wb), 1000, O.ptr(np.ascontiguousarray(x)), O.ptr(ref))
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print("forward max rel err (bf16 operands, fp32 accumulation)", err)
+    assert err < 3e-2
+
+
+def test_gradient_vs_oracle(ctx):
+    w = rand_weights(3, 0.5)
+    ctx.mlp_set_weights(w.astype(np.float32))
+    wb = np.asarray(ctx.mlp_weights(), dtype=np.float64)
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-1, 1, size=(512, 8))
+    t = rng.normal(size=512)
+    loss, g = ctx.mlp_loss_grad(x.astype(np.float32), t.astype(np.float32))
+    gref = np.zeros_like(wb)
+    lref = O.oracle().orc_mlp_loss_grad(O.ptr(wb), 512, O.ptr(np.ascontiguousarray(x)), O.ptr(t), O.ptr(gref))
+    assert loss == pytest.approx(lref, rel=2e-2)
+    # per-layer relative error of the gradient
+    off = 0
+    for din, dout in [(8, 128), (128, 128), (128, 128), (128, 128), (128, 1)]:
+        n = din * dout + dout
+        a, b = g[off:off + n], gref[off:off + n]
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        print(f"layer {din}->{dout} gradient rel err {rel:.2e}")
+        assert rel < 5e-2
+        off += n
+
+
+def _small_box(ctx, n=12, nb=6):
+    cells = synth.box_cells(n, 0.0, 400.0)
+    sv = lib.survey_box(n, 0.0, 400.0)
+    lo, hi = synth.linear_bands(nb, 0.01, 0.2)
+    ctx.model_real(GOLD["pk_k"], GOLD["pk_P"], lo, hi, lib.OPSEC_XI_CLOSED)
+    ctx.cells_upload(cells, sv)
+    return cells, lo, hi
+
+
+def test_training_and_ablation(ctx):
+    """Train on seeded exact integrals; the surrogate then serves the far
+    pairs: fewer quadrature node evaluations, bounded error in Q and in F."""
+    n, nb = 12, 6
+    cells, lo, hi = _small_box(ctx, n, nb)
+    q = ctx.quad(lib.OPSEC_QUAD_GAUSS, nodes=8, near=1)
+    opts = ctx.mlp_opts(nsamples=60000, epochs=30, batch=2048, seed=7, lr=2e-3, near=2.0)
+    loss = ctx.mlp_train(opts, q)
+    print("final training loss", loss)
+    assert loss < 2e-3
+    funcs = list(range(1, nb + 1))
+    # quadrature-only tables
+    ctx.pair_tables(funcs, q)
+    st_q = ctx.pair_stats()
+    Qq = np.array([ctx.pair_table(f) for f in funcs])
+    qs = ctx.quad(lib.OPSEC_QUAD_SURROGATE, nodes=8, spot_frac=0.01, spot_tol=1e-2, spot_seed=11)
+    qs.near = 2
+    ctx.pair_tables(funcs, qs)
+    st_s = ctx.pair_stats()
+    Qs = np.array([ctx.pair_table(f) for f in funcs])
+    scale = np.array([(b ** 3 - a ** 3) / (6 * np.pi ** 2) for a, b in zip(lo, hi)])
+    err = np.abs(Qs - Qq).max(axis=1) / scale
+    print("max |Q_surrogate - Q_quad| / xi_p(0) per band", err)
+    print("quadrature-only stats", st_q, "surrogate stats", st_s)
+    assert st_s["surrogate"] > 0.9 * nb * n ** 3 * 0.9
+    assert st_s["nodes"] < 0.2 * st_q["nodes"]
+    assert err.max() < 5e-2
+    # Fisher matrices through the same modes with and without the surrogate
+    N = len(cells)
+    m = 120
+    B = np.linalg.qr(np.random.default_rng(5).normal(size=(N, m)))[0].T.copy()
+    ctx.modes_upload(B)
+    lam = np.linspace(3, 0.1, m)
+    y = np.sqrt(1 + lam) * np.random.default_rng(6).normal(size=m)
+    prior = ctx.params()
+    ctx.pair_tables(funcs, q)
+    ctx.project(funcs, lib.OPSEC_PREC_FP64, keep_on_device=True)
+    Fq = ctx.estimate_device(lam, y, prior)["F"]
+    ctx.pair_tables(funcs, qs)
+    ctx.project(funcs, lib.OPSEC_PREC_FP64, keep_on_device=True)
+    Fs = ctx.estimate_device(lam, y, prior)["F"]
+    rel = np.abs(np.diag(Fs) - np.diag(Fq)) / np.abs(np.diag(Fq))
+    print("Fisher diagonal relative change with the surrogate", rel)
+    assert rel.max() < 5e-2

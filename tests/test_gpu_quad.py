@@ -14,8 +14,9 @@ GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_vect
 LO, HI = GOLD["c1_band_lo"], GOLD["c1_band_hi"]
 
 
-@pytest.fixture(scope="module")
+@pytest.fixture
 def c1(ctx):
+    """The C1 RealModel (reference xi tables), set up afresh for every test."""
     ctx.model_real(GOLD["pk_k"], GOLD["pk_P"], LO, HI, lib.OPSEC_XI_TABLE)
     return ctx
 
@@ -47,8 +48,6 @@ def test_closed_form_bands_vs_oracle(ctx):
         exp = np.array([orc.orc_band_xi(LO[p], HI[p], x) for x in r])
         scale = HI[p] ** 3 / (6 * np.pi ** 2)
         assert np.abs(got - exp).max() < 1e-14 * scale
-    # restore the table model for the following tests
-    ctx.model_real(GOLD["pk_k"], GOLD["pk_P"], LO, HI, lib.OPSEC_XI_TABLE)
 
 
 def test_gm_pairs_vs_reference(c1):
@@ -106,8 +105,7 @@ def test_testmodel_vs_reference(ctx):
     for f in range(3):
         for i, (a, b) in enumerate(GOLD["pairs"]):
             Q, _ = ctx.pair_integral(cells[a], cells[b], f, sv, q)
-            assert Q == pytest.approx(GOLD["test_gauss_Q"][f, i], rel=1e-12, abs=1e-300)
-    ctx.model_real(GOLD["pk_k"], GOLD["pk_P"], LO, HI, lib.OPSEC_XI_TABLE)
+            assert Q == pytest.approx(GOLD["test_gauss_Q"][f, i], rel=1e-12, abs=1e-300), (f, a, b)
 
 
 def test_gauss_sig_block_vs_oracle(c1):

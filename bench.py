@@ -74,6 +74,7 @@ def run_path(ctx, L, funcs, quad, prec, trace_prec, lam, y, prior, steps, warmup
     """Time `steps` S->Fisher passes; returns (ms per step, est, kernel ms)."""
     import ctypes as C
     s1, s2 = C.c_float(), C.c_float()
+    kg, k1, k2, kn = C.c_float(), C.c_float(), C.c_float(), C.c_int()
     times, est, kms = [], None, []
     for it in range(warmup + steps):
         ctx.sync()
@@ -84,9 +85,10 @@ def run_path(ctx, L, funcs, quad, prec, trace_prec, lam, y, prior, steps, warmup
         ctx.sync()
         dt = time.perf_counter() - t0
         ctx.lib.opsec_internal_project_times(ctx.h, C.byref(s1), C.byref(s2))
+        ctx.lib.opsec_internal_project_kernel_ms(ctx.h, C.byref(kg), C.byref(k1), C.byref(k2), C.byref(kn))
         if it >= warmup:
             times.append(dt)
-            kms.append((s1.value, s2.value))
+            kms.append((s1.value, s2.value, kg.value, k1.value, k2.value, kn.value))
         log(f"[{tag}] step {it - warmup if it >= warmup else 'warmup'}: {dt * 1e3:.1f} ms "
             f"(projection stage 1 {s1.value:.1f} ms, stage 2 {s2.value:.1f} ms)")
     return 1e3 * float(np.mean(times)), est, np.mean(np.array(kms), axis=0)
@@ -219,9 +221,11 @@ def main():
                                f"quadrature", "path": main_path},
     }
     if "fp64" in results:
+        k64 = results["fp64"]["kms"]
         out["fp64_path"] = {"ms_per_step": results["fp64"]["ms"],
                             "value": pairs / (results["fp64"]["ms"] / 1e3),
-                            "projection_ms": list(map(float, results["fp64"]["kms"]))}
+                            "projection_ms": [float(k64[0]), float(k64[1])],
+                            "stage1_tflops": 2.0 * N * N * M * nb / (float(k64[0]) / 1e3) / 1e12}
     if "fp64" in results and "surrogate" in results:
         F64, Fs = results["fp64"]["est"]["F"], results["surrogate"]["est"]["F"]
         d64 = np.diag(F64)
@@ -234,21 +238,28 @@ def main():
         st = results["surrogate"]["stats"]
         out["surrogate_stats"] = {k2: int(v) for k2, v in st.items()}
         out["surrogate_train_s"] = t_train
-    # roofline of the dominant kernel: projection stage 1 (C_p = S_p B', all bands)
-    s1_ms = float(R["kms"][0])
-    flops = 2.0 * N * N * M * nb
+    # roofline of the dominant kernel: the stage-1 projection GEMM C_p = S_p B'
+    # (useful flops 2 N^2 M per band), average launch time from HIP events on
+    # the context's stream
     if main_path == "surrogate":
-        achieved = flops / (s1_ms / 1e3) / 1e12
-        out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TF["bf16"] / 3, "unit": "TFLOP/s",
-                           "frac": achieved / (PEAK_TF["bf16"] / 3), "traffic": None,
-                           "kernel": "gemm_bf16x3_kernel (projection stage 1, 40 bands batched)",
-                           "note": "useful flops 2 N^2 M per band; peak = dense bf16 2516.6 TF / 3 products",
-                           "mfma_utilisation": 3 * achieved / PEAK_TF["bf16"]}
+        kg_ms, k1_ms, k2_ms, launches = [float(x) for x in R["kms"][2:6]]
+        per_launch = k1_ms / launches
+        achieved = 2.0 * N * N * M / (per_launch / 1e3) / 1e12
+        peak = PEAK_TF["bf16"] / 3
+        out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                           "frac": achieved / peak, "traffic": None,
+                           "kernel": "nt_bf16x3_kernel<EpiPlanes> (stage 1, one launch per band)",
+                           "launch_ms": per_launch,
+                           "note": "useful flops 2 N^2 M per launch; peak = dense bf16 2516.6 TF / 3 products",
+                           "mfma_utilisation": 3 * achieved / PEAK_TF["bf16"],
+                           "per_step_ms": {"gen_split": kg_ms, "stage1_gemm": k1_ms, "stage2_gemm": k2_ms}}
     else:
-        achieved = flops / (s1_ms / 1e3) / 1e12
+        s1_ms = float(R["kms"][0])
+        achieved = 2.0 * N * N * M * nb / (s1_ms / 1e3) / 1e12
         out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TF["fp64"], "unit": "TFLOP/s",
                            "frac": achieved / PEAK_TF["fp64"], "traffic": None,
-                           "kernel": "gemm_kernel<double> (projection stage 1, 40 bands batched)"}
+                           "kernel": "gemm_kernel<double> (projection stage 1, all bands in one launch)",
+                           "launch_ms": s1_ms}
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(cfg, cells, k, P, lo, hi, B)

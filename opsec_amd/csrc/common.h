@@ -159,4 +159,13 @@ PairGeo grid_pair_geo(const SurveyDev& sv, int kx, int ky, int kz);
 // Rows of the packed upper triangle: column j starts at j(j+1)/2.
 inline int64_t packed_size(int64_t m) { return m * (m + 1) / 2; }
 
+// Blocking copy ordered on the context's stream.  (hipMemcpy runs on the null
+// stream, which does not order against the context's non-blocking stream, so
+// a kernel queued on the context's stream could overtake it.)
+inline hipError_t stream_copy(opsec_ctx* ctx, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e;
+}
+
 }  // namespace opsec

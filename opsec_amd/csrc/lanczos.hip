@@ -499,11 +499,11 @@ static int dense_eigen_top(Dev& D, double* dA, int K, int nev, std::vector<doubl
     hipLaunchKernelGGL(diag_kernel, dim3((K + 255) / 256), dim3(256), 0, D.s, dA, K, dd, de + (K >= 2 ? K - 2 : 0));
     int rc = D.sync();
     if (rc) return rc;
-    OPSEC_HIP_TRY(ctx, hipMemcpy(d.data(), dd, (size_t)K * 8, hipMemcpyDeviceToHost));
-    if (K >= 2) OPSEC_HIP_TRY(ctx, hipMemcpy(e.data(), de, (size_t)(K - 1) * 8, hipMemcpyDeviceToHost));
+    OPSEC_HIP_TRY(ctx, stream_copy(ctx, d.data(), dd, (size_t)K * 8, hipMemcpyDeviceToHost));
+    if (K >= 2) OPSEC_HIP_TRY(ctx, stream_copy(ctx, e.data(), de, (size_t)(K - 1) * 8, hipMemcpyDeviceToHost));
     std::vector<double> Z;
     tridiag_top(K, d, e, nev, evals, Z);
-    OPSEC_HIP_TRY(ctx, hipMemcpy(dZ, Z.data(), (size_t)K * nev * 8, hipMemcpyHostToDevice));
+    OPSEC_HIP_TRY(ctx, stream_copy(ctx, dZ, Z.data(), (size_t)K * nev * 8, hipMemcpyHostToDevice));
     for (int k = K - 3; k >= 0; k--) {
         const int m = K - k - 1;
         hipLaunchKernelGGL(hh_apply_dot_kernel, dim3(nev), dim3(256), 0, D.s, dA, K, k, dZ, nev, dp);
@@ -591,7 +591,7 @@ static int orthonormalise_block(Dev& D, LanczosWork& L, int N, int b, int kprev,
             D.gemm_tn(X, N, X, N, N, b, b, L.small);
             if ((rc = D.sync())) return rc;
             std::vector<double> G((size_t)b * b), w, U;
-            OPSEC_HIP_TRY(ctx, hipMemcpy(G.data(), L.small, (size_t)b * b * 8, hipMemcpyDeviceToHost));
+            OPSEC_HIP_TRY(ctx, stream_copy(ctx, G.data(), L.small, (size_t)b * b * 8, hipMemcpyDeviceToHost));
             for (int i = 0; i < b; i++)
                 for (int j = 0; j < i; j++) G[i * b + j] = G[j * b + i] = 0.5 * (G[i * b + j] + G[j * b + i]);
             host_sym_eigen(b, G, w, U);   // G = U diag(w) U^T, U column i = eigvec i (U[k*b+i])
@@ -609,8 +609,8 @@ static int orthonormalise_block(Dev& D, LanczosWork& L, int N, int b, int kprev,
                     Bnew[i * b + k] = sw * U[k * b + i];
                 }
             }
-            OPSEC_HIP_TRY(ctx, hipMemcpy(L.small, Mh.data(), (size_t)b * b * 8, hipMemcpyHostToDevice));
-            OPSEC_HIP_TRY(ctx, hipMemcpy(L.W, X, (size_t)N * b * 8, hipMemcpyDeviceToDevice));
+            OPSEC_HIP_TRY(ctx, stream_copy(ctx, L.small, Mh.data(), (size_t)b * b * 8, hipMemcpyHostToDevice));
+            OPSEC_HIP_TRY(ctx, stream_copy(ctx, L.W, X, (size_t)N * b * 8, hipMemcpyDeviceToDevice));
             D.gemm_nn(0, X, N, L.W, N, L.small, b, N, b, b);
             if ((rc = D.sync())) return rc;
             // Bmat <- Bnew * Bmat
@@ -624,7 +624,7 @@ static int orthonormalise_block(Dev& D, LanczosWork& L, int N, int b, int kprev,
                 std::vector<double> r((size_t)N);
                 for (int c : bad) {
                     for (int i = 0; i < N; i++) r[i] = nd(rng);
-                    OPSEC_HIP_TRY(ctx, hipMemcpy(X + (size_t)c * N, r.data(), (size_t)N * 8, hipMemcpyHostToDevice));
+                    OPSEC_HIP_TRY(ctx, stream_copy(ctx, X + (size_t)c * N, r.data(), (size_t)N * 8, hipMemcpyHostToDevice));
                     for (int j = 0; j < b; j++) Bmat[c * b + j] = 0.0;
                 }
                 replaced = true;
@@ -671,7 +671,7 @@ static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double
     {
         std::vector<double> r((size_t)N * b);
         for (auto& x : r) x = nd(rng);
-        OPSEC_HIP_TRY(ctx, hipMemcpy(L.V, r.data(), r.size() * 8, hipMemcpyHostToDevice));
+        OPSEC_HIP_TRY(ctx, stream_copy(ctx, L.V, r.data(), r.size() * 8, hipMemcpyHostToDevice));
         std::vector<double> Bm;
         int rc = orthonormalise_block(D, L, N, b, 0, L.V, Bm, rng);
         if (rc) {
@@ -693,7 +693,7 @@ static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double
         D.gemm_tn(Vj, N, Xn, N, N, b, b, L.small);   // A_j = V_j^T S V_j
         if ((rc = D.sync())) break;
         std::vector<double> Aj((size_t)b * b);
-        OPSEC_HIP_TRY(ctx, hipMemcpy(Aj.data(), L.small, (size_t)b * b * 8, hipMemcpyDeviceToHost));
+        OPSEC_HIP_TRY(ctx, stream_copy(ctx, Aj.data(), L.small, (size_t)b * b * 8, hipMemcpyDeviceToHost));
         for (int c = 0; c < b; c++)
             for (int r = 0; r < b; r++)
                 T[(size_t)(j * b + c) * Kmax + j * b + r] = 0.5 * (Aj[(size_t)c * b + r] + Aj[(size_t)r * b + c]);
@@ -719,7 +719,7 @@ static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double
         std::vector<double> Tk((size_t)K * K);
         for (int c = 0; c < K; c++)
             for (int r = 0; r < K; r++) Tk[(size_t)c * K + r] = T[(size_t)c * Kmax + r];
-        OPSEC_HIP_TRY(ctx, hipMemcpy(dT, Tk.data(), Tk.size() * 8, hipMemcpyHostToDevice));
+        OPSEC_HIP_TRY(ctx, stream_copy(ctx, dT, Tk.data(), Tk.size() * 8, hipMemcpyHostToDevice));
         hipFree(dZ);
         dZ = nullptr;
         const int want = std::min(nev, K);
@@ -734,7 +734,7 @@ static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double
         Kr = K;
         std::vector<double> Zl((size_t)b * want);
         for (int c = 0; c < want; c++)
-            OPSEC_HIP_TRY(ctx, hipMemcpy(&Zl[(size_t)c * b], dZ + (size_t)c * K + (K - b), (size_t)b * 8,
+            OPSEC_HIP_TRY(ctx, stream_copy(ctx, &Zl[(size_t)c * b], dZ + (size_t)c * K + (K - b), (size_t)b * 8,
                                          hipMemcpyDeviceToHost));
         double tmax = 0;
         for (double t : ritz) tmax = std::max(tmax, std::fabs(t));
@@ -765,7 +765,7 @@ static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double
                 D.gemm_nn(0, dY, N, L.V, N, dZ, Kr, N, Kr, want);
                 rc = D.sync();
                 if (!rc) {
-                    hipError_t e = hipMemcpy(evecs, dY, (size_t)N * want * 8, hipMemcpyDeviceToHost);
+                    hipError_t e = stream_copy(ctx, evecs, dY, (size_t)N * want * 8, hipMemcpyDeviceToHost);
                     if (e != hipSuccess) rc = set_error(ctx, OPSEC_EHIP, "copy eigenvectors: %s", hipGetErrorString(e));
                 }
                 hipFree(dY);
@@ -795,7 +795,7 @@ int opsec_lanczos_dense(opsec_ctx* ctx, int n, const double* A, int nev, double 
     OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
     double* dS = nullptr;
     OPSEC_HIP_TRY(ctx, hipMalloc(&dS, (size_t)n * n * 8));
-    OPSEC_HIP_TRY(ctx, hipMemcpy(dS, A, (size_t)n * n * 8, hipMemcpyHostToDevice));
+    OPSEC_HIP_TRY(ctx, stream_copy(ctx, dS, A, (size_t)n * n * 8, hipMemcpyHostToDevice));
     int rc = block_lanczos(ctx, dS, n, nev, tol, block, maxdim, evals, evecs, nconv);
     hipFree(dS);
     return rc;

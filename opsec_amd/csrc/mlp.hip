@@ -6,17 +6,27 @@
 // accumulation.  Weight layout (shared with the oracle's FP64 MLP): for each
 // layer, W (out x in, row-major) then b (out).
 //
-// Features of a box pair (key = |dd| per axis in cells, cell width h) and a
-// band [klo, khi]:
-//   x0..x2  sorted |dd| / 32
-//   x3      |D| h^-1 / 32           (centre separation in cells)
-//   x4, x5  klo h / pi, khi h / pi
-//   x6      log(|D|/h) / 4
-//   x7      0.5 (klo + khi) |D| / (8 pi)
-// Target: Q / s,  s = (khi^3 - klo^3) / (6 pi^2) = xi_band(0).
-// Pairs with |D| < near cell widths always use quadrature; a seeded fraction
-// of the other pairs is spot-checked against quadrature and any pair whose
-// error exceeds spot_tol * s is replaced by its quadrature value.
+// What the network learns.  For a box pair at minimum-image offset d (cells,
+// cell width a, |d| = D) and a band [klo, khi], the pair integral is the
+// band's xi averaged over the tent kernel of the cube pair.  In Fourier space
+// that kernel is prod_i sinc^2(k_i a / 2); replacing it by its average over
+// directions, Wbar(k a), gives the isotropic baseline
+//   base(D) = 1/(2 pi^2) int_klo^khi k^2 j0(k D a) Wbar(k a) dk
+// (16 Gauss-Legendre nodes per band; Wbar by 48 x 48-point quadrature over an
+// octant of the sphere).  The network predicts the rest -- the anisotropic
+// part of the kernel and the periodic wrap near half the box -- scaled to
+// O(1):  Q = base + s r_p y,  s = (khi^3 - klo^3) / (6 pi^2) = xi_band(0),
+// r_p = RMS of (Q - base) / s over the band's training pairs.
+// Features:
+//   x0..x2  sorted |d_i| / (n_i / 2)
+//   x3, x4  klo a / pi, khi a / pi
+//   x5      log(D) / 4
+//   x6      (klo + khi) D a / (16 pi)
+//   x7      (khi - klo) D a / (2 pi)
+// Pairs with D < near, and pairs whose kernel crosses half a periodic box
+// (where the minimum image folds the integrand), always use quadrature; a
+// seeded fraction of the other pairs is spot-checked against quadrature and
+// any pair whose error exceeds spot_tol * s keeps its quadrature value.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -51,6 +61,24 @@ struct MlpState {
     float* delta2 = nullptr;  // B x 128
     float* out = nullptr;     // B
     int cap_batch = 0;
+    // isotropic baseline per function index (band functions only)
+    std::vector<struct SurBand> bands;
+    struct SurBand* d_bands = nullptr;
+    size_t cap_bands = 0;
+};
+
+constexpr int SUR_NK = 16;
+struct SurBand {
+    double klo = 0, khi = 0, sc = 0, rs = 1;
+    double cell = 0;            // cell width the nodes were made for
+    double kk[SUR_NK], wk[SUR_NK];
+};
+
+// Box geometry for the device-side features.
+struct SurGeo {
+    int n1, n2, n3;
+    int per1, per2, per3;       // minimum image along the axis
+    double a;                   // cell width
 };
 
 __device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
@@ -267,7 +295,7 @@ static int ensure_mlp(opsec_ctx* ctx) {
         std::normal_distribution<float> nd(0.f, std::sqrt(2.f / din));
         for (int i = 0; i < din * dout; i++) w[mlp_w_off(l) + i] = nd(rng) * (l == 4 ? 0.1f : 1.f);
     }
-    OPSEC_HIP_TRY(ctx, hipMemcpy(s->w, w.data(), MLP_NW * 4, hipMemcpyHostToDevice));
+    OPSEC_HIP_TRY(ctx, stream_copy(ctx, s->w, w.data(), MLP_NW * 4, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(to_bf16_kernel, dim3((MLP_NW + 255) / 256), dim3(256), 0, ctx->stream, s->w, s->wb, MLP_NW);
     hipLaunchKernelGGL(zero_f_kernel, dim3((MLP_NW + 255) / 256), dim3(256), 0, ctx->stream, s->m, (size_t)MLP_NW);
     hipLaunchKernelGGL(zero_f_kernel, dim3((MLP_NW + 255) / 256), dim3(256), 0, ctx->stream, s->v, (size_t)MLP_NW);
@@ -330,27 +358,164 @@ static void loss_grad_device(opsec_ctx* ctx, const float* dx, const float* dt, i
     }
 }
 
-// Features of pairs (host, one row per (pair, band)).
-static void pair_features(double kx, double ky, double kz, double h, double klo, double khi, float* x) {
-    double a[3] = {kx, ky, kz};
-    std::sort(a, a + 3, [](double p, double q) { return p > q; });
-    double D = std::sqrt(kx * kx + ky * ky + kz * kz);
-    x[0] = (float)(a[0] / 32);
-    x[1] = (float)(a[1] / 32);
-    x[2] = (float)(a[2] / 32);
-    x[3] = (float)(D / 32);
-    x[4] = (float)(klo * h / M_PI);
-    x[5] = (float)(khi * h / M_PI);
-    x[6] = (float)(std::log(std::max(D, 1e-3)) / 4);
-    x[7] = (float)(0.5 * (klo + khi) * D * h / (8 * M_PI));
-}
-
 static double band_scale(double klo, double khi) { return (khi * khi * khi - klo * klo * klo) / (6 * M_PI * M_PI); }
 
 // minimum-image cell offset along one axis of a periodic box
 static int mi_cells(int k, int n, double L, double extent) {
     if (L + 1e-9 * extent >= extent && L <= extent * (1 + 1e-12)) return std::min(k, n - k);
     return k;
+}
+
+static bool axis_periodic(double L, double extent) {
+    return L + 1e-9 * extent >= extent && L <= extent * (1 + 1e-12);
+}
+
+// True when the cube-pair kernel along some periodic axis reaches past half
+// the box (2 |d_i| + 2 > n_i): the minimum image folds the integrand there and
+// the pair goes to quadrature.
+static bool wraps(const SurveyDev& sv, int kx, int ky, int kz) {
+    const int n[3] = {sv.n1, sv.n2, sv.n3}, k[3] = {kx, ky, kz};
+    const double ext[3] = {sv.max1 - sv.min1, sv.max2 - sv.min2, sv.max3 - sv.min3};
+    for (int i = 0; i < 3; i++)
+        if (axis_periodic(sv.L, ext[i]) && 2 * std::min(k[i], n[i] - k[i]) + 2 > n[i]) return true;
+    return false;
+}
+
+static SurGeo sur_geo(const SurveyDev& sv) {
+    SurGeo g;
+    g.n1 = sv.n1;
+    g.n2 = sv.n2;
+    g.n3 = sv.n3;
+    g.per1 = axis_periodic(sv.L, sv.max1 - sv.min1);
+    g.per2 = axis_periodic(sv.L, sv.max2 - sv.min2);
+    g.per3 = axis_periodic(sv.L, sv.max3 - sv.min3);
+    g.a = (sv.max1 - sv.min1) / sv.n1;
+    return g;
+}
+
+// Direction average of prod_i sinc^2(x n_i / 2) over the unit sphere (by
+// symmetry over one octant: mu = cos(theta) in [0, 1], phi in [0, pi/2]).
+static double cube_kernel_avg(double x) {
+    static std::vector<double> t48, w48;
+    if (t48.empty()) {   // nodes on [-1, 1]
+        t48.resize(48);
+        w48.resize(48);
+        gauss_legendre(48, t48.data(), w48.data());
+    }
+    auto sinc2 = [](double u) {
+        if (std::fabs(u) < 1e-4) return 1.0 - u * u / 3.0;
+        double s = std::sin(u) / u;
+        return s * s;
+    };
+    double acc = 0;
+    for (int i = 0; i < 48; i++) {
+        const double mu = 0.5 * (t48[i] + 1), wm = 0.5 * w48[i];
+        const double st = std::sqrt(std::max(0.0, 1 - mu * mu));
+        for (int j = 0; j < 48; j++) {
+            const double phi = 0.25 * M_PI * (t48[j] + 1), wp = 0.25 * M_PI * w48[j];
+            const double n1 = st * std::cos(phi), n2 = st * std::sin(phi), n3 = mu;
+            acc += wm * wp * sinc2(0.5 * x * n1) * sinc2(0.5 * x * n2) * sinc2(0.5 * x * n3);
+        }
+    }
+    return acc / (0.5 * M_PI);
+}
+
+// (Re)build the baseline nodes of every band function for cell width a and
+// upload them.  Training scales (rs) survive while a band is unchanged.
+static int prepare_bands(opsec_ctx* ctx, double a) {
+    MlpState* s = ctx->mlp;
+    const size_t nf = ctx->funcs.size();
+    if (s->bands.size() != nf) s->bands.assign(nf, SurBand());
+    std::vector<double> t(SUR_NK), w(SUR_NK);
+    gauss_legendre(SUR_NK, t.data(), w.data());
+    bool changed = false;
+    for (size_t f = 0; f < nf; f++) {
+        const XiDesc& d = ctx->funcs[f];
+        SurBand& B = s->bands[f];
+        if (!(d.b > d.a && d.b > 0)) continue;
+        if (B.klo == d.a && B.khi == d.b && B.cell == a) continue;
+        B = SurBand();
+        B.klo = d.a;
+        B.khi = d.b;
+        B.cell = a;
+        B.sc = band_scale(d.a, d.b);
+        for (int n = 0; n < SUR_NK; n++) {
+            const double k = 0.5 * (d.a + d.b) + 0.5 * (d.b - d.a) * t[n];
+            B.kk[n] = k;
+            B.wk[n] = 0.5 * (d.b - d.a) * w[n] * k * k * cube_kernel_avg(k * a) / (2 * M_PI * M_PI);
+        }
+        changed = true;
+    }
+    int rc;
+    if ((rc = dev_realloc(ctx, (void**)&s->d_bands, &s->cap_bands, sizeof(SurBand) * nf))) return rc;
+    (void)changed;
+    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(s->d_bands, s->bands.data(), sizeof(SurBand) * nf, hipMemcpyHostToDevice,
+                                      ctx->stream));
+    return OPSEC_OK;
+}
+
+__device__ __forceinline__ int sur_mi(int k, int n, int per) { return per ? min(k, n - k) : k; }
+
+// Rows of (pair key, band function): features and baseline.  Explicit rows
+// (keys[i], fidx[i]) when nfar == 0, else row i = (keys[i % nfar],
+// fidx[i / nfar]).
+__global__ __launch_bounds__(256) void sur_rows_kernel(const int* __restrict__ keys, const int* __restrict__ fidx,
+                                                       long long nrows, int nfar, const SurBand* __restrict__ bands,
+                                                       SurGeo g, float* __restrict__ X, double* __restrict__ base) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    const int key = nfar ? keys[i % nfar] : keys[i];
+    const int f = nfar ? fidx[i / nfar] : fidx[i];
+    const SurBand& B = bands[f];
+    const int kz = key % g.n3, ky = (key / g.n3) % g.n2, kx = key / (g.n3 * g.n2);
+    double e[3] = {(double)sur_mi(kx, g.n1, g.per1) / (0.5 * g.n1), (double)sur_mi(ky, g.n2, g.per2) / (0.5 * g.n2),
+                   (double)sur_mi(kz, g.n3, g.per3) / (0.5 * g.n3)};
+    const double dx = sur_mi(kx, g.n1, g.per1), dy = sur_mi(ky, g.n2, g.per2), dz = sur_mi(kz, g.n3, g.per3);
+    const double D = sqrt(dx * dx + dy * dy + dz * dz);
+    // sort descending
+    if (e[0] < e[1]) { double t = e[0]; e[0] = e[1]; e[1] = t; }
+    if (e[1] < e[2]) { double t = e[1]; e[1] = e[2]; e[2] = t; }
+    if (e[0] < e[1]) { double t = e[0]; e[0] = e[1]; e[1] = t; }
+    float* x = X + i * MLP_IN;
+    x[0] = (float)e[0];
+    x[1] = (float)e[1];
+    x[2] = (float)e[2];
+    x[3] = (float)(B.klo * g.a / M_PI);
+    x[4] = (float)(B.khi * g.a / M_PI);
+    x[5] = (float)(log(fmax(D, 1e-3)) / 4);
+    x[6] = (float)((B.klo + B.khi) * D * g.a / (16 * M_PI));
+    x[7] = (float)((B.khi - B.klo) * D * g.a / (2 * M_PI));
+    const double r = D * g.a;
+    double acc = 0;
+    for (int n = 0; n < SUR_NK; n++) {
+        const double u = B.kk[n] * r;
+        const double j0 = (u < 1e-4) ? 1.0 - u * u / 6.0 : sin(u) / u;
+        acc += B.wk[n] * j0;
+    }
+    base[i] = acc;
+}
+
+// Q[f][key] = base + s r y for the surrogate rows; spot-checked keys keep their
+// quadrature value and count a failure when the prediction is off by more
+// than spot_tol * s.
+__global__ __launch_bounds__(256) void sur_combine_kernel(const int* __restrict__ keys, const int* __restrict__ fidx,
+                                                          long long nrows, int nfar, const SurBand* __restrict__ bands,
+                                                          const float* __restrict__ Y, const double* __restrict__ base,
+                                                          const unsigned char* __restrict__ is_spot, double* Q,
+                                                          size_t qstride, double spot_tol,
+                                                          unsigned long long* fails) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    const int key = keys[i % nfar];
+    const int f = fidx[i / nfar];
+    const SurBand& B = bands[f];
+    const double pred = base[i] + B.sc * B.rs * (double)Y[i];
+    double* q = Q + (size_t)f * qstride + key;
+    if (is_spot[key]) {
+        if (fabs(pred - *q) > spot_tol * B.sc) atomicAdd(fails, 1ULL);
+    } else {
+        *q = pred;
+    }
 }
 
 }  // namespace opsec
@@ -396,7 +561,7 @@ int surrogate_tables(opsec_ctx* ctx, int nf_all, const int* funcs_all, const ops
                ez = mi_cells(kz, sv.n3, sv.L, sv.max3 - sv.min3);
         double D = std::sqrt(ex * ex + ey * ey + ez * ez);
         Dk[i] = D;
-        if (D < q->near) {
+        if (D < q->near || wraps(sv, kx, ky, kz)) {
             qkeys.push_back(key);
             qgeo.push_back(geo[i]);
             qnear.push_back(1);
@@ -418,54 +583,59 @@ int surrogate_tables(opsec_ctx* ctx, int nf_all, const int* funcs_all, const ops
     for (size_t i = 0; i < qkeys.size(); i++) nearflag_q[i] = (qkeys[i] == 0);
     int rc = run_quadrature_public(ctx, nf, funcs, qkeys, qgeo, nearflag_q, &qq, ctx->d_Q, ctx->nkeys);
     if (rc) return rc;
-    // keep the quadrature values of the spot-check pairs
-    std::vector<double> Qq((size_t)ctx->funcs.size() * ctx->nkeys);
-    OPSEC_HIP_TRY(ctx, hipMemcpy(Qq.data(), ctx->d_Q, Qq.size() * 8, hipMemcpyDeviceToHost));
-    // surrogate inference for all far pairs of every band function
-    size_t nrows = far.size() * (size_t)nf;
-    std::vector<float> X(nrows * MLP_IN);
-    for (int t = 0; t < nf; t++) {
-        const XiDesc& d = ctx->funcs[funcs[t]];
-        for (size_t r = 0; r < far.size(); r++) {
-            int key = keys[far[r]];
-            int kz = key % sv.n3, ky = (key / sv.n3) % sv.n2, kx = key / (sv.n3 * sv.n2);
-            double ex = mi_cells(kx, sv.n1, sv.L, sv.max1 - sv.min1), ey = mi_cells(ky, sv.n2, sv.L, sv.max2 - sv.min2),
-                   ez = mi_cells(kz, sv.n3, sv.L, sv.max3 - sv.min3);
-            pair_features(ex, ey, ez, h, d.a, d.b, &X[((size_t)t * far.size() + r) * MLP_IN]);
-        }
-    }
-    float *dx = nullptr, *dout = nullptr;
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dx, X.size() * 4 + 4));
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dout, nrows * 4 + 4));
-    OPSEC_HIP_TRY(ctx, hipMemcpy(dx, X.data(), X.size() * 4, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(mlp_forward_kernel<false>, dim3((unsigned)((nrows + FT - 1) / FT)), dim3(256), 0, ctx->stream,
-                       ctx->mlp->wb, ctx->mlp->w, dx, (int)nrows, dout, (float*)nullptr, (__bf16*)nullptr, 0);
-    OPSEC_HIP_TRY(ctx, hipGetLastError());
-    std::vector<float> Y(nrows);
-    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(Y.data(), dout, nrows * 4, hipMemcpyDeviceToHost, ctx->stream));
-    OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    hipFree(dx);
-    hipFree(dout);
-    std::vector<char> is_spot(ctx->nkeys, 0);
+    // surrogate rows: every far key x every band function, on the device
+    if ((rc = prepare_bands(ctx, h))) return rc;
+    std::vector<int> farkeys(far.size());
+    for (size_t r = 0; r < far.size(); r++) farkeys[r] = keys[far[r]];
+    std::vector<unsigned char> is_spot(ctx->nkeys, 0);
     for (int k : skeys) is_spot[k] = 1;
-    int64_t spot = 0, fail = 0;
-    for (int t = 0; t < nf; t++) {
-        const int f = funcs[t];
-        const XiDesc& d = ctx->funcs[f];
-        const double sc = band_scale(d.a, d.b);
-        double* Qf = &Qq[(size_t)f * ctx->nkeys];
-        for (size_t r = 0; r < far.size(); r++) {
-            int key = keys[far[r]];
-            double pred = (double)Y[(size_t)t * far.size() + r] * sc;
-            if (is_spot[key]) {
-                spot++;
-                if (std::fabs(pred - Qf[key]) > q->spot_tol * sc) fail++;   // keep quadrature value
-                continue;
-            }
-            Qf[key] = pred;
+    const long long nrows = (long long)far.size() * nf;
+    int64_t fail = 0;
+    if (nrows > 0) {
+        int *dkeys = nullptr, *dfidx = nullptr;
+        unsigned char* dspot = nullptr;
+        float *dx = nullptr, *dout = nullptr;
+        double* dbase = nullptr;
+        unsigned long long* dfail = nullptr;
+        auto release = [&]() {
+            hipFree(dkeys);
+            hipFree(dfidx);
+            hipFree(dspot);
+            hipFree(dx);
+            hipFree(dout);
+            hipFree(dbase);
+            hipFree(dfail);
+        };
+        if (hipMalloc(&dkeys, farkeys.size() * 4) != hipSuccess || hipMalloc(&dfidx, (size_t)nf * 4) != hipSuccess ||
+            hipMalloc(&dspot, (size_t)ctx->nkeys) != hipSuccess ||
+            hipMalloc(&dx, (size_t)nrows * MLP_IN * 4) != hipSuccess ||
+            hipMalloc(&dout, (size_t)nrows * 4) != hipSuccess || hipMalloc(&dbase, (size_t)nrows * 8) != hipSuccess ||
+            hipMalloc(&dfail, 8) != hipSuccess) {
+            release();
+            return set_error(ctx, OPSEC_ENOMEM, "surrogate workspace for %lld rows", nrows);
         }
+        hipStream_t st = ctx->stream;
+        hipMemcpyAsync(dkeys, farkeys.data(), farkeys.size() * 4, hipMemcpyHostToDevice, st);
+        hipMemcpyAsync(dfidx, funcs, (size_t)nf * 4, hipMemcpyHostToDevice, st);
+        hipMemcpyAsync(dspot, is_spot.data(), (size_t)ctx->nkeys, hipMemcpyHostToDevice, st);
+        hipMemsetAsync(dfail, 0, 8, st);
+        const SurGeo g = sur_geo(sv);
+        const unsigned nblk = (unsigned)((nrows + 255) / 256);
+        hipLaunchKernelGGL(sur_rows_kernel, dim3(nblk), dim3(256), 0, st, dkeys, dfidx, nrows, (int)far.size(),
+                           ctx->mlp->d_bands, g, dx, dbase);
+        hipLaunchKernelGGL(mlp_forward_kernel<false>, dim3((unsigned)((nrows + FT - 1) / FT)), dim3(256), 0, st,
+                           ctx->mlp->wb, ctx->mlp->w, dx, (int)nrows, dout, (float*)nullptr, (__bf16*)nullptr, 0);
+        hipLaunchKernelGGL(sur_combine_kernel, dim3(nblk), dim3(256), 0, st, dkeys, dfidx, nrows, (int)far.size(),
+                           ctx->mlp->d_bands, dout, dbase, dspot, ctx->d_Q, (size_t)ctx->nkeys, q->spot_tol, dfail);
+        unsigned long long hf = 0;
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(&hf, dfail, 8, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        release();
+        if (e != hipSuccess) return set_error(ctx, OPSEC_EHIP, "surrogate inference: %s", hipGetErrorString(e));
+        fail = (int64_t)hf;
     }
-    OPSEC_HIP_TRY(ctx, hipMemcpy(ctx->d_Q, Qq.data(), Qq.size() * 8, hipMemcpyHostToDevice));
+    const int64_t spot = (int64_t)skeys.size() * nf;
     ctx->st_surrogate += (int64_t)(far.size() * nf) - (int64_t)skeys.size() * nf;
     ctx->st_spot += spot;
     ctx->st_spot_fail += fail;
@@ -489,6 +659,7 @@ void opsec_internal_free_mlp(opsec_ctx* ctx) {
     hipFree(s->delta);
     hipFree(s->delta2);
     hipFree(s->out);
+    hipFree(s->d_bands);
     delete s;
     ctx->mlp = nullptr;
 }
@@ -514,7 +685,7 @@ int opsec_mlp_get_weights(opsec_ctx* ctx, float* w) {
     if (!ctx || !w) return OPSEC_EINVAL;
     int rc = ensure_mlp(ctx);
     if (rc) return rc;
-    OPSEC_HIP_TRY(ctx, hipMemcpy(w, ctx->mlp->w, MLP_NW * 4, hipMemcpyDeviceToHost));
+    OPSEC_HIP_TRY(ctx, stream_copy(ctx, w, ctx->mlp->w, MLP_NW * 4, hipMemcpyDeviceToHost));
     return OPSEC_OK;
 }
 
@@ -522,7 +693,7 @@ int opsec_mlp_set_weights(opsec_ctx* ctx, const float* w) {
     if (!ctx || !w) return OPSEC_EINVAL;
     int rc = ensure_mlp(ctx);
     if (rc) return rc;
-    OPSEC_HIP_TRY(ctx, hipMemcpy(ctx->mlp->w, w, MLP_NW * 4, hipMemcpyHostToDevice));
+    OPSEC_HIP_TRY(ctx, stream_copy(ctx, ctx->mlp->w, w, MLP_NW * 4, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(to_bf16_kernel, dim3((MLP_NW + 255) / 256), dim3(256), 0, ctx->stream, ctx->mlp->w, ctx->mlp->wb,
                        MLP_NW);
     OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -537,7 +708,7 @@ int opsec_mlp_forward(opsec_ctx* ctx, int n, const float* x, float* out) {
     float *dx = nullptr, *dout = nullptr;
     OPSEC_HIP_TRY(ctx, hipMalloc(&dx, (size_t)n * MLP_IN * 4));
     OPSEC_HIP_TRY(ctx, hipMalloc(&dout, (size_t)n * 4));
-    OPSEC_HIP_TRY(ctx, hipMemcpy(dx, x, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice));
+    OPSEC_HIP_TRY(ctx, stream_copy(ctx, dx, x, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(mlp_forward_kernel<false>, dim3((n + FT - 1) / FT), dim3(256), 0, ctx->stream, ctx->mlp->wb,
                        ctx->mlp->w, dx, n, dout, (float*)nullptr, (__bf16*)nullptr, 0);
     OPSEC_HIP_TRY(ctx, hipGetLastError());
@@ -558,15 +729,15 @@ int opsec_mlp_loss_grad(opsec_ctx* ctx, int n, const float* x, const float* t, d
     OPSEC_HIP_TRY(ctx, hipMalloc(&dx, (size_t)n * MLP_IN * 4));
     OPSEC_HIP_TRY(ctx, hipMalloc(&dt, (size_t)n * 4));
     OPSEC_HIP_TRY(ctx, hipMalloc(&dl, 8));
-    OPSEC_HIP_TRY(ctx, hipMemcpy(dx, x, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice));
-    OPSEC_HIP_TRY(ctx, hipMemcpy(dt, t, (size_t)n * 4, hipMemcpyHostToDevice));
-    OPSEC_HIP_TRY(ctx, hipMemset(dl, 0, 8));
+    OPSEC_HIP_TRY(ctx, stream_copy(ctx, dx, x, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice));
+    OPSEC_HIP_TRY(ctx, stream_copy(ctx, dt, t, (size_t)n * 4, hipMemcpyHostToDevice));
+    OPSEC_HIP_TRY(ctx, hipMemsetAsync(dl, 0, 8, ctx->stream));
     loss_grad_device(ctx, dx, dt, n, dl);
     OPSEC_HIP_TRY(ctx, hipGetLastError());
     OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     double l = 0;
-    OPSEC_HIP_TRY(ctx, hipMemcpy(&l, dl, 8, hipMemcpyDeviceToHost));
-    OPSEC_HIP_TRY(ctx, hipMemcpy(grad, ctx->mlp->g, MLP_NW * 4, hipMemcpyDeviceToHost));
+    OPSEC_HIP_TRY(ctx, stream_copy(ctx, &l, dl, 8, hipMemcpyDeviceToHost));
+    OPSEC_HIP_TRY(ctx, stream_copy(ctx, grad, ctx->mlp->g, MLP_NW * 4, hipMemcpyDeviceToHost));
     *loss = l / n;
     hipFree(dx);
     hipFree(dt);
@@ -592,7 +763,7 @@ int opsec_mlp_train(opsec_ctx* ctx, const opsec_mlp_train_opts* o, const opsec_q
         double ex = mi_cells(kx, sv.n1, sv.L, sv.max1 - sv.min1), ey = mi_cells(ky, sv.n2, sv.L, sv.max2 - sv.min2),
                ez = mi_cells(kz, sv.n3, sv.L, sv.max3 - sv.min3);
         double D = std::sqrt(ex * ex + ey * ey + ez * ez);
-        if (D >= o->near) far.push_back({D, key});
+        if (D >= o->near && !wraps(sv, kx, ky, kz)) far.push_back({D, key});
     }
     if (far.empty()) return set_error(ctx, OPSEC_EINVAL, "no pairs beyond %g cell widths to train on", o->near);
     std::sort(far.begin(), far.end());
@@ -601,7 +772,6 @@ int opsec_mlp_train(opsec_ctx* ctx, const opsec_mlp_train_opts* o, const opsec_q
     std::uniform_int_distribution<int> bb(1, nb);
     std::vector<std::vector<int>> keys_of(nb + 1);
     std::vector<int> tkeys, tf;
-    std::vector<float> X;
     for (int i = 0; i < o->nsamples; i++) {
         double D = uD(rng);
         auto it = std::lower_bound(far.begin(), far.end(), std::make_pair(D, -1));
@@ -612,12 +782,6 @@ int opsec_mlp_train(opsec_ctx* ctx, const opsec_mlp_train_opts* o, const opsec_q
         std::uniform_int_distribution<long> pick(0, (long)(hi - lo) - 1);
         int key = (lo + pick(rng))->second;
         int f = bb(rng);
-        int kz = key % sv.n3, ky = (key / sv.n3) % sv.n2, kx = key / (sv.n3 * sv.n2);
-        double ex = mi_cells(kx, sv.n1, sv.L, sv.max1 - sv.min1), ey = mi_cells(ky, sv.n2, sv.L, sv.max2 - sv.min2),
-               ez = mi_cells(kz, sv.n3, sv.L, sv.max3 - sv.min3);
-        float x[MLP_IN];
-        pair_features(ex, ey, ez, h, ctx->funcs[f].a, ctx->funcs[f].b, x);
-        X.insert(X.end(), x, x + MLP_IN);
         tkeys.push_back(key);
         tf.push_back(f);
         keys_of[f].push_back(key);
@@ -647,13 +811,51 @@ int opsec_mlp_train(opsec_ctx* ctx, const opsec_mlp_train_opts* o, const opsec_q
         integrals += (int64_t)ks.size();
     }
     std::vector<double> Qall((size_t)ctx->funcs.size() * ctx->nkeys);
-    OPSEC_HIP_TRY(ctx, hipMemcpy(Qall.data(), dQs, Qall.size() * 8, hipMemcpyDeviceToHost));
+    OPSEC_HIP_TRY(ctx, stream_copy(ctx, Qall.data(), dQs, Qall.size() * 8, hipMemcpyDeviceToHost));
     hipFree(dQs);
-    std::vector<float> T(ns);
-    for (int i = 0; i < ns; i++) {
-        const XiDesc& d = ctx->funcs[tf[i]];
-        T[i] = (float)(Qall[(size_t)tf[i] * ctx->nkeys + tkeys[i]] / band_scale(d.a, d.b));
+    // features and isotropic baseline of the samples (the inference kernel)
+    if ((rc = prepare_bands(ctx, h))) return rc;
+    std::vector<float> X((size_t)ns * MLP_IN);
+    std::vector<double> base(ns);
+    {
+        int *dk = nullptr, *df = nullptr;
+        float* dxf = nullptr;
+        double* db = nullptr;
+        hipError_t e = hipMalloc(&dk, (size_t)ns * 4);
+        if (e == hipSuccess) e = hipMalloc(&df, (size_t)ns * 4);
+        if (e == hipSuccess) e = hipMalloc(&dxf, (size_t)ns * MLP_IN * 4);
+        if (e == hipSuccess) e = hipMalloc(&db, (size_t)ns * 8);
+        if (e == hipSuccess) e = hipMemcpyAsync(dk, tkeys.data(), (size_t)ns * 4, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(df, tf.data(), (size_t)ns * 4, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(sur_rows_kernel, dim3((ns + 255) / 256), dim3(256), 0, ctx->stream, dk, df,
+                               (long long)ns, 0, ctx->mlp->d_bands, sur_geo(sv), dxf, db);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(X.data(), dxf, X.size() * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(base.data(), db, (size_t)ns * 8, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        hipFree(dk);
+        hipFree(df);
+        hipFree(dxf);
+        hipFree(db);
+        if (e != hipSuccess) return set_error(ctx, OPSEC_EHIP, "surrogate features: %s", hipGetErrorString(e));
     }
+    // residual targets, scaled per band to unit RMS
+    std::vector<double> res(ns), ss(ctx->funcs.size(), 0.0);
+    std::vector<int> cnt(ctx->funcs.size(), 0);
+    for (int i = 0; i < ns; i++) {
+        const SurBand& Bd = ctx->mlp->bands[tf[i]];
+        res[i] = (Qall[(size_t)tf[i] * ctx->nkeys + tkeys[i]] - base[i]) / Bd.sc;
+        ss[tf[i]] += res[i] * res[i];
+        cnt[tf[i]]++;
+    }
+    for (size_t f = 0; f < ctx->funcs.size(); f++)
+        if (cnt[f] > 0) ctx->mlp->bands[f].rs = std::max(std::sqrt(ss[f] / cnt[f]), 1e-14);
+    OPSEC_HIP_TRY(ctx, stream_copy(ctx, ctx->mlp->d_bands, ctx->mlp->bands.data(),
+                                   sizeof(SurBand) * ctx->funcs.size(), hipMemcpyHostToDevice));
+    std::vector<float> T(ns);
+    for (int i = 0; i < ns; i++) T[i] = (float)(res[i] / ctx->mlp->bands[tf[i]].rs);
     ctx->st_integrals = integrals;
     // device copies
     float *dX = nullptr, *dT = nullptr;

@@ -11,6 +11,15 @@
 // Precisions: FP64 (v_mfma_f64_16x16x4f64), FP32 (v_mfma_f32_16x16x4f32) and
 // split-BF16x3 (x = hi + lo in bf16, hi*hi + hi*lo + lo*hi on
 // v_mfma_f32_16x16x32_bf16 with FP32 accumulation).
+//
+// The split-BF16x3 path runs per band on pre-split operand planes (bf16 hi
+// and lo, k contiguous, zero padded to multiples of 128):
+//   gen      S_p planes from the pair table (HBM-write bound, ~1 ms per band at
+//            32^3 cells)
+//   stage 1  C_p = S_p B'     NT GEMM, epilogue writes C_p already split
+//   stage 2  P_p = B'^T C_p   NT GEMM on the same kernel, packed FP64 output
+// B' is split once per mode upload.  Splitting outside the GEMM keeps its
+// inner loop to 16-byte loads, LDS staging and MFMAs.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -31,6 +40,17 @@ struct ProjState {
     int prec = 0;
     std::vector<int> funcs;
     float last_ms_stage1 = 0, last_ms_stage2 = 0;
+    // split-BF16x3 planes: B' (Mp x Kp), one band's S_p (Kp x Kp), its C_p (Mp x Kp)
+    __bf16* d_Bp = nullptr;      // hi plane then lo plane
+    size_t cap_Bp = 0;
+    bool bp_valid = false;
+    __bf16* d_Sp = nullptr;
+    size_t cap_Sp = 0;
+    __bf16* d_Cp = nullptr;
+    size_t cap_Cp = 0;
+    // per-kernel sums over the bands of the last split-BF16x3 projection (ms)
+    float last_ms_gen = 0, last_ms_gemm1 = 0, last_ms_gemm2 = 0;
+    int last_launches = 0;
 };
 
 // ---- FP64 / FP32 MFMA GEMM -----------------------------------------------------
@@ -197,87 +217,204 @@ struct GB_ColMajorT {
     __device__ __forceinline__ void p_shift(int z, size_t s) { p += (size_t)z * s; }
 };
 
-// ---- split-BF16x3 GEMM -----------------------------------------------------------
+// ---- split-BF16x3 on pre-split planes --------------------------------------------
 //
-// Tile 128 x 128 x 32.  LDS holds hi and lo bf16 tiles in [m][k] order (k
-// contiguous, 8 bf16 per lane per fragment).  Accumulation in FP32.
+// A plane pair (hi, lo) holds X = hi + lo for a rows x ld row-major matrix in
+// bf16 with k contiguous; ld and the row count are multiples of 128 and the
+// padding is zero, so the GEMM needs no bounds checks.
 
-constexpr int BKB = 32;
-constexpr int BPAD = 8;   // bf16 padding per LDS row
+constexpr int PT = 128;      // tile edge
+constexpr int PK = 32;       // k per step
+constexpr int PLDS = PK + 8; // LDS row stride in bf16 (80 bytes: 16-byte aligned rows)
 
-__device__ __forceinline__ void split_bf16(double x, __bf16& hi, __bf16& lo) {
-    float f = (float)x;
-    hi = (__bf16)f;
-    lo = (__bf16)(f - (float)hi);
+inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+__device__ __forceinline__ void split2(float x, __bf16& hi, __bf16& lo) {
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
 }
 
-template <typename Src>
-__device__ __forceinline__ void tile_fetch32(const Src& s, int i0, int k0, int nI, int nK, double (&r)[16]) {
-    const int t = threadIdx.x;
-    if (Src::k_contig) {
-        int i = i0 + (t >> 1), kb = k0 + (t & 1) * 16;
+// planes[r][c] = split(X[r * ldx + c]) for r < rows, c < cols; zero elsewhere
+// (r < rows_p, c < ld).  One thread per 8 consecutive columns.
+__global__ __launch_bounds__(256) void split_plane_kernel(const double* __restrict__ X, size_t ldx, int rows, int cols,
+                                                          __bf16* __restrict__ hi, __bf16* __restrict__ lo, size_t ld,
+                                                          int rows_p) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t per_row = ld / 8;
+    const size_t r = t / per_row;
+    if (r >= (size_t)rows_p) return;
+    const int c0 = (int)(t % per_row) * 8;
+    bf8_t h, l;
 #pragma unroll
-        for (int e = 0; e < 16; e++) r[e] = (i < nI && kb + e < nK) ? s.get(i, kb + e) : 0.0;
-    } else {
-        int k = k0 + (t >> 3), ib = i0 + (t & 7) * 16;
-#pragma unroll
-        for (int e = 0; e < 16; e++) r[e] = (k < nK && ib + e < nI) ? s.get(ib + e, k) : 0.0;
+    for (int e = 0; e < 8; e++) {
+        const int c = c0 + e;
+        const float v = (r < (size_t)rows && c < cols) ? (float)X[r * ldx + c] : 0.0f;
+        __bf16 a, b;
+        split2(v, a, b);
+        h[e] = a;
+        l[e] = b;
     }
+    *(bf8_t*)(hi + r * ld + c0) = h;
+    *(bf8_t*)(lo + r * ld + c0) = l;
 }
 
-template <typename Src>
-__device__ __forceinline__ void tile_store32(__bf16 (*hi)[BKB + BPAD], __bf16 (*lo)[BKB + BPAD], const double (&r)[16]) {
-    const int t = threadIdx.x;
-    if (Src::k_contig) {
-        int i = t >> 1, kb = (t & 1) * 16;
-#pragma unroll
-        for (int e = 0; e < 16; e++) split_bf16(r[e], hi[i][kb + e], lo[i][kb + e]);
-    } else {
-        int k = t >> 3, ib = (t & 7) * 16;
-#pragma unroll
-        for (int e = 0; e < 16; e++) split_bf16(r[e], hi[ib + e][k], lo[ib + e][k]);
+// S_p planes from the pair table: S[a][b] = Q[key(a, b)] for a, b < N (the
+// sqrt(Nbar) factors live in B').  One thread per 8 consecutive columns.
+__global__ __launch_bounds__(256) void gen_split_kernel(const double* __restrict__ Q, const int* __restrict__ cellidx,
+                                                        int Ny, int Nz, int N, __bf16* __restrict__ hi,
+                                                        __bf16* __restrict__ lo, size_t ld, int rows_p) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t per_row = ld / 8;
+    const size_t a = t / per_row;
+    if (a >= (size_t)rows_p) return;
+    const int b0 = (int)(t % per_row) * 8;
+    bf8_t h, l;
+    int xa = 0, ya = 0, za = 0;
+    if (a < (size_t)N) {
+        xa = cellidx[3 * a];
+        ya = cellidx[3 * a + 1];
+        za = cellidx[3 * a + 2];
     }
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int b = b0 + e;
+        float v = 0.0f;
+        if (a < (size_t)N && b < N) {
+            const int* cb = cellidx + 3 * b;
+            const int key = (abs(xa - cb[0]) * Ny + abs(ya - cb[1])) * Nz + abs(za - cb[2]);
+            v = (float)Q[key];
+        }
+        __bf16 x, y;
+        split2(v, x, y);
+        h[e] = x;
+        l[e] = y;
+    }
+    *(bf8_t*)(hi + a * ld + b0) = h;
+    *(bf8_t*)(lo + a * ld + b0) = l;
 }
 
-template <typename SrcA, typename SrcB, typename Out, bool UPPER>
-__global__ __launch_bounds__(GT_THREADS) void gemm_bf16x3_kernel(SrcA A, size_t a_batch, SrcB B, size_t b_batch, Out O,
-                                                                 int M, int N, int K) {
-    __shared__ __bf16 Ah[GT_M][BKB + BPAD], Al[GT_M][BKB + BPAD];
-    __shared__ __bf16 Bh[GT_N][BKB + BPAD], Bl[GT_N][BKB + BPAD];
-    const int m0 = blockIdx.x * GT_M, n0 = blockIdx.y * GT_N;
-    if (UPPER && m0 > n0 + GT_N - 1) return;
-    A.p_shift(blockIdx.z, a_batch);
-    B.p_shift(blockIdx.z, b_batch);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+// Tile order: tiles are grouped 8 m-blocks x 8 n-blocks so that the blocks
+// resident at one time share A and B slices in L2, and consecutive groups are
+// dealt to the eight XCDs (workgroups go to XCDs round-robin) so that each XCD
+// works through its own range of groups.
+__device__ __forceinline__ void tile_of(int id, int nmb, int nnb, int& mb, int& nbk) {
+    const int total = nmb * nnb;
+    if (total % 8 == 0) id = (id % 8) * (total / 8) + id / 8;
+    const int G = 8;
+    const int group = id / (G * nnb);
+    const int first = group * G;
+    const int gs = min(nmb - first, G);
+    const int in = id % (G * nnb);
+    mb = first + in % gs;
+    nbk = in / gs;
+}
+
+// D[m][n] -> planes [n][m] (the stage-1 product C_p, cells contiguous), split.
+struct EpiPlanes {
+    __bf16* hi;
+    __bf16* lo;
+    size_t ld;
+    __device__ __forceinline__ void put4(int m, int n, const f4_t& v) const {
+        typedef __bf16 bf4_t __attribute__((ext_vector_type(4)));
+        bf4_t h, l;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            __bf16 x, y;
+            split2(v[r], x, y);
+            h[r] = x;
+            l[r] = y;
+        }
+        *(bf4_t*)(hi + (size_t)n * ld + m) = h;
+        *(bf4_t*)(lo + (size_t)n * ld + m) = l;
+    }
+    static constexpr bool upper = false;
+};
+
+// D[m][n] -> packed upper triangle, FP64 (m <= n < M).
+struct EpiPacked {
+    double* p;
+    int M;
+    __device__ __forceinline__ void put4(int m, int n, const f4_t& v) const {
+        if (n >= M) return;
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            if (m + r <= n) p[(size_t)n * (n + 1) / 2 + m + r] = (double)v[r];
+    }
+    static constexpr bool upper = true;
+};
+
+// D = A B^T on plane pairs: A (nmb*128 rows) and B (nnb*128 rows), both with
+// K (multiple of 32) contiguous.  128 x 128 tile per workgroup, 4 waves of
+// 64 x 64, v_mfma_f32_16x16x32_bf16 with FP32 accumulation.
+template <typename Epi>
+__global__ __launch_bounds__(256) void nt_bf16x3_kernel(const __bf16* __restrict__ Ah, const __bf16* __restrict__ Al,
+                                                        size_t lda, const __bf16* __restrict__ Bh,
+                                                        const __bf16* __restrict__ Bl, size_t ldb, int K, int nmb,
+                                                        int nnb, Epi E) {
+    __shared__ __attribute__((aligned(16))) __bf16 sAh[PT][PLDS];
+    __shared__ __attribute__((aligned(16))) __bf16 sAl[PT][PLDS];
+    __shared__ __attribute__((aligned(16))) __bf16 sBh[PT][PLDS];
+    __shared__ __attribute__((aligned(16))) __bf16 sBl[PT][PLDS];
+    int mb, nbk;
+    tile_of(blockIdx.x, nmb, nnb, mb, nbk);
+    const int m0 = mb * PT, n0 = nbk * PT;
+    if (Epi::upper && m0 > n0 + PT - 1) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    // this thread's two 16-byte chunks of each 128 x 32 plane tile
+    const int r0 = tid >> 2, r1 = r0 + 64, kc = (tid & 3) * 8;
+    const __bf16* pAh0 = Ah + (size_t)(m0 + r0) * lda + kc;
+    const __bf16* pAh1 = Ah + (size_t)(m0 + r1) * lda + kc;
+    const __bf16* pAl0 = Al + (size_t)(m0 + r0) * lda + kc;
+    const __bf16* pAl1 = Al + (size_t)(m0 + r1) * lda + kc;
+    const __bf16* pBh0 = Bh + (size_t)(n0 + r0) * ldb + kc;
+    const __bf16* pBh1 = Bh + (size_t)(n0 + r1) * ldb + kc;
+    const __bf16* pBl0 = Bl + (size_t)(n0 + r0) * ldb + kc;
+    const __bf16* pBl1 = Bl + (size_t)(n0 + r1) * ldb + kc;
     f4_t acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = f4_t{0, 0, 0, 0};
-    double ra[16], rb[16];
-    tile_fetch32(A, m0, 0, M, K, ra);
-    tile_fetch32(B, n0, 0, N, K, rb);
-    for (int k0 = 0; k0 < K; k0 += BKB) {
+    bf8_t g[8];
+    g[0] = *(const bf8_t*)pAh0;
+    g[1] = *(const bf8_t*)pAh1;
+    g[2] = *(const bf8_t*)pAl0;
+    g[3] = *(const bf8_t*)pAl1;
+    g[4] = *(const bf8_t*)pBh0;
+    g[5] = *(const bf8_t*)pBh1;
+    g[6] = *(const bf8_t*)pBl0;
+    g[7] = *(const bf8_t*)pBl1;
+    const int kq = 8 * (lane >> 4), fr = lane & 15;
+    for (int k0 = 0; k0 < K; k0 += PK) {
         __syncthreads();
-        tile_store32<SrcA>(Ah, Al, ra);
-        tile_store32<SrcB>(Bh, Bl, rb);
+        *(bf8_t*)&sAh[r0][kc] = g[0];
+        *(bf8_t*)&sAh[r1][kc] = g[1];
+        *(bf8_t*)&sAl[r0][kc] = g[2];
+        *(bf8_t*)&sAl[r1][kc] = g[3];
+        *(bf8_t*)&sBh[r0][kc] = g[4];
+        *(bf8_t*)&sBh[r1][kc] = g[5];
+        *(bf8_t*)&sBl[r0][kc] = g[6];
+        *(bf8_t*)&sBl[r1][kc] = g[7];
         __syncthreads();
-        if (k0 + BKB < K) {
-            tile_fetch32(A, m0, k0 + BKB, M, K, ra);
-            tile_fetch32(B, n0, k0 + BKB, N, K, rb);
+        if (k0 + PK < K) {
+            const int o = k0 + PK;
+            g[0] = *(const bf8_t*)(pAh0 + o);
+            g[1] = *(const bf8_t*)(pAh1 + o);
+            g[2] = *(const bf8_t*)(pAl0 + o);
+            g[3] = *(const bf8_t*)(pAl1 + o);
+            g[4] = *(const bf8_t*)(pBh0 + o);
+            g[5] = *(const bf8_t*)(pBh1 + o);
+            g[6] = *(const bf8_t*)(pBl0 + o);
+            g[7] = *(const bf8_t*)(pBl1 + o);
         }
-        const int kq = 8 * (lane >> 4);
         bf8_t ah[4], al[4], bh[4], bl[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            ah[i] = *(const bf8_t*)&Ah[wm + i * 16 + (lane & 15)][kq];
-            al[i] = *(const bf8_t*)&Al[wm + i * 16 + (lane & 15)][kq];
-        }
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            bh[j] = *(const bf8_t*)&Bh[wn + j * 16 + (lane & 15)][kq];
-            bl[j] = *(const bf8_t*)&Bl[wn + j * 16 + (lane & 15)][kq];
+            ah[i] = *(const bf8_t*)&sAh[wm + i * 16 + fr][kq];
+            al[i] = *(const bf8_t*)&sAl[wm + i * 16 + fr][kq];
+            bh[i] = *(const bf8_t*)&sBh[wn + i * 16 + fr][kq];
+            bl[i] = *(const bf8_t*)&sBl[wn + i * 16 + fr][kq];
         }
 #pragma unroll
         for (int i = 0; i < 4; i++)
@@ -291,14 +428,8 @@ __global__ __launch_bounds__(GT_THREADS) void gemm_bf16x3_kernel(SrcA A, size_t 
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int n = n0 + wn + j * 16 + (lane & 15);
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int m = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
-                O.put(blockIdx.z, m, n, M, N, (double)acc[i][j][r]);
-            }
-        }
+        for (int j = 0; j < 4; j++)
+            E.put4(m0 + wm + i * 16 + 4 * (lane >> 4), n0 + wn + j * 16 + fr, acc[i][j]);
 }
 
 // B' = diag(w) B with w = sqrt(Nbar)
@@ -316,8 +447,91 @@ static dim3 grid_for(int M, int N, int nb) {
     return dim3((M + GT_M - 1) / GT_M, (N + GT_N - 1) / GT_N, nb);
 }
 
+static dim3 blocks_for(size_t threads) { return dim3((unsigned)((threads + 255) / 256)); }
+
+// Split a row-major double matrix (rows x cols, ldx) into a plane pair of
+// rows_p x ld.
+static int split_planes(opsec_ctx* ctx, const double* X, size_t ldx, int rows, int cols, __bf16* planes, int rows_p,
+                        size_t ld) {
+    __bf16* hi = planes;
+    __bf16* lo = planes + (size_t)rows_p * ld;
+    hipLaunchKernelGGL(split_plane_kernel, blocks_for((size_t)rows_p * ld / 8), dim3(256), 0, ctx->stream, X, ldx,
+                       rows, cols, hi, lo, ld, rows_p);
+    OPSEC_HIP_TRY(ctx, hipGetLastError());
+    return OPSEC_OK;
+}
+
+// C = A B^T (stage 1, epilogue to split planes) or P = A B^T (stage 2, packed).
+template <typename Epi>
+static void launch_nt(opsec_ctx* ctx, const __bf16* A, int arows, const __bf16* B, int brows, size_t ld, const Epi& E) {
+    const int nmb = arows / PT, nnb = brows / PT;
+    hipLaunchKernelGGL(nt_bf16x3_kernel<Epi>, dim3(nmb * nnb), dim3(256), 0, ctx->stream, A, A + (size_t)arows * ld, ld,
+                       B, B + (size_t)brows * ld, ld, (int)ld, nmb, nnb, E);
+}
+
+// Split-BF16x3 projection of the bands funcs[0..nb) into d_P, band by band.
+static int run_project_bf16x3(opsec_ctx* ctx, int nb, const int* funcs) {
+    ProjState* ps = ctx->proj;
+    const int N = ctx->ncells, M = ctx->nmodes;
+    const size_t Kp = round_up(N, PT), Mp = round_up(M, PT);
+    int rc;
+    if ((rc = dev_realloc(ctx, (void**)&ps->d_Bp, &ps->cap_Bp, 2 * Mp * Kp * sizeof(__bf16)))) return rc;
+    if ((rc = dev_realloc(ctx, (void**)&ps->d_Sp, &ps->cap_Sp, 2 * Kp * Kp * sizeof(__bf16)))) return rc;
+    if ((rc = dev_realloc(ctx, (void**)&ps->d_Cp, &ps->cap_Cp, 2 * Mp * Kp * sizeof(__bf16)))) return rc;
+    if ((rc = dev_realloc(ctx, &ps->d_P, &ps->cap_P, (size_t)nb * packed_size(M) * 8))) return rc;
+    if (!ps->bp_valid) {
+        // B' is column-major N x M: row n of the plane is mode n, cells contiguous
+        if ((rc = split_planes(ctx, ps->d_Bw, (size_t)N, M, N, ps->d_Bp, (int)Mp, Kp))) return rc;
+        ps->bp_valid = true;
+    }
+    std::vector<hipEvent_t> ev(4 * (size_t)nb);
+    for (auto& e : ev) OPSEC_HIP_TRY(ctx, hipEventCreate(&e));
+    __bf16* Sh = ps->d_Sp;
+    __bf16* Sl = ps->d_Sp + Kp * Kp;
+    EpiPlanes toC{ps->d_Cp, ps->d_Cp + Mp * Kp, Kp};
+    for (int z = 0; z < nb; z++) {
+        hipEventRecord(ev[4 * z], ctx->stream);
+        hipLaunchKernelGGL(gen_split_kernel, blocks_for(Kp * Kp / 8), dim3(256), 0, ctx->stream,
+                           ctx->d_Q + (size_t)funcs[z] * ctx->nkeys, ctx->d_cellidx, ctx->sv.n2, ctx->sv.n3, N, Sh,
+                           Sl, Kp, (int)Kp);
+        hipEventRecord(ev[4 * z + 1], ctx->stream);
+        launch_nt(ctx, ps->d_Sp, (int)Kp, ps->d_Bp, (int)Mp, Kp, toC);
+        hipEventRecord(ev[4 * z + 2], ctx->stream);
+        EpiPacked toP{(double*)ps->d_P + (size_t)z * packed_size(M), M};
+        launch_nt(ctx, ps->d_Bp, (int)Mp, ps->d_Cp, (int)Mp, Kp, toP);
+        hipEventRecord(ev[4 * z + 3], ctx->stream);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = set_error(ctx, OPSEC_EHIP, "projection launch: %s", hipGetErrorString(e));
+    hipError_t es = hipEventSynchronize(ev[4 * nb - 1]);
+    if (!rc && es != hipSuccess) rc = set_error(ctx, OPSEC_EHIP, "projection: %s", hipGetErrorString(es));
+    float tg = 0, t1 = 0, t2 = 0;
+    if (!rc)
+        for (int z = 0; z < nb; z++) {
+            float a, b, c;
+            hipEventElapsedTime(&a, ev[4 * z], ev[4 * z + 1]);
+            hipEventElapsedTime(&b, ev[4 * z + 1], ev[4 * z + 2]);
+            hipEventElapsedTime(&c, ev[4 * z + 2], ev[4 * z + 3]);
+            tg += a;
+            t1 += b;
+            t2 += c;
+        }
+    for (auto& x : ev) hipEventDestroy(x);
+    if (rc) return rc;
+    ps->last_ms_gen = tg;
+    ps->last_ms_gemm1 = t1;
+    ps->last_ms_gemm2 = t2;
+    ps->last_launches = nb;
+    ps->last_ms_stage1 = tg + t1;
+    ps->last_ms_stage2 = t2;
+    ps->nb = nb;
+    ps->prec = OPSEC_PREC_BF16X3;
+    return OPSEC_OK;
+}
+
 // Stage 1 + stage 2 for nb functions (device pair tables), results to d_P.
-static int run_project(opsec_ctx* ctx, int nb, const int* d_funcs, int prec) {
+static int run_project(opsec_ctx* ctx, int nb, const int* d_funcs, const int* funcs, int prec) {
+    if (prec == OPSEC_PREC_BF16X3) return run_project_bf16x3(ctx, nb, funcs);
     ProjState* ps = ctx->proj;
     const int N = ctx->ncells, M = ctx->nmodes;
     const size_t esize = (prec == OPSEC_PREC_FP64) ? 8 : 4;
@@ -360,14 +574,6 @@ static int run_project(opsec_ctx* ctx, int nb, const int* d_funcs, int prec) {
         GB_ColMajorT<float> B2{(const float*)ps->d_C, (size_t)N};
         hipLaunchKernelGGL((gemm_kernel<float, GA_ColMajorT, GB_ColMajorT<float>, PutPacked, true>), g2,
                            dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, (size_t)N * M, outP, M, M, N);
-    } else {
-        PutColMajor<float> outC{(float*)ps->d_C, (size_t)N, (size_t)N * M};
-        hipLaunchKernelGGL((gemm_bf16x3_kernel<GA_PairTable, GB_ColMajor, PutColMajor<float>, false>), g1,
-                           dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, 0, outC, N, M, N);
-        hipEventRecord(e1, ctx->stream);
-        GB_ColMajorT<float> B2{(const float*)ps->d_C, (size_t)N};
-        hipLaunchKernelGGL((gemm_bf16x3_kernel<GA_ColMajorT, GB_ColMajorT<float>, PutPacked, true>), g2,
-                           dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, (size_t)N * M, outP, M, M, N);
     }
     hipEventRecord(e2, ctx->stream);
     hipError_t e = hipGetLastError();
@@ -394,6 +600,9 @@ void opsec_internal_free_proj(opsec_ctx* ctx) {
     dev_free((void**)&ctx->proj->d_Bw);
     dev_free(&ctx->proj->d_C);
     dev_free(&ctx->proj->d_P);
+    dev_free((void**)&ctx->proj->d_Bp);
+    dev_free((void**)&ctx->proj->d_Sp);
+    dev_free((void**)&ctx->proj->d_Cp);
     delete ctx->proj;
     ctx->proj = nullptr;
 }
@@ -419,6 +628,7 @@ int opsec_modes_upload(opsec_ctx* ctx, const double* B, int ncells, int nmodes) 
     OPSEC_HIP_TRY(ctx, hipGetLastError());
     OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->nmodes = nmodes;
+    ps->bp_valid = false;
     return OPSEC_OK;
 }
 
@@ -435,7 +645,7 @@ int opsec_project(opsec_ctx* ctx, int nf, const int* funcs, int prec, double* P_
     int* d_funcs = nullptr;
     OPSEC_HIP_TRY(ctx, hipMalloc(&d_funcs, nf * sizeof(int)));
     OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_funcs, funcs, nf * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-    int rc = run_project(ctx, nf, d_funcs, prec);
+    int rc = run_project(ctx, nf, d_funcs, funcs, prec);
     hipStreamSynchronize(ctx->stream);
     hipFree(d_funcs);
     if (rc) return rc;
@@ -453,6 +663,17 @@ int opsec_internal_project_times(opsec_ctx* ctx, float* s1, float* s2) {
     if (!ctx || !ctx->proj) return OPSEC_EINVAL;
     *s1 = ctx->proj->last_ms_stage1;
     *s2 = ctx->proj->last_ms_stage2;
+    return OPSEC_OK;
+}
+
+// Per-kernel times of the last split-BF16x3 projection, summed over its bands
+// (ms): S_p plane generation, stage-1 GEMM, stage-2 GEMM; *launches = bands.
+int opsec_internal_project_kernel_ms(opsec_ctx* ctx, float* gen, float* gemm1, float* gemm2, int* launches) {
+    if (!ctx || !ctx->proj) return OPSEC_EINVAL;
+    *gen = ctx->proj->last_ms_gen;
+    *gemm1 = ctx->proj->last_ms_gemm1;
+    *gemm2 = ctx->proj->last_ms_gemm2;
+    *launches = ctx->proj->last_launches;
     return OPSEC_OK;
 }
 
@@ -498,12 +719,29 @@ int opsec_project_dense(opsec_ctx* ctx, int n, const double* S, int nmodes, cons
         hipLaunchKernelGGL((gemm_kernel<float, GA_ColMajorT, GB_ColMajorT<float>, PutPacked, true>), g2,
                            dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, 0, outP, M, M, N);
     } else {
-        PutColMajor<float> outC{(float*)dC, (size_t)N, 0};
-        hipLaunchKernelGGL((gemm_bf16x3_kernel<GA_RowMajor, GB_ColMajor, PutColMajor<float>, false>), g1,
-                           dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, 0, outC, N, M, N);
-        GB_ColMajorT<float> B2{(const float*)dC, (size_t)N};
-        hipLaunchKernelGGL((gemm_bf16x3_kernel<GA_ColMajorT, GB_ColMajorT<float>, PutPacked, true>), g2,
-                           dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, 0, outP, M, M, N);
+        const size_t Kp = round_up(N, PT), Mp = round_up(M, PT);
+        __bf16 *Sp = nullptr, *Bp = nullptr, *Cp = nullptr;
+        OPSEC_HIP_TRY(ctx, hipMalloc(&Sp, 2 * Kp * Kp * sizeof(__bf16)));
+        OPSEC_HIP_TRY(ctx, hipMalloc(&Bp, 2 * Mp * Kp * sizeof(__bf16)));
+        OPSEC_HIP_TRY(ctx, hipMalloc(&Cp, 2 * Mp * Kp * sizeof(__bf16)));
+        int rc2 = split_planes(ctx, dS, (size_t)N, N, N, Sp, (int)Kp, Kp);
+        // B is row-major nmodes x n here: row = mode, cells contiguous
+        if (!rc2) rc2 = split_planes(ctx, dB, (size_t)N, M, N, Bp, (int)Mp, Kp);
+        if (!rc2) {
+            launch_nt(ctx, Sp, (int)Kp, Bp, (int)Mp, Kp, EpiPlanes{Cp, Cp + Mp * Kp, Kp});
+            launch_nt(ctx, Bp, (int)Mp, Cp, (int)Mp, Kp, EpiPacked{dP, M});
+        }
+        hipStreamSynchronize(ctx->stream);
+        hipFree(Sp);
+        hipFree(Bp);
+        hipFree(Cp);
+        if (rc2) {
+            hipFree(dS);
+            hipFree(dB);
+            hipFree(dC);
+            hipFree(dP);
+            return rc2;
+        }
     }
     hipError_t e = hipGetLastError();
     int rc = OPSEC_OK;

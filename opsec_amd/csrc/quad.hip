@@ -1022,12 +1022,12 @@ int opsec_internal_gm_trace(opsec_ctx* ctx, const opsec_cell* c1, const opsec_ce
     if (!ctx || !trace) return OPSEC_EINVAL;
     double* dtr = nullptr;
     OPSEC_HIP_TRY(ctx, hipMalloc(&dtr, 300 * 8));
-    OPSEC_HIP_TRY(ctx, hipMemset(dtr, 0, 300 * 8));
+    OPSEC_HIP_TRY(ctx, hipMemsetAsync(dtr, 0, 300 * 8, ctx->stream));
     g_gm_trace = dtr;
     double err = 0;
     int rc = opsec_pair_integral(ctx, c1, c2, f, sv, q, Q, &err);
     g_gm_trace = nullptr;
-    hipMemcpy(trace, dtr, 300 * 8, hipMemcpyDeviceToHost);
+    stream_copy(ctx, trace, dtr, 300 * 8, hipMemcpyDeviceToHost);
     hipFree(dtr);
     return rc;
 }

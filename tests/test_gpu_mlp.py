@@ -72,6 +72,17 @@ def _small_box(ctx, n=12, nb=6):
     return cells, lo, hi
 
 
+def _far_keys(n, near):
+    """Keys served by the surrogate: minimum-image separation >= near cells and
+    no axis whose cube-pair kernel crosses half the periodic box."""
+    k = np.arange(n ** 3)
+    d = np.stack([k // (n * n), (k // n) % n, k % n])
+    d = np.minimum(d, n - d)
+    D = np.sqrt((d ** 2).sum(0))
+    wrap = (2 * d + 2 > n).any(0)
+    return (D >= near) & ~wrap
+
+
 def test_training_and_ablation(ctx):
     """Train on seeded exact integrals; the surrogate then serves the far
     pairs: fewer quadrature node evaluations, bounded error in Q and in F."""
@@ -80,14 +91,14 @@ def test_training_and_ablation(ctx):
     q = ctx.quad(lib.OPSEC_QUAD_GAUSS, nodes=8, near=1)
     opts = ctx.mlp_opts(nsamples=60000, epochs=30, batch=2048, seed=7, lr=2e-3, near=2.0)
     loss = ctx.mlp_train(opts, q)
-    print("final training loss", loss)
-    assert loss < 2e-3
+    print("final training loss (residual / per-band RMS)", loss)
+    assert loss < 5e-2
     funcs = list(range(1, nb + 1))
     # quadrature-only tables
     ctx.pair_tables(funcs, q)
     st_q = ctx.pair_stats()
     Qq = np.array([ctx.pair_table(f) for f in funcs])
-    qs = ctx.quad(lib.OPSEC_QUAD_SURROGATE, nodes=8, spot_frac=0.01, spot_tol=1e-2, spot_seed=11)
+    qs = ctx.quad(lib.OPSEC_QUAD_SURROGATE, nodes=8, spot_frac=0.01, spot_tol=1e-3, spot_seed=11)
     qs.near = 2
     ctx.pair_tables(funcs, qs)
     st_s = ctx.pair_stats()
@@ -96,9 +107,12 @@ def test_training_and_ablation(ctx):
     err = np.abs(Qs - Qq).max(axis=1) / scale
     print("max |Q_surrogate - Q_quad| / xi_p(0) per band", err)
     print("quadrature-only stats", st_q, "surrogate stats", st_s)
-    assert st_s["surrogate"] > 0.9 * nb * n ** 3 * 0.9
-    assert st_s["nodes"] < 0.2 * st_q["nodes"]
-    assert err.max() < 5e-2
+    nfar = int(_far_keys(n, 2.0).sum())
+    assert st_s["surrogate"] >= 0.95 * nb * nfar
+    print("node evaluations with / without the surrogate", st_s["nodes"] / st_q["nodes"])
+    assert st_s["nodes"] < 0.6 * st_q["nodes"]
+    assert st_s["spot_fail"] <= 0.1 * max(1, st_s["spot"])
+    assert err.max() < 1e-3
     # Fisher matrices through the same modes with and without the surrogate
     N = len(cells)
     m = 120
@@ -113,6 +127,6 @@ def test_training_and_ablation(ctx):
     ctx.pair_tables(funcs, qs)
     ctx.project(funcs, lib.OPSEC_PREC_FP64, keep_on_device=True)
     Fs = ctx.estimate_device(lam, y, prior)["F"]
-    rel = np.abs(np.diag(Fs) - np.diag(Fq)) / np.abs(np.diag(Fq))
-    print("Fisher diagonal relative change with the surrogate", rel)
-    assert rel.max() < 5e-2
+    rel = np.abs(Fs - Fq).max() / np.abs(Fq).max()
+    print("Fisher max |dF| / max |F| with the surrogate", rel)
+    assert rel < 1e-3

@@ -94,6 +94,54 @@ def run_path(ctx, L, funcs, quad, prec, trace_prec, lam, y, prior, steps, warmup
     return 1e3 * float(np.mean(times)), est, np.mean(np.array(kms), axis=0)
 
 
+def compare_estimates(e64, e, info_floor=1e-8):
+    """Error of an estimate against the FP64 one.
+
+    F: entry-wise in correlation units |dF_pq| / sqrt(F_pp F_qq), and on its
+    diagonal.  Band powers: with 4000 KL modes the highest bands are almost
+    unconstrained (F is numerically singular), so p = F^-1 (q - f) is compared
+    on the constrained subspace -- the eigen-directions of the FP64 F holding
+    at least info_floor of the largest eigenvalue -- through the pseudo-inverse
+    F+ there: dp = F+ d(q - f) in units of sigma_p = sqrt(F+_pp), and as the
+    whitened shift w^-1/2 V^T d(q - f) (one unit = one sigma along an
+    independent band combination)."""
+    F64, F = e64["F"], e["F"]
+    d = np.sqrt(np.abs(np.diag(F64)))
+    w, V = np.linalg.eigh(F64)
+    keep = w >= info_floor * w.max()
+    Vc, wc = V[:, keep], w[keep]
+    r = (e["q"] - e["f"]) - (e64["q"] - e64["f"])
+    z = (Vc.T @ r) / np.sqrt(wc)
+    Fp = (Vc / wc) @ Vc.T
+    dp = Fp @ r
+    sig = np.sqrt(np.diag(Fp))
+    return {
+        "fisher_max_rel_err_diag": float(np.max(np.abs(np.diag(F) - np.diag(F64)) / np.abs(np.diag(F64)))),
+        "fisher_max_err_corr_units": float(np.max(np.abs(F - F64) / np.outer(d, d))),
+        "fisher_max_abs_err_over_max": float(np.max(np.abs(F - F64)) / np.max(np.abs(F64))),
+        "fisher_condition_number": float(w.max() / max(abs(w.min()), 1e-300)),
+        "constrained_directions": int(keep.sum()),
+        "band_power_max_shift_sigma": float(np.max(np.abs(dp) / sig)),
+        "band_power_max_shift_sigma_whitened": float(np.max(np.abs(z))),
+    }
+
+
+def diagnose(ctx, L, funcs, q64, qs, lam, y, prior, e64):
+    """Split the error: quadrature tables with the mixed-precision projection
+    and traces, and surrogate tables with the FP64 projection and traces."""
+    out = {}
+    for name, quad, prec, tprec in (("mixed_precision_only", q64, L.OPSEC_PREC_BF16X3, L.OPSEC_PREC_FP32),
+                                    ("surrogate_only", qs, L.OPSEC_PREC_FP64, L.OPSEC_PREC_FP64)):
+        ctx.pair_tables(funcs, quad)
+        ctx.project(funcs[1:], prec, keep_on_device=True)
+        e = ctx.estimate_device(lam, y, prior, L.OPSEC_MIX_INVERSE, tprec)
+        c = compare_estimates(e64, e)
+        c["fisher_rel_err_diag_per_band"] = [float(x) for x in
+                                             np.abs(np.diag(e["F"]) - np.diag(e64["F"])) / np.abs(np.diag(e64["F"]))]
+        out[name] = c
+    return out
+
+
 def cpu_baseline(cfg, cells, k, P, lo, hi, B, budget_s=20.0):
     """Oracle (FP64 C restatement, OpenMP) on a bounded sample of the same
     workload, extrapolated to the full S->Fisher pass."""
@@ -143,6 +191,14 @@ def main():
     ap.add_argument("--config", default="C2", choices=["C1", "C2"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--path", default="both", choices=["both", "fp64", "surrogate"])
+    ap.add_argument("--mlp-samples", type=int, default=200000, help="surrogate training pairs (setup)")
+    ap.add_argument("--mlp-epochs", type=int, default=40, help="surrogate training epochs (setup)")
+    ap.add_argument("--diagnose", action="store_true",
+                    help="also attribute the Fisher error to the surrogate and to the mixed precision (untimed)")
+    ap.add_argument("--nt-tile", type=int, default=1, choices=[0, 1],
+                    help="split-BF16x3 GEMM tile: 0 = 128 x 128, 1 = 256 x 128")
+    ap.add_argument("--copy-gib", type=float, default=0.0,
+                    help="also time a plain device copy of this many GiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU band sharding "
@@ -162,6 +218,7 @@ def main():
     funcs = list(range(nb + 1))
 
     ctx = L.Context(0)
+    ctx.lib.opsec_internal_set_nt_tile(args.nt_tile)
     ncu, arch = ctx.device_info()
     log(f"device {arch} with {ncu} CUs; workload {args.config}: {N} cells, {nb} bands, {M} modes")
     ctx.model_real(k, P, lo, hi, L.OPSEC_XI_CLOSED)
@@ -183,7 +240,7 @@ def main():
     t0 = time.perf_counter()
     qs = ctx.quad(L.OPSEC_QUAD_SURROGATE, nodes=cfg["nodes"])
     qs.near = 2
-    opts = ctx.mlp_opts(nsamples=200000, epochs=40, batch=4096, seed=7, lr=2e-3, near=2.0)
+    opts = ctx.mlp_opts(nsamples=args.mlp_samples, epochs=args.mlp_epochs, batch=4096, seed=7, lr=2e-3, near=2.0)
     loss = ctx.mlp_train(opts, q64)
     t_train = time.perf_counter() - t0
     log(f"surrogate trained on {opts.nsamples} exact integrals in {t_train:.1f} s, final loss {loss:.3e}")
@@ -227,17 +284,14 @@ def main():
                             "projection_ms": [float(k64[0]), float(k64[1])],
                             "stage1_tflops": 2.0 * N * N * M * nb / (float(k64[0]) / 1e3) / 1e12}
     if "fp64" in results and "surrogate" in results:
-        F64, Fs = results["fp64"]["est"]["F"], results["surrogate"]["est"]["F"]
-        d64 = np.diag(F64)
         out["speedup_vs_fp64"] = results["fp64"]["ms"] / results["surrogate"]["ms"]
-        out["fisher_max_rel_err_diag"] = float(np.max(np.abs(np.diag(Fs) - d64) / np.abs(d64)))
-        out["fisher_max_abs_err_over_max"] = float(np.max(np.abs(Fs - F64)) / np.max(np.abs(F64)))
-        sig = np.sqrt(np.diag(results["fp64"]["est"]["cov"]))
-        out["band_power_max_shift_sigma"] = float(np.max(np.abs(results["surrogate"]["est"]["phat"] -
-                                                                results["fp64"]["est"]["phat"]) / sig))
+        out.update(compare_estimates(results["fp64"]["est"], results["surrogate"]["est"]))
         st = results["surrogate"]["stats"]
         out["surrogate_stats"] = {k2: int(v) for k2, v in st.items()}
         out["surrogate_train_s"] = t_train
+        out["surrogate_train_loss"] = loss
+        if args.diagnose:
+            out["error_sources"] = diagnose(ctx, L, funcs, q64, qs, lam, y, prior, results["fp64"]["est"])
     # roofline of the dominant kernel: the stage-1 projection GEMM C_p = S_p B'
     # (useful flops 2 N^2 M per band), average launch time from HIP events on
     # the context's stream
@@ -260,6 +314,20 @@ def main():
                            "frac": achieved / PEAK_TF["fp64"], "traffic": None,
                            "kernel": "gemm_kernel<double> (projection stage 1, all bands in one launch)",
                            "launch_ms": s1_ms}
+    if args.copy_gib > 0:
+        import ctypes as C
+        nbytes = int(args.copy_gib * (1 << 30)) // 16 * 16
+        src, dst = ctx.dev_alloc(nbytes), ctx.dev_alloc(nbytes)
+        ms = C.c_float()
+        best = None
+        for _ in range(3):
+            ctx.check(ctx.lib.opsec_internal_copy(ctx.h, C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes),
+                                                  C.byref(ms)), "copy")
+            best = ms.value if best is None else min(best, ms.value)
+        out["copy_calibration"] = {"bytes": nbytes, "ms": best, "GB_per_s": 2 * nbytes / (best / 1e3) / 1e9,
+                                   "kernel": "copy16_kernel"}
+        ctx.dev_free(src)
+        ctx.dev_free(dst)
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(cfg, cells, k, P, lo, hi, B)

@@ -317,6 +317,13 @@ static int upload_desc(opsec_ctx* ctx) {
 
 using namespace opsec;
 
+namespace opsec {
+__global__ __launch_bounds__(256) void copy16_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+}  // namespace opsec
+
 extern "C" {
 
 int opsec_abi_version(void) { return OPSEC_HIP_ABI_VERSION; }
@@ -398,6 +405,29 @@ int opsec_dev_free(opsec_ctx* ctx, void* dptr) {
     ctx->user_bufs.erase(it);
     OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     OPSEC_HIP_TRY(ctx, hipFree(dptr));
+    return OPSEC_OK;
+}
+
+// Plain 16-byte-per-lane copy of `bytes` (a multiple of 16) between device
+// buffers on the context's stream, timed with HIP events: the reference
+// kernel whose FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic
+// figures of the other kernels.
+int opsec_internal_copy(opsec_ctx* ctx, void* dst, const void* src, size_t bytes, float* ms) {
+    if (!ctx || !dst || !src || bytes % 16) return OPSEC_EINVAL;
+    hipEvent_t e0, e1;
+    OPSEC_HIP_TRY(ctx, hipEventCreate(&e0));
+    OPSEC_HIP_TRY(ctx, hipEventCreate(&e1));
+    const size_t n = bytes / 16;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, (size_t)ctx->num_cus * 32);
+    hipEventRecord(e0, ctx->stream);
+    hipLaunchKernelGGL(opsec::copy16_kernel, dim3(blocks), dim3(256), 0, ctx->stream, (const uint4*)src, (uint4*)dst, n);
+    hipEventRecord(e1, ctx->stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess && ms) e = hipEventElapsedTime(ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (e != hipSuccess) return set_error(ctx, OPSEC_EHIP, "copy: %s", hipGetErrorString(e));
     return OPSEC_OK;
 }
 

@@ -643,7 +643,7 @@ static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double
     nev = std::min(nev, N);
     if (b <= 0) b = std::min(128, std::max(1, N / 4));
     b = std::max(1, std::min(b, N));
-    if (maxdim <= 0) maxdim = std::max(3 * nev + 4 * b, 8 * b);
+    if (maxdim <= 0) maxdim = std::max(3 * nev + 4 * b, 16 * b);
     if (maxdim >= N) {
         // the whole space fits: use a block size that divides N
         while (N % b) b--;

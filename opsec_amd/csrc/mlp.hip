@@ -7,16 +7,18 @@
 // layer, W (out x in, row-major) then b (out).
 //
 // What the network learns.  For a box pair at minimum-image offset d (cells,
-// cell width a, |d| = D) and a band [klo, khi], the pair integral is the
-// band's xi averaged over the tent kernel of the cube pair.  In Fourier space
-// that kernel is prod_i sinc^2(k_i a / 2); replacing it by its average over
-// directions, Wbar(k a), gives the isotropic baseline
-//   base(D) = 1/(2 pi^2) int_klo^khi k^2 j0(k D a) Wbar(k a) dk
-// (16 Gauss-Legendre nodes per band; Wbar by 48 x 48-point quadrature over an
-// octant of the sphere).  The network predicts the rest -- the anisotropic
-// part of the kernel and the periodic wrap near half the box -- scaled to
-// O(1):  Q = base + s r_p y,  s = (khi^3 - klo^3) / (6 pi^2) = xi_band(0),
-// r_p = RMS of (Q - base) / s over the band's training pairs.
+// cell width a, |d| = D, direction u = d / D) and a band [klo, khi], the pair
+// integral is the band's xi averaged over the tent kernel of the cube pair:
+//   Q = (2 pi)^-3 int_band d^3k e^{i k.d a} T(k),  T(k) = prod_i sinc^2(k_i a/2).
+// T is cubic-invariant, T(k n) = sum_l c_l(k a) K_l(n) over the cubic
+// harmonics (l = 0, 4, 6, 8, 10, ...), and the Rayleigh expansion of the
+// plane wave turns each term into a radial integral:
+//   Q = 1/(2 pi^2) sum_l i^l K_l(u) int_klo^khi k^2 j_l(k D a) c_l(k a) dk.
+// The analytic baseline keeps l <= 8 (16 Gauss-Legendre nodes per band; c_l by
+// 64 x 64-point quadrature over an octant of the sphere).  The network
+// predicts the rest (l >= 10) scaled to O(1):  Q = base + s r_p y,
+// s = (khi^3 - klo^3) / (6 pi^2) = xi_band(0), r_p = RMS of (Q - base) / s
+// over the band's training pairs.
 // Features:
 //   x0..x2  sorted |d_i| / (n_i / 2)
 //   x3, x4  klo a / pi, khi a / pi
@@ -35,6 +37,7 @@
 
 #include "common.h"
 #include "gemm.h"
+#include "sur_math.h"
 
 namespace opsec {
 
@@ -61,7 +64,7 @@ struct MlpState {
     float* delta2 = nullptr;  // B x 128
     float* out = nullptr;     // B
     int cap_batch = 0;
-    // isotropic baseline per function index (band functions only)
+    // analytic baseline per function index (band functions only)
     std::vector<struct SurBand> bands;
     struct SurBand* d_bands = nullptr;
     size_t cap_bands = 0;
@@ -71,14 +74,16 @@ constexpr int SUR_NK = 16;
 struct SurBand {
     double klo = 0, khi = 0, sc = 0, rs = 1;
     double cell = 0;            // cell width the nodes were made for
-    double kk[SUR_NK], wk[SUR_NK];
+    double kk[SUR_NK];
+    double wk[SUR_NL][SUR_NK];  // i^l w_n k_n^2 c_l(k_n a) / (2 pi^2)
 };
 
-// Box geometry for the device-side features.
+// Box geometry for the device-side features, and the cubic harmonics.
 struct SurGeo {
     int n1, n2, n3;
     int per1, per2, per3;       // minimum image along the axis
     double a;                   // cell width
+    SurHarm K;
 };
 
 __device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
@@ -381,6 +386,8 @@ static bool wraps(const SurveyDev& sv, int kx, int ky, int kz) {
     return false;
 }
 
+static const SurHarm& cubic_harmonics();
+
 static SurGeo sur_geo(const SurveyDev& sv) {
     SurGeo g;
     g.n1 = sv.n1;
@@ -390,34 +397,92 @@ static SurGeo sur_geo(const SurveyDev& sv) {
     g.per2 = axis_periodic(sv.L, sv.max2 - sv.min2);
     g.per3 = axis_periodic(sv.L, sv.max3 - sv.min3);
     g.a = (sv.max1 - sv.min1) / sv.n1;
+    g.K = cubic_harmonics();
     return g;
 }
 
-// Direction average of prod_i sinc^2(x n_i / 2) over the unit sphere (by
-// symmetry over one octant: mu = cos(theta) in [0, 1], phi in [0, pi/2]).
-static double cube_kernel_avg(double x) {
-    static std::vector<double> t48, w48;
-    if (t48.empty()) {   // nodes on [-1, 1]
-        t48.resize(48);
-        w48.resize(48);
-        gauss_legendre(48, t48.data(), w48.data());
-    }
-    auto sinc2 = [](double u) {
-        if (std::fabs(u) < 1e-4) return 1.0 - u * u / 3.0;
-        double s = std::sin(u) / u;
-        return s * s;
-    };
-    double acc = 0;
-    for (int i = 0; i < 48; i++) {
-        const double mu = 0.5 * (t48[i] + 1), wm = 0.5 * w48[i];
-        const double st = std::sqrt(std::max(0.0, 1 - mu * mu));
-        for (int j = 0; j < 48; j++) {
-            const double phi = 0.25 * M_PI * (t48[j] + 1), wp = 0.25 * M_PI * w48[j];
-            const double n1 = st * std::cos(phi), n2 = st * std::sin(phi), n3 = mu;
-            acc += wm * wp * sinc2(0.5 * x * n1) * sinc2(0.5 * x * n2) * sinc2(0.5 * x * n3);
+// Octant quadrature (64 x 64 Gauss-Legendre in mu = cos(theta) and phi) for
+// integrals of cubic-invariant functions over the sphere.
+struct Octant {
+    std::vector<double> x, y, z, w;
+};
+static const Octant& octant() {
+    static Octant o;
+    if (o.w.empty()) {
+        const int n = 64;
+        std::vector<double> t(n), wt(n);
+        gauss_legendre(n, t.data(), wt.data());
+        for (int i = 0; i < n; i++) {
+            const double mu = 0.5 * (t[i] + 1), st = std::sqrt(std::max(0.0, 1 - mu * mu));
+            for (int j = 0; j < n; j++) {
+                const double phi = 0.25 * M_PI * (t[j] + 1);
+                o.x.push_back(st * std::cos(phi));
+                o.y.push_back(st * std::sin(phi));
+                o.z.push_back(mu);
+                o.w.push_back(0.5 * wt[i] * 0.25 * M_PI * wt[j]);
+            }
         }
     }
-    return acc / (0.5 * M_PI);
+    return o;
+}
+
+// Cubic harmonics K_0, K_4, K_6, K_8 as combinations of (1, e2, e3, e2^2):
+// Gram-Schmidt of that sequence over the sphere.
+static const SurHarm& cubic_harmonics() {
+    static SurHarm H;
+    static bool done = false;
+    if (!done) {
+        const Octant& o = octant();
+        const size_t n = o.w.size();
+        std::vector<std::vector<double>> K;
+        for (int l = 0; l < SUR_NL; l++) {
+            double c[4] = {0, 0, 0, 0};
+            c[l] = 1.0;
+            std::vector<double> f(n);
+            for (size_t p = 0; p < n; p++) {
+                double m[4];
+                sur_invariants(o.x[p], o.y[p], o.z[p], m);
+                f[p] = m[l];
+            }
+            for (int q = 0; q < l; q++) {
+                double num = 0, den = 0;
+                for (size_t p = 0; p < n; p++) {
+                    num += o.w[p] * f[p] * K[q][p];
+                    den += o.w[p] * K[q][p] * K[q][p];
+                }
+                const double pr = num / den;
+                for (size_t p = 0; p < n; p++) f[p] -= pr * K[q][p];
+                for (int i = 0; i < 4; i++) c[i] -= pr * H.c[q][i];
+            }
+            for (int i = 0; i < 4; i++) H.c[l][i] = c[i];
+            K.push_back(f);
+        }
+        done = true;
+    }
+    return H;
+}
+
+// c_l(x) = <T, K_l> / <K_l, K_l> for T(n) = prod_i sinc^2(x n_i / 2).
+static void kernel_harmonics(double x, double out[SUR_NL]) {
+    const Octant& o = octant();
+    const SurHarm& H = cubic_harmonics();
+    auto sinc2 = [](double u) {
+        if (std::fabs(u) < 1e-4) return 1.0 - u * u / 3.0;
+        const double s = std::sin(u) / u;
+        return s * s;
+    };
+    double num[SUR_NL] = {0, 0, 0, 0}, den[SUR_NL] = {0, 0, 0, 0};
+    for (size_t p = 0; p < o.w.size(); p++) {
+        const double T = sinc2(0.5 * x * o.x[p]) * sinc2(0.5 * x * o.y[p]) * sinc2(0.5 * x * o.z[p]);
+        double m[4];
+        sur_invariants(o.x[p], o.y[p], o.z[p], m);
+        for (int l = 0; l < SUR_NL; l++) {
+            const double K = H.c[l][0] * m[0] + H.c[l][1] * m[1] + H.c[l][2] * m[2] + H.c[l][3] * m[3];
+            num[l] += o.w[p] * T * K;
+            den[l] += o.w[p] * K * K;
+        }
+    }
+    for (int l = 0; l < SUR_NL; l++) out[l] = num[l] / den[l];
 }
 
 // (Re)build the baseline nodes of every band function for cell width a and
@@ -439,10 +504,14 @@ static int prepare_bands(opsec_ctx* ctx, double a) {
         B.khi = d.b;
         B.cell = a;
         B.sc = band_scale(d.a, d.b);
+        const double il[SUR_NL] = {1.0, 1.0, -1.0, 1.0};   // i^l for l = 0, 4, 6, 8
         for (int n = 0; n < SUR_NK; n++) {
             const double k = 0.5 * (d.a + d.b) + 0.5 * (d.b - d.a) * t[n];
+            double c[SUR_NL];
+            kernel_harmonics(k * a, c);
             B.kk[n] = k;
-            B.wk[n] = 0.5 * (d.b - d.a) * w[n] * k * k * cube_kernel_avg(k * a) / (2 * M_PI * M_PI);
+            for (int l = 0; l < SUR_NL; l++)
+                B.wk[l][n] = il[l] * 0.5 * (d.b - d.a) * w[n] * k * k * c[l] / (2 * M_PI * M_PI);
         }
         changed = true;
     }
@@ -486,11 +555,14 @@ __global__ __launch_bounds__(256) void sur_rows_kernel(const int* __restrict__ k
     x[6] = (float)((B.klo + B.khi) * D * g.a / (16 * M_PI));
     x[7] = (float)((B.khi - B.klo) * D * g.a / (2 * M_PI));
     const double r = D * g.a;
+    double m[4], K[SUR_NL];
+    sur_invariants(dx / D, dy / D, dz / D, m);
+    for (int l = 0; l < SUR_NL; l++) K[l] = g.K.c[l][0] * m[0] + g.K.c[l][1] * m[1] + g.K.c[l][2] * m[2] + g.K.c[l][3] * m[3];
     double acc = 0;
     for (int n = 0; n < SUR_NK; n++) {
-        const double u = B.kk[n] * r;
-        const double j0 = (u < 1e-4) ? 1.0 - u * u / 6.0 : sin(u) / u;
-        acc += B.wk[n] * j0;
+        double j[SUR_NL];
+        sph_bessel_0468(B.kk[n] * r, j);
+        for (int l = 0; l < SUR_NL; l++) acc += K[l] * B.wk[l][n] * j[l];
     }
     base[i] = acc;
 }

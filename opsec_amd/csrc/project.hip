@@ -220,7 +220,7 @@ struct GB_ColMajorT {
 // ---- split-BF16x3 on pre-split planes --------------------------------------------
 //
 // A plane pair (hi, lo) holds X = hi + lo for a rows x ld row-major matrix in
-// bf16 with k contiguous; ld and the row count are multiples of 128 and the
+// bf16 with k contiguous; ld and the row count are multiples of 256 and the
 // padding is zero, so the GEMM needs no bounds checks.
 
 constexpr int PT = 128;      // tile edge
@@ -343,92 +343,101 @@ struct EpiPacked {
     static constexpr bool upper = true;
 };
 
-// D = A B^T on plane pairs: A (nmb*128 rows) and B (nnb*128 rows), both with
-// K (multiple of 32) contiguous.  128 x 128 tile per workgroup, 4 waves of
-// 64 x 64, v_mfma_f32_16x16x32_bf16 with FP32 accumulation.
-template <typename Epi>
-__global__ __launch_bounds__(256) void nt_bf16x3_kernel(const __bf16* __restrict__ Ah, const __bf16* __restrict__ Al,
+// D = A B^T on plane pairs: A (nmb*BM rows) and B (nnb*BN rows), both with
+// K (multiple of 32) contiguous.  4 waves in a 2 x 2 arrangement, each owning
+// FM x FN fragments of 16 x 16 (BM = 32 FM, BN = 32 FN), on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation.  Operand tiles are staged
+// through LDS ([row][k], 80-byte rows) with the next tile's global loads
+// issued before the current tile's MFMAs.
+template <typename Epi, int FM, int FN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void nt_bf16x3_kernel(const __bf16* __restrict__ Ah, const __bf16* __restrict__ Al,
                                                         size_t lda, const __bf16* __restrict__ Bh,
                                                         const __bf16* __restrict__ Bl, size_t ldb, int K, int nmb,
                                                         int nnb, Epi E) {
-    __shared__ __attribute__((aligned(16))) __bf16 sAh[PT][PLDS];
-    __shared__ __attribute__((aligned(16))) __bf16 sAl[PT][PLDS];
-    __shared__ __attribute__((aligned(16))) __bf16 sBh[PT][PLDS];
-    __shared__ __attribute__((aligned(16))) __bf16 sBl[PT][PLDS];
+    constexpr int BM = 32 * FM, BN = 32 * FN;
+    constexpr int CA = BM * 4 / 256, CB = BN * 4 / 256;   // 16-byte chunks per thread and plane
+    __shared__ __attribute__((aligned(16))) __bf16 sAh[BM][PLDS];
+    __shared__ __attribute__((aligned(16))) __bf16 sAl[BM][PLDS];
+    __shared__ __attribute__((aligned(16))) __bf16 sBh[BN][PLDS];
+    __shared__ __attribute__((aligned(16))) __bf16 sBl[BN][PLDS];
     int mb, nbk;
     tile_of(blockIdx.x, nmb, nnb, mb, nbk);
-    const int m0 = mb * PT, n0 = nbk * PT;
-    if (Epi::upper && m0 > n0 + PT - 1) return;
+    const int m0 = mb * BM, n0 = nbk * BN;
+    if (Epi::upper && m0 > n0 + BN - 1) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-    // this thread's two 16-byte chunks of each 128 x 32 plane tile
-    const int r0 = tid >> 2, r1 = r0 + 64, kc = (tid & 3) * 8;
-    const __bf16* pAh0 = Ah + (size_t)(m0 + r0) * lda + kc;
-    const __bf16* pAh1 = Ah + (size_t)(m0 + r1) * lda + kc;
-    const __bf16* pAl0 = Al + (size_t)(m0 + r0) * lda + kc;
-    const __bf16* pAl1 = Al + (size_t)(m0 + r1) * lda + kc;
-    const __bf16* pBh0 = Bh + (size_t)(n0 + r0) * ldb + kc;
-    const __bf16* pBh1 = Bh + (size_t)(n0 + r1) * ldb + kc;
-    const __bf16* pBl0 = Bl + (size_t)(n0 + r0) * ldb + kc;
-    const __bf16* pBl1 = Bl + (size_t)(n0 + r1) * ldb + kc;
-    f4_t acc[4][4];
+    const int wm = (wave >> 1) * (16 * FM), wn = (wave & 1) * (16 * FN);
+    // chunk c of a plane tile: row c / 4, k offset (c % 4) * 8
+    const int kc = (tid & 3) * 8, r0 = tid >> 2;
+    const __bf16* pA = Ah + (size_t)(m0 + r0) * lda + kc;
+    const __bf16* pAl = Al + (size_t)(m0 + r0) * lda + kc;
+    const __bf16* pB = Bh + (size_t)(n0 + r0) * ldb + kc;
+    const __bf16* pBl = Bl + (size_t)(n0 + r0) * ldb + kc;
+    const size_t sa = (size_t)64 * lda, sb = (size_t)64 * ldb;   // 64 rows between a thread's chunks
+    f4_t acc[FM][FN];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < FM; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = f4_t{0, 0, 0, 0};
-    bf8_t g[8];
-    g[0] = *(const bf8_t*)pAh0;
-    g[1] = *(const bf8_t*)pAh1;
-    g[2] = *(const bf8_t*)pAl0;
-    g[3] = *(const bf8_t*)pAl1;
-    g[4] = *(const bf8_t*)pBh0;
-    g[5] = *(const bf8_t*)pBh1;
-    g[6] = *(const bf8_t*)pBl0;
-    g[7] = *(const bf8_t*)pBl1;
+        for (int j = 0; j < FN; j++) acc[i][j] = f4_t{0, 0, 0, 0};
+    bf8_t ga[2 * CA], gb[2 * CB];
+#pragma unroll
+    for (int c = 0; c < CA; c++) {
+        ga[c] = *(const bf8_t*)(pA + c * sa);
+        ga[CA + c] = *(const bf8_t*)(pAl + c * sa);
+    }
+#pragma unroll
+    for (int c = 0; c < CB; c++) {
+        gb[c] = *(const bf8_t*)(pB + c * sb);
+        gb[CB + c] = *(const bf8_t*)(pBl + c * sb);
+    }
     const int kq = 8 * (lane >> 4), fr = lane & 15;
     for (int k0 = 0; k0 < K; k0 += PK) {
         __syncthreads();
-        *(bf8_t*)&sAh[r0][kc] = g[0];
-        *(bf8_t*)&sAh[r1][kc] = g[1];
-        *(bf8_t*)&sAl[r0][kc] = g[2];
-        *(bf8_t*)&sAl[r1][kc] = g[3];
-        *(bf8_t*)&sBh[r0][kc] = g[4];
-        *(bf8_t*)&sBh[r1][kc] = g[5];
-        *(bf8_t*)&sBl[r0][kc] = g[6];
-        *(bf8_t*)&sBl[r1][kc] = g[7];
+#pragma unroll
+        for (int c = 0; c < CA; c++) {
+            *(bf8_t*)&sAh[r0 + 64 * c][kc] = ga[c];
+            *(bf8_t*)&sAl[r0 + 64 * c][kc] = ga[CA + c];
+        }
+#pragma unroll
+        for (int c = 0; c < CB; c++) {
+            *(bf8_t*)&sBh[r0 + 64 * c][kc] = gb[c];
+            *(bf8_t*)&sBl[r0 + 64 * c][kc] = gb[CB + c];
+        }
         __syncthreads();
         if (k0 + PK < K) {
             const int o = k0 + PK;
-            g[0] = *(const bf8_t*)(pAh0 + o);
-            g[1] = *(const bf8_t*)(pAh1 + o);
-            g[2] = *(const bf8_t*)(pAl0 + o);
-            g[3] = *(const bf8_t*)(pAl1 + o);
-            g[4] = *(const bf8_t*)(pBh0 + o);
-            g[5] = *(const bf8_t*)(pBh1 + o);
-            g[6] = *(const bf8_t*)(pBl0 + o);
-            g[7] = *(const bf8_t*)(pBl1 + o);
-        }
-        bf8_t ah[4], al[4], bh[4], bl[4];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            ah[i] = *(const bf8_t*)&sAh[wm + i * 16 + fr][kq];
-            al[i] = *(const bf8_t*)&sAl[wm + i * 16 + fr][kq];
-            bh[i] = *(const bf8_t*)&sBh[wn + i * 16 + fr][kq];
-            bl[i] = *(const bf8_t*)&sBl[wn + i * 16 + fr][kq];
-        }
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+            for (int c = 0; c < CA; c++) {
+                ga[c] = *(const bf8_t*)(pA + c * sa + o);
+                ga[CA + c] = *(const bf8_t*)(pAl + c * sa + o);
             }
+#pragma unroll
+            for (int c = 0; c < CB; c++) {
+                gb[c] = *(const bf8_t*)(pB + c * sb + o);
+                gb[CB + c] = *(const bf8_t*)(pBl + c * sb + o);
+            }
+        }
+        bf8_t bh[FN], bl[FN];
+#pragma unroll
+        for (int j = 0; j < FN; j++) {
+            bh[j] = *(const bf8_t*)&sBh[wn + j * 16 + fr][kq];
+            bl[j] = *(const bf8_t*)&sBl[wn + j * 16 + fr][kq];
+        }
+#pragma unroll
+        for (int i = 0; i < FM; i++) {
+            const bf8_t ah = *(const bf8_t*)&sAh[wm + i * 16 + fr][kq];
+            const bf8_t al = *(const bf8_t*)&sAl[wm + i * 16 + fr][kq];
+#pragma unroll
+            for (int j = 0; j < FN; j++) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
+            }
+        }
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < FM; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++)
+        for (int j = 0; j < FN; j++)
             E.put4(m0 + wm + i * 16 + 4 * (lane >> 4), n0 + wn + j * 16 + fr, acc[i][j]);
 }
 
@@ -461,19 +470,29 @@ static int split_planes(opsec_ctx* ctx, const double* X, size_t ldx, int rows, i
     return OPSEC_OK;
 }
 
+// Tile shape of the split-BF16x3 GEMM: 0 = 128 x 128, 1 = 256 x 128 (default).
+static int g_nt_tile = 1;
+
 // C = A B^T (stage 1, epilogue to split planes) or P = A B^T (stage 2, packed).
+// Plane row counts are multiples of 256.
 template <typename Epi>
 static void launch_nt(opsec_ctx* ctx, const __bf16* A, int arows, const __bf16* B, int brows, size_t ld, const Epi& E) {
-    const int nmb = arows / PT, nnb = brows / PT;
-    hipLaunchKernelGGL(nt_bf16x3_kernel<Epi>, dim3(nmb * nnb), dim3(256), 0, ctx->stream, A, A + (size_t)arows * ld, ld,
-                       B, B + (size_t)brows * ld, ld, (int)ld, nmb, nnb, E);
+    if (g_nt_tile == 0) {
+        const int nmb = arows / 128, nnb = brows / 128;
+        hipLaunchKernelGGL((nt_bf16x3_kernel<Epi, 4, 4>), dim3(nmb * nnb), dim3(256), 0, ctx->stream, A,
+                           A + (size_t)arows * ld, ld, B, B + (size_t)brows * ld, ld, (int)ld, nmb, nnb, E);
+    } else {
+        const int nmb = arows / 256, nnb = brows / 128;
+        hipLaunchKernelGGL((nt_bf16x3_kernel<Epi, 8, 4>), dim3(nmb * nnb), dim3(256), 0, ctx->stream, A,
+                           A + (size_t)arows * ld, ld, B, B + (size_t)brows * ld, ld, (int)ld, nmb, nnb, E);
+    }
 }
 
 // Split-BF16x3 projection of the bands funcs[0..nb) into d_P, band by band.
 static int run_project_bf16x3(opsec_ctx* ctx, int nb, const int* funcs) {
     ProjState* ps = ctx->proj;
     const int N = ctx->ncells, M = ctx->nmodes;
-    const size_t Kp = round_up(N, PT), Mp = round_up(M, PT);
+    const size_t Kp = round_up(N, 256), Mp = round_up(M, 256);
     int rc;
     if ((rc = dev_realloc(ctx, (void**)&ps->d_Bp, &ps->cap_Bp, 2 * Mp * Kp * sizeof(__bf16)))) return rc;
     if ((rc = dev_realloc(ctx, (void**)&ps->d_Sp, &ps->cap_Sp, 2 * Kp * Kp * sizeof(__bf16)))) return rc;
@@ -666,6 +685,13 @@ int opsec_internal_project_times(opsec_ctx* ctx, float* s1, float* s2) {
     return OPSEC_OK;
 }
 
+// Tile shape of the split-BF16x3 GEMM (0: 128 x 128, 1: 256 x 128), for tuning.
+int opsec_internal_set_nt_tile(int v) {
+    if (v != 0 && v != 1) return OPSEC_EINVAL;
+    g_nt_tile = v;
+    return OPSEC_OK;
+}
+
 // Per-kernel times of the last split-BF16x3 projection, summed over its bands
 // (ms): S_p plane generation, stage-1 GEMM, stage-2 GEMM; *launches = bands.
 int opsec_internal_project_kernel_ms(opsec_ctx* ctx, float* gen, float* gemm1, float* gemm2, int* launches) {
@@ -719,7 +745,7 @@ int opsec_project_dense(opsec_ctx* ctx, int n, const double* S, int nmodes, cons
         hipLaunchKernelGGL((gemm_kernel<float, GA_ColMajorT, GB_ColMajorT<float>, PutPacked, true>), g2,
                            dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, 0, outP, M, M, N);
     } else {
-        const size_t Kp = round_up(N, PT), Mp = round_up(M, PT);
+        const size_t Kp = round_up(N, 256), Mp = round_up(M, 256);
         __bf16 *Sp = nullptr, *Bp = nullptr, *Cp = nullptr;
         OPSEC_HIP_TRY(ctx, hipMalloc(&Sp, 2 * Kp * Kp * sizeof(__bf16)));
         OPSEC_HIP_TRY(ctx, hipMalloc(&Bp, 2 * Mp * Kp * sizeof(__bf16)));

@@ -140,7 +140,8 @@ def load(path: str | None = None):
 
 
 def _ptr(a):
-    return None if a is None else a.ctypes.data
+    # data_as keeps a reference to the array for as long as the pointer lives
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
 def f64(a):

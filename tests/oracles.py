@@ -101,7 +101,9 @@ def ref():
 
 
 def ptr(a):
-    return None if a is None else a.ctypes.data
+    """Pointer to an array's data that keeps the array alive for the call
+    (numpy's data_as holds a reference), so temporaries are safe."""
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
 # ---- convenience wrappers -----------------------------------------------------

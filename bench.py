@@ -94,6 +94,15 @@ def run_path(ctx, L, funcs, quad, prec, trace_prec, lam, y, prior, steps, warmup
     return 1e3 * float(np.mean(times)), est, np.mean(np.array(kms), axis=0)
 
 
+def stage1_kernel_name(tile, gen):
+    if tile == 2:
+        return ("gemm3_kernel<ASrcTable,EpiPlanes> (stage 1, S_p from the pair table, one launch per band)" if gen
+                else "gemm3_kernel<ASrcPlanes,EpiPlanes> (stage 1, S_p planes, one launch per band)")
+    if gen:
+        return "proj_gen_kernel<8,4> (stage 1, S_p from the pair table, one launch per band)"
+    return "nt_bf16x3_kernel<EpiPlanes> (stage 1, S_p planes, one launch per band)"
+
+
 def compare_estimates(e64, e, info_floor=1e-8):
     """Error of an estimate against the FP64 one.
 
@@ -131,6 +140,8 @@ def diagnose(ctx, L, funcs, q64, qs, lam, y, prior, e64):
     and traces, and surrogate tables with the FP64 projection and traces."""
     out = {}
     for name, quad, prec, tprec in (("mixed_precision_only", q64, L.OPSEC_PREC_BF16X3, L.OPSEC_PREC_FP32),
+                                    ("bf16x3_projection_only", q64, L.OPSEC_PREC_BF16X3, L.OPSEC_PREC_FP64),
+                                    ("fp32_traces_only", q64, L.OPSEC_PREC_FP64, L.OPSEC_PREC_FP32),
                                     ("surrogate_only", qs, L.OPSEC_PREC_FP64, L.OPSEC_PREC_FP64)):
         ctx.pair_tables(funcs, quad)
         ctx.project(funcs[1:], prec, keep_on_device=True)
@@ -195,8 +206,12 @@ def main():
     ap.add_argument("--mlp-epochs", type=int, default=40, help="surrogate training epochs (setup)")
     ap.add_argument("--diagnose", action="store_true",
                     help="also attribute the Fisher error to the surrogate and to the mixed precision (untimed)")
-    ap.add_argument("--nt-tile", type=int, default=1, choices=[0, 1],
-                    help="split-BF16x3 GEMM tile: 0 = 128 x 128, 1 = 256 x 128")
+    ap.add_argument("--nt-tile", type=int, default=2, choices=[0, 1, 2],
+                    help="split-BF16x3 GEMM tile: 0 = 128 x 128, 1 = 256 x 128, 2 = 256 x 256 double-buffered")
+    ap.add_argument("--nt-gen", type=int, default=1, choices=[0, 1],
+                    help="stage 1 generates S_p from the pair table (1) or reads S_p planes (0)")
+    ap.add_argument("--nt-group", type=int, default=8, help="split-BF16x3 GEMM tile-group edge")
+    ap.add_argument("--nt-remap", type=int, default=0, choices=[0, 1], help="deal tile groups to XCDs")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also time a plain device copy of this many GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -219,6 +234,8 @@ def main():
 
     ctx = L.Context(0)
     ctx.lib.opsec_internal_set_nt_tile(args.nt_tile)
+    ctx.lib.opsec_internal_set_nt_order(args.nt_group, args.nt_remap)
+    ctx.lib.opsec_internal_set_nt_gen(args.nt_gen)
     ncu, arch = ctx.device_info()
     log(f"device {arch} with {ncu} CUs; workload {args.config}: {N} cells, {nb} bands, {M} modes")
     ctx.model_real(k, P, lo, hi, L.OPSEC_XI_CLOSED)
@@ -302,7 +319,7 @@ def main():
         peak = PEAK_TF["bf16"] / 3
         out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                            "frac": achieved / peak, "traffic": None,
-                           "kernel": "nt_bf16x3_kernel<EpiPlanes> (stage 1, one launch per band)",
+                           "kernel": stage1_kernel_name(args.nt_tile, args.nt_gen),
                            "launch_ms": per_launch,
                            "note": "useful flops 2 N^2 M per launch; peak = dense bf16 2516.6 TF / 3 products",
                            "mfma_utilisation": 3 * achieved / PEAK_TF["bf16"],

@@ -49,13 +49,20 @@ template <typename T>
 __global__ __launch_bounds__(256) void fisher_gram_kernel(const double* D, long long K, int nb, const double* lambda,
                                                           const double* y, const double* prior, double* G) {
     typedef typename FMfma<T>::acc_t acc_t;
+    // FP32 MFMA partial sums are closed at the end of every 128-entry tile and
+    // carried in FP64 registers, so each FP32 accumulation chain is 32 terms
+    constexpr bool fp32 = sizeof(T) == 4;
     __shared__ T X[FG][FK_TILE + 4];     // scaled rows
     __shared__ T Dt[FG][FK_TILE + 4];    // plain rows
     __shared__ double wk[FK_TILE], rk[FK_TILE], yk[FK_TILE];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     acc_t acc[3][3];
+    double acc64[3][3][4];
     for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) acc[i][j] = acc_t{0, 0, 0, 0};
+        for (int j = 0; j < 3; j++) {
+            acc[i][j] = acc_t{0, 0, 0, 0};
+            for (int r = 0; r < 4; r++) acc64[i][j][r] = 0.0;
+        }
     for (long long k0 = (long long)blockIdx.x * FK_TILE; k0 < K; k0 += (long long)gridDim.x * FK_TILE) {
         __syncthreads();
         // weights, R diagonal part and y_i y_j per packed entry
@@ -115,12 +122,19 @@ __global__ __launch_bounds__(256) void fisher_gram_kernel(const double* D, long 
             for (int i = 0; i < 3; i++)
                 for (int j = 0; j < 3; j++) acc[i][j] = FMfma<T>::mma(a[i], b[j], acc[i][j]);
         }
+        if (fp32)
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    for (int r = 0; r < 4; r++) acc64[i][j][r] += (double)acc[i][j][r];
+                    acc[i][j] = acc_t{0, 0, 0, 0};
+                }
     }
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++)
             for (int r = 0; r < 4; r++) {
                 int row = i * 16 + acc_row<T>(lane, r), col = j * 16 + (lane & 15);
-                atomicAdd(&G[row * FG + col], (double)acc[i][j][r]);
+                const double v = fp32 ? acc64[i][j][r] : (double)acc[i][j][r];
+                atomicAdd(&G[row * FG + col], v);
             }
 }
 

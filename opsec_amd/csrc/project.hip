@@ -30,7 +30,8 @@
 namespace opsec {
 
 struct ProjState {
-    double* d_Bw = nullptr;      // B' = diag(sqrt(Nbar)) B, ncells x nmodes column-major
+    double* d_Bw = nullptr;      // B' = diag(sqrt(Nbar)) B, ncells x nmodes column-major, pitch ldw
+    size_t ldw = 0;              // ncells + 32 (a power-of-two pitch aliases L2 sets and HBM channels)
     size_t cap_Bw = 0;
     void* d_C = nullptr;         // stage-1 products, nb x ncells x nmodes (double or float)
     size_t cap_C = 0;
@@ -48,6 +49,10 @@ struct ProjState {
     size_t cap_Sp = 0;
     __bf16* d_Cp = nullptr;
     size_t cap_Cp = 0;
+    unsigned* d_Qhl = nullptr;   // one band's pair table as (hi | lo << 16) words
+    size_t cap_Qhl = 0;
+    unsigned* d_cpk = nullptr;   // packed cell coordinates, padded to Kp
+    size_t cap_cpk = 0;
     // per-kernel sums over the bands of the last split-BF16x3 projection (ms)
     float last_ms_gen = 0, last_ms_gemm1 = 0, last_ms_gemm2 = 0;
     int last_launches = 0;
@@ -224,10 +229,21 @@ struct GB_ColMajorT {
 // padding is zero, so the GEMM needs no bounds checks.
 
 constexpr int PT = 128;      // tile edge
-constexpr int PK = 32;       // k per step
-constexpr int PLDS = PK + 8; // LDS row stride in bf16 (80 bytes: 16-byte aligned rows)
+constexpr int PK = 32;       // k per step: one 64-byte LDS row per tile row
+
+// LDS slot of 16-byte chunk c (0..3) of tile row r: chunks are XOR-swizzled by
+// ((r >> 1) ^ (r >> 2)) & 3, which makes both the staging writes (4 lanes per
+// row) and the fragment reads (16 rows, one chunk) free of bank conflicts for
+// 32 or 64 LDS banks.
+__device__ __forceinline__ int lds_slot(int r, int c) { return r * PK + ((c ^ (((r >> 1) ^ (r >> 2)) & 3)) << 3); }
 
 inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+// Row pitch of a plane (bf16): the K extent plus 256 bytes.  A power-of-two
+// pitch (64 KB at 32^3) maps every row of a tile onto the same L2 sets and
+// HBM channels; the offset spreads them (measured: stage 1 at 32^3 x 4000
+// modes, 26 ms -> 18.5 ms per band).  The padding columns are never read.
+inline size_t plane_pitch(size_t k) { return k + 128; }
 
 __device__ __forceinline__ void split2(float x, __bf16& hi, __bf16& lo) {
     hi = (__bf16)x;
@@ -297,10 +313,9 @@ __global__ __launch_bounds__(256) void gen_split_kernel(const double* __restrict
 // resident at one time share A and B slices in L2, and consecutive groups are
 // dealt to the eight XCDs (workgroups go to XCDs round-robin) so that each XCD
 // works through its own range of groups.
-__device__ __forceinline__ void tile_of(int id, int nmb, int nnb, int& mb, int& nbk) {
+__device__ __forceinline__ void tile_of(int id, int nmb, int nnb, int G, int remap, int& mb, int& nbk) {
     const int total = nmb * nnb;
-    if (total % 8 == 0) id = (id % 8) * (total / 8) + id / 8;
-    const int G = 8;
+    if (remap && total % 8 == 0) id = (id % 8) * (total / 8) + id / 8;
     const int group = id / (G * nnb);
     const int first = group * G;
     const int gs = min(nmb - first, G);
@@ -347,21 +362,21 @@ struct EpiPacked {
 // K (multiple of 32) contiguous.  4 waves in a 2 x 2 arrangement, each owning
 // FM x FN fragments of 16 x 16 (BM = 32 FM, BN = 32 FN), on
 // v_mfma_f32_16x16x32_bf16 with FP32 accumulation.  Operand tiles are staged
-// through LDS ([row][k], 80-byte rows) with the next tile's global loads
-// issued before the current tile's MFMAs.
+// through LDS (64-byte rows, swizzled chunks) with the next tile's global
+// loads issued before the current tile's MFMAs.
 template <typename Epi, int FM, int FN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void nt_bf16x3_kernel(const __bf16* __restrict__ Ah, const __bf16* __restrict__ Al,
                                                         size_t lda, const __bf16* __restrict__ Bh,
                                                         const __bf16* __restrict__ Bl, size_t ldb, int K, int nmb,
-                                                        int nnb, Epi E) {
+                                                        int nnb, int G, int remap, Epi E) {
     constexpr int BM = 32 * FM, BN = 32 * FN;
     constexpr int CA = BM * 4 / 256, CB = BN * 4 / 256;   // 16-byte chunks per thread and plane
-    __shared__ __attribute__((aligned(16))) __bf16 sAh[BM][PLDS];
-    __shared__ __attribute__((aligned(16))) __bf16 sAl[BM][PLDS];
-    __shared__ __attribute__((aligned(16))) __bf16 sBh[BN][PLDS];
-    __shared__ __attribute__((aligned(16))) __bf16 sBl[BN][PLDS];
+    __shared__ __attribute__((aligned(16))) __bf16 sAh[BM * PK];
+    __shared__ __attribute__((aligned(16))) __bf16 sAl[BM * PK];
+    __shared__ __attribute__((aligned(16))) __bf16 sBh[BN * PK];
+    __shared__ __attribute__((aligned(16))) __bf16 sBl[BN * PK];
     int mb, nbk;
-    tile_of(blockIdx.x, nmb, nnb, mb, nbk);
+    tile_of(blockIdx.x, nmb, nnb, G, remap, mb, nbk);
     const int m0 = mb * BM, n0 = nbk * BN;
     if (Epi::upper && m0 > n0 + BN - 1) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -389,18 +404,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void n
         gb[c] = *(const bf8_t*)(pB + c * sb);
         gb[CB + c] = *(const bf8_t*)(pBl + c * sb);
     }
-    const int kq = 8 * (lane >> 4), fr = lane & 15;
+    const int kq = lane >> 4, fr = lane & 15, kcw = tid & 3;
     for (int k0 = 0; k0 < K; k0 += PK) {
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < CA; c++) {
-            *(bf8_t*)&sAh[r0 + 64 * c][kc] = ga[c];
-            *(bf8_t*)&sAl[r0 + 64 * c][kc] = ga[CA + c];
+            const int o = lds_slot(r0 + 64 * c, kcw);
+            *(bf8_t*)&sAh[o] = ga[c];
+            *(bf8_t*)&sAl[o] = ga[CA + c];
         }
 #pragma unroll
         for (int c = 0; c < CB; c++) {
-            *(bf8_t*)&sBh[r0 + 64 * c][kc] = gb[c];
-            *(bf8_t*)&sBl[r0 + 64 * c][kc] = gb[CB + c];
+            const int o = lds_slot(r0 + 64 * c, kcw);
+            *(bf8_t*)&sBh[o] = gb[c];
+            *(bf8_t*)&sBl[o] = gb[CB + c];
         }
         __syncthreads();
         if (k0 + PK < K) {
@@ -419,19 +436,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void n
         bf8_t bh[FN], bl[FN];
 #pragma unroll
         for (int j = 0; j < FN; j++) {
-            bh[j] = *(const bf8_t*)&sBh[wn + j * 16 + fr][kq];
-            bl[j] = *(const bf8_t*)&sBl[wn + j * 16 + fr][kq];
+            const int o = lds_slot(wn + j * 16 + fr, kq);
+            bh[j] = *(const bf8_t*)&sBh[o];
+            bl[j] = *(const bf8_t*)&sBl[o];
         }
 #pragma unroll
         for (int i = 0; i < FM; i++) {
-            const bf8_t ah = *(const bf8_t*)&sAh[wm + i * 16 + fr][kq];
-            const bf8_t al = *(const bf8_t*)&sAl[wm + i * 16 + fr][kq];
+            const int o = lds_slot(wm + i * 16 + fr, kq);
+            const bf8_t ah = *(const bf8_t*)&sAh[o];
+            const bf8_t al = *(const bf8_t*)&sAl[o];
+            // three sweeps over independent accumulators (lo*hi, hi*lo, hi*hi):
+            // consecutive MFMAs never wait on each other's result
 #pragma unroll
-            for (int j = 0; j < FN; j++) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
-            }
+            for (int j = 0; j < FN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < FN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < FN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
         }
     }
 #pragma unroll
@@ -441,13 +462,363 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void n
             E.put4(m0 + wm + i * 16 + 4 * (lane >> 4), n0 + wn + j * 16 + fr, acc[i][j]);
 }
 
+// ---- stage 1 with S_p generated from the pair table ------------------------------
+//
+// The A operand of C_p = S_p B' is never stored: each of the 256 threads owns
+// one row a of the 256-row tile and builds its 32 entries per k-step from the
+// band's pair table, pre-split into 32-bit words (hi bf16 | lo bf16 << 16),
+// Q_p[key(a, b)] with key = (|dx| Ny + |dy|) Nz + |dz|.  The table (128 KB at
+// 32^3) stays in L1/L2; a 256 x 32 tile touches about 256 distinct keys.  Cell
+// coordinates are packed z | y << 10 | x << 20 (padding columns 0xffffffff).
+// When the 16 columns of a half-chunk are consecutive-z cells of one (x, y)
+// line -- always so for a box in the reference's cell order -- the test is
+// wave-uniform and each key costs one v_sad_u32: key = base + |za - zb|.
+
+constexpr unsigned CELL_NONE = 0xffffffffu;
+
+__global__ void pack_table_kernel(const double* __restrict__ Q, unsigned* __restrict__ out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float v = (float)Q[i];
+    const __bf16 h = (__bf16)v;
+    const __bf16 l = (__bf16)(v - (float)h);
+    out[i] = (unsigned)__builtin_bit_cast(unsigned short, h) | ((unsigned)__builtin_bit_cast(unsigned short, l) << 16);
+}
+
+__global__ void pack_cells_kernel(const int* __restrict__ cellidx, int N, int Kp, unsigned* __restrict__ out) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= Kp) return;
+    out[a] = (a < N) ? ((unsigned)cellidx[3 * a + 2] | ((unsigned)cellidx[3 * a + 1] << 10) |
+                        ((unsigned)cellidx[3 * a] << 20))
+                     : CELL_NONE;
+}
+
+// hi halves / lo halves of two table words as one dword of two bf16
+__device__ __forceinline__ unsigned pair_hi(unsigned w0, unsigned w1) { return (w0 & 0xffffu) | (w1 << 16); }
+__device__ __forceinline__ unsigned pair_lo(unsigned w0, unsigned w1) { return (w0 >> 16) | (w1 & 0xffff0000u); }
+
+// 16 table words for row (xa, ya, za) and the columns of c[0..15].
+__device__ __forceinline__ void gather16(const unsigned* __restrict__ Qhl, const unsigned (&c)[16], unsigned xa,
+                                         unsigned ya, unsigned za, bool rowv, int Ny, int Nz, unsigned* w) {
+    // consecutive z within one (x, y) line, no carry out of the z field
+    bool line = c[0] != CELL_NONE && (c[0] & 1023u) <= 1008u;
+#pragma unroll
+    for (int e = 1; e < 16; e++) line = line && (c[e] == c[0] + e);
+    if (line) {
+        const unsigned xb = c[0] >> 20, yb = (c[0] >> 10) & 1023u, zb = c[0] & 1023u;
+        const unsigned base = (__sad((int)xa, (int)xb, 0u) * Ny + __sad((int)ya, (int)yb, 0u)) * Nz;
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const unsigned key = rowv ? __sad((int)za, (int)(zb + e), base) : 0u;   // padding rows read key 0
+            const unsigned v = Qhl[key];
+            w[e] = rowv ? v : 0u;
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const unsigned cb = c[e];
+            const bool ok = rowv && cb != CELL_NONE;
+            const unsigned key = ok ? (__sad((int)xa, (int)(cb >> 20), 0u) * Ny + __sad((int)ya, (int)((cb >> 10) & 1023u), 0u)) * Nz +
+                                          __sad((int)za, (int)(cb & 1023u), 0u)
+                                    : 0u;
+            const unsigned v = Qhl[key];
+            w[e] = ok ? v : 0u;
+        }
+    }
+}
+
+template <int FM, int FN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void proj_gen_kernel(
+    const unsigned* __restrict__ Qhl, const unsigned* __restrict__ cpk, int Ny, int Nz, const __bf16* __restrict__ Bh,
+    const __bf16* __restrict__ Bl, size_t ldb, int K, int nmb, int nnb, int G, int remap, EpiPlanes E) {
+    constexpr int BM = 32 * FM, BN = 32 * FN;
+    static_assert(BM == 256, "one A row per thread");
+    constexpr int CB = BN * 4 / 256;
+    __shared__ __attribute__((aligned(16))) __bf16 sAh[BM * PK];
+    __shared__ __attribute__((aligned(16))) __bf16 sAl[BM * PK];
+    __shared__ __attribute__((aligned(16))) __bf16 sBh[BN * PK];
+    __shared__ __attribute__((aligned(16))) __bf16 sBl[BN * PK];
+    int mb, nbk;
+    tile_of(blockIdx.x, nmb, nnb, G, remap, mb, nbk);
+    const int m0 = mb * BM, n0 = nbk * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * (16 * FM), wn = (wave & 1) * (16 * FN);
+    const unsigned ca = cpk[m0 + tid];
+    const bool rowv = ca != CELL_NONE;
+    const unsigned xa = ca >> 20, ya = (ca >> 10) & 1023u, za = ca & 1023u;
+    const int kc = (tid & 3) * 8, r0 = tid >> 2;
+    const __bf16* pB = Bh + (size_t)(n0 + r0) * ldb + kc;
+    const __bf16* pBl = Bl + (size_t)(n0 + r0) * ldb + kc;
+    const size_t sb = (size_t)64 * ldb;
+    f4_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++) acc[i][j] = f4_t{0, 0, 0, 0};
+    bf8_t gb[2 * CB];
+    unsigned w[32];
+    auto fetch = [&](int k) {
+#pragma unroll
+        for (int c = 0; c < CB; c++) {
+            gb[c] = *(const bf8_t*)(pB + c * sb + k);
+            gb[CB + c] = *(const bf8_t*)(pBl + c * sb + k);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            unsigned c[16];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint4 v = *(const uint4*)(cpk + k + 16 * h + 4 * q);
+                c[4 * q] = v.x;
+                c[4 * q + 1] = v.y;
+                c[4 * q + 2] = v.z;
+                c[4 * q + 3] = v.w;
+            }
+            gather16(Qhl, c, xa, ya, za, rowv, Ny, Nz, w + 16 * h);
+        }
+    };
+    fetch(0);
+    const int kq = lane >> 4, fr = lane & 15, kcw = tid & 3;
+    for (int k0 = 0; k0 < K; k0 += PK) {
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int o = lds_slot(tid, c);
+            uint4 h, l;
+            h.x = pair_hi(w[8 * c], w[8 * c + 1]);
+            h.y = pair_hi(w[8 * c + 2], w[8 * c + 3]);
+            h.z = pair_hi(w[8 * c + 4], w[8 * c + 5]);
+            h.w = pair_hi(w[8 * c + 6], w[8 * c + 7]);
+            l.x = pair_lo(w[8 * c], w[8 * c + 1]);
+            l.y = pair_lo(w[8 * c + 2], w[8 * c + 3]);
+            l.z = pair_lo(w[8 * c + 4], w[8 * c + 5]);
+            l.w = pair_lo(w[8 * c + 6], w[8 * c + 7]);
+            *(uint4*)&sAh[o] = h;
+            *(uint4*)&sAl[o] = l;
+        }
+#pragma unroll
+        for (int c = 0; c < CB; c++) {
+            const int o = lds_slot(r0 + 64 * c, kcw);
+            *(bf8_t*)&sBh[o] = gb[c];
+            *(bf8_t*)&sBl[o] = gb[CB + c];
+        }
+        __syncthreads();
+        if (k0 + PK < K) fetch(k0 + PK);
+        bf8_t bh[FN], bl[FN];
+#pragma unroll
+        for (int j = 0; j < FN; j++) {
+            const int o = lds_slot(wn + j * 16 + fr, kq);
+            bh[j] = *(const bf8_t*)&sBh[o];
+            bl[j] = *(const bf8_t*)&sBl[o];
+        }
+#pragma unroll
+        for (int i = 0; i < FM; i++) {
+            const int o = lds_slot(wm + i * 16 + fr, kq);
+            const bf8_t ah = *(const bf8_t*)&sAh[o];
+            const bf8_t al = *(const bf8_t*)&sAl[o];
+            // three sweeps over independent accumulators (lo*hi, hi*lo, hi*hi):
+            // consecutive MFMAs never wait on each other's result
+#pragma unroll
+            for (int j = 0; j < FN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < FN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < FN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++) E.put4(m0 + wm + i * 16 + 4 * (lane >> 4), n0 + wn + j * 16 + fr, acc[i][j]);
+}
+
+// ---- 256 x 256 split-BF16x3 GEMM, double-buffered LDS ------------------------------
+//
+// D = A B^T with B from plane pairs and A either from plane pairs (ASrcPlanes)
+// or generated from the pair table (ASrcTable).  512 threads = 8 waves in a
+// 2 (m) x 4 (n) arrangement, each owning 8 x 4 fragments of 16 x 16; K in
+// steps of 32 through two LDS stages of 64 KB, one barrier per step: the next
+// step's global loads (or table gathers) are issued before the current step's
+// MFMAs and stored to the other stage after them.
+
+constexpr int G3 = 256;          // tile edge
+constexpr int G3_STAGE = G3 * PK; // bf16 per plane and stage
+
+struct ASrcPlanes {
+    const __bf16* hi;
+    const __bf16* lo;
+    size_t ld;
+    struct Regs {
+        bf8_t v[4];
+    };
+    __device__ __forceinline__ void init(int, Regs&) const {}
+    // thread t: rows (t >> 2) and (t >> 2) + 128, chunk t & 3, both planes
+    __device__ __forceinline__ void fetch(int m0, int k, Regs& R) const {
+        const int t = threadIdx.x, r = t >> 2, kc = (t & 3) * 8;
+        const size_t o0 = (size_t)(m0 + r) * ld + k + kc, o1 = o0 + (size_t)128 * ld;
+        R.v[0] = *(const bf8_t*)(hi + o0);
+        R.v[1] = *(const bf8_t*)(hi + o1);
+        R.v[2] = *(const bf8_t*)(lo + o0);
+        R.v[3] = *(const bf8_t*)(lo + o1);
+    }
+    __device__ __forceinline__ void store(__bf16* sh, __bf16* sl, const Regs& R) const {
+        const int t = threadIdx.x, r = t >> 2, c = t & 3;
+        *(bf8_t*)&sh[lds_slot(r, c)] = R.v[0];
+        *(bf8_t*)&sh[lds_slot(r + 128, c)] = R.v[1];
+        *(bf8_t*)&sl[lds_slot(r, c)] = R.v[2];
+        *(bf8_t*)&sl[lds_slot(r + 128, c)] = R.v[3];
+    }
+};
+
+struct ASrcTable {
+    const unsigned* Qhl;
+    const unsigned* cpk;
+    int Ny, Nz;
+    struct Regs {
+        unsigned w[16];
+        unsigned xa, ya, za;
+        bool rowv;
+    };
+    // thread t: row t >> 1 of the tile, columns 16 (t & 1) .. + 15 of the step
+    __device__ __forceinline__ void init(int m0, Regs& R) const {
+        const unsigned ca = cpk[m0 + (threadIdx.x >> 1)];
+        R.rowv = ca != CELL_NONE;
+        R.xa = ca >> 20;
+        R.ya = (ca >> 10) & 1023u;
+        R.za = ca & 1023u;
+    }
+    __device__ __forceinline__ void fetch(int, int k, Regs& R) const {
+        const int h = threadIdx.x & 1;
+        unsigned c[16];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint4 v = *(const uint4*)(cpk + k + 16 * h + 4 * q);
+            c[4 * q] = v.x;
+            c[4 * q + 1] = v.y;
+            c[4 * q + 2] = v.z;
+            c[4 * q + 3] = v.w;
+        }
+        gather16(Qhl, c, R.xa, R.ya, R.za, R.rowv, Ny, Nz, R.w);
+    }
+    __device__ __forceinline__ void store(__bf16* sh, __bf16* sl, const Regs& R) const {
+        const int r = threadIdx.x >> 1, h = threadIdx.x & 1;
+#pragma unroll
+        for (int cc = 0; cc < 2; cc++) {
+            const unsigned* w = R.w + 8 * cc;
+            uint4 a, b;
+            a.x = pair_hi(w[0], w[1]);
+            a.y = pair_hi(w[2], w[3]);
+            a.z = pair_hi(w[4], w[5]);
+            a.w = pair_hi(w[6], w[7]);
+            b.x = pair_lo(w[0], w[1]);
+            b.y = pair_lo(w[2], w[3]);
+            b.z = pair_lo(w[4], w[5]);
+            b.w = pair_lo(w[6], w[7]);
+            const int o = lds_slot(r, 2 * h + cc);
+            *(uint4*)&sh[o] = a;
+            *(uint4*)&sl[o] = b;
+        }
+    }
+};
+
+template <typename ASrc, typename Epi>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void gemm3_kernel(
+    ASrc A, const __bf16* __restrict__ Bh, const __bf16* __restrict__ Bl, size_t ldb, int K, int nmb, int nnb, int G,
+    int remap, Epi E) {
+    extern __shared__ __attribute__((aligned(16))) __bf16 g3_lds[];   // 2 stages x (Ah, Al, Bh, Bl)
+    int mb, nbk;
+    tile_of(blockIdx.x, nmb, nnb, G, remap, mb, nbk);
+    const int m0 = mb * G3, n0 = nbk * G3;
+    if (Epi::upper && m0 > n0 + G3 - 1) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 2) * 128, wn = (wave & 3) * 64;
+    typename ASrc::Regs ra;
+    A.init(m0, ra);
+    // B chunks: rows (t >> 2), + 128, chunk t & 3, both planes
+    const int br = tid >> 2, bkc = (tid & 3) * 8;
+    const __bf16* pBh = Bh + (size_t)(n0 + br) * ldb + bkc;
+    const __bf16* pBl = Bl + (size_t)(n0 + br) * ldb + bkc;
+    const size_t b128 = (size_t)128 * ldb;
+    bf8_t rb[4];
+    auto fetchB = [&](int k) {
+        rb[0] = *(const bf8_t*)(pBh + k);
+        rb[1] = *(const bf8_t*)(pBh + b128 + k);
+        rb[2] = *(const bf8_t*)(pBl + k);
+        rb[3] = *(const bf8_t*)(pBl + b128 + k);
+    };
+    auto storeB = [&](__bf16* sh, __bf16* sl) {
+        const int c = tid & 3;
+        *(bf8_t*)&sh[lds_slot(br, c)] = rb[0];
+        *(bf8_t*)&sh[lds_slot(br + 128, c)] = rb[1];
+        *(bf8_t*)&sl[lds_slot(br, c)] = rb[2];
+        *(bf8_t*)&sl[lds_slot(br + 128, c)] = rb[3];
+    };
+    auto stage = [&](int s, int plane) { return g3_lds + (size_t)(4 * s + plane) * G3_STAGE; };
+    f4_t acc[8][4];
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = f4_t{0, 0, 0, 0};
+    A.fetch(m0, 0, ra);
+    fetchB(0);
+    A.store(stage(0, 0), stage(0, 1), ra);
+    storeB(stage(0, 2), stage(0, 3));
+    __syncthreads();
+    const int kq = lane >> 4, fr = lane & 15;
+    const int nk = K / PK;
+    for (int kt = 0; kt < nk; kt++) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < nk;
+        if (more) {
+            A.fetch(m0, (kt + 1) * PK, ra);
+            fetchB((kt + 1) * PK);
+        }
+        const __bf16* sAh = stage(cur, 0);
+        const __bf16* sAl = stage(cur, 1);
+        const __bf16* sBh = stage(cur, 2);
+        const __bf16* sBl = stage(cur, 3);
+        bf8_t bh[4], bl[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int o = lds_slot(wn + j * 16 + fr, kq);
+            bh[j] = *(const bf8_t*)&sBh[o];
+            bl[j] = *(const bf8_t*)&sBl[o];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int o = lds_slot(wm + i * 16 + fr, kq);
+            const bf8_t ah = *(const bf8_t*)&sAh[o];
+            const bf8_t al = *(const bf8_t*)&sAl[o];
+            // three sweeps over independent accumulators (lo*hi, hi*lo, hi*hi):
+            // consecutive MFMAs never wait on each other's result
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
+        }
+        if (more) {
+            A.store(stage(cur ^ 1, 0), stage(cur ^ 1, 1), ra);
+            storeB(stage(cur ^ 1, 2), stage(cur ^ 1, 3));
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) E.put4(m0 + wm + i * 16 + 4 * (lane >> 4), n0 + wn + j * 16 + fr, acc[i][j]);
+}
+
+constexpr size_t G3_LDS_BYTES = (size_t)8 * G3_STAGE * sizeof(__bf16);   // 128 KB
+
 // B' = diag(w) B with w = sqrt(Nbar)
-__global__ void scale_rows_kernel(const double* B, const double* nbar, int N, int M, double* Bw) {
+__global__ void scale_rows_kernel(const double* B, const double* nbar, int N, int M, double* Bw, size_t ldw) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t tot = (size_t)N * M;
     if (i >= tot) return;
-    int a = (int)(i % N);
-    Bw[i] = B[i] * sqrt(nbar[a]);
+    const size_t a = i % N, n = i / N;
+    Bw[n * ldw + a] = B[i] * sqrt(nbar[a]);
 }
 
 // ---- host --------------------------------------------------------------------------
@@ -470,21 +841,48 @@ static int split_planes(opsec_ctx* ctx, const double* X, size_t ldx, int rows, i
     return OPSEC_OK;
 }
 
-// Tile shape of the split-BF16x3 GEMM: 0 = 128 x 128, 1 = 256 x 128 (default).
-static int g_nt_tile = 1;
+// Tile shape of the split-BF16x3 GEMM (0 = 128 x 128, 1 = 256 x 128), tile
+// group edge and XCD remapping of the tile order.
+static int g_nt_tile = 2, g_nt_group = 8, g_nt_remap = 0;
+// Stage 1 generates S_p from the pair table (1) or reads S_p planes written by
+// gen_split_kernel (0).
+static int g_nt_gen = 1;
+
+template <typename ASrc, typename Epi>
+static void launch_g3(opsec_ctx* ctx, const ASrc& A, int arows, const __bf16* B, int brows, size_t ld, int K,
+                      const Epi& E) {
+    static bool attr = false;
+    if (!attr) {   // 128 KB of dynamic LDS per workgroup
+        (void)hipFuncSetAttribute((const void*)gemm3_kernel<ASrcPlanes, EpiPlanes>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)G3_LDS_BYTES);
+        (void)hipFuncSetAttribute((const void*)gemm3_kernel<ASrcPlanes, EpiPacked>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)G3_LDS_BYTES);
+        (void)hipFuncSetAttribute((const void*)gemm3_kernel<ASrcTable, EpiPlanes>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)G3_LDS_BYTES);
+        attr = true;
+    }
+    const int nmb = arows / G3, nnb = brows / G3;
+    hipLaunchKernelGGL((gemm3_kernel<ASrc, Epi>), dim3(nmb * nnb), dim3(512), G3_LDS_BYTES, ctx->stream, A, B,
+                       B + (size_t)brows * ld, ld, K, nmb, nnb, g_nt_group, g_nt_remap, E);
+}
 
 // C = A B^T (stage 1, epilogue to split planes) or P = A B^T (stage 2, packed).
 // Plane row counts are multiples of 256.
 template <typename Epi>
-static void launch_nt(opsec_ctx* ctx, const __bf16* A, int arows, const __bf16* B, int brows, size_t ld, const Epi& E) {
-    if (g_nt_tile == 0) {
+static void launch_nt(opsec_ctx* ctx, const __bf16* A, int arows, const __bf16* B, int brows, size_t ld, int K,
+                      const Epi& E) {
+    if (g_nt_tile == 2) {
+        launch_g3(ctx, ASrcPlanes{A, A + (size_t)arows * ld, ld}, arows, B, brows, ld, K, E);
+    } else if (g_nt_tile == 0) {
         const int nmb = arows / 128, nnb = brows / 128;
         hipLaunchKernelGGL((nt_bf16x3_kernel<Epi, 4, 4>), dim3(nmb * nnb), dim3(256), 0, ctx->stream, A,
-                           A + (size_t)arows * ld, ld, B, B + (size_t)brows * ld, ld, (int)ld, nmb, nnb, E);
+                           A + (size_t)arows * ld, ld, B, B + (size_t)brows * ld, ld, K, nmb, nnb, g_nt_group,
+                           g_nt_remap, E);
     } else {
         const int nmb = arows / 256, nnb = brows / 128;
         hipLaunchKernelGGL((nt_bf16x3_kernel<Epi, 8, 4>), dim3(nmb * nnb), dim3(256), 0, ctx->stream, A,
-                           A + (size_t)arows * ld, ld, B, B + (size_t)brows * ld, ld, (int)ld, nmb, nnb, E);
+                           A + (size_t)arows * ld, ld, B, B + (size_t)brows * ld, ld, K, nmb, nnb, g_nt_group,
+                           g_nt_remap, E);
     }
 }
 
@@ -492,32 +890,55 @@ static void launch_nt(opsec_ctx* ctx, const __bf16* A, int arows, const __bf16* 
 static int run_project_bf16x3(opsec_ctx* ctx, int nb, const int* funcs) {
     ProjState* ps = ctx->proj;
     const int N = ctx->ncells, M = ctx->nmodes;
-    const size_t Kp = round_up(N, 256), Mp = round_up(M, 256);
+    const size_t Kp = round_up(N, 256), Mp = round_up(M, 256), Pp = plane_pitch(Kp);
     int rc;
-    if ((rc = dev_realloc(ctx, (void**)&ps->d_Bp, &ps->cap_Bp, 2 * Mp * Kp * sizeof(__bf16)))) return rc;
-    if ((rc = dev_realloc(ctx, (void**)&ps->d_Sp, &ps->cap_Sp, 2 * Kp * Kp * sizeof(__bf16)))) return rc;
-    if ((rc = dev_realloc(ctx, (void**)&ps->d_Cp, &ps->cap_Cp, 2 * Mp * Kp * sizeof(__bf16)))) return rc;
+    if ((rc = dev_realloc(ctx, (void**)&ps->d_Bp, &ps->cap_Bp, 2 * Mp * Pp * sizeof(__bf16)))) return rc;
+    if (!(g_nt_gen && ctx->sv.n1 < 1024 && ctx->sv.n2 < 1024 && ctx->sv.n3 < 1024))
+        if ((rc = dev_realloc(ctx, (void**)&ps->d_Sp, &ps->cap_Sp, 2 * Kp * Pp * sizeof(__bf16)))) return rc;
+    if ((rc = dev_realloc(ctx, (void**)&ps->d_Cp, &ps->cap_Cp, 2 * Mp * Pp * sizeof(__bf16)))) return rc;
     if ((rc = dev_realloc(ctx, &ps->d_P, &ps->cap_P, (size_t)nb * packed_size(M) * 8))) return rc;
     if (!ps->bp_valid) {
         // B' is column-major N x M: row n of the plane is mode n, cells contiguous
-        if ((rc = split_planes(ctx, ps->d_Bw, (size_t)N, M, N, ps->d_Bp, (int)Mp, Kp))) return rc;
+        if ((rc = split_planes(ctx, ps->d_Bw, ps->ldw, M, N, ps->d_Bp, (int)Mp, Pp))) return rc;
         ps->bp_valid = true;
+    }
+    // S_p from the pair table inside the GEMM needs grid indices below 1024
+    const bool gen = g_nt_gen && ctx->sv.n1 < 1024 && ctx->sv.n2 < 1024 && ctx->sv.n3 < 1024;
+    if (gen) {
+        if ((rc = dev_realloc(ctx, (void**)&ps->d_Qhl, &ps->cap_Qhl, (size_t)ctx->nkeys * 4))) return rc;
+        if ((rc = dev_realloc(ctx, (void**)&ps->d_cpk, &ps->cap_cpk, Kp * 4))) return rc;
+        hipLaunchKernelGGL(pack_cells_kernel, dim3((unsigned)((Kp + 255) / 256)), dim3(256), 0, ctx->stream,
+                           ctx->d_cellidx, N, (int)Kp, ps->d_cpk);
     }
     std::vector<hipEvent_t> ev(4 * (size_t)nb);
     for (auto& e : ev) OPSEC_HIP_TRY(ctx, hipEventCreate(&e));
     __bf16* Sh = ps->d_Sp;
-    __bf16* Sl = ps->d_Sp + Kp * Kp;
-    EpiPlanes toC{ps->d_Cp, ps->d_Cp + Mp * Kp, Kp};
+    __bf16* Sl = ps->d_Sp + Kp * Pp;
+    EpiPlanes toC{ps->d_Cp, ps->d_Cp + Mp * Pp, Pp};
     for (int z = 0; z < nb; z++) {
+        const double* Qf = ctx->d_Q + (size_t)funcs[z] * ctx->nkeys;
         hipEventRecord(ev[4 * z], ctx->stream);
-        hipLaunchKernelGGL(gen_split_kernel, blocks_for(Kp * Kp / 8), dim3(256), 0, ctx->stream,
-                           ctx->d_Q + (size_t)funcs[z] * ctx->nkeys, ctx->d_cellidx, ctx->sv.n2, ctx->sv.n3, N, Sh,
-                           Sl, Kp, (int)Kp);
+        if (gen)
+            hipLaunchKernelGGL(pack_table_kernel, dim3((ctx->nkeys + 255) / 256), dim3(256), 0, ctx->stream, Qf,
+                               ps->d_Qhl, ctx->nkeys);
+        else
+            hipLaunchKernelGGL(gen_split_kernel, blocks_for(Kp * Pp / 8), dim3(256), 0, ctx->stream, Qf,
+                               ctx->d_cellidx, ctx->sv.n2, ctx->sv.n3, N, Sh, Sl, Pp, (int)Kp);
         hipEventRecord(ev[4 * z + 1], ctx->stream);
-        launch_nt(ctx, ps->d_Sp, (int)Kp, ps->d_Bp, (int)Mp, Kp, toC);
+        if (gen && g_nt_tile == 2) {
+            launch_g3(ctx, ASrcTable{ps->d_Qhl, ps->d_cpk, ctx->sv.n2, ctx->sv.n3}, (int)Kp, ps->d_Bp, (int)Mp, Pp,
+                      (int)Kp, toC);
+        } else if (gen) {
+            const int nmb = (int)(Kp / 256), nnb = (int)(Mp / 128);
+            hipLaunchKernelGGL((proj_gen_kernel<8, 4>), dim3(nmb * nnb), dim3(256), 0, ctx->stream, ps->d_Qhl,
+                               ps->d_cpk, ctx->sv.n2, ctx->sv.n3, ps->d_Bp, ps->d_Bp + Mp * Pp, Pp, (int)Kp, nmb, nnb,
+                               g_nt_group, g_nt_remap, toC);
+        } else {
+            launch_nt(ctx, ps->d_Sp, (int)Kp, ps->d_Bp, (int)Mp, Pp, (int)Kp, toC);
+        }
         hipEventRecord(ev[4 * z + 2], ctx->stream);
         EpiPacked toP{(double*)ps->d_P + (size_t)z * packed_size(M), M};
-        launch_nt(ctx, ps->d_Bp, (int)Mp, ps->d_Cp, (int)Mp, Kp, toP);
+        launch_nt(ctx, ps->d_Bp, (int)Mp, ps->d_Cp, (int)Mp, Pp, (int)Kp, toP);
         hipEventRecord(ev[4 * z + 3], ctx->stream);
     }
     hipError_t e = hipGetLastError();
@@ -554,8 +975,9 @@ static int run_project(opsec_ctx* ctx, int nb, const int* d_funcs, const int* fu
     ProjState* ps = ctx->proj;
     const int N = ctx->ncells, M = ctx->nmodes;
     const size_t esize = (prec == OPSEC_PREC_FP64) ? 8 : 4;
+    const size_t ldc = ps->ldw;   // stage-1 products use the same padded pitch as B'
     int rc;
-    if ((rc = dev_realloc(ctx, &ps->d_C, &ps->cap_C, (size_t)nb * N * M * esize))) return rc;
+    if ((rc = dev_realloc(ctx, &ps->d_C, &ps->cap_C, (size_t)nb * ldc * M * esize))) return rc;
     if ((rc = dev_realloc(ctx, &ps->d_P, &ps->cap_P, (size_t)nb * packed_size(M) * 8))) return rc;
     GA_PairTable A;
     A.Q = ctx->d_Q;
@@ -566,7 +988,7 @@ static int run_project(opsec_ctx* ctx, int nb, const int* d_funcs, const int* fu
     A.funcs = d_funcs;
     GB_ColMajor Bop;
     Bop.p = ps->d_Bw;
-    Bop.ld = N;
+    Bop.ld = ps->ldw;
     hipEvent_t e0, e1, e2;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
@@ -575,24 +997,24 @@ static int run_project(opsec_ctx* ctx, int nb, const int* d_funcs, const int* fu
     dim3 g1 = grid_for(N, M, nb), g2 = grid_for(M, M, nb);
     GA_ColMajorT A2;
     A2.p = ps->d_Bw;
-    A2.ld = N;
+    A2.ld = ps->ldw;
     PutPacked outP{(double*)ps->d_P, (size_t)packed_size(M)};
     if (prec == OPSEC_PREC_FP64) {
-        PutColMajor<double> outC{(double*)ps->d_C, (size_t)N, (size_t)N * M};
+        PutColMajor<double> outC{(double*)ps->d_C, ldc, ldc * M};
         hipLaunchKernelGGL((gemm_kernel<double, GA_PairTable, GB_ColMajor, PutColMajor<double>, false>), g1,
                            dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, 0, outC, N, M, N);
         hipEventRecord(e1, ctx->stream);
-        GB_ColMajorT<double> B2{(const double*)ps->d_C, (size_t)N};
+        GB_ColMajorT<double> B2{(const double*)ps->d_C, ldc};
         hipLaunchKernelGGL((gemm_kernel<double, GA_ColMajorT, GB_ColMajorT<double>, PutPacked, true>), g2,
-                           dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, (size_t)N * M, outP, M, M, N);
+                           dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, ldc * M, outP, M, M, N);
     } else if (prec == OPSEC_PREC_FP32) {
-        PutColMajor<float> outC{(float*)ps->d_C, (size_t)N, (size_t)N * M};
+        PutColMajor<float> outC{(float*)ps->d_C, ldc, ldc * M};
         hipLaunchKernelGGL((gemm_kernel<float, GA_PairTable, GB_ColMajor, PutColMajor<float>, false>), g1,
                            dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, 0, outC, N, M, N);
         hipEventRecord(e1, ctx->stream);
-        GB_ColMajorT<float> B2{(const float*)ps->d_C, (size_t)N};
+        GB_ColMajorT<float> B2{(const float*)ps->d_C, ldc};
         hipLaunchKernelGGL((gemm_kernel<float, GA_ColMajorT, GB_ColMajorT<float>, PutPacked, true>), g2,
-                           dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, (size_t)N * M, outP, M, M, N);
+                           dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, ldc * M, outP, M, M, N);
     }
     hipEventRecord(e2, ctx->stream);
     hipError_t e = hipGetLastError();
@@ -622,6 +1044,8 @@ void opsec_internal_free_proj(opsec_ctx* ctx) {
     dev_free((void**)&ctx->proj->d_Bp);
     dev_free((void**)&ctx->proj->d_Sp);
     dev_free((void**)&ctx->proj->d_Cp);
+    dev_free((void**)&ctx->proj->d_Qhl);
+    dev_free((void**)&ctx->proj->d_cpk);
     delete ctx->proj;
     ctx->proj = nullptr;
 }
@@ -639,11 +1063,12 @@ int opsec_modes_upload(opsec_ctx* ctx, const double* B, int ncells, int nmodes) 
     dev_free((void**)&ctx->d_B);
     size_t capB = 0;
     if ((rc = dev_realloc(ctx, (void**)&ctx->d_B, &capB, bytes))) return rc;
-    if ((rc = dev_realloc(ctx, (void**)&ps->d_Bw, &ps->cap_Bw, bytes))) return rc;
+    ps->ldw = (size_t)ncells + 32;
+    if ((rc = dev_realloc(ctx, (void**)&ps->d_Bw, &ps->cap_Bw, ps->ldw * nmodes * 8))) return rc;
     OPSEC_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_B, B, bytes, hipMemcpyHostToDevice, ctx->stream));
     size_t tot = (size_t)ncells * nmodes;
     hipLaunchKernelGGL(scale_rows_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, ctx->d_B,
-                       ctx->d_nbar, ncells, nmodes, ps->d_Bw);
+                       ctx->d_nbar, ncells, nmodes, ps->d_Bw, ps->ldw);
     OPSEC_HIP_TRY(ctx, hipGetLastError());
     OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->nmodes = nmodes;
@@ -685,10 +1110,28 @@ int opsec_internal_project_times(opsec_ctx* ctx, float* s1, float* s2) {
     return OPSEC_OK;
 }
 
-// Tile shape of the split-BF16x3 GEMM (0: 128 x 128, 1: 256 x 128), for tuning.
+// Tile shape of the split-BF16x3 GEMM (0: 128 x 128, 1: 256 x 128, 2: 256 x 256
+// with double-buffered LDS), for tuning.
 int opsec_internal_set_nt_tile(int v) {
-    if (v != 0 && v != 1) return OPSEC_EINVAL;
+    if (v < 0 || v > 2) return OPSEC_EINVAL;
     g_nt_tile = v;
+    return OPSEC_OK;
+}
+
+// Stage-1 A operand of the split-BF16x3 projection, for tuning: generated
+// from the pair table (1) or read from S_p planes (0).
+int opsec_internal_set_nt_gen(int v) {
+    if (v != 0 && v != 1) return OPSEC_EINVAL;
+    g_nt_gen = v;
+    return OPSEC_OK;
+}
+
+// Tile-order knobs of the split-BF16x3 GEMM, for tuning: group edge (>= 1)
+// and XCD remapping (0 or 1).
+int opsec_internal_set_nt_order(int group, int remap) {
+    if (group < 1 || (remap != 0 && remap != 1)) return OPSEC_EINVAL;
+    g_nt_group = group;
+    g_nt_remap = remap;
     return OPSEC_OK;
 }
 
@@ -745,17 +1188,17 @@ int opsec_project_dense(opsec_ctx* ctx, int n, const double* S, int nmodes, cons
         hipLaunchKernelGGL((gemm_kernel<float, GA_ColMajorT, GB_ColMajorT<float>, PutPacked, true>), g2,
                            dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, 0, outP, M, M, N);
     } else {
-        const size_t Kp = round_up(N, 256), Mp = round_up(M, 256);
+        const size_t Kp = round_up(N, 256), Mp = round_up(M, 256), Pp = plane_pitch(Kp);
         __bf16 *Sp = nullptr, *Bp = nullptr, *Cp = nullptr;
-        OPSEC_HIP_TRY(ctx, hipMalloc(&Sp, 2 * Kp * Kp * sizeof(__bf16)));
-        OPSEC_HIP_TRY(ctx, hipMalloc(&Bp, 2 * Mp * Kp * sizeof(__bf16)));
-        OPSEC_HIP_TRY(ctx, hipMalloc(&Cp, 2 * Mp * Kp * sizeof(__bf16)));
-        int rc2 = split_planes(ctx, dS, (size_t)N, N, N, Sp, (int)Kp, Kp);
+        OPSEC_HIP_TRY(ctx, hipMalloc(&Sp, 2 * Kp * Pp * sizeof(__bf16)));
+        OPSEC_HIP_TRY(ctx, hipMalloc(&Bp, 2 * Mp * Pp * sizeof(__bf16)));
+        OPSEC_HIP_TRY(ctx, hipMalloc(&Cp, 2 * Mp * Pp * sizeof(__bf16)));
+        int rc2 = split_planes(ctx, dS, (size_t)N, N, N, Sp, (int)Kp, Pp);
         // B is row-major nmodes x n here: row = mode, cells contiguous
-        if (!rc2) rc2 = split_planes(ctx, dB, (size_t)N, M, N, Bp, (int)Mp, Kp);
+        if (!rc2) rc2 = split_planes(ctx, dB, (size_t)N, M, N, Bp, (int)Mp, Pp);
         if (!rc2) {
-            launch_nt(ctx, Sp, (int)Kp, Bp, (int)Mp, Kp, EpiPlanes{Cp, Cp + Mp * Kp, Kp});
-            launch_nt(ctx, Bp, (int)Mp, Cp, (int)Mp, Kp, EpiPacked{dP, M});
+            launch_nt(ctx, Sp, (int)Kp, Bp, (int)Mp, Pp, (int)Kp, EpiPlanes{Cp, Cp + Mp * Pp, Pp});
+            launch_nt(ctx, Bp, (int)Mp, Cp, (int)Mp, Pp, (int)Kp, EpiPacked{dP, M});
         }
         hipStreamSynchronize(ctx->stream);
         hipFree(Sp);

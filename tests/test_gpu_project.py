@@ -156,3 +156,29 @@ def test_estimate_device_after_projection(ctx):
     b = ctx.estimate(lam, D, y, prior)
     for k in ("F", "q", "f", "phat"):
         np.testing.assert_allclose(a[k], b[k], rtol=1e-12, atol=1e-14 * np.abs(b[k]).max())
+
+
+@pytest.mark.parametrize("n,shuffle", [(6, True), (20, False)])
+def test_project_bf16x3_generated_layouts(ctx, n, shuffle):
+    """Split-BF16x3 projection with S_p generated from the pair table inside the
+    GEMM, against the FP64 projection of the same tables: padded tiles
+    (n^3 not a multiple of 256), a shuffled cell order (general key path) and
+    z-lines that straddle 16-column chunks (mixed fast and general paths)."""
+    cells = synth.box_cells(n, 0.0, 25.0 * n)
+    if shuffle:
+        cells = cells[np.random.default_rng(3).permutation(len(cells))].copy()
+    sv = lib.survey_box(n, 0.0, 25.0 * n)
+    blo, bhi = synth.linear_bands(3, 0.02, 0.2)
+    ctx.model_real(GOLD["pk_k"], GOLD["pk_P"], blo, bhi, lib.OPSEC_XI_CLOSED)
+    ctx.cells_upload(cells, sv)
+    funcs = [0, 1, 2, 3]
+    ctx.pair_tables(funcs, ctx.quad(lib.OPSEC_QUAD_GAUSS, nodes=4, near=1))
+    N = len(cells)
+    m = 72
+    B = np.linalg.qr(np.random.default_rng(n).normal(size=(N, m)))[0].T.copy()
+    ctx.modes_upload(B)
+    ref = ctx.project(funcs, lib.OPSEC_PREC_FP64)
+    got = ctx.project(funcs, lib.OPSEC_PREC_BF16X3)
+    err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
+    print("generated-S_p BF16x3 projection max rel err per function", err)
+    assert err.max() < 5e-5

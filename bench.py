@@ -34,6 +34,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-pair integrals/s + S→Fisher wall time, surrogate vs FP64 at ΔF/F≤1e-3"
 PEAK_TF = {"fp64": 78.6, "fp32": 157.3, "bf16": 2516.6}
+# HBM bytes per launch of the stage-1 GEMM (gemm3_kernel<ASrcPlanes,EpiPlanes> at 32^3 cells x 4000
+# modes): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB from two rocprofv3 --pmc passes, the factors being
+# what the same counters read for copy16_kernel on a 2 GiB copy (profiles/projection_c2_hbm_counters.txt)
+STAGE1_TRAFFIC_BYTES = {("C2", 2, 0): (18912985 * 2.0 + 885280 * 1.0) * 1024}
 
 
 def log(*a):
@@ -154,8 +158,10 @@ def diagnose(ctx, L, funcs, q64, qs, lam, y, prior, e64):
 
 
 def cpu_baseline(cfg, cells, k, P, lo, hi, B, budget_s=20.0):
-    """Oracle (FP64 C restatement, OpenMP) on a bounded sample of the same
-    workload, extrapolated to the full S->Fisher pass."""
+    """Oracle (FP64 C restatement, OpenMP over the host threads) on a bounded
+    sample of the same workload, extrapolated to the full S->Fisher pass: a
+    block of rows of one band's S_p by the 6^3 Gauss rule, then S_p B for a
+    block of modes, each sized to a few seconds from a short pilot."""
     import ctypes as Cc
     from tests import oracles as O
     orc = O.oracle()
@@ -163,33 +169,44 @@ def cpu_baseline(cfg, cells, k, P, lo, hi, B, budget_s=20.0):
     N = n ** 3
     M = cfg["nmodes"]
     nb = cfg["nbands"]
-    cores = os.cpu_count() or 1
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     xi = O.XiHandle(1, a=float(lo[-1]), b=float(hi[-1]))     # closed-form band (highest k)
-    R = 512                                                    # row block of S_p
-    rows = np.arange(R, dtype=np.int32)
     cols = np.arange(N, dtype=np.int32)
-    S = np.zeros((R, N))
+
+    def rows_block(R):
+        rows = np.arange(R, dtype=np.int32)
+        S = np.zeros((R, N))
+        t0 = time.perf_counter()
+        orc.orc_sig_block(Cc.byref(xi.s), cells.ctypes.data, N, n, n, n, 400.0, O.ptr(rows), R, O.ptr(cols), N,
+                          O.ptr(S), N, 1, 0.0, 0.0, 0, cfg["nodes"], 1)
+        return S, time.perf_counter() - t0
+
+    _, t_pilot = rows_block(512)
+    R = int(min(N, max(512, (0.5 * budget_s / max(t_pilot, 1e-6)) * 512)) // 512 * 512)
+    S, t_asm = rows_block(R)
+    msub = 16
+    Cm = np.zeros((msub, R))
+    Bs = np.ascontiguousarray(B[:msub])
     t0 = time.perf_counter()
-    orc.orc_sig_block(Cc.byref(xi.s), cells.ctypes.data, N, n, n, n, 400.0, O.ptr(rows), R, O.ptr(cols), N,
-                      O.ptr(S), N, 1, 0.0, 0.0, 0, cfg["nodes"], 1)
-    t_asm = time.perf_counter() - t0
-    msub = 32
+    orc.orc_gemm_rows(R, N, O.ptr(S), msub, O.ptr(Bs), O.ptr(Cm))
+    t_pg = time.perf_counter() - t0
+    msub = int(min(M, max(16, 0.5 * budget_s / max(t_pg, 1e-6) * 16)))
     Bs = np.ascontiguousarray(B[:msub])
     Cm = np.zeros((msub, R))
     t0 = time.perf_counter()
     orc.orc_gemm_rows(R, N, O.ptr(S), msub, O.ptr(Bs), O.ptr(Cm))
     t_gemm = time.perf_counter() - t0
     # extrapolate: assembly scales with rows (all N) and functions (nb + 1);
-    # S B with rows and modes; B^T C (2 N M^2 / 2 for the upper triangle) from
-    # the measured S B flop rate
+    # the projection (2 N^2 M for S_p B', N M^2 for the upper triangle of
+    # B'^T C) at the measured S_p B flop rate
     rate = 2.0 * R * N * msub / t_gemm
     t_full = (nb + 1) * t_asm * (N / R) + nb * (2.0 * N * N * M + 1.0 * N * M * M) / rate
     pairs = float(N) * N * (nb + 1)
     return {
         "value": pairs / t_full, "unit": "cell-pair integrals/s", "cores": cores, "kind": "port",
-        "sample": (f"oracle on {R} of {N} rows of one band's S_p ({t_asm:.2f} s, 6^3 Gauss) and S_p B for "
-                   f"{msub} of {M} modes ({t_gemm:.2f} s, {rate / 1e9:.1f} GFLOP/s); extrapolated to "
-                   f"{nb + 1} matrices and the full projection: {t_full:.0f} s per S->Fisher pass"),
+        "sample": (f"oracle on {R} of {N} rows of one band's S_p ({t_asm:.1f} s, 6^3 Gauss) and S_p B for "
+                   f"{msub} of {M} modes ({t_gemm:.1f} s, {rate / 1e9:.1f} GFLOP/s), {cores} threads; extrapolated "
+                   f"to {nb + 1} matrices and the full projection: {t_full:.0f} s per S->Fisher pass"),
         "s_per_step_extrapolated": t_full,
     }
 
@@ -208,7 +225,7 @@ def main():
                     help="also attribute the Fisher error to the surrogate and to the mixed precision (untimed)")
     ap.add_argument("--nt-tile", type=int, default=2, choices=[0, 1, 2],
                     help="split-BF16x3 GEMM tile: 0 = 128 x 128, 1 = 256 x 128, 2 = 256 x 256 double-buffered")
-    ap.add_argument("--nt-gen", type=int, default=1, choices=[0, 1],
+    ap.add_argument("--nt-gen", type=int, default=0, choices=[0, 1],
                     help="stage 1 generates S_p from the pair table (1) or reads S_p planes (0)")
     ap.add_argument("--nt-group", type=int, default=8, help="split-BF16x3 GEMM tile-group edge")
     ap.add_argument("--nt-remap", type=int, default=0, choices=[0, 1], help="deal tile groups to XCDs")
@@ -318,12 +335,13 @@ def main():
         achieved = 2.0 * N * N * M / (per_launch / 1e3) / 1e12
         peak = PEAK_TF["bf16"] / 3
         out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                           "frac": achieved / peak, "traffic": None,
+                           "frac": achieved / peak,
+                           "traffic": STAGE1_TRAFFIC_BYTES.get((args.config, args.nt_tile, args.nt_gen)),
                            "kernel": stage1_kernel_name(args.nt_tile, args.nt_gen),
                            "launch_ms": per_launch,
                            "note": "useful flops 2 N^2 M per launch; peak = dense bf16 2516.6 TF / 3 products",
                            "mfma_utilisation": 3 * achieved / PEAK_TF["bf16"],
-                           "per_step_ms": {"gen_split": kg_ms, "stage1_gemm": k1_ms, "stage2_gemm": k2_ms}}
+                           "per_step_ms": {"s_planes": kg_ms, "stage1_gemm": k1_ms, "stage2_gemm": k2_ms}}
     else:
         s1_ms = float(R["kms"][0])
         achieved = 2.0 * N * N * M * nb / (s1_ms / 1e3) / 1e12

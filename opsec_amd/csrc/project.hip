@@ -527,6 +527,51 @@ __device__ __forceinline__ void gather16(const unsigned* __restrict__ Qhl, const
     }
 }
 
+// S_p planes from the packed pair table, 16 columns per thread (the gather of
+// the table-generated GEMM): row a, columns 16 t .. 16 t + 15 of the pitch;
+// columns at or beyond Kp (the pitch padding) are written as zero.
+__global__ __launch_bounds__(256) void gen_planes_kernel(const unsigned* __restrict__ Qhl,
+                                                         const unsigned* __restrict__ cpk, int Ny, int Nz, int Kp,
+                                                         __bf16* __restrict__ hi, __bf16* __restrict__ lo, size_t ld) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t per_row = ld / 16;
+    const size_t a = t / per_row;
+    if (a >= (size_t)Kp) return;
+    const int b0 = (int)(t % per_row) * 16;
+    unsigned w[16];
+    if (b0 < Kp) {
+        const unsigned ca = cpk[a];
+        unsigned c[16];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint4 v = *(const uint4*)(cpk + b0 + 4 * q);
+            c[4 * q] = v.x;
+            c[4 * q + 1] = v.y;
+            c[4 * q + 2] = v.z;
+            c[4 * q + 3] = v.w;
+        }
+        gather16(Qhl, c, ca >> 20, (ca >> 10) & 1023u, ca & 1023u, ca != CELL_NONE, Ny, Nz, w);
+    } else {
+#pragma unroll
+        for (int e = 0; e < 16; e++) w[e] = 0u;
+    }
+#pragma unroll
+    for (int cc = 0; cc < 2; cc++) {
+        const unsigned* v = w + 8 * cc;
+        uint4 h, l;
+        h.x = pair_hi(v[0], v[1]);
+        h.y = pair_hi(v[2], v[3]);
+        h.z = pair_hi(v[4], v[5]);
+        h.w = pair_hi(v[6], v[7]);
+        l.x = pair_lo(v[0], v[1]);
+        l.y = pair_lo(v[2], v[3]);
+        l.z = pair_lo(v[4], v[5]);
+        l.w = pair_lo(v[6], v[7]);
+        *(uint4*)(hi + a * ld + b0 + 8 * cc) = h;
+        *(uint4*)(lo + a * ld + b0 + 8 * cc) = l;
+    }
+}
+
 template <int FM, int FN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void proj_gen_kernel(
     const unsigned* __restrict__ Qhl, const unsigned* __restrict__ cpk, int Ny, int Nz, const __bf16* __restrict__ Bh,
@@ -846,7 +891,7 @@ static int split_planes(opsec_ctx* ctx, const double* X, size_t ldx, int rows, i
 static int g_nt_tile = 2, g_nt_group = 8, g_nt_remap = 0;
 // Stage 1 generates S_p from the pair table (1) or reads S_p planes written by
 // gen_split_kernel (0).
-static int g_nt_gen = 1;
+static int g_nt_gen = 0;
 
 template <typename ASrc, typename Epi>
 static void launch_g3(opsec_ctx* ctx, const ASrc& A, int arows, const __bf16* B, int brows, size_t ld, int K,
@@ -902,9 +947,10 @@ static int run_project_bf16x3(opsec_ctx* ctx, int nb, const int* funcs) {
         if ((rc = split_planes(ctx, ps->d_Bw, ps->ldw, M, N, ps->d_Bp, (int)Mp, Pp))) return rc;
         ps->bp_valid = true;
     }
-    // S_p from the pair table inside the GEMM needs grid indices below 1024
-    const bool gen = g_nt_gen && ctx->sv.n1 < 1024 && ctx->sv.n2 < 1024 && ctx->sv.n3 < 1024;
-    if (gen) {
+    // packed coordinates need grid indices below 1024
+    const bool packable = ctx->sv.n1 < 1024 && ctx->sv.n2 < 1024 && ctx->sv.n3 < 1024;
+    const bool gen = g_nt_gen && packable;
+    if (packable) {
         if ((rc = dev_realloc(ctx, (void**)&ps->d_Qhl, &ps->cap_Qhl, (size_t)ctx->nkeys * 4))) return rc;
         if ((rc = dev_realloc(ctx, (void**)&ps->d_cpk, &ps->cap_cpk, Kp * 4))) return rc;
         hipLaunchKernelGGL(pack_cells_kernel, dim3((unsigned)((Kp + 255) / 256)), dim3(256), 0, ctx->stream,
@@ -918,10 +964,13 @@ static int run_project_bf16x3(opsec_ctx* ctx, int nb, const int* funcs) {
     for (int z = 0; z < nb; z++) {
         const double* Qf = ctx->d_Q + (size_t)funcs[z] * ctx->nkeys;
         hipEventRecord(ev[4 * z], ctx->stream);
-        if (gen)
+        if (packable)
             hipLaunchKernelGGL(pack_table_kernel, dim3((ctx->nkeys + 255) / 256), dim3(256), 0, ctx->stream, Qf,
                                ps->d_Qhl, ctx->nkeys);
-        else
+        if (!gen && packable)
+            hipLaunchKernelGGL(gen_planes_kernel, blocks_for(Kp * Pp / 16), dim3(256), 0, ctx->stream, ps->d_Qhl,
+                               ps->d_cpk, ctx->sv.n2, ctx->sv.n3, (int)Kp, Sh, Sl, Pp);
+        else if (!gen)
             hipLaunchKernelGGL(gen_split_kernel, blocks_for(Kp * Pp / 8), dim3(256), 0, ctx->stream, Qf,
                                ctx->d_cellidx, ctx->sv.n2, ctx->sv.n3, N, Sh, Sl, Pp, (int)Kp);
         hipEventRecord(ev[4 * z + 1], ctx->stream);

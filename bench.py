@@ -99,11 +99,10 @@ def run_path(ctx, L, funcs, quad, prec, trace_prec, lam, y, prior, steps, warmup
 
 
 def stage1_kernel_name(tile, gen):
-    if tile == 2:
-        return ("gemm3_kernel<ASrcTable,EpiPlanes> (stage 1, S_p from the pair table, one launch per band)" if gen
-                else "gemm3_kernel<ASrcPlanes,EpiPlanes> (stage 1, S_p planes, one launch per band)")
     if gen:
-        return "proj_gen_kernel<8,4> (stage 1, S_p from the pair table, one launch per band)"
+        return "gemm3_kernel<ASrcTable,EpiPlanes> (stage 1, S_p from the pair table, one launch per band)"
+    if tile == 2:
+        return "gemm3_kernel<ASrcPlanes,EpiPlanes> (stage 1, S_p planes, one launch per band)"
     return "nt_bf16x3_kernel<EpiPlanes> (stage 1, S_p planes, one launch per band)"
 
 
@@ -229,6 +228,7 @@ def main():
                     help="KL modes (setup, untimed): the periodic box's analytic Fourier modes, or the klt solver")
     ap.add_argument("--nt-gen", type=int, default=0, choices=[0, 1],
                     help="stage 1 generates S_p from the pair table (1) or reads S_p planes (0)")
+    ap.add_argument("--s2-split", type=int, default=4, help="split-K factor of the split-BF16x3 stage 2")
     ap.add_argument("--nt-group", type=int, default=8, help="split-BF16x3 GEMM tile-group edge")
     ap.add_argument("--nt-remap", type=int, default=0, choices=[0, 1], help="deal tile groups to XCDs")
     ap.add_argument("--copy-gib", type=float, default=0.0,
@@ -255,6 +255,7 @@ def main():
     ctx.lib.opsec_internal_set_nt_tile(args.nt_tile)
     ctx.lib.opsec_internal_set_nt_order(args.nt_group, args.nt_remap)
     ctx.lib.opsec_internal_set_nt_gen(args.nt_gen)
+    ctx.lib.opsec_internal_set_stage2_split(args.s2_split)
     ncu, arch = ctx.device_info()
     log(f"device {arch} with {ncu} CUs; workload {args.config}: {N} cells, {nb} bands, {M} modes")
     ctx.model_real(k, P, lo, hi, L.OPSEC_XI_CLOSED)

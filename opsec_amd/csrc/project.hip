@@ -37,6 +37,8 @@ struct ProjState {
     size_t cap_C = 0;
     void* d_P = nullptr;         // packed results, nb x M(M+1)/2 (double)
     size_t cap_P = 0;
+    double* d_Ppart = nullptr;   // split-K partials of one band's stage 2
+    size_t cap_Ppart = 0;
     int nb = 0;                  // bands in d_P
     int prec = 0;
     std::vector<int> funcs;
@@ -329,6 +331,7 @@ struct EpiPlanes {
     __bf16* hi;
     __bf16* lo;
     size_t ld;
+    __device__ __forceinline__ EpiPlanes part(int) const { return *this; }
     __device__ __forceinline__ void put4(int m, int n, const f4_t& v) const {
         typedef __bf16 bf4_t __attribute__((ext_vector_type(4)));
         bf4_t h, l;
@@ -345,10 +348,17 @@ struct EpiPlanes {
     static constexpr bool upper = false;
 };
 
-// D[m][n] -> packed upper triangle, FP64 (m <= n < M).
+// D[m][n] -> packed upper triangle, FP64 (m <= n < M).  With split-K, part s
+// of the K range writes to p + s * part_stride.
 struct EpiPacked {
     double* p;
     int M;
+    size_t part_stride = 0;
+    __device__ __forceinline__ EpiPacked part(int s) const {
+        EpiPacked e = *this;
+        e.p += (size_t)s * part_stride;
+        return e;
+    }
     __device__ __forceinline__ void put4(int m, int n, const f4_t& v) const {
         if (n >= M) return;
 #pragma unroll
@@ -572,111 +582,6 @@ __global__ __launch_bounds__(256) void gen_planes_kernel(const unsigned* __restr
     }
 }
 
-template <int FM, int FN>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void proj_gen_kernel(
-    const unsigned* __restrict__ Qhl, const unsigned* __restrict__ cpk, int Ny, int Nz, const __bf16* __restrict__ Bh,
-    const __bf16* __restrict__ Bl, size_t ldb, int K, int nmb, int nnb, int G, int remap, EpiPlanes E) {
-    constexpr int BM = 32 * FM, BN = 32 * FN;
-    static_assert(BM == 256, "one A row per thread");
-    constexpr int CB = BN * 4 / 256;
-    __shared__ __attribute__((aligned(16))) __bf16 sAh[BM * PK];
-    __shared__ __attribute__((aligned(16))) __bf16 sAl[BM * PK];
-    __shared__ __attribute__((aligned(16))) __bf16 sBh[BN * PK];
-    __shared__ __attribute__((aligned(16))) __bf16 sBl[BN * PK];
-    int mb, nbk;
-    tile_of(blockIdx.x, nmb, nnb, G, remap, mb, nbk);
-    const int m0 = mb * BM, n0 = nbk * BN;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * (16 * FM), wn = (wave & 1) * (16 * FN);
-    const unsigned ca = cpk[m0 + tid];
-    const bool rowv = ca != CELL_NONE;
-    const unsigned xa = ca >> 20, ya = (ca >> 10) & 1023u, za = ca & 1023u;
-    const int kc = (tid & 3) * 8, r0 = tid >> 2;
-    const __bf16* pB = Bh + (size_t)(n0 + r0) * ldb + kc;
-    const __bf16* pBl = Bl + (size_t)(n0 + r0) * ldb + kc;
-    const size_t sb = (size_t)64 * ldb;
-    f4_t acc[FM][FN];
-#pragma unroll
-    for (int i = 0; i < FM; i++)
-#pragma unroll
-        for (int j = 0; j < FN; j++) acc[i][j] = f4_t{0, 0, 0, 0};
-    bf8_t gb[2 * CB];
-    unsigned w[32];
-    auto fetch = [&](int k) {
-#pragma unroll
-        for (int c = 0; c < CB; c++) {
-            gb[c] = *(const bf8_t*)(pB + c * sb + k);
-            gb[CB + c] = *(const bf8_t*)(pBl + c * sb + k);
-        }
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            unsigned c[16];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const uint4 v = *(const uint4*)(cpk + k + 16 * h + 4 * q);
-                c[4 * q] = v.x;
-                c[4 * q + 1] = v.y;
-                c[4 * q + 2] = v.z;
-                c[4 * q + 3] = v.w;
-            }
-            gather16(Qhl, c, xa, ya, za, rowv, Ny, Nz, w + 16 * h);
-        }
-    };
-    fetch(0);
-    const int kq = lane >> 4, fr = lane & 15, kcw = tid & 3;
-    for (int k0 = 0; k0 < K; k0 += PK) {
-        __syncthreads();
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const int o = lds_slot(tid, c);
-            uint4 h, l;
-            h.x = pair_hi(w[8 * c], w[8 * c + 1]);
-            h.y = pair_hi(w[8 * c + 2], w[8 * c + 3]);
-            h.z = pair_hi(w[8 * c + 4], w[8 * c + 5]);
-            h.w = pair_hi(w[8 * c + 6], w[8 * c + 7]);
-            l.x = pair_lo(w[8 * c], w[8 * c + 1]);
-            l.y = pair_lo(w[8 * c + 2], w[8 * c + 3]);
-            l.z = pair_lo(w[8 * c + 4], w[8 * c + 5]);
-            l.w = pair_lo(w[8 * c + 6], w[8 * c + 7]);
-            *(uint4*)&sAh[o] = h;
-            *(uint4*)&sAl[o] = l;
-        }
-#pragma unroll
-        for (int c = 0; c < CB; c++) {
-            const int o = lds_slot(r0 + 64 * c, kcw);
-            *(bf8_t*)&sBh[o] = gb[c];
-            *(bf8_t*)&sBl[o] = gb[CB + c];
-        }
-        __syncthreads();
-        if (k0 + PK < K) fetch(k0 + PK);
-        bf8_t bh[FN], bl[FN];
-#pragma unroll
-        for (int j = 0; j < FN; j++) {
-            const int o = lds_slot(wn + j * 16 + fr, kq);
-            bh[j] = *(const bf8_t*)&sBh[o];
-            bl[j] = *(const bf8_t*)&sBl[o];
-        }
-#pragma unroll
-        for (int i = 0; i < FM; i++) {
-            const int o = lds_slot(wm + i * 16 + fr, kq);
-            const bf8_t ah = *(const bf8_t*)&sAh[o];
-            const bf8_t al = *(const bf8_t*)&sAl[o];
-            // three sweeps over independent accumulators (lo*hi, hi*lo, hi*hi):
-            // consecutive MFMAs never wait on each other's result
-#pragma unroll
-            for (int j = 0; j < FN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-            for (int j = 0; j < FN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-            for (int j = 0; j < FN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < FM; i++)
-#pragma unroll
-        for (int j = 0; j < FN; j++) E.put4(m0 + wm + i * 16 + 4 * (lane >> 4), n0 + wn + j * 16 + fr, acc[i][j]);
-}
-
 // ---- 256 x 256 split-BF16x3 GEMM, double-buffered LDS ------------------------------
 //
 // D = A B^T with B from plane pairs and A either from plane pairs (ASrcPlanes)
@@ -769,10 +674,13 @@ struct ASrcTable {
 template <typename ASrc, typename Epi>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void gemm3_kernel(
     ASrc A, const __bf16* __restrict__ Bh, const __bf16* __restrict__ Bl, size_t ldb, int K, int nmb, int nnb, int G,
-    int remap, Epi E) {
+    int remap, int ksplit, Epi E0) {
     extern __shared__ __attribute__((aligned(16))) __bf16 g3_lds[];   // 2 stages x (Ah, Al, Bh, Bl)
+    // split-K: workgroup b takes part b % ksplit of the K steps of tile b / ksplit
+    const int split = blockIdx.x % ksplit;
+    const Epi E = E0.part(split);
     int mb, nbk;
-    tile_of(blockIdx.x, nmb, nnb, G, remap, mb, nbk);
+    tile_of(blockIdx.x / ksplit, nmb, nnb, G, remap, mb, nbk);
     const int m0 = mb * G3, n0 = nbk * G3;
     if (Epi::upper && m0 > n0 + G3 - 1) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -804,15 +712,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void g
     for (int i = 0; i < 8; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = f4_t{0, 0, 0, 0};
-    A.fetch(m0, 0, ra);
-    fetchB(0);
+    // balanced K ranges: every part has at least one step (the host keeps
+    // ksplit <= K / PK)
+    const int nkall = K / PK;
+    const int kt0 = (int)((long long)split * nkall / ksplit), nk = (int)((long long)(split + 1) * nkall / ksplit);
+    A.fetch(m0, kt0 * PK, ra);
+    fetchB(kt0 * PK);
     A.store(stage(0, 0), stage(0, 1), ra);
     storeB(stage(0, 2), stage(0, 3));
     __syncthreads();
     const int kq = lane >> 4, fr = lane & 15;
-    const int nk = K / PK;
-    for (int kt = 0; kt < nk; kt++) {
-        const int cur = kt & 1;
+    for (int kt = kt0; kt < nk; kt++) {
+        const int cur = (kt - kt0) & 1;
         const bool more = kt + 1 < nk;
         if (more) {
             A.fetch(m0, (kt + 1) * PK, ra);
@@ -857,6 +768,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void g
 
 constexpr size_t G3_LDS_BYTES = (size_t)8 * G3_STAGE * sizeof(__bf16);   // 128 KB
 
+// out[i] = sum_s part[s][i] in a fixed order (split-K reduction of stage 2)
+__global__ void sum_parts_kernel(const double* __restrict__ part, int nparts, size_t n, double* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int s = 0; s < nparts; s++) acc += part[(size_t)s * n + i];
+    out[i] = acc;
+}
+
 // B' = diag(w) B with w = sqrt(Nbar)
 __global__ void scale_rows_kernel(const double* B, const double* nbar, int N, int M, double* Bw, size_t ldw) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -892,10 +812,12 @@ static int g_nt_tile = 2, g_nt_group = 8, g_nt_remap = 0;
 // Stage 1 generates S_p from the pair table (1) or reads S_p planes written by
 // gen_split_kernel (0).
 static int g_nt_gen = 0;
+// Split-K factor of stage 2 on the 256 x 256 kernel.
+static int g_s2_split = 4;
 
 template <typename ASrc, typename Epi>
 static void launch_g3(opsec_ctx* ctx, const ASrc& A, int arows, const __bf16* B, int brows, size_t ld, int K,
-                      const Epi& E) {
+                      const Epi& E, int ksplit = 1) {
     static bool attr = false;
     if (!attr) {   // 128 KB of dynamic LDS per workgroup
         (void)hipFuncSetAttribute((const void*)gemm3_kernel<ASrcPlanes, EpiPlanes>,
@@ -907,8 +829,9 @@ static void launch_g3(opsec_ctx* ctx, const ASrc& A, int arows, const __bf16* B,
         attr = true;
     }
     const int nmb = arows / G3, nnb = brows / G3;
-    hipLaunchKernelGGL((gemm3_kernel<ASrc, Epi>), dim3(nmb * nnb), dim3(512), G3_LDS_BYTES, ctx->stream, A, B,
-                       B + (size_t)brows * ld, ld, K, nmb, nnb, g_nt_group, g_nt_remap, E);
+    ksplit = std::max(1, std::min(ksplit, K / PK));
+    hipLaunchKernelGGL((gemm3_kernel<ASrc, Epi>), dim3(nmb * nnb * ksplit), dim3(512), G3_LDS_BYTES, ctx->stream, A,
+                       B, B + (size_t)brows * ld, ld, K, nmb, nnb, g_nt_group, g_nt_remap, ksplit, E);
 }
 
 // C = A B^T (stage 1, epilogue to split planes) or P = A B^T (stage 2, packed).
@@ -942,6 +865,9 @@ static int run_project_bf16x3(opsec_ctx* ctx, int nb, const int* funcs) {
         if ((rc = dev_realloc(ctx, (void**)&ps->d_Sp, &ps->cap_Sp, 2 * Kp * Pp * sizeof(__bf16)))) return rc;
     if ((rc = dev_realloc(ctx, (void**)&ps->d_Cp, &ps->cap_Cp, 2 * Mp * Pp * sizeof(__bf16)))) return rc;
     if ((rc = dev_realloc(ctx, &ps->d_P, &ps->cap_P, (size_t)nb * packed_size(M) * 8))) return rc;
+    if (g_s2_split > 1 &&
+        (rc = dev_realloc(ctx, (void**)&ps->d_Ppart, &ps->cap_Ppart, (size_t)g_s2_split * packed_size(M) * 8)))
+        return rc;
     if (!ps->bp_valid) {
         // B' is column-major N x M: row n of the plane is mode n, cells contiguous
         if ((rc = split_planes(ctx, ps->d_Bw, ps->ldw, M, N, ps->d_Bp, (int)Mp, Pp))) return rc;
@@ -974,20 +900,25 @@ static int run_project_bf16x3(opsec_ctx* ctx, int nb, const int* funcs) {
             hipLaunchKernelGGL(gen_split_kernel, blocks_for(Kp * Pp / 8), dim3(256), 0, ctx->stream, Qf,
                                ctx->d_cellidx, ctx->sv.n2, ctx->sv.n3, N, Sh, Sl, Pp, (int)Kp);
         hipEventRecord(ev[4 * z + 1], ctx->stream);
-        if (gen && g_nt_tile == 2) {
+        if (gen) {
             launch_g3(ctx, ASrcTable{ps->d_Qhl, ps->d_cpk, ctx->sv.n2, ctx->sv.n3}, (int)Kp, ps->d_Bp, (int)Mp, Pp,
                       (int)Kp, toC);
-        } else if (gen) {
-            const int nmb = (int)(Kp / 256), nnb = (int)(Mp / 128);
-            hipLaunchKernelGGL((proj_gen_kernel<8, 4>), dim3(nmb * nnb), dim3(256), 0, ctx->stream, ps->d_Qhl,
-                               ps->d_cpk, ctx->sv.n2, ctx->sv.n3, ps->d_Bp, ps->d_Bp + Mp * Pp, Pp, (int)Kp, nmb, nnb,
-                               g_nt_group, g_nt_remap, toC);
         } else {
             launch_nt(ctx, ps->d_Sp, (int)Kp, ps->d_Bp, (int)Mp, Pp, (int)Kp, toC);
         }
         hipEventRecord(ev[4 * z + 2], ctx->stream);
-        EpiPacked toP{(double*)ps->d_P + (size_t)z * packed_size(M), M};
-        launch_nt(ctx, ps->d_Bp, (int)Mp, ps->d_Cp, (int)Mp, Pp, (int)Kp, toP);
+        double* Pz = (double*)ps->d_P + (size_t)z * packed_size(M);
+        const int s2 = std::max(1, std::min(g_s2_split, (int)(Kp / PK)));
+        if (g_nt_tile >= 2 && s2 > 1) {
+            const size_t np = (size_t)packed_size(M);
+            EpiPacked toP{ps->d_Ppart, M, np};
+            launch_g3(ctx, ASrcPlanes{ps->d_Bp, ps->d_Bp + Mp * Pp, Pp}, (int)Mp, ps->d_Cp, (int)Mp, Pp, (int)Kp, toP,
+                      s2);
+            hipLaunchKernelGGL(sum_parts_kernel, blocks_for(np), dim3(256), 0, ctx->stream, ps->d_Ppart, s2, np, Pz);
+        } else {
+            EpiPacked toP{Pz, M};
+            launch_nt(ctx, ps->d_Bp, (int)Mp, ps->d_Cp, (int)Mp, Pp, (int)Kp, toP);
+        }
         hipEventRecord(ev[4 * z + 3], ctx->stream);
     }
     hipError_t e = hipGetLastError();
@@ -1094,6 +1025,7 @@ void opsec_internal_free_proj(opsec_ctx* ctx) {
     dev_free((void**)&ctx->proj->d_Sp);
     dev_free((void**)&ctx->proj->d_Cp);
     dev_free((void**)&ctx->proj->d_Qhl);
+    dev_free((void**)&ctx->proj->d_Ppart);
     dev_free((void**)&ctx->proj->d_cpk);
     delete ctx->proj;
     ctx->proj = nullptr;
@@ -1164,6 +1096,14 @@ int opsec_internal_project_times(opsec_ctx* ctx, float* s1, float* s2) {
 int opsec_internal_set_nt_tile(int v) {
     if (v < 0 || v > 2) return OPSEC_EINVAL;
     g_nt_tile = v;
+    return OPSEC_OK;
+}
+
+// Split-K factor of the split-BF16x3 stage 2 (its upper-triangle tiles alone
+// fill about half of the CUs), for tuning.
+int opsec_internal_set_stage2_split(int v) {
+    if (v < 1 || v > 16) return OPSEC_EINVAL;
+    g_s2_split = v;
     return OPSEC_OK;
 }
 

@@ -908,6 +908,136 @@ int opsec_assemble_device(opsec_ctx* ctx, int f, double* dS, int ld) {
 }
 
 // Drop-in for ComputeSignalMatrixBlock (box surveys).
+// Spherical signal block (src/sig.cpp:125-213): cells are grouped into arcs
+// h = d_r * Nmu + d_mu (cell order within an arc).  For each (row arc, column
+// arc) the first (row, column) pair met with a given k = |d_phi_a - d_phi_b|
+// is integrated by the adaptive rule (lower cell index first) and reused for
+// every pair of the two arcs with that k.  The distinct integrals run as one
+// batch of wavefront jobs; the assembly is a gather on the device.
+__global__ void sph_assemble_kernel(const int* __restrict__ rarc, const int* __restrict__ rphi,
+                                    const double* __restrict__ rnb, int nrows, const int* __restrict__ carc,
+                                    const int* __restrict__ cphi, const double* __restrict__ cnb, int ncols,
+                                    const int* __restrict__ table, int ncarcs, int Nphi, const double* __restrict__ Q,
+                                    double* __restrict__ out, size_t lld, int row0) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = row0 + blockIdx.y;
+    if (j >= ncols || i >= nrows) return;
+    const int k = abs(rphi[i] - cphi[j]);
+    const int id = table[((size_t)rarc[i] * ncarcs + carc[j]) * Nphi + k];
+    out[(size_t)i * lld + j] = sqrt(rnb[i] * cnb[j]) * Q[id];
+}
+
+static int sig_block_sph(opsec_ctx* ctx, const opsec_cell* cells, int ncells, const int* rows, int nrows,
+                         const int* cols, int ncols, int f, const SurveyDev& svd, const opsec_quad* q, double* s,
+                         int lld) {
+    OPSEC_CHECK_ARG(ctx, q->method == OPSEC_QUAD_GM,
+                    "spherical surveys use the adaptive (GM) rule; the tent reduction needs box cells");
+    const int Nmu = svd.n2, Nphi = svd.n3, narcs = svd.n1 * svd.n2;
+    std::vector<int> rg(ncells, -1), cg(ncells, -1);
+    for (int i = 0; i < nrows; i++) rg[rows[i]] = i;
+    for (int j = 0; j < ncols; j++) cg[cols[j]] = j;
+    std::vector<std::vector<int>> arcs(narcs);
+    for (int a = 0; a < ncells; a++) arcs[cells[a].d1 * Nmu + cells[a].d2].push_back(a);
+    // arcs holding rows / columns of the block
+    std::vector<int> rai(narcs, -1), cai(narcs, -1), rlist, clist;
+    for (int h = 0; h < narcs; h++) {
+        bool hr = false, hc = false;
+        for (int a : arcs[h]) {
+            hr = hr || rg[a] >= 0;
+            hc = hc || cg[a] >= 0;
+        }
+        if (hr) {
+            rai[h] = (int)rlist.size();
+            rlist.push_back(h);
+        }
+        if (hc) {
+            cai[h] = (int)clist.size();
+            clist.push_back(h);
+        }
+    }
+    const size_t tsize = rlist.size() * clist.size() * (size_t)Nphi;
+    std::vector<int> table(tsize, -1);
+    std::vector<PairGeo> geo;
+    std::vector<int> keys;
+    for (size_t x = 0; x < rlist.size(); x++)
+        for (size_t y = 0; y < clist.size(); y++) {
+            int* tab = &table[(x * clist.size() + y) * Nphi];
+            int left = Nphi;
+            for (int a : arcs[rlist[x]]) {
+                if (rg[a] < 0) continue;
+                for (int b : arcs[clist[y]]) {
+                    if (cg[b] < 0) continue;
+                    const int k = std::abs(cells[a].d3 - cells[b].d3);
+                    if (tab[k] >= 0) continue;
+                    tab[k] = (int)keys.size();
+                    keys.push_back((int)keys.size());
+                    geo.push_back(pair_geo(cells[std::min(a, b)], cells[std::max(a, b)]));
+                    if (--left == 0) break;
+                }
+                if (left == 0) break;
+            }
+        }
+    const int nd = (int)keys.size();
+    std::vector<int> rarc(nrows), rphi(nrows), carc(ncols), cphi(ncols);
+    std::vector<double> rnb(nrows), cnb(ncols);
+    for (int i = 0; i < nrows; i++) {
+        const opsec_cell& c = cells[rows[i]];
+        rarc[i] = rai[c.d1 * Nmu + c.d2];
+        rphi[i] = c.d3;
+        rnb[i] = c.Nbar;
+    }
+    for (int j = 0; j < ncols; j++) {
+        const opsec_cell& c = cells[cols[j]];
+        carc[j] = cai[c.d1 * Nmu + c.d2];
+        cphi[j] = c.d3;
+        cnb[j] = c.Nbar;
+    }
+    SurveyDev saved = ctx->sv;
+    ctx->sv = svd;
+    Scratch sQ, sT, sR, sC, sRn, sCn, sOut;
+    int rc;
+    const size_t nfun = ctx->funcs.size();
+    hipStream_t st = ctx->stream;
+    std::vector<int> nearflag(nd, 0);
+    int fs[1] = {f};
+    if ((rc = dev_realloc(ctx, &sQ.p, &sQ.cap, std::max<size_t>(1, nfun * nd) * 8))) goto out;
+    if ((rc = dev_realloc(ctx, &sT.p, &sT.cap, std::max<size_t>(1, tsize) * 4))) goto out;
+    if ((rc = dev_realloc(ctx, &sR.p, &sR.cap, (size_t)nrows * 8))) goto out;
+    if ((rc = dev_realloc(ctx, &sC.p, &sC.cap, (size_t)ncols * 8))) goto out;
+    if ((rc = dev_realloc(ctx, &sRn.p, &sRn.cap, (size_t)nrows * 8))) goto out;
+    if ((rc = dev_realloc(ctx, &sCn.p, &sCn.cap, (size_t)ncols * 8))) goto out;
+    if ((rc = dev_realloc(ctx, &sOut.p, &sOut.cap, (size_t)nrows * lld * 8))) goto out;
+    rc = run_quadrature(ctx, 1, fs, keys, geo, nearflag, q, (double*)sQ.p, (size_t)nd);
+    if (rc) goto out;
+    {
+        hipError_t e = hipMemcpyAsync(sT.p, table.data(), tsize * 4, hipMemcpyHostToDevice, st);
+        int* dR = (int*)sR.p;
+        int* dC = (int*)sC.p;
+        if (e == hipSuccess) e = hipMemcpyAsync(dR, rarc.data(), (size_t)nrows * 4, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dR + nrows, rphi.data(), (size_t)nrows * 4, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dC, carc.data(), (size_t)ncols * 4, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dC + ncols, cphi.data(), (size_t)ncols * 4, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(sRn.p, rnb.data(), (size_t)nrows * 8, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(sCn.p, cnb.data(), (size_t)ncols * 8, hipMemcpyHostToDevice, st);
+        for (int r0 = 0; e == hipSuccess && r0 < nrows; r0 += 32768) {
+            const int nr = std::min(32768, nrows - r0);
+            hipLaunchKernelGGL(sph_assemble_kernel, dim3((ncols + 255) / 256, nr), dim3(256), 0, st, dR, dR + nrows,
+                               (const double*)sRn.p, nrows, dC, dC + ncols, (const double*)sCn.p, ncols,
+                               (const int*)sT.p, (int)clist.size(), Nphi, (const double*)sQ.p + (size_t)f * nd,
+                               (double*)sOut.p, (size_t)lld, r0);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess)
+            e = hipMemcpy2DAsync(s, (size_t)lld * 8, sOut.p, (size_t)lld * 8, (size_t)ncols * 8, nrows,
+                                 hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = set_error(ctx, OPSEC_EHIP, "spherical block: %s", hipGetErrorString(e));
+    }
+out:
+    ctx->sv = saved;
+    return rc;
+}
+
 int opsec_sig_block(opsec_ctx* ctx, const opsec_cell* cells, int ncells, const int* rows, int nrows, const int* cols,
                     int ncols, int f, const opsec_survey* sv, const opsec_quad* q, double* s, int lld) {
     if (!ctx) return OPSEC_EINVAL;
@@ -922,7 +1052,6 @@ int opsec_sig_block(opsec_ctx* ctx, const opsec_cell* cells, int ncells, const i
     OPSEC_CHECK_ARG(ctx, rows && cols && s, "NULL rows/cols/output");
     SurveyDev svd;
     if ((rc = survey_from_abi(ctx, sv, &svd))) return rc;
-    OPSEC_CHECK_ARG(ctx, svd.coordsys == OPSEC_COORD_CARTESIAN, "opsec_sig_block: box surveys only");
     for (int i = 0; i < nrows; i++) OPSEC_CHECK_ARG(ctx, rows[i] >= 0 && rows[i] < ncells, "row index out of range");
     for (int j = 0; j < ncols; j++) OPSEC_CHECK_ARG(ctx, cols[j] >= 0 && cols[j] < ncells, "column index out of range");
     for (int a = 0; a < ncells; a++) {
@@ -931,6 +1060,8 @@ int opsec_sig_block(opsec_ctx* ctx, const opsec_cell* cells, int ncells, const i
                         "cell %d has grid index outside the survey grid", a);
     }
     OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (svd.coordsys == OPSEC_COORD_SPHERICAL)
+        return sig_block_sph(ctx, cells, ncells, rows, nrows, cols, ncols, f, svd, q, s, lld);
     SurveyDev saved = ctx->sv;
     ctx->sv = svd;
     const int nk = svd.n1 * svd.n2 * svd.n3;

@@ -415,3 +415,15 @@ def survey_box(n, lo=0.0, hi=400.0, L=None, ny=None, nz=None):
     s.max1 = s.max2 = s.max3 = hi
     s.L = (hi - lo) if L is None else L
     return s
+
+
+def survey_sph(nr, nmu, nphi, rmin=200.0, rmax=1000.0, mumin=0.5, mumax=1.0, phimin=0.0, phimax=2 * 3.141592653589793):
+    """Spherical survey descriptor (SphericalSurvey keys RMin..PhiMax, Nr, Nmu, Nphi)."""
+    s = Survey()
+    s.coordsys = OPSEC_COORD_SPHERICAL
+    s.n1, s.n2, s.n3 = nr, nmu, nphi
+    s.min1, s.max1 = rmin, rmax
+    s.min2, s.max2 = mumin, mumax
+    s.min3, s.max3 = phimin, phimax
+    s.L = 0.0
+    return s

@@ -12,6 +12,32 @@ import numpy as np
 from .lib import CELL_DTYPE
 
 
+def shell_cells(nr, nmu, nphi, rmin=200.0, rmax=1000.0, mumin=0.5, mumax=1.0, phimin=0.0, phimax=2 * np.pi,
+                nbar=lambda r: 5e-4 * np.exp(-(r / 600.0) ** 2)):
+    """Spherical-shell cells (d1, d2, d3) = (d_r, d_mu, d_phi) in the
+    reference's order (r slowest, phi fastest), radial edges uniform in r;
+    Nbar = nbar(r at the cell's volume-weighted mid radius) x V."""
+    re = np.linspace(rmin, rmax, nr + 1)
+    me = np.linspace(mumin, mumax, nmu + 1)
+    pe = np.linspace(phimin, phimax, nphi + 1)
+    cells = np.zeros(nr * nmu * nphi, dtype=CELL_DTYPE)
+    a = 0
+    for i in range(nr):
+        for j in range(nmu):
+            for k in range(nphi):
+                c = cells[a]
+                c["a"], c["d1"], c["d2"], c["d3"] = a, i, j, k
+                c["min1"], c["max1"] = re[i], re[i + 1]
+                c["min2"], c["max2"] = me[j], me[j + 1]
+                c["min3"], c["max3"] = pe[k], pe[k + 1]
+                V = (re[i + 1] ** 3 - re[i] ** 3) / 3 * (me[j + 1] - me[j]) * (pe[k + 1] - pe[k])
+                rm = ((re[i] ** 3 + re[i + 1] ** 3) / 2) ** (1 / 3)
+                c["Veff"] = V
+                c["Nbar"] = nbar(rm) * V
+                a += 1
+    return cells
+
+
 def box_cells(n, lo=0.0, hi=400.0, nbar=3e-4):
     d = np.arange(n)
     g1, g2, g3 = np.meshgrid(d, d, d, indexing="ij")

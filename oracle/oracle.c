@@ -235,6 +235,9 @@ static double sph_sep(double r1, double mu1, double ph1, double r2, double mu2, 
     return sqrt(fmax(r1 * r1 + r2 * r2 - 2 * r1 * r2 * cg, 0.0));
 }
 
+/* src/SphericalSurvey.cpp:19-23 */
+double orc_sph_r(const double* p1, const double* p2) { return sph_sep(p1[0], p1[1], p1[2], p2[0], p2[1], p2[2]); }
+
 /* ------------------------------------------------------------------------ */
 /* Genz-Malik 7/5 rule and the adaptive driver                               */
 
@@ -1000,4 +1003,88 @@ void orc_gemm_rows(int R, int n, const double* S, int m, const double* B, double
             C[(size_t)j * R + a] = acc;
         }
     }
+}
+
+
+/* ------------------------------------------------------------------------ */
+/* spherical signal block (src/sig.cpp:125-213)                              */
+/* Cells are grouped into arcs h = d_r * Nmu + d_mu (cell order within an arc);
+ * for each (row arc, column arc) the first (row, column) pair met with a given
+ * |d_phi_a - d_phi_b| is integrated (lower cell index first) and reused for
+ * every pair of the two arcs with that |d_phi| difference. */
+int orc_sig_block_sph(const orc_xi* f, const orc_cell* cells, int ncells, int Nr, int Nmu, int Nphi,
+                      const int* rows, int nrows, const int* cols, int ncols, double* s, int lld, double epsrel,
+                      double epsabs, unsigned maxeval) {
+    if (lld <= 0) lld = ncols;
+    if (lld < ncols) return -1;
+    const double unset = -1e100;
+    int* rg = malloc(sizeof(int) * ncells);
+    int* cg = malloc(sizeof(int) * ncells);
+    for (int a = 0; a < ncells; a++) rg[a] = cg[a] = -1;
+    for (int i = 0; i < nrows; i++) rg[rows[i]] = i;
+    for (int j = 0; j < ncols; j++) cg[cols[j]] = j;
+    const int narcs = Nr * Nmu;
+    int* count = calloc(narcs, sizeof(int));
+    for (int a = 0; a < ncells; a++) count[cells[a].d1 * Nmu + cells[a].d2]++;
+    int* start = malloc(sizeof(int) * (narcs + 1));
+    start[0] = 0;
+    for (int h = 0; h < narcs; h++) start[h + 1] = start[h] + count[h];
+    int* members = malloc(sizeof(int) * ncells);
+    int* fill = calloc(narcs, sizeof(int));
+    for (int a = 0; a < ncells; a++) {
+        int h = cells[a].d1 * Nmu + cells[a].d2;
+        members[start[h] + fill[h]++] = a;
+    }
+    double* cache = malloc(sizeof(double) * Nphi);
+    for (int ha = 0; ha < narcs; ha++)
+        for (int hb = 0; hb < narcs; hb++) {
+            for (int k = 0; k < Nphi; k++) cache[k] = unset;
+            for (int ia = start[ha]; ia < start[ha + 1]; ia++) {
+                int a = members[ia], i = rg[a];
+                if (i < 0) continue;
+                for (int ib = start[hb]; ib < start[hb + 1]; ib++) {
+                    int b = members[ib], j = cg[b];
+                    if (j < 0) continue;
+                    int k = abs(cells[a].d3 - cells[b].d3);
+                    if (cache[k] == unset)
+                        cache[k] = (a <= b) ? orc_gm_integrate(f, 2, 0.0, &cells[a], &cells[b], epsrel, epsabs, maxeval, NULL, NULL)
+                                            : orc_gm_integrate(f, 2, 0.0, &cells[b], &cells[a], epsrel, epsabs, maxeval, NULL, NULL);
+                    s[(size_t)i * lld + j] = sqrt(cells[a].Nbar * cells[b].Nbar) * cache[k];
+                }
+            }
+        }
+    free(rg);
+    free(cg);
+    free(count);
+    free(start);
+    free(members);
+    free(fill);
+    free(cache);
+    return 0;
+}
+
+/* Brute-force tensor Gauss-Legendre (n points per dimension) of the
+ * spherical cell-pair mean, in the volume-uniform variables of
+ * src/sig.cpp:436-462 (an independent check of the adaptive rule). */
+double orc_brute6_sph(const orc_xi* f, const orc_cell* c1, const orc_cell* c2, int n) {
+    double x[64], w[64];
+    orc_gauss_legendre(n, x, w);
+    double lo[6] = {c1->min1 * c1->min1 * c1->min1, c2->min1 * c2->min1 * c2->min1, c1->min2, c2->min2, c1->min3, c2->min3};
+    double hi[6] = {c1->max1 * c1->max1 * c1->max1, c2->max1 * c2->max1 * c2->max1, c1->max2, c2->max2, c1->max3, c2->max3};
+    double acc = 0;
+    long tot = 1;
+    for (int d = 0; d < 6; d++) tot *= n;
+    for (long t = 0; t < tot; t++) {
+        long q = t;
+        double wt = 1, v[6];
+        for (int d = 0; d < 6; d++) {
+            int id = (int)(q % n);
+            q /= n;
+            double u = 0.5 * (x[id] + 1);
+            wt *= 0.5 * w[id];
+            v[d] = (1 - u) * lo[d] + u * hi[d];
+        }
+        acc += wt * orc_xi_eval(f, sph_sep(cbrt(v[0]), v[2], v[4], cbrt(v[1]), v[3], v[5]));
+    }
+    return acc;
 }

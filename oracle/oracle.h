@@ -15,6 +15,12 @@
  *                                      closed form (tests/testsig.cpp:41-66)
  *   orc_gauss_jacobi10                 scipy.special.roots_jacobi
  *   orc_sig_block                      oracle/_ref ComputeSignalMatrixBlock
+ *   orc_sph_r                          law-of-cosines known answers (same ray, same radius,
+ *                                      antipodes); the reference's SphericalSurvey needs
+ *                                      cfitsio and is not built
+ *   orc_gm_integrate (spherical)       xi=1 known answer; orc_brute6_sph on smooth xi
+ *   orc_sig_block_sph                  xi=1 (S = sqrt(Nbar Nbar)); every element against its
+ *                                      own pair integral (phi-rotation invariance)
  *   orc_project                        brute force triple loop + B^T S B = diag(lambda)
  *   orc_estimate                       hand-derived diagonal cases, W row sums,
  *                                      inverse/sqrtm known answers (tests/testMyMatrix.cpp:67-104)
@@ -62,6 +68,8 @@ double orc_xi_eval(const orc_xi* f, double r);
 
 /* ---- geometry ---- */
 double orc_box_r(const double* p1, const double* p2, double L);
+/* spherical separation, points (r, mu, phi) (src/SphericalSurvey.cpp:19-23) */
+double orc_sph_r(const double* p1, const double* p2);
 
 /* ---- pair integrals ---- */
 /* Genz-Malik 7/5 adaptive cubature of the cell-pair mean with the reference's
@@ -80,6 +88,12 @@ void orc_gauss_jacobi10(int n, double* t, double* w);
 
 /* Signal-matrix block with the translation cache (src/sig.cpp:82-123).
  * method 0: GM (epsrel, epsabs, maxeval), 1: GAUSS (nodes, near). */
+/* spherical block with the arc cache (src/sig.cpp:125-213); Genz-Malik pairs */
+int orc_sig_block_sph(const orc_xi* f, const orc_cell* cells, int ncells, int Nr, int Nmu, int Nphi,
+                      const int* rows, int nrows, const int* cols, int ncols, double* s, int lld, double epsrel,
+                      double epsabs, unsigned maxeval);
+/* tensor Gauss-Legendre (n per dimension) of a spherical cell-pair mean */
+double orc_brute6_sph(const orc_xi* f, const orc_cell* c1, const orc_cell* c2, int n);
 int orc_sig_block(const orc_xi* f, const orc_cell* cells, int ncells, int Nx, int Ny, int Nz, double L,
                   const int* rows, int nrows, const int* cols, int ncols, double* s, int lld, int method,
                   double epsrel, double epsabs, unsigned maxeval, int nodes, int nearc);

@@ -31,6 +31,9 @@ _ORC = {
     "orc_band_param": ([_I, _P, _P, _D, _D], _D),
     "orc_xi_eval": ([C.POINTER(OrcXi), _D], _D),
     "orc_box_r": ([_P, _P, _D], _D),
+    "orc_sph_r": ([_P, _P], _D),
+    "orc_sig_block_sph": ([C.POINTER(OrcXi), _P, _I, _I, _I, _I, _P, _I, _P, _I, _P, _I, _D, _D, C.c_uint], _I),
+    "orc_brute6_sph": ([C.POINTER(OrcXi), _P, _P, _I], _D),
     "orc_gm_integrate": ([C.POINTER(OrcXi), _I, _D, _P, _P, _D, _D, C.c_uint, C.POINTER(_D),
                           C.POINTER(C.c_int64)], _D),
     "orc_tent_gauss": ([C.POINTER(OrcXi), _D, _P, _P, _I], _D),

@@ -182,3 +182,35 @@ def test_project_bf16x3_generated_layouts(ctx, n, shuffle):
     err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
     print("generated-S_p BF16x3 projection max rel err per function", err)
     assert err.max() < 5e-5
+
+
+@pytest.mark.parametrize("tile,gen", [(1, 0), (2, 0), (2, 1)])
+def test_project_bf16x3_gemm_variants(ctx, tile, gen):
+    """The split-BF16x3 GEMM variants (256 x 128; 256 x 256 double-buffered,
+    with S_p planes or S_p generated from the pair table), with split-K in
+    stage 2, against the FP64 projection of the same pair tables."""
+    n = 10
+    cells = synth.box_cells(n, 0.0, 250.0)
+    sv = lib.survey_box(n, 0.0, 250.0)
+    blo, bhi = synth.linear_bands(2, 0.02, 0.2)
+    ctx.model_real(GOLD["pk_k"], GOLD["pk_P"], blo, bhi, lib.OPSEC_XI_CLOSED)
+    ctx.cells_upload(cells, sv)
+    funcs = [0, 1, 2]
+    ctx.pair_tables(funcs, ctx.quad(lib.OPSEC_QUAD_GAUSS, nodes=4, near=1))
+    N = len(cells)
+    m = 300
+    B = np.linalg.qr(np.random.default_rng(11).normal(size=(N, m)))[0].T.copy()
+    ctx.modes_upload(B)
+    ref = ctx.project(funcs, lib.OPSEC_PREC_FP64)
+    try:
+        assert ctx.lib.opsec_internal_set_nt_tile(tile) == 0
+        assert ctx.lib.opsec_internal_set_nt_gen(gen) == 0
+        assert ctx.lib.opsec_internal_set_stage2_split(3) == 0
+        got = ctx.project(funcs, lib.OPSEC_PREC_BF16X3)
+    finally:
+        ctx.lib.opsec_internal_set_nt_tile(2)
+        ctx.lib.opsec_internal_set_nt_gen(0)
+        ctx.lib.opsec_internal_set_stage2_split(4)
+    err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
+    print("tile", tile, "gen", gen, "max rel err per function", err)
+    assert err.max() < 5e-5

@@ -159,3 +159,29 @@ def test_gauss_pair_tables_c2_shape(c1):
         # permutation symmetry of the cubic grid
         np.testing.assert_allclose(Q, Q.transpose(1, 0, 2), rtol=1e-12, atol=1e-15 * np.abs(Q).max())
         np.testing.assert_allclose(Q, Q.transpose(2, 1, 0), rtol=1e-12, atol=1e-15 * np.abs(Q).max())
+
+
+def test_spherical_sig_block_vs_oracle(ctx):
+    """Spherical shell: the device arc-cache block (GM pairs) against the
+    oracle's restatement of ComputeSignalMatrixBlockS, on row/column subsets
+    with a padded stride, for the closed-form band and the test Gaussian."""
+    nr, nmu, nphi = 3, 2, 6
+    cells = synth.shell_cells(nr, nmu, nphi, rmin=250.0, rmax=550.0)
+    N = len(cells)
+    sv = lib.survey_sph(nr, nmu, nphi, 250.0, 550.0)
+    rows = np.array([0, 5, 6, 13, 20, 35], dtype=np.int32)
+    cols = np.array([1, 2, 3, 7, 11, 12, 18, 24, 29, 30, 33, 35], dtype=np.int32)
+    lld = len(cols) + 3
+    orc = O.oracle()
+    for kind, setup in (("band", lambda: ctx.model_real(GOLD["pk_k"], GOLD["pk_P"], [0.02], [0.06], lib.OPSEC_XI_CLOSED)),
+                        ("gauss", lambda: ctx.model_test(1, 2.5, 1.0 / (2 * 120.0 ** 2)))):
+        setup()
+        f = 1 if kind == "band" else 0
+        xi = O.XiHandle(1, a=0.02, b=0.06) if kind == "band" else O.XiHandle(3, a=2.5, b=1.0 / (2 * 120.0 ** 2))
+        q = ctx.quad(lib.OPSEC_QUAD_GM)
+        got = ctx.sig_block(cells, rows, cols, f, sv, q, lld=lld)
+        ref = np.zeros((len(rows), lld))
+        assert orc.orc_sig_block_sph(C.byref(xi.s), cells.ctypes.data, N, nr, nmu, nphi, O.ptr(rows), len(rows),
+                                     O.ptr(cols), len(cols), O.ptr(ref), lld, q.epsrel, q.epsabs, q.maxeval) == 0
+        np.testing.assert_allclose(got[:, :len(cols)], ref[:, :len(cols)], rtol=1e-12,
+                                   atol=1e-13 * np.abs(ref).max(), err_msg=kind)

@@ -228,7 +228,7 @@ def main():
                     help="KL modes (setup, untimed): the periodic box's analytic Fourier modes, or the klt solver")
     ap.add_argument("--nt-gen", type=int, default=0, choices=[0, 1],
                     help="stage 1 generates S_p from the pair table (1) or reads S_p planes (0)")
-    ap.add_argument("--s2-split", type=int, default=4, help="split-K factor of the split-BF16x3 stage 2")
+    ap.add_argument("--s2-split", type=int, default=8, help="split-K factor of the split-BF16x3 stage 2")
     ap.add_argument("--nt-group", type=int, default=8, help="split-BF16x3 GEMM tile-group edge")
     ap.add_argument("--nt-remap", type=int, default=0, choices=[0, 1], help="deal tile groups to XCDs")
     ap.add_argument("--copy-gib", type=float, default=0.0,

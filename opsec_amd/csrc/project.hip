@@ -537,48 +537,41 @@ __device__ __forceinline__ void gather16(const unsigned* __restrict__ Qhl, const
     }
 }
 
-// S_p planes from the packed pair table, 16 columns per thread (the gather of
-// the table-generated GEMM): row a, columns 16 t .. 16 t + 15 of the pitch;
-// columns at or beyond Kp (the pitch padding) are written as zero.
+// S_p planes from the packed pair table.  Block (x, y) writes row y, columns
+// 2048 x .. 2048 x + 2047 of the pitch; each lane takes column pairs 512 apart,
+// so the coordinate loads, the table gathers (neighbouring columns have
+// neighbouring keys) and the 4-byte stores of both planes coalesce across the
+// wave.  Columns at or beyond Kp (the pitch padding) are written as zero.
 __global__ __launch_bounds__(256) void gen_planes_kernel(const unsigned* __restrict__ Qhl,
                                                          const unsigned* __restrict__ cpk, int Ny, int Nz, int Kp,
-                                                         __bf16* __restrict__ hi, __bf16* __restrict__ lo, size_t ld) {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t per_row = ld / 16;
-    const size_t a = t / per_row;
-    if (a >= (size_t)Kp) return;
-    const int b0 = (int)(t % per_row) * 16;
-    unsigned w[16];
-    if (b0 < Kp) {
-        const unsigned ca = cpk[a];
-        unsigned c[16];
+                                                         __bf16* __restrict__ hi, __bf16* __restrict__ lo, size_t ld,
+                                                         int row0) {
+    const size_t a = (size_t)row0 + blockIdx.y;
+    const unsigned ca = cpk[a];
+    const bool rowv = ca != CELL_NONE;
+    const int xa = (int)(ca >> 20), ya = (int)((ca >> 10) & 1023u), za = (int)(ca & 1023u);
+    unsigned* ph = (unsigned*)(hi + a * ld);
+    unsigned* pl = (unsigned*)(lo + a * ld);
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const uint4 v = *(const uint4*)(cpk + b0 + 4 * q);
-            c[4 * q] = v.x;
-            c[4 * q + 1] = v.y;
-            c[4 * q + 2] = v.z;
-            c[4 * q + 3] = v.w;
+    for (int e = 0; e < 4; e++) {
+        const int b = 2048 * blockIdx.x + 512 * e + 2 * threadIdx.x;
+        if (b >= (int)ld) break;
+        unsigned w0 = 0u, w1 = 0u;
+        if (b < Kp) {
+            const uint2 cb = *(const uint2*)(cpk + b);
+            if (rowv && cb.x != CELL_NONE) {
+                const unsigned key = (__sad(xa, (int)(cb.x >> 20), 0u) * Ny + __sad(ya, (int)((cb.x >> 10) & 1023u), 0u)) * Nz +
+                                     __sad(za, (int)(cb.x & 1023u), 0u);
+                w0 = Qhl[key];
+            }
+            if (rowv && cb.y != CELL_NONE) {
+                const unsigned key = (__sad(xa, (int)(cb.y >> 20), 0u) * Ny + __sad(ya, (int)((cb.y >> 10) & 1023u), 0u)) * Nz +
+                                     __sad(za, (int)(cb.y & 1023u), 0u);
+                w1 = Qhl[key];
+            }
         }
-        gather16(Qhl, c, ca >> 20, (ca >> 10) & 1023u, ca & 1023u, ca != CELL_NONE, Ny, Nz, w);
-    } else {
-#pragma unroll
-        for (int e = 0; e < 16; e++) w[e] = 0u;
-    }
-#pragma unroll
-    for (int cc = 0; cc < 2; cc++) {
-        const unsigned* v = w + 8 * cc;
-        uint4 h, l;
-        h.x = pair_hi(v[0], v[1]);
-        h.y = pair_hi(v[2], v[3]);
-        h.z = pair_hi(v[4], v[5]);
-        h.w = pair_hi(v[6], v[7]);
-        l.x = pair_lo(v[0], v[1]);
-        l.y = pair_lo(v[2], v[3]);
-        l.z = pair_lo(v[4], v[5]);
-        l.w = pair_lo(v[6], v[7]);
-        *(uint4*)(hi + a * ld + b0 + 8 * cc) = h;
-        *(uint4*)(lo + a * ld + b0 + 8 * cc) = l;
+        ph[b >> 1] = pair_hi(w0, w1);
+        pl[b >> 1] = pair_lo(w0, w1);
     }
 }
 
@@ -813,7 +806,7 @@ static int g_nt_tile = 2, g_nt_group = 8, g_nt_remap = 0;
 // gen_split_kernel (0).
 static int g_nt_gen = 0;
 // Split-K factor of stage 2 on the 256 x 256 kernel.
-static int g_s2_split = 4;
+static int g_s2_split = 8;
 
 template <typename ASrc, typename Epi>
 static void launch_g3(opsec_ctx* ctx, const ASrc& A, int arows, const __bf16* B, int brows, size_t ld, int K,
@@ -894,8 +887,10 @@ static int run_project_bf16x3(opsec_ctx* ctx, int nb, const int* funcs) {
             hipLaunchKernelGGL(pack_table_kernel, dim3((ctx->nkeys + 255) / 256), dim3(256), 0, ctx->stream, Qf,
                                ps->d_Qhl, ctx->nkeys);
         if (!gen && packable)
-            hipLaunchKernelGGL(gen_planes_kernel, blocks_for(Kp * Pp / 16), dim3(256), 0, ctx->stream, ps->d_Qhl,
-                               ps->d_cpk, ctx->sv.n2, ctx->sv.n3, (int)Kp, Sh, Sl, Pp);
+            for (size_t r0 = 0; r0 < Kp; r0 += 32768)
+                hipLaunchKernelGGL(gen_planes_kernel, dim3((unsigned)((Pp + 2047) / 2048), (unsigned)std::min<size_t>(32768, Kp - r0)),
+                                   dim3(256), 0, ctx->stream, ps->d_Qhl, ps->d_cpk, ctx->sv.n2, ctx->sv.n3, (int)Kp, Sh, Sl,
+                                   Pp, (int)r0);
         else if (!gen)
             hipLaunchKernelGGL(gen_split_kernel, blocks_for(Kp * Pp / 8), dim3(256), 0, ctx->stream, Qf,
                                ctx->d_cellidx, ctx->sv.n2, ctx->sv.n3, N, Sh, Sl, Pp, (int)Kp);

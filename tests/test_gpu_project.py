@@ -210,7 +210,7 @@ def test_project_bf16x3_gemm_variants(ctx, tile, gen):
     finally:
         ctx.lib.opsec_internal_set_nt_tile(2)
         ctx.lib.opsec_internal_set_nt_gen(0)
-        ctx.lib.opsec_internal_set_stage2_split(4)
+        ctx.lib.opsec_internal_set_stage2_split(8)
     err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
     print("tile", tile, "gen", gen, "max rel err per function", err)
     assert err.max() < 5e-5

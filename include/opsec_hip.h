@@ -204,6 +204,30 @@ int opsec_project(opsec_ctx* ctx, int nf, const int* funcs, int prec, double* P_
 int opsec_project_dense(opsec_ctx* ctx, int n, const double* S, int nmodes,
                         const double* B, int prec, double* P_packed);
 
+/* ---- pixel values of a galaxy catalogue (the dot stage) ------------------ */
+/* Galaxy record of a galfile: "4f" abn records of (x1, x2, x3, w) -- (x, y, z)
+ * or (r, mu, phi) and a weight (src/Survey.h:13-19, src/BoxSurvey.cpp:105-133). */
+typedef struct opsec_galaxy {
+    float x1, x2, x3, w;
+} opsec_galaxy;
+
+/* Weighted galaxy counts per cell of the device survey (src/dot.cpp:105-128):
+ * counts[a] (a < Ncells) = sum of the weights of the galaxies in cell a, each
+ * added in catalogue order as the reference loop adds them, so the sums are
+ * bit-identical to it; counts[Ncells] = the weight of galaxies in no cell.
+ * A galaxy's grid position is Survey::GetGridIndex (src/Survey.cpp:93-101).
+ * Unlike the reference, grid position 0 counts as inside (src/dot.cpp:127
+ * tests e > 0) and a grid position that has no cell counts as outside (the
+ * reference indexes counts[-1]).  counts holds Ncells + 1 doubles; with
+ * accumulate != 0 the sums continue from its contents, so a catalogue passed
+ * in pieces, in order, gives the same bits as one call. */
+int opsec_count_galaxies(opsec_ctx* ctx, const opsec_galaxy* gals, int64_t ngals, int accumulate,
+                         double* counts);
+/* Pixel values y_i = sum_a B_ia x_a, x_a = (counts[a] - Nbar_a)/sqrt(Nbar_a),
+ * with the uploaded modes (src/dot.cpp:130-153).  counts: Ncells doubles
+ * (an extra outside bin is ignored); x (optional): Ncells; y: Nmodes. */
+int opsec_pixel_values(opsec_ctx* ctx, const double* counts, double* x, double* y);
+
 /* ---- Fisher matrix and band powers (K4) --------------------------------- */
 enum { OPSEC_MIX_INVERSE = 0, OPSEC_MIX_INVERSE_SQRT = 1, OPSEC_MIX_IDENTITY = 2 };
 

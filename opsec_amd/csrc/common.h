@@ -61,6 +61,7 @@ struct DevBuf {
 
 struct MlpState;  // surrogate network (mlp.hip)
 struct ProjState; // projection results (project.hip)
+struct DotState;  // galaxy counts and pixel values (dot.hip)
 
 }  // namespace opsec
 
@@ -103,6 +104,7 @@ struct opsec_ctx {
     double* d_B = nullptr;                // ncells x nmodes, column-major
     opsec::ProjState* proj = nullptr;
     opsec::MlpState* mlp = nullptr;
+    opsec::DotState* dot = nullptr;
 
     std::vector<void*> user_bufs;
 };

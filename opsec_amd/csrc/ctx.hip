@@ -352,6 +352,7 @@ int opsec_ctx_create(int device, opsec_ctx** out) {
 
 void opsec_internal_free_proj(opsec_ctx* ctx);
 void opsec_internal_free_mlp(opsec_ctx* ctx);
+void opsec_internal_free_dot(opsec_ctx* ctx);
 
 int opsec_ctx_destroy(opsec_ctx* ctx) {
     if (!ctx) return OPSEC_OK;
@@ -364,6 +365,7 @@ int opsec_ctx_destroy(opsec_ctx* ctx) {
     dev_free((void**)&ctx->d_B);
     opsec_internal_free_proj(ctx);
     opsec_internal_free_mlp(ctx);
+    opsec_internal_free_dot(ctx);
     for (void* p : ctx->user_bufs) hipFree(p);
     hipStreamDestroy(ctx->stream);
     delete ctx;

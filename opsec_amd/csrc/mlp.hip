@@ -822,6 +822,7 @@ int opsec_mlp_train(opsec_ctx* ctx, const opsec_mlp_train_opts* o, const opsec_q
     if (!ctx || !o || !q) return OPSEC_EINVAL;
     if (ctx->ncells == 0) return set_error(ctx, OPSEC_ESTATE, "no survey uploaded");
     if (ctx->model_kind != 0) return set_error(ctx, OPSEC_ESTATE, "the surrogate needs a RealModel");
+    OPSEC_CHECK_ARG(ctx, ctx->sv.coordsys == OPSEC_COORD_CARTESIAN, "the surrogate supports box surveys");
     OPSEC_CHECK_ARG(ctx, o->nsamples >= 256 && o->batch >= 64 && o->epochs >= 1 && o->lr > 0, "bad training options");
     int rc = ensure_mlp(ctx);
     if (rc) return rc;

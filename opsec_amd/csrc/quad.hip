@@ -808,7 +808,6 @@ int opsec_cells_upload(opsec_ctx* ctx, const opsec_cell* cells, int ncells, cons
     SurveyDev s;
     int rc = survey_from_abi(ctx, sv, &s);
     if (rc) return rc;
-    OPSEC_CHECK_ARG(ctx, s.coordsys == OPSEC_COORD_CARTESIAN, "device-resident tables support box surveys");
     for (int a = 0; a < ncells; a++) {
         const opsec_cell& c = cells[a];
         OPSEC_CHECK_ARG(ctx, c.d1 >= 0 && c.d1 < s.n1 && c.d2 >= 0 && c.d2 < s.n2 && c.d3 >= 0 && c.d3 < s.n3,
@@ -852,6 +851,7 @@ int opsec_pair_tables(opsec_ctx* ctx, int nf, const int* funcs, const opsec_quad
     if (!ctx) return OPSEC_EINVAL;
     if (ctx->ncells == 0) return set_error(ctx, OPSEC_ESTATE, "no survey uploaded");
     if (ctx->model_kind < 0) return set_error(ctx, OPSEC_ESTATE, "no model loaded");
+    OPSEC_CHECK_ARG(ctx, ctx->sv.coordsys == OPSEC_COORD_CARTESIAN, "device-resident pair tables support box surveys");
     int rc = check_quad(ctx, q);
     if (rc) return rc;
     OPSEC_CHECK_ARG(ctx, nf >= 1 && funcs, "no functions");

@@ -53,32 +53,7 @@ int main(int argc, char** argv) {
     if (!load_survey(cfg, &sv)) prog_exit(1);
 
     // modes (src/comma-scalapack.cpp:52-98)
-    FILE* fm = std::fopen(modefile.c_str(), "rb");
-    if (!fm) {
-        error("comma: cannot open modes file %s\n", modefile.c_str());
-        prog_abort(1);
-    }
-    AbnHeader mh;
-    if (!abn_read_header(fm, &mh)) prog_abort(1);
-    int hv;
-    if (mh.opts.has("Nmodes") && (!mh.opts.get_int("Nmodes", &hv) || hv != Nmodes)) {
-        error("comma: modefile has Nmodes = %s, expected %d\n", mh.opts.get("Nmodes").c_str(), Nmodes);
-        prog_abort(1);
-    }
-    if (mh.opts.has("Ncells") && (!mh.opts.get_int("Ncells", &hv) || hv != Ncells)) {
-        error("comma: modefile has Ncells = %s, expected %d\n", mh.opts.get("Ncells").c_str(), Ncells);
-        prog_abort(1);
-    }
-    std::vector<double> B((size_t)Nmodes * Ncells);
-    for (int j = 0; j < Nmodes; j++) {
-        size_t nread = std::fread(&B[(size_t)j * Ncells], sizeof(double), Ncells, fm);
-        if (nread != (size_t)Ncells) {
-            error("comma: error reading modes (%s,%d,%zu,%d)\n", modefile.c_str(), j, nread * sizeof(double), Ncells);
-            prog_abort(1);
-        }
-    }
-    std::fclose(fm);
-    if (mh.endian != abn_endian()) abn_byte_swap((char*)B.data(), B.size(), "d");
+    std::vector<double> B = read_modes(modefile, Nmodes, Ncells, false, "comma");
 
     FILE* fout = std::fopen(covfile.c_str(), "wb");
     if (!fout) {

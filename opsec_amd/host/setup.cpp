@@ -55,6 +55,65 @@ std::vector<opsec_cell> read_cells(const std::string& path, Config* cellopts) {
     return cells;
 }
 
+std::vector<double> read_modes(const std::string& path, int Nmodes, int Ncells, bool exact, const char* prog) {
+    FILE* fm = std::fopen(path.c_str(), "rb");
+    if (!fm) {
+        error("%s: cannot open modes file %s\n", prog, path.c_str());
+        prog_abort(1);
+    }
+    AbnHeader mh;
+    if (!abn_read_header(fm, &mh)) prog_abort(1);
+    int hv = 0;
+    if ((exact || mh.opts.has("Nmodes")) && (!mh.opts.get_int("Nmodes", &hv) || hv != Nmodes)) {
+        error("%s: modefile has Nmodes = %s, expected %d\n", prog, mh.opts.get("Nmodes").c_str(), Nmodes);
+        prog_abort(1);
+    }
+    if ((exact || mh.opts.has("Ncells")) && (!mh.opts.get_int("Ncells", &hv) || hv != Ncells)) {
+        error("%s: modefile has Ncells = %s, expected %d\n", prog, mh.opts.get("Ncells").c_str(), Ncells);
+        prog_abort(1);
+    }
+    if (exact && (mh.n != (size_t)Nmodes * Ncells || mh.size != sizeof(double))) {
+        error("%s: modefile holds %zu items of %zu bytes, expected %zu doubles\n", prog, mh.n, mh.size,
+              (size_t)Nmodes * Ncells);
+        prog_abort(1);
+    }
+    std::vector<double> B((size_t)Nmodes * Ncells);
+    for (int j = 0; j < Nmodes; j++) {
+        size_t nread = std::fread(&B[(size_t)j * Ncells], sizeof(double), Ncells, fm);
+        if (nread != (size_t)Ncells) {
+            error("%s: error reading modes (%s,%d,%zu,%d)\n", prog, path.c_str(), j, nread * sizeof(double), Ncells);
+            prog_abort(1);
+        }
+    }
+    std::fclose(fm);
+    if (mh.endian != abn_endian()) abn_byte_swap((char*)B.data(), B.size(), "d");
+    return B;
+}
+
+std::vector<opsec_galaxy> read_galaxies(const std::string& path, const char* who) {
+    std::vector<opsec_galaxy> gals;
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) {
+        error("%s: could not open file '%s'\n", who, path.c_str());
+        return gals;
+    }
+    AbnHeader h;
+    if (!abn_read_header(f, &h) || h.size != sizeof(opsec_galaxy) || h.fmt != "4f") {
+        error("%s: error reading galaxies from '%s'\n", who, path.c_str());
+        std::fclose(f);
+        return gals;
+    }
+    gals.resize(h.n);
+    size_t nread = std::fread(gals.data(), sizeof(opsec_galaxy), h.n, f);
+    if (nread != h.n) {
+        error("%s: expecting %zu galaxies from '%s', got %zu\n", who, h.n, path.c_str(), nread);
+        gals.resize(nread);
+    }
+    std::fclose(f);
+    if (h.endian != abn_endian()) abn_byte_swap((char*)gals.data(), gals.size(), h.fmt);
+    return gals;
+}
+
 // Two-column text table (src/Spline.cpp:49-88): '#' lines skipped.
 static bool load_pkfile(const std::string& path, std::vector<double>& k, std::vector<double>& P) {
     FILE* f = std::fopen(path.c_str(), "r");
@@ -160,6 +219,27 @@ bool load_survey(const Config& cfg, opsec_survey* sv) {
         s.get_int("Ny", &sv->n2);
         s.get_int("Nz", &sv->n3);
         if (!s.get_double("L", &sv->L)) sv->L = 0;
+        return true;
+    }
+    if (type == "SphericalSurvey") {
+        std::string miss = s.missing("RMin,RMax,MuMin,MuMax,PhiMin,PhiMax,Nr,Nmu,Nphi");
+        if (!miss.empty()) {
+            error("Survey: missing configuration options %s\n", miss.c_str());
+            prog_abort(1);
+        }
+        // MuMin comes from "MuMin"; the reference reads the misspelled key
+        // "MuMuin" (src/Survey.cpp:39) and so always gets 0.
+        sv->coordsys = OPSEC_COORD_SPHERICAL;
+        s.get_double("RMin", &sv->min1);
+        s.get_double("RMax", &sv->max1);
+        s.get_double("MuMin", &sv->min2);
+        s.get_double("MuMax", &sv->max2);
+        s.get_double("PhiMin", &sv->min3);
+        s.get_double("PhiMax", &sv->max3);
+        s.get_int("Nr", &sv->n1);
+        s.get_int("Nmu", &sv->n2);
+        s.get_int("Nphi", &sv->n3);
+        sv->L = 0;
         return true;
     }
     error("InitializeSurvey: unrecognized survey type '%s'\n", type.c_str());

@@ -18,6 +18,20 @@ namespace opsec_host {
 // header carries Ncells.  Aborts on unreadable or inconsistent files.
 std::vector<opsec_cell> read_cells(const std::string& path, Config* cellopts = nullptr);
 
+// Mode vectors of a modefile (klt output, src/klt.cpp:210-248): Nmodes
+// consecutive vectors of Ncells doubles.  With exact, the header must carry
+// Nmodes and Ncells equal to the arguments and exactly Nmodes x Ncells
+// doubles, as ReadModesHeader requires (src/sig.cpp:621-645); otherwise header
+// values, where present, must match and the first Nmodes vectors are read.
+// Aborts on any mismatch.
+std::vector<double> read_modes(const std::string& path, int Nmodes, int Ncells, bool exact, const char* prog);
+
+// Galaxies of a galfile: "4f" abn records (x1, x2, x3, w), byte-swapped if
+// written on a machine of the other endianness (src/BoxSurvey.cpp:105-133,
+// src/SphericalSurvey.cpp:142-170).  Prints why and returns what was read
+// (nothing if the file or its header is unusable).
+std::vector<opsec_galaxy> read_galaxies(const std::string& path, const char* who);
+
 // Model from the "model" key and its "<Model>." sub-config
 // (src/Model.cpp:26-51, src/RealModel.cpp:33-76, src/TestModel.cpp:34-45).
 // Returns false (after printing why) if the model cannot be built.
@@ -32,7 +46,8 @@ struct ModelSpec {
 bool load_model(const Config& cfg, ModelSpec* m);
 int upload_model(opsec_ctx* ctx, const ModelSpec& m, int xi_mode);
 
-// Survey from "survey.type" and the "survey." sub-config (src/Survey.cpp:140-165).
+// Survey from "survey.type" and the "survey." sub-config (src/Survey.cpp:140-165):
+// BoxSurvey (src/BoxSurvey.cpp:85-91) or SphericalSurvey (src/Survey.cpp:31-44).
 bool load_survey(const Config& cfg, opsec_survey* sv);
 
 // New keys of this implementation (prefix "hip."):

@@ -38,6 +38,9 @@ CELL_DTYPE = np.dtype([("a", "<i4"), ("d1", "<i4"), ("d2", "<i4"), ("d3", "<i4")
                        ("min3", "<f8"), ("max3", "<f8"), ("Veff", "<f8"), ("Nbar", "<f8")])
 assert CELL_DTYPE.itemsize == 80
 
+# galfile record, src/Survey.h:13-19 ("4f")
+GALAXY_DTYPE = np.dtype([("x1", "<f4"), ("x2", "<f4"), ("x3", "<f4"), ("w", "<f4")])
+
 
 class Survey(C.Structure):
     _fields_ = [("coordsys", C.c_int32), ("n1", C.c_int32), ("n2", C.c_int32), ("n3", C.c_int32),
@@ -102,6 +105,8 @@ _SIGS = {
     "opsec_modes_upload": ([_P, _P, _I, _I], _I),
     "opsec_project": ([_P, _I, _P, _I, _P], _I),
     "opsec_project_dense": ([_P, _I, _P, _I, _P, _I, _P], _I),
+    "opsec_count_galaxies": ([_P, _P, C.c_int64, _I, _P], _I),
+    "opsec_pixel_values": ([_P, _P, _P, _P], _I),
     "opsec_estimate_run": ([_P, _I, _I, _P, _P, _P, _P, _I, _I, C.POINTER(Estimate)], _I),
     "opsec_estimate_device": ([_P, _P, _P, _P, _I, _I, C.POINTER(Estimate)], _I),
     "opsec_mlp_defaults": ([C.POINTER(MlpTrainOpts)], _I),
@@ -326,6 +331,26 @@ class Context:
         B = f64(B)
         self.check(self.lib.opsec_modes_upload(self.h, _ptr(B), B.shape[1], B.shape[0]), "modes_upload")
         self._nmodes = B.shape[0]
+
+    def count_galaxies(self, gals, counts=None):
+        """Weighted counts per cell of a GALAXY_DTYPE array; Ncells + 1 values,
+        the last for galaxies in no cell.  With `counts` given the sums continue
+        from it (the catalogue passed in pieces, in order)."""
+        g = np.ascontiguousarray(gals, dtype=GALAXY_DTYPE)
+        acc = counts is not None
+        out = np.array(counts, dtype=np.float64) if acc else np.zeros(len(self._cells) + 1)
+        assert out.shape == (len(self._cells) + 1,)
+        self.check(self.lib.opsec_count_galaxies(self.h, _ptr(g), len(g), int(acc), _ptr(out)), "count_galaxies")
+        return out
+
+    def pixel_values(self, counts):
+        """(x, y): x_a = (N_a - Nbar_a)/sqrt(Nbar_a) and y = B^T x."""
+        c = f64(counts)
+        assert len(c) >= len(self._cells)
+        x = np.zeros(len(self._cells))
+        y = np.zeros(self._nmodes)
+        self.check(self.lib.opsec_pixel_values(self.h, _ptr(c), _ptr(x), _ptr(y)), "pixel_values")
+        return x, y
 
     def project(self, funcs, prec=OPSEC_PREC_FP64, keep_on_device=False):
         fs = np.ascontiguousarray(funcs, dtype=np.int32)

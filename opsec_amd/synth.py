@@ -77,3 +77,38 @@ CONFIGS = {
     "C1": dict(n=12, nbands=20, kmin=0.01, kmax=0.20, nmodes=500, nodes=8, nbar=3e-4),
     "C2": dict(n=32, nbands=40, kmin=0.005, kmax=0.25, nmodes=4000, nodes=6, nbar=3e-4),
 }
+
+
+def drop_cells(cells, frac, seed=0):
+    """A cell list with a random fraction of grid positions removed and the
+    rest renumbered a = 0..n-1 (an irregular survey footprint)."""
+    rng = np.random.default_rng(seed)
+    keep = rng.random(len(cells)) >= frac
+    out = cells[keep].copy()
+    out["a"] = np.arange(len(out))
+    return out
+
+
+def catalogue(n, lo, hi, seed=0, margin=0.05, weights="unit"):
+    """n synthetic galaxies (GALAXY_DTYPE) uniform in the box [lo, hi) per
+    axis widened by `margin` of its extent on each side, so some fall outside
+    the survey.  lo/hi: scalars or 3-sequences.  weights: "unit", "int"
+    (1..5), or "wide" (log-uniform over 1e-3..1e3, so that the rounding of a
+    per-cell sum depends on the order of its terms)."""
+    from .lib import GALAXY_DTYPE
+    rng = np.random.default_rng(seed)
+    lo = np.broadcast_to(np.asarray(lo, dtype=np.float64), (3,))
+    hi = np.broadcast_to(np.asarray(hi, dtype=np.float64), (3,))
+    ext = hi - lo
+    g = np.zeros(n, dtype=GALAXY_DTYPE)
+    for k, name in enumerate(("x1", "x2", "x3")):
+        g[name] = rng.uniform(lo[k] - margin * ext[k], hi[k] + margin * ext[k], n)
+    if weights == "unit":
+        g["w"] = 1.0
+    elif weights == "int":
+        g["w"] = rng.integers(1, 6, n)
+    elif weights == "wide":
+        g["w"] = 10.0 ** rng.uniform(-3, 3, n)
+    else:
+        raise ValueError(weights)
+    return g

@@ -1088,3 +1088,59 @@ double orc_brute6_sph(const orc_xi* f, const orc_cell* c1, const orc_cell* c2, i
     }
     return acc;
 }
+
+/* ---- pixel values (the dot stage) ---- */
+
+/* Survey::GetGridIndex (src/Survey.cpp:93-101).  The reference casts
+ * floor(.) to int and then range-tests; testing the floored double gives the
+ * same answer wherever that cast is defined, and sends NaN and values beyond
+ * the int range outside, as the x86 conversion does. */
+static int orc_grid_coord(double x, double lo, double hi, int n) {
+    double t = floor(n * (x - lo) / (hi - lo));
+    if (!(t >= 0 && t < n)) return -1;
+    return (int)t;
+}
+
+int64_t orc_grid_index(const double* geom, const int* n, double x1, double x2, double x3) {
+    int d1 = orc_grid_coord(x1, geom[0], geom[1], n[0]);
+    int d2 = orc_grid_coord(x2, geom[2], geom[3], n[1]);
+    int d3 = orc_grid_coord(x3, geom[4], geom[5], n[2]);
+    if (d1 < 0 || d2 < 0 || d3 < 0) return -1;
+    return ((int64_t)d1 * n[1] + d2) * n[2] + d3;
+}
+
+/* Weighted counts (src/dot.cpp:105-128): cellmap filled in cell order (the
+ * last cell at a grid position wins), then every galaxy's weight added to its
+ * cell in catalogue order.  Grid position 0 is inside (the reference's e > 0
+ * is a defect) and a grid position with no cell goes to counts[ncells] (the
+ * reference indexes counts[-1]).  counts: ncells + 1, accumulated into. */
+void orc_count_galaxies(const orc_cell* cells, int ncells, const double* geom, const int* n,
+                        const float* gals, int64_t ngals, double* counts) {
+    int64_t ngrid = (int64_t)n[0] * n[1] * n[2];
+    int* cellmap = (int*)malloc(sizeof(int) * (size_t)ngrid);
+    for (int64_t e = 0; e < ngrid; e++) cellmap[e] = -1;
+    for (int a = 0; a < ncells; a++) {
+        int64_t e = ((int64_t)cells[a].d1 * n[1] + cells[a].d2) * n[2] + cells[a].d3;
+        cellmap[e] = a;
+    }
+    for (int64_t g = 0; g < ngals; g++) {
+        const float* p = gals + 4 * g;
+        int64_t e = orc_grid_index(geom, n, p[0], p[1], p[2]);
+        int a = (e >= 0 && cellmap[e] >= 0) ? cellmap[e] : ncells;
+        counts[a] += p[3];
+    }
+    free(cellmap);
+}
+
+/* x_a = (N_a - Nbar_a)/sqrt(Nbar_a), y_i = sum_a B_ia x_a in index order
+ * (src/dot.cpp:130-153); B is nmodes x ncells, mode-major. */
+void orc_pixel_values(const orc_cell* cells, int ncells, int nmodes, const double* B,
+                      const double* counts, double* x, double* y) {
+    for (int a = 0; a < ncells; a++) x[a] = (counts[a] - cells[a].Nbar) / sqrt(cells[a].Nbar);
+    for (int i = 0; i < nmodes; i++) {
+        const double* b = B + (size_t)i * ncells;
+        double s = 0;
+        for (int a = 0; a < ncells; a++) s += b[a] * x[a];
+        y[i] = s;
+    }
+}

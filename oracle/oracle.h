@@ -26,6 +26,14 @@
  *                                      inverse/sqrtm known answers (tests/testMyMatrix.cpp:67-104)
  *   orc_sym_eigen                      J_x spectrum (tests/testeig.cpp:78-121), numpy eigh
  *   orc_mlp_*                          finite differences of orc_mlp_forward
+ *   orc_grid_index                     oracle/_ref (reference Survey::GetGridIndex) on
+ *                                      interior, boundary and outside points
+ *   orc_count_galaxies                 known answers: order-dependent rounding
+ *                                      (2^60 + 1 - 2^60), grid cell 0, missing cells,
+ *                                      outside points; integer weights against an
+ *                                      independent numpy histogram
+ *   orc_pixel_values                   x against numpy IEEE arithmetic; y against
+ *                                      numpy's BLAS product B @ x
  */
 #ifndef OPSEC_ORACLE_H
 #define OPSEC_ORACLE_H
@@ -118,6 +126,15 @@ void orc_sym_eigen(int n, const double* A, double* evals, double* V);
 int orc_mlp_nweights(void);
 void orc_mlp_forward(const double* w, int n, const double* x, double* out);
 double orc_mlp_loss_grad(const double* w, int n, const double* x, const double* t, double* grad);
+
+/* ---- pixel values (dot) ---- */
+/* geom = {min1, max1, min2, max2, min3, max3}, n = {N1, N2, N3}; -1 outside */
+int64_t orc_grid_index(const double* geom, const int* n, double x1, double x2, double x3);
+/* gals: ngals x (x1, x2, x3, w) floats; counts (ncells + 1) is accumulated into */
+void orc_count_galaxies(const orc_cell* cells, int ncells, const double* geom, const int* n,
+                        const float* gals, int64_t ngals, double* counts);
+void orc_pixel_values(const orc_cell* cells, int ncells, int nmodes, const double* B,
+                      const double* counts, double* x, double* y);
 
 #ifdef __cplusplus
 }

@@ -184,3 +184,24 @@ REF_API int ref_read_cells(const char* path, void* out, int maxcells) {
 }
 
 REF_API int ref_sizeof_cell(void) { return (int)sizeof(Cell); }
+
+/* Survey::GetGridIndex of the box survey (src/Survey.cpp:93-101) at n points
+ * given as float triples (galfile coordinates). */
+REF_API void ref_grid_index(const char* survey_cfgtext, int n, const float* xyz, int* out) {
+    Survey* s = make_box(survey_cfgtext);
+    for (int i = 0; i < n; i++) out[i] = s->GetGridIndex(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    delete s;
+}
+
+/* Galaxies of survey.galfile read by BoxSurvey::GetGalaxies
+ * (src/BoxSurvey.cpp:105-133): returns the count, copies up to maxgals
+ * (x1, x2, x3, w) records. */
+REF_API int ref_read_galaxies(const char* survey_cfgtext, float* out, int maxgals) {
+    Survey* s = make_box(survey_cfgtext);
+    std::vector<Galaxy> gals;
+    s->GetGalaxies(gals);
+    delete s;
+    int n = (int)gals.size();
+    if (n <= maxgals && n > 0) memcpy(out, &gals[0], sizeof(Galaxy) * n);
+    return n;
+}

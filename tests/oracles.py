@@ -52,6 +52,9 @@ _ORC = {
     "orc_mlp_nweights": ([], _I),
     "orc_mlp_forward": ([_P, _I, _P, _P], None),
     "orc_mlp_loss_grad": ([_P, _I, _P, _P, _P], _D),
+    "orc_grid_index": ([_P, _P, _D, _D, _D], C.c_int64),
+    "orc_count_galaxies": ([_P, _I, _P, _P, _P, C.c_int64, _P], None),
+    "orc_pixel_values": ([_P, _I, _I, _P, _P, _P, _P], None),
 }
 
 _REF = {
@@ -67,6 +70,8 @@ _REF = {
     "ref_sig_block": ([_P, _I, C.c_char_p, _P, _I, _P, _I, _P, _I, _P, _I, _D, _D], _I),
     "ref_read_cells": ([C.c_char_p, _P, _I], _I),
     "ref_sizeof_cell": ([], _I),
+    "ref_grid_index": ([C.c_char_p, _I, _P, _P], None),
+    "ref_read_galaxies": ([C.c_char_p, _P, _I], _I),
 }
 
 
@@ -159,3 +164,28 @@ def ref_real_model(pkfile, band_lo, band_hi):
     txt = (f"model = RealModel\nRealModel.pkfile = {pkfile}\nRealModel.Nbands = {nb}\n"
            f"RealModel.kmin = {float(band_lo[0])!r}\nRealModel.kmax = {float(band_hi[-1])!r}\n")
     return ref().ref_model_new(txt.encode())
+
+
+def survey_geom(sv):
+    """(geom, n) arrays of an opsec_survey for the oracle's grid functions."""
+    geom = np.array([sv.min1, sv.max1, sv.min2, sv.max2, sv.min3, sv.max3])
+    return geom, np.array([sv.n1, sv.n2, sv.n3], dtype=np.int32)
+
+
+def count_galaxies(cells, sv, gals, counts=None):
+    """Oracle weighted counts (Ncells + 1), accumulated into `counts` if given."""
+    geom, n = survey_geom(sv)
+    out = np.zeros(len(cells) + 1) if counts is None else np.array(counts, dtype=np.float64)
+    g = np.ascontiguousarray(gals)
+    oracle().orc_count_galaxies(ptr(cells), len(cells), ptr(geom), ptr(n), ptr(g), len(g), ptr(out))
+    return out
+
+
+def pixel_values(cells, B, counts):
+    """Oracle (x, y) for mode-major B (nmodes x ncells)."""
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    c = np.ascontiguousarray(counts, dtype=np.float64)
+    x = np.zeros(len(cells))
+    y = np.zeros(B.shape[0])
+    oracle().orc_pixel_values(ptr(cells), len(cells), B.shape[0], ptr(B), ptr(c), ptr(x), ptr(y))
+    return x, y

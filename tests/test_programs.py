@@ -18,17 +18,36 @@ def run(prog, *args, cwd=None):
     return subprocess.run([os.path.join(BIN, prog), *args], capture_output=True, text=True, cwd=cwd, timeout=600)
 
 
-@pytest.mark.parametrize("prog", ["klt", "comma", "estimate"])
+@pytest.mark.parametrize("prog", ["klt", "comma", "dot", "estimate"])
 def test_help_exits_zero(prog):
     r = run(prog, "-h")
     assert r.returncode == 0 and "Usage:" in r.stdout
 
 
-@pytest.mark.parametrize("prog,keys", [("klt", "cellfile"), ("comma", "covfile"), ("estimate", "pixfile")])
+@pytest.mark.parametrize("prog,keys", [("klt", "cellfile"), ("comma", "covfile"), ("dot", "pixfile"),
+                                       ("estimate", "pixfile")])
 def test_missing_keys_exit_one(prog, keys):
     r = run(prog, "Nmodes=3")
     assert r.returncode == 1
     assert "missing configuration options" in r.stderr
+
+
+def test_dot_does_not_require_countfile(tmp_path):
+    """The reference requires countfile and never uses it (src/dot.cpp:54);
+    here it is optional: without it dot gets past the key check."""
+    r = run("dot", f"cellfile={tmp_path}/nope.abn", "Nmodes=1", f"modefile={tmp_path}/m", f"pixfile={tmp_path}/p")
+    assert "missing configuration options" not in r.stderr
+    assert "ReadCells: could not open" in r.stderr
+
+
+def test_dot_rejects_inconsistent_modefile(tmp_path):
+    cells = synth.box_cells(3)
+    formats.write_cells(tmp_path / "cells.abn", cells, {"coordsys": "cartesian"})
+    formats.write(tmp_path / "modes.abn", np.zeros(2 * 27), "d", {"Nmodes": 2, "Ncells": 27})
+    r = run("dot", f"cellfile={tmp_path}/cells.abn", "Nmodes=3", f"modefile={tmp_path}/modes.abn",
+            f"pixfile={tmp_path}/p")
+    assert r.returncode != 0
+    assert "modefile has Nmodes = 2, expected 3" in r.stderr
 
 
 def test_unreadable_cellfile_aborts(tmp_path):

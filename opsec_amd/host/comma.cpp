@@ -6,7 +6,9 @@
 // projection GEMMs for all bands run on the GPU (libopsec_hip).  As in the
 // reference, the sig.* keys are ignored and the default cubature tolerances
 // are used (src/comma-scalapack.cpp:237-243).
+#include <algorithm>
 #include <cstdio>
+#include <future>
 #include <vector>
 
 #include "formats.h"
@@ -80,16 +82,33 @@ int main(int argc, char** argv) {
     for (int p = 0; p < Nparams; p++) funcs[p] = p + 1;
     info("Computing pair integrals for %d parameters\n", Nparams);
     check(ctx, opsec_pair_tables(ctx, Nparams, funcs.data(), &q), "pair integrals");
-    // project in groups that fit comfortably in HBM (stage-1 products are
-    // Ncells x Nmodes per band)
+    // Project in groups of bands and write each group while the next one is
+    // computed: two host buffers, the file writes on a second thread.  A group
+    // is at most 4 bands (64 MB of packed output per band at 4000 modes) and
+    // its stage-1 products (Ncells x Nmodes per band) stay well inside HBM.
     const size_t per_band = (size_t)Ncells * Nmodes * (prec == OPSEC_PREC_FP64 ? 8 : 4);
-    int group = (int)std::max<size_t>(1, std::min<size_t>(Nparams, ((size_t)96 << 30) / std::max<size_t>(per_band, 1)));
-    std::vector<double> P((size_t)group * K);
+    int group = (int)std::max<size_t>(1, std::min<size_t>({(size_t)Nparams, (size_t)4,
+                                                           ((size_t)96 << 30) / std::max<size_t>(per_band, 1)}));
+    std::vector<double> P[2] = {std::vector<double>((size_t)group * K), std::vector<double>((size_t)group * K)};
+    std::future<size_t> pending;
+    size_t expect = 0;
+    int buf = 0;
     for (int p0 = 0; p0 < Nparams; p0 += group) {
         int ng = std::min(group, Nparams - p0);
         info("Projecting parameters %d-%d onto %d modes\n", p0 + 1, p0 + ng, Nmodes);
-        check(ctx, opsec_project(ctx, ng, funcs.data() + p0, prec, P.data()), "projection");
-        std::fwrite(P.data(), sizeof(double), (size_t)ng * K, fout);
+        check(ctx, opsec_project(ctx, ng, funcs.data() + p0, prec, P[buf].data()), "projection");
+        if (pending.valid() && pending.get() != expect) {
+            error("comma: error writing '%s'\n", covfile.c_str());
+            prog_exit(1);
+        }
+        expect = (size_t)ng * K;
+        const double* src = P[buf].data();
+        pending = std::async(std::launch::async, [=] { return std::fwrite(src, sizeof(double), expect, fout); });
+        buf ^= 1;
+    }
+    if (pending.valid() && pending.get() != expect) {
+        error("comma: error writing '%s'\n", covfile.c_str());
+        prog_exit(1);
     }
     std::fclose(fout);
     opsec_ctx_destroy(ctx);

@@ -91,13 +91,13 @@ int main(int argc, char** argv) {
 
     // covariance derivatives (src/estimate-mpi.cpp:152-330)
     info("Reading covariance matrix derivatives...\n");
-    FILE* fc = std::fopen(covfile.c_str(), "rb");
-    if (!fc) {
+    SplitReader fc;
+    if (!fc.open(covfile)) {
         error("estimate: cannot open covfile (%s)\n", covfile.c_str());
         prog_abort(1);
     }
     AbnHeader ch;
-    if (!abn_read_header(fc, &ch)) {
+    if (!abn_read_header(fc.stream(), &ch)) {
         error("ReadCovHeader: error reading covfile header (%s)\n", covfile.c_str());
         prog_abort(1);
     }
@@ -121,23 +121,23 @@ int main(int argc, char** argv) {
     for (int m = 0; m < Nparams; m++) {
         double* dst = &D[(size_t)m * Ncoeff];
         if (packed) {
-            if (std::fread(dst, sizeof(double), Ncoeff, fc) != Ncoeff) {
-                error("ReadCovMatrix: read error (%s,%d)\n", covfile.c_str(), m);
+            if (fc.read(dst, sizeof(double) * Ncoeff) != sizeof(double) * Ncoeff) {
+                error("ReadCovMatrix: read error (%s,%d)\n", fc.filename().c_str(), m);
                 prog_abort(1);
             }
         } else {
             // full storage: keep rows 0..j of each column j
             std::vector<double> col(Nmodes);
             for (int j = 0; j < Nmodes; j++) {
-                if (std::fread(col.data(), sizeof(double), Nmodes, fc) != (size_t)Nmodes) {
-                    error("ReadCovMatrix: read error (%s,%d,%d)\n", covfile.c_str(), m, j);
+                if (fc.read(col.data(), sizeof(double) * Nmodes) != sizeof(double) * Nmodes) {
+                    error("ReadCovMatrix: read error (%s,%d,%d)\n", fc.filename().c_str(), m, j);
                     prog_abort(1);
                 }
                 std::memcpy(dst + (size_t)j * (j + 1) / 2, col.data(), sizeof(double) * (j + 1));
             }
         }
     }
-    std::fclose(fc);
+    fc.close();
 
     info("Reading pixel values...\n");
     AbnHeader ph;

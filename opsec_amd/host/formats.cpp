@@ -265,4 +265,39 @@ void error(const char* fmt, ...) {
     std::fflush(stderr);
 }
 
+bool SplitReader::open(const std::string& path) {
+    close();
+    cur_ = path;
+    index_ = 0;
+    f_ = std::fopen(path.c_str(), "rb");
+    return f_ != nullptr;
+}
+
+void SplitReader::close() {
+    if (f_) std::fclose(f_);
+    f_ = nullptr;
+}
+
+void SplitReader::open_next() {
+    if (!f_) return;
+    std::fclose(f_);
+    f_ = nullptr;
+    index_++;
+    if (cur_.size() < 2 || index_ > 99) return;
+    cur_[cur_.size() - 2] = (char)('0' + index_ / 10);
+    cur_[cur_.size() - 1] = (char)('0' + index_ % 10);
+    f_ = std::fopen(cur_.c_str(), "rb");
+}
+
+size_t SplitReader::read(void* buf, size_t bytes) {
+    char* p = (char*)buf;
+    size_t done = 0;
+    while (done < bytes && f_ && !std::ferror(f_)) {
+        size_t n = std::fread(p + done, 1, bytes - done, f_);
+        done += n;
+        if (n == 0 && std::feof(f_)) open_next();
+    }
+    return done;
+}
+
 }  // namespace opsec_host

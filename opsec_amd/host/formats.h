@@ -61,6 +61,31 @@ size_t abn_write(FILE* f, const void* data, size_t n, const std::string& fmt, co
 bool abn_read_header(FILE* f, AbnHeader* h);
 void abn_byte_swap(char* data, size_t n, const std::string& fmt);
 
+// ---- files cut into pieces ------------------------------------------------
+// Sequential reader over a file that may have been cut into pieces by
+// `split -d -a 2`: when one piece ends, the next is the current name with its
+// last two characters replaced by the piece number 01, 02, ... (SplitFile,
+// src/SplitFile.cpp:77-170).  The abn header is read from stream(); data
+// reads may then run on into later pieces.  A file that was never split reads
+// exactly as with fread.
+class SplitReader {
+public:
+    SplitReader() = default;
+    SplitReader(const SplitReader&) = delete;
+    SplitReader& operator=(const SplitReader&) = delete;
+    ~SplitReader() { close(); }
+    bool open(const std::string& path);
+    void close();
+    FILE* stream() const { return f_; }
+    const std::string& filename() const { return cur_; }
+    size_t read(void* buf, size_t bytes);   // bytes actually read
+private:
+    void open_next();
+    std::string cur_;
+    int index_ = 0;
+    FILE* f_ = nullptr;
+};
+
 // ---- logging --------------------------------------------------------------
 void info(const char* fmt, ...);
 void error(const char* fmt, ...);

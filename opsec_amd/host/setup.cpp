@@ -56,13 +56,13 @@ std::vector<opsec_cell> read_cells(const std::string& path, Config* cellopts) {
 }
 
 std::vector<double> read_modes(const std::string& path, int Nmodes, int Ncells, bool exact, const char* prog) {
-    FILE* fm = std::fopen(path.c_str(), "rb");
-    if (!fm) {
+    SplitReader fm;
+    if (!fm.open(path)) {
         error("%s: cannot open modes file %s\n", prog, path.c_str());
         prog_abort(1);
     }
     AbnHeader mh;
-    if (!abn_read_header(fm, &mh)) prog_abort(1);
+    if (!abn_read_header(fm.stream(), &mh)) prog_abort(1);
     int hv = 0;
     if ((exact || mh.opts.has("Nmodes")) && (!mh.opts.get_int("Nmodes", &hv) || hv != Nmodes)) {
         error("%s: modefile has Nmodes = %s, expected %d\n", prog, mh.opts.get("Nmodes").c_str(), Nmodes);
@@ -79,13 +79,13 @@ std::vector<double> read_modes(const std::string& path, int Nmodes, int Ncells, 
     }
     std::vector<double> B((size_t)Nmodes * Ncells);
     for (int j = 0; j < Nmodes; j++) {
-        size_t nread = std::fread(&B[(size_t)j * Ncells], sizeof(double), Ncells, fm);
-        if (nread != (size_t)Ncells) {
-            error("%s: error reading modes (%s,%d,%zu,%d)\n", prog, path.c_str(), j, nread * sizeof(double), Ncells);
+        size_t nread = fm.read(&B[(size_t)j * Ncells], sizeof(double) * Ncells);
+        if (nread != sizeof(double) * Ncells) {
+            error("%s: error reading modes (%s,%d,%zu,%d)\n", prog, fm.filename().c_str(), j, nread, Ncells);
             prog_abort(1);
         }
     }
-    std::fclose(fm);
+    fm.close();
     if (mh.endian != abn_endian()) abn_byte_swap((char*)B.data(), B.size(), "d");
     return B;
 }

@@ -95,9 +95,56 @@ def _write_inputs(tmp_path, n=6, nb=3, lo=0.0, hi=150.0):
     return cells, cfile, np.loadtxt(pk)
 
 
+def _split(path, piece):
+    """Cut a file into pieces as `split -d -a 2 -b piece` does; returns the
+    first piece's name (path + ".00")."""
+    data = open(path, "rb").read()
+    for i in range(0, len(data), piece):
+        with open(f"{path}.{i // piece:02d}", "wb") as f:
+            f.write(data[i:i + piece])
+    return f"{path}.00"
+
+
+def test_split_pieces_named_like_split(tmp_path):
+    p = tmp_path / "x.abn"
+    p.write_bytes(bytes(range(250)))
+    first = _split(p, 100)
+    assert first.endswith("x.abn.00")
+    assert [len(open(f"{p}.{i:02d}", "rb").read()) for i in range(3)] == [100, 100, 50]
+
+
+def _estimate_inputs(tmp_path, nb=3, M=10):
+    """Model config, evalfile and pixfile for estimate, and a packed covfile
+    of nb bands (estimate reads all three before it opens the device)."""
+    _, cfile, _ = _write_inputs(tmp_path, 3, nb)
+    formats.write(tmp_path / "evals.abn", np.linspace(2.0, 1.0, M), "d", {"Nmodes": M})
+    formats.write(tmp_path / "pix.abn", np.ones(M), "d", comments=("Pixel values",))
+    K = M * (M + 1) // 2
+    formats.write(tmp_path / "cov.abn", np.arange(nb * K, dtype=np.float64), "d",
+                  {"Nmodes": M, "Nparams": nb, "packed": 1})
+    return cfile
+
+
+def test_estimate_reads_covfile_pieces(tmp_path):
+    """A covfile cut into pieces reads through (SplitFile); with a piece
+    missing the read fails with the reference's message."""
+    cfile = _estimate_inputs(tmp_path)
+    first = _split(tmp_path / "cov.abn", 500)
+    args = ["-c", str(cfile), "Nmodes=10", f"evalfile={tmp_path}/evals.abn", f"pixfile={tmp_path}/pix.abn",
+            f"estfile={tmp_path}/est.abn", f"covfile={first}"]
+    r = run("estimate", *args)
+    assert "ReadCovMatrix" not in r.stderr and "Reading pixel values" in r.stdout + r.stderr
+    os.remove(f"{tmp_path}/cov.abn.02")
+    r = run("estimate", *args)
+    assert r.returncode != 0 and "ReadCovMatrix: read error" in r.stderr
+
+
 @pytest.mark.gpu
 def test_pipeline_end_to_end_vs_oracle(tmp_path):
-    n, nb, M = 6, 3, 40
+    """klt -> comma -> estimate.  comma reads the modefile and estimate the
+    covfile cut into pieces (SplitFile); with 6 bands comma projects and
+    writes in two groups."""
+    n, nb, M = 6, 6, 40
     cells, cfile, kP = _write_inputs(tmp_path, n, nb)
     N = len(cells)
     r = run("klt", "-c", str(cfile), f"Nmodes={M}", f"modefile={tmp_path}/modes.abn", f"evalfile={tmp_path}/evals.abn",
@@ -122,7 +169,8 @@ def test_pipeline_end_to_end_vs_oracle(tmp_path):
     np.testing.assert_allclose(lam, w, atol=1e-9 * w[0])
     np.testing.assert_allclose(B @ mats[0] @ B.T, np.diag(lam), atol=1e-9 * w[0])
 
-    r = run("comma", "-c", str(cfile), f"Nmodes={M}", f"modefile={tmp_path}/modes.abn", f"covfile={tmp_path}/cov.abn")
+    modes_split = _split(tmp_path / "modes.abn", 20000)
+    r = run("comma", "-c", str(cfile), f"Nmodes={M}", f"modefile={modes_split}", f"covfile={tmp_path}/cov.abn")
     assert r.returncode == 0, r.stderr
     copts, (D,) = formats.read_doubles(tmp_path / "cov.abn")
     assert copts == {"Nmodes": str(M), "Nparams": str(nb), "packed": "1"}
@@ -136,7 +184,8 @@ def test_pipeline_end_to_end_vs_oracle(tmp_path):
     y = np.sqrt(1 + lam) * np.random.default_rng(42).normal(size=M)
     formats.write(tmp_path / "pix.abn", y, "d", comments=("Pixel values",))
     r = run("estimate", "-c", str(cfile), f"Nmodes={M}", f"evalfile={tmp_path}/evals.abn",
-            f"covfile={tmp_path}/cov.abn", f"pixfile={tmp_path}/pix.abn", f"estfile={tmp_path}/est.abn")
+            f"covfile={_split(tmp_path / 'cov.abn', 30000)}", f"pixfile={tmp_path}/pix.abn",
+            f"estfile={tmp_path}/est.abn")
     assert r.returncode == 0, r.stderr
     assert "p_n          error" in r.stdout
     eo, blocks = formats.read_doubles(tmp_path / "est.abn")

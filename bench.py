@@ -228,6 +228,9 @@ def main():
                     help="KL modes (setup, untimed): the periodic box's analytic Fourier modes, or the klt solver")
     ap.add_argument("--nt-gen", type=int, default=0, choices=[0, 1],
                     help="stage 1 generates S_p from the pair table (1) or reads S_p planes (0)")
+    ap.add_argument("--projection", default="dense", choices=["dense", "toeplitz"],
+                    help="split-BF16x3 projection: dense S_p (2 N^2 M stage 1) or Toeplitz (DFT along x, "
+                         "2 (Nx + 1) slab GEMMs); the FP64 path is always dense")
     ap.add_argument("--s2-split", type=int, default=8, help="split-K factor of the split-BF16x3 stage 2")
     ap.add_argument("--nt-group", type=int, default=8, help="split-BF16x3 GEMM tile-group edge")
     ap.add_argument("--nt-remap", type=int, default=0, choices=[0, 1], help="deal tile groups to XCDs")
@@ -256,6 +259,7 @@ def main():
     ctx.lib.opsec_internal_set_nt_order(args.nt_group, args.nt_remap)
     ctx.lib.opsec_internal_set_nt_gen(args.nt_gen)
     ctx.lib.opsec_internal_set_stage2_split(args.s2_split)
+    ctx.lib.opsec_internal_set_projection(1 if args.projection == "toeplitz" else 0)
     ncu, arch = ctx.device_info()
     log(f"device {arch} with {ncu} CUs; workload {args.config}: {N} cells, {nb} bands, {M} modes")
     ctx.model_real(k, P, lo, hi, L.OPSEC_XI_CLOSED)
@@ -324,7 +328,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.config}: {n}^3 periodic box 400 Mpc/h, nbar 3e-4, {nb} linear bands "
                                f"[{cfg['kmin']},{cfg['kmax']}] h/Mpc, {M} KL modes, {cfg['nodes']}^3 Gauss pair "
-                               f"quadrature", "path": main_path},
+                               f"quadrature", "path": main_path,
+                   "projection": args.projection if main_path == "surrogate" else "dense"},
     }
     if "fp64" in results:
         k64 = results["fp64"]["kms"]
@@ -344,7 +349,27 @@ def main():
     # roofline of the dominant kernel: the stage-1 projection GEMM C_p = S_p B'
     # (useful flops 2 N^2 M per band), average launch time from HIP events on
     # the context's stream
-    if main_path == "surrogate":
+    if main_path == "surrogate" and args.projection == "toeplitz":
+        # stage 1 = 2 Nx real GEMMs of G x G by G x M per band (G = Ny Nz), stage 2 = N M^2
+        kg_ms, k1_ms, k2_ms, launches = [float(x) for x in R["kms"][2:6]]
+        G = n * n
+        s1_flops, s2_flops = 2.0 * (2 * n) * G * G * M, float(N) * M * M
+        dom = "stage1" if k1_ms >= k2_ms else "stage2"
+        per_launch = (k1_ms if dom == "stage1" else k2_ms) / launches
+        achieved = (s1_flops if dom == "stage1" else s2_flops) / (per_launch / 1e3) / 1e12
+        peak = PEAK_TF["bf16"] / 3
+        out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                           "frac": achieved / peak, "traffic": None,
+                           "kernel": ("gemm3_kernel<ASrcPlanes,EpiYhat> (Toeplitz stage 1, 2 (Nx + 1) slab GEMMs "
+                                      "per launch)" if dom == "stage1" else
+                                      "gemm3_kernel<ASrcPlanes,EpiPacked> (stage 2, split-K, one launch per band)"),
+                           "launch_ms": per_launch,
+                           "note": ("useful flops per launch: stage 1 2 (2 Nx) G^2 M, stage 2 N M^2; "
+                                    "peak = dense bf16 2516.6 TF / 3 products"),
+                           "per_step_ms": {"that_and_inverse_dft": kg_ms, "stage1_gemm": k1_ms, "stage2_gemm": k2_ms},
+                           "stage1_tflops": s1_flops / (k1_ms / launches / 1e3) / 1e12,
+                           "stage2_tflops": s2_flops / (k2_ms / launches / 1e3) / 1e12}
+    elif main_path == "surrogate":
         kg_ms, k1_ms, k2_ms, launches = [float(x) for x in R["kms"][2:6]]
         per_launch = k1_ms / launches
         achieved = 2.0 * N * N * M / (per_launch / 1e3) / 1e12

@@ -37,7 +37,7 @@ struct ProjState {
     size_t cap_C = 0;
     void* d_P = nullptr;         // packed results, nb x M(M+1)/2 (double)
     size_t cap_P = 0;
-    double* d_Ppart = nullptr;   // split-K partials of one band's stage 2
+    float* d_Ppart = nullptr;    // split-K partials of one band's stage 2 (FP32)
     size_t cap_Ppart = 0;
     int nb = 0;                  // bands in d_P
     int prec = 0;
@@ -58,6 +58,26 @@ struct ProjState {
     // per-kernel sums over the bands of the last split-BF16x3 projection (ms)
     float last_ms_gen = 0, last_ms_gemm1 = 0, last_ms_gemm2 = 0;
     int last_launches = 0;
+    // Toeplitz projection of box surveys (grid order, DFT along x)
+    bool toep_valid = false;     // grid planes of B' and its DFT match the uploaded modes
+    int* d_gmap = nullptr;       // grid index -> cell number, -1 where the grid has no cell
+    size_t cap_gmap = 0;
+    __bf16* d_Bg = nullptr;      // B' in grid order: Mp x Pg, hi plane then lo plane
+    size_t cap_Bg = 0;
+    __bf16* d_Bhat = nullptr;    // DFT of B' along x: nbat planes Mp x Gp (hi), then nbat (lo)
+    size_t cap_Bhat = 0;
+    __bf16* d_That = nullptr;    // one band's That_k, k = 0..n1: Gr x Gp each (hi), then (lo)
+    size_t cap_That = 0;
+    double* d_Qhat = nullptr;    // one band's Qhat_k(|dy|, |dz|): (n1 + 1) x G
+    size_t cap_Qhat = 0;
+    float* d_Yhat = nullptr;     // stage-1 products in frequency: M x nbat x G
+    size_t cap_Yhat = 0;
+    __bf16* d_Cg = nullptr;      // Y = S_p B' in grid order: Mp x Pg, hi then lo
+    size_t cap_Cg = 0;
+    double* d_trig = nullptr;    // cos(2 pi j / L), then sin(2 pi j / L), j < L
+    size_t cap_trig = 0;
+    float* d_W = nullptr;        // inverse-DFT weights, transposed: nbat x n1p (n1p = n1 rounded up to 4)
+    size_t cap_W = 0;
 };
 
 // ---- FP64 / FP32 MFMA GEMM -----------------------------------------------------
@@ -332,6 +352,7 @@ struct EpiPlanes {
     __bf16* lo;
     size_t ld;
     __device__ __forceinline__ EpiPlanes part(int) const { return *this; }
+    __device__ __forceinline__ EpiPlanes batch(int) const { return *this; }
     __device__ __forceinline__ void put4(int m, int n, const f4_t& v) const {
         typedef __bf16 bf4_t __attribute__((ext_vector_type(4)));
         bf4_t h, l;
@@ -359,11 +380,34 @@ struct EpiPacked {
         e.p += (size_t)s * part_stride;
         return e;
     }
+    __device__ __forceinline__ EpiPacked batch(int) const { return *this; }
     __device__ __forceinline__ void put4(int m, int n, const f4_t& v) const {
         if (n >= M) return;
 #pragma unroll
         for (int r = 0; r < 4; r++)
             if (m + r <= n) p[(size_t)n * (n + 1) / 2 + m + r] = (double)v[r];
+    }
+    static constexpr bool upper = true;
+};
+
+// Split-K partials of the packed upper triangle, FP32 (the accumulators'
+// own precision, so nothing is lost); part s at p + s * part_stride, summed in
+// FP64 by sum_parts_kernel.
+struct EpiPackedParts {
+    float* p;
+    int M;
+    size_t part_stride;
+    __device__ __forceinline__ EpiPackedParts part(int s) const {
+        EpiPackedParts e = *this;
+        e.p += (size_t)s * part_stride;
+        return e;
+    }
+    __device__ __forceinline__ EpiPackedParts batch(int) const { return *this; }
+    __device__ __forceinline__ void put4(int m, int n, const f4_t& v) const {
+        if (n >= M) return;
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            if (m + r <= n) p[(size_t)n * (n + 1) / 2 + m + r] = v[r];
     }
     static constexpr bool upper = true;
 };
@@ -582,7 +626,7 @@ __global__ __launch_bounds__(256) void gen_planes_kernel(const unsigned* __restr
 // 2 (m) x 4 (n) arrangement, each owning 8 x 4 fragments of 16 x 16; K in
 // steps of 32 through two LDS stages of 64 KB, one barrier per step: the next
 // step's global loads (or table gathers) are issued before the current step's
-// MFMAs and stored to the other stage after them.
+// MFMAs and stored to the other stage before its last two row fragments.
 
 constexpr int G3 = 256;          // tile edge
 constexpr int G3_STAGE = G3 * PK; // bf16 per plane and stage
@@ -594,6 +638,7 @@ struct ASrcPlanes {
     struct Regs {
         bf8_t v[4];
     };
+    __device__ __forceinline__ ASrcPlanes at(size_t off) const { return ASrcPlanes{hi + off, lo + off, ld}; }
     __device__ __forceinline__ void init(int, Regs&) const {}
     // thread t: rows (t >> 2) and (t >> 2) + 128, chunk t & 3, both planes
     __device__ __forceinline__ void fetch(int m0, int k, Regs& R) const {
@@ -622,6 +667,7 @@ struct ASrcTable {
         unsigned xa, ya, za;
         bool rowv;
     };
+    __device__ __forceinline__ ASrcTable at(size_t) const { return *this; }
     // thread t: row t >> 1 of the tile, columns 16 (t & 1) .. + 15 of the step
     __device__ __forceinline__ void init(int m0, Regs& R) const {
         const unsigned ca = cpk[m0 + (threadIdx.x >> 1)];
@@ -664,16 +710,23 @@ struct ASrcTable {
     }
 };
 
+// Batched form: workgroup id = (batch * nmb * nnb + tile) * ksplit + split;
+// batch b multiplies A at offset (b >> a_shift) * a_bstride by B at offset
+// b * b_bstride into the epilogue's batch b.
 template <typename ASrc, typename Epi>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void gemm3_kernel(
-    ASrc A, const __bf16* __restrict__ Bh, const __bf16* __restrict__ Bl, size_t ldb, int K, int nmb, int nnb, int G,
-    int remap, int ksplit, Epi E0) {
+    ASrc A0, const __bf16* __restrict__ Bh0, const __bf16* __restrict__ Bl0, size_t ldb, int K, int nmb, int nnb, int G,
+    int remap, int ksplit, Epi E0, size_t a_bstride, size_t b_bstride, int a_shift) {
     extern __shared__ __attribute__((aligned(16))) __bf16 g3_lds[];   // 2 stages x (Ah, Al, Bh, Bl)
     // split-K: workgroup b takes part b % ksplit of the K steps of tile b / ksplit
     const int split = blockIdx.x % ksplit;
-    const Epi E = E0.part(split);
+    const int tiles = nmb * nnb, id = blockIdx.x / ksplit, bat = id / tiles;
+    const Epi E = E0.part(split).batch(bat);
+    const ASrc A = A0.at((size_t)(bat >> a_shift) * a_bstride);
+    const __bf16* __restrict__ Bh = Bh0 + (size_t)bat * b_bstride;
+    const __bf16* __restrict__ Bl = Bl0 + (size_t)bat * b_bstride;
     int mb, nbk;
-    tile_of(blockIdx.x / ksplit, nmb, nnb, G, remap, mb, nbk);
+    tile_of(id - bat * tiles, nmb, nnb, G, remap, mb, nbk);
     const int m0 = mb * G3, n0 = nbk * G3;
     if (Epi::upper && m0 > n0 + G3 - 1) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -735,6 +788,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void g
         }
 #pragma unroll
         for (int i = 0; i < 8; i++) {
+            // the next step's LDS stores go out while the last two row
+            // fragments' MFMAs run (measured 2-3 % faster than after them)
+            if (i == 6 && more) {
+                A.store(stage(cur ^ 1, 0), stage(cur ^ 1, 1), ra);
+                storeB(stage(cur ^ 1, 2), stage(cur ^ 1, 3));
+            }
             const int o = lds_slot(wm + i * 16 + fr, kq);
             const bf8_t ah = *(const bf8_t*)&sAh[o];
             const bf8_t al = *(const bf8_t*)&sAl[o];
@@ -747,10 +806,6 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void g
 #pragma unroll
             for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
         }
-        if (more) {
-            A.store(stage(cur ^ 1, 0), stage(cur ^ 1, 1), ra);
-            storeB(stage(cur ^ 1, 2), stage(cur ^ 1, 3));
-        }
         __syncthreads();
     }
 #pragma unroll
@@ -761,12 +816,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void g
 
 constexpr size_t G3_LDS_BYTES = (size_t)8 * G3_STAGE * sizeof(__bf16);   // 128 KB
 
-// out[i] = sum_s part[s][i] in a fixed order (split-K reduction of stage 2)
-__global__ void sum_parts_kernel(const double* __restrict__ part, int nparts, size_t n, double* __restrict__ out) {
+// out[i] = sum_s part[s][i] in FP64, in a fixed order (split-K reduction of stage 2)
+__global__ void sum_parts_kernel(const float* __restrict__ part, int nparts, size_t n, double* __restrict__ out) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double acc = 0.0;
-    for (int s = 0; s < nparts; s++) acc += part[(size_t)s * n + i];
+    for (int s = 0; s < nparts; s++) acc += (double)part[(size_t)s * n + i];
     out[i] = acc;
 }
 
@@ -777,6 +832,212 @@ __global__ void scale_rows_kernel(const double* B, const double* nbar, int N, in
     if (i >= tot) return;
     const size_t a = i % N, n = i / N;
     Bw[n * ldw + a] = B[i] * sqrt(nbar[a]);
+}
+
+// ---- Toeplitz projection of box surveys ---------------------------------------------
+//
+// On a box grid S_p[a][b] = Q_p(|dx|, |dy|, |dz|) (the cache key of
+// src/sig.cpp:109-112).  Along x, S_p is a symmetric block-Toeplitz matrix of
+// blocks T_d (d = |dx|), each G x G over the (y, z) slab (G = n2 n3).  Embedded
+// in a circulant of length L = 2 n1 (T_d at d and L - d, zero at n1), the product
+// Y = S_p B' becomes, after a length-L DFT along x,
+//     Yhat_k = That_k Bhat_k,  That_k = sum_d w_d cos(2 pi d k / L) T_d  (real),
+// for k = 0..n1 (the others follow by Hermitian symmetry), and Y is the inverse
+// DFT of Yhat at x < n1.  That_k is a 2-level Toeplitz matrix generated from
+// Qhat_k(|dy|, |dz|).  Stage 1 becomes nbat = 2 (n1 + 1) GEMMs of G x G by G x M
+// (real part R_k and imaginary part I_k of Bhat_k; I_0 = I_n1 = 0 are kept so
+// that batch b uses That_(b/2)), about n1 / 2 times fewer flops than N x N by
+// N x M.  Grid positions without a cell have B' = 0, so any cell subset of the
+// grid is exact.
+
+// B' in grid order, split: planes [m][g] of Mp x Pg, zero where no cell / padding.
+__global__ void grid_planes_kernel(const double* __restrict__ Bw, size_t ldw, const int* __restrict__ gmap, int Ng,
+                                   int M, __bf16* __restrict__ hi, __bf16* __restrict__ lo, size_t Pg) {
+    const size_t g0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 8;
+    const int m = blockIdx.y;
+    if (g0 >= Pg) return;
+    bf8_t h, l;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const size_t g = g0 + q;
+        double v = 0.0;
+        if (m < M && g < (size_t)Ng) {
+            const int a = gmap[g];
+            if (a >= 0) v = Bw[(size_t)m * ldw + a];
+        }
+        __bf16 x, y;
+        split2((float)v, x, y);
+        h[q] = x;
+        l[q] = y;
+    }
+    *(bf8_t*)(hi + (size_t)m * Pg + g0) = h;
+    *(bf8_t*)(lo + (size_t)m * Pg + g0) = l;
+}
+
+// Bhat planes: batch b = 2k holds R_k = sum_x cos(2 pi x k / L) B'_x, b = 2k + 1
+// holds I_k = -sum_x sin(2 pi x k / L) B'_x (FP64 sums, then split).  Block: one
+// mode, 256 slab positions; the n1 grid values of each position sit in LDS.
+__global__ __launch_bounds__(256) void bhat_kernel(const double* __restrict__ Bw, size_t ldw,
+                                                    const int* __restrict__ gmap, int n1, int G, int M,
+                                                    const double* __restrict__ trig, __bf16* __restrict__ hi,
+                                                    __bf16* __restrict__ lo, size_t Gp, size_t plane, int nbat) {
+    extern __shared__ double bh_v[];   // n1 x 256
+    const int m = blockIdx.y, t = threadIdx.x;
+    const int r = blockIdx.x * 256 + t;
+    const int L = 2 * n1;
+    for (int x = 0; x < n1; x++) {
+        double v = 0.0;
+        if (m < M && r < G) {
+            const int a = gmap[(size_t)x * G + r];
+            if (a >= 0) v = Bw[(size_t)m * ldw + a];
+        }
+        bh_v[x * 256 + t] = v;
+    }
+    if (r >= (int)Gp) return;
+    for (int b = 0; b < nbat; b++) {
+        const int k = b >> 1;
+        double acc = 0.0;
+        if (b & 1) {
+            for (int x = 0; x < n1; x++) acc -= trig[L + (x * k) % L] * bh_v[x * 256 + t];
+        } else {
+            for (int x = 0; x < n1; x++) acc += trig[(x * k) % L] * bh_v[x * 256 + t];
+        }
+        __bf16 h, l;
+        split2((float)acc, h, l);
+        const size_t o = (size_t)b * plane + (size_t)m * Gp + r;
+        hi[o] = h;
+        lo[o] = l;
+    }
+}
+
+// Qhat_k(e) = Q(0, e) + 2 sum_{d=1}^{n1-1} cos(2 pi d k / L) Q(d, e), e = |dy| n3 + |dz|.
+__global__ void qhat_kernel(const double* __restrict__ Q, int n1, int G, const double* __restrict__ trig,
+                            double* __restrict__ qhat) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (n1 + 1) * G) return;
+    const int k = i / G, e = i - k * G, L = 2 * n1;
+    double acc = Q[e];
+    for (int d = 1; d < n1; d++) acc += 2.0 * trig[(d * k) % L] * Q[(size_t)d * G + e];
+    qhat[i] = acc;
+}
+
+// That_k planes: row r, column c (slab positions) -> Qhat_k(|y_r - y_c| n3 + |z_r - z_c|),
+// zero outside G x G; k = blockIdx.z, 8 columns per thread.
+__global__ void that_kernel(const double* __restrict__ qhat, int G, int n3, __bf16* __restrict__ hi,
+                            __bf16* __restrict__ lo, size_t Gr, size_t Gp) {
+    const size_t c0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 8;
+    const int r = blockIdx.y, k = blockIdx.z;
+    if (c0 >= Gp) return;
+    const double* qk = qhat + (size_t)k * G;
+    const int yr = r / n3, zr = r - yr * n3;
+    bf8_t h, l;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int c = (int)(c0 + q);
+        float v = 0.f;
+        if (r < G && c < G) {
+            const int yc = c / n3, zc = c - yc * n3;
+            v = (float)qk[abs(yr - yc) * n3 + abs(zr - zc)];
+        }
+        __bf16 x, y;
+        split2(v, x, y);
+        h[q] = x;
+        l[q] = y;
+    }
+    const size_t o = ((size_t)k * Gr + r) * Gp + c0;
+    *(bf8_t*)(hi + o) = h;
+    *(bf8_t*)(lo + o) = l;
+}
+
+// Stage-1 epilogue of the Toeplitz path: D[r][m] of batch b -> Yhat[m][b][r] (FP32).
+struct EpiYhat {
+    float* y;
+    int M, G, nbat;
+    __device__ __forceinline__ EpiYhat part(int) const { return *this; }
+    __device__ __forceinline__ EpiYhat batch(int b) const {
+        EpiYhat e = *this;
+        e.y += (size_t)b * G;
+        return e;
+    }
+    __device__ __forceinline__ void put4(int m, int n, const f4_t& v) const {
+        if (n >= M) return;
+        float* d = y + (size_t)n * nbat * G + m;
+        if (m + 3 < G && (G & 3) == 0) {
+            *(f4_t*)d = v;
+            return;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            if (m + r < G) d[r] = v[r];
+    }
+    static constexpr bool upper = false;
+};
+
+// Inverse DFT along x: Y[m][x G + r] = sum_b W[x][b] Yhat[m][b][r] for x < n1,
+// written as split planes (the stage-2 operand).  Block: one mode, 128 slab
+// positions; the Yhat slice and W (stored as [b][x]) sit in LDS; each thread
+// forms a 4 (x) x 4 (r) block, so two 16-byte LDS reads feed 16 FMAs, and
+// writes 4 consecutive positions per plane with one 8-byte store.
+__global__ __launch_bounds__(256) void idft_kernel(const float* __restrict__ yhat, int nbat, int G, int n1,
+                                                    const float* __restrict__ W, __bf16* __restrict__ hi,
+                                                    __bf16* __restrict__ lo, size_t Pg) {
+    extern __shared__ float id_s[];   // nbat x 128 Yhat, then nbat x n1p W^T
+    const int n1p = (n1 + 3) & ~3;
+    float* sY = id_s;
+    float* sW = id_s + nbat * 128;
+    const int m = blockIdx.y, r0 = blockIdx.x * 128, t = threadIdx.x;
+    const float* ym = yhat + (size_t)m * nbat * G;
+    if ((G & 3) == 0) {
+        for (int i = t; i < nbat * 32; i += 256) {
+            const int b = i >> 5, r = (i & 31) * 4;
+            *(f4_t*)&sY[b * 128 + r] = (r0 + r < G) ? *(const f4_t*)&ym[(size_t)b * G + r0 + r] : f4_t{0, 0, 0, 0};
+        }
+    } else {
+        for (int i = t; i < nbat * 128; i += 256) {
+            const int b = i >> 7, r = i & 127;
+            sY[i] = (r0 + r < G) ? ym[(size_t)b * G + r0 + r] : 0.f;
+        }
+    }
+    for (int i = t; i < nbat * n1p / 4; i += 256) *(f4_t*)&sW[4 * i] = *(const f4_t*)&W[4 * i];
+    __syncthreads();
+    typedef __bf16 bf4_t __attribute__((ext_vector_type(4)));
+    const int rq = (t & 31) * 4;
+    for (int xq = (t >> 5) * 4; xq < n1; xq += 32) {
+        f4_t acc[4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+        for (int b = 0; b < nbat; b++) {
+            const f4_t y = *(const f4_t*)&sY[b * 128 + rq];
+            const f4_t w = *(const f4_t*)&sW[b * n1p + xq];
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc[u] += w[u] * y;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int x = xq + u;
+            if (x >= n1) break;
+            const size_t o = (size_t)m * Pg + (size_t)x * G + r0 + rq;
+            if ((G & 3) == 0 && r0 + rq + 3 < G) {
+                bf4_t h, l;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    __bf16 a, c;
+                    split2(acc[u][q], a, c);
+                    h[q] = a;
+                    l[q] = c;
+                }
+                *(bf4_t*)(hi + o) = h;
+                *(bf4_t*)(lo + o) = l;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    if (r0 + rq + q >= G) break;
+                    __bf16 a, c;
+                    split2(acc[u][q], a, c);
+                    hi[o + q] = a;
+                    lo[o + q] = c;
+                }
+            }
+        }
+    }
 }
 
 // ---- host --------------------------------------------------------------------------
@@ -810,7 +1071,8 @@ static int g_s2_split = 8;
 
 template <typename ASrc, typename Epi>
 static void launch_g3(opsec_ctx* ctx, const ASrc& A, int arows, const __bf16* B, int brows, size_t ld, int K,
-                      const Epi& E, int ksplit = 1) {
+                      const Epi& E, int ksplit = 1, int nbat = 1, size_t a_bstride = 0, size_t b_bstride = 0,
+                      int a_shift = 0, size_t b_lo_offset = 0) {
     static bool attr = false;
     if (!attr) {   // 128 KB of dynamic LDS per workgroup
         (void)hipFuncSetAttribute((const void*)gemm3_kernel<ASrcPlanes, EpiPlanes>,
@@ -819,12 +1081,19 @@ static void launch_g3(opsec_ctx* ctx, const ASrc& A, int arows, const __bf16* B,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)G3_LDS_BYTES);
         (void)hipFuncSetAttribute((const void*)gemm3_kernel<ASrcTable, EpiPlanes>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)G3_LDS_BYTES);
+        (void)hipFuncSetAttribute((const void*)gemm3_kernel<ASrcPlanes, EpiYhat>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)G3_LDS_BYTES);
+        (void)hipFuncSetAttribute((const void*)gemm3_kernel<ASrcPlanes, EpiPackedParts>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)G3_LDS_BYTES);
         attr = true;
     }
     const int nmb = arows / G3, nnb = brows / G3;
     ksplit = std::max(1, std::min(ksplit, K / PK));
-    hipLaunchKernelGGL((gemm3_kernel<ASrc, Epi>), dim3(nmb * nnb * ksplit), dim3(512), G3_LDS_BYTES, ctx->stream, A,
-                       B, B + (size_t)brows * ld, ld, K, nmb, nnb, g_nt_group, g_nt_remap, ksplit, E);
+    // the lo plane follows the hi plane (brows x ld) unless an offset is given
+    const size_t blo = b_lo_offset ? b_lo_offset : (size_t)brows * ld;
+    hipLaunchKernelGGL((gemm3_kernel<ASrc, Epi>), dim3(nmb * nnb * ksplit * nbat), dim3(512), G3_LDS_BYTES, ctx->stream,
+                       A, B, B + blo, ld, K, nmb, nnb, g_nt_group, g_nt_remap, ksplit, E, a_bstride, b_bstride,
+                       a_shift);
 }
 
 // C = A B^T (stage 1, epilogue to split planes) or P = A B^T (stage 2, packed).
@@ -859,7 +1128,7 @@ static int run_project_bf16x3(opsec_ctx* ctx, int nb, const int* funcs) {
     if ((rc = dev_realloc(ctx, (void**)&ps->d_Cp, &ps->cap_Cp, 2 * Mp * Pp * sizeof(__bf16)))) return rc;
     if ((rc = dev_realloc(ctx, &ps->d_P, &ps->cap_P, (size_t)nb * packed_size(M) * 8))) return rc;
     if (g_s2_split > 1 &&
-        (rc = dev_realloc(ctx, (void**)&ps->d_Ppart, &ps->cap_Ppart, (size_t)g_s2_split * packed_size(M) * 8)))
+        (rc = dev_realloc(ctx, (void**)&ps->d_Ppart, &ps->cap_Ppart, (size_t)g_s2_split * packed_size(M) * 4)))
         return rc;
     if (!ps->bp_valid) {
         // B' is column-major N x M: row n of the plane is mode n, cells contiguous
@@ -906,7 +1175,7 @@ static int run_project_bf16x3(opsec_ctx* ctx, int nb, const int* funcs) {
         const int s2 = std::max(1, std::min(g_s2_split, (int)(Kp / PK)));
         if (g_nt_tile >= 2 && s2 > 1) {
             const size_t np = (size_t)packed_size(M);
-            EpiPacked toP{ps->d_Ppart, M, np};
+            EpiPackedParts toP{ps->d_Ppart, M, np};
             launch_g3(ctx, ASrcPlanes{ps->d_Bp, ps->d_Bp + Mp * Pp, Pp}, (int)Mp, ps->d_Cp, (int)Mp, Pp, (int)Kp, toP,
                       s2);
             hipLaunchKernelGGL(sum_parts_kernel, blocks_for(np), dim3(256), 0, ctx->stream, ps->d_Ppart, s2, np, Pz);
@@ -944,8 +1213,151 @@ static int run_project_bf16x3(opsec_ctx* ctx, int nb, const int* funcs) {
     return OPSEC_OK;
 }
 
+// Projection method of the split-BF16x3 path: 0 = dense (S_p generated N x N),
+// 1 = Toeplitz (DFT along x, box grids).
+static int g_projection = 0;
+
+// cos / sin of 2 pi j / L, exact at the quarter points
+static void trig_table(int L, std::vector<double>& t) {
+    t.assign(2 * (size_t)L, 0.0);
+    for (int j = 0; j < L; j++) {
+        double c, sn;
+        if (j == 0) { c = 1; sn = 0; }
+        else if (2 * j == L) { c = -1; sn = 0; }
+        else if (4 * j == L) { c = 0; sn = 1; }
+        else if (4 * j == 3 * L) { c = 0; sn = -1; }
+        else { c = std::cos(2.0 * M_PI * j / L); sn = std::sin(2.0 * M_PI * j / L); }
+        t[j] = c;
+        t[L + j] = sn;
+    }
+}
+
+static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
+    ProjState* ps = ctx->proj;
+    const SurveyDev& sv = ctx->sv;
+    const int N = ctx->ncells, M = ctx->nmodes;
+    const int n1 = sv.n1, G = sv.n2 * sv.n3, Ng = n1 * G, L = 2 * n1, nbat = 2 * (n1 + 1);
+    if (sv.coordsys != OPSEC_COORD_CARTESIAN) return set_error(ctx, OPSEC_EINVAL, "Toeplitz projection needs a box survey");
+    if (n1 > 64) return set_error(ctx, OPSEC_EINVAL, "Toeplitz projection supports Nx <= 64 (Nx = %d)", n1);
+    const size_t Mp = round_up(M, 256), Gr = round_up(G, 256), Gk = round_up(G, 32), Gp = plane_pitch(Gk);
+    const size_t Kg = round_up(Ng, 256), Pg = plane_pitch(Kg);
+    const size_t plane_b = Mp * Gp, plane_t = Gr * Gp;
+    int rc;
+    if ((rc = dev_realloc(ctx, (void**)&ps->d_Bg, &ps->cap_Bg, 2 * Mp * Pg * sizeof(__bf16)))) return rc;
+    if ((rc = dev_realloc(ctx, (void**)&ps->d_Bhat, &ps->cap_Bhat, 2 * (size_t)nbat * plane_b * sizeof(__bf16))))
+        return rc;
+    if ((rc = dev_realloc(ctx, (void**)&ps->d_That, &ps->cap_That, 2 * (size_t)(n1 + 1) * plane_t * sizeof(__bf16))))
+        return rc;
+    if ((rc = dev_realloc(ctx, (void**)&ps->d_Qhat, &ps->cap_Qhat, (size_t)(n1 + 1) * G * 8))) return rc;
+    if ((rc = dev_realloc(ctx, (void**)&ps->d_Yhat, &ps->cap_Yhat, (size_t)M * nbat * G * 4))) return rc;
+    if ((rc = dev_realloc(ctx, &ps->d_P, &ps->cap_P, (size_t)nb * packed_size(M) * 8))) return rc;
+    const int s2 = std::max(1, std::min(g_s2_split, (int)(Kg / PK)));
+    if ((rc = dev_realloc(ctx, (void**)&ps->d_Ppart, &ps->cap_Ppart, (size_t)s2 * packed_size(M) * 4))) return rc;
+    if (!ps->toep_valid) {
+        // grid map (cells must occupy distinct grid positions)
+        std::vector<int> gmap((size_t)Ng, -1);
+        for (int a = 0; a < N; a++) {
+            const opsec_cell& c = ctx->h_cells[a];
+            const size_t g = ((size_t)c.d1 * sv.n2 + c.d2) * sv.n3 + c.d3;
+            if (gmap[g] >= 0)
+                return set_error(ctx, OPSEC_EINVAL, "cells %d and %d share a grid position", gmap[g], a);
+            gmap[g] = a;
+        }
+        if ((rc = dev_realloc(ctx, (void**)&ps->d_gmap, &ps->cap_gmap, (size_t)Ng * 4))) return rc;
+        std::vector<double> trig;
+        trig_table(L, trig);
+        if ((rc = dev_realloc(ctx, (void**)&ps->d_trig, &ps->cap_trig, trig.size() * 8))) return rc;
+        // inverse weights W[x][b]: (c_k / L) cos and -(c_k / L) sin, c_0 = c_n1 = 1, else 2
+        // stored transposed and padded, [b][x] with x < n1p = n1 rounded up to 4
+        const int n1p = (n1 + 3) & ~3;
+        std::vector<float> W((size_t)nbat * n1p, 0.f);
+        for (int x = 0; x < n1; x++)
+            for (int b = 0; b < nbat; b++) {
+                const int k = b >> 1, j = (x * k) % L;
+                const double ck = (k == 0 || k == n1) ? 1.0 : 2.0;
+                W[(size_t)b * n1p + x] = (float)((b & 1) ? -ck * trig[L + j] / L : ck * trig[j] / L);
+            }
+        if ((rc = dev_realloc(ctx, (void**)&ps->d_W, &ps->cap_W, W.size() * 4))) return rc;
+        if ((rc = dev_realloc(ctx, (void**)&ps->d_Cg, &ps->cap_Cg, 2 * Mp * Pg * sizeof(__bf16)))) return rc;
+        OPSEC_HIP_TRY(ctx, stream_copy(ctx, ps->d_gmap, gmap.data(), gmap.size() * 4, hipMemcpyHostToDevice));
+        OPSEC_HIP_TRY(ctx, stream_copy(ctx, ps->d_trig, trig.data(), trig.size() * 8, hipMemcpyHostToDevice));
+        OPSEC_HIP_TRY(ctx, stream_copy(ctx, ps->d_W, W.data(), W.size() * 4, hipMemcpyHostToDevice));
+        // the inverse DFT writes rows m < M and columns g < Ng of Cg; the rest stays zero
+        OPSEC_HIP_TRY(ctx, hipMemsetAsync(ps->d_Cg, 0, 2 * Mp * Pg * sizeof(__bf16), ctx->stream));
+        hipLaunchKernelGGL(grid_planes_kernel, dim3((unsigned)((Pg / 8 + 255) / 256), (unsigned)Mp), dim3(256), 0,
+                           ctx->stream, ps->d_Bw, ps->ldw, ps->d_gmap, Ng, M, ps->d_Bg, ps->d_Bg + Mp * Pg, Pg);
+        const size_t bh_lds = (size_t)n1 * 256 * 8;
+        OPSEC_HIP_TRY(ctx, hipFuncSetAttribute((const void*)bhat_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)bh_lds));
+        hipLaunchKernelGGL(bhat_kernel, dim3((unsigned)((Gp + 255) / 256), (unsigned)Mp), dim3(256),
+                           bh_lds, ctx->stream, ps->d_Bw, ps->ldw, ps->d_gmap, n1, G, M, ps->d_trig,
+                           ps->d_Bhat, ps->d_Bhat + (size_t)nbat * plane_b, Gp, plane_b, nbat);
+        OPSEC_HIP_TRY(ctx, hipGetLastError());
+        ps->toep_valid = true;
+    }
+    const size_t id_lds = ((size_t)nbat * 128 + (size_t)((n1 + 3) & ~3) * nbat) * 4;
+    OPSEC_HIP_TRY(ctx, hipFuncSetAttribute((const void*)idft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)id_lds));
+    __bf16* Th = ps->d_That;
+    __bf16* Tl = ps->d_That + (size_t)(n1 + 1) * plane_t;
+    EpiYhat toY{ps->d_Yhat, M, G, nbat};
+    const size_t np = (size_t)packed_size(M);
+    // events per band: start, That generated, stage-1 GEMMs, inverse DFT, stage 2
+    std::vector<hipEvent_t> ev(5 * (size_t)nb);
+    for (auto& e : ev) OPSEC_HIP_TRY(ctx, hipEventCreate(&e));
+    float tg = 0, t1 = 0, t2 = 0;
+    for (int z = 0; z < nb; z++) {
+        const double* Qf = ctx->d_Q + (size_t)funcs[z] * ctx->nkeys;
+        hipEventRecord(ev[5 * z], ctx->stream);
+        hipLaunchKernelGGL(qhat_kernel, dim3((unsigned)(((n1 + 1) * G + 255) / 256)), dim3(256), 0, ctx->stream, Qf,
+                           n1, G, ps->d_trig, ps->d_Qhat);
+        hipLaunchKernelGGL(that_kernel, dim3((unsigned)((Gp / 8 + 255) / 256), (unsigned)Gr, (unsigned)(n1 + 1)),
+                           dim3(256), 0, ctx->stream, ps->d_Qhat, G, sv.n3, Th, Tl, Gr, Gp);
+        hipEventRecord(ev[5 * z + 1], ctx->stream);
+        launch_g3(ctx, ASrcPlanes{Th, Tl, Gp}, (int)Gr, ps->d_Bhat, (int)Mp, Gp, (int)Gk, toY, 1, nbat, plane_t,
+                  plane_b, 1, (size_t)nbat * plane_b);
+        hipEventRecord(ev[5 * z + 2], ctx->stream);
+        hipLaunchKernelGGL(idft_kernel, dim3((unsigned)((G + 127) / 128), (unsigned)M), dim3(256),
+                           id_lds, ctx->stream, ps->d_Yhat, nbat, G, n1, ps->d_W,
+                           ps->d_Cg, ps->d_Cg + Mp * Pg, Pg);
+        hipEventRecord(ev[5 * z + 3], ctx->stream);
+        double* Pz = (double*)ps->d_P + (size_t)z * np;
+        EpiPackedParts toP{ps->d_Ppart, M, np};
+        launch_g3(ctx, ASrcPlanes{ps->d_Bg, ps->d_Bg + Mp * Pg, Pg}, (int)Mp, ps->d_Cg, (int)Mp, Pg, (int)Kg, toP, s2);
+        hipLaunchKernelGGL(sum_parts_kernel, blocks_for(np), dim3(256), 0, ctx->stream, ps->d_Ppart, s2, np, Pz);
+        hipEventRecord(ev[5 * z + 4], ctx->stream);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = set_error(ctx, OPSEC_EHIP, "projection launch: %s", hipGetErrorString(e));
+    hipError_t es = hipEventSynchronize(ev[5 * nb - 1]);
+    if (!rc && es != hipSuccess) rc = set_error(ctx, OPSEC_EHIP, "projection: %s", hipGetErrorString(es));
+    if (!rc)
+        for (int z = 0; z < nb; z++) {
+            float a, b, c, d;
+            hipEventElapsedTime(&a, ev[5 * z], ev[5 * z + 1]);
+            hipEventElapsedTime(&b, ev[5 * z + 1], ev[5 * z + 2]);
+            hipEventElapsedTime(&c, ev[5 * z + 2], ev[5 * z + 3]);
+            hipEventElapsedTime(&d, ev[5 * z + 3], ev[5 * z + 4]);
+            tg += a + c;   // That generation and inverse DFT
+            t1 += b;
+            t2 += d;
+        }
+    for (auto& x : ev) hipEventDestroy(x);
+    if (rc) return rc;
+    ps->last_ms_gen = tg;
+    ps->last_ms_gemm1 = t1;
+    ps->last_ms_gemm2 = t2;
+    ps->last_launches = nb;
+    ps->last_ms_stage1 = tg + t1;
+    ps->last_ms_stage2 = t2;
+    ps->nb = nb;
+    ps->prec = OPSEC_PREC_BF16X3;
+    return OPSEC_OK;
+}
+
 // Stage 1 + stage 2 for nb functions (device pair tables), results to d_P.
 static int run_project(opsec_ctx* ctx, int nb, const int* d_funcs, const int* funcs, int prec) {
+    if (prec == OPSEC_PREC_BF16X3 && g_projection == 1) return run_project_toeplitz(ctx, nb, funcs);
     if (prec == OPSEC_PREC_BF16X3) return run_project_bf16x3(ctx, nb, funcs);
     ProjState* ps = ctx->proj;
     const int N = ctx->ncells, M = ctx->nmodes;
@@ -1022,6 +1434,15 @@ void opsec_internal_free_proj(opsec_ctx* ctx) {
     dev_free((void**)&ctx->proj->d_Qhl);
     dev_free((void**)&ctx->proj->d_Ppart);
     dev_free((void**)&ctx->proj->d_cpk);
+    dev_free((void**)&ctx->proj->d_gmap);
+    dev_free((void**)&ctx->proj->d_Bg);
+    dev_free((void**)&ctx->proj->d_Bhat);
+    dev_free((void**)&ctx->proj->d_That);
+    dev_free((void**)&ctx->proj->d_Qhat);
+    dev_free((void**)&ctx->proj->d_Yhat);
+    dev_free((void**)&ctx->proj->d_Cg);
+    dev_free((void**)&ctx->proj->d_trig);
+    dev_free((void**)&ctx->proj->d_W);
     delete ctx->proj;
     ctx->proj = nullptr;
 }
@@ -1049,6 +1470,7 @@ int opsec_modes_upload(opsec_ctx* ctx, const double* B, int ncells, int nmodes) 
     OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->nmodes = nmodes;
     ps->bp_valid = false;
+    ps->toep_valid = false;
     return OPSEC_OK;
 }
 
@@ -1099,6 +1521,14 @@ int opsec_internal_set_nt_tile(int v) {
 int opsec_internal_set_stage2_split(int v) {
     if (v < 1 || v > 16) return OPSEC_EINVAL;
     g_s2_split = v;
+    return OPSEC_OK;
+}
+
+// Projection method of the split-BF16x3 path: 0 = dense, 1 = Toeplitz (box
+// grids: a DFT along x turns stage 1 into 2 (Nx + 1) slab-sized GEMMs).
+int opsec_internal_set_projection(int v) {
+    if (v < 0 || v > 1) return OPSEC_EINVAL;
+    g_projection = v;
     return OPSEC_OK;
 }
 

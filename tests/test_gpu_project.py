@@ -214,3 +214,60 @@ def test_project_bf16x3_gemm_variants(ctx, tile, gen):
     err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
     print("tile", tile, "gen", gen, "max rel err per function", err)
     assert err.max() < 5e-5
+
+
+@pytest.mark.parametrize("dims,drop,shuffle", [((8, 8, 8), 0.0, False), ((6, 5, 4), 0.2, True), ((3, 7, 9), 0.0, True)])
+def test_project_toeplitz_vs_fp64(ctx, dims, drop, shuffle):
+    """Toeplitz projection (DFT along x, 2 (Nx + 1) slab GEMMs in split-BF16x3)
+    against the dense FP64 projection of the same pair tables: a full cubic
+    grid, an irregular subset of a non-cubic grid in shuffled cell order, and a
+    grid whose slab (63 positions) is not a multiple of the 256-row tile."""
+    n1, n2, n3 = dims
+    h = 25.0
+    cells = synth.box_cells(max(dims), 0.0, h * max(dims))
+    keep = (cells["d1"] < n1) & (cells["d2"] < n2) & (cells["d3"] < n3)
+    cells = cells[keep].copy()
+    rng = np.random.default_rng(sum(dims))
+    if drop:
+        cells = cells[rng.random(len(cells)) >= drop].copy()
+    if shuffle:
+        cells = cells[rng.permutation(len(cells))].copy()
+    cells["a"] = np.arange(len(cells))
+    sv = lib.survey_box(n1, 0.0, h * n1, ny=n2, nz=n3)
+    sv.max2, sv.max3 = h * n2, h * n3
+    blo, bhi = synth.linear_bands(3, 0.02, 0.2)
+    ctx.model_real(GOLD["pk_k"], GOLD["pk_P"], blo, bhi, lib.OPSEC_XI_CLOSED)
+    ctx.cells_upload(cells, sv)
+    funcs = [0, 1, 2, 3]
+    ctx.pair_tables(funcs, ctx.quad(lib.OPSEC_QUAD_GAUSS, nodes=4, near=1))
+    N = len(cells)
+    m = min(N - 1, 90)
+    B = np.linalg.qr(rng.normal(size=(N, m)))[0].T.copy()
+    ctx.modes_upload(B)
+    ref = ctx.project(funcs, lib.OPSEC_PREC_FP64)
+    try:
+        assert ctx.lib.opsec_internal_set_projection(1) == 0
+        got = ctx.project(funcs, lib.OPSEC_PREC_BF16X3)
+        again = ctx.project(funcs, lib.OPSEC_PREC_BF16X3)
+    finally:
+        ctx.lib.opsec_internal_set_projection(0)
+    assert np.array_equal(got, again)
+    err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
+    print("Toeplitz BF16x3 projection", dims, "max rel err per function", err)
+    assert err.max() < 5e-5
+
+
+def test_project_toeplitz_rejects_shared_grid_position(ctx):
+    cells = synth.box_cells(4, 0.0, 100.0)
+    dup = np.concatenate([cells, cells[:1]])
+    dup["a"] = np.arange(len(dup))
+    ctx.model_real(GOLD["pk_k"], GOLD["pk_P"], [0.02], [0.1], lib.OPSEC_XI_CLOSED)
+    ctx.cells_upload(dup, lib.survey_box(4, 0.0, 100.0))
+    ctx.pair_tables([1], ctx.quad(lib.OPSEC_QUAD_GAUSS, nodes=4, near=1))
+    ctx.modes_upload(np.linalg.qr(np.random.default_rng(0).normal(size=(len(dup), 8)))[0].T.copy())
+    try:
+        assert ctx.lib.opsec_internal_set_projection(1) == 0
+        with pytest.raises(lib.OpsecError, match="share a grid position"):
+            ctx.project([1], lib.OPSEC_PREC_BF16X3)
+    finally:
+        ctx.lib.opsec_internal_set_projection(0)

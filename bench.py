@@ -34,10 +34,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-pair integrals/s + S→Fisher wall time, surrogate vs FP64 at ΔF/F≤1e-3"
 PEAK_TF = {"fp64": 78.6, "fp32": 157.3, "bf16": 2516.6}
-# HBM bytes per launch of the stage-1 GEMM (gemm3_kernel<ASrcPlanes,EpiPlanes> at 32^3 cells x 4000
-# modes): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB from two rocprofv3 --pmc passes, the factors being
-# what the same counters read for copy16_kernel on a 2 GiB copy (profiles/projection_c2_hbm_counters.txt)
-STAGE1_TRAFFIC_BYTES = {("C2", 2, 0): (18912985 * 2.0 + 885280 * 1.0) * 1024}
+# HBM bytes per launch of the dominant projection kernel at 32^3 cells x 4000 modes:
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB from two rocprofv3 --pmc passes, the factors being what
+# the same counters read for copy16_kernel on a 2 GiB copy (profiles/projection_c2_hbm_counters.txt).
+# Keys: (config, projection, kernel).
+TRAFFIC_BYTES = {("C2", "dense", "stage1"): (18912985 * 2.0 + 885280 * 1.0) * 1024}
 
 
 def log(*a):
@@ -228,7 +229,7 @@ def main():
                     help="KL modes (setup, untimed): the periodic box's analytic Fourier modes, or the klt solver")
     ap.add_argument("--nt-gen", type=int, default=0, choices=[0, 1],
                     help="stage 1 generates S_p from the pair table (1) or reads S_p planes (0)")
-    ap.add_argument("--projection", default="dense", choices=["dense", "toeplitz"],
+    ap.add_argument("--projection", default="toeplitz", choices=["dense", "toeplitz"],
                     help="split-BF16x3 projection: dense S_p (2 N^2 M stage 1) or Toeplitz (DFT along x, "
                          "2 (Nx + 1) slab GEMMs); the FP64 path is always dense")
     ap.add_argument("--s2-split", type=int, default=8, help="split-K factor of the split-BF16x3 stage 2")
@@ -259,7 +260,7 @@ def main():
     ctx.lib.opsec_internal_set_nt_order(args.nt_group, args.nt_remap)
     ctx.lib.opsec_internal_set_nt_gen(args.nt_gen)
     ctx.lib.opsec_internal_set_stage2_split(args.s2_split)
-    ctx.lib.opsec_internal_set_projection(1 if args.projection == "toeplitz" else 0)
+    ctx.set_projection(L.OPSEC_PROJ_TOEPLITZ if args.projection == "toeplitz" else L.OPSEC_PROJ_DENSE)
     ncu, arch = ctx.device_info()
     log(f"device {arch} with {ncu} CUs; workload {args.config}: {N} cells, {nb} bands, {M} modes")
     ctx.model_real(k, P, lo, hi, L.OPSEC_XI_CLOSED)
@@ -309,6 +310,13 @@ def main():
                                 args.steps, args.warmup, "surrogate")
         st = ctx.pair_stats()
         results["surrogate"] = dict(ms=ms, est=ests, kms=ks, stats=st)
+        if args.projection == "toeplitz":
+            # the same mixed path with the dense projection, for comparison
+            ctx.set_projection(L.OPSEC_PROJ_DENSE)
+            ms_d, est_d, ks_d = run_path(ctx, L, funcs, qs, L.OPSEC_PREC_BF16X3, L.OPSEC_PREC_FP32, lam, y, prior,
+                                         args.steps, 1, "surrogate, dense projection")
+            ctx.set_projection(L.OPSEC_PROJ_TOEPLITZ)
+            results["surrogate_dense"] = dict(ms=ms_d, est=est_d, kms=ks_d)
 
     pairs = float(N) * N * (nb + 1)
     main_path = "surrogate" if "surrogate" in results else "fp64"
@@ -337,6 +345,17 @@ def main():
                             "value": pairs / (results["fp64"]["ms"] / 1e3),
                             "projection_ms": [float(k64[0]), float(k64[1])],
                             "stage1_tflops": 2.0 * N * N * M * nb / (float(k64[0]) / 1e3) / 1e12}
+    if "surrogate_dense" in results:
+        kd = results["surrogate_dense"]["kms"]
+        out["dense_projection_path"] = {
+            "ms_per_step": results["surrogate_dense"]["ms"],
+            "stage1_gemm_ms_per_band": float(kd[3]) / float(kd[5]),
+            "stage1_useful_tflops": 2.0 * N * N * M / (float(kd[3]) / float(kd[5]) / 1e3) / 1e12,
+            "per_step_ms": {"s_planes": float(kd[2]), "stage1_gemm": float(kd[3]), "stage2_gemm": float(kd[4])}}
+        if "fp64" in results:
+            out["dense_projection_path"]["speedup_vs_fp64"] = results["fp64"]["ms"] / results["surrogate_dense"]["ms"]
+            out["dense_projection_path"].update(compare_estimates(results["fp64"]["est"],
+                                                                  results["surrogate_dense"]["est"]))
     if "fp64" in results and "surrogate" in results:
         out["speedup_vs_fp64"] = results["fp64"]["ms"] / results["surrogate"]["ms"]
         out.update(compare_estimates(results["fp64"]["est"], results["surrogate"]["est"]))
@@ -359,10 +378,11 @@ def main():
         achieved = (s1_flops if dom == "stage1" else s2_flops) / (per_launch / 1e3) / 1e12
         peak = PEAK_TF["bf16"] / 3
         out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                           "frac": achieved / peak, "traffic": None,
+                           "frac": achieved / peak,
+                           "traffic": TRAFFIC_BYTES.get((args.config, "toeplitz", dom)),
                            "kernel": ("gemm3_kernel<ASrcPlanes,EpiYhat> (Toeplitz stage 1, 2 (Nx + 1) slab GEMMs "
                                       "per launch)" if dom == "stage1" else
-                                      "gemm3_kernel<ASrcPlanes,EpiPacked> (stage 2, split-K, one launch per band)"),
+                                      "gemm3_kernel<ASrcPlanes,EpiPackedParts> (stage 2, split-K, one launch per band)"),
                            "launch_ms": per_launch,
                            "note": ("useful flops per launch: stage 1 2 (2 Nx) G^2 M, stage 2 N M^2; "
                                     "peak = dense bf16 2516.6 TF / 3 products"),
@@ -376,7 +396,8 @@ def main():
         peak = PEAK_TF["bf16"] / 3
         out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                            "frac": achieved / peak,
-                           "traffic": STAGE1_TRAFFIC_BYTES.get((args.config, args.nt_tile, args.nt_gen)),
+                           "traffic": (TRAFFIC_BYTES.get((args.config, "dense", "stage1"))
+                                       if (args.nt_tile, args.nt_gen) == (2, 0) else None),
                            "kernel": stage1_kernel_name(args.nt_tile, args.nt_gen),
                            "launch_ms": per_launch,
                            "note": "useful flops 2 N^2 M per launch; peak = dense bf16 2516.6 TF / 3 products",

@@ -101,6 +101,7 @@ struct opsec_ctx {
 
     // modes
     int nmodes = 0;
+    int projection = OPSEC_PROJ_DENSE;    // method of the split-BF16x3 projection
     double* d_B = nullptr;                // ncells x nmodes, column-major
     opsec::ProjState* proj = nullptr;
     opsec::MlpState* mlp = nullptr;

@@ -1213,10 +1213,6 @@ static int run_project_bf16x3(opsec_ctx* ctx, int nb, const int* funcs) {
     return OPSEC_OK;
 }
 
-// Projection method of the split-BF16x3 path: 0 = dense (S_p generated N x N),
-// 1 = Toeplitz (DFT along x, box grids).
-static int g_projection = 0;
-
 // cos / sin of 2 pi j / L, exact at the quarter points
 static void trig_table(int L, std::vector<double>& t) {
     t.assign(2 * (size_t)L, 0.0);
@@ -1357,7 +1353,7 @@ static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
 
 // Stage 1 + stage 2 for nb functions (device pair tables), results to d_P.
 static int run_project(opsec_ctx* ctx, int nb, const int* d_funcs, const int* funcs, int prec) {
-    if (prec == OPSEC_PREC_BF16X3 && g_projection == 1) return run_project_toeplitz(ctx, nb, funcs);
+    if (prec == OPSEC_PREC_BF16X3 && ctx->projection == OPSEC_PROJ_TOEPLITZ) return run_project_toeplitz(ctx, nb, funcs);
     if (prec == OPSEC_PREC_BF16X3) return run_project_bf16x3(ctx, nb, funcs);
     ProjState* ps = ctx->proj;
     const int N = ctx->ncells, M = ctx->nmodes;
@@ -1524,11 +1520,11 @@ int opsec_internal_set_stage2_split(int v) {
     return OPSEC_OK;
 }
 
-// Projection method of the split-BF16x3 path: 0 = dense, 1 = Toeplitz (box
-// grids: a DFT along x turns stage 1 into 2 (Nx + 1) slab-sized GEMMs).
-int opsec_internal_set_projection(int v) {
-    if (v < 0 || v > 1) return OPSEC_EINVAL;
-    g_projection = v;
+int opsec_set_projection(opsec_ctx* ctx, int method) {
+    if (!ctx) return OPSEC_EINVAL;
+    OPSEC_CHECK_ARG(ctx, method == OPSEC_PROJ_DENSE || method == OPSEC_PROJ_TOEPLITZ, "unknown projection method %d",
+                    method);
+    ctx->projection = method;
     return OPSEC_OK;
 }
 

@@ -27,7 +27,8 @@ static const char* usage =
     "  covfile=FILE   Write covariance matrices to FILE (required)\n"
     "  Nmodes=NUM     Number of KL modes (required)\n"
     "  model=NAME     Use specified model (required)\n"
-    "  hip.quad=gm|gauss|surrogate, hip.nodes=N, hip.xi=table|closed, hip.precision=fp64|fp32|bf16x3\n";
+    "  hip.quad=gm|gauss|surrogate, hip.nodes=N, hip.xi=table|closed, hip.precision=fp64|fp32|bf16x3,\n"
+    "  hip.projection=dense|toeplitz\n";
 
 int main(int argc, char** argv) {
     Config cfg;
@@ -75,6 +76,7 @@ int main(int argc, char** argv) {
     opsec_quad q = quad_from_config(cfg, 1e-5, 1e-10);
     const int prec = precision_from_config(cfg);
     opsec_ctx* ctx = open_context(cfg);
+    check(ctx, opsec_set_projection(ctx, projection_from_config(cfg)), "projection method");
     check(ctx, upload_model(ctx, model, xi_mode_from_config(cfg)), "model");
     check(ctx, opsec_cells_upload(ctx, cells.data(), Ncells, &sv), "cells");
     check(ctx, opsec_modes_upload(ctx, B.data(), Ncells, Nmodes), "modes");

@@ -263,6 +263,11 @@ int xi_mode_from_config(const Config& cfg) {
     return cfg.get_enum("hip.xi", {{"table", OPSEC_XI_TABLE}, {"closed", OPSEC_XI_CLOSED}}, OPSEC_XI_TABLE);
 }
 
+int projection_from_config(const Config& cfg) {
+    return cfg.get_enum("hip.projection", {{"dense", OPSEC_PROJ_DENSE}, {"toeplitz", OPSEC_PROJ_TOEPLITZ}},
+                        OPSEC_PROJ_DENSE);
+}
+
 int precision_from_config(const Config& cfg) {
     return cfg.get_enum("hip.precision", {{"fp64", OPSEC_PREC_FP64}, {"fp32", OPSEC_PREC_FP32},
                                           {"bf16x3", OPSEC_PREC_BF16X3}}, OPSEC_PREC_FP64);

@@ -56,11 +56,13 @@ bool load_survey(const Config& cfg, opsec_survey* sv);
 //   hip.near      = near-pair threshold in cells
 //   hip.xi        = table | closed           (default table: the reference's tables)
 //   hip.precision = fp64 | fp32 | bf16x3     (default fp64)
+//   hip.projection = dense | toeplitz        (split-BF16x3 projection method; default dense)
 //   hip.device    = device index             (default 0)
 //   hip.tol, hip.block, hip.maxdim           (klt eigen-solver)
 opsec_quad quad_from_config(const Config& cfg, double epsrel, double epsabs);
 int xi_mode_from_config(const Config& cfg);
 int precision_from_config(const Config& cfg);
+int projection_from_config(const Config& cfg);
 
 opsec_ctx* open_context(const Config& cfg);
 void check(opsec_ctx* ctx, int rc, const char* what);   // aborts with the library message

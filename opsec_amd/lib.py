@@ -26,6 +26,8 @@ OPSEC_QUAD_SURROGATE = 2
 OPSEC_PREC_FP64 = 0
 OPSEC_PREC_FP32 = 1
 OPSEC_PREC_BF16X3 = 2
+OPSEC_PROJ_DENSE = 0
+OPSEC_PROJ_TOEPLITZ = 1
 OPSEC_MIX_INVERSE = 0
 OPSEC_MIX_INVERSE_SQRT = 1
 OPSEC_MIX_IDENTITY = 2
@@ -104,6 +106,7 @@ _SIGS = {
     "opsec_lanczos_dense": ([_P, _I, _P, _I, _D, _I, _I, _P, _P, C.POINTER(_I)], _I),
     "opsec_modes_upload": ([_P, _P, _I, _I], _I),
     "opsec_project": ([_P, _I, _P, _I, _P], _I),
+    "opsec_set_projection": ([_P, _I], _I),
     "opsec_project_dense": ([_P, _I, _P, _I, _P, _I, _P], _I),
     "opsec_count_galaxies": ([_P, _P, C.c_int64, _I, _P], _I),
     "opsec_pixel_values": ([_P, _P, _P, _P], _I),
@@ -351,6 +354,10 @@ class Context:
         y = np.zeros(self._nmodes)
         self.check(self.lib.opsec_pixel_values(self.h, _ptr(c), _ptr(x), _ptr(y)), "pixel_values")
         return x, y
+
+    def set_projection(self, method):
+        """OPSEC_PROJ_DENSE or OPSEC_PROJ_TOEPLITZ (split-BF16x3 projections)."""
+        self.check(self.lib.opsec_set_projection(self.h, method), "set_projection")
 
     def project(self, funcs, prec=OPSEC_PREC_FP64, keep_on_device=False):
         fs = np.ascontiguousarray(funcs, dtype=np.int32)

@@ -246,11 +246,11 @@ def test_project_toeplitz_vs_fp64(ctx, dims, drop, shuffle):
     ctx.modes_upload(B)
     ref = ctx.project(funcs, lib.OPSEC_PREC_FP64)
     try:
-        assert ctx.lib.opsec_internal_set_projection(1) == 0
+        ctx.set_projection(lib.OPSEC_PROJ_TOEPLITZ)
         got = ctx.project(funcs, lib.OPSEC_PREC_BF16X3)
         again = ctx.project(funcs, lib.OPSEC_PREC_BF16X3)
     finally:
-        ctx.lib.opsec_internal_set_projection(0)
+        ctx.set_projection(lib.OPSEC_PROJ_DENSE)
     assert np.array_equal(got, again)
     err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
     print("Toeplitz BF16x3 projection", dims, "max rel err per function", err)
@@ -266,8 +266,8 @@ def test_project_toeplitz_rejects_shared_grid_position(ctx):
     ctx.pair_tables([1], ctx.quad(lib.OPSEC_QUAD_GAUSS, nodes=4, near=1))
     ctx.modes_upload(np.linalg.qr(np.random.default_rng(0).normal(size=(len(dup), 8)))[0].T.copy())
     try:
-        assert ctx.lib.opsec_internal_set_projection(1) == 0
+        ctx.set_projection(lib.OPSEC_PROJ_TOEPLITZ)
         with pytest.raises(lib.OpsecError, match="share a grid position"):
             ctx.project([1], lib.OPSEC_PREC_BF16X3)
     finally:
-        ctx.lib.opsec_internal_set_projection(0)
+        ctx.set_projection(lib.OPSEC_PROJ_DENSE)

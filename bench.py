@@ -38,7 +38,9 @@ PEAK_TF = {"fp64": 78.6, "fp32": 157.3, "bf16": 2516.6}
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB from two rocprofv3 --pmc passes, the factors being what
 # the same counters read for copy16_kernel on a 2 GiB copy (profiles/projection_c2_hbm_counters.txt).
 # Keys: (config, projection, kernel).
-TRAFFIC_BYTES = {("C2", "dense", "stage1"): (18912985 * 2.0 + 885280 * 1.0) * 1024}
+TRAFFIC_BYTES = {("C2", "dense", "stage1"): (18876549 * 2.0 + 885395 * 1.0) * 1024,
+                 ("C2", "toeplitz", "stage1"): (2308298 * 2.0 + 1156031 * 1.0) * 1024,
+                 ("C2", "toeplitz", "stage2"): (1092405 * 2.0 + 283867 * 1.0) * 1024}
 
 
 def log(*a):
@@ -386,7 +388,8 @@ def main():
                            "launch_ms": per_launch,
                            "note": ("useful flops per launch: stage 1 2 (2 Nx) G^2 M, stage 2 N M^2; "
                                     "peak = dense bf16 2516.6 TF / 3 products"),
-                           "per_step_ms": {"that_and_inverse_dft": kg_ms, "stage1_gemm": k1_ms, "stage2_gemm": k2_ms},
+                           "per_step_ms": {"that_inverse_dft_and_split_k_sum": kg_ms, "stage1_gemm": k1_ms,
+                                           "stage2_gemm": k2_ms},
                            "stage1_tflops": s1_flops / (k1_ms / launches / 1e3) / 1e12,
                            "stage2_tflops": s2_flops / (k2_ms / launches / 1e3) / 1e12}
     elif main_path == "surrogate":

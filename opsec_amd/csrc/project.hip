@@ -1298,43 +1298,46 @@ static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
     __bf16* Tl = ps->d_That + (size_t)(n1 + 1) * plane_t;
     EpiYhat toY{ps->d_Yhat, M, G, nbat};
     const size_t np = (size_t)packed_size(M);
-    // events per band: start, That generated, stage-1 GEMMs, inverse DFT, stage 2
-    std::vector<hipEvent_t> ev(5 * (size_t)nb);
+    // events per band: start, That generated, stage-1 GEMMs, inverse DFT, stage-2
+    // GEMM, split-K reduction
+    std::vector<hipEvent_t> ev(6 * (size_t)nb);
     for (auto& e : ev) OPSEC_HIP_TRY(ctx, hipEventCreate(&e));
     float tg = 0, t1 = 0, t2 = 0;
     for (int z = 0; z < nb; z++) {
         const double* Qf = ctx->d_Q + (size_t)funcs[z] * ctx->nkeys;
-        hipEventRecord(ev[5 * z], ctx->stream);
+        hipEventRecord(ev[6 * z], ctx->stream);
         hipLaunchKernelGGL(qhat_kernel, dim3((unsigned)(((n1 + 1) * G + 255) / 256)), dim3(256), 0, ctx->stream, Qf,
                            n1, G, ps->d_trig, ps->d_Qhat);
         hipLaunchKernelGGL(that_kernel, dim3((unsigned)((Gp / 8 + 255) / 256), (unsigned)Gr, (unsigned)(n1 + 1)),
                            dim3(256), 0, ctx->stream, ps->d_Qhat, G, sv.n3, Th, Tl, Gr, Gp);
-        hipEventRecord(ev[5 * z + 1], ctx->stream);
+        hipEventRecord(ev[6 * z + 1], ctx->stream);
         launch_g3(ctx, ASrcPlanes{Th, Tl, Gp}, (int)Gr, ps->d_Bhat, (int)Mp, Gp, (int)Gk, toY, 1, nbat, plane_t,
                   plane_b, 1, (size_t)nbat * plane_b);
-        hipEventRecord(ev[5 * z + 2], ctx->stream);
+        hipEventRecord(ev[6 * z + 2], ctx->stream);
         hipLaunchKernelGGL(idft_kernel, dim3((unsigned)((G + 127) / 128), (unsigned)M), dim3(256),
                            id_lds, ctx->stream, ps->d_Yhat, nbat, G, n1, ps->d_W,
                            ps->d_Cg, ps->d_Cg + Mp * Pg, Pg);
-        hipEventRecord(ev[5 * z + 3], ctx->stream);
+        hipEventRecord(ev[6 * z + 3], ctx->stream);
         double* Pz = (double*)ps->d_P + (size_t)z * np;
         EpiPackedParts toP{ps->d_Ppart, M, np};
         launch_g3(ctx, ASrcPlanes{ps->d_Bg, ps->d_Bg + Mp * Pg, Pg}, (int)Mp, ps->d_Cg, (int)Mp, Pg, (int)Kg, toP, s2);
+        hipEventRecord(ev[6 * z + 4], ctx->stream);
         hipLaunchKernelGGL(sum_parts_kernel, blocks_for(np), dim3(256), 0, ctx->stream, ps->d_Ppart, s2, np, Pz);
-        hipEventRecord(ev[5 * z + 4], ctx->stream);
+        hipEventRecord(ev[6 * z + 5], ctx->stream);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = set_error(ctx, OPSEC_EHIP, "projection launch: %s", hipGetErrorString(e));
-    hipError_t es = hipEventSynchronize(ev[5 * nb - 1]);
+    hipError_t es = hipEventSynchronize(ev[6 * nb - 1]);
     if (!rc && es != hipSuccess) rc = set_error(ctx, OPSEC_EHIP, "projection: %s", hipGetErrorString(es));
     if (!rc)
         for (int z = 0; z < nb; z++) {
-            float a, b, c, d;
-            hipEventElapsedTime(&a, ev[5 * z], ev[5 * z + 1]);
-            hipEventElapsedTime(&b, ev[5 * z + 1], ev[5 * z + 2]);
-            hipEventElapsedTime(&c, ev[5 * z + 2], ev[5 * z + 3]);
-            hipEventElapsedTime(&d, ev[5 * z + 3], ev[5 * z + 4]);
-            tg += a + c;   // That generation and inverse DFT
+            float a, b, c, d, e2;
+            hipEventElapsedTime(&a, ev[6 * z], ev[6 * z + 1]);
+            hipEventElapsedTime(&b, ev[6 * z + 1], ev[6 * z + 2]);
+            hipEventElapsedTime(&c, ev[6 * z + 2], ev[6 * z + 3]);
+            hipEventElapsedTime(&d, ev[6 * z + 3], ev[6 * z + 4]);
+            hipEventElapsedTime(&e2, ev[6 * z + 4], ev[6 * z + 5]);
+            tg += a + c + e2;   // That generation, inverse DFT, split-K reduction
             t1 += b;
             t2 += d;
         }

@@ -39,8 +39,7 @@ PEAK_TF = {"fp64": 78.6, "fp32": 157.3, "bf16": 2516.6}
 # the same counters read for copy16_kernel on a 2 GiB copy (profiles/projection_c2_hbm_counters.txt).
 # Keys: (config, projection, kernel).
 TRAFFIC_BYTES = {("C2", "dense", "stage1"): (18876549 * 2.0 + 885395 * 1.0) * 1024,
-                 ("C2", "toeplitz", "stage1"): (2308298 * 2.0 + 1156031 * 1.0) * 1024,
-                 ("C2", "toeplitz", "stage2"): (1092405 * 2.0 + 283867 * 1.0) * 1024}
+                 ("C2", "toeplitz", "stage1"): (2308298 * 2.0 + 1156031 * 1.0) * 1024}
 
 
 def log(*a):
@@ -371,27 +370,32 @@ def main():
     # (useful flops 2 N^2 M per band), average launch time from HIP events on
     # the context's stream
     if main_path == "surrogate" and args.projection == "toeplitz":
-        # stage 1 = 2 Nx real GEMMs of G x G by G x M per band (G = Ny Nz), stage 2 = N M^2
+        # stage 1 = 2 Nx real GEMMs of G x G by G x M per band (G = Ny Nz), one launch per band;
+        # stage 2 = N M^2 per band, one launch per group of bands
+        import ctypes as C
         kg_ms, k1_ms, k2_ms, launches = [float(x) for x in R["kms"][2:6]]
+        l1, l2 = C.c_int(), C.c_int()
+        ctx.lib.opsec_internal_project_launches(ctx.h, C.byref(l1), C.byref(l2))
         G = n * n
-        s1_flops, s2_flops = 2.0 * (2 * n) * G * G * M, float(N) * M * M
+        s1_flops, s2_flops = 2.0 * (2 * n) * G * G * M * nb, float(N) * M * M * nb   # per step
         dom = "stage1" if k1_ms >= k2_ms else "stage2"
-        per_launch = (k1_ms if dom == "stage1" else k2_ms) / launches
-        achieved = (s1_flops if dom == "stage1" else s2_flops) / (per_launch / 1e3) / 1e12
+        per_launch = k1_ms / l1.value if dom == "stage1" else k2_ms / l2.value
+        achieved = (s1_flops / k1_ms if dom == "stage1" else s2_flops / k2_ms) * 1e3 / 1e12
         peak = PEAK_TF["bf16"] / 3
         out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                            "frac": achieved / peak,
                            "traffic": TRAFFIC_BYTES.get((args.config, "toeplitz", dom)),
                            "kernel": ("gemm3_kernel<ASrcPlanes,EpiYhat> (Toeplitz stage 1, 2 (Nx + 1) slab GEMMs "
-                                      "per launch)" if dom == "stage1" else
-                                      "gemm3_kernel<ASrcPlanes,EpiPackedParts> (stage 2, split-K, one launch per band)"),
+                                      "per launch, one launch per band)" if dom == "stage1" else
+                                      f"gemm3_kernel<ASrcPlanes,EpiPackedParts> (stage 2, split-K, "
+                                      f"{nb // l2.value} bands per launch)"),
                            "launch_ms": per_launch,
-                           "note": ("useful flops per launch: stage 1 2 (2 Nx) G^2 M, stage 2 N M^2; "
+                           "note": ("useful flops per band: stage 1 2 (2 Nx) G^2 M, stage 2 N M^2; "
                                     "peak = dense bf16 2516.6 TF / 3 products"),
                            "per_step_ms": {"that_inverse_dft_and_split_k_sum": kg_ms, "stage1_gemm": k1_ms,
                                            "stage2_gemm": k2_ms},
-                           "stage1_tflops": s1_flops / (k1_ms / launches / 1e3) / 1e12,
-                           "stage2_tflops": s2_flops / (k2_ms / launches / 1e3) / 1e12}
+                           "stage1_tflops": s1_flops / k1_ms * 1e3 / 1e12,
+                           "stage2_tflops": s2_flops / k2_ms * 1e3 / 1e12}
     elif main_path == "surrogate":
         kg_ms, k1_ms, k2_ms, launches = [float(x) for x in R["kms"][2:6]]
         per_launch = k1_ms / launches

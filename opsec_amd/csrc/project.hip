@@ -57,7 +57,8 @@ struct ProjState {
     size_t cap_cpk = 0;
     // per-kernel sums over the bands of the last split-BF16x3 projection (ms)
     float last_ms_gen = 0, last_ms_gemm1 = 0, last_ms_gemm2 = 0;
-    int last_launches = 0;
+    int last_launches = 0;       // stage-1 launches (one per band)
+    int last_launches2 = 0;      // stage-2 launches (one per band, or per group of bands)
     // Toeplitz projection of box surveys (grid order, DFT along x)
     bool toep_valid = false;     // grid planes of B' and its DFT match the uploaded modes
     int* d_gmap = nullptr;       // grid index -> cell number, -1 where the grid has no cell
@@ -72,7 +73,7 @@ struct ProjState {
     size_t cap_Qhat = 0;
     float* d_Yhat = nullptr;     // stage-1 products in frequency: M x nbat x G
     size_t cap_Yhat = 0;
-    __bf16* d_Cg = nullptr;      // Y = S_p B' in grid order: Mp x Pg, hi then lo
+    __bf16* d_Cg = nullptr;      // Y = S_p B' in grid order, one per band of a group: Mp x Pg, hi then lo
     size_t cap_Cg = 0;
     double* d_trig = nullptr;    // cos(2 pi j / L), then sin(2 pi j / L), j < L
     size_t cap_trig = 0;
@@ -397,12 +398,17 @@ struct EpiPackedParts {
     float* p;
     int M;
     size_t part_stride;
+    size_t batch_stride = 0;   // batch b (one band) writes its parts at p + b * batch_stride
     __device__ __forceinline__ EpiPackedParts part(int s) const {
         EpiPackedParts e = *this;
         e.p += (size_t)s * part_stride;
         return e;
     }
-    __device__ __forceinline__ EpiPackedParts batch(int) const { return *this; }
+    __device__ __forceinline__ EpiPackedParts batch(int b) const {
+        EpiPackedParts e = *this;
+        e.p += (size_t)b * batch_stride;
+        return e;
+    }
     __device__ __forceinline__ void put4(int m, int n, const f4_t& v) const {
         if (n >= M) return;
 #pragma unroll
@@ -1206,12 +1212,15 @@ static int run_project_bf16x3(opsec_ctx* ctx, int nb, const int* funcs) {
     ps->last_ms_gemm1 = t1;
     ps->last_ms_gemm2 = t2;
     ps->last_launches = nb;
+    ps->last_launches2 = nb;
     ps->last_ms_stage1 = tg + t1;
     ps->last_ms_stage2 = t2;
     ps->nb = nb;
     ps->prec = OPSEC_PREC_BF16X3;
     return OPSEC_OK;
 }
+
+constexpr int TOEP_GROUP = 4;   // bands per stage-2 launch of the Toeplitz path
 
 // cos / sin of 2 pi j / L, exact at the quarter points
 static void trig_table(int L, std::vector<double>& t) {
@@ -1248,7 +1257,12 @@ static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
     if ((rc = dev_realloc(ctx, (void**)&ps->d_Yhat, &ps->cap_Yhat, (size_t)M * nbat * G * 4))) return rc;
     if ((rc = dev_realloc(ctx, &ps->d_P, &ps->cap_P, (size_t)nb * packed_size(M) * 8))) return rc;
     const int s2 = std::max(1, std::min(g_s2_split, (int)(Kg / PK)));
-    if ((rc = dev_realloc(ctx, (void**)&ps->d_Ppart, &ps->cap_Ppart, (size_t)s2 * packed_size(M) * 4))) return rc;
+    // stage 2 runs once per group of up to TOEP_GROUP bands: the 136 upper tiles x
+    // split-K parts of one band fill 4.25 rounds of the CUs, four bands 17
+    const int grp = std::min(nb, TOEP_GROUP);
+    if ((rc = dev_realloc(ctx, (void**)&ps->d_Ppart, &ps->cap_Ppart, (size_t)grp * s2 * packed_size(M) * 4)))
+        return rc;
+    const size_t cg_pair = 2 * Mp * Pg;   // one band's Y plane pair (bf16 elements)
     if (!ps->toep_valid) {
         // grid map (cells must occupy distinct grid positions)
         std::vector<int> gmap((size_t)Ng, -1);
@@ -1274,12 +1288,12 @@ static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
                 W[(size_t)b * n1p + x] = (float)((b & 1) ? -ck * trig[L + j] / L : ck * trig[j] / L);
             }
         if ((rc = dev_realloc(ctx, (void**)&ps->d_W, &ps->cap_W, W.size() * 4))) return rc;
-        if ((rc = dev_realloc(ctx, (void**)&ps->d_Cg, &ps->cap_Cg, 2 * Mp * Pg * sizeof(__bf16)))) return rc;
+        if ((rc = dev_realloc(ctx, (void**)&ps->d_Cg, &ps->cap_Cg, TOEP_GROUP * cg_pair * sizeof(__bf16)))) return rc;
         OPSEC_HIP_TRY(ctx, stream_copy(ctx, ps->d_gmap, gmap.data(), gmap.size() * 4, hipMemcpyHostToDevice));
         OPSEC_HIP_TRY(ctx, stream_copy(ctx, ps->d_trig, trig.data(), trig.size() * 8, hipMemcpyHostToDevice));
         OPSEC_HIP_TRY(ctx, stream_copy(ctx, ps->d_W, W.data(), W.size() * 4, hipMemcpyHostToDevice));
         // the inverse DFT writes rows m < M and columns g < Ng of Cg; the rest stays zero
-        OPSEC_HIP_TRY(ctx, hipMemsetAsync(ps->d_Cg, 0, 2 * Mp * Pg * sizeof(__bf16), ctx->stream));
+        OPSEC_HIP_TRY(ctx, hipMemsetAsync(ps->d_Cg, 0, TOEP_GROUP * cg_pair * sizeof(__bf16), ctx->stream));
         hipLaunchKernelGGL(grid_planes_kernel, dim3((unsigned)((Pg / 8 + 255) / 256), (unsigned)Mp), dim3(256), 0,
                            ctx->stream, ps->d_Bw, ps->ldw, ps->d_gmap, Ng, M, ps->d_Bg, ps->d_Bg + Mp * Pg, Pg);
         const size_t bh_lds = (size_t)n1 * 256 * 8;
@@ -1298,55 +1312,72 @@ static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
     __bf16* Tl = ps->d_That + (size_t)(n1 + 1) * plane_t;
     EpiYhat toY{ps->d_Yhat, M, G, nbat};
     const size_t np = (size_t)packed_size(M);
-    // events per band: start, That generated, stage-1 GEMMs, inverse DFT, stage-2
-    // GEMM, split-K reduction
-    std::vector<hipEvent_t> ev(6 * (size_t)nb);
+    // events per band: start, That generated, stage-1 GEMMs, inverse DFT; per
+    // group: stage-2 GEMM, split-K reductions
+    const int ngroups = (nb + grp - 1) / grp;
+    std::vector<hipEvent_t> ev(4 * (size_t)nb + 2 * (size_t)ngroups);
     for (auto& e : ev) OPSEC_HIP_TRY(ctx, hipEventCreate(&e));
+    hipEvent_t* evb = ev.data();
+    hipEvent_t* evg = ev.data() + 4 * (size_t)nb;
     float tg = 0, t1 = 0, t2 = 0;
-    for (int z = 0; z < nb; z++) {
-        const double* Qf = ctx->d_Q + (size_t)funcs[z] * ctx->nkeys;
-        hipEventRecord(ev[6 * z], ctx->stream);
-        hipLaunchKernelGGL(qhat_kernel, dim3((unsigned)(((n1 + 1) * G + 255) / 256)), dim3(256), 0, ctx->stream, Qf,
-                           n1, G, ps->d_trig, ps->d_Qhat);
-        hipLaunchKernelGGL(that_kernel, dim3((unsigned)((Gp / 8 + 255) / 256), (unsigned)Gr, (unsigned)(n1 + 1)),
-                           dim3(256), 0, ctx->stream, ps->d_Qhat, G, sv.n3, Th, Tl, Gr, Gp);
-        hipEventRecord(ev[6 * z + 1], ctx->stream);
-        launch_g3(ctx, ASrcPlanes{Th, Tl, Gp}, (int)Gr, ps->d_Bhat, (int)Mp, Gp, (int)Gk, toY, 1, nbat, plane_t,
-                  plane_b, 1, (size_t)nbat * plane_b);
-        hipEventRecord(ev[6 * z + 2], ctx->stream);
-        hipLaunchKernelGGL(idft_kernel, dim3((unsigned)((G + 127) / 128), (unsigned)M), dim3(256),
-                           id_lds, ctx->stream, ps->d_Yhat, nbat, G, n1, ps->d_W,
-                           ps->d_Cg, ps->d_Cg + Mp * Pg, Pg);
-        hipEventRecord(ev[6 * z + 3], ctx->stream);
-        double* Pz = (double*)ps->d_P + (size_t)z * np;
-        EpiPackedParts toP{ps->d_Ppart, M, np};
-        launch_g3(ctx, ASrcPlanes{ps->d_Bg, ps->d_Bg + Mp * Pg, Pg}, (int)Mp, ps->d_Cg, (int)Mp, Pg, (int)Kg, toP, s2);
-        hipEventRecord(ev[6 * z + 4], ctx->stream);
-        hipLaunchKernelGGL(sum_parts_kernel, blocks_for(np), dim3(256), 0, ctx->stream, ps->d_Ppart, s2, np, Pz);
-        hipEventRecord(ev[6 * z + 5], ctx->stream);
+    for (int z0 = 0, gi = 0; z0 < nb; z0 += grp, gi++) {
+        const int g = std::min(grp, nb - z0);
+        for (int q = 0; q < g; q++) {
+            const int z = z0 + q;
+            __bf16* Cz = ps->d_Cg + (size_t)q * cg_pair;
+            const double* Qf = ctx->d_Q + (size_t)funcs[z] * ctx->nkeys;
+            hipEventRecord(evb[4 * z], ctx->stream);
+            hipLaunchKernelGGL(qhat_kernel, dim3((unsigned)(((n1 + 1) * G + 255) / 256)), dim3(256), 0, ctx->stream,
+                               Qf, n1, G, ps->d_trig, ps->d_Qhat);
+            hipLaunchKernelGGL(that_kernel, dim3((unsigned)((Gp / 8 + 255) / 256), (unsigned)Gr, (unsigned)(n1 + 1)),
+                               dim3(256), 0, ctx->stream, ps->d_Qhat, G, sv.n3, Th, Tl, Gr, Gp);
+            hipEventRecord(evb[4 * z + 1], ctx->stream);
+            launch_g3(ctx, ASrcPlanes{Th, Tl, Gp}, (int)Gr, ps->d_Bhat, (int)Mp, Gp, (int)Gk, toY, 1, nbat, plane_t,
+                      plane_b, 1, (size_t)nbat * plane_b);
+            hipEventRecord(evb[4 * z + 2], ctx->stream);
+            hipLaunchKernelGGL(idft_kernel, dim3((unsigned)((G + 127) / 128), (unsigned)M), dim3(256), id_lds,
+                               ctx->stream, ps->d_Yhat, nbat, G, n1, ps->d_W, Cz, Cz + Mp * Pg, Pg);
+            hipEventRecord(evb[4 * z + 3], ctx->stream);
+        }
+        // stage 2 of the group's bands in one launch: batch q = band z0 + q
+        EpiPackedParts toP{ps->d_Ppart, M, np, (size_t)s2 * np};
+        launch_g3(ctx, ASrcPlanes{ps->d_Bg, ps->d_Bg + Mp * Pg, Pg}, (int)Mp, ps->d_Cg, (int)Mp, Pg, (int)Kg, toP, s2, g,
+                  0, cg_pair, 0, Mp * Pg);
+        hipEventRecord(evg[2 * gi], ctx->stream);
+        for (int q = 0; q < g; q++)
+            hipLaunchKernelGGL(sum_parts_kernel, blocks_for(np), dim3(256), 0, ctx->stream,
+                               ps->d_Ppart + (size_t)q * s2 * np, s2, np, (double*)ps->d_P + (size_t)(z0 + q) * np);
+        hipEventRecord(evg[2 * gi + 1], ctx->stream);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = set_error(ctx, OPSEC_EHIP, "projection launch: %s", hipGetErrorString(e));
-    hipError_t es = hipEventSynchronize(ev[6 * nb - 1]);
+    hipError_t es = hipEventSynchronize(evg[2 * ngroups - 1]);
     if (!rc && es != hipSuccess) rc = set_error(ctx, OPSEC_EHIP, "projection: %s", hipGetErrorString(es));
-    if (!rc)
+    if (!rc) {
         for (int z = 0; z < nb; z++) {
-            float a, b, c, d, e2;
-            hipEventElapsedTime(&a, ev[6 * z], ev[6 * z + 1]);
-            hipEventElapsedTime(&b, ev[6 * z + 1], ev[6 * z + 2]);
-            hipEventElapsedTime(&c, ev[6 * z + 2], ev[6 * z + 3]);
-            hipEventElapsedTime(&d, ev[6 * z + 3], ev[6 * z + 4]);
-            hipEventElapsedTime(&e2, ev[6 * z + 4], ev[6 * z + 5]);
-            tg += a + c + e2;   // That generation, inverse DFT, split-K reduction
+            float a, b, c;
+            hipEventElapsedTime(&a, evb[4 * z], evb[4 * z + 1]);
+            hipEventElapsedTime(&b, evb[4 * z + 1], evb[4 * z + 2]);
+            hipEventElapsedTime(&c, evb[4 * z + 2], evb[4 * z + 3]);
+            tg += a + c;   // That generation, inverse DFT
             t1 += b;
-            t2 += d;
         }
+        for (int gi = 0; gi < ngroups; gi++) {
+            const int zl = std::min(nb, (gi + 1) * grp) - 1;   // last band of the group
+            float d, e2;
+            hipEventElapsedTime(&d, evb[4 * zl + 3], evg[2 * gi]);
+            hipEventElapsedTime(&e2, evg[2 * gi], evg[2 * gi + 1]);
+            t2 += d;
+            tg += e2;      // split-K reductions
+        }
+    }
     for (auto& x : ev) hipEventDestroy(x);
     if (rc) return rc;
     ps->last_ms_gen = tg;
     ps->last_ms_gemm1 = t1;
     ps->last_ms_gemm2 = t2;
     ps->last_launches = nb;
+    ps->last_launches2 = ngroups;
     ps->last_ms_stage1 = tg + t1;
     ps->last_ms_stage2 = t2;
     ps->nb = nb;
@@ -1556,6 +1587,14 @@ int opsec_internal_project_kernel_ms(opsec_ctx* ctx, float* gen, float* gemm1, f
     *gemm1 = ctx->proj->last_ms_gemm1;
     *gemm2 = ctx->proj->last_ms_gemm2;
     *launches = ctx->proj->last_launches;
+    return OPSEC_OK;
+}
+
+// Launch counts of the last split-BF16x3 projection: stage 1, stage 2.
+int opsec_internal_project_launches(opsec_ctx* ctx, int* stage1, int* stage2) {
+    if (!ctx || !ctx->proj) return OPSEC_EINVAL;
+    *stage1 = ctx->proj->last_launches;
+    *stage2 = ctx->proj->last_launches2;
     return OPSEC_OK;
 }
 

@@ -717,18 +717,18 @@ struct ASrcTable {
 };
 
 // Batched form: workgroup id = (batch * nmb * nnb + tile) * ksplit + split;
-// batch b multiplies A at offset (b >> a_shift) * a_bstride by B at offset
-// b * b_bstride into the epilogue's batch b.
+// batch b multiplies A at offset ((b + a_add) >> a_shift) * a_bstride by B at
+// offset b * b_bstride into the epilogue's batch b.
 template <typename ASrc, typename Epi>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void gemm3_kernel(
     ASrc A0, const __bf16* __restrict__ Bh0, const __bf16* __restrict__ Bl0, size_t ldb, int K, int nmb, int nnb, int G,
-    int remap, int ksplit, Epi E0, size_t a_bstride, size_t b_bstride, int a_shift) {
+    int remap, int ksplit, Epi E0, size_t a_bstride, size_t b_bstride, int a_add, int a_shift) {
     extern __shared__ __attribute__((aligned(16))) __bf16 g3_lds[];   // 2 stages x (Ah, Al, Bh, Bl)
     // split-K: workgroup b takes part b % ksplit of the K steps of tile b / ksplit
     const int split = blockIdx.x % ksplit;
     const int tiles = nmb * nnb, id = blockIdx.x / ksplit, bat = id / tiles;
     const Epi E = E0.part(split).batch(bat);
-    const ASrc A = A0.at((size_t)(bat >> a_shift) * a_bstride);
+    const ASrc A = A0.at((size_t)((bat + a_add) >> a_shift) * a_bstride);
     const __bf16* __restrict__ Bh = Bh0 + (size_t)bat * b_bstride;
     const __bf16* __restrict__ Bl = Bl0 + (size_t)bat * b_bstride;
     int mb, nbk;
@@ -850,10 +850,10 @@ __global__ void scale_rows_kernel(const double* B, const double* nbar, int N, in
 //     Yhat_k = That_k Bhat_k,  That_k = sum_d w_d cos(2 pi d k / L) T_d  (real),
 // for k = 0..n1 (the others follow by Hermitian symmetry), and Y is the inverse
 // DFT of Yhat at x < n1.  That_k is a 2-level Toeplitz matrix generated from
-// Qhat_k(|dy|, |dz|).  Stage 1 becomes nbat = 2 (n1 + 1) GEMMs of G x G by G x M
-// (real part R_k and imaginary part I_k of Bhat_k; I_0 = I_n1 = 0 are kept so
-// that batch b uses That_(b/2)), about n1 / 2 times fewer flops than N x N by
-// N x M.  Grid positions without a cell have B' = 0, so any cell subset of the
+// Qhat_k(|dy|, |dz|).  Stage 1 becomes nbat = 2 n1 GEMMs of G x G by G x M:
+// batch 0 is R_0, batch 2k - 1 the real part R_k (k = 1..n1) and batch 2k the
+// imaginary part I_k (k = 1..n1-1) of Bhat_k (I_0 = I_n1 = 0), so batch b uses
+// That_((b+1)/2); about n1 / 2 times fewer flops than N x N by N x M.  Grid positions without a cell have B' = 0, so any cell subset of the
 // grid is exact.
 
 // B' in grid order, split: planes [m][g] of Mp x Pg, zero where no cell / padding.
@@ -880,9 +880,10 @@ __global__ void grid_planes_kernel(const double* __restrict__ Bw, size_t ldw, co
     *(bf8_t*)(lo + (size_t)m * Pg + g0) = l;
 }
 
-// Bhat planes: batch b = 2k holds R_k = sum_x cos(2 pi x k / L) B'_x, b = 2k + 1
-// holds I_k = -sum_x sin(2 pi x k / L) B'_x (FP64 sums, then split).  Block: one
-// mode, 256 slab positions; the n1 grid values of each position sit in LDS.
+// Bhat planes: batch b holds R_k = sum_x cos(2 pi x k / L) B'_x (b = 0 or odd) or
+// I_k = -sum_x sin(2 pi x k / L) B'_x (b even, > 0), k = (b + 1) / 2 (FP64 sums,
+// then split).  Block: one mode, 256 slab positions; the n1 grid values of each
+// position sit in LDS.
 __global__ __launch_bounds__(256) void bhat_kernel(const double* __restrict__ Bw, size_t ldw,
                                                     const int* __restrict__ gmap, int n1, int G, int M,
                                                     const double* __restrict__ trig, __bf16* __restrict__ hi,
@@ -901,9 +902,9 @@ __global__ __launch_bounds__(256) void bhat_kernel(const double* __restrict__ Bw
     }
     if (r >= (int)Gp) return;
     for (int b = 0; b < nbat; b++) {
-        const int k = b >> 1;
+        const int k = (b + 1) >> 1;
         double acc = 0.0;
-        if (b & 1) {
+        if (b > 0 && !(b & 1)) {
             for (int x = 0; x < n1; x++) acc -= trig[L + (x * k) % L] * bh_v[x * 256 + t];
         } else {
             for (int x = 0; x < n1; x++) acc += trig[(x * k) % L] * bh_v[x * 256 + t];
@@ -1078,7 +1079,7 @@ static int g_s2_split = 8;
 template <typename ASrc, typename Epi>
 static void launch_g3(opsec_ctx* ctx, const ASrc& A, int arows, const __bf16* B, int brows, size_t ld, int K,
                       const Epi& E, int ksplit = 1, int nbat = 1, size_t a_bstride = 0, size_t b_bstride = 0,
-                      int a_shift = 0, size_t b_lo_offset = 0) {
+                      int a_add = 0, int a_shift = 0, size_t b_lo_offset = 0) {
     static bool attr = false;
     if (!attr) {   // 128 KB of dynamic LDS per workgroup
         (void)hipFuncSetAttribute((const void*)gemm3_kernel<ASrcPlanes, EpiPlanes>,
@@ -1099,7 +1100,7 @@ static void launch_g3(opsec_ctx* ctx, const ASrc& A, int arows, const __bf16* B,
     const size_t blo = b_lo_offset ? b_lo_offset : (size_t)brows * ld;
     hipLaunchKernelGGL((gemm3_kernel<ASrc, Epi>), dim3(nmb * nnb * ksplit * nbat), dim3(512), G3_LDS_BYTES, ctx->stream,
                        A, B, B + blo, ld, K, nmb, nnb, g_nt_group, g_nt_remap, ksplit, E, a_bstride, b_bstride,
-                       a_shift);
+                       a_add, a_shift);
 }
 
 // C = A B^T (stage 1, epilogue to split planes) or P = A B^T (stage 2, packed).
@@ -1241,7 +1242,7 @@ static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
     ProjState* ps = ctx->proj;
     const SurveyDev& sv = ctx->sv;
     const int N = ctx->ncells, M = ctx->nmodes;
-    const int n1 = sv.n1, G = sv.n2 * sv.n3, Ng = n1 * G, L = 2 * n1, nbat = 2 * (n1 + 1);
+    const int n1 = sv.n1, G = sv.n2 * sv.n3, Ng = n1 * G, L = 2 * n1, nbat = 2 * n1;
     if (sv.coordsys != OPSEC_COORD_CARTESIAN) return set_error(ctx, OPSEC_EINVAL, "Toeplitz projection needs a box survey");
     if (n1 > 64) return set_error(ctx, OPSEC_EINVAL, "Toeplitz projection supports Nx <= 64 (Nx = %d)", n1);
     const size_t Mp = round_up(M, 256), Gr = round_up(G, 256), Gk = round_up(G, 32), Gp = plane_pitch(Gk);
@@ -1283,9 +1284,10 @@ static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
         std::vector<float> W((size_t)nbat * n1p, 0.f);
         for (int x = 0; x < n1; x++)
             for (int b = 0; b < nbat; b++) {
-                const int k = b >> 1, j = (x * k) % L;
+                const int k = (b + 1) >> 1, j = (x * k) % L;
                 const double ck = (k == 0 || k == n1) ? 1.0 : 2.0;
-                W[(size_t)b * n1p + x] = (float)((b & 1) ? -ck * trig[L + j] / L : ck * trig[j] / L);
+                const bool imag = b > 0 && !(b & 1);
+                W[(size_t)b * n1p + x] = (float)(imag ? -ck * trig[L + j] / L : ck * trig[j] / L);
             }
         if ((rc = dev_realloc(ctx, (void**)&ps->d_W, &ps->cap_W, W.size() * 4))) return rc;
         if ((rc = dev_realloc(ctx, (void**)&ps->d_Cg, &ps->cap_Cg, TOEP_GROUP * cg_pair * sizeof(__bf16)))) return rc;
@@ -1333,7 +1335,7 @@ static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
                                dim3(256), 0, ctx->stream, ps->d_Qhat, G, sv.n3, Th, Tl, Gr, Gp);
             hipEventRecord(evb[4 * z + 1], ctx->stream);
             launch_g3(ctx, ASrcPlanes{Th, Tl, Gp}, (int)Gr, ps->d_Bhat, (int)Mp, Gp, (int)Gk, toY, 1, nbat, plane_t,
-                      plane_b, 1, (size_t)nbat * plane_b);
+                      plane_b, 1, 1, (size_t)nbat * plane_b);
             hipEventRecord(evb[4 * z + 2], ctx->stream);
             hipLaunchKernelGGL(idft_kernel, dim3((unsigned)((G + 127) / 128), (unsigned)M), dim3(256), id_lds,
                                ctx->stream, ps->d_Yhat, nbat, G, n1, ps->d_W, Cz, Cz + Mp * Pg, Pg);
@@ -1342,7 +1344,7 @@ static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
         // stage 2 of the group's bands in one launch: batch q = band z0 + q
         EpiPackedParts toP{ps->d_Ppart, M, np, (size_t)s2 * np};
         launch_g3(ctx, ASrcPlanes{ps->d_Bg, ps->d_Bg + Mp * Pg, Pg}, (int)Mp, ps->d_Cg, (int)Mp, Pg, (int)Kg, toP, s2, g,
-                  0, cg_pair, 0, Mp * Pg);
+                  0, cg_pair, 0, 0, Mp * Pg);
         hipEventRecord(evg[2 * gi], ctx->stream);
         for (int q = 0; q < g; q++)
             hipLaunchKernelGGL(sum_parts_kernel, blocks_for(np), dim3(256), 0, ctx->stream,

@@ -216,12 +216,14 @@ def test_project_bf16x3_gemm_variants(ctx, tile, gen):
     assert err.max() < 5e-5
 
 
-@pytest.mark.parametrize("dims,drop,shuffle", [((8, 8, 8), 0.0, False), ((6, 5, 4), 0.2, True), ((3, 7, 9), 0.0, True)])
+@pytest.mark.parametrize("dims,drop,shuffle", [((8, 8, 8), 0.0, False), ((6, 5, 4), 0.2, True), ((3, 7, 9), 0.0, True),
+                                               ((1, 8, 6), 0.0, False), ((2, 6, 6), 0.1, True)])
 def test_project_toeplitz_vs_fp64(ctx, dims, drop, shuffle):
     """Toeplitz projection (DFT along x, 2 (Nx + 1) slab GEMMs in split-BF16x3)
     against the dense FP64 projection of the same pair tables: a full cubic
     grid, an irregular subset of a non-cubic grid in shuffled cell order, and a
-    grid whose slab (63 positions) is not a multiple of the 256-row tile."""
+    grid whose slab (63 positions) is not a multiple of the 256-row tile, and
+    the shortest x extents (1 and 2 planes)."""
     n1, n2, n3 = dims
     h = 25.0
     cells = synth.box_cells(max(dims), 0.0, h * max(dims))

@@ -81,11 +81,13 @@ def run_path(ctx, L, funcs, quad, prec, trace_prec, lam, y, prior, steps, warmup
     import ctypes as C
     s1, s2 = C.c_float(), C.c_float()
     kg, k1, k2, kn = C.c_float(), C.c_float(), C.c_float(), C.c_int()
-    times, est, kms = [], None, []
+    times, est, kms, qms = [], None, [], []
     for it in range(warmup + steps):
         ctx.sync()
         t0 = time.perf_counter()
         ctx.pair_tables(funcs, quad)
+        ctx.sync()
+        tq = time.perf_counter() - t0
         ctx.project(funcs[1:], prec, keep_on_device=True)
         est = ctx.estimate_device(lam, y, prior, L.OPSEC_MIX_INVERSE, trace_prec)
         ctx.sync()
@@ -94,7 +96,8 @@ def run_path(ctx, L, funcs, quad, prec, trace_prec, lam, y, prior, steps, warmup
         ctx.lib.opsec_internal_project_kernel_ms(ctx.h, C.byref(kg), C.byref(k1), C.byref(k2), C.byref(kn))
         if it >= warmup:
             times.append(dt)
-            kms.append((s1.value, s2.value, kg.value, k1.value, k2.value, kn.value))
+            qms.append(tq)
+            kms.append((s1.value, s2.value, kg.value, k1.value, k2.value, kn.value, 1e3 * tq))
         log(f"[{tag}] step {it - warmup if it >= warmup else 'warmup'}: {dt * 1e3:.1f} ms "
             f"(projection stage 1 {s1.value:.1f} ms, stage 2 {s2.value:.1f} ms)")
     return 1e3 * float(np.mean(times)), est, np.mean(np.array(kms), axis=0)
@@ -287,6 +290,19 @@ def main():
         lam, B = ev, V
     ctx.modes_upload(B)
     prior = ctx.params()
+    # K2 assembly of the dense S_0 (what klt's matvecs read; the projection never
+    # stores S_p): 8 N^2 bytes written, timed once in the setup
+    dS = ctx.dev_alloc(N * N * 8)
+    ctx.sync()
+    ta = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        ctx.check(ctx.lib.opsec_assemble_device(ctx.h, 0, dS, N), "assemble")
+        ctx.sync()
+        ta.append(time.perf_counter() - t1)
+    ctx.dev_free(dS)
+    assembly = {"bytes": 8 * N * N, "ms": 1e3 * min(ta), "GB_per_s": 8 * N * N / min(ta) / 1e9,
+                "kernel": "assemble_kernel (S_0, FP64, row-major)", "timer": "host clock around the call, best of 3"}
     z = np.random.Generator(np.random.Philox(42)).normal(size=M)
     y = np.sqrt(1 + lam) * z
     log(f"setup: KL modes (lambda {lam[0]:.3g} .. {lam[-1]:.3g}) in {time.perf_counter() - t0:.1f} s")
@@ -346,6 +362,23 @@ def main():
                             "value": pairs / (results["fp64"]["ms"] / 1e3),
                             "projection_ms": [float(k64[0]), float(k64[1])],
                             "stage1_tflops": 2.0 * N * N * M * nb / (float(k64[0]) / 1e3) / 1e12}
+    # pair quadrature (K1): distinct integrals and node evaluations per second
+    # against the device's measured FP64 sincos rate (band nodes need two pairs)
+    import ctypes as C
+    sc = C.c_double()
+    ctx.check(ctx.lib.opsec_internal_sincos_rate(ctx.h, C.byref(sc)), "sincos probe")
+    out["quadrature"] = {"sincos_pairs_per_s": sc.value}
+    out["assembly"] = assembly
+    for name in ("fp64", "surrogate"):
+        if name in results:
+            qm = float(results[name]["kms"][6])
+            st = results[name]["stats"]
+            out["quadrature"][name + "_path"] = {
+                "ms_per_step": qm, "integrals": int(st["integrals"]), "node_evaluations": int(st["nodes"]),
+                "surrogate_served": int(st["surrogate"]),
+                "integrals_per_s": st["integrals"] / (qm / 1e3),
+                "node_evaluations_per_s": st["nodes"] / (qm / 1e3),
+                "node_rate_over_half_sincos_rate": st["nodes"] / (qm / 1e3) / (sc.value / 2)}
     if "surrogate_dense" in results:
         kd = results["surrogate_dense"]["kms"]
         out["dense_projection_path"] = {

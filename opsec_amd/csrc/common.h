@@ -108,6 +108,8 @@ struct opsec_ctx {
     opsec::DotState* dot = nullptr;
 
     std::vector<void*> user_bufs;
+    void* d_jobs = nullptr;               // quadrature job buffer (reused across calls)
+    size_t cap_jobs = 0;
 };
 
 namespace opsec {

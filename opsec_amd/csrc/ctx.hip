@@ -322,6 +322,21 @@ __global__ __launch_bounds__(256) void copy16_kernel(const uint4* __restrict__ s
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
 }
+
+// FP64 sin + cos throughput: each thread evaluates `iters` (sin x, cos x) pairs
+// at arguments spread over [0.1, 0.1 + 0.37 iters] rad (the range of k r in the
+// band correlation functions); the sum is stored so nothing is optimised away.
+__global__ __launch_bounds__(256) void sincos_probe_kernel(double* out, int iters) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double x = 0.1 + 1e-7 * (double)(i & 1023), acc = 0.0;
+    for (int k = 0; k < iters; k++) {
+        double sn, cs;
+        sincos(x, &sn, &cs);
+        acc += sn * cs;
+        x += 0.37;
+    }
+    out[i] = acc;
+}
 }  // namespace opsec
 
 extern "C" {
@@ -363,6 +378,7 @@ int opsec_ctx_destroy(opsec_ctx* ctx) {
     dev_free((void**)&ctx->d_nbar);
     dev_free((void**)&ctx->d_Q);
     dev_free((void**)&ctx->d_B);
+    dev_free(&ctx->d_jobs);
     opsec_internal_free_proj(ctx);
     opsec_internal_free_mlp(ctx);
     opsec_internal_free_dot(ctx);
@@ -430,6 +446,32 @@ int opsec_internal_copy(opsec_ctx* ctx, void* dst, const void* src, size_t bytes
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (e != hipSuccess) return set_error(ctx, OPSEC_EHIP, "copy: %s", hipGetErrorString(e));
+    return OPSEC_OK;
+}
+
+// Measured ceiling of FP64 (sin, cos) pairs per second on the whole device.
+int opsec_internal_sincos_rate(opsec_ctx* ctx, double* pairs_per_s) {
+    if (!ctx || !pairs_per_s) return OPSEC_EINVAL;
+    const unsigned blocks = (unsigned)ctx->num_cus * 16;
+    const int iters = 2000;
+    double* d_out = nullptr;
+    OPSEC_HIP_TRY(ctx, hipMalloc(&d_out, (size_t)blocks * 256 * 8));
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipLaunchKernelGGL(opsec::sincos_probe_kernel, dim3(blocks), dim3(256), 0, ctx->stream, d_out, 8);   // warm-up
+    hipEventRecord(e0, ctx->stream);
+    hipLaunchKernelGGL(opsec::sincos_probe_kernel, dim3(blocks), dim3(256), 0, ctx->stream, d_out, iters);
+    hipEventRecord(e1, ctx->stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(d_out);
+    if (e != hipSuccess) return set_error(ctx, OPSEC_EHIP, "sincos probe: %s", hipGetErrorString(e));
+    *pairs_per_s = (double)blocks * 256 * iters / (ms * 1e-3);
     return OPSEC_OK;
 }
 

@@ -500,6 +500,18 @@ struct Scratch {
     ~Scratch() { dev_free(&p); }
 };
 
+// Jobs of nf functions from one per-pair template: job t * npairs + i is
+// template i with function funcs[t].
+__global__ void expand_jobs_kernel(const GaussJob* __restrict__ tmpl, int npairs, const int* __restrict__ funcs,
+                                   int nf, GaussJob* __restrict__ out) {
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (size_t)npairs * nf) return;
+    const int t = (int)(j / npairs), i = (int)(j - (size_t)t * npairs);
+    GaussJob J = tmpl[i];
+    J.f = funcs[t];
+    out[j] = J;
+}
+
 static int make_tent_rule(opsec_ctx* ctx, int nodes, TentRule* r) {
     OPSEC_CHECK_ARG(ctx, nodes >= 2 && nodes <= 32 && nodes % 2 == 0, "gauss nodes per axis must be even, 2..32");
     int m = nodes / 2;
@@ -659,45 +671,56 @@ static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::v
         int rc = make_tent_rule(ctx, q->nodes, &rule);
         if (rc) return rc;
         make_near_rule(&nrule);
+        // one template per pair on the host; the nf x pairs jobs are expanded on
+        // the device into the context's job buffer (far jobs, then near jobs)
         std::vector<GaussJob> far, nearj;
-        for (int t = 0; t < nf; t++)
-            for (size_t i = 0; i < npairs; i++) {
-                const PairGeo& g = geo[i];
-                GaussJob J;
-                J.f = funcs[t];
-                J.key = keys[i];
-                for (int d = 0; d < 3; d++) {
-                    double h1 = g.hi1[d] - g.lo1[d], h2 = g.hi2[d] - g.lo2[d];
-                    if (std::fabs(h1 - h2) > 1e-9 * std::fabs(h1))
-                        return set_error(ctx, OPSEC_EINVAL, "GAUSS quadrature needs equal cell widths");
-                    J.h[d] = h1;
-                    J.D[d] = 0.5 * (g.lo2[d] + g.hi2[d]) - 0.5 * (g.lo1[d] + g.hi1[d]);
-                }
-                (nearflag[i] ? nearj : far).push_back(J);
+        for (size_t i = 0; i < npairs; i++) {
+            const PairGeo& g = geo[i];
+            GaussJob J;
+            J.f = -1;
+            J.key = keys[i];
+            for (int d = 0; d < 3; d++) {
+                double h1 = g.hi1[d] - g.lo1[d], h2 = g.hi2[d] - g.lo2[d];
+                if (std::fabs(h1 - h2) > 1e-9 * std::fabs(h1))
+                    return set_error(ctx, OPSEC_EINVAL, "GAUSS quadrature needs equal cell widths");
+                J.h[d] = h1;
+                J.D[d] = 0.5 * (g.lo2[d] + g.hi2[d]) - 0.5 * (g.lo1[d] + g.hi1[d]);
             }
-        Scratch sj;
-        size_t nj = far.size() + nearj.size();
-        if ((rc = dev_realloc(ctx, &sj.p, &sj.cap, sizeof(GaussJob) * nj))) return rc;
-        GaussJob* dj = (GaussJob*)sj.p;
+            (nearflag[i] ? nearj : far).push_back(J);
+        }
+        const size_t nfar = far.size() * nf, nnear = nearj.size() * nf, nj = nfar + nnear;
+        const size_t ntmpl = far.size() + nearj.size();
+        if ((rc = dev_realloc(ctx, &ctx->d_jobs, &ctx->cap_jobs, sizeof(GaussJob) * (nj + ntmpl) + sizeof(int) * nf)))
+            return rc;
+        GaussJob* dj = (GaussJob*)ctx->d_jobs;
+        GaussJob* dt = dj + nj;
+        int* dfun = (int*)(dt + ntmpl);
         if (!far.empty())
-            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dj, far.data(), sizeof(GaussJob) * far.size(), hipMemcpyHostToDevice, ctx->stream));
+            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dt, far.data(), sizeof(GaussJob) * far.size(), hipMemcpyHostToDevice,
+                                              ctx->stream));
         if (!nearj.empty())
-            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dj + far.size(), nearj.data(), sizeof(GaussJob) * nearj.size(),
+            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dt + far.size(), nearj.data(), sizeof(GaussJob) * nearj.size(),
                                               hipMemcpyHostToDevice, ctx->stream));
-        if (!far.empty()) {
-            hipLaunchKernelGGL(tent_kernel, dim3((unsigned)((far.size() + 255) / 256)), dim3(256), 0, ctx->stream, mv, dj,
-                               (int)far.size(), rule, ctx->sv.L, dQ, nkeys);
+        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dfun, funcs, sizeof(int) * nf, hipMemcpyHostToDevice, ctx->stream));
+        if (!far.empty())
+            hipLaunchKernelGGL(expand_jobs_kernel, dim3((unsigned)((nfar + 255) / 256)), dim3(256), 0, ctx->stream, dt,
+                               (int)far.size(), dfun, nf, dj);
+        if (!nearj.empty())
+            hipLaunchKernelGGL(expand_jobs_kernel, dim3((unsigned)((nnear + 255) / 256)), dim3(256), 0, ctx->stream,
+                               dt + far.size(), (int)nearj.size(), dfun, nf, dj + nfar);
+        if (nfar) {
+            hipLaunchKernelGGL(tent_kernel, dim3((unsigned)((nfar + 255) / 256)), dim3(256), 0, ctx->stream, mv, dj,
+                               (int)nfar, rule, ctx->sv.L, dQ, nkeys);
             OPSEC_HIP_TRY(ctx, hipGetLastError());
         }
-        if (!nearj.empty()) {
-            hipLaunchKernelGGL(tent_near_kernel, dim3((unsigned)nearj.size()), dim3(256), 0, ctx->stream, mv,
-                               dj + far.size(), nrule, ctx->sv.L, dQ, nkeys);
+        if (nnear) {
+            hipLaunchKernelGGL(tent_near_kernel, dim3((unsigned)nnear), dim3(256), 0, ctx->stream, mv, dj + nfar, nrule,
+                               ctx->sv.L, dQ, nkeys);
             OPSEC_HIP_TRY(ctx, hipGetLastError());
         }
         OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         ctx->st_integrals += (int64_t)nj;
-        ctx->st_nodes += (int64_t)far.size() * rule.n * rule.n * rule.n +
-                         (int64_t)nearj.size() * nrule.n * nrule.n * nrule.n;
+        ctx->st_nodes += (int64_t)nfar * rule.n * rule.n * rule.n + (int64_t)nnear * nrule.n * nrule.n * nrule.n;
         return OPSEC_OK;
     }
     return set_error(ctx, OPSEC_EINVAL, "unknown quadrature method %d", method);

@@ -237,7 +237,8 @@ def main():
                     help="split-BF16x3 projection: dense S_p (2 N^2 M stage 1) or Toeplitz (DFT along x, "
                          "2 (Nx + 1) slab GEMMs); the FP64 path is always dense")
     ap.add_argument("--s2-split", type=int, default=8, help="split-K factor of the split-BF16x3 stage 2")
-    ap.add_argument("--nt-group", type=int, default=8, help="split-BF16x3 GEMM tile-group edge")
+    ap.add_argument("--nt-group", type=int, default=8, help="split-BF16x3 GEMM tile-group edge (stage 1)")
+    ap.add_argument("--s2-group", type=int, default=2, help="split-BF16x3 stage-2 tile-group edge")
     ap.add_argument("--nt-remap", type=int, default=0, choices=[0, 1], help="deal tile groups to XCDs")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also time a plain device copy of this many GiB (HBM counter calibration)")
@@ -264,6 +265,7 @@ def main():
     ctx.lib.opsec_internal_set_nt_order(args.nt_group, args.nt_remap)
     ctx.lib.opsec_internal_set_nt_gen(args.nt_gen)
     ctx.lib.opsec_internal_set_stage2_split(args.s2_split)
+    ctx.lib.opsec_internal_set_stage2_group(args.s2_group)
     ctx.set_projection(L.OPSEC_PROJ_TOEPLITZ if args.projection == "toeplitz" else L.OPSEC_PROJ_DENSE)
     ncu, arch = ctx.device_info()
     log(f"device {arch} with {ncu} CUs; workload {args.config}: {N} cells, {nb} bands, {M} modes")

@@ -1073,13 +1073,15 @@ static int g_nt_tile = 2, g_nt_group = 8, g_nt_remap = 0;
 // Stage 1 generates S_p from the pair table (1) or reads S_p planes written by
 // gen_split_kernel (0).
 static int g_nt_gen = 0;
-// Split-K factor of stage 2 on the 256 x 256 kernel.
+// Split-K factor of stage 2 on the 256 x 256 kernel, and its tile-group edge
+// (2 measured best for stage 2; 8 for the stage-1 launches).
 static int g_s2_split = 8;
+static int g_s2_group = 2;
 
 template <typename ASrc, typename Epi>
 static void launch_g3(opsec_ctx* ctx, const ASrc& A, int arows, const __bf16* B, int brows, size_t ld, int K,
                       const Epi& E, int ksplit = 1, int nbat = 1, size_t a_bstride = 0, size_t b_bstride = 0,
-                      int a_add = 0, int a_shift = 0, size_t b_lo_offset = 0, int remap = -1) {
+                      int a_add = 0, int a_shift = 0, size_t b_lo_offset = 0, int remap = -1, int group = -1) {
     static bool attr = false;
     if (!attr) {   // 128 KB of dynamic LDS per workgroup
         (void)hipFuncSetAttribute((const void*)gemm3_kernel<ASrcPlanes, EpiPlanes>,
@@ -1099,8 +1101,8 @@ static void launch_g3(opsec_ctx* ctx, const ASrc& A, int arows, const __bf16* B,
     // the lo plane follows the hi plane (brows x ld) unless an offset is given
     const size_t blo = b_lo_offset ? b_lo_offset : (size_t)brows * ld;
     hipLaunchKernelGGL((gemm3_kernel<ASrc, Epi>), dim3(nmb * nnb * ksplit * nbat), dim3(512), G3_LDS_BYTES, ctx->stream,
-                       A, B, B + blo, ld, K, nmb, nnb, g_nt_group, remap < 0 ? g_nt_remap : remap, ksplit, E,
-                       a_bstride, b_bstride, a_add, a_shift);
+                       A, B, B + blo, ld, K, nmb, nnb, group < 0 ? g_nt_group : group, remap < 0 ? g_nt_remap : remap,
+                       ksplit, E, a_bstride, b_bstride, a_add, a_shift);
 }
 
 // C = A B^T (stage 1, epilogue to split planes) or P = A B^T (stage 2, packed).
@@ -1184,7 +1186,7 @@ static int run_project_bf16x3(opsec_ctx* ctx, int nb, const int* funcs) {
             const size_t np = (size_t)packed_size(M);
             EpiPackedParts toP{ps->d_Ppart, M, np};
             launch_g3(ctx, ASrcPlanes{ps->d_Bp, ps->d_Bp + Mp * Pp, Pp}, (int)Mp, ps->d_Cp, (int)Mp, Pp, (int)Kp, toP,
-                      s2);
+                      s2, 1, 0, 0, 0, 0, 0, -1, g_s2_group);
             hipLaunchKernelGGL(sum_parts_kernel, blocks_for(np), dim3(256), 0, ctx->stream, ps->d_Ppart, s2, np, Pz);
         } else {
             EpiPacked toP{Pz, M};
@@ -1347,7 +1349,7 @@ static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
         // stage 2 of the group's bands in one launch: batch q = band z0 + q
         EpiPackedParts toP{ps->d_Ppart, M, np, (size_t)s2 * np};
         launch_g3(ctx, ASrcPlanes{ps->d_Bg, ps->d_Bg + Mp * Pg, Pg}, (int)Mp, ps->d_Cg, (int)Mp, Pg, (int)Kg, toP, s2, g,
-                  0, cg_pair, 0, 0, Mp * Pg);
+                  0, cg_pair, 0, 0, Mp * Pg, -1, g_s2_group);
         hipEventRecord(evg[2 * gi], ctx->stream);
         for (int q = 0; q < g; q++)
             hipLaunchKernelGGL(sum_parts_kernel, blocks_for(np), dim3(256), 0, ctx->stream,
@@ -1556,6 +1558,13 @@ int opsec_internal_set_nt_tile(int v) {
 int opsec_internal_set_stage2_split(int v) {
     if (v < 1 || v > 16) return OPSEC_EINVAL;
     g_s2_split = v;
+    return OPSEC_OK;
+}
+
+// Tile-group edge of the split-BF16x3 stage 2, for tuning.
+int opsec_internal_set_stage2_group(int v) {
+    if (v < 1 || v > 128) return OPSEC_EINVAL;
+    g_s2_group = v;
     return OPSEC_OK;
 }
 

@@ -38,8 +38,9 @@ PEAK_TF = {"fp64": 78.6, "fp32": 157.3, "bf16": 2516.6}
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB from two rocprofv3 --pmc passes, the factors being what
 # the same counters read for copy16_kernel on a 2 GiB copy (profiles/projection_c2_hbm_counters.txt).
 # Keys: (config, projection, kernel).
-TRAFFIC_BYTES = {("C2", "dense", "stage1"): (18876549 * 2.0 + 885395 * 1.0) * 1024,
-                 ("C2", "toeplitz", "stage1"): (2308298 * 2.0 + 1156031 * 1.0) * 1024}
+TRAFFIC_BYTES = {("C2", "dense", "stage1"): (18956888 * 2.0 + 885416 * 1.0) * 1024,
+                 ("C2", "toeplitz", "stage1"): (1319407 * 2.0 + 1067077 * 1.0) * 1024,
+                 ("C2", "toeplitz", "stage2"): (4888851 * 2.0 + 1153599 * 1.0) * 1024}
 
 
 def log(*a):
@@ -81,6 +82,7 @@ def run_path(ctx, L, funcs, quad, prec, trace_prec, lam, y, prior, steps, warmup
     import ctypes as C
     s1, s2 = C.c_float(), C.c_float()
     kg, k1, k2, kn = C.c_float(), C.c_float(), C.c_float(), C.c_int()
+    l1, l2 = C.c_int(), C.c_int()
     times, est, kms, qms = [], None, [], []
     for it in range(warmup + steps):
         ctx.sync()
@@ -94,10 +96,11 @@ def run_path(ctx, L, funcs, quad, prec, trace_prec, lam, y, prior, steps, warmup
         dt = time.perf_counter() - t0
         ctx.lib.opsec_internal_project_times(ctx.h, C.byref(s1), C.byref(s2))
         ctx.lib.opsec_internal_project_kernel_ms(ctx.h, C.byref(kg), C.byref(k1), C.byref(k2), C.byref(kn))
+        ctx.lib.opsec_internal_project_launches(ctx.h, C.byref(l1), C.byref(l2))
         if it >= warmup:
             times.append(dt)
             qms.append(tq)
-            kms.append((s1.value, s2.value, kg.value, k1.value, k2.value, kn.value, 1e3 * tq))
+            kms.append((s1.value, s2.value, kg.value, k1.value, k2.value, kn.value, 1e3 * tq, l1.value, l2.value))
         log(f"[{tag}] step {it - warmup if it >= warmup else 'warmup'}: {dt * 1e3:.1f} ms "
             f"(projection stage 1 {s1.value:.1f} ms, stage 2 {s2.value:.1f} ms)")
     return 1e3 * float(np.mean(times)), est, np.mean(np.array(kms), axis=0)
@@ -407,14 +410,12 @@ def main():
     if main_path == "surrogate" and args.projection == "toeplitz":
         # stage 1 = 2 Nx real GEMMs of G x G by G x M per band (G = Ny Nz), one launch per band;
         # stage 2 = N M^2 per band, one launch per group of bands
-        import ctypes as C
         kg_ms, k1_ms, k2_ms, launches = [float(x) for x in R["kms"][2:6]]
-        l1, l2 = C.c_int(), C.c_int()
-        ctx.lib.opsec_internal_project_launches(ctx.h, C.byref(l1), C.byref(l2))
+        l1, l2 = int(round(float(R["kms"][7]))), int(round(float(R["kms"][8])))
         G = n * n
         s1_flops, s2_flops = 2.0 * (2 * n) * G * G * M * nb, float(N) * M * M * nb   # per step
         dom = "stage1" if k1_ms >= k2_ms else "stage2"
-        per_launch = k1_ms / l1.value if dom == "stage1" else k2_ms / l2.value
+        per_launch = k1_ms / l1 if dom == "stage1" else k2_ms / l2
         achieved = (s1_flops / k1_ms if dom == "stage1" else s2_flops / k2_ms) * 1e3 / 1e12
         peak = PEAK_TF["bf16"] / 3
         out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
@@ -423,7 +424,7 @@ def main():
                            "kernel": ("gemm3_kernel<ASrcPlanes,EpiYhat> (Toeplitz stage 1, 2 (Nx + 1) slab GEMMs "
                                       "per launch, one launch per band)" if dom == "stage1" else
                                       f"gemm3_kernel<ASrcPlanes,EpiPackedParts> (stage 2, split-K, "
-                                      f"{nb // l2.value} bands per launch)"),
+                                      f"{nb // l2} bands per launch)"),
                            "launch_ms": per_launch,
                            "note": ("useful flops per band: stage 1 2 (2 Nx) G^2 M, stage 2 N M^2; "
                                     "peak = dense bf16 2516.6 TF / 3 products"),

@@ -1460,6 +1460,15 @@ using namespace opsec;
 
 extern "C" {
 
+// A new cell set invalidates the uploaded modes and everything derived from them.
+void opsec_internal_invalidate_modes(opsec_ctx* ctx) {
+    ctx->nmodes = 0;
+    if (!ctx->proj) return;
+    ctx->proj->bp_valid = false;
+    ctx->proj->toep_valid = false;
+    ctx->proj->nb = 0;
+}
+
 void opsec_internal_free_proj(opsec_ctx* ctx) {
     if (!ctx->proj) return;
     dev_free((void**)&ctx->proj->d_Bw);

@@ -825,6 +825,8 @@ int opsec_quad_defaults(int method, opsec_quad* q) {
     return (method >= 0 && method <= 2) ? OPSEC_OK : OPSEC_EINVAL;
 }
 
+void opsec_internal_invalidate_modes(opsec_ctx* ctx);
+
 int opsec_cells_upload(opsec_ctx* ctx, const opsec_cell* cells, int ncells, const opsec_survey* sv) {
     if (!ctx) return OPSEC_EINVAL;
     OPSEC_CHECK_ARG(ctx, cells && ncells > 0, "no cells");
@@ -837,6 +839,7 @@ int opsec_cells_upload(opsec_ctx* ctx, const opsec_cell* cells, int ncells, cons
                         "cell %d has grid index outside the survey grid", a);
     }
     OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    opsec_internal_invalidate_modes(ctx);   // modes are weighted by the old cells' Nbar
     ctx->sv = s;
     ctx->ncells = ncells;
     ctx->nkeys = s.n1 * s.n2 * s.n3;

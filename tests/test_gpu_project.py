@@ -273,3 +273,22 @@ def test_project_toeplitz_rejects_shared_grid_position(ctx):
             ctx.project([1], lib.OPSEC_PREC_BF16X3)
     finally:
         ctx.set_projection(lib.OPSEC_PROJ_DENSE)
+
+
+def test_new_cells_invalidate_modes(ctx):
+    """Modes are weighted by the cells' Nbar: a new cell upload drops them until the next
+    mode upload, so a projection cannot run on modes of another cell set."""
+    cells = synth.box_cells(4, 0.0, 100.0)
+    sv = lib.survey_box(4, 0.0, 100.0)
+    ctx.model_real(GOLD["pk_k"], GOLD["pk_P"], [0.02], [0.1], lib.OPSEC_XI_CLOSED)
+    ctx.cells_upload(cells, sv)
+    ctx.pair_tables([1], ctx.quad(lib.OPSEC_QUAD_GAUSS, nodes=4, near=1))
+    B = np.linalg.qr(np.random.default_rng(1).normal(size=(len(cells), 8)))[0].T.copy()
+    ctx.modes_upload(B)
+    first = ctx.project([1], lib.OPSEC_PREC_FP64)
+    ctx.cells_upload(cells, sv)
+    ctx.pair_tables([1], ctx.quad(lib.OPSEC_QUAD_GAUSS, nodes=4, near=1))
+    with pytest.raises(lib.OpsecError, match="no modes uploaded"):
+        ctx.project([1], lib.OPSEC_PREC_FP64)
+    ctx.modes_upload(B)
+    assert np.array_equal(ctx.project([1], lib.OPSEC_PREC_FP64), first)

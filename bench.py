@@ -238,7 +238,7 @@ def main():
                     help="stage 1 generates S_p from the pair table (1) or reads S_p planes (0)")
     ap.add_argument("--projection", default="toeplitz", choices=["dense", "toeplitz"],
                     help="split-BF16x3 projection: dense S_p (2 N^2 M stage 1) or Toeplitz (DFT along x, "
-                         "2 (Nx + 1) slab GEMMs); the FP64 path is always dense")
+                         "2 Nx slab GEMMs); the FP64 path is always dense")
     ap.add_argument("--s2-split", type=int, default=8, help="split-K factor of the split-BF16x3 stage 2")
     ap.add_argument("--nt-group", type=int, default=8, help="split-BF16x3 GEMM tile-group edge (stage 1)")
     ap.add_argument("--s2-group", type=int, default=2, help="split-BF16x3 stage-2 tile-group edge")
@@ -421,7 +421,7 @@ def main():
         out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                            "frac": achieved / peak,
                            "traffic": TRAFFIC_BYTES.get((args.config, "toeplitz", dom)),
-                           "kernel": ("gemm3_kernel<ASrcPlanes,EpiYhat> (Toeplitz stage 1, 2 (Nx + 1) slab GEMMs "
+                           "kernel": ("gemm3_kernel<ASrcPlanes,EpiYhat> (Toeplitz stage 1, 2 Nx slab GEMMs "
                                       "per launch, one launch per band)" if dom == "stage1" else
                                       f"gemm3_kernel<ASrcPlanes,EpiPackedParts> (stage 2, split-K, "
                                       f"{nb // l2} bands per launch)"),

@@ -53,6 +53,27 @@ struct SurveyDev {
     double L;
 };
 
+// Pair-table key of two cells from their grid indices.  Box surveys:
+// (|dx| Ny + |dy|) Nz + |dz| (src/sig.cpp:109-112).  Spherical surveys: the
+// arc-pair cache of src/sig.cpp:150-190, (h_a narcs + h_b) Nphi + |dphi_a - dphi_b|
+// with arcs h = d_r Nmu + d_mu.  narcs = 0 selects the box key.
+struct PairKey {
+    int n2, n3, narcs;
+    __host__ __device__ __forceinline__ int operator()(int a1, int a2, int a3, int b1, int b2, int b3) const {
+        const int dz = a3 > b3 ? a3 - b3 : b3 - a3;
+        if (narcs) return ((a1 * n2 + a2) * narcs + b1 * n2 + b2) * n3 + dz;
+        return ((a1 > b1 ? a1 - b1 : b1 - a1) * n2 + (a2 > b2 ? a2 - b2 : b2 - a2)) * n3 + dz;
+    }
+};
+inline PairKey pair_key_of(const SurveyDev& s) {
+    return PairKey{s.n2, s.n3, s.coordsys == OPSEC_COORD_SPHERICAL ? s.n1 * s.n2 : 0};
+}
+// Number of pair-table keys: n1 n2 n3 (box), (n1 n2)^2 n3 (spherical).
+inline long long pair_nkeys_of(const SurveyDev& s) {
+    const long long g = (long long)s.n1 * s.n2;
+    return s.coordsys == OPSEC_COORD_SPHERICAL ? g * g * s.n3 : g * s.n3;
+}
+
 // ---- context -------------------------------------------------------------
 struct DevBuf {
     void* p = nullptr;

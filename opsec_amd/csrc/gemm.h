@@ -64,18 +64,25 @@ struct SrcKColMajor {          // X[i][k] = p[i*ld + k], i.e. a column-major K x
     __device__ __forceinline__ double get(int i, int k) const { return p[(size_t)i * ld + k]; }
 };
 
-// Signal matrix of a box survey generated from its pair table.
-struct SrcPairTable {
+// Signal matrix generated from the pair table: box key (SPH false) or the
+// arc-pair key of a spherical survey (SPH true, narcs = Nr Nmu).
+template <bool SPH>
+struct SrcPairTableT {
     const double* Q;           // pair table of one function
     const int* cellidx;        // ncells x 3 grid indices
-    int Ny, Nz;
+    int Ny, Nz, narcs;
     static constexpr bool k_contig = true;
     __device__ __forceinline__ double get(int a, int b) const {
         const int* ca = cellidx + 3 * a;
         const int* cb = cellidx + 3 * b;
-        int key = (abs(ca[0] - cb[0]) * Ny + abs(ca[1] - cb[1])) * Nz + abs(ca[2] - cb[2]);
+        int key;
+        if constexpr (SPH)
+            key = ((ca[0] * Ny + ca[1]) * narcs + cb[0] * Ny + cb[1]) * Nz + abs(ca[2] - cb[2]);
+        else
+            key = (abs(ca[0] - cb[0]) * Ny + abs(ca[1] - cb[1])) * Nz + abs(ca[2] - cb[2]);
         return Q[key];
     }
 };
+typedef SrcPairTableT<false> SrcPairTable;
 
 }  // namespace opsec

@@ -203,11 +203,14 @@ __global__ __launch_bounds__(GT_THREADS) void gemm_kernel(SrcA A, size_t a_batch
 }
 
 // Sources with batch shifting (z index = band)
-struct GA_PairTable : SrcPairTable {
+template <bool SPH>
+struct GA_PairTableT : SrcPairTableT<SPH> {
     size_t qstride;
     const int* funcs;
-    __device__ __forceinline__ void p_shift(int z, size_t) { Q += (size_t)funcs[z] * qstride; }
+    __device__ __forceinline__ void p_shift(int z, size_t) { this->Q += (size_t)funcs[z] * qstride; }
 };
+typedef GA_PairTableT<false> GA_PairTable;
+typedef GA_PairTableT<true> GA_PairTableSph;
 struct GA_RowMajor : SrcDenseRowMajor {
     __device__ __forceinline__ void p_shift(int z, size_t s) { p += (size_t)z * s; }
 };
@@ -300,7 +303,7 @@ __global__ __launch_bounds__(256) void split_plane_kernel(const double* __restri
 // S_p planes from the pair table: S[a][b] = Q[key(a, b)] for a, b < N (the
 // sqrt(Nbar) factors live in B').  One thread per 8 consecutive columns.
 __global__ __launch_bounds__(256) void gen_split_kernel(const double* __restrict__ Q, const int* __restrict__ cellidx,
-                                                        int Ny, int Nz, int N, __bf16* __restrict__ hi,
+                                                        PairKey K, int N, __bf16* __restrict__ hi,
                                                         __bf16* __restrict__ lo, size_t ld, int rows_p) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t per_row = ld / 8;
@@ -320,8 +323,7 @@ __global__ __launch_bounds__(256) void gen_split_kernel(const double* __restrict
         float v = 0.0f;
         if (a < (size_t)N && b < N) {
             const int* cb = cellidx + 3 * b;
-            const int key = (abs(xa - cb[0]) * Ny + abs(ya - cb[1])) * Nz + abs(za - cb[2]);
-            v = (float)Q[key];
+            v = (float)Q[K(xa, ya, za, cb[0], cb[1], cb[2])];
         }
         __bf16 x, y;
         split2(v, x, y);
@@ -1132,7 +1134,12 @@ static int run_project_bf16x3(opsec_ctx* ctx, int nb, const int* funcs) {
     const size_t Kp = round_up(N, 256), Mp = round_up(M, 256), Pp = plane_pitch(Kp);
     int rc;
     if ((rc = dev_realloc(ctx, (void**)&ps->d_Bp, &ps->cap_Bp, 2 * Mp * Pp * sizeof(__bf16)))) return rc;
-    if (!(g_nt_gen && ctx->sv.n1 < 1024 && ctx->sv.n2 < 1024 && ctx->sv.n3 < 1024))
+    // packed coordinates (box grids with indices below 1024) let the S_p planes be
+    // generated from 32-bit cell words; other grids use the generic generator
+    const bool packable = ctx->sv.coordsys == OPSEC_COORD_CARTESIAN && ctx->sv.n1 < 1024 && ctx->sv.n2 < 1024 &&
+                          ctx->sv.n3 < 1024;
+    const bool gen = g_nt_gen && packable;
+    if (!gen)
         if ((rc = dev_realloc(ctx, (void**)&ps->d_Sp, &ps->cap_Sp, 2 * Kp * Pp * sizeof(__bf16)))) return rc;
     if ((rc = dev_realloc(ctx, (void**)&ps->d_Cp, &ps->cap_Cp, 2 * Mp * Pp * sizeof(__bf16)))) return rc;
     if ((rc = dev_realloc(ctx, &ps->d_P, &ps->cap_P, (size_t)nb * packed_size(M) * 8))) return rc;
@@ -1144,9 +1151,6 @@ static int run_project_bf16x3(opsec_ctx* ctx, int nb, const int* funcs) {
         if ((rc = split_planes(ctx, ps->d_Bw, ps->ldw, M, N, ps->d_Bp, (int)Mp, Pp))) return rc;
         ps->bp_valid = true;
     }
-    // packed coordinates need grid indices below 1024
-    const bool packable = ctx->sv.n1 < 1024 && ctx->sv.n2 < 1024 && ctx->sv.n3 < 1024;
-    const bool gen = g_nt_gen && packable;
     if (packable) {
         if ((rc = dev_realloc(ctx, (void**)&ps->d_Qhl, &ps->cap_Qhl, (size_t)ctx->nkeys * 4))) return rc;
         if ((rc = dev_realloc(ctx, (void**)&ps->d_cpk, &ps->cap_cpk, Kp * 4))) return rc;
@@ -1171,7 +1175,7 @@ static int run_project_bf16x3(opsec_ctx* ctx, int nb, const int* funcs) {
                                    Pp, (int)r0);
         else if (!gen)
             hipLaunchKernelGGL(gen_split_kernel, blocks_for(Kp * Pp / 8), dim3(256), 0, ctx->stream, Qf,
-                               ctx->d_cellidx, ctx->sv.n2, ctx->sv.n3, N, Sh, Sl, Pp, (int)Kp);
+                               ctx->d_cellidx, pair_key_of(ctx->sv), N, Sh, Sl, Pp, (int)Kp);
         hipEventRecord(ev[4 * z + 1], ctx->stream);
         if (gen) {
             launch_g3(ctx, ASrcTable{ps->d_Qhl, ps->d_cpk, ctx->sv.n2, ctx->sv.n3}, (int)Kp, ps->d_Bp, (int)Mp, Pp,
@@ -1403,13 +1407,23 @@ static int run_project(opsec_ctx* ctx, int nb, const int* d_funcs, const int* fu
     int rc;
     if ((rc = dev_realloc(ctx, &ps->d_C, &ps->cap_C, (size_t)nb * ldc * M * esize))) return rc;
     if ((rc = dev_realloc(ctx, &ps->d_P, &ps->cap_P, (size_t)nb * packed_size(M) * 8))) return rc;
+    const bool sph = ctx->sv.coordsys == OPSEC_COORD_SPHERICAL;
     GA_PairTable A;
     A.Q = ctx->d_Q;
     A.cellidx = ctx->d_cellidx;
     A.Ny = ctx->sv.n2;
     A.Nz = ctx->sv.n3;
+    A.narcs = ctx->sv.n1 * ctx->sv.n2;
     A.qstride = ctx->nkeys;
     A.funcs = d_funcs;
+    GA_PairTableSph As;
+    As.Q = A.Q;
+    As.cellidx = A.cellidx;
+    As.Ny = A.Ny;
+    As.Nz = A.Nz;
+    As.narcs = A.narcs;
+    As.qstride = A.qstride;
+    As.funcs = A.funcs;
     GB_ColMajor Bop;
     Bop.p = ps->d_Bw;
     Bop.ld = ps->ldw;
@@ -1425,16 +1439,24 @@ static int run_project(opsec_ctx* ctx, int nb, const int* d_funcs, const int* fu
     PutPacked outP{(double*)ps->d_P, (size_t)packed_size(M)};
     if (prec == OPSEC_PREC_FP64) {
         PutColMajor<double> outC{(double*)ps->d_C, ldc, ldc * M};
-        hipLaunchKernelGGL((gemm_kernel<double, GA_PairTable, GB_ColMajor, PutColMajor<double>, false>), g1,
-                           dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, 0, outC, N, M, N);
+        if (sph)
+            hipLaunchKernelGGL((gemm_kernel<double, GA_PairTableSph, GB_ColMajor, PutColMajor<double>, false>), g1,
+                               dim3(GT_THREADS), 0, ctx->stream, As, 0, Bop, 0, outC, N, M, N);
+        else
+            hipLaunchKernelGGL((gemm_kernel<double, GA_PairTable, GB_ColMajor, PutColMajor<double>, false>), g1,
+                               dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, 0, outC, N, M, N);
         hipEventRecord(e1, ctx->stream);
         GB_ColMajorT<double> B2{(const double*)ps->d_C, ldc};
         hipLaunchKernelGGL((gemm_kernel<double, GA_ColMajorT, GB_ColMajorT<double>, PutPacked, true>), g2,
                            dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, ldc * M, outP, M, M, N);
     } else if (prec == OPSEC_PREC_FP32) {
         PutColMajor<float> outC{(float*)ps->d_C, ldc, ldc * M};
-        hipLaunchKernelGGL((gemm_kernel<float, GA_PairTable, GB_ColMajor, PutColMajor<float>, false>), g1,
-                           dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, 0, outC, N, M, N);
+        if (sph)
+            hipLaunchKernelGGL((gemm_kernel<float, GA_PairTableSph, GB_ColMajor, PutColMajor<float>, false>), g1,
+                               dim3(GT_THREADS), 0, ctx->stream, As, 0, Bop, 0, outC, N, M, N);
+        else
+            hipLaunchKernelGGL((gemm_kernel<float, GA_PairTable, GB_ColMajor, PutColMajor<float>, false>), g1,
+                               dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, 0, outC, N, M, N);
         hipEventRecord(e1, ctx->stream);
         GB_ColMajorT<float> B2{(const float*)ps->d_C, ldc};
         hipLaunchKernelGGL((gemm_kernel<float, GA_ColMajorT, GB_ColMajorT<float>, PutPacked, true>), g2,

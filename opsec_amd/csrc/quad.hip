@@ -453,10 +453,10 @@ __global__ void fill_u64_kernel(unsigned long long* p, size_t n, unsigned long l
     if (i < n) p[i] = v;
 }
 
-// K2: s[i*ld + j] = sqrt(Nbar_a Nbar_b) Q[key(a,b)] (src/sig.cpp:120).
+// K2: s[i*ld + j] = sqrt(Nbar_a Nbar_b) Q[key(a,b)] (src/sig.cpp:120, 203).
 // Tile of 8 rows x 512 columns per 256-thread block; column geometry in LDS.
 __global__ __launch_bounds__(256) void assemble_kernel(const int* cellidx, const double* nbar, const int* rows,
-                                                       int nrows, const int* cols, int ncols, int Ny, int Nz,
+                                                       int nrows, const int* cols, int ncols, PairKey K,
                                                        const double* Qf, double* s, size_t ld) {
     __shared__ int cx[512], cy[512], cz[512];
     __shared__ double cn[512];
@@ -481,7 +481,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(const int* cellidx, const
         for (int t = threadIdx.x; t < 512; t += 256) {
             int j = j0 + t;
             if (j >= ncols) break;
-            int key = box_key(ax, ay, az, cx[t], cy[t], cz[t], Ny, Nz);
+            const int key = K(ax, ay, az, cx[t], cy[t], cz[t]);
             s[(size_t)i * ld + j] = sqrt(na * cn[t]) * Qf[key];
         }
     }
@@ -759,7 +759,57 @@ static int near_of(const opsec_cell& a, const opsec_cell& b, int near) {
     return m < near ? 1 : 0;
 }
 
+// Arc-pair tables of a spherical survey (src/sig.cpp:125-213) for the block of
+// all rows and all columns: for every ordered arc pair (h_a, h_b) and each
+// k = |dphi_a - dphi_b| the integral of the first cell pair met in cell order
+// (row cell of arc h_a, then column cell of arc h_b), taken with the lower
+// cell index first.  Keys that no cell pair carries stay zero.  The integrals
+// run in chunks of row arcs, so the host job list stays bounded.
+static int build_pair_tables_sph(opsec_ctx* ctx, int nf, const int* funcs, const opsec_quad* q) {
+    OPSEC_CHECK_ARG(ctx, q->method == OPSEC_QUAD_GM,
+                    "spherical surveys use the adaptive (GM) rule; the tent reduction and the surrogate need box cells");
+    const SurveyDev& sv = ctx->sv;
+    const int Nmu = sv.n2, Nphi = sv.n3, narcs = sv.n1 * sv.n2, N = ctx->ncells;
+    const std::vector<opsec_cell>& cells = ctx->h_cells;
+    std::vector<std::vector<int>> arcs(narcs);
+    for (int a = 0; a < N; a++) arcs[cells[a].d1 * Nmu + cells[a].d2].push_back(a);
+    for (int t = 0; t < nf; t++)
+        OPSEC_HIP_TRY(ctx, hipMemsetAsync(ctx->d_Q + (size_t)funcs[t] * ctx->nkeys, 0, (size_t)ctx->nkeys * 8,
+                                          ctx->stream));
+    const size_t chunk = std::max<size_t>(4096, ((size_t)1 << 22) / (size_t)nf);   // pairs per quadrature call
+    std::vector<int> keys, nearflag, tab(Nphi);
+    std::vector<PairGeo> geo;
+    for (int ha = 0; ha < narcs; ha++) {
+        if (!arcs[ha].empty())
+            for (int hb = 0; hb < narcs; hb++) {
+                if (arcs[hb].empty()) continue;
+                std::fill(tab.begin(), tab.end(), 0);
+                int left = Nphi;
+                for (int a : arcs[ha]) {
+                    for (int b : arcs[hb]) {
+                        const int k = std::abs(cells[a].d3 - cells[b].d3);
+                        if (tab[k]) continue;
+                        tab[k] = 1;
+                        keys.push_back((ha * narcs + hb) * Nphi + k);
+                        geo.push_back(pair_geo(cells[std::min(a, b)], cells[std::max(a, b)]));
+                        if (--left == 0) break;
+                    }
+                    if (left == 0) break;
+                }
+            }
+        if (keys.size() >= chunk || (ha == narcs - 1 && !keys.empty())) {
+            nearflag.assign(keys.size(), 0);
+            int rc = run_quadrature(ctx, nf, funcs, keys, geo, nearflag, q, ctx->d_Q, (size_t)ctx->nkeys);
+            if (rc) return rc;
+            keys.clear();
+            geo.clear();
+        }
+    }
+    return OPSEC_OK;
+}
+
 int build_pair_tables(opsec_ctx* ctx, int nf, const int* funcs, const opsec_quad* q) {
+    if (ctx->sv.coordsys == OPSEC_COORD_SPHERICAL) return build_pair_tables_sph(ctx, nf, funcs, q);
     // Keys of the full survey matrix: first occurrences along row 0, then the
     // grid-canonical pair for keys that row 0 does not carry.
     const SurveyDev& sv = ctx->sv;
@@ -838,11 +888,13 @@ int opsec_cells_upload(opsec_ctx* ctx, const opsec_cell* cells, int ncells, cons
         OPSEC_CHECK_ARG(ctx, c.d1 >= 0 && c.d1 < s.n1 && c.d2 >= 0 && c.d2 < s.n2 && c.d3 >= 0 && c.d3 < s.n3,
                         "cell %d has grid index outside the survey grid", a);
     }
+    OPSEC_CHECK_ARG(ctx, pair_nkeys_of(s) <= INT32_MAX, "survey grid too large for a pair table (%lld keys)",
+                    pair_nkeys_of(s));
     OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
     opsec_internal_invalidate_modes(ctx);   // modes are weighted by the old cells' Nbar
     ctx->sv = s;
     ctx->ncells = ncells;
-    ctx->nkeys = s.n1 * s.n2 * s.n3;
+    ctx->nkeys = (int)pair_nkeys_of(s);
     ctx->h_cells.assign(cells, cells + ncells);
     std::vector<int> idx(3 * (size_t)ncells);
     std::vector<double> nb(ncells);
@@ -877,7 +929,6 @@ int opsec_pair_tables(opsec_ctx* ctx, int nf, const int* funcs, const opsec_quad
     if (!ctx) return OPSEC_EINVAL;
     if (ctx->ncells == 0) return set_error(ctx, OPSEC_ESTATE, "no survey uploaded");
     if (ctx->model_kind < 0) return set_error(ctx, OPSEC_ESTATE, "no model loaded");
-    OPSEC_CHECK_ARG(ctx, ctx->sv.coordsys == OPSEC_COORD_CARTESIAN, "device-resident pair tables support box surveys");
     int rc = check_quad(ctx, q);
     if (rc) return rc;
     OPSEC_CHECK_ARG(ctx, nf >= 1 && funcs, "no functions");
@@ -927,7 +978,7 @@ int opsec_assemble_device(opsec_ctx* ctx, int f, double* dS, int ld) {
     if (ld <= 0) ld = N;
     OPSEC_CHECK_ARG(ctx, ld >= N, "leading dimension %d < Ncells %d", ld, N);
     hipLaunchKernelGGL(assemble_kernel, dim3((N + 511) / 512, (N + 7) / 8), dim3(256), 0, ctx->stream, ctx->d_cellidx,
-                       ctx->d_nbar, (const int*)nullptr, N, (const int*)nullptr, N, ctx->sv.n2, ctx->sv.n3,
+                       ctx->d_nbar, (const int*)nullptr, N, (const int*)nullptr, N, pair_key_of(ctx->sv),
                        ctx->d_Q + (size_t)f * ctx->nkeys, dS, (size_t)ld);
     OPSEC_HIP_TRY(ctx, hipGetLastError());
     return OPSEC_OK;
@@ -1156,7 +1207,7 @@ int opsec_sig_block(opsec_ctx* ctx, const opsec_cell* cells, int ncells, const i
         if (rc) goto done;
         hipLaunchKernelGGL(assemble_kernel, dim3((ncols + 511) / 512, (nrows + 7) / 8), dim3(256), 0, st,
                            (const int*)s_idx.p, (const double*)s_nb.p, (const int*)s_rows.p, nrows,
-                           (const int*)s_cols.p, ncols, svd.n2, svd.n3, (const double*)s_Q.p + (size_t)f * nk,
+                           (const int*)s_cols.p, ncols, pair_key_of(svd), (const double*)s_Q.p + (size_t)f * nk,
                            (double*)s_out.p, (size_t)lld);
         TRY(hipGetLastError());
         // copy rows back, only the ncols entries of each row (the rest of s is untouched)

@@ -237,6 +237,47 @@ int opsec_count_galaxies(opsec_ctx* ctx, const opsec_galaxy* gals, int64_t ngals
  * (an extra outside bin is ignored); x (optional): Ncells; y: Nmodes. */
 int opsec_pixel_values(opsec_ctx* ctx, const double* counts, double* x, double* y);
 
+/* ---- basis cells and their expected counts (the basis stage) ------------ */
+enum { OPSEC_SEL_UNIFORM = 0, OPSEC_SEL_RADIAL = 1 };
+
+/* Selection function nbar of a survey.  UNIFORM: the constant nbar of a
+ * BoxSurvey (src/BoxSurvey.cpp:59-75).  RADIAL: SphericalSurvey's radial
+ * linear spline (rad_r ascending; zero outside [rad_r[0], rad_r[nrad-1]))
+ * times the HEALPix completeness mask[pixel(acos(mu), phi)] of 12 nside^2
+ * values in RING (nest = 0) or NESTED (nest = 1) order, nside = 0 meaning
+ * completeness 1 (src/SphericalSurvey.cpp:51-82, :104-140). */
+typedef struct opsec_selection {
+    int32_t kind;          /* OPSEC_SEL_*                                   */
+    int32_t nest;
+    int64_t nside;
+    double nbar;           /* UNIFORM                                       */
+    const double* rad_r;   /* RADIAL: nrad nodes                            */
+    const double* rad_n;
+    int32_t nrad;
+    int32_t reserved;
+    const float* mask;     /* RADIAL: 12 nside^2 values, or NULL (nside 0)  */
+} opsec_selection;
+
+/* The cells of the survey grid with their volume Veff and expected count
+ * Nbar = integral of nbar over the cell -- basis's FinalizeCell and cell loop
+ * (src/basis.cpp:89-128, :170-190).  Cells come in grid order with the bounds
+ * and volume of Survey::CreateEmptyCell (src/Survey.cpp:71-91).  Each Nbar is
+ * the reference's adaptive Genz-Malik cubature (maxeval 5e6, epsrel and epsabs
+ * in the reference's swapped argument order, so the absolute tolerance is
+ * epsrel), one GPU wavefront per cell.  Spherical cells are integrated over
+ * r^3 as in the reference and the result divided by 3 (the reference keeps
+ * the factor: its spherical Nbar is 3x the expected count).  Cells with
+ * Nbar <= 0 are dropped and the rest numbered a = 0, 1, ...  out holds
+ * n1 n2 n3 records; *ncells receives the number kept; *nevals (optional) the
+ * integrand evaluations. */
+int opsec_basis(opsec_ctx* ctx, const opsec_survey* sv, const opsec_selection* sel, double epsrel,
+                double epsabs, opsec_cell* out, int* ncells, int64_t* nevals);
+
+/* HEALPix pixel indices of n directions (theta, phi) on the device (nest: 1
+ * NESTED, 0 RING), as the completeness lookup of opsec_basis computes them. */
+int opsec_healpix_pixels(opsec_ctx* ctx, int64_t nside, int nest, int64_t n, const double* theta,
+                         const double* phi, int64_t* pix);
+
 /* ---- Fisher matrix and band powers (K4) --------------------------------- */
 enum { OPSEC_MIX_INVERSE = 0, OPSEC_MIX_INVERSE_SQRT = 1, OPSEC_MIX_IDENTITY = 2 };
 

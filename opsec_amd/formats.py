@@ -75,6 +75,14 @@ def read_doubles(path):
     return blocks[0][0], [np.frombuffer(b[2], dtype="<f8") for b in blocks]
 
 
+def read_cells(path):
+    """(options, cells) of a cellfile ("4i8d" records)."""
+    from .lib import CELL_DTYPE
+    opts, fmt, raw, n = read_blocks(path)[0]
+    assert fmt == "4i8d", fmt
+    return opts, np.frombuffer(raw, dtype=CELL_DTYPE).copy()
+
+
 def write_cells(path, cells, survey_opts):
     opts = dict(survey_opts)
     opts["Ncells"] = len(cells)

@@ -115,7 +115,7 @@ std::vector<opsec_galaxy> read_galaxies(const std::string& path, const char* who
 }
 
 // Two-column text table (src/Spline.cpp:49-88): '#' lines skipped.
-static bool load_pkfile(const std::string& path, std::vector<double>& k, std::vector<double>& P) {
+bool load_columns(const std::string& path, std::vector<double>& k, std::vector<double>& P) {
     FILE* f = std::fopen(path.c_str(), "r");
     if (!f) {
         error("LoadData: could not read file '%s'\n", path.c_str());
@@ -142,7 +142,7 @@ bool load_model(const Config& cfg, ModelSpec* m) {
             error("RealModel: must set config option 'pkfile'\n");
             return false;
         }
-        if (!load_pkfile(sub.get("pkfile"), m->pk_k, m->pk_P) || m->pk_k.size() < 2) return false;
+        if (!load_columns(sub.get("pkfile"), m->pk_k, m->pk_P) || m->pk_k.size() < 2) return false;
         if (sub.has("bands")) {
             // RealModel.bands: explicit edges (src/RealModel.cpp:44-59).  The
             // reference always sizes the edge list to 100 (50 bands); here the

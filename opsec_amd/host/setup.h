@@ -32,6 +32,10 @@ std::vector<double> read_modes(const std::string& path, int Nmodes, int Ncells, 
 // (nothing if the file or its header is unusable).
 std::vector<opsec_galaxy> read_galaxies(const std::string& path, const char* who);
 
+// First two columns of a text table, '#' lines skipped (LoadData,
+// src/Spline.cpp:49-88).  Returns false (after printing why) if unreadable.
+bool load_columns(const std::string& path, std::vector<double>& x, std::vector<double>& y);
+
 // Model from the "model" key and its "<Model>." sub-config
 // (src/Model.cpp:26-51, src/RealModel.cpp:33-76, src/TestModel.cpp:34-45).
 // Returns false (after printing why) if the model cannot be built.

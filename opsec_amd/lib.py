@@ -31,6 +31,8 @@ OPSEC_PROJ_TOEPLITZ = 1
 OPSEC_MIX_INVERSE = 0
 OPSEC_MIX_INVERSE_SQRT = 1
 OPSEC_MIX_IDENTITY = 2
+OPSEC_SEL_UNIFORM = 0
+OPSEC_SEL_RADIAL = 1
 
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ECONV", -5: "ESTATE", -6: "ENODEV"}
 
@@ -54,6 +56,12 @@ class Quad(C.Structure):
     _fields_ = [("method", C.c_int32), ("nodes", C.c_int32), ("near", C.c_int32), ("maxeval", C.c_int32),
                 ("epsrel", C.c_double), ("epsabs", C.c_double), ("spot_frac", C.c_double),
                 ("spot_tol", C.c_double), ("spot_seed", C.c_uint64)]
+
+
+class Selection(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("nest", C.c_int32), ("nside", C.c_int64), ("nbar", C.c_double),
+                ("rad_r", C.c_void_p), ("rad_n", C.c_void_p), ("nrad", C.c_int32), ("reserved", C.c_int32),
+                ("mask", C.c_void_p)]
 
 
 class Estimate(C.Structure):
@@ -110,6 +118,9 @@ _SIGS = {
     "opsec_project_dense": ([_P, _I, _P, _I, _P, _I, _P], _I),
     "opsec_count_galaxies": ([_P, _P, C.c_int64, _I, _P], _I),
     "opsec_pixel_values": ([_P, _P, _P, _P], _I),
+    "opsec_basis": ([_P, C.POINTER(Survey), C.POINTER(Selection), C.c_double, C.c_double, _P, C.POINTER(_I),
+                     C.POINTER(C.c_int64)], _I),
+    "opsec_healpix_pixels": ([_P, C.c_int64, _I, C.c_int64, _P, _P, _P], _I),
     "opsec_estimate_run": ([_P, _I, _I, _P, _P, _P, _P, _I, _I, C.POINTER(Estimate)], _I),
     "opsec_estimate_device": ([_P, _P, _P, _P, _I, _I, C.POINTER(Estimate)], _I),
     "opsec_mlp_defaults": ([C.POINTER(MlpTrainOpts)], _I),
@@ -354,6 +365,28 @@ class Context:
         y = np.zeros(self._nmodes)
         self.check(self.lib.opsec_pixel_values(self.h, _ptr(c), _ptr(x), _ptr(y)), "pixel_values")
         return x, y
+
+    def basis(self, survey, kind=OPSEC_SEL_UNIFORM, nbar=0.0, rad_r=None, rad_n=None, mask=None, nest=0,
+              epsrel=1e-5, epsabs=1e-10):
+        """Basis cells of `survey` with their Nbar (opsec_basis); returns
+        (cells, integrand evaluations)."""
+        rr = None if rad_r is None else f64(rad_r)
+        rn = None if rad_n is None else f64(rad_n)
+        mk = None if mask is None else np.ascontiguousarray(mask, dtype=np.float32)
+        nside = 0 if mk is None else int(round(np.sqrt(len(mk) / 12)))
+        sel = Selection(kind, nest, nside, nbar, _ptr(rr), _ptr(rn), 0 if rr is None else len(rr), 0, _ptr(mk))
+        out = np.zeros(survey.n1 * survey.n2 * survey.n3, dtype=CELL_DTYPE)
+        n, ev = C.c_int(), C.c_int64()
+        self.check(self.lib.opsec_basis(self.h, C.byref(survey), C.byref(sel), epsrel, epsabs, _ptr(out), C.byref(n),
+                                        C.byref(ev)), "basis")
+        return out[:n.value].copy(), ev.value
+
+    def healpix_pixels(self, nside, nest, theta, phi):
+        t, p = f64(theta), f64(phi)
+        out = np.zeros(len(t), dtype=np.int64)
+        self.check(self.lib.opsec_healpix_pixels(self.h, nside, int(nest), len(t), _ptr(t), _ptr(p), _ptr(out)),
+                   "healpix_pixels")
+        return out
 
     def set_projection(self, method):
         """OPSEC_PROJ_DENSE or OPSEC_PROJ_TOEPLITZ (split-BF16x3 projections)."""

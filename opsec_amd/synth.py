@@ -112,3 +112,42 @@ def catalogue(n, lo, hi, seed=0, margin=0.05, weights="unit"):
     else:
         raise ValueError(weights)
     return g
+
+
+def write_healpix_fits(path, values, nest=False, coordsys="C", per_row=None):
+    """A HEALPix map as a FITS binary table (the maskfile of a SphericalSurvey):
+    an empty primary HDU, then a BINTABLE whose column 1 holds the 12 nside^2
+    float32 values, `per_row` to a row (default: all in rows of up to 1024),
+    with NSIDE, ORDERING and COORDSYS in its header."""
+    v = np.asarray(values, dtype=">f4").ravel()
+    npix = len(v)
+    nside = int(round(np.sqrt(npix / 12)))
+    assert 12 * nside * nside == npix, "length must be 12 nside^2"
+    if per_row is None:
+        per_row = min(npix, 1024)
+    assert npix % per_row == 0
+    nrows = npix // per_row
+
+    def card(key, val):
+        if isinstance(val, bool):
+            s = f"{key:<8}= {'T' if val else 'F':>20}"
+        elif isinstance(val, (int, np.integer)):
+            s = f"{key:<8}= {int(val):>20}"
+        else:
+            s = f"{key:<8}= '{str(val):<8}'"
+        return s.ljust(80).encode("ascii")
+
+    def header(cards):
+        b = b"".join(cards) + b"END".ljust(80)
+        return b + b" " * (-len(b) % 2880)
+
+    prim = header([card("SIMPLE", True), card("BITPIX", 8), card("NAXIS", 0), card("EXTEND", True)])
+    ext = header([card("XTENSION", "BINTABLE"), card("BITPIX", 8), card("NAXIS", 2), card("NAXIS1", 4 * per_row),
+                  card("NAXIS2", nrows), card("PCOUNT", 0), card("GCOUNT", 1), card("TFIELDS", 1),
+                  card("TTYPE1", "SIGNAL"), card("TFORM1", f"{per_row}E"), card("PIXTYPE", "HEALPIX"),
+                  card("ORDERING", "NESTED" if nest else "RING"), card("NSIDE", nside), card("COORDSYS", coordsys),
+                  card("FIRSTPIX", 0), card("LASTPIX", npix - 1)])
+    data = v.tobytes()
+    data += b"\0" * (-len(data) % 2880)
+    with open(path, "wb") as f:
+        f.write(prim + ext + data)

@@ -242,13 +242,16 @@ double orc_sph_r(const double* p1, const double* p2) { return sph_sep(p1[0], p1[
 /* Genz-Malik 7/5 rule and the adaptive driver                               */
 
 typedef struct {
+    int ndim;                  /* 6: a cell pair's xi; 3: a selection function over one cell */
     const orc_xi* f;
+    const orc_selection* sel;
     int coordsys;
     double L;
     double lo[6], hi[6];
 } integrand_t;
 
 static double integrand(const integrand_t* I, const double* u) {
+    if (I->ndim == 3) return orc_selection_eval(I->sel, I->coordsys, u);
     if (I->coordsys == 1) {
         double p1[3], p2[3];
         for (int d = 0; d < 3; d++) {
@@ -273,7 +276,7 @@ typedef struct {
  * (+-l2, +-l4 per axis), (+-l4,+-l4 pairs), (+-l5)^6 corners. */
 static void gm_rule(const integrand_t* I, region_t* R) {
     const double l2 = sqrt(9.0 / 70.0), l4 = sqrt(9.0 / 10.0), l5 = sqrt(9.0 / 19.0);
-    const int n = 6;
+    const int n = I->ndim;
     double u[6], v0;
     int i, j, g;
     memcpy(u, R->c, sizeof u);
@@ -305,7 +308,7 @@ static void gm_rule(const integrand_t* I, region_t* R) {
             u[i] = R->c[i] - ri; s4 += integrand(I, u);
             u[i] = R->c[i]; u[j] = R->c[j];
         }
-    for (g = 0; g < 64; g++) {
+    for (g = 0; g < (1 << n); g++) {
         int signs = g ^ (g >> 1);
         for (i = 0; i < n; i++) {
             double r5 = R->h[i] * l5;
@@ -316,7 +319,7 @@ static void gm_rule(const integrand_t* I, region_t* R) {
     double vol = 1;
     for (i = 0; i < n; i++) vol *= 2 * R->h[i];
     double w1 = (12824.0 - 9120 * n + 400 * n * n) / 19683, w2 = 980.0 / 6561, w3 = (1820.0 - 400 * n) / 19683;
-    double w4 = 200.0 / 19683, w5 = 6859.0 / 19683 / 64;
+    double w4 = 200.0 / 19683, w5 = 6859.0 / 19683 / (1 << n);
     double e1 = (729.0 - 950 * n + 50 * n * n) / 729, e2 = 245.0 / 486, e3 = (265.0 - 100 * n) / 1458, e4 = 25.0 / 729;
     double r7 = vol * (w1 * v0 + w2 * s2 + w3 * s3 + w4 * s4 + w5 * s5);
     double r5 = vol * (e1 * v0 + e2 * s2 + e3 * s3 + e4 * s4);
@@ -374,9 +377,15 @@ static int small_enough(double val, double err, double absTol, double relTol) {
     return err <= absTol || (val == 0 ? HUGE_VAL : fabs(err / val)) <= relTol;
 }
 
+/* The adaptive driver (src/cubature.c:885-998) from the first region R0. */
+static double gm_adapt(const integrand_t* I, region_t R0, double absTol, double relTol, unsigned maxeval,
+                       double* err, int64_t* neval);
+
 double orc_gm_integrate(const orc_xi* f, int coordsys, double L, const orc_cell* c1, const orc_cell* c2,
                         double epsrel, double epsabs, unsigned maxeval, double* err, int64_t* neval) {
     integrand_t I;
+    I.ndim = 6;
+    I.sel = NULL;
     I.f = f;
     I.coordsys = coordsys;
     I.L = L;
@@ -395,17 +404,23 @@ double orc_gm_integrate(const orc_xi* f, int coordsys, double L, const orc_cell*
         I.lo[4] = c1->min3; I.hi[4] = c1->max3;
         I.lo[5] = c2->min3; I.hi[5] = c2->max3;
     }
-    /* the reference passes (epsrel, epsabs) into (absolute, relative) */
-    const double absTol = epsrel, relTol = epsabs;
-    rheap H = {0};
     region_t R0;
     for (int d = 0; d < 6; d++) {
         R0.c[d] = 0.5;
         R0.h[d] = 0.5;
     }
-    gm_rule(&I, &R0);
+    /* the reference passes (epsrel, epsabs) into (absolute, relative) */
+    return gm_adapt(&I, R0, epsrel, epsabs, maxeval, err, neval);
+}
+
+static double gm_adapt(const integrand_t* I, region_t R0, double absTol, double relTol, unsigned maxeval,
+                       double* err, int64_t* neval) {
+    const int n = I->ndim;
+    const unsigned npts = 1 + 4 * n + 2 * n * (n - 1) + (1u << n);   /* 149 in 6-D, 33 in 3-D */
+    rheap H = {0};
+    gm_rule(I, &R0);
     rh_push(&H, R0);
-    uint64_t ne = 149;
+    uint64_t ne = npts;
     region_t* batch = NULL;
     int bcap = 0;
     while (ne < maxeval || !maxeval) {
@@ -426,10 +441,10 @@ double orc_gm_integrate(const orc_xi* f, int coordsys, double L, const orc_cell*
             b.c[d] = r.c[d] + a.h[d];
             batch[nb++] = a;
             batch[nb++] = b;
-            ne += 2 * 149;
+            ne += 2 * npts;
             if (small_enough(tot, rem, absTol, relTol)) break;
         } while (H.n > 0 && (ne < maxeval || !maxeval));
-        for (int t = 0; t < nb; t++) gm_rule(&I, &batch[t]);
+        for (int t = 0; t < nb; t++) gm_rule(I, &batch[t]);
         for (int t = 0; t < nb; t++) rh_push(&H, batch[t]);
     }
     double v = 0, e = 0;
@@ -1143,4 +1158,210 @@ void orc_pixel_values(const orc_cell* cells, int ncells, int nmodes, const doubl
         for (int a = 0; a < ncells; a++) s += b[a] * x[a];
         y[i] = s;
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Basis cells: selection function and FinalizeCell (src/basis.cpp:89-190)   */
+
+/* HEALPix (Gorski et al. 2005, ApJ 622, 759): z = cos(theta), t = phi/(pi/2)
+ * in [0, 4).  In |z| <= 2/3 the pixel edges are the lines
+ * n (1/2 + t) -+ (3/4) n z; in the caps the same lines on sqrt(3 (1 - |z|)). */
+static double hp_t(double phi) {
+    double p = fmod(phi, 2 * M_PI);
+    if (p < 0) p += 2 * M_PI;
+    double t = p / (M_PI / 2);
+    return t >= 4 ? 0 : t;
+}
+
+int64_t orc_ang2pix_ring(int64_t n, double theta, double phi) {
+    double z = cos(theta), za = fabs(z), t = hp_t(phi);
+    if (za <= 2.0 / 3.0) {
+        double up = n * (0.5 + t) - n * z * 0.75, dn = n * (0.5 + t) + n * z * 0.75;
+        int64_t jp = (int64_t)up, jm = (int64_t)dn;
+        int64_t ring = n + 1 + jp - jm;               /* 1 .. 2n+1 below z = 2/3 */
+        int64_t odd = ring & 1;
+        int64_t j = ((jp + jm - n + (1 - odd) + 1 + 8 * n) / 2) % (4 * n);
+        return 2 * n * (n - 1) + (ring - 1) * 4 * n + j;
+    }
+    double frac = t - floor(t), w = n * sqrt(3 * (1 - za));
+    int64_t jp = (int64_t)(frac * w), jm = (int64_t)((1 - frac) * w);
+    int64_t ring = jp + jm + 1;
+    int64_t j = ((int64_t)(t * ring)) % (4 * ring);
+    if (z > 0) return 2 * ring * (ring - 1) + j;
+    return 12 * n * n - 2 * ring * (ring + 1) + j;
+}
+
+static uint64_t interleave0(uint64_t v) {
+    uint64_t r = 0;
+    for (int b = 0; b < 32; b++) r |= ((v >> b) & 1) << (2 * b);
+    return r;
+}
+
+static uint64_t deinterleave0(uint64_t v) {
+    uint64_t r = 0;
+    for (int b = 0; b < 32; b++) r |= ((v >> (2 * b)) & 1) << b;
+    return r;
+}
+
+int64_t orc_ang2pix_nest(int64_t n, double theta, double phi) {
+    double z = cos(theta), za = fabs(z), t = hp_t(phi);
+    int64_t face, x, y;
+    if (za <= 2.0 / 3.0) {
+        int64_t jp = (int64_t)(n * (0.5 + t) - n * z * 0.75), jm = (int64_t)(n * (0.5 + t) + n * z * 0.75);
+        int64_t fp = jp / n, fm = jm / n;
+        if (fp == fm) face = fp | 4;
+        else if (fp < fm) face = fp;
+        else face = fm + 8;
+        x = jm % n;
+        y = n - (jp % n) - 1;
+    } else {
+        int64_t q = (int64_t)t;
+        if (q > 3) q = 3;
+        double frac = t - q, w = n * sqrt(3 * (1 - za));
+        int64_t jp = (int64_t)(frac * w), jm = (int64_t)((1 - frac) * w);
+        if (jp > n - 1) jp = n - 1;
+        if (jm > n - 1) jm = n - 1;
+        if (z >= 0) {
+            face = q;
+            x = n - jm - 1;
+            y = n - jp - 1;
+        } else {
+            face = q + 8;
+            x = jp;
+            y = jm;
+        }
+    }
+    return face * n * n + (int64_t)(interleave0((uint64_t)x) | (interleave0((uint64_t)y) << 1));
+}
+
+static int64_t isqrt64(int64_t v) {
+    int64_t r = (int64_t)sqrt((double)v);
+    while (r * r > v) r--;
+    while ((r + 1) * (r + 1) <= v) r++;
+    return r;
+}
+
+void orc_pix2ang_ring(int64_t n, int64_t p, double* theta, double* phi) {
+    int64_t npix = 12 * n * n, ncap = 2 * n * (n - 1);
+    double z, ph;
+    if (p < ncap) {
+        int64_t i = (1 + isqrt64(1 + 2 * p)) / 2, j = p + 1 - 2 * i * (i - 1);
+        z = 1 - (double)(i * i) / (3.0 * n * n);
+        ph = (j - 0.5) * M_PI / (2.0 * i);
+    } else if (p < npix - ncap) {
+        int64_t q = p - ncap, i = q / (4 * n) + n, j = q % (4 * n) + 1;
+        double shift = ((i + n) & 1) ? 1.0 : 0.5;
+        z = (2 * n - i) * 2.0 / (3.0 * n);
+        ph = (j - shift) * M_PI / (2.0 * n);
+    } else {
+        int64_t q = npix - p, i = (1 + isqrt64(2 * q - 1)) / 2, j = 4 * i + 1 - (q - 2 * i * (i - 1));
+        z = -1 + (double)(i * i) / (3.0 * n * n);
+        ph = (j - 0.5) * M_PI / (2.0 * i);
+    }
+    *theta = acos(z);
+    *phi = ph;
+}
+
+void orc_pix2ang_nest(int64_t n, int64_t p, double* theta, double* phi) {
+    static const int ring_of_face[12] = {2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4};
+    static const int col_of_face[12] = {1, 3, 5, 7, 0, 2, 4, 6, 1, 3, 5, 7};
+    int64_t face = p / (n * n), q = p % (n * n);
+    int64_t x = (int64_t)deinterleave0((uint64_t)q), y = (int64_t)deinterleave0((uint64_t)q >> 1);
+    int64_t jr = ring_of_face[face] * n - x - y - 1, nr, shift;
+    double z;
+    if (jr < n) {
+        nr = jr;
+        z = 1 - (double)(nr * nr) / (3.0 * n * n);
+        shift = 0;
+    } else if (jr > 3 * n) {
+        nr = 4 * n - jr;
+        z = (double)(nr * nr) / (3.0 * n * n) - 1;
+        shift = 0;
+    } else {
+        nr = n;
+        z = (2 * n - jr) * 2.0 / (3.0 * n);
+        shift = (jr - n) & 1;
+    }
+    int64_t jp = (col_of_face[face] * nr + x - y + 1 + shift) / 2;
+    if (jp > 4 * n) jp -= 4 * n;
+    if (jp < 1) jp += 4 * n;
+    *theta = acos(z);
+    *phi = (jp - (shift + 1) * 0.5) * (M_PI / 2) / nr;
+}
+
+/* src/BoxSurvey.cpp:59-75 (constant) and src/SphericalSurvey.cpp:65-82 with
+ * r = cbrt(u) (src/basis.cpp:67-79) */
+double orc_selection_eval(const orc_selection* s, int coordsys, const double* u) {
+    (void)coordsys;
+    if (s->kind == 0) return s->nbar;
+    double r = cbrt(u[0]), mu = u[1], ph = u[2];
+    double rad = linear_at(r, s->nrad, s->rr, s->rn);
+    double ang = 1.0;
+    if (s->nside > 0) {
+        double th = acos(mu);
+        int64_t ip = s->nest ? orc_ang2pix_nest(s->nside, th, ph) : orc_ang2pix_ring(s->nside, th, ph);
+        ang = s->mask[ip];
+    }
+    return ang * rad;
+}
+
+static double cube3(double x) { return x * x * x; }
+
+double orc_cell_integral(int coordsys, const orc_selection* s, const orc_cell* c, double epsrel, double epsabs,
+                         unsigned maxeval, int64_t* neval) {
+    integrand_t I;
+    memset(&I, 0, sizeof I);
+    I.ndim = 3;
+    I.sel = s;
+    I.coordsys = coordsys;
+    double lo[3] = {c->min1, c->min2, c->min3}, hi[3] = {c->max1, c->max2, c->max3};
+    if (coordsys == 2) {
+        lo[0] = cube3(c->min1);
+        hi[0] = cube3(c->max1);
+    }
+    region_t R0;
+    memset(&R0, 0, sizeof R0);
+    for (int d = 0; d < 3; d++) {
+        R0.c[d] = 0.5 * (lo[d] + hi[d]);   /* src/cubature.c:165-177 */
+        R0.h[d] = 0.5 * (hi[d] - lo[d]);
+    }
+    return gm_adapt(&I, R0, epsrel, epsabs, maxeval, NULL, neval);
+}
+
+int orc_basis(int coordsys, const double* g, const int* n, const orc_selection* s, double epsrel, double epsabs,
+              orc_cell* out, int64_t* nevals) {
+    const int64_t ng = (int64_t)n[0] * n[1] * n[2];
+    orc_cell* all = malloc(sizeof(orc_cell) * (size_t)ng);
+    int64_t* ne = malloc(sizeof(int64_t) * (size_t)ng);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t i = 0; i < ng; i++) {
+        int d1 = (int)(i / ((int64_t)n[1] * n[2])), d2 = (int)((i / n[2]) % n[1]), d3 = (int)(i % n[2]);
+        orc_cell c;
+        c.a = -1;
+        c.d1 = d1;
+        c.d2 = d2;
+        c.d3 = d3;
+        c.min1 = g[0] + d1 * (g[1] - g[0]) / n[0];
+        c.max1 = g[0] + (d1 + 1) * (g[1] - g[0]) / n[0];
+        c.min2 = g[2] + d2 * (g[3] - g[2]) / n[1];
+        c.max2 = g[2] + (d2 + 1) * (g[3] - g[2]) / n[1];
+        c.min3 = g[4] + d3 * (g[5] - g[4]) / n[2];
+        c.max3 = g[4] + (d3 + 1) * (g[5] - g[4]) / n[2];
+        c.Veff = coordsys == 2 ? (cube3(c.max1) - cube3(c.min1)) / 3 * (c.max2 - c.min2) * (c.max3 - c.min3)
+                               : (c.max1 - c.min1) * (c.max2 - c.min2) * (c.max3 - c.min3);
+        double N = orc_cell_integral(coordsys, s, &c, epsrel, epsabs, 5000000u, &ne[i]);
+        c.Nbar = coordsys == 2 ? N / 3 : N;
+        all[i] = c;
+    }
+    int kept = 0;
+    for (int64_t i = 0; i < ng; i++)
+        if (all[i].Nbar > 0) {
+            out[kept] = all[i];
+            out[kept].a = kept;
+            if (nevals) nevals[kept] = ne[i];
+            kept++;
+        }
+    free(all);
+    free(ne);
+    return kept;
 }

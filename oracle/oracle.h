@@ -34,6 +34,16 @@
  *                                      independent numpy histogram
  *   orc_pixel_values                   x against numpy IEEE arithmetic; y against
  *                                      numpy's BLAS product B @ x
+ *   orc_cell_integral                  oracle/_ref (the reference's adapt_integrate_v in
+ *                                      3-D with its LinearSpline, src/basis.cpp:89-128);
+ *                                      uniform nbar = nbar V; polynomial radial profiles
+ *                                      in closed form
+ *   orc_ang2pix_ring / _nest,          pix2ang(ang2pix) identities, NESTED hierarchy
+ *   orc_pix2ang_ring / _nest           (pixel >> 2 at nside / 2), RING and NESTED give the
+ *                                      same pixel centre, equal pixel areas, the known
+ *                                      base-pixel centres
+ *   orc_basis                          cell bounds and volumes against Survey::CreateEmptyCell
+ *                                      formulas; box Nbar = nbar V
  */
 #ifndef OPSEC_ORACLE_H
 #define OPSEC_ORACLE_H
@@ -126,6 +136,39 @@ void orc_sym_eigen(int n, const double* A, double* evals, double* V);
 int orc_mlp_nweights(void);
 void orc_mlp_forward(const double* w, int n, const double* x, double* out);
 double orc_mlp_loss_grad(const double* w, int n, const double* x, const double* t, double* grad);
+
+/* ---- basis cells (basis) ---- */
+/* Selection function: kind 0 uniform nbar; kind 1 radial linear spline (rr, rn;
+ * zero outside [rr[0], rr[nrad-1])) times a HEALPix completeness mask
+ * (nside 0: none) at (acos(mu), phi) (src/SphericalSurvey.cpp:65-82). */
+typedef struct orc_selection {
+    int kind, nest;
+    int64_t nside;
+    double nbar;
+    const double* rr;
+    const double* rn;
+    int nrad;
+    const float* mask;
+} orc_selection;
+/* value at u = (x, y, z) (coordsys 1) or (r^3, mu, phi) (coordsys 2) */
+double orc_selection_eval(const orc_selection* s, int coordsys, const double* u);
+/* FinalizeCell (src/basis.cpp:89-128): 3-D Genz-Malik adaptive cubature of the
+ * selection over the cell (r^3 for spherical cells), the reference's swapped
+ * tolerances; returns the reference's integral (spherical: 3x the count). */
+double orc_cell_integral(int coordsys, const orc_selection* s, const orc_cell* c, double epsrel, double epsabs,
+                         unsigned maxeval, int64_t* neval);
+/* All grid cells (src/basis.cpp:170-190) with Veff and Nbar (spherical: the
+ * integral / 3); cells with Nbar <= 0 dropped.  geom = {min1, max1, min2,
+ * max2, min3, max3}, n = {N1, N2, N3}; out holds N1 N2 N3 records, nevals
+ * (optional) the integrand evaluations of each kept cell; returns the number
+ * kept. */
+int orc_basis(int coordsys, const double* geom, const int* n, const orc_selection* s, double epsrel, double epsabs,
+              orc_cell* out, int64_t* nevals);
+/* HEALPix (Gorski et al. 2005): pixel of a direction and centre of a pixel */
+int64_t orc_ang2pix_ring(int64_t nside, double theta, double phi);
+int64_t orc_ang2pix_nest(int64_t nside, double theta, double phi);
+void orc_pix2ang_ring(int64_t nside, int64_t pix, double* theta, double* phi);
+void orc_pix2ang_nest(int64_t nside, int64_t pix, double* theta, double* phi);
 
 /* ---- pixel values (dot) ---- */
 /* geom = {min1, max1, min2, max2, min3, max3}, n = {N1, N2, N3}; -1 outside */

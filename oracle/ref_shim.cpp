@@ -18,6 +18,7 @@
  */
 
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -35,6 +36,7 @@
 #include "XiFunc.h"
 #include "abn.h"
 #include "cfg.h"
+#include "cubature.h"
 #include "sig.h"
 
 #define REF_API extern "C" __attribute__((visibility("default")))
@@ -204,4 +206,38 @@ REF_API int ref_read_galaxies(const char* survey_cfgtext, float* out, int maxgal
     int n = (int)gals.size();
     if (n <= maxgals && n > 0) memcpy(out, &gals[0], sizeof(Galaxy) * n);
     return n;
+}
+
+/* FinalizeCell's integral (src/basis.cpp:89-110) through the reference's own
+ * adapt_integrate_v, called with basis's arguments: maxeval 5e6 and
+ * (epsrel, epsabs) in its argument order.  The integrand restates basis's
+ * (src/basis.cpp:54-80): a constant nbar (kind 0) or, for spherical cells, the
+ * reference's LinearSpline of (rr, rn) at r = cbrt(u0) (kind 1, no angular
+ * mask: the HEALPix reader needs cfitsio).  lo/hi: the integration box
+ * ((x, y, z), or (r^3, mu, phi)).  Returns the reference's integral. */
+namespace {
+struct ShimSel {
+    int kind;
+    double nbar;
+    Spline radial;
+};
+void shim_sel_integrand(unsigned, unsigned npt, const double* uu, void* data, unsigned, double* fval) {
+    ShimSel* s = (ShimSel*)data;
+    for (unsigned i = 0; i < npt; i++) {
+        const double* u = &uu[3 * i];
+        fval[i] = s->kind == 0 ? s->nbar : 1.0 * s->radial(cbrt(u[0]));
+    }
+}
+}  // namespace
+
+REF_API double ref_cell_integral(int kind, double nbar, int nrad, const double* rr, const double* rn,
+                                 const double* lo, const double* hi, double epsrel, double epsabs, int64_t* neval) {
+    ShimSel s;
+    s.kind = kind;
+    s.nbar = nbar;
+    if (kind == 1) s.radial = LinearSpline(nrad, rr, rn);
+    double N = 0, err = 0;
+    adapt_integrate_v(1, shim_sel_integrand, &s, 3, lo, hi, 5000000, epsrel, epsabs, &N, &err);
+    if (neval) *neval = 0;
+    return N;
 }

@@ -21,6 +21,11 @@ class OrcXi(C.Structure):
     _fields_ = [("kind", C.c_int), ("r768", _P), ("y768", _P), ("a", _D), ("b", _D)]
 
 
+class OrcSelection(C.Structure):
+    _fields_ = [("kind", C.c_int), ("nest", C.c_int), ("nside", C.c_int64), ("nbar", _D), ("rr", _P), ("rn", _P),
+                ("nrad", C.c_int), ("mask", _P)]
+
+
 _ORC = {
     "orc_fmm_spline": ([_I, _P, _P, _P, _P, _P], None),
     "orc_cubic_eval": ([_I, _P, _P, _P, _P, _P, _I, _P, _P], None),
@@ -55,6 +60,13 @@ _ORC = {
     "orc_grid_index": ([_P, _P, _D, _D, _D], C.c_int64),
     "orc_count_galaxies": ([_P, _I, _P, _P, _P, C.c_int64, _P], None),
     "orc_pixel_values": ([_P, _I, _I, _P, _P, _P, _P], None),
+    "orc_selection_eval": ([C.POINTER(OrcSelection), _I, _P], _D),
+    "orc_cell_integral": ([_I, C.POINTER(OrcSelection), _P, _D, _D, C.c_uint, C.POINTER(C.c_int64)], _D),
+    "orc_basis": ([_I, _P, _P, C.POINTER(OrcSelection), _D, _D, _P, _P], _I),
+    "orc_ang2pix_ring": ([C.c_int64, _D, _D], C.c_int64),
+    "orc_ang2pix_nest": ([C.c_int64, _D, _D], C.c_int64),
+    "orc_pix2ang_ring": ([C.c_int64, C.c_int64, C.POINTER(_D), C.POINTER(_D)], None),
+    "orc_pix2ang_nest": ([C.c_int64, C.c_int64, C.POINTER(_D), C.POINTER(_D)], None),
 }
 
 _REF = {
@@ -72,6 +84,7 @@ _REF = {
     "ref_sizeof_cell": ([], _I),
     "ref_grid_index": ([C.c_char_p, _I, _P, _P], None),
     "ref_read_galaxies": ([C.c_char_p, _P, _I], _I),
+    "ref_cell_integral": ([_I, _D, _I, _P, _P, _P, _P, _D, _D, C.POINTER(C.c_int64)], _D),
 }
 
 
@@ -189,3 +202,26 @@ def pixel_values(cells, B, counts):
     y = np.zeros(B.shape[0])
     oracle().orc_pixel_values(ptr(cells), len(cells), B.shape[0], ptr(B), ptr(c), ptr(x), ptr(y))
     return x, y
+
+
+class Selection:
+    """An oracle selection function; keeps its arrays alive."""
+
+    def __init__(self, kind=0, nbar=0.0, rr=None, rn=None, mask=None, nside=0, nest=0):
+        self.rr = None if rr is None else np.ascontiguousarray(rr, dtype=np.float64)
+        self.rn = None if rn is None else np.ascontiguousarray(rn, dtype=np.float64)
+        self.mask = None if mask is None else np.ascontiguousarray(mask, dtype=np.float32)
+        self.s = OrcSelection(kind, nest, nside, nbar, ptr(self.rr), ptr(self.rn),
+                              0 if self.rr is None else len(self.rr), ptr(self.mask))
+
+
+def basis(coordsys, sv, sel, epsrel=1e-5, epsabs=1e-10, with_evals=False):
+    """Oracle basis cells of survey `sv` (an opsec_survey) with selection
+    `sel`; with_evals: also the integrand evaluations of each cell."""
+    from opsec_amd.lib import CELL_DTYPE
+    g = np.array([sv.min1, sv.max1, sv.min2, sv.max2, sv.min3, sv.max3])
+    n = np.array([sv.n1, sv.n2, sv.n3], dtype=np.int32)
+    out = np.zeros(int(n.prod()), dtype=CELL_DTYPE)
+    ne = np.zeros(int(n.prod()), dtype=np.int64)
+    k = oracle().orc_basis(coordsys, ptr(g), ptr(n), C.byref(sel.s), epsrel, epsabs, out.ctypes.data, ptr(ne))
+    return (out[:k].copy(), ne[:k].copy()) if with_evals else out[:k].copy()

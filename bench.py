@@ -218,12 +218,91 @@ def cpu_baseline(cfg, cells, k, P, lo, hi, B, budget_s=20.0):
     }
 
 
+def spherical_main(args):
+    """The spherical-shell workload: r in [200, 1000] Mpc/h over the 60-degree cap
+    mu >= 0.5, Nr x Nmu x Nphi = (64, 32, 32) / scale cells, nbar(r) = 5e-4 exp(-(r/600)^2)
+    as a radial table, 50 log-spaced bands over [0.003, 0.3] h/Mpc, 6000 / scale^3 KL modes.
+    Setup (untimed): basis cells on the GPU, the prior's arc-pair tables and klt.  One step:
+    arc-pair tables of the prior and all bands (adaptive Genz-Malik pairs; spherical pairs
+    have no surrogate), the projection and the estimate -- FP64, and split-BF16x3 + FP32
+    traces."""
+    import ctypes as Cc
+    from opsec_amd import lib as L
+    from opsec_amd import synth
+    s = args.scale
+    nr, nmu, nphi = 64 // s, 32 // s, 32 // s
+    nb, M = 50, max(16, 6000 // s ** 3)
+    ctx = L.Context(0)
+    ncu, arch = ctx.device_info()
+    sv = L.survey_sph(nr, nmu, nphi, 200.0, 1000.0, 0.5, 1.0, 0.0, 2 * np.pi)
+    rr = np.linspace(150.0, 1100.0, 1901)
+    rn = 5e-4 * np.exp(-(rr / 600.0) ** 2)
+    t0 = time.perf_counter()
+    cells, nev = ctx.basis(sv, L.OPSEC_SEL_RADIAL, rad_r=rr, rad_n=rn)
+    t_basis = time.perf_counter() - t0
+    N = len(cells)
+    log(f"device {arch} with {ncu} CUs; spherical shell {nr} x {nmu} x {nphi} = {N} cells, {nb} bands, {M} modes; "
+        f"basis {t_basis:.2f} s ({nev:.3g} integrand evaluations)")
+    edges = np.logspace(np.log10(0.003), np.log10(0.3), nb + 1)
+    lo, hi = edges[:-1], edges[1:]
+    k, P = synth.pk_table()
+    ctx.model_real(k, P, lo, hi, L.OPSEC_XI_CLOSED)
+    ctx.cells_upload(cells, sv)
+    q = ctx.quad(L.OPSEC_QUAD_GM)
+    funcs = list(range(nb + 1))
+    t0 = time.perf_counter()
+    ctx.pair_tables([0], q)
+    t_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    # the shell's spectrum converges more slowly than the box's: a Krylov space of 6 M
+    lam, B, nconv = ctx.klt(M, tol=1e-8, maxdim=min(N, 6 * M + 512))
+    t_klt = time.perf_counter() - t0
+    log(f"setup: prior arc-pair table {t_s:.2f} s, klt {M} modes {t_klt:.1f} s (nconv {nconv})")
+    ctx.modes_upload(B)
+    prior = ctx.params()
+    y = np.sqrt(1 + lam) * np.random.Generator(np.random.Philox(42)).normal(size=M)
+    r64 = run_path(ctx, L, funcs, q, L.OPSEC_PREC_FP64, L.OPSEC_PREC_FP64, lam, y, prior, args.steps, args.warmup,
+                   "spherical fp64")
+    st64 = ctx.pair_stats()
+    rmx = run_path(ctx, L, funcs, q, L.OPSEC_PREC_BF16X3, L.OPSEC_PREC_FP32, lam, y, prior, args.steps, args.warmup,
+                   "spherical mixed")
+    pairs = float(N) * N * (nb + 1)
+    qm = float(rmx[2][6])
+    out = {
+        "metric": METRIC, "value": pairs / (rmx[0] / 1e3), "unit": "cell-pair integrals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": rmx[0], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3+fp32", "data": "synthetic",
+        "config": {"workload": (f"C5/{s}: spherical shell r 200-1000 Mpc/h, mu >= 0.5 (60 deg cap), {nr} x {nmu} x "
+                                f"{nphi} = {N} cells, nbar(r) = 5e-4 exp(-(r/600)^2), {nb} log bands "
+                                f"[0.003, 0.3] h/Mpc, {M} KL modes, adaptive Genz-Malik pairs"),
+                   "path": "mixed", "projection": "dense"},
+        "fp64_path": {"ms_per_step": r64[0], "value": pairs / (r64[0] / 1e3),
+                      "projection_ms": [float(r64[2][0]), float(r64[2][1])]},
+        "speedup_vs_fp64": r64[0] / rmx[0],
+        "quadrature": {"ms_per_step": qm, "integrals": int(st64["integrals"]), "node_evaluations": int(st64["nodes"]),
+                       "integrals_per_s": st64["integrals"] / (qm / 1e3),
+                       "node_evaluations_per_s": st64["nodes"] / (qm / 1e3)},
+        "basis": {"s": t_basis, "cells": N, "integrand_evaluations": int(nev), "evaluations_per_s": nev / t_basis},
+        "klt": {"s": t_klt, "nconv": int(nconv), "prior_table_s": t_s},
+    }
+    out.update(compare_estimates(r64[1], rmx[1]))
+    s1 = float(rmx[2][3]) / max(1.0, float(rmx[2][5]))
+    out["roofline"] = {"bound": "mfma", "achieved": 2.0 * N * N * M / (s1 / 1e3) / 1e12, "peak": PEAK_TF["bf16"] / 3,
+                       "unit": "TFLOP/s", "traffic": None, "launch_ms": s1,
+                       "kernel": "gemm3_kernel<ASrcPlanes,EpiPlanes> (stage 1, S_p planes, one launch per band)"}
+    out["roofline"]["frac"] = out["roofline"]["achieved"] / out["roofline"]["peak"]
+    ctx.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="C2", choices=["C1", "C2"])
+    ap.add_argument("--config", default="C2", choices=["C1", "C2", "C5"])
+    ap.add_argument("--scale", type=int, default=2, choices=[1, 2, 4],
+                    help="C5 only: grid divisor per dimension (1 = the full 65536-cell shell)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--path", default="both", choices=["both", "fp64", "surrogate"])
     ap.add_argument("--mlp-samples", type=int, default=200000, help="surrogate training pairs (setup)")
@@ -250,6 +329,9 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU band sharding "
               "is a design, not part of this build)", file=sys.stderr)
         sys.exit(2)
+    if args.config == "C5":
+        spherical_main(args)
+        return
 
     from opsec_amd import lib as L
     from opsec_amd import synth

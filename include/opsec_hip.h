@@ -150,7 +150,10 @@ int opsec_pair_integral(opsec_ctx* ctx, const opsec_cell* c1, const opsec_cell* 
 int opsec_cells_upload(opsec_ctx* ctx, const opsec_cell* cells, int ncells,
                        const opsec_survey* sv);
 /* K1: pair-integral tables Q_f[key] for functions funcs[0..nf-1] on the device
- * survey (box: key = (|dx| Ny + |dy|) Nz + |dz|, src/sig.cpp:109-112). */
+ * survey.  Box: key = (|dx| Ny + |dy|) Nz + |dz| (src/sig.cpp:109-112), every
+ * method.  Spherical: the arc cache of src/sig.cpp:150-190 for the block of all
+ * rows and columns, key = (h_a narcs + h_b) Nphi + |dphi_a - dphi_b| with arcs
+ * h = d_r Nmu + d_mu, narcs = Nr Nmu; adaptive (GM) rule only. */
 int opsec_pair_tables(opsec_ctx* ctx, int nf, const int* funcs, const opsec_quad* q);
 /* Copy table of function f to the host (nkeys doubles); nkeys via opsec_pair_nkeys. */
 int opsec_pair_nkeys(opsec_ctx* ctx, int* nkeys);
@@ -293,8 +296,9 @@ typedef struct opsec_estimate {
 
 /* estimate (src/estimate-mpi.cpp:480-654) from host inputs: lambda (M S/N
  * eigenvalues), D (nparams packed matrices, covfile layout), y (M pixel
- * values), prior (nparams p_n).  prec selects FP64 or FP32-accumulated traces.
- * Outputs are host buffers; any pointer in *out may be NULL. */
+ * values), prior (nparams p_n, 1 <= nparams <= 62).  prec selects FP64 or
+ * FP32-accumulated traces.  Outputs are host buffers; any pointer in *out may
+ * be NULL. */
 int opsec_estimate_run(opsec_ctx* ctx, int nmodes, int nparams, const double* lambda,
                        const double* D, const double* y, const double* prior, int mixing,
                        int prec, opsec_estimate* out);

@@ -7,10 +7,11 @@
 //   f_n  = sum_k w_k D_n[k] R[k],      R[k] = delta_ij C_i - sum_n p_n D_n[k]
 //   q_n  = sum_k w_k D_n[k] y_i y_j
 // i.e. one Gram product G = X D^T with X = [w D_0 .. w D_{nb-1}, w R, w Y]
-// (nb + 2 rows, padded to 48) and D^T (padded to 48 columns), run on MFMA
-// (FP64 16x16x4, or FP32 16x16x4 for the FP32 traces) with workgroup partial
-// sums reduced by atomics.  The 48 x 48 result is tiny; the inverse / matrix
-// square root / window normalisation on nb x nb runs on the host in FP64.
+// (nb + 2 rows, padded to 48, or to 64 for nb > 46) and D^T (padded likewise),
+// run on MFMA (FP64 16x16x4, or FP32 16x16x4 for the FP32 traces) with
+// workgroup partial sums reduced by atomics.  The 48 x 48 (64 x 64) result is
+// tiny; the inverse / matrix square root / window normalisation on nb x nb
+// runs on the host in FP64.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -21,8 +22,15 @@
 
 namespace opsec {
 
-constexpr int FK_TILE = 128;   // packed entries per workgroup iteration
-constexpr int FG = 48;         // padded Gram size
+// NF 16-row fragments per side: the padded Gram size is 16 NF (48 or 64).
+// Packed entries per workgroup iteration: 128 for 48, 48 for 64 (LDS stays
+// below 64 KB in FP64).
+template <int NF>
+struct FShape {
+    static constexpr int FG = 16 * NF;
+    static constexpr int KT = NF <= 3 ? 128 : 48;
+};
+constexpr int FG_MAX = 64;
 
 template <typename T>
 struct FMfma;
@@ -45,21 +53,22 @@ __device__ __forceinline__ int packed_col(long long k) {
     return (int)j;
 }
 
-template <typename T>
+template <typename T, int NF>
 __global__ __launch_bounds__(256) void fisher_gram_kernel(const double* D, long long K, int nb, const double* lambda,
                                                           const double* y, const double* prior, double* G) {
     typedef typename FMfma<T>::acc_t acc_t;
-    // FP32 MFMA partial sums are closed at the end of every 128-entry tile and
-    // carried in FP64 registers, so each FP32 accumulation chain is 32 terms
+    constexpr int FG = FShape<NF>::FG, FK_TILE = FShape<NF>::KT;
+    // FP32 MFMA partial sums are closed at the end of every tile and carried
+    // in FP64 registers, so each FP32 accumulation chain is FK_TILE / 4 terms
     constexpr bool fp32 = sizeof(T) == 4;
     __shared__ T X[FG][FK_TILE + 4];     // scaled rows
     __shared__ T Dt[FG][FK_TILE + 4];    // plain rows
     __shared__ double wk[FK_TILE], rk[FK_TILE], yk[FK_TILE];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    acc_t acc[3][3];
-    double acc64[3][3][4];
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) {
+    acc_t acc[NF][NF];
+    double acc64[NF][NF][4];
+    for (int i = 0; i < NF; i++)
+        for (int j = 0; j < NF; j++) {
             acc[i][j] = acc_t{0, 0, 0, 0};
             for (int r = 0; r < 4; r++) acc64[i][j][r] = 0.0;
         }
@@ -112,25 +121,25 @@ __global__ __launch_bounds__(256) void fisher_gram_kernel(const double* D, long 
             X[row][t] = (T)v;
         }
         __syncthreads();
-        // each wave takes 32 entries = 8 k-steps of 4
+        // each wave takes FK_TILE / 4 entries = FK_TILE / 16 k-steps of 4
 #pragma unroll
-        for (int s = 0; s < 8; s++) {
-            const int kk = wave * 32 + s * 4 + (lane >> 4);
-            T a[3], b[3];
-            for (int i = 0; i < 3; i++) a[i] = X[i * 16 + (lane & 15)][kk];
-            for (int j = 0; j < 3; j++) b[j] = Dt[j * 16 + (lane & 15)][kk];
-            for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++) acc[i][j] = FMfma<T>::mma(a[i], b[j], acc[i][j]);
+        for (int s = 0; s < FK_TILE / 16; s++) {
+            const int kk = wave * (FK_TILE / 4) + s * 4 + (lane >> 4);
+            T a[NF], b[NF];
+            for (int i = 0; i < NF; i++) a[i] = X[i * 16 + (lane & 15)][kk];
+            for (int j = 0; j < NF; j++) b[j] = Dt[j * 16 + (lane & 15)][kk];
+            for (int i = 0; i < NF; i++)
+                for (int j = 0; j < NF; j++) acc[i][j] = FMfma<T>::mma(a[i], b[j], acc[i][j]);
         }
         if (fp32)
-            for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++) {
+            for (int i = 0; i < NF; i++)
+                for (int j = 0; j < NF; j++) {
                     for (int r = 0; r < 4; r++) acc64[i][j][r] += (double)acc[i][j][r];
                     acc[i][j] = acc_t{0, 0, 0, 0};
                 }
     }
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++)
+    for (int i = 0; i < NF; i++)
+        for (int j = 0; j < NF; j++)
             for (int r = 0; r < 4; r++) {
                 int row = i * 16 + acc_row<T>(lane, r), col = j * 16 + (lane & 15);
                 const double v = fp32 ? acc64[i][j][r] : (double)acc[i][j][r];
@@ -221,7 +230,8 @@ static void sym_sqrtm(int n, const std::vector<double>& A0, std::vector<double>&
         }
 }
 
-static int post_process(opsec_ctx* ctx, int nb, const std::vector<double>& G, int mixing, opsec_estimate* out) {
+static int post_process(opsec_ctx* ctx, int nb, const std::vector<double>& G, int FG, int mixing,
+                        opsec_estimate* out) {
     std::vector<double> F((size_t)nb * nb), f(nb), q(nb);
     for (int m = 0; m < nb; m++)
         for (int n = 0; n < nb; n++) F[m * nb + n] = 0.5 * (G[m * FG + n] + G[n * FG + m]);
@@ -277,35 +287,50 @@ static int post_process(opsec_ctx* ctx, int nb, const std::vector<double>& G, in
 
 static int run_fisher(opsec_ctx* ctx, const double* dD, int M, int nb, const double* lambda, const double* y,
                       const double* prior, int mixing, int prec, opsec_estimate* out) {
-    OPSEC_CHECK_ARG(ctx, nb >= 1 && nb + 2 <= FG, "Nparams must be 1..%d", FG - 2);
+    OPSEC_CHECK_ARG(ctx, nb >= 1 && nb + 2 <= FG_MAX, "Nparams must be 1..%d", FG_MAX - 2);
     OPSEC_CHECK_ARG(ctx, M >= 1, "Nmodes must be positive");
     OPSEC_CHECK_ARG(ctx, mixing >= 0 && mixing <= 2, "unknown mixing mode %d", mixing);
     OPSEC_CHECK_ARG(ctx, prec == OPSEC_PREC_FP64 || prec == OPSEC_PREC_FP32 || prec == OPSEC_PREC_BF16X3,
                     "unknown precision %d", prec);
     OPSEC_CHECK_ARG(ctx, lambda && y && prior && out, "NULL input");
     long long K = (long long)M * (M + 1) / 2;
-    double *d_aux = nullptr, *d_G = nullptr;
-    OPSEC_HIP_TRY(ctx, hipMalloc(&d_aux, ((size_t)2 * M + nb) * 8));
-    OPSEC_HIP_TRY(ctx, hipMalloc(&d_G, FG * FG * 8));
+    const bool big = nb + 2 > FShape<3>::FG;
+    const int FG = big ? FShape<4>::FG : FShape<3>::FG, KT = big ? FShape<4>::KT : FShape<3>::KT;
+    struct DevBufs {
+        double* aux = nullptr;
+        double* G = nullptr;
+        ~DevBufs() {
+            if (aux) (void)hipFree(aux);
+            if (G) (void)hipFree(G);
+        }
+    } b;
+    OPSEC_HIP_TRY(ctx, hipMalloc(&b.aux, ((size_t)2 * M + nb) * 8));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&b.G, (size_t)FG * FG * 8));
+    double* d_aux = b.aux;
+    double* d_G = b.G;
     OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_aux, lambda, M * 8, hipMemcpyHostToDevice, ctx->stream));
     OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_aux + M, y, M * 8, hipMemcpyHostToDevice, ctx->stream));
     OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_aux + 2 * M, prior, nb * 8, hipMemcpyHostToDevice, ctx->stream));
-    OPSEC_HIP_TRY(ctx, hipMemsetAsync(d_G, 0, FG * FG * 8, ctx->stream));
-    long long tiles = (K + FK_TILE - 1) / FK_TILE;
+    OPSEC_HIP_TRY(ctx, hipMemsetAsync(d_G, 0, (size_t)FG * FG * 8, ctx->stream));
+    long long tiles = (K + KT - 1) / KT;
     int grid = (int)std::min<long long>(tiles, (long long)ctx->num_cus * 8);
-    if (prec == OPSEC_PREC_FP64)
-        hipLaunchKernelGGL(fisher_gram_kernel<double>, dim3(grid), dim3(256), 0, ctx->stream, dD, K, nb, d_aux,
+    if (prec == OPSEC_PREC_FP64 && !big)
+        hipLaunchKernelGGL((fisher_gram_kernel<double, 3>), dim3(grid), dim3(256), 0, ctx->stream, dD, K, nb, d_aux,
+                           d_aux + M, d_aux + 2 * M, d_G);
+    else if (prec == OPSEC_PREC_FP64)
+        hipLaunchKernelGGL((fisher_gram_kernel<double, 4>), dim3(grid), dim3(256), 0, ctx->stream, dD, K, nb, d_aux,
+                           d_aux + M, d_aux + 2 * M, d_G);
+    else if (!big)
+        hipLaunchKernelGGL((fisher_gram_kernel<float, 3>), dim3(grid), dim3(256), 0, ctx->stream, dD, K, nb, d_aux,
                            d_aux + M, d_aux + 2 * M, d_G);
     else
-        hipLaunchKernelGGL(fisher_gram_kernel<float>, dim3(grid), dim3(256), 0, ctx->stream, dD, K, nb, d_aux,
+        hipLaunchKernelGGL((fisher_gram_kernel<float, 4>), dim3(grid), dim3(256), 0, ctx->stream, dD, K, nb, d_aux,
                            d_aux + M, d_aux + 2 * M, d_G);
     OPSEC_HIP_TRY(ctx, hipGetLastError());
-    std::vector<double> G(FG * FG);
-    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(G.data(), d_G, FG * FG * 8, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<double> G((size_t)FG * FG);
+    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(G.data(), d_G, (size_t)FG * FG * 8, hipMemcpyDeviceToHost, ctx->stream));
     OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    hipFree(d_aux);
-    hipFree(d_G);
-    return post_process(ctx, nb, G, mixing, out);
+    return post_process(ctx, nb, G, FG, mixing, out);
 }
 
 void* proj_device_results(opsec_ctx* ctx, int* nb, int* M);

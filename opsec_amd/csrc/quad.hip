@@ -149,56 +149,76 @@ __device__ __forceinline__ void gm_node(int p, const double* c, const double* hw
     }
 }
 
-// Evaluate region ri: all lanes compute nodes, lane 0 sums in node order.
-__device__ void gm_eval_region(const XiModelDev& m, const XiDesc& xd, const GMJob& J, const GMParams& P,
-                               GMSlot& S, int R, int ri, double* vals) {
-    const int lane = threadIdx.x;
-    double c[6], hw[6];
-    for (int d = 0; d < 6; d++) {
-        c[d] = S.c[d * R + ri];
-        hw[d] = S.hw[d * R + ri];
-    }
-    for (int p = lane; p < GM_NPTS; p += 64) {
-        double u[6];
-        gm_node(p, c, hw, u);
-        vals[p] = gm_integrand(m, xd, J, P, u);
-    }
-    __syncthreads();
-    if (lane == 0) {
-        const double ratio = (GM_L2 * GM_L2) / (GM_L4 * GM_L4);
-        const double w1 = (double)(12824 - 9120 * 6 + 400 * 36) / 19683.0;
-        const double w2 = 980. / 6561.;
-        const double w3 = (double)(1820 - 400 * 6) / 19683.0;
-        const double w4 = 200. / 19683.;
-        const double w5 = 6859.0 / 19683.0 / 64.0;
-        const double e1 = (double)(729 - 950 * 6 + 50 * 36) / 729.0;
-        const double e2 = 245. / 486.;
-        const double e3 = (double)(265 - 100 * 6) / 1458.0;
-        const double e4 = 25. / 729.;
-        double val0 = vals[0], sum2 = 0, sum3 = 0, sum4 = 0, sum5 = 0;
-        double maxdiff = 0;
-        int split = 0;
-        for (int k = 0; k < 6; k++) {
-            double v0 = vals[1 + 4 * k], v1 = vals[2 + 4 * k], v2 = vals[3 + 4 * k], v3 = vals[4 + 4 * k];
-            sum2 += v0 + v1;
-            sum3 += v2 + v3;
-            double diff = 0.0 + fabs(v0 + v1 - 2 * val0 - ratio * (v2 + v3 - 2 * val0));
-            if (diff > maxdiff) {
-                maxdiff = diff;
-                split = k;
-            }
+// The degree-7 value, the degree-5 error and the split axis of one region from
+// its 149 node values, summed in node order (src/cubature.c:445-613).
+__device__ __forceinline__ void gm_region_sum(const double* vals, const double* hw, double& result, double& err,
+                                              int& split) {
+    const double ratio = (GM_L2 * GM_L2) / (GM_L4 * GM_L4);
+    const double w1 = (double)(12824 - 9120 * 6 + 400 * 36) / 19683.0;
+    const double w2 = 980. / 6561.;
+    const double w3 = (double)(1820 - 400 * 6) / 19683.0;
+    const double w4 = 200. / 19683.;
+    const double w5 = 6859.0 / 19683.0 / 64.0;
+    const double e1 = (double)(729 - 950 * 6 + 50 * 36) / 729.0;
+    const double e2 = 245. / 486.;
+    const double e3 = (double)(265 - 100 * 6) / 1458.0;
+    const double e4 = 25. / 729.;
+    double val0 = vals[0], sum2 = 0, sum3 = 0, sum4 = 0, sum5 = 0;
+    double maxdiff = 0;
+    split = 0;
+    for (int k = 0; k < 6; k++) {
+        double v0 = vals[1 + 4 * k], v1 = vals[2 + 4 * k], v2 = vals[3 + 4 * k], v3 = vals[4 + 4 * k];
+        sum2 += v0 + v1;
+        sum3 += v2 + v3;
+        double diff = 0.0 + fabs(v0 + v1 - 2 * val0 - ratio * (v2 + v3 - 2 * val0));
+        if (diff > maxdiff) {
+            maxdiff = diff;
+            split = k;
         }
-        for (int k = 0; k < 60; k++) sum4 += vals[25 + k];
-        for (int k = 0; k < 64; k++) sum5 += vals[85 + k];
-        double vol = 1;
-        for (int d = 0; d < 6; d++) vol *= 2 * hw[d];
-        double result = vol * (w1 * val0 + w2 * sum2 + w3 * sum3 + w4 * sum4 + w5 * sum5);
-        double res5th = vol * (e1 * val0 + e2 * sum2 + e3 * sum3 + e4 * sum4);
-        S.val[ri] = result;
-        S.err[ri] = fabs(res5th - result);
-        S.split[ri] = split;
     }
-    __syncthreads();
+    for (int k = 0; k < 60; k++) sum4 += vals[25 + k];
+    for (int k = 0; k < 64; k++) sum5 += vals[85 + k];
+    double vol = 1;
+    for (int d = 0; d < 6; d++) vol *= 2 * hw[d];
+    result = vol * (w1 * val0 + w2 * sum2 + w3 * sum3 + w4 * sum4 + w5 * sum5);
+    const double res5th = vol * (e1 * val0 + e2 * sum2 + e3 * sum3 + e4 * sum4);
+    err = fabs(res5th - result);
+}
+
+constexpr int GM_SUB = 16;   // regions evaluated together by one wavefront
+
+// Evaluate the regions list[0..n): GM_SUB at a time, the lanes share out their
+// GM_SUB x 149 nodes, then lane t forms the rule of region t.
+__device__ void gm_eval_regions(const XiModelDev& m, const XiDesc& xd, const GMJob& J, const GMParams& P,
+                                GMSlot& S, int R, const int* list, int n, double (*vals)[GM_NPTS]) {
+    const int lane = threadIdx.x;
+    for (int t0 = 0; t0 < n; t0 += GM_SUB) {
+        const int nt = min(GM_SUB, n - t0);
+        for (int i = lane; i < nt * GM_NPTS; i += 64) {
+            const int t = i / GM_NPTS, p = i - t * GM_NPTS;
+            const int ri = list[t0 + t];
+            double c[6], hw[6], u[6];
+            for (int d = 0; d < 6; d++) {
+                c[d] = S.c[d * R + ri];
+                hw[d] = S.hw[d * R + ri];
+            }
+            gm_node(p, c, hw, u);
+            vals[t][p] = gm_integrand(m, xd, J, P, u);
+        }
+        __syncthreads();
+        if (lane < nt) {
+            const int ri = list[t0 + lane];
+            double hw[6];
+            for (int d = 0; d < 6; d++) hw[d] = S.hw[d * R + ri];
+            double result, err;
+            int split;
+            gm_region_sum(vals[lane], hw, result, err, split);
+            S.val[ri] = result;
+            S.err[ri] = err;
+            S.split[ri] = split;
+        }
+        __syncthreads();
+    }
 }
 
 // Binary max-heap on region errors (Cormen-Leiserson-Rivest), lane 0 only.
@@ -239,11 +259,13 @@ __device__ __forceinline__ bool gm_converged(double val, double err, const GMPar
     return err <= P.absTol || rel <= P.relTol;
 }
 
-__global__ __launch_bounds__(64) void gm_kernel(XiModelDev m, const GMJob* jobs, int njobs, int* counter,
-                                                double* dpool, int* ipool, GMParams P, double* Q, size_t qstride,
-                                                double* Qerr, int* jobfail, unsigned long long* nevals,
-                                                double* trace) {
-    __shared__ double vals[GM_NPTS];
+// One wavefront per integral; jobs idx[0..njobs) (all jobs when idx is null)
+// are taken from a counter.
+__global__ __launch_bounds__(64) void gm_kernel(XiModelDev m, const GMJob* jobs, const int* idx, int njobs,
+                                                int* counter, double* dpool, int* ipool, GMParams P, double* Q,
+                                                size_t qstride, double* Qerr, int* jobfail,
+                                                unsigned long long* nevals, double* trace) {
+    __shared__ double vals[GM_SUB][GM_NPTS];
     __shared__ int sh_job, sh_nR, sh_done;
     const int lane = threadIdx.x;
     const int R = P.rmax;
@@ -251,9 +273,10 @@ __global__ __launch_bounds__(64) void gm_kernel(XiModelDev m, const GMJob* jobs,
     for (;;) {
         if (lane == 0) sh_job = atomicAdd(counter, 1);
         __syncthreads();
-        int jb = sh_job;
+        const int jl = sh_job;
         __syncthreads();
-        if (jb >= njobs) break;
+        if (jl >= njobs) break;
+        const int jb = idx ? idx[jl] : jl;
         const GMJob J = jobs[jb];
         const XiDesc xd = m.desc[J.f];
         // region 0 = [0,1]^6
@@ -261,8 +284,9 @@ __global__ __launch_bounds__(64) void gm_kernel(XiModelDev m, const GMJob* jobs,
             S.c[lane * R] = 0.5;
             S.hw[lane * R] = 0.5;
         }
+        if (lane == 0) S.list[0] = 0;
         __syncthreads();
-        gm_eval_region(m, xd, J, P, S, R, 0, vals);
+        gm_eval_regions(m, xd, J, P, S, R, S.list, 1, vals);
         // lane-0 state (heap size, totals, evaluation count)
         int nheap = 0, nregions = 1;
         double tv = 0, te = 0;
@@ -271,7 +295,7 @@ __global__ __launch_bounds__(64) void gm_kernel(XiModelDev m, const GMJob* jobs,
         const bool tr = trace && jb == 0;
         int iter = 0, status = 0;
         if (tr && lane == 0) {
-            for (int p = 0; p < GM_NPTS; p++) trace[p] = vals[p];
+            for (int p = 0; p < GM_NPTS; p++) trace[p] = vals[0][p];
             trace[149] = S.val[0];
             trace[150] = S.err[0];
             trace[151] = S.split[0];
@@ -333,7 +357,7 @@ __global__ __launch_bounds__(64) void gm_kernel(XiModelDev m, const GMJob* jobs,
                 break;
             }
             __threadfence_block();
-            for (int t = 0; t < nR; t++) gm_eval_region(m, xd, J, P, S, R, S.list[t], vals);
+            gm_eval_regions(m, xd, J, P, S, R, S.list, nR, vals);
             if (lane == 0)
                 for (int t = 0; t < nR; t++) heap_push(S, nheap, S.list[t], tv, te);
         }
@@ -354,6 +378,97 @@ __global__ __launch_bounds__(64) void gm_kernel(XiModelDev m, const GMJob* jobs,
         }
         __syncthreads();
     }
+}
+
+// Job t * npairs + i = template i with function funcs[t].
+__global__ void expand_gm_jobs_kernel(const GMJob* __restrict__ tmpl, int npairs, const int* __restrict__ funcs,
+                                      int nf, GMJob* __restrict__ jobs) {
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (long long)npairs * nf) return;
+    GMJob J = tmpl[j % npairs];
+    J.f = funcs[j / npairs];
+    jobs[j] = J;
+}
+
+// Indices of the jobs (idx[0..n) or 0..n) whose region pool ran out.
+__global__ void list_failed_kernel(const int* __restrict__ fail, const int* __restrict__ idx, int n,
+                                   int* __restrict__ out, int* __restrict__ nout) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int j = idx ? idx[t] : t;
+    if (fail[j]) out[atomicAdd(nout, 1)] = j;
+}
+
+// First region of every job, one thread per job: the rule on [0,1]^6 with the
+// node values summed in the order of gm_region_sum.  A job whose first region
+// already meets the tolerance (the driver's test before any split) ends here
+// with the value gm_kernel would give; the others are listed in `pending` for
+// gm_kernel.  Most band integrals end here: one region, no heap, no atomics
+// per job.
+__global__ __launch_bounds__(256) void gm_first_kernel(XiModelDev m, const GMJob* jobs, int njobs, GMParams P,
+                                                       double* Q, size_t qstride, double* Qerr, int* jobfail,
+                                                       int* pending, int* npending, unsigned long long* nevals) {
+    __shared__ unsigned long long blk;
+    if (threadIdx.x == 0) blk = 0;
+    __syncthreads();
+    const int jb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (jb < njobs) {
+        const GMJob J = jobs[jb];
+        const XiDesc xd = m.desc[J.f];
+        double c[6], hw[6], u[6];
+        for (int d = 0; d < 6; d++) {
+            c[d] = 0.5;
+            hw[d] = 0.5;
+        }
+        const double ratio = (GM_L2 * GM_L2) / (GM_L4 * GM_L4);
+        const double w1 = (double)(12824 - 9120 * 6 + 400 * 36) / 19683.0;
+        const double w2 = 980. / 6561.;
+        const double w3 = (double)(1820 - 400 * 6) / 19683.0;
+        const double w4 = 200. / 19683.;
+        const double w5 = 6859.0 / 19683.0 / 64.0;
+        const double e1 = (double)(729 - 950 * 6 + 50 * 36) / 729.0;
+        const double e2 = 245. / 486.;
+        const double e3 = (double)(265 - 100 * 6) / 1458.0;
+        const double e4 = 25. / 729.;
+        gm_node(0, c, hw, u);
+        const double val0 = gm_integrand(m, xd, J, P, u);
+        double sum2 = 0, sum3 = 0, sum4 = 0, sum5 = 0;
+        for (int k = 0; k < 6; k++) {
+            double v[4];
+            for (int e = 0; e < 4; e++) {
+                gm_node(1 + 4 * k + e, c, hw, u);
+                v[e] = gm_integrand(m, xd, J, P, u);
+            }
+            sum2 += v[0] + v[1];
+            sum3 += v[2] + v[3];
+        }
+        for (int p = 25; p < 85; p++) {
+            gm_node(p, c, hw, u);
+            sum4 += gm_integrand(m, xd, J, P, u);
+        }
+        for (int p = 85; p < GM_NPTS; p++) {
+            gm_node(p, c, hw, u);
+            sum5 += gm_integrand(m, xd, J, P, u);
+        }
+        double vol = 1;
+        for (int d = 0; d < 6; d++) vol *= 2 * hw[d];
+        const double result = vol * (w1 * val0 + w2 * sum2 + w3 * sum3 + w4 * sum4 + w5 * sum5);
+        const double res5th = vol * (e1 * val0 + e2 * sum2 + e3 * sum3 + e4 * sum4);
+        const double err = fabs(res5th - result);
+        (void)ratio;
+        // the heap totals after the first push are 0 + val and 0 + err
+        const double tv = 0.0 + result, te = 0.0 + err;
+        if (!((unsigned)GM_NPTS < P.maxEval || !P.maxEval) || gm_converged(tv, te, P)) {
+            Q[(size_t)J.f * qstride + J.key] = tv;
+            if (Qerr) Qerr[(size_t)J.f * qstride + J.key] = te;
+            jobfail[jb] = 0;
+            atomicAdd(&blk, (unsigned long long)GM_NPTS);
+        } else {
+            pending[atomicAdd(npending, 1)] = jb;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && blk) atomicAdd(nevals, blk);
 }
 
 // ---- GAUSS: one thread per integral ----------------------------------------
@@ -578,51 +693,54 @@ static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::v
     if (npairs == 0 || nf == 0) return OPSEC_OK;
     int method = q->method;
     if (method == OPSEC_QUAD_GM) {
-        std::vector<GMJob> jobs;
-        jobs.reserve(npairs * nf);
-        // functions with the most adaptive work (the prior) first
-        for (int t = 0; t < nf; t++)
-            for (size_t i = 0; i < npairs; i++) {
-                GMJob J;
-                J.f = funcs[t];
-                J.key = keys[i];
-                const PairGeo& g = geo[i];
-                if (ctx->sv.coordsys == OPSEC_COORD_CARTESIAN) {
-                    for (int d = 0; d < 3; d++) {
-                        J.lo[d] = g.lo1[d];
-                        J.hi[d] = g.hi1[d];
-                        J.lo[3 + d] = g.lo2[d];
-                        J.hi[3 + d] = g.hi2[d];
-                    }
-                } else {
-                    J.lo[0] = g.lo1[0] * g.lo1[0] * g.lo1[0];
-                    J.hi[0] = g.hi1[0] * g.hi1[0] * g.hi1[0];
-                    J.lo[1] = g.lo2[0] * g.lo2[0] * g.lo2[0];
-                    J.hi[1] = g.hi2[0] * g.hi2[0] * g.hi2[0];
-                    J.lo[2] = g.lo1[1];
-                    J.hi[2] = g.hi1[1];
-                    J.lo[3] = g.lo2[1];
-                    J.hi[3] = g.hi2[1];
-                    J.lo[4] = g.lo1[2];
-                    J.hi[4] = g.hi1[2];
-                    J.lo[5] = g.lo2[2];
-                    J.hi[5] = g.hi2[2];
+        // one template per pair (f unset); the nf x npairs jobs, function-major
+        // (the prior, with the most adaptive work, first), are expanded on the
+        // device
+        std::vector<GMJob> tmpl(npairs);
+        for (size_t i = 0; i < npairs; i++) {
+            GMJob& J = tmpl[i];
+            J.f = -1;
+            J.key = keys[i];
+            const PairGeo& g = geo[i];
+            if (ctx->sv.coordsys == OPSEC_COORD_CARTESIAN) {
+                for (int d = 0; d < 3; d++) {
+                    J.lo[d] = g.lo1[d];
+                    J.hi[d] = g.hi1[d];
+                    J.lo[3 + d] = g.lo2[d];
+                    J.hi[3 + d] = g.hi2[d];
                 }
-                jobs.push_back(J);
+            } else {
+                J.lo[0] = g.lo1[0] * g.lo1[0] * g.lo1[0];
+                J.hi[0] = g.hi1[0] * g.hi1[0] * g.hi1[0];
+                J.lo[1] = g.lo2[0] * g.lo2[0] * g.lo2[0];
+                J.hi[1] = g.hi2[0] * g.hi2[0] * g.hi2[0];
+                J.lo[2] = g.lo1[1];
+                J.hi[2] = g.hi1[1];
+                J.lo[3] = g.lo2[1];
+                J.hi[3] = g.hi2[1];
+                J.lo[4] = g.lo1[2];
+                J.hi[4] = g.hi1[2];
+                J.lo[5] = g.lo2[2];
+                J.hi[5] = g.hi2[2];
             }
+        }
         GMParams P;
         P.absTol = q->epsrel;   // reference argument order, src/sig.cpp:427-430
         P.relTol = q->epsabs;
         P.maxEval = (unsigned)q->maxeval;
         P.coordsys = ctx->sv.coordsys;
         P.L = ctx->sv.L;
-        // Pass 1: every job with a 4096-region pool per wavefront slot.  Pass 2:
+        // Pass 0: every job's first region (one thread per job), then the jobs
+        // that need splits with a 4096-region pool per wavefront slot.  Pass 1:
         // jobs that exhausted it, with pools that hold every region the
         // evaluation cap allows (maxeval / 149 + 2), on fewer slots.
+        const size_t njobs_all = npairs * (size_t)nf;
+        if (njobs_all > (size_t)INT32_MAX) return set_error(ctx, OPSEC_EINVAL, "too many pair integrals in one call");
         unsigned long long nev_total = 0;
-        const int njobs_all = (int)jobs.size();
-        for (int pass = 0; pass < 2 && !jobs.empty(); pass++) {
-            int njobs = (int)jobs.size();
+        std::vector<GMJob> again;   // pass-1 jobs
+        for (int pass = 0; pass < 2; pass++) {
+            const int njobs = pass == 0 ? (int)njobs_all : (int)again.size();
+            if (njobs == 0) break;
             long long cap_regions = P.maxEval ? (long long)P.maxEval / GM_NPTS + 2 : (1LL << 20);
             P.rmax = pass == 0 ? 4096 : (int)std::min<long long>(cap_regions, 1LL << 22);
             int nslots = pass == 0 ? std::min(njobs, ctx->num_cus * 8) : std::min(njobs, 64);
@@ -631,38 +749,80 @@ static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::v
                 size_t budget = (size_t)16 << 30;
                 nslots = (int)std::max<size_t>(1, std::min<size_t>(nslots, budget / per_slot));
             }
-            Scratch sj, sd, si, sc, sf;
+            Scratch sj, sd, si, sc, sf, sp, st, sx;
             int rc;
             if ((rc = dev_realloc(ctx, &sj.p, &sj.cap, sizeof(GMJob) * njobs))) return rc;
             if ((rc = dev_realloc(ctx, &sd.p, &sd.cap, sizeof(double) * 14 * (size_t)P.rmax * nslots))) return rc;
             if ((rc = dev_realloc(ctx, &si.p, &si.cap, sizeof(int) * 3 * (size_t)P.rmax * nslots))) return rc;
             if ((rc = dev_realloc(ctx, &sc.p, &sc.cap, 64))) return rc;
             if ((rc = dev_realloc(ctx, &sf.p, &sf.cap, sizeof(int) * njobs))) return rc;
-            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(sj.p, jobs.data(), sizeof(GMJob) * njobs, hipMemcpyHostToDevice, ctx->stream));
+            if ((rc = dev_realloc(ctx, &sp.p, &sp.cap, sizeof(int) * njobs))) return rc;
+            if ((rc = dev_realloc(ctx, &sx.p, &sx.cap, sizeof(int) * njobs))) return rc;
             OPSEC_HIP_TRY(ctx, hipMemsetAsync(sc.p, 0, 64, ctx->stream));
             int* d_counter = (int*)sc.p;
             unsigned long long* d_nev = (unsigned long long*)((char*)sc.p + 8);
-            hipLaunchKernelGGL(gm_kernel, dim3(nslots), dim3(64), 0, ctx->stream, mv, (const GMJob*)sj.p, njobs,
-                               d_counter, (double*)sd.p, (int*)si.p, P, dQ, nkeys, dQerr, (int*)sf.p, d_nev,
-                               g_gm_trace);
-            OPSEC_HIP_TRY(ctx, hipGetLastError());
+            int* d_npend = (int*)((char*)sc.p + 16);
+            int* d_nfail = (int*)((char*)sc.p + 20);
+            const int* d_idx = nullptr;
+            int nrun = njobs;
+            if (pass == 0) {
+                if ((rc = dev_realloc(ctx, &st.p, &st.cap, sizeof(GMJob) * npairs + sizeof(int) * nf))) return rc;
+                int* dfun = (int*)((GMJob*)st.p + npairs);
+                OPSEC_HIP_TRY(ctx, hipMemcpyAsync(st.p, tmpl.data(), sizeof(GMJob) * npairs, hipMemcpyHostToDevice,
+                                                  ctx->stream));
+                OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dfun, funcs, sizeof(int) * nf, hipMemcpyHostToDevice, ctx->stream));
+                hipLaunchKernelGGL(expand_gm_jobs_kernel, dim3((unsigned)((njobs + 255) / 256)), dim3(256), 0,
+                                   ctx->stream, (const GMJob*)st.p, (int)npairs, dfun, nf, (GMJob*)sj.p);
+                OPSEC_HIP_TRY(ctx, hipGetLastError());
+                if (!g_gm_trace) {
+                    hipLaunchKernelGGL(gm_first_kernel, dim3((unsigned)((njobs + 255) / 256)), dim3(256), 0,
+                                       ctx->stream, mv, (const GMJob*)sj.p, njobs, P, dQ, nkeys, dQerr, (int*)sf.p,
+                                       (int*)sp.p, d_npend, d_nev);
+                    OPSEC_HIP_TRY(ctx, hipGetLastError());
+                    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nrun, d_npend, 4, hipMemcpyDeviceToHost, ctx->stream));
+                    OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+                    d_idx = (const int*)sp.p;
+                }
+            } else {
+                OPSEC_HIP_TRY(ctx, hipMemcpyAsync(sj.p, again.data(), sizeof(GMJob) * njobs, hipMemcpyHostToDevice,
+                                                  ctx->stream));
+            }
+            if (nrun > 0) {
+                hipLaunchKernelGGL(gm_kernel, dim3(std::min(nslots, nrun)), dim3(64), 0, ctx->stream, mv,
+                                   (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ, nkeys,
+                                   dQerr, (int*)sf.p, d_nev, g_gm_trace);
+                OPSEC_HIP_TRY(ctx, hipGetLastError());
+            }
+            // jobs whose region pool ran out, listed on the device
+            int nfail = 0;
+            if (nrun > 0) {
+                hipLaunchKernelGGL(list_failed_kernel, dim3((unsigned)((nrun + 255) / 256)), dim3(256), 0, ctx->stream,
+                                   (const int*)sf.p, d_idx, nrun, (int*)sx.p, d_nfail);
+                OPSEC_HIP_TRY(ctx, hipGetLastError());
+            }
             unsigned long long nev = 0;
-            std::vector<int> fail(njobs);
             OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nev, d_nev, 8, hipMemcpyDeviceToHost, ctx->stream));
-            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(fail.data(), sf.p, sizeof(int) * njobs, hipMemcpyDeviceToHost, ctx->stream));
+            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nfail, d_nfail, 4, hipMemcpyDeviceToHost, ctx->stream));
             OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
             nev_total += nev;
-            std::vector<GMJob> again;
-            for (int i = 0; i < njobs; i++)
-                if (fail[i]) again.push_back(jobs[i]);
-            if (!again.empty() && pass == 1)
+            if (nfail > 0 && pass == 1)
                 return set_error(ctx, OPSEC_ECONV, "adaptive cubature exceeded %d regions", P.rmax);
-            jobs.swap(again);
+            std::vector<int> failed(nfail);
+            if (nfail > 0) {
+                OPSEC_HIP_TRY(ctx, hipMemcpyAsync(failed.data(), sx.p, sizeof(int) * nfail, hipMemcpyDeviceToHost,
+                                                  ctx->stream));
+                OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            }
+            std::sort(failed.begin(), failed.end());
+            again.clear();
+            for (int j : failed) {
+                GMJob J = tmpl[(size_t)j % npairs];
+                J.f = funcs[(size_t)j / npairs];
+                again.push_back(J);
+            }
         }
-        const int njobs = njobs_all;
-        const unsigned long long nev = nev_total;
-        ctx->st_integrals += njobs;
-        ctx->st_nodes += (int64_t)nev;
+        ctx->st_integrals += (int64_t)njobs_all;
+        ctx->st_nodes += (int64_t)nev_total;
         return OPSEC_OK;
     }
     if (method == OPSEC_QUAD_GAUSS) {
@@ -776,7 +936,7 @@ static int build_pair_tables_sph(opsec_ctx* ctx, int nf, const int* funcs, const
     for (int t = 0; t < nf; t++)
         OPSEC_HIP_TRY(ctx, hipMemsetAsync(ctx->d_Q + (size_t)funcs[t] * ctx->nkeys, 0, (size_t)ctx->nkeys * 8,
                                           ctx->stream));
-    const size_t chunk = std::max<size_t>(4096, ((size_t)1 << 22) / (size_t)nf);   // pairs per quadrature call
+    const size_t chunk = std::max<size_t>(4096, ((size_t)1 << 24) / (size_t)nf);   // pairs per quadrature call
     std::vector<int> keys, nearflag, tab(Nphi);
     std::vector<PairGeo> geo;
     for (int ha = 0; ha < narcs; ha++) {

@@ -125,6 +125,28 @@ def test_estimate_fp64_vs_oracle(ctx, mixing):
     np.testing.assert_allclose(got["W"].sum(1), 1.0, rtol=1e-12)
 
 
+@pytest.mark.parametrize("npar,prec,tol", [(46, lib.OPSEC_PREC_FP64, 1e-10), (47, lib.OPSEC_PREC_FP64, 1e-10),
+                                            (50, lib.OPSEC_PREC_FP64, 1e-10), (62, lib.OPSEC_PREC_FP32, 1e-5)])
+def test_estimate_many_bands(ctx, npar, prec, tol):
+    """Up to 46 parameters the Gram product is 48 x 48, above it 64 x 64 (the
+    50 bands of the reference's RealModel.bands); 63 is rejected."""
+    rng = np.random.default_rng(npar)
+    M = 90
+    lam = np.sort(rng.uniform(0.05, 5, M))[::-1].copy()
+    D = np.array([_packed((lambda A: A + A.T + np.diag(rng.uniform(0.5, 1.5, M)))(rng.normal(size=(M, M)) * 0.1))
+                  for _ in range(npar)])
+    y = np.sqrt(1 + lam) * rng.normal(size=M)
+    prior = rng.uniform(0.5, 2.0, npar)
+    got = ctx.estimate(lam, D, y, prior, lib.OPSEC_MIX_IDENTITY, prec)
+    ref = _orc_estimate(lam, D, y, prior, lib.OPSEC_MIX_IDENTITY)
+    for k in ("F", "q", "f"):
+        np.testing.assert_allclose(got[k], ref[k], rtol=tol, atol=tol * np.abs(ref[k]).max(), err_msg=k)
+    if npar == 62:
+        D2 = np.concatenate([D, D[:1]])
+        with pytest.raises(lib.OpsecError, match="Nparams must be 1..62"):
+            ctx.estimate(lam, D2, y, np.append(prior, 1.0), lib.OPSEC_MIX_IDENTITY, prec)
+
+
 def test_estimate_fp32_traces(ctx):
     rng = np.random.default_rng(5)
     M, npar = 300, 5

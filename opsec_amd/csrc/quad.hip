@@ -745,8 +745,12 @@ static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::v
             P.rmax = pass == 0 ? 4096 : (int)std::min<long long>(cap_regions, 1LL << 22);
             int nslots = pass == 0 ? std::min(njobs, ctx->num_cus * 8) : std::min(njobs, 64);
             if (pass == 1) {
-                size_t per_slot = (size_t)P.rmax * (14 * 8 + 3 * 4);
-                size_t budget = (size_t)16 << 30;
+                // as many wavefronts as half the free device memory holds pools for
+                size_t fre = 0, tot = 0;
+                if (hipMemGetInfo(&fre, &tot) != hipSuccess) fre = (size_t)16 << 30;
+                const size_t per_slot = (size_t)P.rmax * (14 * 8 + 3 * 4);
+                const size_t budget = std::max<size_t>((size_t)1 << 30, fre / 2);
+                nslots = std::min(njobs, ctx->num_cus * 8);
                 nslots = (int)std::max<size_t>(1, std::min<size_t>(nslots, budget / per_slot));
             }
             Scratch sj, sd, si, sc, sf, sp, st, sx;

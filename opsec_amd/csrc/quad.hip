@@ -471,6 +471,149 @@ __global__ __launch_bounds__(256) void gm_first_kernel(XiModelDev m, const GMJob
     if (threadIdx.x == 0 && blk) atomicAdd(nevals, blk);
 }
 
+// The rule on region ri of S evaluated by one thread: node values summed on the
+// fly in the order of gm_region_sum.
+__device__ void gm_thread_region(const XiModelDev& m, const XiDesc& xd, const GMJob& J, const GMParams& P, GMSlot& S,
+                                 int R, int ri) {
+    double c[6], hw[6], u[6];
+    for (int d = 0; d < 6; d++) {
+        c[d] = S.c[d * R + ri];
+        hw[d] = S.hw[d * R + ri];
+    }
+    const double ratio = (GM_L2 * GM_L2) / (GM_L4 * GM_L4);
+    const double w1 = (double)(12824 - 9120 * 6 + 400 * 36) / 19683.0;
+    const double w2 = 980. / 6561.;
+    const double w3 = (double)(1820 - 400 * 6) / 19683.0;
+    const double w4 = 200. / 19683.;
+    const double w5 = 6859.0 / 19683.0 / 64.0;
+    const double e1 = (double)(729 - 950 * 6 + 50 * 36) / 729.0;
+    const double e2 = 245. / 486.;
+    const double e3 = (double)(265 - 100 * 6) / 1458.0;
+    const double e4 = 25. / 729.;
+    gm_node(0, c, hw, u);
+    const double val0 = gm_integrand(m, xd, J, P, u);
+    double sum2 = 0, sum3 = 0, sum4 = 0, sum5 = 0, maxdiff = 0;
+    int split = 0;
+    for (int k = 0; k < 6; k++) {
+        double v[4];
+        for (int e = 0; e < 4; e++) {
+            gm_node(1 + 4 * k + e, c, hw, u);
+            v[e] = gm_integrand(m, xd, J, P, u);
+        }
+        sum2 += v[0] + v[1];
+        sum3 += v[2] + v[3];
+        const double diff = 0.0 + fabs(v[0] + v[1] - 2 * val0 - ratio * (v[2] + v[3] - 2 * val0));
+        if (diff > maxdiff) {
+            maxdiff = diff;
+            split = k;
+        }
+    }
+    for (int p = 25; p < 85; p++) {
+        gm_node(p, c, hw, u);
+        sum4 += gm_integrand(m, xd, J, P, u);
+    }
+    for (int p = 85; p < GM_NPTS; p++) {
+        gm_node(p, c, hw, u);
+        sum5 += gm_integrand(m, xd, J, P, u);
+    }
+    double vol = 1;
+    for (int d = 0; d < 6; d++) vol *= 2 * hw[d];
+    const double result = vol * (w1 * val0 + w2 * sum2 + w3 * sum3 + w4 * sum4 + w5 * sum5);
+    const double res5th = vol * (e1 * val0 + e2 * sum2 + e3 * sum3 + e4 * sum4);
+    S.val[ri] = result;
+    S.err[ri] = fabs(res5th - result);
+    S.split[ri] = split;
+}
+
+constexpr int GM_TR = 16;   // regions a thread keeps in private memory
+
+// The adaptive driver of gm_kernel's lane 0, one thread per integral, regions
+// in private memory: integrals that converge within GM_TR regions end here
+// with the same value; the others are listed in `deeper` for gm_kernel.
+__global__ __launch_bounds__(256) void gm_thread_kernel(XiModelDev m, const GMJob* jobs, const int* idx, int n,
+                                                        GMParams P, double* Q, size_t qstride, double* Qerr,
+                                                        int* jobfail, int* deeper, int* ndeeper,
+                                                        unsigned long long* nevals) {
+    __shared__ unsigned long long blk;
+    if (threadIdx.x == 0) blk = 0;
+    __syncthreads();
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        const int jb = idx[t];
+        const GMJob J = jobs[jb];
+        const XiDesc xd = m.desc[J.f];
+        double dreg[14 * GM_TR];
+        int ireg[3 * GM_TR];
+        GMSlot S;
+        S.c = dreg;
+        S.hw = dreg + 6 * GM_TR;
+        S.val = dreg + 12 * GM_TR;
+        S.err = dreg + 13 * GM_TR;
+        S.split = ireg;
+        S.heap = ireg + GM_TR;
+        S.list = ireg + 2 * GM_TR;
+        const int R = GM_TR;
+        for (int d = 0; d < 6; d++) {
+            S.c[d * R] = 0.5;
+            S.hw[d * R] = 0.5;
+        }
+        gm_thread_region(m, xd, J, P, S, R, 0);
+        int nheap = 0, nregions = 1;
+        double tv = 0, te = 0;
+        unsigned numEval = GM_NPTS;
+        heap_push(S, nheap, 0, tv, te);
+        bool overflow = false;
+        for (;;) {
+            if (!(numEval < P.maxEval || !P.maxEval)) break;
+            if (gm_converged(tv, te, P)) break;
+            const double eev = tv;
+            double eee = te;
+            int nR = 0;
+            do {
+                if (nregions + 1 > R) {
+                    overflow = true;
+                    break;
+                }
+                const int ri = heap_pop(S, nheap, tv, te);
+                eee -= S.err[ri];
+                const int d = S.split[ri];
+                const int r2 = nregions++;
+                for (int k = 0; k < 6; k++) {
+                    S.c[k * R + r2] = S.c[k * R + ri];
+                    S.hw[k * R + r2] = S.hw[k * R + ri];
+                }
+                const double h = S.hw[d * R + ri] * 0.5;
+                S.hw[d * R + ri] = h;
+                S.hw[d * R + r2] = h;
+                S.c[d * R + ri] -= h;
+                S.c[d * R + r2] += h;
+                numEval += GM_NPTS * 2;
+                S.list[nR++] = ri;
+                S.list[nR++] = r2;
+                if (gm_converged(eev, eee, P)) break;
+            } while (nheap > 0 && (numEval < P.maxEval || !P.maxEval));
+            if (overflow) break;
+            for (int q = 0; q < nR; q++) gm_thread_region(m, xd, J, P, S, R, S.list[q]);
+            for (int q = 0; q < nR; q++) heap_push(S, nheap, S.list[q], tv, te);
+        }
+        if (overflow) {
+            deeper[atomicAdd(ndeeper, 1)] = jb;
+        } else {
+            double val = 0, err = 0;
+            for (int i = 0; i < nheap; i++) {
+                val += S.val[S.heap[i]];
+                err += S.err[S.heap[i]];
+            }
+            Q[(size_t)J.f * qstride + J.key] = val;
+            if (Qerr) Qerr[(size_t)J.f * qstride + J.key] = err;
+            jobfail[jb] = 0;
+            atomicAdd(&blk, (unsigned long long)numEval);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && blk) atomicAdd(nevals, blk);
+}
+
 // ---- GAUSS: one thread per integral ----------------------------------------
 
 __global__ __launch_bounds__(256) void tent_kernel(XiModelDev m, const GaussJob* jobs, int njobs, TentRule rule,
@@ -767,6 +910,7 @@ static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::v
             unsigned long long* d_nev = (unsigned long long*)((char*)sc.p + 8);
             int* d_npend = (int*)((char*)sc.p + 16);
             int* d_nfail = (int*)((char*)sc.p + 20);
+            int* d_ndeep = (int*)((char*)sc.p + 24);
             const int* d_idx = nullptr;
             int nrun = njobs;
             if (pass == 0) {
@@ -779,12 +923,23 @@ static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::v
                                    ctx->stream, (const GMJob*)st.p, (int)npairs, dfun, nf, (GMJob*)sj.p);
                 OPSEC_HIP_TRY(ctx, hipGetLastError());
                 if (!g_gm_trace) {
+                    // first regions, then the shallow integrals one thread each
                     hipLaunchKernelGGL(gm_first_kernel, dim3((unsigned)((njobs + 255) / 256)), dim3(256), 0,
                                        ctx->stream, mv, (const GMJob*)sj.p, njobs, P, dQ, nkeys, dQerr, (int*)sf.p,
-                                       (int*)sp.p, d_npend, d_nev);
+                                       (int*)sx.p, d_npend, d_nev);
                     OPSEC_HIP_TRY(ctx, hipGetLastError());
-                    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nrun, d_npend, 4, hipMemcpyDeviceToHost, ctx->stream));
+                    int npend = 0;
+                    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&npend, d_npend, 4, hipMemcpyDeviceToHost, ctx->stream));
                     OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+                    nrun = 0;
+                    if (npend > 0) {
+                        hipLaunchKernelGGL(gm_thread_kernel, dim3((unsigned)((npend + 255) / 256)), dim3(256), 0,
+                                           ctx->stream, mv, (const GMJob*)sj.p, (const int*)sx.p, npend, P, dQ, nkeys,
+                                           dQerr, (int*)sf.p, (int*)sp.p, d_ndeep, d_nev);
+                        OPSEC_HIP_TRY(ctx, hipGetLastError());
+                        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nrun, d_ndeep, 4, hipMemcpyDeviceToHost, ctx->stream));
+                        OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+                    }
                     d_idx = (const int*)sp.p;
                 }
             } else {
@@ -940,7 +1095,9 @@ static int build_pair_tables_sph(opsec_ctx* ctx, int nf, const int* funcs, const
     for (int t = 0; t < nf; t++)
         OPSEC_HIP_TRY(ctx, hipMemsetAsync(ctx->d_Q + (size_t)funcs[t] * ctx->nkeys, 0, (size_t)ctx->nkeys * 8,
                                           ctx->stream));
-    const size_t chunk = std::max<size_t>(4096, ((size_t)1 << 24) / (size_t)nf);   // pairs per quadrature call
+    // pairs per quadrature call: up to 2^28 jobs (30 GB of job records), so the
+    // deepest adaptive integrals of a call, which set its duration, are met once
+    const size_t chunk = std::max<size_t>(4096, ((size_t)1 << 28) / (size_t)nf);
     std::vector<int> keys, nearflag, tab(Nphi);
     std::vector<PairGeo> geo;
     for (int ha = 0; ha < narcs; ha++) {

@@ -75,6 +75,7 @@ struct GMSlot {
     double* hw;      // 6 x R half-widths
     double* val;     // R
     double* err;     // R
+    double* hkey;    // R: err of the region at each heap position
     int* split;      // R
     int* heap;       // R (region indices)
     int* list;       // R
@@ -82,12 +83,13 @@ struct GMSlot {
 
 __device__ __forceinline__ GMSlot gm_slot(double* dpool, int* ipool, int slot, int R) {
     GMSlot s;
-    double* d = dpool + (size_t)slot * 14 * R;
+    double* d = dpool + (size_t)slot * 15 * R;
     int* q = ipool + (size_t)slot * 3 * R;
     s.c = d;
     s.hw = d + 6 * R;
     s.val = d + 12 * R;
     s.err = d + 13 * R;
+    s.hkey = d + 14 * R;
     s.split = q;
     s.heap = q + R;
     s.list = q + 2 * R;
@@ -222,31 +224,46 @@ __device__ void gm_eval_regions(const XiModelDev& m, const XiDesc& xd, const GMJ
 }
 
 // Binary max-heap on region errors (Cormen-Leiserson-Rivest), lane 0 only.
+// The keys (region errors) sit beside the region indices, hkey[k] =
+// err[heap[k]], so each level of a sift costs independent loads.
 __device__ void heap_push(GMSlot& S, int& n, int ri, double& tv, double& te) {
     tv += S.val[ri];
     te += S.err[ri];
     int insert = n++;
-    double key = S.err[ri];
+    const double key = S.err[ri];
     while (insert) {
-        int parent = (insert - 1) / 2;
-        if (key <= S.err[S.heap[parent]]) break;
+        const int parent = (insert - 1) / 2;
+        const double pk = S.hkey[parent];
+        if (key <= pk) break;
         S.heap[insert] = S.heap[parent];
+        S.hkey[insert] = pk;
         insert = parent;
     }
     S.heap[insert] = ri;
+    S.hkey[insert] = key;
 }
 
 __device__ int heap_pop(GMSlot& S, int& n, double& tv, double& te) {
-    int ret = S.heap[0];
+    const int ret = S.heap[0];
     int i = 0, child;
-    S.heap[0] = S.heap[--n];
+    --n;
+    S.heap[0] = S.heap[n];
+    S.hkey[0] = S.hkey[n];
     while ((child = i * 2 + 1) < n) {
-        int largest = (S.err[S.heap[child]] <= S.err[S.heap[i]]) ? i : child;
-        if (++child < n && S.err[S.heap[largest]] < S.err[S.heap[child]]) largest = child;
+        const double ki = S.hkey[i], kc = S.hkey[child];
+        const double kc2 = child + 1 < n ? S.hkey[child + 1] : 0.0;
+        int largest = (kc <= ki) ? i : child;
+        double kl = (kc <= ki) ? ki : kc;
+        if (++child < n && kl < kc2) {
+            largest = child;
+            kl = kc2;
+        }
         if (largest == i) break;
-        int t = S.heap[i];
+        const int t = S.heap[i];
         S.heap[i] = S.heap[largest];
         S.heap[largest] = t;
+        S.hkey[largest] = ki;
+        S.hkey[i] = kl;
         i = largest;
     }
     tv -= S.val[ret];
@@ -542,13 +559,14 @@ __global__ __launch_bounds__(256) void gm_thread_kernel(XiModelDev m, const GMJo
         const int jb = idx[t];
         const GMJob J = jobs[jb];
         const XiDesc xd = m.desc[J.f];
-        double dreg[14 * GM_TR];
+        double dreg[15 * GM_TR];
         int ireg[3 * GM_TR];
         GMSlot S;
         S.c = dreg;
         S.hw = dreg + 6 * GM_TR;
         S.val = dreg + 12 * GM_TR;
         S.err = dreg + 13 * GM_TR;
+        S.hkey = dreg + 14 * GM_TR;
         S.split = ireg;
         S.heap = ireg + GM_TR;
         S.list = ireg + 2 * GM_TR;
@@ -891,7 +909,7 @@ static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::v
                 // as many wavefronts as half the free device memory holds pools for
                 size_t fre = 0, tot = 0;
                 if (hipMemGetInfo(&fre, &tot) != hipSuccess) fre = (size_t)16 << 30;
-                const size_t per_slot = (size_t)P.rmax * (14 * 8 + 3 * 4);
+                const size_t per_slot = (size_t)P.rmax * (15 * 8 + 3 * 4);
                 const size_t budget = std::max<size_t>((size_t)1 << 30, fre / 2);
                 nslots = std::min(njobs, ctx->num_cus * 8);
                 nslots = (int)std::max<size_t>(1, std::min<size_t>(nslots, budget / per_slot));
@@ -899,7 +917,7 @@ static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::v
             Scratch sj, sd, si, sc, sf, sp, st, sx;
             int rc;
             if ((rc = dev_realloc(ctx, &sj.p, &sj.cap, sizeof(GMJob) * njobs))) return rc;
-            if ((rc = dev_realloc(ctx, &sd.p, &sd.cap, sizeof(double) * 14 * (size_t)P.rmax * nslots))) return rc;
+            if ((rc = dev_realloc(ctx, &sd.p, &sd.cap, sizeof(double) * 15 * (size_t)P.rmax * nslots))) return rc;
             if ((rc = dev_realloc(ctx, &si.p, &si.cap, sizeof(int) * 3 * (size_t)P.rmax * nslots))) return rc;
             if ((rc = dev_realloc(ctx, &sc.p, &sc.cap, 64))) return rc;
             if ((rc = dev_realloc(ctx, &sf.p, &sf.cap, sizeof(int) * njobs))) return rc;

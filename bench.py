@@ -218,6 +218,64 @@ def cpu_baseline(cfg, cells, k, P, lo, hi, B, budget_s=20.0):
     }
 
 
+def cpu_baseline_sph(cells, dims, k, P, lo, hi, B, budget_s=25.0):
+    """Oracle on a bounded sample of the spherical workload, extrapolated: the
+    arc-cache block (Genz-Malik pairs, src/sig.cpp:125-213) of the rows of a few
+    arcs against all columns, for the prior and the lowest and highest bands,
+    one arc per host thread; then S_p B for a block of modes."""
+    import ctypes as Cc
+    from concurrent.futures import ThreadPoolExecutor
+    from tests import oracles as O
+    orc = O.oracle()
+    nr, nmu, nphi = dims
+    narcs = nr * nmu
+    N, M, nb = len(cells), B.shape[0], len(lo)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    r768, tabs = O.oracle_tables(k, P, [lo[0], lo[-1]], [hi[0], hi[-1]])
+    xis = {"prior": O.XiHandle(0, r768, tabs[0]), "band_lo": O.XiHandle(1, a=float(lo[0]), b=float(hi[0])),
+           "band_hi": O.XiHandle(1, a=float(lo[-1]), b=float(hi[-1]))}
+    arc = cells["d1"] * nmu + cells["d2"]
+    cols = np.arange(N, dtype=np.int32)
+    q = (1e-5, 1e-10, 50000000)
+
+    def block(args):
+        name, h = args
+        rows = np.nonzero(arc == h)[0].astype(np.int32)
+        S = np.zeros((len(rows), N))
+        t0 = time.perf_counter()
+        orc.orc_sig_block_sph(Cc.byref(xis[name].s), cells.ctypes.data, N, nr, nmu, nphi, O.ptr(rows), len(rows),
+                              O.ptr(cols), N, O.ptr(S), N, *q)
+        return name, time.perf_counter() - t0
+
+    nsamp = max(1, min(cores, narcs))
+    arcs = [int(h) for h in np.linspace(0, narcs - 1, nsamp)]
+    times = {n: [] for n in xis}
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=cores) as ex:
+        for name, dt in ex.map(block, [(n, h) for n in xis for h in arcs]):
+            times[name].append(dt)
+    t_sample = time.perf_counter() - t0
+    per_arc = {n: float(np.mean(v)) for n, v in times.items()}
+    t_tables = narcs * (per_arc["prior"] + nb * 0.5 * (per_arc["band_lo"] + per_arc["band_hi"])) / cores
+    # projection at the oracle's measured FP64 GEMM rate (OpenMP over the host threads)
+    R = min(N, 2048)
+    S = np.random.default_rng(0).normal(size=(R, N))
+    msub = min(M, 64)
+    Cm = np.zeros((msub, R))
+    t1 = time.perf_counter()
+    orc.orc_gemm_rows(R, N, O.ptr(S), msub, O.ptr(np.ascontiguousarray(B[:msub])), O.ptr(Cm))
+    rate = 2.0 * R * N * msub / (time.perf_counter() - t1)
+    t_full = t_tables + nb * (2.0 * N * N * M + 1.0 * N * M * M) / rate
+    pairs = float(N) * N * (nb + 1)
+    return {"value": pairs / t_full, "unit": "cell-pair integrals/s", "cores": cores, "kind": "port",
+            "sample": (f"oracle arc-cache blocks (Genz-Malik) of the rows of {nsamp} of {narcs} arcs against all "
+                       f"{N} columns for the prior, the lowest and the highest band ({t_sample:.1f} s on {cores} "
+                       f"threads; per arc {per_arc['prior']:.2f} / {per_arc['band_lo']:.2f} / "
+                       f"{per_arc['band_hi']:.2f} s), bands taken as the mean of the lowest and highest, and S_p B "
+                       f"at {rate / 1e9:.1f} GFLOP/s; extrapolated: {t_full:.0f} s per S->Fisher pass"),
+            "s_per_step_extrapolated": t_full}
+
+
 def spherical_main(args):
     """The spherical-shell workload: r in [200, 1000] Mpc/h over the 60-degree cap
     mu >= 0.5, Nr x Nmu x Nphi = (64, 32, 32) / scale cells, nbar(r) = 5e-4 exp(-(r/600)^2)
@@ -286,6 +344,10 @@ def spherical_main(args):
         "klt": {"s": t_klt, "nconv": int(nconv), "prior_table_s": t_s},
     }
     out.update(compare_estimates(r64[1], rmx[1]))
+    if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
+        out["cpu_baseline"] = cpu_baseline_sph(cells, (nr, nmu, nphi), k, P, lo, hi, B)
+        log(f"cpu baseline sample took {time.perf_counter() - t0:.1f} s")
     s1 = float(rmx[2][3]) / max(1.0, float(rmx[2][5]))
     out["roofline"] = {"bound": "mfma", "achieved": 2.0 * N * N * M / (s1 / 1e3) / 1e12, "peak": PEAK_TF["bf16"] / 3,
                        "unit": "TFLOP/s", "traffic": None, "launch_ms": s1,

@@ -471,11 +471,25 @@ def main():
                                   args.steps, args.warmup, "fp64")
         st = ctx.pair_stats()
         results["fp64"] = dict(ms=ms, est=est64, kms=k64, stats=st)
+        tables64 = [ctx.pair_table(f) for f in funcs]   # the quadrature tables of S and every C_{,p}
     if args.path in ("both", "surrogate"):
         ms, ests, ks = run_path(ctx, L, funcs, qs, L.OPSEC_PREC_BF16X3, L.OPSEC_PREC_FP32, lam, y, prior,
                                 args.steps, args.warmup, "surrogate")
         st = ctx.pair_stats()
         results["surrogate"] = dict(ms=ms, est=ests, kms=ks, stats=st)
+        if "fp64" in results:
+            # error of the surrogate path's S and C_{,p} (their distinct pair integrals) against quadrature:
+            # per function max |dQ| / max |Q|, and the largest per-entry error relative to the entry
+            errs, rel = [], []
+            for f in funcs:
+                q_s, q_64 = ctx.pair_table(f), tables64[f]
+                errs.append(float(np.max(np.abs(q_s - q_64)) / np.max(np.abs(q_64))))
+                big = np.abs(q_64) > 1e-3 * np.max(np.abs(q_64))
+                rel.append(float(np.max(np.abs(q_s - q_64)[big] / np.abs(q_64[big]))))
+            results["s_error"] = {"max_abs_over_function_max": max(errs), "prior_max_abs_over_max": errs[0],
+                                  "max_rel_on_entries_above_1e-3_of_max": max(rel),
+                                  "note": "S and all C_{,p}: the surrogate path's pair tables against the 6^3 "
+                                          "Gauss quadrature tables of the FP64 path, per function"}
         if args.projection == "toeplitz":
             # the same mixed path with the dense projection, for comparison
             ctx.set_projection(L.OPSEC_PROJ_DENSE)
@@ -544,6 +558,8 @@ def main():
         out.update(compare_estimates(results["fp64"]["est"], results["surrogate"]["est"]))
         st = results["surrogate"]["stats"]
         out["surrogate_stats"] = {k2: int(v) for k2, v in st.items()}
+        if "s_error" in results:
+            out["s_error"] = results["s_error"]
         out["surrogate_train_s"] = t_train
         out["surrogate_train_loss"] = loss
         if args.diagnose:

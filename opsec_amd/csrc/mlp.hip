@@ -607,10 +607,15 @@ int surrogate_tables(opsec_ctx* ctx, int nf_all, const int* funcs_all, const ops
         (d.b > d.a && d.b > 0 ? band_funcs : other).push_back(funcs_all[t]);
     }
     if (!other.empty()) {
+        // the plain method's rule: the composite near rule for the self pair
+        // only (key 0), the Gauss rule for every other pair
         opsec_quad qq = *q;
         qq.method = OPSEC_QUAD_GAUSS;
         qq.near = 1;
-        int rc = run_quadrature_public(ctx, (int)other.size(), other.data(), keys, geo, nearflag, &qq, ctx->d_Q,
+        std::vector<int> self(keys.size());
+        for (size_t i = 0; i < keys.size(); i++) self[i] = keys[i] == 0;
+        (void)nearflag;
+        int rc = run_quadrature_public(ctx, (int)other.size(), other.data(), keys, geo, self, &qq, ctx->d_Q,
                                        ctx->nkeys);
         if (rc) return rc;
     }

@@ -178,7 +178,9 @@ int opsec_d2h(opsec_ctx* ctx, void* dst, const void* src, size_t bytes);
  * re-orthogonalisation.  evals: nev values in descending order; evecs: nev x
  * Ncells, mode-major (the modefile layout, src/klt.cpp:219-248).  *nconv =
  * number of modes whose residual |S v - lambda v| <= tol |lambda_max|.
- * block <= 0 picks the default block size; maxdim caps the Krylov dimension. */
+ * block <= 0 picks the default block size; maxdim caps the Krylov dimension
+ * (maxdim <= 0: 3 nev + 4 block, doubled up to twice, to at most Ncells,
+ * while some of the nev modes have not converged). */
 int opsec_klt(opsec_ctx* ctx, int nev, double tol, int block, int maxdim,
               double* evals, double* evecs, int* nconv);
 /* Same solver on an arbitrary symmetric host matrix A (n x n row-major);

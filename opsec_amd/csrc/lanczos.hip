@@ -814,6 +814,17 @@ int opsec_klt(opsec_ctx* ctx, int nev, double tol, int block, int maxdim, double
     OPSEC_HIP_TRY(ctx, hipMalloc(&dS, (size_t)N * N * 8));
     int rc = opsec_assemble_device(ctx, 0, dS, N);
     if (!rc) rc = block_lanczos(ctx, dS, N, nev, tol, block, maxdim, evals, evecs, nconv);
+    if (maxdim <= 0) {
+        // automatic Krylov dimension: when some modes have not converged (slowly
+        // decaying spectra, e.g. spherical shells), run again with twice the
+        // space, up to N, at most twice
+        const int b = block > 0 ? std::min(block, N) : std::min(128, std::max(1, N / 4));
+        int md = std::max(3 * std::min(nev, N) + 4 * b, 16 * b);
+        for (int retry = 0; retry < 2 && !rc && *nconv < nev && md < N; retry++) {
+            md = std::min(N, 2 * md);
+            rc = block_lanczos(ctx, dS, N, nev, tol, block, md, evals, evecs, nconv);
+        }
+    }
     hipFree(dS);
     return rc;
 }

@@ -289,7 +289,7 @@ def spherical_main(args):
     from opsec_amd import synth
     s = args.scale
     nr, nmu, nphi = 64 // s, 32 // s, 32 // s
-    nb, M = 50, max(16, 6000 // s ** 3)
+    nb, M = 50, (args.modes if args.modes > 0 else max(16, 6000 // s ** 3))
     ctx = L.Context(0)
     ncu, arch = ctx.device_info()
     sv = L.survey_sph(nr, nmu, nphi, 200.0, 1000.0, 0.5, 1.0, 0.0, 2 * np.pi)
@@ -365,6 +365,8 @@ def main():
     ap.add_argument("--config", default="C2", choices=["C1", "C2", "C5"])
     ap.add_argument("--scale", type=int, default=2, choices=[1, 2, 4],
                     help="C5 only: grid divisor per dimension (1 = the full 65536-cell shell)")
+    ap.add_argument("--modes", type=int, default=0,
+                    help="C5 only: KL modes (default 6000 / scale^3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--path", default="both", choices=["both", "fp64", "surrogate"])
     ap.add_argument("--mlp-samples", type=int, default=200000, help="surrogate training pairs (setup)")

@@ -1113,35 +1113,49 @@ static int build_pair_tables_sph(opsec_ctx* ctx, int nf, const int* funcs, const
     for (int t = 0; t < nf; t++)
         OPSEC_HIP_TRY(ctx, hipMemsetAsync(ctx->d_Q + (size_t)funcs[t] * ctx->nkeys, 0, (size_t)ctx->nkeys * 8,
                                           ctx->stream));
-    // pairs per quadrature call: up to 2^28 jobs (30 GB of job records), so the
-    // deepest adaptive integrals of a call, which set its duration, are met once
-    const size_t chunk = std::max<size_t>(4096, ((size_t)1 << 28) / (size_t)nf);
+    // Functions without band edges (the prior, whose near-diagonal integrals
+    // run to the evaluation cap) and band functions are integrated in separate
+    // sweeps.  A quadrature call takes up to 2^28 jobs (30 GB of job records);
+    // its duration is set by its deepest integrals, so grouping the deep ones
+    // into few calls pays that wait few times.
+    std::vector<int> deep, shallow;
+    for (int t = 0; t < nf; t++) {
+        const XiDesc& d = ctx->funcs[funcs[t]];
+        (d.b > d.a && d.b > 0 ? shallow : deep).push_back(funcs[t]);
+    }
     std::vector<int> keys, nearflag, tab(Nphi);
     std::vector<PairGeo> geo;
-    for (int ha = 0; ha < narcs; ha++) {
-        if (!arcs[ha].empty())
-            for (int hb = 0; hb < narcs; hb++) {
-                if (arcs[hb].empty()) continue;
-                std::fill(tab.begin(), tab.end(), 0);
-                int left = Nphi;
-                for (int a : arcs[ha]) {
-                    for (int b : arcs[hb]) {
-                        const int k = std::abs(cells[a].d3 - cells[b].d3);
-                        if (tab[k]) continue;
-                        tab[k] = 1;
-                        keys.push_back((ha * narcs + hb) * Nphi + k);
-                        geo.push_back(pair_geo(cells[std::min(a, b)], cells[std::max(a, b)]));
-                        if (--left == 0) break;
+    for (const std::vector<int>* group : {&deep, &shallow}) {
+        if (group->empty()) continue;
+        const int ng = (int)group->size();
+        const size_t chunk = std::max<size_t>(4096, ((size_t)1 << 28) / (size_t)ng);
+        keys.clear();
+        geo.clear();
+        for (int ha = 0; ha < narcs; ha++) {
+            if (!arcs[ha].empty())
+                for (int hb = 0; hb < narcs; hb++) {
+                    if (arcs[hb].empty()) continue;
+                    std::fill(tab.begin(), tab.end(), 0);
+                    int left = Nphi;
+                    for (int a : arcs[ha]) {
+                        for (int b : arcs[hb]) {
+                            const int k = std::abs(cells[a].d3 - cells[b].d3);
+                            if (tab[k]) continue;
+                            tab[k] = 1;
+                            keys.push_back((ha * narcs + hb) * Nphi + k);
+                            geo.push_back(pair_geo(cells[std::min(a, b)], cells[std::max(a, b)]));
+                            if (--left == 0) break;
+                        }
+                        if (left == 0) break;
                     }
-                    if (left == 0) break;
                 }
+            if (keys.size() >= chunk || (ha == narcs - 1 && !keys.empty())) {
+                nearflag.assign(keys.size(), 0);
+                int rc = run_quadrature(ctx, ng, group->data(), keys, geo, nearflag, q, ctx->d_Q, (size_t)ctx->nkeys);
+                if (rc) return rc;
+                keys.clear();
+                geo.clear();
             }
-        if (keys.size() >= chunk || (ha == narcs - 1 && !keys.empty())) {
-            nearflag.assign(keys.size(), 0);
-            int rc = run_quadrature(ctx, nf, funcs, keys, geo, nearflag, q, ctx->d_Q, (size_t)ctx->nkeys);
-            if (rc) return rc;
-            keys.clear();
-            geo.clear();
         }
     }
     return OPSEC_OK;

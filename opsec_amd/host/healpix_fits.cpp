@@ -124,7 +124,8 @@ bool read_healpix_map(const std::string& path, HealpixMap* out, std::string* why
     const long long rowbytes = ext.get_int("NAXIS1"), nrows = ext.get_int("NAXIS2");
     const long long nside = ext.get_int("NSIDE", 0);
     if (rowbytes <= 0 || nrows <= 0) return fail("bad table size");
-    if (nside <= 0 || nside > (1LL << 29)) return fail("missing or bad NSIDE");
+    // nside 16384 is 3.2e9 pixels (12.9 GB as float); larger maps are refused
+    if (nside <= 0 || nside > (1LL << 14)) return fail("missing or bad NSIDE");
     const long long npix = 12 * nside * nside;
     if (npix % nrows != 0) return fail("Problem with npix");
     const long long per_row = npix / nrows;
@@ -146,6 +147,13 @@ bool read_healpix_map(const std::string& path, HealpixMap* out, std::string* why
     }
     if (repeat < per_row || repeat * width > rowbytes) return fail("column 1 holds fewer values than npix / NAXIS2");
     const double scal = ext.get_double("TSCAL1", 1.0), zero = ext.get_double("TZERO1", 0.0);
+    // the table must be in the file before anything is allocated for it
+    const long here = std::ftell(f);
+    if (std::fseek(f, 0, SEEK_END) != 0) return fail("unreadable file");
+    const long end = std::ftell(f);
+    if (here < 0 || end < here || (unsigned long long)(end - here) < (unsigned long long)rowbytes * nrows)
+        return fail("truncated table");
+    if (std::fseek(f, here, SEEK_SET) != 0) return fail("unreadable file");
     std::vector<unsigned char> row((size_t)rowbytes);
     out->map.assign((size_t)npix, 0.0f);
     for (long long r = 0; r < nrows; r++) {

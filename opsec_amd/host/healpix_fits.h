@@ -19,8 +19,8 @@ struct HealpixMap {
 // from its header, 12 NSIDE^2 values read row by row from column 1 (NAXIS2
 // rows of npix / NAXIS2 values each), converted to float; NaN values become
 // the HEALPix null value -1.6375e30.  Column types E, D, J, I, K and B are
-// read, with TSCAL1 / TZERO1 applied.  Returns false with *why set on any
-// problem.
+// read, with TSCAL1 / TZERO1 applied; NSIDE up to 16384.  Returns false with
+// *why set on any problem (checked before the map is allocated).
 bool read_healpix_map(const std::string& path, HealpixMap* out, std::string* why);
 
 }  // namespace opsec_host

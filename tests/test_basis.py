@@ -185,6 +185,16 @@ def test_basis_rejects_bad_maskfile(tmp_path):
     (tmp_path / "m2.fits").write_bytes(raw)
     r = run(*args, f"survey.maskfile={tmp_path}/m2.fits")
     assert r.returncode == 1 and "unrecognized HEALPix ordering: SPIRAL" in r.stderr
+    # a map whose table is cut short, and one claiming an absurd NSIDE
+    synth.write_healpix_fits(tmp_path / "m3.fits", np.ones(12 * 64 * 64, np.float32), per_row=1024)
+    raw = (tmp_path / "m3.fits").read_bytes()
+    (tmp_path / "m3.fits").write_bytes(raw[:2 * 2880 + 5000])
+    r = run(*args, f"survey.maskfile={tmp_path}/m3.fits")
+    assert r.returncode == 1 and "truncated table" in r.stderr
+    raw = (tmp_path / "m2.fits").read_bytes().replace(b"NSIDE   =                    1", b"NSIDE   =           1073741824")
+    (tmp_path / "m4.fits").write_bytes(raw)
+    r = run(*args, f"survey.maskfile={tmp_path}/m4.fits")
+    assert r.returncode == 1 and "bad NSIDE" in r.stderr
     r = run(*args[:-1], "survey.radial=compute", f"survey.maskfile={tmp_path}/m2.fits")
     assert r.returncode == 1 and "'radial = compute' not implemented" in r.stderr
     r = run(*args[:1], "coordsys=cartesian", *args[2:], f"survey.maskfile={tmp_path}/m2.fits")

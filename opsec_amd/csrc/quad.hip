@@ -196,7 +196,7 @@ __device__ void gm_eval_regions(const XiModelDev& m, const XiDesc& xd, const GMJ
     const int lane = threadIdx.x;
     for (int t0 = 0; t0 < n; t0 += GM_SUB) {
         const int nt = min(GM_SUB, n - t0);
-        for (int i = lane; i < nt * GM_NPTS; i += 64) {
+        for (int i = lane; i < nt * GM_NPTS; i += blockDim.x) {
             const int t = i / GM_NPTS, p = i - t * GM_NPTS;
             const int ri = list[t0 + t];
             double c[6], hw[6], u[6];
@@ -278,7 +278,11 @@ __device__ __forceinline__ bool gm_converged(double val, double err, const GMPar
 
 // One wavefront per integral; jobs idx[0..njobs) (all jobs when idx is null)
 // are taken from a counter.
-__global__ __launch_bounds__(64) void gm_kernel(XiModelDev m, const GMJob* jobs, const int* idx, int njobs,
+// NT threads per integral: 64 in the first pass, 512 in the rerun of the deep
+// integrals, whose split batches are short and follow one another, so the
+// nodes of a batch are spread over more lanes.
+template <int NT>
+__global__ __launch_bounds__(NT) void gm_kernel(XiModelDev m, const GMJob* jobs, const int* idx, int njobs,
                                                 int* counter, double* dpool, int* ipool, GMParams P, double* Q,
                                                 size_t qstride, double* Qerr, int* jobfail,
                                                 unsigned long long* nevals, double* trace) {
@@ -965,9 +969,14 @@ static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::v
                                                   ctx->stream));
             }
             if (nrun > 0) {
-                hipLaunchKernelGGL(gm_kernel, dim3(std::min(nslots, nrun)), dim3(64), 0, ctx->stream, mv,
-                                   (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ, nkeys,
-                                   dQerr, (int*)sf.p, d_nev, g_gm_trace);
+                if (pass == 0)
+                    hipLaunchKernelGGL(gm_kernel<64>, dim3(std::min(nslots, nrun)), dim3(64), 0, ctx->stream, mv,
+                                       (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
+                                       nkeys, dQerr, (int*)sf.p, d_nev, g_gm_trace);
+                else
+                    hipLaunchKernelGGL(gm_kernel<512>, dim3(std::min(nslots, nrun)), dim3(512), 0, ctx->stream, mv,
+                                       (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
+                                       nkeys, dQerr, (int*)sf.p, d_nev, g_gm_trace);
                 OPSEC_HIP_TRY(ctx, hipGetLastError());
             }
             // jobs whose region pool ran out, listed on the device

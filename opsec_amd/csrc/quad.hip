@@ -278,9 +278,10 @@ __device__ __forceinline__ bool gm_converged(double val, double err, const GMPar
 
 // One wavefront per integral; jobs idx[0..njobs) (all jobs when idx is null)
 // are taken from a counter.
-// NT threads per integral: 64 in the first pass, 512 in the rerun of the deep
-// integrals, whose split batches are short and follow one another, so the
-// nodes of a batch are spread over more lanes.
+// NT threads per integral: 64 in the first pass; in the rerun of the deep
+// integrals, whose split batches are short and follow one another, up to 512
+// when there are few of them, so the nodes of a batch are spread over more
+// lanes while all of them stay resident.
 template <int NT>
 __global__ __launch_bounds__(NT) void gm_kernel(XiModelDev m, const GMJob* jobs, const int* idx, int njobs,
                                                 int* counter, double* dpool, int* ipool, GMParams P, double* Q,
@@ -973,8 +974,16 @@ static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::v
                     hipLaunchKernelGGL(gm_kernel<64>, dim3(std::min(nslots, nrun)), dim3(64), 0, ctx->stream, mv,
                                        (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                        nkeys, dQerr, (int*)sf.p, d_nev, g_gm_trace);
-                else
+                else if (nrun <= 2 * ctx->num_cus)   // few deep integrals: 8 waves each
                     hipLaunchKernelGGL(gm_kernel<512>, dim3(std::min(nslots, nrun)), dim3(512), 0, ctx->stream, mv,
+                                       (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
+                                       nkeys, dQerr, (int*)sf.p, d_nev, g_gm_trace);
+                else if (nrun <= 4 * ctx->num_cus)
+                    hipLaunchKernelGGL(gm_kernel<256>, dim3(std::min(nslots, nrun)), dim3(256), 0, ctx->stream, mv,
+                                       (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
+                                       nkeys, dQerr, (int*)sf.p, d_nev, g_gm_trace);
+                else   // many: one wave each, all resident at once
+                    hipLaunchKernelGGL(gm_kernel<64>, dim3(std::min(nslots, nrun)), dim3(64), 0, ctx->stream, mv,
                                        (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                        nkeys, dQerr, (int*)sf.p, d_nev, g_gm_trace);
                 OPSEC_HIP_TRY(ctx, hipGetLastError());

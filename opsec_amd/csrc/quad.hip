@@ -224,34 +224,55 @@ __device__ void gm_eval_regions(const XiModelDev& m, const XiDesc& xd, const GMJ
 }
 
 // Binary max-heap on region errors (Cormen-Leiserson-Rivest), lane 0 only.
-// The keys (region errors) sit beside the region indices, hkey[k] =
-// err[heap[k]], so each level of a sift costs independent loads.
-__device__ void heap_push(GMSlot& S, int& n, int ri, double& tv, double& te) {
+// Storage of the region heap: position p holds region idx(p) with key(p) =
+// its error, keys beside the indices so a sift level costs independent loads.
+// Positions below nl sit in LDS (the top levels, which every pop walks), the
+// rest in the slot's arrays.
+struct HeapStore {
+    double* gk;
+    int* gi;
+    double* lk;
+    int* li;
+    int nl;
+    __device__ __forceinline__ double key(int p) const { return p < nl ? lk[p] : gk[p]; }
+    __device__ __forceinline__ int idx(int p) const { return p < nl ? li[p] : gi[p]; }
+    __device__ __forceinline__ void set(int p, double k, int i) const {
+        if (p < nl) {
+            lk[p] = k;
+            li[p] = i;
+        } else {
+            gk[p] = k;
+            gi[p] = i;
+        }
+    }
+};
+
+// Binary max-heap on region errors (Cormen-Leiserson-Rivest), one lane.
+__device__ void heap_push(const GMSlot& S, const HeapStore& H, int& n, int ri, double& tv, double& te) {
     tv += S.val[ri];
     te += S.err[ri];
     int insert = n++;
     const double key = S.err[ri];
     while (insert) {
         const int parent = (insert - 1) / 2;
-        const double pk = S.hkey[parent];
+        const double pk = H.key(parent);
         if (key <= pk) break;
-        S.heap[insert] = S.heap[parent];
-        S.hkey[insert] = pk;
+        H.set(insert, pk, H.idx(parent));
         insert = parent;
     }
-    S.heap[insert] = ri;
-    S.hkey[insert] = key;
+    H.set(insert, key, ri);
 }
 
-__device__ int heap_pop(GMSlot& S, int& n, double& tv, double& te) {
-    const int ret = S.heap[0];
-    int i = 0, child;
+__device__ int heap_pop(const GMSlot& S, const HeapStore& H, int& n, double& tv, double& te) {
+    const int ret = H.idx(0);
     --n;
-    S.heap[0] = S.heap[n];
-    S.hkey[0] = S.hkey[n];
+    const double ki = H.key(n);   // the last element, sifted down from the root
+    const int ii = H.idx(n);
+    H.set(0, ki, ii);
+    int i = 0, child;
     while ((child = i * 2 + 1) < n) {
-        const double ki = S.hkey[i], kc = S.hkey[child];
-        const double kc2 = child + 1 < n ? S.hkey[child + 1] : 0.0;
+        const double kc = H.key(child);
+        const double kc2 = child + 1 < n ? H.key(child + 1) : 0.0;
         int largest = (kc <= ki) ? i : child;
         double kl = (kc <= ki) ? ki : kc;
         if (++child < n && kl < kc2) {
@@ -259,11 +280,8 @@ __device__ int heap_pop(GMSlot& S, int& n, double& tv, double& te) {
             kl = kc2;
         }
         if (largest == i) break;
-        const int t = S.heap[i];
-        S.heap[i] = S.heap[largest];
-        S.heap[largest] = t;
-        S.hkey[largest] = ki;
-        S.hkey[i] = kl;
+        H.set(i, kl, H.idx(largest));
+        H.set(largest, ki, ii);
         i = largest;
     }
     tv -= S.val[ret];
@@ -282,16 +300,20 @@ __device__ __forceinline__ bool gm_converged(double val, double err, const GMPar
 // integrals, whose split batches are short and follow one another, up to 512
 // when there are few of them, so the nodes of a batch are spread over more
 // lanes while all of them stay resident.
-template <int NT>
+// NL > 0: the first NL heap positions in LDS.
+template <int NT, int NL>
 __global__ __launch_bounds__(NT) void gm_kernel(XiModelDev m, const GMJob* jobs, const int* idx, int njobs,
                                                 int* counter, double* dpool, int* ipool, GMParams P, double* Q,
                                                 size_t qstride, double* Qerr, int* jobfail,
                                                 unsigned long long* nevals, double* trace) {
     __shared__ double vals[GM_SUB][GM_NPTS];
+    __shared__ double lkey[NL > 0 ? NL : 1];
+    __shared__ int lidx[NL > 0 ? NL : 1];
     __shared__ int sh_job, sh_nR, sh_done;
     const int lane = threadIdx.x;
     const int R = P.rmax;
     GMSlot S = gm_slot(dpool, ipool, blockIdx.x, R);
+    const HeapStore H{S.hkey, S.heap, lkey, lidx, NL < R ? NL : R};
     for (;;) {
         if (lane == 0) sh_job = atomicAdd(counter, 1);
         __syncthreads();
@@ -313,7 +335,7 @@ __global__ __launch_bounds__(NT) void gm_kernel(XiModelDev m, const GMJob* jobs,
         int nheap = 0, nregions = 1;
         double tv = 0, te = 0;
         unsigned numEval = GM_NPTS;
-        if (lane == 0) heap_push(S, nheap, 0, tv, te);
+        if (lane == 0) heap_push(S, H, nheap, 0, tv, te);
         const bool tr = trace && jb == 0;
         int iter = 0, status = 0;
         if (tr && lane == 0) {
@@ -334,7 +356,7 @@ __global__ __launch_bounds__(NT) void gm_kernel(XiModelDev m, const GMJob* jobs,
                 t[2] = nheap;
                 t[3] = nregions;
                 t[4] = numEval;
-                t[5] = S.err[S.heap[0]];
+                t[5] = S.err[H.idx(0)];
             }
             iter++;
             if (lane == 0) {
@@ -348,20 +370,27 @@ __global__ __launch_bounds__(NT) void gm_kernel(XiModelDev m, const GMJob* jobs,
                             done = 2;
                             break;
                         }
-                        int ri = heap_pop(S, nheap, tv, te);
+                        int ri = heap_pop(S, H, nheap, tv, te);
                         eee -= S.err[ri];
                         // cut along the split dimension (src/cubature.c:210-224)
                         int d = S.split[ri];
                         int r2 = nregions++;
+                        // the region's geometry through registers: all 12 loads
+                        // first, so they overlap, then the stores
+                        double cc[6], hh[6];
+#pragma unroll
                         for (int k = 0; k < 6; k++) {
-                            S.c[k * R + r2] = S.c[k * R + ri];
-                            S.hw[k * R + r2] = S.hw[k * R + ri];
+                            cc[k] = S.c[k * R + ri];
+                            hh[k] = S.hw[k * R + ri];
                         }
-                        double h = S.hw[d * R + ri] * 0.5;
+                        const double h = hh[d] * 0.5;
+#pragma unroll
+                        for (int k = 0; k < 6; k++) {
+                            S.c[k * R + r2] = k == d ? cc[k] + h : cc[k];
+                            S.hw[k * R + r2] = k == d ? h : hh[k];
+                        }
                         S.hw[d * R + ri] = h;
-                        S.hw[d * R + r2] = h;
-                        S.c[d * R + ri] -= h;
-                        S.c[d * R + r2] += h;
+                        S.c[d * R + ri] = cc[d] - h;
                         numEval += GM_NPTS * 2;
                         S.list[nR++] = ri;
                         S.list[nR++] = r2;
@@ -381,7 +410,7 @@ __global__ __launch_bounds__(NT) void gm_kernel(XiModelDev m, const GMJob* jobs,
             __threadfence_block();
             gm_eval_regions(m, xd, J, P, S, R, S.list, nR, vals);
             if (lane == 0)
-                for (int t = 0; t < nR; t++) heap_push(S, nheap, S.list[t], tv, te);
+                for (int t = 0; t < nR; t++) heap_push(S, H, nheap, S.list[t], tv, te);
         }
         if (lane == 0) {
             if (status == 2) {
@@ -389,8 +418,8 @@ __global__ __launch_bounds__(NT) void gm_kernel(XiModelDev m, const GMJob* jobs,
             } else {
                 double val = 0, err = 0;
                 for (int i = 0; i < nheap; i++) {
-                    val += S.val[S.heap[i]];
-                    err += S.err[S.heap[i]];
+                    val += S.val[H.idx(i)];
+                    err += S.err[H.idx(i)];
                 }
                 Q[(size_t)J.f * qstride + J.key] = val;
                 if (Qerr) Qerr[(size_t)J.f * qstride + J.key] = err;
@@ -576,6 +605,7 @@ __global__ __launch_bounds__(256) void gm_thread_kernel(XiModelDev m, const GMJo
         S.heap = ireg + GM_TR;
         S.list = ireg + 2 * GM_TR;
         const int R = GM_TR;
+        const HeapStore H{S.hkey, S.heap, nullptr, nullptr, 0};
         for (int d = 0; d < 6; d++) {
             S.c[d * R] = 0.5;
             S.hw[d * R] = 0.5;
@@ -584,7 +614,7 @@ __global__ __launch_bounds__(256) void gm_thread_kernel(XiModelDev m, const GMJo
         int nheap = 0, nregions = 1;
         double tv = 0, te = 0;
         unsigned numEval = GM_NPTS;
-        heap_push(S, nheap, 0, tv, te);
+        heap_push(S, H, nheap, 0, tv, te);
         bool overflow = false;
         for (;;) {
             if (!(numEval < P.maxEval || !P.maxEval)) break;
@@ -597,19 +627,24 @@ __global__ __launch_bounds__(256) void gm_thread_kernel(XiModelDev m, const GMJo
                     overflow = true;
                     break;
                 }
-                const int ri = heap_pop(S, nheap, tv, te);
+                const int ri = heap_pop(S, H, nheap, tv, te);
                 eee -= S.err[ri];
                 const int d = S.split[ri];
                 const int r2 = nregions++;
+                double cc[6], hh[6];
+#pragma unroll
                 for (int k = 0; k < 6; k++) {
-                    S.c[k * R + r2] = S.c[k * R + ri];
-                    S.hw[k * R + r2] = S.hw[k * R + ri];
+                    cc[k] = S.c[k * R + ri];
+                    hh[k] = S.hw[k * R + ri];
                 }
-                const double h = S.hw[d * R + ri] * 0.5;
+                const double h = hh[d] * 0.5;
+#pragma unroll
+                for (int k = 0; k < 6; k++) {
+                    S.c[k * R + r2] = k == d ? cc[k] + h : cc[k];
+                    S.hw[k * R + r2] = k == d ? h : hh[k];
+                }
                 S.hw[d * R + ri] = h;
-                S.hw[d * R + r2] = h;
-                S.c[d * R + ri] -= h;
-                S.c[d * R + r2] += h;
+                S.c[d * R + ri] = cc[d] - h;
                 numEval += GM_NPTS * 2;
                 S.list[nR++] = ri;
                 S.list[nR++] = r2;
@@ -617,15 +652,15 @@ __global__ __launch_bounds__(256) void gm_thread_kernel(XiModelDev m, const GMJo
             } while (nheap > 0 && (numEval < P.maxEval || !P.maxEval));
             if (overflow) break;
             for (int q = 0; q < nR; q++) gm_thread_region(m, xd, J, P, S, R, S.list[q]);
-            for (int q = 0; q < nR; q++) heap_push(S, nheap, S.list[q], tv, te);
+            for (int q = 0; q < nR; q++) heap_push(S, H, nheap, S.list[q], tv, te);
         }
         if (overflow) {
             deeper[atomicAdd(ndeeper, 1)] = jb;
         } else {
             double val = 0, err = 0;
             for (int i = 0; i < nheap; i++) {
-                val += S.val[S.heap[i]];
-                err += S.err[S.heap[i]];
+                val += S.val[H.idx(i)];
+                err += S.err[H.idx(i)];
             }
             Q[(size_t)J.f * qstride + J.key] = val;
             if (Qerr) Qerr[(size_t)J.f * qstride + J.key] = err;
@@ -971,19 +1006,19 @@ static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::v
             }
             if (nrun > 0) {
                 if (pass == 0)
-                    hipLaunchKernelGGL(gm_kernel<64>, dim3(std::min(nslots, nrun)), dim3(64), 0, ctx->stream, mv,
+                    hipLaunchKernelGGL((gm_kernel<64, 0>), dim3(std::min(nslots, nrun)), dim3(64), 0, ctx->stream, mv,
                                        (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                        nkeys, dQerr, (int*)sf.p, d_nev, g_gm_trace);
                 else if (nrun <= 2 * ctx->num_cus)   // few deep integrals: 8 waves each
-                    hipLaunchKernelGGL(gm_kernel<512>, dim3(std::min(nslots, nrun)), dim3(512), 0, ctx->stream, mv,
+                    hipLaunchKernelGGL((gm_kernel<512, 2048>), dim3(std::min(nslots, nrun)), dim3(512), 0, ctx->stream, mv,
                                        (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                        nkeys, dQerr, (int*)sf.p, d_nev, g_gm_trace);
                 else if (nrun <= 4 * ctx->num_cus)
-                    hipLaunchKernelGGL(gm_kernel<256>, dim3(std::min(nslots, nrun)), dim3(256), 0, ctx->stream, mv,
+                    hipLaunchKernelGGL((gm_kernel<256, 1024>), dim3(std::min(nslots, nrun)), dim3(256), 0, ctx->stream, mv,
                                        (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                        nkeys, dQerr, (int*)sf.p, d_nev, g_gm_trace);
                 else   // many: one wave each, all resident at once
-                    hipLaunchKernelGGL(gm_kernel<64>, dim3(std::min(nslots, nrun)), dim3(64), 0, ctx->stream, mv,
+                    hipLaunchKernelGGL((gm_kernel<64, 0>), dim3(std::min(nslots, nrun)), dim3(64), 0, ctx->stream, mv,
                                        (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                        nkeys, dQerr, (int*)sf.p, d_nev, g_gm_trace);
                 OPSEC_HIP_TRY(ctx, hipGetLastError());

@@ -68,6 +68,10 @@ struct MlpState {
     std::vector<struct SurBand> bands;
     struct SurBand* d_bands = nullptr;
     size_t cap_bands = 0;
+    // inference workspace (pair keys, functions, spot flags, features,
+    // outputs, baselines, failure count), kept across calls
+    void* d_inf = nullptr;
+    size_t cap_inf = 0;
 };
 
 constexpr int SUR_NK = 16;
@@ -669,28 +673,21 @@ int surrogate_tables(opsec_ctx* ctx, int nf_all, const int* funcs_all, const ops
     const long long nrows = (long long)far.size() * nf;
     int64_t fail = 0;
     if (nrows > 0) {
-        int *dkeys = nullptr, *dfidx = nullptr;
-        unsigned char* dspot = nullptr;
-        float *dx = nullptr, *dout = nullptr;
-        double* dbase = nullptr;
-        unsigned long long* dfail = nullptr;
-        auto release = [&]() {
-            hipFree(dkeys);
-            hipFree(dfidx);
-            hipFree(dspot);
-            hipFree(dx);
-            hipFree(dout);
-            hipFree(dbase);
-            hipFree(dfail);
-        };
-        if (hipMalloc(&dkeys, farkeys.size() * 4) != hipSuccess || hipMalloc(&dfidx, (size_t)nf * 4) != hipSuccess ||
-            hipMalloc(&dspot, (size_t)ctx->nkeys) != hipSuccess ||
-            hipMalloc(&dx, (size_t)nrows * MLP_IN * 4) != hipSuccess ||
-            hipMalloc(&dout, (size_t)nrows * 4) != hipSuccess || hipMalloc(&dbase, (size_t)nrows * 8) != hipSuccess ||
-            hipMalloc(&dfail, 8) != hipSuccess) {
-            release();
-            return set_error(ctx, OPSEC_ENOMEM, "surrogate workspace for %lld rows", nrows);
-        }
+        // one workspace, each part 256-byte aligned
+        size_t off[8];
+        const size_t sz[7] = {farkeys.size() * 4, (size_t)nf * 4, (size_t)ctx->nkeys, (size_t)nrows * MLP_IN * 4,
+                              (size_t)nrows * 4, (size_t)nrows * 8, 8};
+        off[0] = 0;
+        for (int i = 0; i < 7; i++) off[i + 1] = (off[i] + sz[i] + 255) & ~(size_t)255;
+        if ((rc = dev_realloc(ctx, &ctx->mlp->d_inf, &ctx->mlp->cap_inf, off[7]))) return rc;
+        char* ws = (char*)ctx->mlp->d_inf;
+        int* dkeys = (int*)(ws + off[0]);
+        int* dfidx = (int*)(ws + off[1]);
+        unsigned char* dspot = (unsigned char*)(ws + off[2]);
+        float* dx = (float*)(ws + off[3]);
+        float* dout = (float*)(ws + off[4]);
+        double* dbase = (double*)(ws + off[5]);
+        unsigned long long* dfail = (unsigned long long*)(ws + off[6]);
         hipStream_t st = ctx->stream;
         hipMemcpyAsync(dkeys, farkeys.data(), farkeys.size() * 4, hipMemcpyHostToDevice, st);
         hipMemcpyAsync(dfidx, funcs, (size_t)nf * 4, hipMemcpyHostToDevice, st);
@@ -708,7 +705,6 @@ int surrogate_tables(opsec_ctx* ctx, int nf_all, const int* funcs_all, const ops
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpyAsync(&hf, dfail, 8, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
-        release();
         if (e != hipSuccess) return set_error(ctx, OPSEC_EHIP, "surrogate inference: %s", hipGetErrorString(e));
         fail = (int64_t)hf;
     }
@@ -737,6 +733,7 @@ void opsec_internal_free_mlp(opsec_ctx* ctx) {
     hipFree(s->delta2);
     hipFree(s->out);
     hipFree(s->d_bands);
+    hipFree(s->d_inf);
     delete s;
     ctx->mlp = nullptr;
 }

@@ -672,24 +672,29 @@ __global__ __launch_bounds__(256) void gm_thread_kernel(XiModelDev m, const GMJo
     if (threadIdx.x == 0 && blk) atomicAdd(nevals, blk);
 }
 
-// ---- GAUSS: one thread per integral ----------------------------------------
+// ---- GAUSS: the tensor rule, a node slab per thread -------------------------
 
+// One thread per (integral, a): the b and c sums of x-node a; the a == 0
+// thread of each integral then adds the n slab sums in node order, so the
+// result is the plain a-outer triple sum.  256 / n integrals per workgroup.
 __global__ __launch_bounds__(256) void tent_kernel(XiModelDev m, const GaussJob* jobs, int njobs, TentRule rule,
                                                    double L, double* Q, size_t qstride) {
     __shared__ double s_s[64], s_w[64];
-    int n = rule.n;
+    __shared__ double slab[256];
+    const int n = rule.n;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         s_s[i] = rule.s[i];
         s_w[i] = rule.w[i];
     }
     __syncthreads();
-    int jb = blockIdx.x * blockDim.x + threadIdx.x;
-    if (jb >= njobs) return;
-    const GaussJob J = jobs[jb];
-    const XiDesc xd = m.desc[J.f];
-    double acc = 0;
-    for (int a = 0; a < n; a++) {
-        double rx = min_image(J.D[0] + s_s[a] * J.h[0], L);
+    const int per = blockDim.x / n;   // integrals per workgroup
+    const int local = threadIdx.x / n, a = threadIdx.x - local * n;
+    const long long jb = (long long)blockIdx.x * per + local;
+    const bool live = local < per && jb < njobs;
+    if (live) {
+        const GaussJob J = jobs[jb];
+        const XiDesc xd = m.desc[J.f];
+        const double rx = min_image(J.D[0] + s_s[a] * J.h[0], L);
         double accy = 0;
         for (int b = 0; b < n; b++) {
             double ry = min_image(J.D[1] + s_s[b] * J.h[1], L);
@@ -701,17 +706,25 @@ __global__ __launch_bounds__(256) void tent_kernel(XiModelDev m, const GaussJob*
             }
             accy += s_w[b] * accz;
         }
-        acc += s_w[a] * accy;
+        slab[threadIdx.x] = accy;
     }
-    Q[(size_t)J.f * qstride + J.key] = acc;
+    __syncthreads();
+    if (live && a == 0) {
+        double acc = 0;
+        for (int i = 0; i < n; i++) acc += s_w[i] * slab[threadIdx.x + i];
+        const GaussJob& J = jobs[jb];
+        Q[(size_t)J.f * qstride + J.key] = acc;
+    }
 }
 
-// Near pairs: composite rule, one 256-thread workgroup per integral.
+// Near pairs: composite rule.  Each integral is spread over gridDim.y
+// workgroups of 256 threads, one (a, b) node column per thread; a workgroup's
+// tree-reduced partial goes to part[job * gridDim.y + y].
 __global__ __launch_bounds__(256) void tent_near_kernel(XiModelDev m, const GaussJob* jobs, TentRule rule, double L,
-                                                        double* Q, size_t qstride) {
+                                                        double* part) {
     __shared__ double s_s[64], s_w[64];
     __shared__ double red[256];
-    int n = rule.n;
+    const int n = rule.n;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         s_s[i] = rule.s[i];
         s_w[i] = rule.w[i];
@@ -719,10 +732,10 @@ __global__ __launch_bounds__(256) void tent_near_kernel(XiModelDev m, const Gaus
     __syncthreads();
     const GaussJob J = jobs[blockIdx.x];
     const XiDesc xd = m.desc[J.f];
+    const int ab = blockIdx.y * blockDim.x + threadIdx.x;
     double acc = 0;
-    // thread owns (a, b) pairs; inner c loop
-    for (int ab = threadIdx.x; ab < n * n; ab += blockDim.x) {
-        int a = ab / n, b = ab % n;
+    if (ab < n * n) {
+        const int a = ab / n, b = ab % n;
         double rx = min_image(J.D[0] + s_s[a] * J.h[0], L);
         double ry = min_image(J.D[1] + s_s[b] * J.h[1], L);
         double rxy2 = rx * rx + ry * ry;
@@ -731,7 +744,7 @@ __global__ __launch_bounds__(256) void tent_near_kernel(XiModelDev m, const Gaus
             double rz = min_image(J.D[2] + s_s[c] * J.h[2], L);
             accz += s_w[c] * xi_eval(m, xd, sqrt(rxy2 + rz * rz));
         }
-        acc += s_w[a] * s_w[b] * accz;
+        acc = s_w[a] * s_w[b] * accz;
     }
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -739,7 +752,17 @@ __global__ __launch_bounds__(256) void tent_near_kernel(XiModelDev m, const Gaus
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) Q[(size_t)J.f * qstride + J.key] = red[0];
+    if (threadIdx.x == 0) part[(size_t)blockIdx.x * gridDim.y + blockIdx.y] = red[0];
+}
+
+// The near integrals: the nsplit partials of each, added in order.
+__global__ void tent_near_sum_kernel(const GaussJob* jobs, int njobs, int nsplit, const double* part, double* Q,
+                                     size_t qstride) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= njobs) return;
+    double acc = 0;
+    for (int i = 0; i < nsplit; i++) acc += part[(size_t)j * nsplit + i];
+    Q[(size_t)jobs[j].f * qstride + jobs[j].key] = acc;
 }
 
 // ---- first-pair search and job construction --------------------------------
@@ -1080,11 +1103,15 @@ static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::v
         }
         const size_t nfar = far.size() * nf, nnear = nearj.size() * nf, nj = nfar + nnear;
         const size_t ntmpl = far.size() + nearj.size();
-        if ((rc = dev_realloc(ctx, &ctx->d_jobs, &ctx->cap_jobs, sizeof(GaussJob) * (nj + ntmpl) + sizeof(int) * nf)))
+        // near partials after the jobs, templates and function list
+        const int nsplit = (nrule.n * nrule.n + 255) / 256;
+        const size_t part_off = (sizeof(GaussJob) * (nj + ntmpl) + sizeof(int) * nf + 255) & ~(size_t)255;
+        if ((rc = dev_realloc(ctx, &ctx->d_jobs, &ctx->cap_jobs, part_off + sizeof(double) * nnear * nsplit)))
             return rc;
         GaussJob* dj = (GaussJob*)ctx->d_jobs;
         GaussJob* dt = dj + nj;
         int* dfun = (int*)(dt + ntmpl);
+        double* dpart = (double*)((char*)ctx->d_jobs + part_off);
         if (!far.empty())
             OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dt, far.data(), sizeof(GaussJob) * far.size(), hipMemcpyHostToDevice,
                                               ctx->stream));
@@ -1099,13 +1126,16 @@ static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::v
             hipLaunchKernelGGL(expand_jobs_kernel, dim3((unsigned)((nnear + 255) / 256)), dim3(256), 0, ctx->stream,
                                dt + far.size(), (int)nearj.size(), dfun, nf, dj + nfar);
         if (nfar) {
-            hipLaunchKernelGGL(tent_kernel, dim3((unsigned)((nfar + 255) / 256)), dim3(256), 0, ctx->stream, mv, dj,
-                               (int)nfar, rule, ctx->sv.L, dQ, nkeys);
+            const size_t per = 256 / rule.n;
+            hipLaunchKernelGGL(tent_kernel, dim3((unsigned)((nfar + per - 1) / per)), dim3(256), 0, ctx->stream, mv,
+                               dj, (int)nfar, rule, ctx->sv.L, dQ, nkeys);
             OPSEC_HIP_TRY(ctx, hipGetLastError());
         }
         if (nnear) {
-            hipLaunchKernelGGL(tent_near_kernel, dim3((unsigned)nnear), dim3(256), 0, ctx->stream, mv, dj + nfar, nrule,
-                               ctx->sv.L, dQ, nkeys);
+            hipLaunchKernelGGL(tent_near_kernel, dim3((unsigned)nnear, (unsigned)nsplit), dim3(256), 0, ctx->stream, mv,
+                               dj + nfar, nrule, ctx->sv.L, dpart);
+            hipLaunchKernelGGL(tent_near_sum_kernel, dim3((unsigned)((nnear + 255) / 256)), dim3(256), 0, ctx->stream,
+                               dj + nfar, (int)nnear, nsplit, dpart, dQ, nkeys);
             OPSEC_HIP_TRY(ctx, hipGetLastError());
         }
         OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));

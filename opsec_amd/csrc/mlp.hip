@@ -167,6 +167,100 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const __bf16* __restri
     }
 }
 
+// Inference forward: the A fragments of the four hidden layers stay in
+// registers (each wave holds its 32 output rows of every layer) and each
+// workgroup runs 64-sample tiles in a loop, so the weights are read once per
+// workgroup.  Same operands and operation order as mlp_forward_kernel, so the
+// outputs are identical to the training forward's.
+__global__ __launch_bounds__(256) void mlp_infer_kernel(const __bf16* __restrict__ wb, const float* __restrict__ w32,
+                                                        const float* __restrict__ x, int n, float* __restrict__ out) {
+    __shared__ __bf16 act[2][FT][AP];
+    __shared__ float w4[MLP_H + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    bf8_t wf[4][4][2];    // [layer][k step of 32][row fragment]; layer 0 has one k step
+    float bs[4][2][4];    // bias of each accumulator row
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+        const int din = mlp_in_dim(l), nks = l == 0 ? 1 : MLP_H / 32;
+        const __bf16* W = wb + mlp_w_off(l);
+        const float* bias = w32 + mlp_w_off(l) + din * MLP_H;
+#pragma unroll
+        for (int ks = 0; ks < nks; ks++) {
+            const int kq = ks * 32 + 8 * (lane >> 4);
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const int o = wave * 32 + i * 16 + (lane & 15);
+                bf8_t t;
+#pragma unroll
+                for (int e = 0; e < 8; e++) t[e] = (kq + e < din) ? W[(size_t)o * din + kq + e] : (__bf16)0.f;
+                wf[l][ks][i] = t;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) bs[l][i][r] = bias[wave * 32 + i * 16 + 4 * (lane >> 4) + r];
+    }
+    for (int i = threadIdx.x; i <= MLP_H; i += 256) w4[i] = w32[mlp_w_off(4) + i];
+    const int ntiles = (n + FT - 1) / FT;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int s0 = tile * FT;
+        __syncthreads();   // the previous tile's output layer is done with act
+        for (int e = threadIdx.x; e < FT * 32; e += 256) {
+            int s = e / 32, k = e % 32;
+            float v = (k < MLP_IN && s0 + s < n) ? x[(size_t)(s0 + s) * MLP_IN + k] : 0.f;
+            act[0][s][k] = (__bf16)v;
+        }
+        __syncthreads();
+        int cur = 0;
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            const int nks = l == 0 ? 1 : MLP_H / 32;
+            f4_t acc[2][4];
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] = f4_t{0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < nks; ks++) {
+                const int kq = ks * 32 + 8 * (lane >> 4);
+                bf8_t bfr[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) bfr[j] = *(const bf8_t*)&act[cur][j * 16 + (lane & 15)][kq];
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[l][ks][i], bfr[j], acc[i][j], 0, 0, 0);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int o = wave * 32 + i * 16 + 4 * (lane >> 4) + r;
+                        const int s = j * 16 + (lane & 15);
+                        act[cur ^ 1][s][o] = (__bf16)silu(acc[i][j][r] + bs[l][i][r]);
+                    }
+            __syncthreads();
+            cur ^= 1;
+        }
+        if (threadIdx.x < FT) {
+            const int s = threadIdx.x;
+            float accv = w4[MLP_H];
+            for (int i = 0; i < MLP_H; i++) accv += w4[i] * (float)act[cur][s][i];
+            if (s0 + s < n) out[s0 + s] = accv;
+        }
+    }
+}
+
+// Workgroups for mlp_infer_kernel: two per CU at most, one per tile at least.
+static unsigned infer_blocks(long long n) {
+    return (unsigned)std::max(1LL, std::min((n + FT - 1) / FT, 2LL * 256));
+}
+
 // ---- generic BF16 MFMA GEMM for training: C (M x N, fp32) (+)= A (M x K) B (K x N)
 // Element (i, k) of A at a[i*ars + k*acs] (fp32 or bf16), same for B.
 template <typename TA, typename TB>
@@ -697,8 +791,8 @@ int surrogate_tables(opsec_ctx* ctx, int nf_all, const int* funcs_all, const ops
         const unsigned nblk = (unsigned)((nrows + 255) / 256);
         hipLaunchKernelGGL(sur_rows_kernel, dim3(nblk), dim3(256), 0, st, dkeys, dfidx, nrows, (int)far.size(),
                            ctx->mlp->d_bands, g, dx, dbase);
-        hipLaunchKernelGGL(mlp_forward_kernel<false>, dim3((unsigned)((nrows + FT - 1) / FT)), dim3(256), 0, st,
-                           ctx->mlp->wb, ctx->mlp->w, dx, (int)nrows, dout, (float*)nullptr, (__bf16*)nullptr, 0);
+        hipLaunchKernelGGL(mlp_infer_kernel, dim3(infer_blocks(nrows)), dim3(256), 0, st, ctx->mlp->wb, ctx->mlp->w,
+                           dx, (int)nrows, dout);
         hipLaunchKernelGGL(sur_combine_kernel, dim3(nblk), dim3(256), 0, st, dkeys, dfidx, nrows, (int)far.size(),
                            ctx->mlp->d_bands, dout, dbase, dspot, ctx->d_Q, (size_t)ctx->nkeys, q->spot_tol, dfail);
         unsigned long long hf = 0;
@@ -783,8 +877,8 @@ int opsec_mlp_forward(opsec_ctx* ctx, int n, const float* x, float* out) {
     OPSEC_HIP_TRY(ctx, hipMalloc(&dx, (size_t)n * MLP_IN * 4));
     OPSEC_HIP_TRY(ctx, hipMalloc(&dout, (size_t)n * 4));
     OPSEC_HIP_TRY(ctx, stream_copy(ctx, dx, x, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(mlp_forward_kernel<false>, dim3((n + FT - 1) / FT), dim3(256), 0, ctx->stream, ctx->mlp->wb,
-                       ctx->mlp->w, dx, n, dout, (float*)nullptr, (__bf16*)nullptr, 0);
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(infer_blocks(n)), dim3(256), 0, ctx->stream, ctx->mlp->wb, ctx->mlp->w,
+                       dx, n, dout);
     OPSEC_HIP_TRY(ctx, hipGetLastError());
     OPSEC_HIP_TRY(ctx, hipMemcpyAsync(out, dout, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
     OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));

@@ -41,6 +41,23 @@ def test_forward_vs_oracle(ctx):
     assert err < 3e-2
 
 
+@pytest.mark.parametrize("n", [1, 1000, 70001])
+def test_inference_equals_training_forward(ctx, n):
+    """The inference kernel (weights in registers, a loop over 64-sample
+    tiles) and the training forward run the same operations: with the
+    inference outputs as targets, every training residual is exactly zero.
+    70001 samples are 1094 tiles, more than the inference grid, so the tile
+    loop runs several times per workgroup."""
+    ctx.mlp_set_weights(rand_weights(5).astype(np.float32))
+    x = np.random.default_rng(6).uniform(-1, 1, size=(n, 8)).astype(np.float32)
+    y = ctx.mlp_forward(x)
+    assert np.all(np.isfinite(y)) and np.abs(y).max() > 0
+    loss, _ = ctx.mlp_loss_grad(x, y)
+    assert loss == 0.0
+    loss1, _ = ctx.mlp_loss_grad(x, y + np.float32(1e-3))
+    assert loss1 > 0
+
+
 def test_gradient_vs_oracle(ctx):
     w = rand_weights(3, 0.5)
     ctx.mlp_set_weights(w.astype(np.float32))

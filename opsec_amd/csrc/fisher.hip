@@ -63,7 +63,6 @@ __global__ __launch_bounds__(256) void fisher_gram_kernel(const double* D, long 
     constexpr bool fp32 = sizeof(T) == 4;
     __shared__ T X[FG][FK_TILE + 4];     // scaled rows
     __shared__ T Dt[FG][FK_TILE + 4];    // plain rows
-    __shared__ double wk[FK_TILE], rk[FK_TILE], yk[FK_TILE];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     acc_t acc[NF][NF];
     double acc64[NF][NF][4];
@@ -72,11 +71,30 @@ __global__ __launch_bounds__(256) void fisher_gram_kernel(const double* D, long 
             acc[i][j] = acc_t{0, 0, 0, 0};
             for (int r = 0; r < 4; r++) acc64[i][j][r] = 0.0;
         }
+    __shared__ double sp[FG];   // prior band powers
+    for (int n = threadIdx.x; n < FG; n += blockDim.x) sp[n] = n < nb ? prior[n] : 0.0;
+    // padding rows never change: zero them once
+    for (int e = threadIdx.x; e < FG * FK_TILE; e += blockDim.x) {
+        const int row = e / FK_TILE, t = e % FK_TILE;
+        if (row >= nb) Dt[row][t] = (T)0;
+        if (row >= nb + 2) X[row][t] = (T)0;
+    }
     for (long long k0 = (long long)blockIdx.x * FK_TILE; k0 < K; k0 += (long long)gridDim.x * FK_TILE) {
-        __syncthreads();
-        // weights, R diagonal part and y_i y_j per packed entry
-        for (int t = threadIdx.x; t < FK_TILE; t += blockDim.x) {
-            long long k = k0 + t;
+        __syncthreads();   // the previous tile's MFMAs are done with X and Dt
+        // one thread per packed entry: its weight, the rows of D (coalesced
+        // along k, loads independent of each other), the residual
+        // R = delta_ij C_i - sum_n p_n D_n in FP64 from the FP64 inputs, and
+        // the scaled rows
+        if (threadIdx.x < FK_TILE) {
+            const int t = threadIdx.x;
+            const long long k = k0 + t;
+            // all FG - 2 row loads are issued first (rows at and beyond nb
+            // load row nb - 1 and count as 0), so one tile costs one memory
+            // latency
+            const long long kc = k < K ? k : K - 1;
+            double v[FG - 2];
+#pragma unroll
+            for (int n = 0; n < FG - 2; n++) v[n] = D[(size_t)min(n, nb - 1) * K + kc];
             double w = 0, r = 0, yy = 0;
             if (k < K) {
                 int j = packed_col(k);
@@ -86,39 +104,16 @@ __global__ __launch_bounds__(256) void fisher_gram_kernel(const double* D, long 
                 r = (i == j) ? ci : 0.0;
                 yy = y[i] * y[j];
             }
-            wk[t] = w;
-            rk[t] = r;
-            yk[t] = yy;
-        }
-        // load D rows (coalesced along k)
-        for (int e = threadIdx.x; e < nb * FK_TILE; e += blockDim.x) {
-            int row = e / FK_TILE, t = e % FK_TILE;
-            long long k = k0 + t;
-            double v = (k < K) ? D[(size_t)row * K + k] : 0.0;
-            Dt[row][t] = (T)v;
-        }
-        for (int e = threadIdx.x; e < (FG - nb) * FK_TILE; e += blockDim.x) {
-            int row = nb + e / FK_TILE, t = e % FK_TILE;
-            Dt[row][t] = (T)0;
-        }
-        __syncthreads();
-        // residual R = delta_ij C_i - sum_n p_n D_n (in FP64 from the FP64 inputs)
-        for (int t = threadIdx.x; t < FK_TILE; t += blockDim.x) {
-            long long k = k0 + t;
-            double r = rk[t];
-            if (k < K)
-                for (int n = 0; n < nb; n++) r -= prior[n] * D[(size_t)n * K + k];
-            rk[t] = r;
-        }
-        __syncthreads();
-        for (int e = threadIdx.x; e < FG * FK_TILE; e += blockDim.x) {
-            int row = e / FK_TILE, t = e % FK_TILE;
-            double v;
-            if (row < nb) v = wk[t] * (double)Dt[row][t];
-            else if (row == nb) v = wk[t] * rk[t];
-            else if (row == nb + 1) v = wk[t] * yk[t];
-            else v = 0.0;
-            X[row][t] = (T)v;
+#pragma unroll
+            for (int n = 0; n < FG - 2; n++) {
+                const double vn = (n < nb && k < K) ? v[n] : 0.0;
+                const T vt = (T)vn;
+                Dt[n][t] = vt;
+                X[n][t] = (T)(w * (double)vt);
+                r -= sp[n] * vn;
+            }
+            X[nb][t] = (T)(w * r);
+            X[nb + 1][t] = (T)(w * yy);
         }
         __syncthreads();
         // each wave takes FK_TILE / 4 entries = FK_TILE / 16 k-steps of 4

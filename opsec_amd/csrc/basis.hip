@@ -305,10 +305,10 @@ __global__ void healpix_kernel(long long nside, int nest, long long n, const dou
     pix[i] = nest ? hp_ang2pix_nest(nside, theta[i], phi[i]) : hp_ang2pix_ring(nside, theta[i], phi[i]);
 }
 
-struct DevScratch {
+struct GrowBuf {
     void* p = nullptr;
     size_t cap = 0;
-    ~DevScratch() { dev_free(&p); }
+    ~GrowBuf() { dev_free(&p); }
 };
 
 inline double cube(double x) { return x * x * x; }
@@ -381,7 +381,7 @@ int opsec_basis(opsec_ctx* ctx, const opsec_survey* sv, const opsec_selection* s
             }
 
     // selection tables on the device
-    DevScratch srad, smask, sjob, sout, sfail, sdp, sip, sctr;
+    GrowBuf srad, smask, sjob, sout, sfail, sdp, sip, sctr;
     SelDev sd{};
     sd.kind = sel->kind;
     sd.nest = sel->nest;
@@ -473,7 +473,7 @@ int opsec_healpix_pixels(opsec_ctx* ctx, int64_t nside, int nest, int64_t n, con
     OPSEC_CHECK_ARG(ctx, n >= 0 && (n == 0 || (theta && phi && pix)), "NULL arrays");
     if (n == 0) return OPSEC_OK;
     OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
-    DevScratch sb;
+    GrowBuf sb;
     int rc;
     if ((rc = dev_realloc(ctx, &sb.p, &sb.cap, (size_t)n * 24))) return rc;
     double* dt = (double*)sb.p;

@@ -154,6 +154,26 @@ int set_error(opsec_ctx* ctx, int code, const char* fmt, ...);
 int dev_realloc(opsec_ctx* ctx, void** p, size_t* cap, size_t bytes);
 void dev_free(void** p);
 
+// Device buffers owned by one call: freed when the scope ends, error returns
+// included (hipFree waits for the work that uses them).
+struct DevScratch {
+    std::vector<void*> bufs;
+    DevScratch() = default;
+    DevScratch(const DevScratch&) = delete;
+    DevScratch& operator=(const DevScratch&) = delete;
+    ~DevScratch() {
+        for (void* q : bufs) (void)hipFree(q);
+    }
+    template <typename T>
+    hipError_t alloc(T** out, size_t bytes) {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, bytes);
+        *out = e == hipSuccess ? (T*)q : nullptr;
+        if (e == hipSuccess) bufs.push_back(q);
+        return e;
+    }
+};
+
 // Model view for kernels.
 XiModelDev model_view(const opsec_ctx* ctx);
 

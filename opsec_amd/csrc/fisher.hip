@@ -344,11 +344,11 @@ int opsec_estimate_run(opsec_ctx* ctx, int nmodes, int nparams, const double* la
     OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
     size_t bytes = (size_t)nparams * packed_size(nmodes) * 8;
     double* dD = nullptr;
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dD, bytes));
+    DevScratch tmp;
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dD, bytes));
     OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dD, D, bytes, hipMemcpyHostToDevice, ctx->stream));
     int rc = run_fisher(ctx, dD, nmodes, nparams, lambda, y, prior, mixing, prec, out);
     hipStreamSynchronize(ctx->stream);
-    hipFree(dD);
     return rc;
 }
 

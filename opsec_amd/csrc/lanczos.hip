@@ -484,10 +484,11 @@ static int dense_eigen_top(Dev& D, double* dA, int K, int nev, std::vector<doubl
     nev = std::min(nev, K);
     std::vector<double> d(K), e(std::max(1, K - 1), 0.0);
     double *dtau = nullptr, *de = nullptr, *dp = nullptr, *dd = nullptr;
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dtau, (size_t)K * 8 + 8));
-    OPSEC_HIP_TRY(ctx, hipMalloc(&de, (size_t)K * 8 + 8));
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dp, (size_t)std::max(K, nev) * 8 + 8));
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dd, (size_t)K * 8 + 8));
+    DevScratch tmp;
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dtau, (size_t)K * 8 + 8));
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&de, (size_t)K * 8 + 8));
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dp, (size_t)std::max(K, nev) * 8 + 8));
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dd, (size_t)K * 8 + 8));
     for (int k = 0; k + 2 < K; k++) {
         const int m = K - k - 1;
         hipLaunchKernelGGL(hh_vector_kernel, dim3(1), dim3(1024), 0, D.s, dA, K, k, dtau, de);
@@ -510,12 +511,7 @@ static int dense_eigen_top(Dev& D, double* dA, int K, int nev, std::vector<doubl
         hipLaunchKernelGGL(hh_apply_upd_kernel, dim3((m + 255) / 256, nev), dim3(256), 0, D.s, dA, K, k, dtau, dZ,
                            nev, dp);
     }
-    rc = D.sync();
-    hipFree(dtau);
-    hipFree(de);
-    hipFree(dp);
-    hipFree(dd);
-    return rc;
+    return D.sync();
 }
 
 // ---- block orthonormalisation (SVQB) -------------------------------------------
@@ -794,11 +790,10 @@ int opsec_lanczos_dense(opsec_ctx* ctx, int n, const double* A, int nev, double 
     OPSEC_CHECK_ARG(ctx, tol > 0, "tolerance must be positive");
     OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
     double* dS = nullptr;
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dS, (size_t)n * n * 8));
+    DevScratch tmp;
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dS, (size_t)n * n * 8));
     OPSEC_HIP_TRY(ctx, stream_copy(ctx, dS, A, (size_t)n * n * 8, hipMemcpyHostToDevice));
-    int rc = block_lanczos(ctx, dS, n, nev, tol, block, maxdim, evals, evecs, nconv);
-    hipFree(dS);
-    return rc;
+    return block_lanczos(ctx, dS, n, nev, tol, block, maxdim, evals, evecs, nconv);
 }
 
 int opsec_klt(opsec_ctx* ctx, int nev, double tol, int block, int maxdim, double* evals, double* evecs, int* nconv) {
@@ -811,7 +806,8 @@ int opsec_klt(opsec_ctx* ctx, int nev, double tol, int block, int maxdim, double
     OPSEC_CHECK_ARG(ctx, tol > 0, "tolerance must be positive");
     OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
     double* dS = nullptr;
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dS, (size_t)N * N * 8));
+    DevScratch tmp;
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dS, (size_t)N * N * 8));
     int rc = opsec_assemble_device(ctx, 0, dS, N);
     if (!rc) rc = block_lanczos(ctx, dS, N, nev, tol, block, maxdim, evals, evecs, nconv);
     if (maxdim <= 0) {
@@ -825,7 +821,6 @@ int opsec_klt(opsec_ctx* ctx, int nev, double tol, int block, int maxdim, double
             rc = block_lanczos(ctx, dS, N, nev, tol, block, md, evals, evecs, nconv);
         }
     }
-    hipFree(dS);
     return rc;
 }
 

@@ -409,11 +409,13 @@ static int ensure_mlp(opsec_ctx* ctx) {
 static int ensure_batch(opsec_ctx* ctx, int B) {
     MlpState* s = ctx->mlp;
     if (s->cap_batch >= B) return OPSEC_OK;
-    hipFree(s->z);
-    hipFree(s->a);
-    hipFree(s->delta);
-    hipFree(s->delta2);
-    hipFree(s->out);
+    // free and forget the old buffers first, so a failed allocation below
+    // leaves no stale pointer for the context's destructor
+    for (void** p : {(void**)&s->z, (void**)&s->a, (void**)&s->delta, (void**)&s->delta2, (void**)&s->out}) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    s->cap_batch = 0;
     OPSEC_HIP_TRY(ctx, hipMalloc(&s->z, (size_t)4 * B * MLP_H * 4));
     OPSEC_HIP_TRY(ctx, hipMalloc(&s->a, (size_t)5 * B * MLP_H * 2));
     OPSEC_HIP_TRY(ctx, hipMalloc(&s->delta, (size_t)B * MLP_H * 4));
@@ -874,16 +876,15 @@ int opsec_mlp_forward(opsec_ctx* ctx, int n, const float* x, float* out) {
     int rc = ensure_mlp(ctx);
     if (rc || n == 0) return rc;
     float *dx = nullptr, *dout = nullptr;
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dx, (size_t)n * MLP_IN * 4));
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dout, (size_t)n * 4));
+    DevScratch tmp;
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dx, (size_t)n * MLP_IN * 4));
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dout, (size_t)n * 4));
     OPSEC_HIP_TRY(ctx, stream_copy(ctx, dx, x, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(mlp_infer_kernel, dim3(infer_blocks(n)), dim3(256), 0, ctx->stream, ctx->mlp->wb, ctx->mlp->w,
                        dx, n, dout);
     OPSEC_HIP_TRY(ctx, hipGetLastError());
     OPSEC_HIP_TRY(ctx, hipMemcpyAsync(out, dout, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
     OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    hipFree(dx);
-    hipFree(dout);
     return OPSEC_OK;
 }
 
@@ -894,9 +895,10 @@ int opsec_mlp_loss_grad(opsec_ctx* ctx, int n, const float* x, const float* t, d
     if ((rc = ensure_batch(ctx, n))) return rc;
     float *dx = nullptr, *dt = nullptr;
     double* dl = nullptr;
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dx, (size_t)n * MLP_IN * 4));
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dt, (size_t)n * 4));
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dl, 8));
+    DevScratch tmp;
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dx, (size_t)n * MLP_IN * 4));
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dt, (size_t)n * 4));
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dl, 8));
     OPSEC_HIP_TRY(ctx, stream_copy(ctx, dx, x, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice));
     OPSEC_HIP_TRY(ctx, stream_copy(ctx, dt, t, (size_t)n * 4, hipMemcpyHostToDevice));
     OPSEC_HIP_TRY(ctx, hipMemsetAsync(dl, 0, 8, ctx->stream));
@@ -907,9 +909,6 @@ int opsec_mlp_loss_grad(opsec_ctx* ctx, int n, const float* x, const float* t, d
     OPSEC_HIP_TRY(ctx, stream_copy(ctx, &l, dl, 8, hipMemcpyDeviceToHost));
     OPSEC_HIP_TRY(ctx, stream_copy(ctx, grad, ctx->mlp->g, MLP_NW * 4, hipMemcpyDeviceToHost));
     *loss = l / n;
-    hipFree(dx);
-    hipFree(dt);
-    hipFree(dl);
     return OPSEC_OK;
 }
 
@@ -960,7 +959,8 @@ int opsec_mlp_train(opsec_ctx* ctx, const opsec_mlp_train_opts* o, const opsec_q
     opsec_quad qq = *q;
     if (qq.method == OPSEC_QUAD_SURROGATE) qq.method = OPSEC_QUAD_GAUSS;
     double* dQs = nullptr;
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dQs, (size_t)ctx->funcs.size() * ctx->nkeys * 8));
+    DevScratch tmp;
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dQs, (size_t)ctx->funcs.size() * ctx->nkeys * 8));
     int64_t integrals = 0;
     for (int f = 1; f <= nb; f++) {
         std::vector<int>& ks = keys_of[f];
@@ -974,14 +974,12 @@ int opsec_mlp_train(opsec_ctx* ctx, const opsec_mlp_train_opts* o, const opsec_q
             geo.push_back(grid_pair_geo(sv, kx, ky, kz));
         }
         if ((rc = run_quadrature_public(ctx, 1, &f, ks, geo, nearflag, &qq, dQs, ctx->nkeys))) {
-            hipFree(dQs);
             return rc;
         }
         integrals += (int64_t)ks.size();
     }
     std::vector<double> Qall((size_t)ctx->funcs.size() * ctx->nkeys);
     OPSEC_HIP_TRY(ctx, stream_copy(ctx, Qall.data(), dQs, Qall.size() * 8, hipMemcpyDeviceToHost));
-    hipFree(dQs);
     // features and isotropic baseline of the samples (the inference kernel)
     if ((rc = prepare_bands(ctx, h))) return rc;
     std::vector<float> X((size_t)ns * MLP_IN);
@@ -1029,9 +1027,10 @@ int opsec_mlp_train(opsec_ctx* ctx, const opsec_mlp_train_opts* o, const opsec_q
     // device copies
     float *dX = nullptr, *dT = nullptr;
     double* dl = nullptr;
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dX, (size_t)ns * MLP_IN * 4));
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dT, (size_t)ns * 4));
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dl, 8));
+    DevScratch dev;
+    OPSEC_HIP_TRY(ctx, dev.alloc(&dX, (size_t)ns * MLP_IN * 4));
+    OPSEC_HIP_TRY(ctx, dev.alloc(&dT, (size_t)ns * 4));
+    OPSEC_HIP_TRY(ctx, dev.alloc(&dl, 8));
     const int B = std::min(o->batch, ns);
     if ((rc = ensure_batch(ctx, B))) return rc;
     MlpState* s = ctx->mlp;
@@ -1067,9 +1066,6 @@ int opsec_mlp_train(opsec_ctx* ctx, const opsec_mlp_train_opts* o, const opsec_q
         OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         last = l / ((double)nbatch * B);
     }
-    hipFree(dX);
-    hipFree(dT);
-    hipFree(dl);
     s->trained = true;
     if (final_loss) *final_loss = last;
     return OPSEC_OK;

@@ -1553,11 +1553,11 @@ int opsec_project(opsec_ctx* ctx, int nf, const int* funcs, int prec, double* P_
     }
     OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
     int* d_funcs = nullptr;
-    OPSEC_HIP_TRY(ctx, hipMalloc(&d_funcs, nf * sizeof(int)));
+    DevScratch tmp;
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&d_funcs, nf * sizeof(int)));
     OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_funcs, funcs, nf * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     int rc = run_project(ctx, nf, d_funcs, funcs, prec);
     hipStreamSynchronize(ctx->stream);
-    hipFree(d_funcs);
     if (rc) return rc;
     ctx->proj->funcs.assign(funcs, funcs + nf);
     if (P_packed) {
@@ -1653,10 +1653,11 @@ int opsec_project_dense(opsec_ctx* ctx, int n, const double* S, int nmodes, cons
     double *dS = nullptr, *dB = nullptr, *dP = nullptr;
     void* dC = nullptr;
     size_t esize = prec == OPSEC_PREC_FP64 ? 8 : 4;
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dS, (size_t)N * N * 8));
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dB, (size_t)N * M * 8));
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dC, (size_t)N * M * esize));
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dP, (size_t)packed_size(M) * 8));
+    DevScratch tmp;
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dS, (size_t)N * N * 8));
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dB, (size_t)N * M * 8));
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dC, (size_t)N * M * esize));
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dP, (size_t)packed_size(M) * 8));
     OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dS, S, (size_t)N * N * 8, hipMemcpyHostToDevice, ctx->stream));
     OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dB, B, (size_t)N * M * 8, hipMemcpyHostToDevice, ctx->stream));
     GA_RowMajor A;
@@ -1687,9 +1688,10 @@ int opsec_project_dense(opsec_ctx* ctx, int n, const double* S, int nmodes, cons
     } else {
         const size_t Kp = round_up(N, 256), Mp = round_up(M, 256), Pp = plane_pitch(Kp);
         __bf16 *Sp = nullptr, *Bp = nullptr, *Cp = nullptr;
-        OPSEC_HIP_TRY(ctx, hipMalloc(&Sp, 2 * Kp * Pp * sizeof(__bf16)));
-        OPSEC_HIP_TRY(ctx, hipMalloc(&Bp, 2 * Mp * Pp * sizeof(__bf16)));
-        OPSEC_HIP_TRY(ctx, hipMalloc(&Cp, 2 * Mp * Pp * sizeof(__bf16)));
+        DevScratch planes;
+        OPSEC_HIP_TRY(ctx, planes.alloc(&Sp, 2 * Kp * Pp * sizeof(__bf16)));
+        OPSEC_HIP_TRY(ctx, planes.alloc(&Bp, 2 * Mp * Pp * sizeof(__bf16)));
+        OPSEC_HIP_TRY(ctx, planes.alloc(&Cp, 2 * Mp * Pp * sizeof(__bf16)));
         int rc2 = split_planes(ctx, dS, (size_t)N, N, N, Sp, (int)Kp, Pp);
         // B is row-major nmodes x n here: row = mode, cells contiguous
         if (!rc2) rc2 = split_planes(ctx, dB, (size_t)N, M, N, Bp, (int)Mp, Pp);
@@ -1698,14 +1700,7 @@ int opsec_project_dense(opsec_ctx* ctx, int n, const double* S, int nmodes, cons
             launch_nt(ctx, Bp, (int)Mp, Cp, (int)Mp, Pp, (int)Kp, EpiPacked{dP, M});
         }
         hipStreamSynchronize(ctx->stream);
-        hipFree(Sp);
-        hipFree(Bp);
-        hipFree(Cp);
         if (rc2) {
-            hipFree(dS);
-            hipFree(dB);
-            hipFree(dC);
-            hipFree(dP);
             return rc2;
         }
     }
@@ -1718,10 +1713,6 @@ int opsec_project_dense(opsec_ctx* ctx, int n, const double* S, int nmodes, cons
         if (e2 != hipSuccess || e3 != hipSuccess)
             rc = set_error(ctx, OPSEC_EHIP, "projection copy-back: %s", hipGetErrorString(e2 ? e2 : e3));
     }
-    hipFree(dS);
-    hipFree(dB);
-    hipFree(dC);
-    hipFree(dP);
     return rc;
 }
 

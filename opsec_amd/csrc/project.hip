@@ -824,13 +824,33 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void g
 
 constexpr size_t G3_LDS_BYTES = (size_t)8 * G3_STAGE * sizeof(__bf16);   // 128 KB
 
-// out[i] = sum_s part[s][i] in FP64, in a fixed order (split-K reduction of stage 2)
+// out[i] = sum_s part[s][i] in FP64, in a fixed order (split-K reduction of stage 2).
+// Four entries per thread; with up to 8 parts and n a multiple of 4 all part
+// loads are 16-byte and issued before the sums.
+constexpr int SUM_PARTS_MAX = 8;
 __global__ void sum_parts_kernel(const float* __restrict__ part, int nparts, size_t n, double* __restrict__ out) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double acc = 0.0;
-    for (int s = 0; s < nparts; s++) acc += (double)part[(size_t)s * n + i];
-    out[i] = acc;
+    const size_t i0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (i0 >= n) return;
+    if ((n & 3) == 0 && nparts <= SUM_PARTS_MAX) {
+        f4_t v[SUM_PARTS_MAX];
+#pragma unroll
+        for (int s = 0; s < SUM_PARTS_MAX; s++)
+            if (s < nparts) v[s] = *(const f4_t*)&part[(size_t)s * n + i0];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int s = 0; s < SUM_PARTS_MAX; s++)
+                if (s < nparts) acc += (double)v[s][q];
+            out[i0 + q] = acc;
+        }
+        return;
+    }
+    for (size_t i = i0; i < i0 + 4 && i < n; i++) {
+        double acc = 0.0;
+        for (int s = 0; s < nparts; s++) acc += (double)part[(size_t)s * n + i];
+        out[i] = acc;
+    }
 }
 
 // B' = diag(w) B with w = sqrt(Nbar)
@@ -1191,7 +1211,8 @@ static int run_project_bf16x3(opsec_ctx* ctx, int nb, const int* funcs) {
             EpiPackedParts toP{ps->d_Ppart, M, np};
             launch_g3(ctx, ASrcPlanes{ps->d_Bp, ps->d_Bp + Mp * Pp, Pp}, (int)Mp, ps->d_Cp, (int)Mp, Pp, (int)Kp, toP,
                       s2, 1, 0, 0, 0, 0, 0, -1, g_s2_group);
-            hipLaunchKernelGGL(sum_parts_kernel, blocks_for(np), dim3(256), 0, ctx->stream, ps->d_Ppart, s2, np, Pz);
+            hipLaunchKernelGGL(sum_parts_kernel, blocks_for((np + 3) / 4), dim3(256), 0, ctx->stream, ps->d_Ppart, s2, np,
+                               Pz);
         } else {
             EpiPacked toP{Pz, M};
             launch_nt(ctx, ps->d_Bp, (int)Mp, ps->d_Cp, (int)Mp, Pp, (int)Kp, toP);
@@ -1356,7 +1377,7 @@ static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
                   0, cg_pair, 0, 0, Mp * Pg, -1, g_s2_group);
         hipEventRecord(evg[2 * gi], ctx->stream);
         for (int q = 0; q < g; q++)
-            hipLaunchKernelGGL(sum_parts_kernel, blocks_for(np), dim3(256), 0, ctx->stream,
+            hipLaunchKernelGGL(sum_parts_kernel, blocks_for((np + 3) / 4), dim3(256), 0, ctx->stream,
                                ps->d_Ppart + (size_t)q * s2 * np, s2, np, (double*)ps->d_P + (size_t)(z0 + q) * np);
         hipEventRecord(evg[2 * gi + 1], ctx->stream);
     }

@@ -1017,9 +1017,22 @@ __global__ __launch_bounds__(256) void idft_kernel(const float* __restrict__ yha
     const int m = blockIdx.y, r0 = blockIdx.x * 128, t = threadIdx.x;
     const float* ym = yhat + (size_t)m * nbat * G;
     if ((G & 3) == 0) {
-        for (int i = t; i < nbat * 32; i += 256) {
+        // every load of the thread is issued before the first LDS store
+        // (nbat = 2 Nx <= 128: at most 16 per thread)
+        constexpr int MAXL = 16;
+        const int nl = (nbat * 32 + 255) / 256;
+        f4_t v[MAXL];
+#pragma unroll
+        for (int u = 0; u < MAXL; u++) {
+            const int i = t + u * 256;
             const int b = i >> 5, r = (i & 31) * 4;
-            *(f4_t*)&sY[b * 128 + r] = (r0 + r < G) ? *(const f4_t*)&ym[(size_t)b * G + r0 + r] : f4_t{0, 0, 0, 0};
+            v[u] = (u < nl && i < nbat * 32 && r0 + r < G) ? *(const f4_t*)&ym[(size_t)b * G + r0 + r]
+                                                           : f4_t{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int u = 0; u < MAXL; u++) {
+            const int i = t + u * 256;
+            if (u < nl && i < nbat * 32) *(f4_t*)&sY[(i >> 5) * 128 + (i & 31) * 4] = v[u];
         }
     } else {
         for (int i = t; i < nbat * 128; i += 256) {

@@ -23,12 +23,13 @@
 namespace opsec {
 
 // NF 16-row fragments per side: the padded Gram size is 16 NF (48 or 64).
-// Packed entries per workgroup iteration: 128 for 48, 48 for 64 (LDS stays
-// below 64 KB in FP64).
-template <int NF>
+// Packed entries per workgroup iteration (one per thread while loading): 256
+// for FP32 traces at 48, 128 for FP64 at 48, 48 for 64 (LDS stays within
+// 100 KB).
+template <typename T, int NF>
 struct FShape {
     static constexpr int FG = 16 * NF;
-    static constexpr int KT = NF <= 3 ? 128 : 48;
+    static constexpr int KT = NF <= 3 ? (sizeof(T) == 4 ? 256 : 128) : 48;
 };
 constexpr int FG_MAX = 64;
 
@@ -57,7 +58,7 @@ template <typename T, int NF>
 __global__ __launch_bounds__(256) void fisher_gram_kernel(const double* D, long long K, int nb, const double* lambda,
                                                           const double* y, const double* prior, double* G) {
     typedef typename FMfma<T>::acc_t acc_t;
-    constexpr int FG = FShape<NF>::FG, FK_TILE = FShape<NF>::KT;
+    constexpr int FG = FShape<T, NF>::FG, FK_TILE = FShape<T, NF>::KT;
     // FP32 MFMA partial sums are closed at the end of every tile and carried
     // in FP64 registers, so each FP32 accumulation chain is FK_TILE / 4 terms
     constexpr bool fp32 = sizeof(T) == 4;
@@ -289,8 +290,11 @@ static int run_fisher(opsec_ctx* ctx, const double* dD, int M, int nb, const dou
                     "unknown precision %d", prec);
     OPSEC_CHECK_ARG(ctx, lambda && y && prior && out, "NULL input");
     long long K = (long long)M * (M + 1) / 2;
-    const bool big = nb + 2 > FShape<3>::FG;
-    const int FG = big ? FShape<4>::FG : FShape<3>::FG, KT = big ? FShape<4>::KT : FShape<3>::KT;
+    const bool big = nb + 2 > FShape<double, 3>::FG;
+    const bool f64 = prec == OPSEC_PREC_FP64;
+    const int FG = big ? FShape<double, 4>::FG : FShape<double, 3>::FG;
+    const int KT = f64 ? (big ? FShape<double, 4>::KT : FShape<double, 3>::KT)
+                       : (big ? FShape<float, 4>::KT : FShape<float, 3>::KT);
     struct DevBufs {
         double* aux = nullptr;
         double* G = nullptr;

@@ -959,14 +959,17 @@ __global__ void that_kernel(const double* __restrict__ qhat, int G, int n3, __bf
     if (c0 >= Gp) return;
     const double* qk = qhat + (size_t)k * G;
     const int yr = r / n3, zr = r - yr * n3;
+    // (y, z) of column c0, then stepped along the 8 columns
+    int yc = (int)(c0 / n3), zc = (int)(c0 - (size_t)yc * n3);
     bf8_t h, l;
 #pragma unroll
     for (int q = 0; q < 8; q++) {
         const int c = (int)(c0 + q);
         float v = 0.f;
-        if (r < G && c < G) {
-            const int yc = c / n3, zc = c - yc * n3;
-            v = (float)qk[abs(yr - yc) * n3 + abs(zr - zc)];
+        if (r < G && c < G) v = (float)qk[abs(yr - yc) * n3 + abs(zr - zc)];
+        if (++zc == n3) {
+            zc = 0;
+            yc++;
         }
         __bf16 x, y;
         split2(v, x, y);

@@ -94,9 +94,9 @@ def run_path(ctx, L, funcs, quad, prec, trace_prec, lam, y, prior, steps, warmup
         est = ctx.estimate_device(lam, y, prior, L.OPSEC_MIX_INVERSE, trace_prec)
         ctx.sync()
         dt = time.perf_counter() - t0
-        ctx.lib.opsec_internal_project_times(ctx.h, C.byref(s1), C.byref(s2))
-        ctx.lib.opsec_internal_project_kernel_ms(ctx.h, C.byref(kg), C.byref(k1), C.byref(k2), C.byref(kn))
-        ctx.lib.opsec_internal_project_launches(ctx.h, C.byref(l1), C.byref(l2))
+        ctx.lib.opsec_diag_project_times(ctx.h, C.byref(s1), C.byref(s2))
+        ctx.lib.opsec_diag_project_kernel_ms(ctx.h, C.byref(kg), C.byref(k1), C.byref(k2), C.byref(kn))
+        ctx.lib.opsec_diag_project_launches(ctx.h, C.byref(l1), C.byref(l2))
         if it >= warmup:
             times.append(dt)
             qms.append(tq)
@@ -410,11 +410,11 @@ def main():
     funcs = list(range(nb + 1))
 
     ctx = L.Context(0)
-    ctx.lib.opsec_internal_set_nt_tile(args.nt_tile)
-    ctx.lib.opsec_internal_set_nt_order(args.nt_group, args.nt_remap)
-    ctx.lib.opsec_internal_set_nt_gen(args.nt_gen)
-    ctx.lib.opsec_internal_set_stage2_split(args.s2_split)
-    ctx.lib.opsec_internal_set_stage2_group(args.s2_group)
+    ctx.lib.opsec_diag_set_nt_tile(args.nt_tile)
+    ctx.lib.opsec_diag_set_nt_order(args.nt_group, args.nt_remap)
+    ctx.lib.opsec_diag_set_nt_gen(args.nt_gen)
+    ctx.lib.opsec_diag_set_stage2_split(args.s2_split)
+    ctx.lib.opsec_diag_set_stage2_group(args.s2_group)
     ctx.set_projection(L.OPSEC_PROJ_TOEPLITZ if args.projection == "toeplitz" else L.OPSEC_PROJ_DENSE)
     ncu, arch = ctx.device_info()
     log(f"device {arch} with {ncu} CUs; workload {args.config}: {N} cells, {nb} bands, {M} modes")
@@ -531,7 +531,7 @@ def main():
     # against the device's measured FP64 sincos rate (band nodes need two pairs)
     import ctypes as C
     sc = C.c_double()
-    ctx.check(ctx.lib.opsec_internal_sincos_rate(ctx.h, C.byref(sc)), "sincos probe")
+    ctx.check(ctx.lib.opsec_diag_sincos_rate(ctx.h, C.byref(sc)), "sincos probe")
     out["quadrature"] = {"sincos_pairs_per_s": sc.value}
     out["assembly"] = assembly
     for name in ("fp64", "surrogate"):
@@ -622,7 +622,7 @@ def main():
         ms = C.c_float()
         best = None
         for _ in range(3):
-            ctx.check(ctx.lib.opsec_internal_copy(ctx.h, C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes),
+            ctx.check(ctx.lib.opsec_diag_copy(ctx.h, C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes),
                                                   C.byref(ms)), "copy")
             best = ms.value if best is None else min(best, ms.value)
         out["copy_calibration"] = {"bytes": nbytes, "ms": best, "GB_per_s": 2 * nbytes / (best / 1e3) / 1e9,

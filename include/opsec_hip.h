@@ -180,11 +180,15 @@ int opsec_d2h(opsec_ctx* ctx, void* dst, const void* src, size_t bytes);
  * number of modes whose residual |S v - lambda v| <= tol |lambda_max|.
  * block <= 0 picks the default block size; maxdim caps the Krylov dimension
  * (maxdim <= 0: 3 nev + 4 block, doubled up to twice, to at most Ncells,
- * while some of the nev modes have not converged). */
+ * while some of the nev modes have not converged; a retry's result replaces
+ * the earlier one only if it converged more modes).  A Krylov space that
+ * would span all but one block of the space is replaced by the dense
+ * symmetric eigen-solver on S itself (same answer, all modes converged). */
 int opsec_klt(opsec_ctx* ctx, int nev, double tol, int block, int maxdim,
               double* evals, double* evecs, int* nconv);
-/* Same solver on an arbitrary symmetric host matrix A (n x n row-major);
- * used for the reference's eigen known answers (tests/testeig.cpp:78-121). */
+/* Same solver (and the same automatic Krylov dimension) on an arbitrary
+ * symmetric host matrix A (n x n row-major); used for the reference's eigen
+ * known answers (tests/testeig.cpp:78-121). */
 int opsec_lanczos_dense(opsec_ctx* ctx, int n, const double* A, int nev, double tol,
                         int block, int maxdim, double* evals, double* evecs, int* nconv);
 
@@ -320,8 +324,10 @@ typedef struct opsec_mlp_train_opts {
 
 int opsec_mlp_defaults(opsec_mlp_train_opts* o);
 /* Generate exact training integrals on the device survey with quadrature q,
- * then train the 8 -> 128^4 -> 1 SiLU network on the GPU. final_loss gets the
- * last epoch's mean squared error (normalised units). */
+ * then train the 8 -> 128^4 -> 1 SiLU network on the GPU, starting from the
+ * initial weights of o->seed (every reduction runs in a fixed order, so the
+ * same options on the same survey give the same network bit for bit).
+ * final_loss gets the last epoch's mean squared error (normalised units). */
 int opsec_mlp_train(opsec_ctx* ctx, const opsec_mlp_train_opts* o, const opsec_quad* q,
                     double* final_loss);
 /* Raw network access for gradient checks: nparams_total weights as FP32. */

@@ -1,0 +1,52 @@
+/* opsec_hip_diag.h -- diagnostics and tuning knobs of libopsec_hip.
+ *
+ * Not part of the drop-in boundary (opsec_hip.h): none of these has a
+ * counterpart in the reference.  The benchmark uses them to time kernels and
+ * to calibrate HBM counters, and the tests use them to run every GEMM
+ * variant and the host tridiagonal eigen-solver.  The tuning setters change
+ * process-wide defaults; leave them at their defaults in production.
+ */
+#ifndef OPSEC_HIP_DIAG_H
+#define OPSEC_HIP_DIAG_H
+
+#include <stddef.h>
+
+#include "opsec_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Device-to-device copy of `bytes` (a multiple of 16) by a plain 16-byte copy
+ * kernel on the context stream; *ms = its duration.  The calibration kernel
+ * for the HBM byte counters. */
+int opsec_diag_copy(opsec_ctx* ctx, void* dst, const void* src, size_t bytes, float* ms);
+/* Throughput of a kernel doing nothing but FP64 sin/cos pairs (pairs/s): the
+ * ceiling against which the quadrature's node rate is read. */
+int opsec_diag_sincos_rate(opsec_ctx* ctx, double* pairs_per_s);
+/* Top-nev eigenpairs of the symmetric tridiagonal (d[n], e[n-1]) by the
+ * solver's host QL + inverse iteration; Z is n x nev column-major. */
+int opsec_diag_tridiag_top(int n, const double* d, const double* e, int nev, double* evals, double* Z);
+
+/* Timings of the last opsec_project call (ms): stage 1 and stage 2 totals;
+ * for the split-BF16x3 paths also plane generation / DFTs, the stage-1 and
+ * stage-2 GEMMs and the launch counts. */
+int opsec_diag_project_times(opsec_ctx* ctx, float* stage1, float* stage2);
+int opsec_diag_project_kernel_ms(opsec_ctx* ctx, float* gen, float* gemm1, float* gemm2, int* launches);
+int opsec_diag_project_launches(opsec_ctx* ctx, int* stage1, int* stage2);
+
+/* Split-BF16x3 GEMM variants: tile (0: 128 x 128, 1: 256 x 128, 2: 256 x 256
+ * double-buffered, the default), stage-1 A operand generated from the pair
+ * table (1) or read from S_p planes (0, default), stage-2 split-K factor
+ * (default 8) and tile-group edge (default 2), stage-1 tile order (group
+ * edge, XCD remap). */
+int opsec_diag_set_nt_tile(int v);
+int opsec_diag_set_nt_gen(int v);
+int opsec_diag_set_stage2_split(int v);
+int opsec_diag_set_stage2_group(int v);
+int opsec_diag_set_nt_order(int group, int remap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OPSEC_HIP_DIAG_H */

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "opsec_hip.h"
+#include "opsec_hip_diag.h"
 
 namespace opsec {
 
@@ -201,6 +202,13 @@ int run_quadrature_public(opsec_ctx* ctx, int nf, const int* funcs, const std::v
 // Geometry of key (kx, ky, kz) of the device survey's grid: the grid cell at
 // the origin and the cell at offset k, as Survey::CreateEmptyCell lays them out.
 PairGeo grid_pair_geo(const SurveyDev& sv, int kx, int ky, int kz);
+
+// Per-unit state teardown (project.hip, mlp.hip, dot.hip) and the mode
+// invalidation a new cell upload triggers (project.hip).
+void free_proj(opsec_ctx* ctx);
+void free_mlp(opsec_ctx* ctx);
+void free_dot(opsec_ctx* ctx);
+void invalidate_modes(opsec_ctx* ctx);
 
 // Rows of the packed upper triangle: column j starts at j(j+1)/2.
 inline int64_t packed_size(int64_t m) { return m * (m + 1) / 2; }

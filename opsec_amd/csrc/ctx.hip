@@ -365,10 +365,6 @@ int opsec_ctx_create(int device, opsec_ctx** out) {
     return OPSEC_OK;
 }
 
-void opsec_internal_free_proj(opsec_ctx* ctx);
-void opsec_internal_free_mlp(opsec_ctx* ctx);
-void opsec_internal_free_dot(opsec_ctx* ctx);
-
 int opsec_ctx_destroy(opsec_ctx* ctx) {
     if (!ctx) return OPSEC_OK;
     hipSetDevice(ctx->device);
@@ -379,9 +375,9 @@ int opsec_ctx_destroy(opsec_ctx* ctx) {
     dev_free((void**)&ctx->d_Q);
     dev_free((void**)&ctx->d_B);
     dev_free(&ctx->d_jobs);
-    opsec_internal_free_proj(ctx);
-    opsec_internal_free_mlp(ctx);
-    opsec_internal_free_dot(ctx);
+    free_proj(ctx);
+    free_mlp(ctx);
+    free_dot(ctx);
     for (void* p : ctx->user_bufs) hipFree(p);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -430,7 +426,7 @@ int opsec_dev_free(opsec_ctx* ctx, void* dptr) {
 // buffers on the context's stream, timed with HIP events: the reference
 // kernel whose FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic
 // figures of the other kernels.
-int opsec_internal_copy(opsec_ctx* ctx, void* dst, const void* src, size_t bytes, float* ms) {
+int opsec_diag_copy(opsec_ctx* ctx, void* dst, const void* src, size_t bytes, float* ms) {
     if (!ctx || !dst || !src || bytes % 16) return OPSEC_EINVAL;
     hipEvent_t e0, e1;
     OPSEC_HIP_TRY(ctx, hipEventCreate(&e0));
@@ -450,7 +446,7 @@ int opsec_internal_copy(opsec_ctx* ctx, void* dst, const void* src, size_t bytes
 }
 
 // Measured ceiling of FP64 (sin, cos) pairs per second on the whole device.
-int opsec_internal_sincos_rate(opsec_ctx* ctx, double* pairs_per_s) {
+int opsec_diag_sincos_rate(opsec_ctx* ctx, double* pairs_per_s) {
     if (!ctx || !pairs_per_s) return OPSEC_EINVAL;
     const unsigned blocks = (unsigned)ctx->num_cus * 16;
     const int iters = 2000;

@@ -165,13 +165,8 @@ int ensure_dot(opsec_ctx* ctx) {
 }
 
 }  // namespace
-}  // namespace opsec
 
-using namespace opsec;
-
-extern "C" {
-
-void opsec_internal_free_dot(opsec_ctx* ctx) {
+void free_dot(opsec_ctx* ctx) {
     DotState* d = ctx->dot;
     if (!d) return;
     dev_free((void**)&d->d_map);
@@ -187,6 +182,13 @@ void opsec_internal_free_dot(opsec_ctx* ctx) {
     delete d;
     ctx->dot = nullptr;
 }
+
+}  // namespace opsec
+
+using namespace opsec;
+
+extern "C" {
+
 
 int opsec_count_galaxies(opsec_ctx* ctx, const opsec_galaxy* gals, int64_t ngals, int accumulate, double* counts) {
     if (!ctx) return OPSEC_EINVAL;

@@ -8,10 +8,11 @@
 //   q_n  = sum_k w_k D_n[k] y_i y_j
 // i.e. one Gram product G = X D^T with X = [w D_0 .. w D_{nb-1}, w R, w Y]
 // (nb + 2 rows, padded to 48, or to 64 for nb > 46) and D^T (padded likewise),
-// run on MFMA (FP64 16x16x4, or FP32 16x16x4 for the FP32 traces) with
-// workgroup partial sums reduced by atomics.  The 48 x 48 (64 x 64) result is
-// tiny; the inverse / matrix square root / window normalisation on nb x nb
-// runs on the host in FP64.
+// run on MFMA (FP64 16x16x4, or FP32 16x16x4 for the FP32 traces).  Each
+// workgroup writes its partial Gram matrix; a second kernel adds the partials
+// in workgroup order, so F, f and q are the same bits on every run.  The
+// 48 x 48 (64 x 64) result is tiny; the inverse / matrix square root / window
+// normalisation on nb x nb runs on the host in FP64.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -32,6 +33,13 @@ struct FShape {
     static constexpr int KT = NF <= 3 ? (sizeof(T) == 4 ? 256 : 128) : 48;
 };
 constexpr int FG_MAX = 64;
+// Partial Gram matrices: one per workgroup, at most FISHER_MAX_PARTS
+// workgroups (with one 100 KB-LDS workgroup resident per CU, two per CU keep
+// every CU busy through the grid-stride loop).
+constexpr int FISHER_MAX_PARTS = 512;
+inline int fisher_parts(const opsec_ctx* ctx, long long tiles) {
+    return (int)std::min<long long>(tiles, std::min(FISHER_MAX_PARTS, 2 * std::max(1, ctx->num_cus)));
+}
 
 template <typename T>
 struct FMfma;
@@ -56,7 +64,7 @@ __device__ __forceinline__ int packed_col(long long k) {
 
 template <typename T, int NF>
 __global__ __launch_bounds__(256) void fisher_gram_kernel(const double* D, long long K, int nb, const double* lambda,
-                                                          const double* y, const double* prior, double* G) {
+                                                          const double* y, const double* prior, double* Gpart) {
     typedef typename FMfma<T>::acc_t acc_t;
     constexpr int FG = FShape<T, NF>::FG, FK_TILE = FShape<T, NF>::KT;
     // FP32 MFMA partial sums are closed at the end of every tile and carried
@@ -134,14 +142,34 @@ __global__ __launch_bounds__(256) void fisher_gram_kernel(const double* D, long 
                     acc[i][j] = acc_t{0, 0, 0, 0};
                 }
     }
-    for (int i = 0; i < NF; i++)
-        for (int j = 0; j < NF; j++)
-            for (int r = 0; r < 4; r++) {
-                int row = i * 16 + acc_row<T>(lane, r), col = j * 16 + (lane & 15);
-                const double v = fp32 ? acc64[i][j][r] : (double)acc[i][j][r];
-                atomicAdd(&G[row * FG + col], v);
-            }
+    // every wave holds the full FG x FG product of its quarter of the tile's
+    // entries: the four waves' sums go through LDS (waves in order), then the
+    // workgroup's partial is written with plain stores
+    __shared__ double red[FG * FG];
+    double* out = Gpart + (size_t)blockIdx.x * FG * FG;
+    for (int w = 0; w < 4; w++) {
+        if (wave == w)
+            for (int i = 0; i < NF; i++)
+                for (int j = 0; j < NF; j++)
+                    for (int r = 0; r < 4; r++) {
+                        const int row = i * 16 + acc_row<T>(lane, r), col = j * 16 + (lane & 15);
+                        const double v = fp32 ? acc64[i][j][r] : (double)acc[i][j][r];
+                        red[row * FG + col] = w == 0 ? v : red[row * FG + col] + v;
+                    }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < FG * FG; e += blockDim.x) out[e] = red[e];
 }
+
+// G[e] = sum over the partials in workgroup order (FP64, fixed order).
+__global__ void fisher_sum_parts_kernel(const double* __restrict__ Gpart, int nparts, int FG, double* __restrict__ G) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= FG * FG) return;
+    double acc = 0.0;
+    for (int s = 0; s < nparts; s++) acc += Gpart[(size_t)s * FG * FG + e];
+    G[e] = acc;
+}
+
 
 // ---- host FP64 dense helpers for nb x nb ----------------------------------
 
@@ -281,50 +309,68 @@ static int post_process(opsec_ctx* ctx, int nb, const std::vector<double>& G, in
     return OPSEC_OK;
 }
 
-static int run_fisher(opsec_ctx* ctx, const double* dD, int M, int nb, const double* lambda, const double* y,
-                      const double* prior, int mixing, int prec, opsec_estimate* out) {
-    OPSEC_CHECK_ARG(ctx, nb >= 1 && nb + 2 <= FG_MAX, "Nparams must be 1..%d", FG_MAX - 2);
-    OPSEC_CHECK_ARG(ctx, M >= 1, "Nmodes must be positive");
-    OPSEC_CHECK_ARG(ctx, mixing >= 0 && mixing <= 2, "unknown mixing mode %d", mixing);
-    OPSEC_CHECK_ARG(ctx, prec == OPSEC_PREC_FP64 || prec == OPSEC_PREC_FP32 || prec == OPSEC_PREC_BF16X3,
-                    "unknown precision %d", prec);
-    OPSEC_CHECK_ARG(ctx, lambda && y && prior && out, "NULL input");
-    long long K = (long long)M * (M + 1) / 2;
+// Padded Gram size and packed entries per tile for nb parameters.
+static void gram_shape(int nb, int prec, int* FG, int* KT) {
     const bool big = nb + 2 > FShape<double, 3>::FG;
     const bool f64 = prec == OPSEC_PREC_FP64;
-    const int FG = big ? FShape<double, 4>::FG : FShape<double, 3>::FG;
-    const int KT = f64 ? (big ? FShape<double, 4>::KT : FShape<double, 3>::KT)
-                       : (big ? FShape<float, 4>::KT : FShape<float, 3>::KT);
-    struct DevBufs {
-        double* aux = nullptr;
-        double* G = nullptr;
-        ~DevBufs() {
-            if (aux) (void)hipFree(aux);
-            if (G) (void)hipFree(G);
-        }
-    } b;
-    OPSEC_HIP_TRY(ctx, hipMalloc(&b.aux, ((size_t)2 * M + nb) * 8));
-    OPSEC_HIP_TRY(ctx, hipMalloc(&b.G, (size_t)FG * FG * 8));
-    double* d_aux = b.aux;
-    double* d_G = b.G;
+    *FG = big ? FShape<double, 4>::FG : FShape<double, 3>::FG;
+    *KT = f64 ? (big ? FShape<double, 4>::KT : FShape<double, 3>::KT)
+              : (big ? FShape<float, 4>::KT : FShape<float, 3>::KT);
+}
+
+// Launch the Gram product of the nb packed matrices dD (M modes) into the
+// padded FG x FG matrix d_G: `nparts` workgroup partials in d_parts (nparts x
+// FG x FG doubles, every entry written), then their fixed-order sum.  Device
+// inputs; nothing is allocated.
+static void launch_gram(opsec_ctx* ctx, const double* dD, int M, int nb, const double* d_lambda, const double* d_y,
+                        const double* d_prior, int prec, int nparts, double* d_parts, double* d_G) {
+    const long long K = (long long)M * (M + 1) / 2;
+    int FG, KT;
+    gram_shape(nb, prec, &FG, &KT);
+    const bool big = FG > FShape<double, 3>::FG;
+    if (prec == OPSEC_PREC_FP64 && !big)
+        hipLaunchKernelGGL((fisher_gram_kernel<double, 3>), dim3(nparts), dim3(256), 0, ctx->stream, dD, K, nb,
+                           d_lambda, d_y, d_prior, d_parts);
+    else if (prec == OPSEC_PREC_FP64)
+        hipLaunchKernelGGL((fisher_gram_kernel<double, 4>), dim3(nparts), dim3(256), 0, ctx->stream, dD, K, nb,
+                           d_lambda, d_y, d_prior, d_parts);
+    else if (!big)
+        hipLaunchKernelGGL((fisher_gram_kernel<float, 3>), dim3(nparts), dim3(256), 0, ctx->stream, dD, K, nb,
+                           d_lambda, d_y, d_prior, d_parts);
+    else
+        hipLaunchKernelGGL((fisher_gram_kernel<float, 4>), dim3(nparts), dim3(256), 0, ctx->stream, dD, K, nb,
+                           d_lambda, d_y, d_prior, d_parts);
+    hipLaunchKernelGGL(fisher_sum_parts_kernel, dim3((FG * FG + 255) / 256), dim3(256), 0, ctx->stream, d_parts,
+                       nparts, FG, d_G);
+}
+
+static int check_gram_args(opsec_ctx* ctx, int M, int nb, int prec) {
+    OPSEC_CHECK_ARG(ctx, nb >= 1 && nb + 2 <= FG_MAX, "Nparams must be 1..%d", FG_MAX - 2);
+    OPSEC_CHECK_ARG(ctx, M >= 1, "Nmodes must be positive");
+    OPSEC_CHECK_ARG(ctx, prec == OPSEC_PREC_FP64 || prec == OPSEC_PREC_FP32 || prec == OPSEC_PREC_BF16X3,
+                    "unknown precision %d", prec);
+    return OPSEC_OK;
+}
+
+static int run_fisher(opsec_ctx* ctx, const double* dD, int M, int nb, const double* lambda, const double* y,
+                      const double* prior, int mixing, int prec, opsec_estimate* out) {
+    int rc = check_gram_args(ctx, M, nb, prec);
+    if (rc) return rc;
+    OPSEC_CHECK_ARG(ctx, mixing >= 0 && mixing <= 2, "unknown mixing mode %d", mixing);
+    OPSEC_CHECK_ARG(ctx, lambda && y && prior && out, "NULL input");
+    const long long K = (long long)M * (M + 1) / 2;
+    int FG, KT;
+    gram_shape(nb, prec, &FG, &KT);
+    const int nparts = fisher_parts(ctx, (K + KT - 1) / KT);
+    double *d_aux = nullptr, *d_G = nullptr, *d_parts = nullptr;
+    DevScratch tmp;
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&d_aux, ((size_t)2 * M + nb) * 8));
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&d_G, (size_t)FG * FG * 8));
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&d_parts, (size_t)nparts * FG * FG * 8));
     OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_aux, lambda, M * 8, hipMemcpyHostToDevice, ctx->stream));
     OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_aux + M, y, M * 8, hipMemcpyHostToDevice, ctx->stream));
     OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_aux + 2 * M, prior, nb * 8, hipMemcpyHostToDevice, ctx->stream));
-    OPSEC_HIP_TRY(ctx, hipMemsetAsync(d_G, 0, (size_t)FG * FG * 8, ctx->stream));
-    long long tiles = (K + KT - 1) / KT;
-    int grid = (int)std::min<long long>(tiles, (long long)ctx->num_cus * 8);
-    if (prec == OPSEC_PREC_FP64 && !big)
-        hipLaunchKernelGGL((fisher_gram_kernel<double, 3>), dim3(grid), dim3(256), 0, ctx->stream, dD, K, nb, d_aux,
-                           d_aux + M, d_aux + 2 * M, d_G);
-    else if (prec == OPSEC_PREC_FP64)
-        hipLaunchKernelGGL((fisher_gram_kernel<double, 4>), dim3(grid), dim3(256), 0, ctx->stream, dD, K, nb, d_aux,
-                           d_aux + M, d_aux + 2 * M, d_G);
-    else if (!big)
-        hipLaunchKernelGGL((fisher_gram_kernel<float, 3>), dim3(grid), dim3(256), 0, ctx->stream, dD, K, nb, d_aux,
-                           d_aux + M, d_aux + 2 * M, d_G);
-    else
-        hipLaunchKernelGGL((fisher_gram_kernel<float, 4>), dim3(grid), dim3(256), 0, ctx->stream, dD, K, nb, d_aux,
-                           d_aux + M, d_aux + 2 * M, d_G);
+    launch_gram(ctx, dD, M, nb, d_aux, d_aux + M, d_aux + 2 * M, prec, nparts, d_parts, d_G);
     OPSEC_HIP_TRY(ctx, hipGetLastError());
     std::vector<double> G((size_t)FG * FG);
     OPSEC_HIP_TRY(ctx, hipMemcpyAsync(G.data(), d_G, (size_t)FG * FG * 8, hipMemcpyDeviceToHost, ctx->stream));

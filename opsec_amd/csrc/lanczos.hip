@@ -54,8 +54,10 @@ struct LPut {
     }
 };
 
+// With split-K (gridDim.z > 1) part z of the K range is written to
+// parts[(z N + n) M + m] and summed in order by lsum_parts_kernel.
 template <typename SrcA, typename SrcB, typename Out>
-__global__ __launch_bounds__(GT_THREADS) void lgemm_kernel(SrcA A, SrcB B, Out O, int M, int N, int K) {
+__global__ __launch_bounds__(GT_THREADS) void lgemm_kernel(SrcA A, SrcB B, Out O, int M, int N, int K, double* parts) {
     __shared__ double As[16][GT_M + 4];
     __shared__ double Bs[16][GT_N + 4];
     const int m0 = blockIdx.x * GT_M, n0 = blockIdx.y * GT_N;
@@ -107,9 +109,18 @@ __global__ __launch_bounds__(GT_THREADS) void lgemm_kernel(SrcA A, SrcB B, Out O
             for (int r = 0; r < 4; r++) {
                 const int m = m0 + wm + i * 16 + acc_row<double>(lane, r);
                 if (gridDim.z == 1) O.put(0, m, n, M, N, acc[i][j][r]);
-                else if (m < M && n < N) atomicAdd(&O.p[(size_t)n * O.ld + m], acc[i][j][r]);
+                else if (m < M && n < N) parts[((size_t)blockIdx.z * N + n) * M + m] = acc[i][j][r];
             }
         }
+}
+
+// H[n ld + m] = sum_z parts[(z N + n) M + m], z in order.
+__global__ void lsum_parts_kernel(const double* __restrict__ parts, int nparts, int M, int N, double* H, size_t ld) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)M * N) return;
+    double acc = 0.0;
+    for (int z = 0; z < nparts; z++) acc += parts[(size_t)z * M * N + i];
+    H[(i / M) * ld + i % M] = acc;
 }
 
 __global__ void fill_kernel(double* p, size_t n, double v) {
@@ -121,6 +132,13 @@ struct Dev {
     opsec_ctx* ctx;
     hipStream_t s;
     int err = 0;
+    // split-K partials of gemm_tn: at most 2 x CUs workgroups of 128 x 128
+    double* parts = nullptr;
+    DevScratch mem;
+    Dev(opsec_ctx* c, hipStream_t st) : ctx(c), s(st) {}
+    hipError_t init() {
+        return mem.alloc(&parts, (size_t)2 * std::max(1, ctx->num_cus) * GT_M * GT_N * 8);
+    }
     void fill(double* p, size_t n, double v) {
         if (n == 0) return;
         hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
@@ -134,11 +152,15 @@ struct Dev {
         B.p = Y;
         B.ld = ldy;
         int gx = (p + GT_M - 1) / GT_M, gy = (q + GT_N - 1) / GT_N;
+        // split-K fills the CUs when the output has few tiles; the partials
+        // (splits x p x q <= 2 CUs x 128 x 128 doubles) are summed in order
         int splits = std::max(1, std::min(64, (2 * ctx->num_cus) / std::max(1, gx * gy)));
         splits = std::min(splits, std::max(1, N / 512));
-        if (splits > 1) fill(H, (size_t)p * q, 0.0);
         hipLaunchKernelGGL((lgemm_kernel<LA_KCol, LA_KCol, LPut<0>>), dim3(gx, gy, splits), dim3(GT_THREADS), 0, s, A,
-                           B, LPut<0>{H, (size_t)p}, p, q, N);
+                           B, LPut<0>{H, (size_t)p}, p, q, N, parts);
+        if (splits > 1)
+            hipLaunchKernelGGL(lsum_parts_kernel, dim3((unsigned)(((size_t)p * q + 255) / 256)), dim3(256), 0, s, parts,
+                               splits, p, q, H, (size_t)p);
     }
     // Y (N x q) op= X (N x p) H (p x q); op: 0 store, 1 subtract
     void gemm_nn(int mode, double* Y, size_t ldy, const double* X, size_t ldx, const double* H, size_t ldh, int N,
@@ -152,10 +174,10 @@ struct Dev {
         dim3 g((N + GT_M - 1) / GT_M, (q + GT_N - 1) / GT_N, 1);
         if (mode == 0)
             hipLaunchKernelGGL((lgemm_kernel<LA_ColMajor, LA_KCol, LPut<0>>), g, dim3(GT_THREADS), 0, s, A, B,
-                               LPut<0>{Y, ldy}, N, q, p);
+                               LPut<0>{Y, ldy}, N, q, p, nullptr);
         else
             hipLaunchKernelGGL((lgemm_kernel<LA_ColMajor, LA_KCol, LPut<1>>), g, dim3(GT_THREADS), 0, s, A, B,
-                               LPut<1>{Y, ldy}, N, q, p);
+                               LPut<1>{Y, ldy}, N, q, p, nullptr);
     }
     // Y (N x q) = S X, S dense row-major N x N (symmetric)
     void apply(const double* S, int N, const double* X, int q, double* Y) {
@@ -167,7 +189,7 @@ struct Dev {
         B.ld = N;
         dim3 g((N + GT_M - 1) / GT_M, (q + GT_N - 1) / GT_N, 1);
         hipLaunchKernelGGL((lgemm_kernel<LA_RowMajor, LA_KCol, LPut<0>>), g, dim3(GT_THREADS), 0, s, A, B,
-                           LPut<0>{Y, (size_t)N}, N, q, N);
+                           LPut<0>{Y, (size_t)N}, N, q, N, nullptr);
     }
     int sync() {
         hipError_t e = hipGetLastError();
@@ -215,8 +237,9 @@ __global__ __launch_bounds__(1024) void hh_vector_kernel(double* A, int K, int k
     if (threadIdx.x == 0) x[0] = 1.0;
 }
 
-// p[i] += sum_{j in tile} A22[i][j] v[j]   (A22 = A[k+1:, k+1:], v = A[k+1:, k])
-__global__ __launch_bounds__(256) void hh_symv_kernel(const double* A, int K, int k, double* p) {
+// part[c][i] = sum_{j in chunk c} A22[i][j] v[j]   (A22 = A[k+1:, k+1:], v = A[k+1:, k]);
+// chunk c = blockIdx.y covers columns 256 c .. 256 c + 255
+__global__ __launch_bounds__(256) void hh_symv_kernel(const double* A, int K, int k, double* part) {
     __shared__ double vs[256];
     const int m = K - k - 1;
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -229,7 +252,17 @@ __global__ __launch_bounds__(256) void hh_symv_kernel(const double* A, int K, in
     const int jn = min(256, m - j0);
     const double* col = A + (size_t)(k + 1 + j0) * K + (k + 1) + i;
     for (int jj = 0; jj < jn; jj++) acc += col[(size_t)jj * K] * vs[jj];
-    atomicAdd(&p[i], acc);
+    part[(size_t)blockIdx.y * m + i] = acc;
+}
+
+// p[i] = sum_c part[c][i], chunks in order
+__global__ __launch_bounds__(256) void hh_symv_sum_kernel(const double* __restrict__ part, int nc, int m,
+                                                          double* __restrict__ p) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    double acc = 0.0;
+    for (int c = 0; c < nc; c++) acc += part[(size_t)c * m + i];
+    p[i] = acc;
 }
 
 // w = tau p - (tau/2)(tau p . v) v  (one workgroup)
@@ -483,8 +516,9 @@ static int dense_eigen_top(Dev& D, double* dA, int K, int nev, std::vector<doubl
     opsec_ctx* ctx = D.ctx;
     nev = std::min(nev, K);
     std::vector<double> d(K), e(std::max(1, K - 1), 0.0);
-    double *dtau = nullptr, *de = nullptr, *dp = nullptr, *dd = nullptr;
+    double *dtau = nullptr, *de = nullptr, *dp = nullptr, *dd = nullptr, *dpart = nullptr;
     DevScratch tmp;
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dpart, (size_t)((K + 255) / 256) * K * 8 + 8));
     OPSEC_HIP_TRY(ctx, tmp.alloc(&dtau, (size_t)K * 8 + 8));
     OPSEC_HIP_TRY(ctx, tmp.alloc(&de, (size_t)K * 8 + 8));
     OPSEC_HIP_TRY(ctx, tmp.alloc(&dp, (size_t)std::max(K, nev) * 8 + 8));
@@ -492,8 +526,9 @@ static int dense_eigen_top(Dev& D, double* dA, int K, int nev, std::vector<doubl
     for (int k = 0; k + 2 < K; k++) {
         const int m = K - k - 1;
         hipLaunchKernelGGL(hh_vector_kernel, dim3(1), dim3(1024), 0, D.s, dA, K, k, dtau, de);
-        D.fill(dp, m, 0.0);
-        hipLaunchKernelGGL(hh_symv_kernel, dim3((m + 255) / 256, (m + 255) / 256), dim3(256), 0, D.s, dA, K, k, dp);
+        const int nc = (m + 255) / 256;
+        hipLaunchKernelGGL(hh_symv_kernel, dim3(nc, nc), dim3(256), 0, D.s, dA, K, k, dpart);
+        hipLaunchKernelGGL(hh_symv_sum_kernel, dim3(nc), dim3(256), 0, D.s, dpart, nc, m, dp);
         hipLaunchKernelGGL(hh_w_kernel, dim3(1), dim3(1024), 0, D.s, dA, K, k, dtau, dp);
         hipLaunchKernelGGL(hh_rank2_kernel, dim3((m + 255) / 256, m), dim3(256), 0, D.s, dA, K, k, dp);
     }
@@ -556,11 +591,11 @@ static void host_sym_eigen(int n, std::vector<double> A, std::vector<double>& w,
 }
 
 struct LanczosWork {
-    double* V = nullptr;      // N x Kmax
+    double* V = nullptr;      // N x (Kmax + b)
     double* W = nullptr;      // N x b
-    double* H = nullptr;      // Kmax x b
-    double* R = nullptr;      // N x b random
+    double* H = nullptr;      // (Kmax + b) x b
     double* small = nullptr;  // b x b
+    DevScratch mem;           // owns the buffers above: freed on every return path
 };
 
 // Orthonormalise the N x b block X against V[:, 0:kprev] (twice) and itself
@@ -635,32 +670,36 @@ static int orthonormalise_block(Dev& D, LanczosWork& L, int N, int b, int kprev,
 // Block Lanczos on the dense symmetric device matrix S (N x N row-major).
 static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double tol, int b, int maxdim, double* evals,
                          double* evecs, int* nconv) {
-    Dev D{ctx, ctx->stream};
+    Dev D(ctx, ctx->stream);
+    OPSEC_HIP_TRY(ctx, D.init());
     nev = std::min(nev, N);
     if (b <= 0) b = std::min(128, std::max(1, N / 4));
     b = std::max(1, std::min(b, N));
     if (maxdim <= 0) maxdim = std::max(3 * nev + 4 * b, 16 * b);
-    if (maxdim >= N) {
-        // the whole space fits: use a block size that divides N
-        while (N % b) b--;
-        maxdim = N;
+    if (maxdim + b > N) {
+        // the Krylov space would span (nearly) the whole space: the dense
+        // symmetric eigen-solver on S itself gives the same answer directly
+        DevScratch mem;
+        double *dA = nullptr, *dZ = nullptr;
+        OPSEC_HIP_TRY(ctx, mem.alloc(&dA, (size_t)N * N * 8));
+        OPSEC_HIP_TRY(ctx, mem.alloc(&dZ, (size_t)N * nev * 8));
+        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dA, S, (size_t)N * N * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        std::vector<double> ev;
+        int rc = dense_eigen_top(D, dA, N, nev, ev, dZ);
+        if (rc) return rc;
+        OPSEC_HIP_TRY(ctx, stream_copy(ctx, evecs, dZ, (size_t)N * nev * 8, hipMemcpyDeviceToHost));
+        for (int c = 0; c < nev; c++) evals[c] = ev[c];
+        *nconv = nev;
+        return OPSEC_OK;
     }
     int Kmax = std::min(N, ((maxdim + b - 1) / b) * b);
     if (Kmax < nev) Kmax = std::min(N, ((nev + b - 1) / b) * b);
     Kmax = std::max(b, (Kmax / b) * b);
     LanczosWork L;
-    auto cleanup = [&]() {
-        hipFree(L.V);
-        hipFree(L.W);
-        hipFree(L.H);
-        hipFree(L.small);
-    };
     size_t ncols = (size_t)Kmax + b;
-    if (hipMalloc(&L.V, (size_t)N * ncols * 8) != hipSuccess || hipMalloc(&L.W, (size_t)N * b * 8) != hipSuccess ||
-        hipMalloc(&L.H, ncols * b * 8) != hipSuccess || hipMalloc(&L.small, (size_t)b * b * 8 + 8) != hipSuccess) {
-        cleanup();
+    if (L.mem.alloc(&L.V, (size_t)N * ncols * 8) != hipSuccess || L.mem.alloc(&L.W, (size_t)N * b * 8) != hipSuccess ||
+        L.mem.alloc(&L.H, ncols * b * 8) != hipSuccess || L.mem.alloc(&L.small, (size_t)b * b * 8 + 8) != hipSuccess)
         return set_error(ctx, OPSEC_ENOMEM, "Lanczos workspace (%d x %zu) does not fit", N, ncols);
-    }
     std::mt19937_64 rng(42);
     std::normal_distribution<double> nd;
     // starting block
@@ -670,15 +709,22 @@ static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double
         OPSEC_HIP_TRY(ctx, stream_copy(ctx, L.V, r.data(), r.size() * 8, hipMemcpyHostToDevice));
         std::vector<double> Bm;
         int rc = orthonormalise_block(D, L, N, b, 0, L.V, Bm, rng);
-        if (rc) {
-            cleanup();
-            return rc;
-        }
+        if (rc) return rc;
     }
     std::vector<double> T((size_t)Kmax * Kmax, 0.0);   // column-major block tridiagonal
     int check_at = std::min(Kmax, std::max(((nev * 3 / 2 + b - 1) / b) * b, 2 * b));
     std::vector<double> ritz;
-    double* dZ = nullptr;
+    // Ritz workspace: T of the current step and its wanted eigenvectors,
+    // owned here so that every return path frees them
+    struct RitzBufs {
+        double* T = nullptr;
+        double* Z = nullptr;
+        ~RitzBufs() {
+            if (T) (void)hipFree(T);
+            if (Z) (void)hipFree(Z);
+        }
+    } rb;
+    double*& dZ = rb.Z;
     int converged = 0, Kr = 0;
     int rc = OPSEC_OK;
     for (int j = 0;; j++) {
@@ -707,11 +753,14 @@ static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double
         }
         if (!last && K < check_at) continue;
         // Rayleigh-Ritz on T[0:K, 0:K]; residual of Ritz pair z: |Bm z[K-b:K]|
-        double* dT = nullptr;
-        if (hipMalloc(&dT, (size_t)K * K * 8) != hipSuccess) {
+        if (rb.T) (void)hipFree(rb.T);
+        rb.T = nullptr;
+        if (hipMalloc(&rb.T, (size_t)K * K * 8) != hipSuccess) {
+            rb.T = nullptr;
             rc = set_error(ctx, OPSEC_ENOMEM, "Ritz workspace");
             break;
         }
+        double* dT = rb.T;
         std::vector<double> Tk((size_t)K * K);
         for (int c = 0; c < K; c++)
             for (int r = 0; r < K; r++) Tk[(size_t)c * K + r] = T[(size_t)c * Kmax + r];
@@ -720,12 +769,13 @@ static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double
         dZ = nullptr;
         const int want = std::min(nev, K);
         if (hipMalloc(&dZ, (size_t)K * want * 8) != hipSuccess) {
-            hipFree(dT);
+            dZ = nullptr;
             rc = set_error(ctx, OPSEC_ENOMEM, "Ritz vectors");
             break;
         }
         rc = dense_eigen_top(D, dT, K, want, ritz, dZ);
-        hipFree(dT);
+        (void)hipFree(rb.T);
+        rb.T = nullptr;
         if (rc) break;
         Kr = K;
         std::vector<double> Zl((size_t)b * want);
@@ -755,7 +805,8 @@ static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double
             int want = std::min(nev, Kr);
             // eigenvectors Y = V[:, 0:Kr] Z  (N x want, column-major = mode-major)
             double* dY = nullptr;
-            if (hipMalloc(&dY, (size_t)N * want * 8) != hipSuccess) {
+            DevScratch ymem;
+            if (ymem.alloc(&dY, (size_t)N * want * 8) != hipSuccess) {
                 rc = set_error(ctx, OPSEC_ENOMEM, "Ritz vector output");
             } else {
                 D.gemm_nn(0, dY, N, L.V, N, dZ, Kr, N, Kr, want);
@@ -764,16 +815,41 @@ static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double
                     hipError_t e = stream_copy(ctx, evecs, dY, (size_t)N * want * 8, hipMemcpyDeviceToHost);
                     if (e != hipSuccess) rc = set_error(ctx, OPSEC_EHIP, "copy eigenvectors: %s", hipGetErrorString(e));
                 }
-                hipFree(dY);
             }
             for (int c = 0; c < want; c++) evals[c] = ritz[c];
             for (int c = want; c < nev; c++) evals[c] = 0.0;
             *nconv = std::min(converged, nev);
         }
     }
-    hipFree(dZ);
-    cleanup();
     return rc;
+}
+
+// block_lanczos with the automatic Krylov dimension (maxdim <= 0): when some
+// of the nev modes have not converged (slowly decaying spectra, e.g. spherical
+// shells), run again with twice the space, up to N, at most twice.  A retry
+// solves into its own buffers and is adopted only when it succeeds with more
+// converged modes; a failed retry (e.g. a larger space that does not fit)
+// keeps the earlier result.
+static int lanczos_auto(opsec_ctx* ctx, const double* dS, int N, int nev, double tol, int block, int maxdim,
+                        double* evals, double* evecs, int* nconv) {
+    int rc = block_lanczos(ctx, dS, N, nev, tol, block, maxdim, evals, evecs, nconv);
+    if (rc || maxdim > 0) return rc;
+    const int b = block > 0 ? std::min(block, N) : std::min(128, std::max(1, N / 4));
+    int md = std::max(3 * std::min(nev, N) + 4 * b, 16 * b);
+    std::vector<double> ev2, evec2;
+    for (int retry = 0; retry < 2 && *nconv < nev && md < N; retry++) {
+        md = std::min(N, 2 * md);
+        int nconv2 = 0;
+        ev2.resize((size_t)nev);
+        evec2.resize((size_t)nev * N);
+        if (block_lanczos(ctx, dS, N, nev, tol, block, md, ev2.data(), evec2.data(), &nconv2)) break;
+        if (nconv2 > *nconv) {
+            std::copy(ev2.begin(), ev2.end(), evals);
+            std::copy(evec2.begin(), evec2.end(), evecs);
+            *nconv = nconv2;
+        }
+    }
+    return OPSEC_OK;
 }
 
 }  // namespace opsec
@@ -793,7 +869,7 @@ int opsec_lanczos_dense(opsec_ctx* ctx, int n, const double* A, int nev, double 
     DevScratch tmp;
     OPSEC_HIP_TRY(ctx, tmp.alloc(&dS, (size_t)n * n * 8));
     OPSEC_HIP_TRY(ctx, stream_copy(ctx, dS, A, (size_t)n * n * 8, hipMemcpyHostToDevice));
-    return block_lanczos(ctx, dS, n, nev, tol, block, maxdim, evals, evecs, nconv);
+    return lanczos_auto(ctx, dS, n, nev, tol, block, maxdim, evals, evecs, nconv);
 }
 
 int opsec_klt(opsec_ctx* ctx, int nev, double tol, int block, int maxdim, double* evals, double* evecs, int* nconv) {
@@ -809,23 +885,12 @@ int opsec_klt(opsec_ctx* ctx, int nev, double tol, int block, int maxdim, double
     DevScratch tmp;
     OPSEC_HIP_TRY(ctx, tmp.alloc(&dS, (size_t)N * N * 8));
     int rc = opsec_assemble_device(ctx, 0, dS, N);
-    if (!rc) rc = block_lanczos(ctx, dS, N, nev, tol, block, maxdim, evals, evecs, nconv);
-    if (maxdim <= 0) {
-        // automatic Krylov dimension: when some modes have not converged (slowly
-        // decaying spectra, e.g. spherical shells), run again with twice the
-        // space, up to N, at most twice
-        const int b = block > 0 ? std::min(block, N) : std::min(128, std::max(1, N / 4));
-        int md = std::max(3 * std::min(nev, N) + 4 * b, 16 * b);
-        for (int retry = 0; retry < 2 && !rc && *nconv < nev && md < N; retry++) {
-            md = std::min(N, 2 * md);
-            rc = block_lanczos(ctx, dS, N, nev, tol, block, md, evals, evecs, nconv);
-        }
-    }
+    if (!rc) rc = lanczos_auto(ctx, dS, N, nev, tol, block, maxdim, evals, evecs, nconv);
     return rc;
 }
 
 // Host tridiagonal eigen-solver, exposed for CPU unit tests (not in the public header).
-int opsec_internal_tridiag_top(int n, const double* d, const double* e, int nev, double* evals, double* Z) {
+int opsec_diag_tridiag_top(int n, const double* d, const double* e, int nev, double* evals, double* Z) {
     if (n < 1 || nev < 1 || nev > n) return OPSEC_EINVAL;
     std::vector<double> dd(d, d + n), ee(e, e + std::max(0, n - 1)), ev, Zv;
     if (ee.empty()) ee.push_back(0.0);

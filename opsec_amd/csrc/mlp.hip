@@ -63,7 +63,12 @@ struct MlpState {
     float* delta = nullptr;   // B x 128
     float* delta2 = nullptr;  // B x 128
     float* out = nullptr;     // B
+    double* lparts = nullptr; // per-block loss partials, B / 256 + 1
     int cap_batch = 0;
+    // gradient partials of the reductions over samples, summed in a fixed
+    // order: up to MLP_GSPLIT split-K parts of a 128 x 128 weight block, or
+    // MLP_RBLOCKS blocks of 129 sums
+    float* gparts = nullptr;
     // analytic baseline per function index (band functions only)
     std::vector<struct SurBand> bands;
     struct SurBand* d_bands = nullptr;
@@ -73,6 +78,11 @@ struct MlpState {
     void* d_inf = nullptr;
     size_t cap_inf = 0;
 };
+
+constexpr int MLP_GSPLIT = 64, MLP_RBLOCKS = 1024;
+inline size_t mlp_gparts_floats() {
+    return std::max((size_t)MLP_GSPLIT * MLP_H * MLP_H, (size_t)MLP_RBLOCKS * (MLP_H + 1));
+}
 
 constexpr int SUR_NK = 16;
 struct SurBand {
@@ -261,11 +271,13 @@ static unsigned infer_blocks(long long n) {
     return (unsigned)std::max(1LL, std::min((n + FT - 1) / FT, 2LL * 256));
 }
 
-// ---- generic BF16 MFMA GEMM for training: C (M x N, fp32) (+)= A (M x K) B (K x N)
-// Element (i, k) of A at a[i*ars + k*acs] (fp32 or bf16), same for B.
+// ---- generic BF16 MFMA GEMM for training: C (M x N, fp32) = A (M x K) B (K x N)
+// Element (i, k) of A at a[i*ars + k*acs] (fp32 or bf16), same for B.  With
+// split-K (gridDim.z > 1) part z goes to c + z * zstride, for a fixed-order sum.
 template <typename TA, typename TB>
 __global__ __launch_bounds__(256) void bgemm_kernel(const TA* a, long long ars, long long acs, const TB* b, long long brs,
-                                                    long long bcs, float* c, int ldc, int M, int N, int K, int accumulate) {
+                                                    long long bcs, float* c, int ldc, int M, int N, int K,
+                                                    long long zstride) {
     __shared__ __bf16 As[64][40];
     __shared__ __bf16 Bs[64][40];   // [n][k]
     const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
@@ -297,16 +309,30 @@ __global__ __launch_bounds__(256) void bgemm_kernel(const TA* a, long long ars, 
         for (int j = 0; j < 2; j++)
             for (int r = 0; r < 4; r++) {
                 int m = m0 + wm + i * 16 + 4 * (lane >> 4) + r, nn = n0 + wn + j * 16 + (lane & 15);
-                if (m < M && nn < N) {
-                    float v = acc[i][j][r];
-                    if (gridDim.z > 1 || accumulate) atomicAdd(&c[(size_t)m * ldc + nn], v);
-                    else c[(size_t)m * ldc + nn] = v;
-                }
+                if (m < M && nn < N) c[(size_t)blockIdx.z * zstride + (size_t)m * ldc + nn] = acc[i][j][r];
             }
 }
 
-// delta4 = (out - t)/B; loss partial sums
-__global__ void mlp_out_delta_kernel(const float* out, const float* t, int n, float invB, float* delta, double* loss) {
+// out[i] = sum_z parts[z n + i], z in order (FP64 sum, then FP32)
+__global__ void sum_f_parts_kernel(const float* __restrict__ parts, int nparts, int n, float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int z = 0; z < nparts; z++) acc += (double)parts[(size_t)z * n + i];
+    out[i] = (float)acc;
+}
+
+// *loss += sum of the per-block partial losses, in block order
+__global__ void sum_loss_kernel(const double* __restrict__ parts, int nparts, double* loss) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double acc = 0.0;
+        for (int z = 0; z < nparts; z++) acc += parts[z];
+        *loss += acc;
+    }
+}
+
+// delta4 = (out - t)/B; per-block loss partial sums (block b -> lparts[b])
+__global__ void mlp_out_delta_kernel(const float* out, const float* t, int n, float invB, float* delta, double* lparts) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     double l = 0;
     if (s < n) {
@@ -322,12 +348,13 @@ __global__ void mlp_out_delta_kernel(const float* out, const float* t, int n, fl
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+    if (threadIdx.x == 0) lparts[blockIdx.x] = red[0];
 }
 
-// last layer: dW4[i] += sum_s d4[s] a4[s][i], db4 += sum d4; d3[s][i] = W4[i] d4[s] silu'(z3[s][i])
+// last layer: block b's partial of dW4[i] = sum_s d4[s] a4[s][i] and db4 = sum d4
+// into gp[b * (MLP_H + 1) + i]; d3[s][i] = W4[i] d4[s] silu'(z3[s][i])
 __global__ void mlp_last_back_kernel(const float* d4, const __bf16* a4, const float* z3, const float* w32, int n,
-                                     float* g, float* d3) {
+                                     float* gp, float* d3) {
     const int i = threadIdx.x;   // 128 threads
     const float* W4 = w32 + mlp_w_off(4);
     float gw = 0, gb = 0;
@@ -337,16 +364,16 @@ __global__ void mlp_last_back_kernel(const float* d4, const __bf16* a4, const fl
         if (i == 0) gb += d;
         d3[(size_t)s * MLP_H + i] = W4[i] * d * dsilu(z3[(size_t)s * MLP_H + i]);
     }
-    atomicAdd(&g[mlp_w_off(4) + i], gw);
-    if (i == 0) atomicAdd(&g[mlp_w_off(4) + MLP_H], gb);
+    gp[(size_t)blockIdx.x * (MLP_H + 1) + i] = gw;
+    if (i == 0) gp[(size_t)blockIdx.x * (MLP_H + 1) + MLP_H] = gb;
 }
 
-// bias gradient: gb[o] += sum_s d[s][o]
-__global__ void mlp_bias_grad_kernel(const float* d, int n, float* gb) {
+// bias gradient: block b's partial of gb[o] = sum_s d[s][o] into gp[b * MLP_H + o]
+__global__ void mlp_bias_grad_kernel(const float* d, int n, float* gp) {
     const int o = threadIdx.x;
     float acc = 0;
     for (int s = blockIdx.x; s < n; s += gridDim.x) acc += d[(size_t)s * MLP_H + o];
-    atomicAdd(&gb[o], acc);
+    gp[(size_t)blockIdx.x * MLP_H + o] = acc;
 }
 
 // dprev[s][i] *= silu'(z[s][i])
@@ -381,6 +408,8 @@ __global__ void zero_f_kernel(float* p, size_t n) {
 
 // ---- host ---------------------------------------------------------------------
 
+static int init_weights(opsec_ctx* ctx, uint32_t seed);
+
 static int ensure_mlp(opsec_ctx* ctx) {
     if (ctx->mlp) return OPSEC_OK;
     MlpState* s = new MlpState;
@@ -390,9 +419,15 @@ static int ensure_mlp(opsec_ctx* ctx) {
     OPSEC_HIP_TRY(ctx, hipMalloc(&s->g, MLP_NW * 4));
     OPSEC_HIP_TRY(ctx, hipMalloc(&s->m, MLP_NW * 4));
     OPSEC_HIP_TRY(ctx, hipMalloc(&s->v, MLP_NW * 4));
-    // He-style initialisation, fixed seed
+    OPSEC_HIP_TRY(ctx, hipMalloc(&s->gparts, mlp_gparts_floats() * 4));
+    return init_weights(ctx, 7);
+}
+
+// He-style initialisation from `seed`, Adam moments and step count reset.
+static int init_weights(opsec_ctx* ctx, uint32_t seed) {
+    MlpState* s = ctx->mlp;
     std::vector<float> w(MLP_NW, 0.f);
-    std::mt19937 rng(7);
+    std::mt19937 rng(seed);
     for (int l = 0; l < MLP_LAYERS; l++) {
         int din = mlp_in_dim(l), dout = mlp_out_dim(l);
         std::normal_distribution<float> nd(0.f, std::sqrt(2.f / din));
@@ -403,6 +438,8 @@ static int ensure_mlp(opsec_ctx* ctx) {
     hipLaunchKernelGGL(zero_f_kernel, dim3((MLP_NW + 255) / 256), dim3(256), 0, ctx->stream, s->m, (size_t)MLP_NW);
     hipLaunchKernelGGL(zero_f_kernel, dim3((MLP_NW + 255) / 256), dim3(256), 0, ctx->stream, s->v, (size_t)MLP_NW);
     OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    s->step = 0;
+    s->trained = false;
     return OPSEC_OK;
 }
 
@@ -411,7 +448,8 @@ static int ensure_batch(opsec_ctx* ctx, int B) {
     if (s->cap_batch >= B) return OPSEC_OK;
     // free and forget the old buffers first, so a failed allocation below
     // leaves no stale pointer for the context's destructor
-    for (void** p : {(void**)&s->z, (void**)&s->a, (void**)&s->delta, (void**)&s->delta2, (void**)&s->out}) {
+    for (void** p : {(void**)&s->z, (void**)&s->a, (void**)&s->delta, (void**)&s->delta2, (void**)&s->out,
+                     (void**)&s->lparts}) {
         (void)hipFree(*p);
         *p = nullptr;
     }
@@ -421,40 +459,50 @@ static int ensure_batch(opsec_ctx* ctx, int B) {
     OPSEC_HIP_TRY(ctx, hipMalloc(&s->delta, (size_t)B * MLP_H * 4));
     OPSEC_HIP_TRY(ctx, hipMalloc(&s->delta2, (size_t)B * MLP_H * 4));
     OPSEC_HIP_TRY(ctx, hipMalloc(&s->out, (size_t)B * 4));
+    OPSEC_HIP_TRY(ctx, hipMalloc(&s->lparts, ((size_t)B / 256 + 1) * 8));
     s->cap_batch = B;
     return OPSEC_OK;
 }
 
 // Forward + backward on n device samples (x: n x 8, t: n); gradient in s->g
-// (zeroed first); returns the mean loss through *loss_out (device double).
+// (every entry written); adds the summed loss to *dloss (device double).
 static void loss_grad_device(opsec_ctx* ctx, const float* dx, const float* dt, int n, double* dloss) {
     MlpState* s = ctx->mlp;
     hipStream_t st = ctx->stream;
     const int B = n;
-    hipLaunchKernelGGL(zero_f_kernel, dim3((MLP_NW + 255) / 256), dim3(256), 0, st, s->g, (size_t)MLP_NW);
     hipLaunchKernelGGL(mlp_forward_kernel<true>, dim3((n + FT - 1) / FT), dim3(256), 0, st, s->wb, s->w, dx, n, s->out,
                        s->z, s->a, B);
-    hipLaunchKernelGGL(mlp_out_delta_kernel, dim3((n + 255) / 256), dim3(256), 0, st, s->out, dt, n, 1.f / n, s->out,
-                       dloss);
+    const int nlb = (n + 255) / 256;
+    hipLaunchKernelGGL(mlp_out_delta_kernel, dim3(nlb), dim3(256), 0, st, s->out, dt, n, 1.f / n, s->out, s->lparts);
+    hipLaunchKernelGGL(sum_loss_kernel, dim3(1), dim3(64), 0, st, s->lparts, nlb, dloss);
+    // every gradient entry is written by a fixed-order sum of partials below
+    // (no atomics): the same samples give the same bits
+    const int nrb = std::min(n, MLP_RBLOCKS);
     // last layer -> delta for layer 3
-    hipLaunchKernelGGL(mlp_last_back_kernel, dim3(std::min(n, 1024)), dim3(MLP_H), 0, st, s->out,
-                       s->a + (size_t)4 * B * MLP_H, s->z + (size_t)3 * B * MLP_H, s->w, n, s->g, s->delta);
+    hipLaunchKernelGGL(mlp_last_back_kernel, dim3(nrb), dim3(MLP_H), 0, st, s->out, s->a + (size_t)4 * B * MLP_H,
+                       s->z + (size_t)3 * B * MLP_H, s->w, n, s->gparts, s->delta);
+    hipLaunchKernelGGL(sum_f_parts_kernel, dim3(1), dim3(MLP_H + 1), 0, st, s->gparts, nrb, MLP_H + 1,
+                       s->g + mlp_w_off(4));
     float* d = s->delta;
     float* dn = s->delta2;
     for (int l = 3; l >= 0; l--) {
         const int din = mlp_in_dim(l);
         // dW_l (128 x din) = d^T (128 x n) . a_l (n x din); a_l row stride 128
         const __bf16* al = s->a + (size_t)l * B * MLP_H;
-        int splits = std::max(1, std::min(64, n / 1024));
+        const int splits = std::max(1, std::min(MLP_GSPLIT, n / 1024));
+        const int nw = MLP_H * din;
         hipLaunchKernelGGL((bgemm_kernel<float, __bf16>), dim3(2, (din + 63) / 64, splits), dim3(256), 0, st, d, 1LL,
-                           (long long)MLP_H, al, (long long)MLP_H, 1LL, s->g + mlp_w_off(l), din, MLP_H, din, n, 1);
-        hipLaunchKernelGGL(mlp_bias_grad_kernel, dim3(std::min(n, 1024)), dim3(MLP_H), 0, st, d, n,
+                           (long long)MLP_H, al, (long long)MLP_H, 1LL, s->gparts, din, MLP_H, din, n, (long long)nw);
+        hipLaunchKernelGGL(sum_f_parts_kernel, dim3((nw + 255) / 256), dim3(256), 0, st, s->gparts, splits, nw,
+                           s->g + mlp_w_off(l));
+        hipLaunchKernelGGL(mlp_bias_grad_kernel, dim3(nrb), dim3(MLP_H), 0, st, d, n, s->gparts);
+        hipLaunchKernelGGL(sum_f_parts_kernel, dim3(1), dim3(MLP_H), 0, st, s->gparts, nrb, MLP_H,
                            s->g + mlp_w_off(l) + din * MLP_H);
         if (l > 0) {
             // dprev (n x 128) = d (n x 128) . W_l (128 x 128)
             hipLaunchKernelGGL((bgemm_kernel<float, __bf16>), dim3((n + 63) / 64, 2, 1), dim3(256), 0, st, d,
                                (long long)MLP_H, 1LL, s->wb + mlp_w_off(l), (long long)MLP_H, 1LL, dn, MLP_H, n, MLP_H,
-                               MLP_H, 0);
+                               MLP_H, 0LL);
             size_t tot = (size_t)n * MLP_H;
             hipLaunchKernelGGL(mlp_dsilu_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, dn,
                                s->z + (size_t)(l - 1) * B * MLP_H, tot);
@@ -811,11 +859,7 @@ int surrogate_tables(opsec_ctx* ctx, int nf_all, const int* funcs_all, const ops
     return OPSEC_OK;
 }
 
-}  // namespace opsec
-
-extern "C" {
-
-void opsec_internal_free_mlp(opsec_ctx* ctx) {
+void free_mlp(opsec_ctx* ctx) {
     MlpState* s = ctx->mlp;
     if (!s) return;
     hipFree(s->w);
@@ -828,11 +872,18 @@ void opsec_internal_free_mlp(opsec_ctx* ctx) {
     hipFree(s->delta);
     hipFree(s->delta2);
     hipFree(s->out);
+    hipFree(s->lparts);
+    hipFree(s->gparts);
     hipFree(s->d_bands);
     hipFree(s->d_inf);
     delete s;
     ctx->mlp = nullptr;
 }
+
+}  // namespace opsec
+
+extern "C" {
+
 
 int opsec_mlp_defaults(opsec_mlp_train_opts* o) {
     if (!o) return OPSEC_EINVAL;
@@ -921,6 +972,9 @@ int opsec_mlp_train(opsec_ctx* ctx, const opsec_mlp_train_opts* o, const opsec_q
     OPSEC_CHECK_ARG(ctx, o->nsamples >= 256 && o->batch >= 64 && o->epochs >= 1 && o->lr > 0, "bad training options");
     int rc = ensure_mlp(ctx);
     if (rc) return rc;
+    // every training starts from the seed's initial weights: the same options
+    // on the same survey give the same network, bit for bit
+    if ((rc = init_weights(ctx, (uint32_t)o->seed))) return rc;
     const SurveyDev& sv = ctx->sv;
     const double h = (sv.max1 - sv.min1) / sv.n1;
     // seeded training pairs: uniform in separation over [near, Dmax] and in band

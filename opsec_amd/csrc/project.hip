@@ -1513,14 +1513,8 @@ static int run_project(opsec_ctx* ctx, int nb, const int* d_funcs, const int* fu
     return OPSEC_OK;
 }
 
-}  // namespace opsec
-
-using namespace opsec;
-
-extern "C" {
-
 // A new cell set invalidates the uploaded modes and everything derived from them.
-void opsec_internal_invalidate_modes(opsec_ctx* ctx) {
+void invalidate_modes(opsec_ctx* ctx) {
     ctx->nmodes = 0;
     if (!ctx->proj) return;
     ctx->proj->bp_valid = false;
@@ -1528,7 +1522,7 @@ void opsec_internal_invalidate_modes(opsec_ctx* ctx) {
     ctx->proj->nb = 0;
 }
 
-void opsec_internal_free_proj(opsec_ctx* ctx) {
+void free_proj(opsec_ctx* ctx) {
     if (!ctx->proj) return;
     dev_free((void**)&ctx->proj->d_Bw);
     dev_free(&ctx->proj->d_C);
@@ -1551,6 +1545,13 @@ void opsec_internal_free_proj(opsec_ctx* ctx) {
     delete ctx->proj;
     ctx->proj = nullptr;
 }
+
+}  // namespace opsec
+
+using namespace opsec;
+
+extern "C" {
+
 
 int opsec_modes_upload(opsec_ctx* ctx, const double* B, int ncells, int nmodes) {
     if (!ctx) return OPSEC_EINVAL;
@@ -1606,7 +1607,7 @@ int opsec_project(opsec_ctx* ctx, int nf, const int* funcs, int prec, double* P_
 }
 
 // Timing of the last projection's two stages (ms), for the benchmark.
-int opsec_internal_project_times(opsec_ctx* ctx, float* s1, float* s2) {
+int opsec_diag_project_times(opsec_ctx* ctx, float* s1, float* s2) {
     if (!ctx || !ctx->proj) return OPSEC_EINVAL;
     *s1 = ctx->proj->last_ms_stage1;
     *s2 = ctx->proj->last_ms_stage2;
@@ -1615,7 +1616,7 @@ int opsec_internal_project_times(opsec_ctx* ctx, float* s1, float* s2) {
 
 // Tile shape of the split-BF16x3 GEMM (0: 128 x 128, 1: 256 x 128, 2: 256 x 256
 // with double-buffered LDS), for tuning.
-int opsec_internal_set_nt_tile(int v) {
+int opsec_diag_set_nt_tile(int v) {
     if (v < 0 || v > 2) return OPSEC_EINVAL;
     g_nt_tile = v;
     return OPSEC_OK;
@@ -1623,14 +1624,14 @@ int opsec_internal_set_nt_tile(int v) {
 
 // Split-K factor of the split-BF16x3 stage 2 (its upper-triangle tiles alone
 // fill about half of the CUs), for tuning.
-int opsec_internal_set_stage2_split(int v) {
+int opsec_diag_set_stage2_split(int v) {
     if (v < 1 || v > 16) return OPSEC_EINVAL;
     g_s2_split = v;
     return OPSEC_OK;
 }
 
 // Tile-group edge of the split-BF16x3 stage 2, for tuning.
-int opsec_internal_set_stage2_group(int v) {
+int opsec_diag_set_stage2_group(int v) {
     if (v < 1 || v > 128) return OPSEC_EINVAL;
     g_s2_group = v;
     return OPSEC_OK;
@@ -1646,7 +1647,7 @@ int opsec_set_projection(opsec_ctx* ctx, int method) {
 
 // Stage-1 A operand of the split-BF16x3 projection, for tuning: generated
 // from the pair table (1) or read from S_p planes (0).
-int opsec_internal_set_nt_gen(int v) {
+int opsec_diag_set_nt_gen(int v) {
     if (v != 0 && v != 1) return OPSEC_EINVAL;
     g_nt_gen = v;
     return OPSEC_OK;
@@ -1654,7 +1655,7 @@ int opsec_internal_set_nt_gen(int v) {
 
 // Tile-order knobs of the split-BF16x3 GEMM, for tuning: group edge (>= 1)
 // and XCD remapping (0 or 1).
-int opsec_internal_set_nt_order(int group, int remap) {
+int opsec_diag_set_nt_order(int group, int remap) {
     if (group < 1 || (remap != 0 && remap != 1)) return OPSEC_EINVAL;
     g_nt_group = group;
     g_nt_remap = remap;
@@ -1663,7 +1664,7 @@ int opsec_internal_set_nt_order(int group, int remap) {
 
 // Per-kernel times of the last split-BF16x3 projection, summed over its bands
 // (ms): S_p plane generation, stage-1 GEMM, stage-2 GEMM; *launches = bands.
-int opsec_internal_project_kernel_ms(opsec_ctx* ctx, float* gen, float* gemm1, float* gemm2, int* launches) {
+int opsec_diag_project_kernel_ms(opsec_ctx* ctx, float* gen, float* gemm1, float* gemm2, int* launches) {
     if (!ctx || !ctx->proj) return OPSEC_EINVAL;
     *gen = ctx->proj->last_ms_gen;
     *gemm1 = ctx->proj->last_ms_gemm1;
@@ -1673,7 +1674,7 @@ int opsec_internal_project_kernel_ms(opsec_ctx* ctx, float* gen, float* gemm1, f
 }
 
 // Launch counts of the last split-BF16x3 projection: stage 1, stage 2.
-int opsec_internal_project_launches(opsec_ctx* ctx, int* stage1, int* stage2) {
+int opsec_diag_project_launches(opsec_ctx* ctx, int* stage1, int* stage2) {
     if (!ctx || !ctx->proj) return OPSEC_EINVAL;
     *stage1 = ctx->proj->last_launches;
     *stage2 = ctx->proj->last_launches2;

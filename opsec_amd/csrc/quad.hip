@@ -305,7 +305,7 @@ template <int NT, int NL>
 __global__ __launch_bounds__(NT) void gm_kernel(XiModelDev m, const GMJob* jobs, const int* idx, int njobs,
                                                 int* counter, double* dpool, int* ipool, GMParams P, double* Q,
                                                 size_t qstride, double* Qerr, int* jobfail,
-                                                unsigned long long* nevals, double* trace) {
+                                                unsigned long long* nevals) {
     __shared__ double vals[GM_SUB][GM_NPTS];
     __shared__ double lkey[NL > 0 ? NL : 1];
     __shared__ int lidx[NL > 0 ? NL : 1];
@@ -336,29 +336,8 @@ __global__ __launch_bounds__(NT) void gm_kernel(XiModelDev m, const GMJob* jobs,
         double tv = 0, te = 0;
         unsigned numEval = GM_NPTS;
         if (lane == 0) heap_push(S, H, nheap, 0, tv, te);
-        const bool tr = trace && jb == 0;
-        int iter = 0, status = 0;
-        if (tr && lane == 0) {
-            for (int p = 0; p < GM_NPTS; p++) trace[p] = vals[0][p];
-            trace[149] = S.val[0];
-            trace[150] = S.err[0];
-            trace[151] = S.split[0];
-            trace[152] = P.absTol;
-            trace[153] = P.relTol;
-            trace[154] = P.maxEval;
-            for (int d = 0; d < 6; d++) trace[155 + d] = J.lo[d], trace[161 + d] = J.hi[d];
-        }
+        int status = 0;
         for (;;) {
-            if (tr && lane == 0 && iter < 20) {
-                double* t = trace + 170 + 6 * iter;
-                t[0] = tv;
-                t[1] = te;
-                t[2] = nheap;
-                t[3] = nregions;
-                t[4] = numEval;
-                t[5] = S.err[H.idx(0)];
-            }
-            iter++;
             if (lane == 0) {
                 int done = 0, nR = 0;
                 if (!(numEval < P.maxEval || !P.maxEval)) done = 1;
@@ -829,10 +808,6 @@ __global__ __launch_bounds__(256) void assemble_kernel(const int* cellidx, const
 // ---- host side ------------------------------------------------------------
 
 
-// Diagnostic: when set, the GM kernel records its first job's node values and
-// per-iteration heap state here (opsec_internal_gm_trace).
-static double* g_gm_trace = nullptr;
-
 struct Scratch {
     void* p = nullptr;
     size_t cap = 0;
@@ -1003,7 +978,7 @@ static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::v
                 hipLaunchKernelGGL(expand_gm_jobs_kernel, dim3((unsigned)((njobs + 255) / 256)), dim3(256), 0,
                                    ctx->stream, (const GMJob*)st.p, (int)npairs, dfun, nf, (GMJob*)sj.p);
                 OPSEC_HIP_TRY(ctx, hipGetLastError());
-                if (!g_gm_trace) {
+                {
                     // first regions, then the shallow integrals one thread each
                     hipLaunchKernelGGL(gm_first_kernel, dim3((unsigned)((njobs + 255) / 256)), dim3(256), 0,
                                        ctx->stream, mv, (const GMJob*)sj.p, njobs, P, dQ, nkeys, dQerr, (int*)sf.p,
@@ -1031,19 +1006,19 @@ static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::v
                 if (pass == 0)
                     hipLaunchKernelGGL((gm_kernel<64, 0>), dim3(std::min(nslots, nrun)), dim3(64), 0, ctx->stream, mv,
                                        (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
-                                       nkeys, dQerr, (int*)sf.p, d_nev, g_gm_trace);
+                                       nkeys, dQerr, (int*)sf.p, d_nev);
                 else if (nrun <= 2 * ctx->num_cus)   // few deep integrals: 8 waves each
                     hipLaunchKernelGGL((gm_kernel<512, 2048>), dim3(std::min(nslots, nrun)), dim3(512), 0, ctx->stream, mv,
                                        (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
-                                       nkeys, dQerr, (int*)sf.p, d_nev, g_gm_trace);
+                                       nkeys, dQerr, (int*)sf.p, d_nev);
                 else if (nrun <= 4 * ctx->num_cus)
                     hipLaunchKernelGGL((gm_kernel<256, 1024>), dim3(std::min(nslots, nrun)), dim3(256), 0, ctx->stream, mv,
                                        (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
-                                       nkeys, dQerr, (int*)sf.p, d_nev, g_gm_trace);
+                                       nkeys, dQerr, (int*)sf.p, d_nev);
                 else   // many: one wave each, all resident at once
                     hipLaunchKernelGGL((gm_kernel<64, 0>), dim3(std::min(nslots, nrun)), dim3(64), 0, ctx->stream, mv,
                                        (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
-                                       nkeys, dQerr, (int*)sf.p, d_nev, g_gm_trace);
+                                       nkeys, dQerr, (int*)sf.p, d_nev);
                 OPSEC_HIP_TRY(ctx, hipGetLastError());
             }
             // jobs whose region pool ran out, listed on the device
@@ -1311,8 +1286,6 @@ int opsec_quad_defaults(int method, opsec_quad* q) {
     return (method >= 0 && method <= 2) ? OPSEC_OK : OPSEC_EINVAL;
 }
 
-void opsec_internal_invalidate_modes(opsec_ctx* ctx);
-
 int opsec_cells_upload(opsec_ctx* ctx, const opsec_cell* cells, int ncells, const opsec_survey* sv) {
     if (!ctx) return OPSEC_EINVAL;
     OPSEC_CHECK_ARG(ctx, cells && ncells > 0, "no cells");
@@ -1327,7 +1300,7 @@ int opsec_cells_upload(opsec_ctx* ctx, const opsec_cell* cells, int ncells, cons
     OPSEC_CHECK_ARG(ctx, pair_nkeys_of(s) <= INT32_MAX, "survey grid too large for a pair table (%lld keys)",
                     pair_nkeys_of(s));
     OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
-    opsec_internal_invalidate_modes(ctx);   // modes are weighted by the old cells' Nbar
+    invalidate_modes(ctx);   // modes are weighted by the old cells' Nbar
     ctx->sv = s;
     ctx->ncells = ncells;
     ctx->nkeys = (int)pair_nkeys_of(s);
@@ -1654,25 +1627,6 @@ int opsec_sig_block(opsec_ctx* ctx, const opsec_cell* cells, int ncells, const i
     }
 done:
     ctx->sv = saved;
-    return rc;
-}
-
-// Diagnostic entry (not in the public header): one GM pair integral with the
-// kernel's trace (290 doubles) copied to `trace`.
-int opsec_pair_integral(opsec_ctx* ctx, const opsec_cell* c1, const opsec_cell* c2, int f, const opsec_survey* sv,
-                        const opsec_quad* q, double* Q, double* err);
-int opsec_internal_gm_trace(opsec_ctx* ctx, const opsec_cell* c1, const opsec_cell* c2, int f, const opsec_survey* sv,
-                            const opsec_quad* q, double* Q, double* trace) {
-    if (!ctx || !trace) return OPSEC_EINVAL;
-    double* dtr = nullptr;
-    OPSEC_HIP_TRY(ctx, hipMalloc(&dtr, 300 * 8));
-    OPSEC_HIP_TRY(ctx, hipMemsetAsync(dtr, 0, 300 * 8, ctx->stream));
-    g_gm_trace = dtr;
-    double err = 0;
-    int rc = opsec_pair_integral(ctx, c1, c2, f, sv, q, Q, &err);
-    g_gm_trace = nullptr;
-    stream_copy(ctx, trace, dtr, 300 * 8, hipMemcpyDeviceToHost);
-    hipFree(dtr);
     return rc;
 }
 

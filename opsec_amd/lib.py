@@ -132,10 +132,31 @@ _SIGS = {
     "opsec_mlp_loss_grad": ([_P, _I, _P, _P, C.POINTER(_D), _P], _I),
 }
 
+# include/opsec_hip_diag.h: timings and tuning knobs (benchmark and tests only)
+_F = C.POINTER(C.c_float)
+_DIAG_SIGS = {
+    "opsec_diag_copy": ([_P, _P, _P, C.c_size_t, _F], _I),
+    "opsec_diag_sincos_rate": ([_P, C.POINTER(_D)], _I),
+    "opsec_diag_tridiag_top": ([_I, _P, _P, _I, _P, _P], _I),
+    "opsec_diag_project_times": ([_P, _F, _F], _I),
+    "opsec_diag_project_kernel_ms": ([_P, _F, _F, _F, C.POINTER(_I)], _I),
+    "opsec_diag_project_launches": ([_P, C.POINTER(_I), C.POINTER(_I)], _I),
+    "opsec_diag_set_nt_tile": ([_I], _I),
+    "opsec_diag_set_nt_gen": ([_I], _I),
+    "opsec_diag_set_stage2_split": ([_I], _I),
+    "opsec_diag_set_stage2_group": ([_I], _I),
+    "opsec_diag_set_nt_order": ([_I, _I], _I),
+}
+
 
 def exported_symbols():
     """Names of every entry point include/opsec_hip.h declares."""
     return sorted(_SIGS)
+
+
+def diag_symbols():
+    """Names of the diagnostics include/opsec_hip_diag.h declares."""
+    return sorted(_DIAG_SIGS)
 
 
 def load(path: str | None = None):
@@ -147,7 +168,7 @@ def load(path: str | None = None):
     if not os.path.exists(p):
         raise OpsecError(f"{p} is missing: build it with __graft_entry__.build() or make -C opsec_amd")
     lib = C.CDLL(p)
-    for name, (argt, rest) in _SIGS.items():
+    for name, (argt, rest) in list(_SIGS.items()) + list(_DIAG_SIGS.items()):
         fn = getattr(lib, name, None)
         if fn is None:   # reported by tests/test_abi.py
             continue

@@ -76,3 +76,38 @@ def test_klt_box_multiplets(ctx):
     np.testing.assert_allclose(V @ S @ V.T, np.diag(ev), atol=1e-9 * w[0])
     np.testing.assert_allclose(V @ V.T, np.eye(nev), atol=1e-9)
     assert nconv == nev
+
+
+def test_auto_krylov_retry_keeps_and_improves(ctx):
+    """A flat spectrum defeats the first Krylov space (maxdim = 3 nev + 4 b):
+    the automatic retries with twice the space converge more modes, and the
+    result never has fewer converged modes than the first pass."""
+    n, nev, block = 3000, 40, 16
+    rng = np.random.default_rng(11)
+    Q = np.linalg.qr(rng.normal(size=(n, n)))[0]
+    lam = 1.0 - 0.02 * np.sort(rng.uniform(0, 1, n))   # all within 2 % of the largest
+    A = (Q * lam) @ Q.T
+    A = 0.5 * (A + A.T)
+    first = 3 * nev + 4 * block
+    ev1, _, nconv1 = ctx.lanczos_dense(A, nev, tol=1e-9, block=block, maxdim=first)
+    ev, V, nconv = ctx.lanczos_dense(A, nev, tol=1e-9, block=block)
+    print(f"flat spectrum: first pass {nconv1} of {nev} converged, automatic retries {nconv}")
+    assert nconv >= nconv1
+    assert nconv > nconv1 or nconv1 == nev
+    w = np.sort(lam)[::-1][:nev]
+    np.testing.assert_allclose(ev[:nconv], w[:nconv], atol=1e-8)
+
+
+def test_dense_path_for_whole_space(ctx):
+    """A Krylov space that would span the whole space goes to the dense
+    eigen-solver: N prime (no block size divides it), all modes converged."""
+    n, nev = 211, 50
+    rng = np.random.default_rng(13)
+    Q = np.linalg.qr(rng.normal(size=(n, n)))[0]
+    lam = np.linspace(3.0, -1.0, n)
+    A = (Q * lam) @ Q.T
+    A = 0.5 * (A + A.T)
+    ev, V, nconv = ctx.lanczos_dense(A, nev, tol=1e-10, block=0)
+    assert nconv == nev
+    np.testing.assert_allclose(ev, lam[:nev], atol=1e-11 * 3)
+    np.testing.assert_allclose(V @ A @ V.T, np.diag(ev), atol=1e-10)

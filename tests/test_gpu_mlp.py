@@ -38,7 +38,8 @@ def test_forward_vs_oracle(ctx):
     O.oracle().orc_mlp_forward(O.ptr(wb), 1000, O.ptr(np.ascontiguousarray(x)), O.ptr(ref))
     err = np.abs(got - ref).max() / np.abs(ref).max()
     print("forward max rel err (bf16 operands, fp32 accumulation)", err)
-    assert err < 3e-2
+    # observed 4.6e-3: BF16 rounding of the activations between layers
+    assert err < 1.5e-2
 
 
 @pytest.mark.parametrize("n", [1, 1000, 70001])
@@ -76,7 +77,8 @@ def test_gradient_vs_oracle(ctx):
         a, b = g[off:off + n], gref[off:off + n]
         rel = np.linalg.norm(a - b) / np.linalg.norm(b)
         print(f"layer {din}->{dout} gradient rel err {rel:.2e}")
-        assert rel < 5e-2
+        # observed at most 2.9e-3 (BF16 operands of the backward GEMMs)
+        assert rel < 1e-2
         off += n
 
 

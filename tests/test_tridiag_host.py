@@ -11,7 +11,7 @@ from opsec_amd import lib
 
 def tridiag_top(d, e, nev):
     L = lib.load()
-    fn = L.opsec_internal_tridiag_top
+    fn = L.opsec_diag_tridiag_top
     fn.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
     fn.restype = C.c_int
     n = len(d)

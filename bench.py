@@ -164,57 +164,85 @@ def diagnose(ctx, L, funcs, q64, qs, lam, y, prior, e64):
     return out
 
 
-def cpu_baseline(cfg, cells, k, P, lo, hi, B, budget_s=20.0):
-    """Oracle (FP64 C restatement, OpenMP over the host threads) on a bounded
-    sample of the same workload, extrapolated to the full S->Fisher pass: a
-    block of rows of one band's S_p by the 6^3 Gauss rule, then S_p B for a
-    block of modes, each sized to a few seconds from a short pilot."""
+def _blas_rate(N, B, R=1024):
+    """FP64 GEMM rate of the host BLAS (numpy's OpenBLAS) on an R x N by N x M
+    block of the projection's first product, S_p B (flop/s)."""
+    S = np.random.default_rng(0).normal(size=(R, N))
+    Bt = np.ascontiguousarray(B.T)            # N x M, the reference's column-major B
+    S[:8] @ Bt                                 # warm the thread pool
+    t0 = time.perf_counter()
+    S @ Bt
+    return 2.0 * R * N * B.shape[0] / (time.perf_counter() - t0)
+
+
+def cpu_baseline(cfg, cells, k, P, lo, hi, B):
+    """The reference's own CPU path on a bounded sample of the same workload,
+    extrapolated to one S->Fisher pass.
+
+    Pair integrals and the S_p fill: the reference's ComputeSignalMatrixBlock
+    (src/sig.cpp:50-123, compiled from its sources into oracle/_ref) with its own
+    method, adaptive Genz-Malik pairs at sig.epsrel/epsabs and the translation
+    cache.  A call on rows [0, R) x all columns costs a_f (the N distinct
+    integrals of function f, all met in row 0) plus R b (the cache fill of a
+    row): a_f is timed for the prior and the lowest and highest band at R = 1,
+    b from the highest band at a larger R.  The projection: the reference's
+    comma does two pdgemm per band (S_p B, then B^T C; src/comma-scalapack.cpp:
+    248-252), 2 N^2 M + 2 N M^2 flops, timed here at the rate of the host BLAS
+    pdgemm sits on (numpy's OpenBLAS) for an R x N by N x M block.  The
+    estimate's packed Gram products (src/estimate-mpi.cpp:501-546) at the
+    same BLAS on a sample of the packed entries."""
     import ctypes as Cc
+    import tempfile
     from tests import oracles as O
-    orc = O.oracle()
     n = cfg["n"]
     N = n ** 3
     M = cfg["nmodes"]
     nb = cfg["nbands"]
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    xi = O.XiHandle(1, a=float(lo[-1]), b=float(hi[-1]))     # closed-form band (highest k)
-    cols = np.arange(N, dtype=np.int32)
+    R = O.ref()
+    with tempfile.TemporaryDirectory() as d:
+        pk = os.path.join(d, "pk.dat")
+        O.write_pkfile(pk, k, P)
+        m = O.ref_real_model(pk, lo, hi)
+        sv = O.survey_text(n, 0.0, 400.0, nbar=cfg["nbar"])
+        cols = np.arange(N, dtype=np.int32)
 
-    def rows_block(R):
-        rows = np.arange(R, dtype=np.int32)
-        S = np.zeros((R, N))
-        t0 = time.perf_counter()
-        orc.orc_sig_block(Cc.byref(xi.s), cells.ctypes.data, N, n, n, n, 400.0, O.ptr(rows), R, O.ptr(cols), N,
-                          O.ptr(S), N, 1, 0.0, 0.0, 0, cfg["nodes"], 1)
-        return S, time.perf_counter() - t0
+        def block(f, nrows):
+            rows = np.arange(nrows, dtype=np.int32)
+            S = np.zeros((nrows, N))
+            t0 = time.perf_counter()
+            R.ref_sig_block(m, f, sv, cells.ctypes.data, N, O.ptr(rows), nrows, O.ptr(cols), N, O.ptr(S), N, 1e-5,
+                            1e-10)
+            return time.perf_counter() - t0
 
-    _, t_pilot = rows_block(512)
-    R = int(min(N, max(512, (0.5 * budget_s / max(t_pilot, 1e-6)) * 512)) // 512 * 512)
-    S, t_asm = rows_block(R)
-    msub = 16
-    Cm = np.zeros((msub, R))
-    Bs = np.ascontiguousarray(B[:msub])
+        a_prior, a_lo, a_hi = block(0, 1), block(1, 1), block(nb, 1)
+        R2 = 2048
+        b_row = max(0.0, (block(nb, R2) - a_hi) / (R2 - 1))
+        R.ref_model_free(m)
+    t_tables = a_prior + nb * 0.5 * (a_lo + a_hi) + (nb + 1) * N * b_row
+    rate = _blas_rate(N, B)
+    t_proj = nb * (2.0 * N * N * M + 2.0 * N * M * M) / rate
+    # estimate: X (nb + 2 rows) times D^T over the packed entries
+    Ks = 1 << 20
+    Dk = np.random.default_rng(1).normal(size=(nb, Ks))
+    Xk = np.random.default_rng(2).normal(size=(nb + 2, Ks))
     t0 = time.perf_counter()
-    orc.orc_gemm_rows(R, N, O.ptr(S), msub, O.ptr(Bs), O.ptr(Cm))
-    t_pg = time.perf_counter() - t0
-    msub = int(min(M, max(16, 0.5 * budget_s / max(t_pg, 1e-6) * 16)))
-    Bs = np.ascontiguousarray(B[:msub])
-    Cm = np.zeros((msub, R))
-    t0 = time.perf_counter()
-    orc.orc_gemm_rows(R, N, O.ptr(S), msub, O.ptr(Bs), O.ptr(Cm))
-    t_gemm = time.perf_counter() - t0
-    # extrapolate: assembly scales with rows (all N) and functions (nb + 1);
-    # the projection (2 N^2 M for S_p B', N M^2 for the upper triangle of
-    # B'^T C) at the measured S_p B flop rate
-    rate = 2.0 * R * N * msub / t_gemm
-    t_full = (nb + 1) * t_asm * (N / R) + nb * (2.0 * N * N * M + 1.0 * N * M * M) / rate
+    Xk @ Dk.T
+    t_est = (time.perf_counter() - t0) * (M * (M + 1) / 2) / Ks
+    t_full = t_tables + t_proj + t_est
     pairs = float(N) * N * (nb + 1)
     return {
-        "value": pairs / t_full, "unit": "cell-pair integrals/s", "cores": cores, "kind": "port",
-        "sample": (f"oracle on {R} of {N} rows of one band's S_p ({t_asm:.1f} s, 6^3 Gauss) and S_p B for "
-                   f"{msub} of {M} modes ({t_gemm:.1f} s, {rate / 1e9:.1f} GFLOP/s), {cores} threads; extrapolated "
-                   f"to {nb + 1} matrices and the full projection: {t_full:.0f} s per S->Fisher pass"),
+        "value": pairs / t_full, "unit": "cell-pair integrals/s", "cores": cores, "kind": "reference",
+        "sample": (f"the reference's ComputeSignalMatrixBlock (oracle/_ref, its Genz-Malik pairs and translation "
+                   f"cache, OpenMP in the integrand) on row 0 of the prior and of the lowest and highest band "
+                   f"({a_prior:.2f} / {a_lo:.2f} / {a_hi:.2f} s: the {N} distinct integrals of each) and on "
+                   f"{R2} rows of the highest band (fill {b_row * 1e3:.3f} ms per row); the two pdgemm products "
+                   f"per band and the estimate's Gram products at the host BLAS rate (numpy OpenBLAS, "
+                   f"{rate / 1e9:.0f} GFLOP/s on a 1024 x {N} by {N} x {M} block); {cores} threads; extrapolated "
+                   f"to {nb + 1} matrices: tables {t_tables:.0f} s + projection {t_proj:.0f} s + estimate "
+                   f"{t_est:.1f} s = {t_full:.0f} s per S->Fisher pass"),
         "s_per_step_extrapolated": t_full,
+        "parts_s": {"tables": t_tables, "projection": t_proj, "estimate": t_est},
     }
 
 
@@ -257,22 +285,18 @@ def cpu_baseline_sph(cells, dims, k, P, lo, hi, B, budget_s=25.0):
     t_sample = time.perf_counter() - t0
     per_arc = {n: float(np.mean(v)) for n, v in times.items()}
     t_tables = narcs * (per_arc["prior"] + nb * 0.5 * (per_arc["band_lo"] + per_arc["band_hi"])) / cores
-    # projection at the oracle's measured FP64 GEMM rate (OpenMP over the host threads)
-    R = min(N, 2048)
-    S = np.random.default_rng(0).normal(size=(R, N))
-    msub = min(M, 64)
-    Cm = np.zeros((msub, R))
-    t1 = time.perf_counter()
-    orc.orc_gemm_rows(R, N, O.ptr(S), msub, O.ptr(np.ascontiguousarray(B[:msub])), O.ptr(Cm))
-    rate = 2.0 * R * N * msub / (time.perf_counter() - t1)
-    t_full = t_tables + nb * (2.0 * N * N * M + 1.0 * N * M * M) / rate
+    # the two pdgemm products per band at the host BLAS rate (numpy's OpenBLAS)
+    rate = _blas_rate(N, B)
+    t_full = t_tables + nb * (2.0 * N * N * M + 2.0 * N * M * M) / rate
     pairs = float(N) * N * (nb + 1)
     return {"value": pairs / t_full, "unit": "cell-pair integrals/s", "cores": cores, "kind": "port",
             "sample": (f"oracle arc-cache blocks (Genz-Malik) of the rows of {nsamp} of {narcs} arcs against all "
                        f"{N} columns for the prior, the lowest and the highest band ({t_sample:.1f} s on {cores} "
                        f"threads; per arc {per_arc['prior']:.2f} / {per_arc['band_lo']:.2f} / "
-                       f"{per_arc['band_hi']:.2f} s), bands taken as the mean of the lowest and highest, and S_p B "
-                       f"at {rate / 1e9:.1f} GFLOP/s; extrapolated: {t_full:.0f} s per S->Fisher pass"),
+                       f"{per_arc['band_hi']:.2f} s), bands taken as the mean of the lowest and highest (the reference's "
+                       f"spherical survey needs cfitsio and is not built, so this is the oracle's restatement of "
+                       f"ComputeSignalMatrixBlockS), and the two pdgemm products per band at the host BLAS rate "
+                       f"(numpy OpenBLAS, {rate / 1e9:.0f} GFLOP/s); extrapolated: {t_full:.0f} s per S->Fisher pass"),
             "s_per_step_extrapolated": t_full}
 
 

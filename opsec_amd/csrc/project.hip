@@ -79,6 +79,24 @@ struct ProjState {
     size_t cap_trig = 0;
     float* d_W = nullptr;        // inverse-DFT weights, transposed: nbat x n1p (n1p = n1 rounded up to 4)
     size_t cap_W = 0;
+    // Toeplitz projection in FP64 / FP32 (same DFT along x, no operand split)
+    bool toep64_valid = false;   // Bg64 and Bhat64 match the uploaded modes
+    int* d_gmap64 = nullptr;     // grid index -> cell number (as d_gmap)
+    size_t cap_gmap64 = 0;
+    int* d_slab = nullptr;       // slab position r -> (y << 16) | z
+    size_t cap_slab = 0;
+    double* d_trig64 = nullptr;  // cos, then sin of 2 pi j / L
+    size_t cap_trig64 = 0;
+    double* d_W64 = nullptr;     // inverse-DFT weights W[x][b], n1 x nbat
+    size_t cap_W64 = 0;
+    double* d_Bg64 = nullptr;    // B' in grid order: M x Ng (pitch ldg), zero where the grid has no cell
+    size_t cap_Bg64 = 0;
+    double* d_Bhat64 = nullptr;  // DFT of B' along x: nbat x M x G ([b][m][r])
+    size_t cap_Bhat64 = 0;
+    double* d_Yhat64 = nullptr;  // one band's stage-1 products: nbat x M x G
+    size_t cap_Yhat64 = 0;
+    double* d_Y64 = nullptr;     // one band's Y = S_p B' in grid order: M x Ng (pitch ldg)
+    size_t cap_Y64 = 0;
 };
 
 // ---- FP64 / FP32 MFMA GEMM -----------------------------------------------------
@@ -1085,6 +1103,90 @@ __global__ __launch_bounds__(256) void idft_kernel(const float* __restrict__ yha
     }
 }
 
+// ---- Toeplitz projection in FP64 / FP32 -------------------------------------------
+//
+// The same identity without splitting the operands: B' in grid order and its
+// DFT along x in FP64, stage 1 as nbat = 2 n1 batched GEMMs (batch b:
+// That_((b+1)/2) times the b-th transformed block, That generated inside the
+// GEMM's A loader from Qhat), the inverse DFT in FP64, stage 2 as the upper
+// triangle of Bg'^T Y over the grid.  MFMA in FP64 (v_mfma_f64_16x16x4f64) or
+// FP32 (v_mfma_f32_16x16x4f32, operands rounded to FP32 at the LDS store).
+
+// B' in grid order: Bg[m][g] = B'[m][gmap[g]] (0 where the grid has no cell).
+__global__ void grid_b64_kernel(const double* __restrict__ Bw, size_t ldw, const int* __restrict__ gmap, int Ng, int M,
+                                double* __restrict__ Bg, size_t ldg) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)M * Ng) return;
+    const size_t m = i / Ng, g = i - m * Ng;
+    const int a = gmap[g];
+    Bg[m * ldg + g] = a >= 0 ? Bw[m * ldw + a] : 0.0;
+}
+
+// Bhat[b][m][r] = sum_x cos(2 pi x k / L) Bg[m][x G + r] (b = 0 or odd) or
+// -sum_x sin(2 pi x k / L) Bg[m][x G + r] (b even, > 0), k = (b + 1) / 2.
+// Block (r tile, m); each thread keeps the n1 <= 64 grid values of its (m, r)
+// in registers.
+constexpr int TOEP_MAXX = 64;
+__global__ __launch_bounds__(256) void bhat64_kernel(const double* __restrict__ Bg, size_t ldg, int n1, int G, int M,
+                                                     const double* __restrict__ trig, double* __restrict__ Bhat,
+                                                     int nbat) {
+    const int m = blockIdx.y, r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= G) return;
+    const int L = 2 * n1;
+    double v[TOEP_MAXX];
+#pragma unroll
+    for (int x = 0; x < TOEP_MAXX; x++) v[x] = x < n1 ? Bg[(size_t)m * ldg + (size_t)x * G + r] : 0.0;
+    for (int b = 0; b < nbat; b++) {
+        const int k = (b + 1) >> 1;
+        const bool imag = b > 0 && !(b & 1);
+        const double* t = trig + (imag ? L : 0);
+        double acc = 0.0;
+#pragma unroll
+        for (int x = 0; x < TOEP_MAXX; x++)
+            if (x < n1) acc += t[(x * k) % L] * v[x];
+        Bhat[((size_t)b * M + m) * G + r] = imag ? -acc : acc;
+    }
+}
+
+// A operand of the FP64 Toeplitz stage 1: That_k[r][c] = Qhat_k(|y_r - y_c| n3
+// + |z_r - z_c|) with k = (batch + 1) / 2; slab[r] = (y << 16) | z.
+struct GA_That {
+    const double* qhat;
+    const int* slab;
+    int G, n3;
+    static constexpr bool k_contig = true;
+    __device__ __forceinline__ void p_shift(int z, size_t) { qhat += (size_t)((z + 1) >> 1) * G; }
+    __device__ __forceinline__ double get(int r, int c) const {
+        const int a = slab[r], b = slab[c];
+        const int dy = abs((a >> 16) - (b >> 16)), dz = abs((a & 0xffff) - (b & 0xffff));
+        return qhat[dy * n3 + dz];
+    }
+};
+
+// Y[m][x G + r] = sum_b W[b][x] Yhat[b][m][r] for x < n1 (FP64).  Block (r
+// tile, m); W (nbat x n1) in LDS, the n1 sums of a thread in registers.
+__global__ __launch_bounds__(256) void idft64_kernel(const double* __restrict__ Yhat, int nbat, int G, int n1, int M,
+                                                     const double* __restrict__ W, double* __restrict__ Y, size_t ldg) {
+    extern __shared__ double id64_w[];   // nbat x n1
+    const int m = blockIdx.y, r = blockIdx.x * 256 + threadIdx.x;
+    for (int i = threadIdx.x; i < nbat * n1; i += 256) id64_w[i] = W[i];
+    __syncthreads();
+    if (r >= G) return;
+    double acc[TOEP_MAXX];
+#pragma unroll
+    for (int x = 0; x < TOEP_MAXX; x++) acc[x] = 0.0;
+    for (int b = 0; b < nbat; b++) {
+        const double yh = Yhat[((size_t)b * M + m) * G + r];
+        const double* w = id64_w + b * n1;
+#pragma unroll
+        for (int x = 0; x < TOEP_MAXX; x++)
+            if (x < n1) acc[x] += w[x] * yh;
+    }
+#pragma unroll
+    for (int x = 0; x < TOEP_MAXX; x++)
+        if (x < n1) Y[(size_t)m * ldg + (size_t)x * G + r] = acc[x];
+}
+
 // ---- host --------------------------------------------------------------------------
 
 static dim3 grid_for(int M, int N, int nb) {
@@ -1433,9 +1535,141 @@ static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
     return OPSEC_OK;
 }
 
+// Toeplitz projection without operand splitting: FP64 (prec FP64) or FP32
+// (prec FP32) MFMA, FP64 DFTs.  Per band: Qhat (FP64 DFT of the table along
+// x), the batched stage-1 GEMMs into Yhat, the inverse DFT into Y (grid
+// order), the upper-triangle stage-2 GEMM into the packed result.
+static int run_project_toeplitz64(opsec_ctx* ctx, int nb, const int* funcs, int prec) {
+    ProjState* ps = ctx->proj;
+    const SurveyDev& sv = ctx->sv;
+    const int N = ctx->ncells, M = ctx->nmodes;
+    const int n1 = sv.n1, G = sv.n2 * sv.n3, Ng = n1 * G, L = 2 * n1, nbat = 2 * n1;
+    if (sv.coordsys != OPSEC_COORD_CARTESIAN) return set_error(ctx, OPSEC_EINVAL, "Toeplitz projection needs a box survey");
+    if (n1 > TOEP_MAXX) return set_error(ctx, OPSEC_EINVAL, "Toeplitz projection supports Nx <= %d (Nx = %d)", TOEP_MAXX, n1);
+    if (sv.n2 >= 65536 || sv.n3 >= 65536) return set_error(ctx, OPSEC_EINVAL, "Toeplitz projection: Ny, Nz < 65536");
+    const size_t ldg = (size_t)Ng + 32;   // pitch of the grid-order operands (not a power of two)
+    int rc;
+    if ((rc = dev_realloc(ctx, (void**)&ps->d_Yhat64, &ps->cap_Yhat64, (size_t)nbat * M * G * 8))) return rc;
+    if ((rc = dev_realloc(ctx, (void**)&ps->d_Y64, &ps->cap_Y64, (size_t)M * ldg * 8))) return rc;
+    if ((rc = dev_realloc(ctx, (void**)&ps->d_Qhat, &ps->cap_Qhat, (size_t)(n1 + 1) * G * 8))) return rc;
+    if ((rc = dev_realloc(ctx, &ps->d_P, &ps->cap_P, (size_t)nb * packed_size(M) * 8))) return rc;
+    if (!ps->toep64_valid) {
+        std::vector<int> gmap((size_t)Ng, -1);
+        for (int a = 0; a < N; a++) {
+            const opsec_cell& c = ctx->h_cells[a];
+            const size_t g = ((size_t)c.d1 * sv.n2 + c.d2) * sv.n3 + c.d3;
+            if (gmap[g] >= 0)
+                return set_error(ctx, OPSEC_EINVAL, "cells %d and %d share a grid position", gmap[g], a);
+            gmap[g] = a;
+        }
+        std::vector<int> slab((size_t)G);
+        for (int r = 0; r < G; r++) slab[r] = ((r / sv.n3) << 16) | (r % sv.n3);
+        std::vector<double> trig;
+        trig_table(L, trig);
+        // inverse weights W[b][x]: (c_k / L) cos and -(c_k / L) sin, c_0 = c_n1 = 1, else 2
+        std::vector<double> W((size_t)nbat * n1);
+        for (int b = 0; b < nbat; b++)
+            for (int x = 0; x < n1; x++) {
+                const int k = (b + 1) >> 1, j = (x * k) % L;
+                const double ck = (k == 0 || k == n1) ? 1.0 : 2.0;
+                const bool imag = b > 0 && !(b & 1);
+                W[(size_t)b * n1 + x] = imag ? -ck * trig[L + j] / L : ck * trig[j] / L;
+            }
+        if ((rc = dev_realloc(ctx, (void**)&ps->d_gmap64, &ps->cap_gmap64, (size_t)Ng * 4))) return rc;
+        if ((rc = dev_realloc(ctx, (void**)&ps->d_slab, &ps->cap_slab, (size_t)G * 4))) return rc;
+        if ((rc = dev_realloc(ctx, (void**)&ps->d_trig64, &ps->cap_trig64, trig.size() * 8))) return rc;
+        if ((rc = dev_realloc(ctx, (void**)&ps->d_W64, &ps->cap_W64, W.size() * 8))) return rc;
+        if ((rc = dev_realloc(ctx, (void**)&ps->d_Bg64, &ps->cap_Bg64, (size_t)M * ldg * 8))) return rc;
+        if ((rc = dev_realloc(ctx, (void**)&ps->d_Bhat64, &ps->cap_Bhat64, (size_t)nbat * M * G * 8))) return rc;
+        OPSEC_HIP_TRY(ctx, stream_copy(ctx, ps->d_gmap64, gmap.data(), gmap.size() * 4, hipMemcpyHostToDevice));
+        OPSEC_HIP_TRY(ctx, stream_copy(ctx, ps->d_slab, slab.data(), slab.size() * 4, hipMemcpyHostToDevice));
+        OPSEC_HIP_TRY(ctx, stream_copy(ctx, ps->d_trig64, trig.data(), trig.size() * 8, hipMemcpyHostToDevice));
+        OPSEC_HIP_TRY(ctx, stream_copy(ctx, ps->d_W64, W.data(), W.size() * 8, hipMemcpyHostToDevice));
+        const size_t tot = (size_t)M * Ng;
+        hipLaunchKernelGGL(grid_b64_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, ps->d_Bw,
+                           ps->ldw, ps->d_gmap64, Ng, M, ps->d_Bg64, ldg);
+        hipLaunchKernelGGL(bhat64_kernel, dim3((unsigned)((G + 255) / 256), (unsigned)M), dim3(256), 0, ctx->stream,
+                           ps->d_Bg64, ldg, n1, G, M, ps->d_trig64, ps->d_Bhat64, nbat);
+        OPSEC_HIP_TRY(ctx, hipGetLastError());
+        ps->toep64_valid = true;
+    }
+    const size_t id_lds = (size_t)nbat * n1 * 8;
+    OPSEC_HIP_TRY(ctx, hipFuncSetAttribute((const void*)idft64_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)id_lds));
+    std::vector<hipEvent_t> ev(4 * (size_t)nb);
+    for (auto& e : ev) OPSEC_HIP_TRY(ctx, hipEventCreate(&e));
+    const size_t np = (size_t)packed_size(M);
+    GA_That A;
+    A.qhat = ps->d_Qhat;
+    A.slab = ps->d_slab;
+    A.G = G;
+    A.n3 = sv.n3;
+    GB_ColMajor Bop;   // batch b: Bhat[b] (B[k = r][n = m] = p[m G + r])
+    Bop.p = ps->d_Bhat64;
+    Bop.ld = G;
+    PutColMajor<double> toYhat{ps->d_Yhat64, (size_t)G, (size_t)M * G};
+    GA_ColMajorT A2;   // Bg'^T: A2[m][g] = Bg[m ldg + g]
+    A2.p = ps->d_Bg64;
+    A2.ld = ldg;
+    GB_ColMajorT<double> B2{ps->d_Y64, ldg};
+    const dim3 g1 = grid_for(G, M, nbat), g2 = grid_for(M, M, 1);
+    for (int z = 0; z < nb; z++) {
+        const double* Qf = ctx->d_Q + (size_t)funcs[z] * ctx->nkeys;
+        PutPacked outP{(double*)ps->d_P + (size_t)z * np, 0};
+        hipEventRecord(ev[4 * z], ctx->stream);
+        hipLaunchKernelGGL(qhat_kernel, dim3((unsigned)(((n1 + 1) * G + 255) / 256)), dim3(256), 0, ctx->stream, Qf, n1,
+                           G, ps->d_trig64, ps->d_Qhat);
+        hipEventRecord(ev[4 * z + 1], ctx->stream);
+        if (prec == OPSEC_PREC_FP64)
+            hipLaunchKernelGGL((gemm_kernel<double, GA_That, GB_ColMajor, PutColMajor<double>, false>), g1,
+                               dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, (size_t)M * G, toYhat, G, M, G);
+        else
+            hipLaunchKernelGGL((gemm_kernel<float, GA_That, GB_ColMajor, PutColMajor<double>, false>), g1,
+                               dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, (size_t)M * G, toYhat, G, M, G);
+        hipLaunchKernelGGL(idft64_kernel, dim3((unsigned)((G + 255) / 256), (unsigned)M), dim3(256), id_lds,
+                           ctx->stream, ps->d_Yhat64, nbat, G, n1, M, ps->d_W64, ps->d_Y64, ldg);
+        hipEventRecord(ev[4 * z + 2], ctx->stream);
+        if (prec == OPSEC_PREC_FP64)
+            hipLaunchKernelGGL((gemm_kernel<double, GA_ColMajorT, GB_ColMajorT<double>, PutPacked, true>), g2,
+                               dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, 0, outP, M, M, Ng);
+        else
+            hipLaunchKernelGGL((gemm_kernel<float, GA_ColMajorT, GB_ColMajorT<double>, PutPacked, true>), g2,
+                               dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, 0, outP, M, M, Ng);
+        hipEventRecord(ev[4 * z + 3], ctx->stream);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = set_error(ctx, OPSEC_EHIP, "projection launch: %s", hipGetErrorString(e));
+    hipError_t es = hipEventSynchronize(ev[4 * nb - 1]);
+    if (!rc && es != hipSuccess) rc = set_error(ctx, OPSEC_EHIP, "projection: %s", hipGetErrorString(es));
+    float tg = 0, t1 = 0, t2 = 0;
+    if (!rc)
+        for (int z = 0; z < nb; z++) {
+            float a, b, c;
+            hipEventElapsedTime(&a, ev[4 * z], ev[4 * z + 1]);
+            hipEventElapsedTime(&b, ev[4 * z + 1], ev[4 * z + 2]);
+            hipEventElapsedTime(&c, ev[4 * z + 2], ev[4 * z + 3]);
+            tg += a;
+            t1 += b;
+            t2 += c;
+        }
+    for (auto& x : ev) hipEventDestroy(x);
+    if (rc) return rc;
+    ps->last_ms_gen = tg;
+    ps->last_ms_gemm1 = t1;     // stage-1 GEMMs and the inverse DFT
+    ps->last_ms_gemm2 = t2;
+    ps->last_launches = nb;
+    ps->last_launches2 = nb;
+    ps->last_ms_stage1 = tg + t1;
+    ps->last_ms_stage2 = t2;
+    ps->nb = nb;
+    ps->prec = prec;
+    return OPSEC_OK;
+}
+
 // Stage 1 + stage 2 for nb functions (device pair tables), results to d_P.
 static int run_project(opsec_ctx* ctx, int nb, const int* d_funcs, const int* funcs, int prec) {
     if (prec == OPSEC_PREC_BF16X3 && ctx->projection == OPSEC_PROJ_TOEPLITZ) return run_project_toeplitz(ctx, nb, funcs);
+    if (ctx->projection == OPSEC_PROJ_TOEPLITZ) return run_project_toeplitz64(ctx, nb, funcs, prec);
     if (prec == OPSEC_PREC_BF16X3) return run_project_bf16x3(ctx, nb, funcs);
     ProjState* ps = ctx->proj;
     const int N = ctx->ncells, M = ctx->nmodes;
@@ -1519,6 +1753,7 @@ void invalidate_modes(opsec_ctx* ctx) {
     if (!ctx->proj) return;
     ctx->proj->bp_valid = false;
     ctx->proj->toep_valid = false;
+    ctx->proj->toep64_valid = false;
     ctx->proj->nb = 0;
 }
 
@@ -1542,6 +1777,14 @@ void free_proj(opsec_ctx* ctx) {
     dev_free((void**)&ctx->proj->d_Cg);
     dev_free((void**)&ctx->proj->d_trig);
     dev_free((void**)&ctx->proj->d_W);
+    dev_free((void**)&ctx->proj->d_gmap64);
+    dev_free((void**)&ctx->proj->d_slab);
+    dev_free((void**)&ctx->proj->d_trig64);
+    dev_free((void**)&ctx->proj->d_W64);
+    dev_free((void**)&ctx->proj->d_Bg64);
+    dev_free((void**)&ctx->proj->d_Bhat64);
+    dev_free((void**)&ctx->proj->d_Yhat64);
+    dev_free((void**)&ctx->proj->d_Y64);
     delete ctx->proj;
     ctx->proj = nullptr;
 }
@@ -1577,6 +1820,7 @@ int opsec_modes_upload(opsec_ctx* ctx, const double* B, int ncells, int nmodes) 
     ctx->nmodes = nmodes;
     ps->bp_valid = false;
     ps->toep_valid = false;
+    ps->toep64_valid = false;
     return OPSEC_OK;
 }
 

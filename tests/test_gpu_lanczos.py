@@ -79,21 +79,23 @@ def test_klt_box_multiplets(ctx):
 
 
 def test_auto_krylov_retry_keeps_and_improves(ctx):
-    """A flat spectrum defeats the first Krylov space (maxdim = 3 nev + 4 b):
-    the automatic retries with twice the space converge more modes, and the
-    result never has fewer converged modes than the first pass."""
+    """A slowly decaying spectrum defeats the first Krylov space (maxdim =
+    3 nev + 4 b): the automatic retries with twice the space converge more
+    modes, and the result never has fewer converged modes than the first pass.
+    (A numpy block Lanczos of the same construction converges 0, 20 and 40 of
+    the 40 modes with spaces of 192, 368 and 736.)"""
     n, nev, block = 3000, 40, 16
     rng = np.random.default_rng(11)
     Q = np.linalg.qr(rng.normal(size=(n, n)))[0]
-    lam = 1.0 - 0.02 * np.sort(rng.uniform(0, 1, n))   # all within 2 % of the largest
+    lam = np.exp(-20.0 * np.arange(n) / n)
     A = (Q * lam) @ Q.T
     A = 0.5 * (A + A.T)
     first = 3 * nev + 4 * block
     ev1, _, nconv1 = ctx.lanczos_dense(A, nev, tol=1e-9, block=block, maxdim=first)
     ev, V, nconv = ctx.lanczos_dense(A, nev, tol=1e-9, block=block)
     print(f"flat spectrum: first pass {nconv1} of {nev} converged, automatic retries {nconv}")
-    assert nconv >= nconv1
-    assert nconv > nconv1 or nconv1 == nev
+    assert nconv1 < nev
+    assert nconv > nconv1
     w = np.sort(lam)[::-1][:nev]
     np.testing.assert_allclose(ev[:nconv], w[:nconv], atol=1e-8)
 

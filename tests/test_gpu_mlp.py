@@ -111,7 +111,8 @@ def test_training_and_ablation(ctx):
     opts = ctx.mlp_opts(nsamples=60000, epochs=30, batch=2048, seed=7, lr=2e-3, near=2.0)
     loss = ctx.mlp_train(opts, q)
     print("final training loss (residual / per-band RMS)", loss)
-    assert loss < 0.2
+    # training is deterministic (fixed-order gradient sums): observed 0.0087
+    assert loss < 0.03
     funcs = list(range(1, nb + 1))
     # quadrature-only tables
     ctx.pair_tables(funcs, q)

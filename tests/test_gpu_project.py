@@ -273,12 +273,20 @@ def test_project_toeplitz_vs_fp64(ctx, dims, drop, shuffle):
         ctx.set_projection(lib.OPSEC_PROJ_TOEPLITZ)
         got = ctx.project(funcs, lib.OPSEC_PREC_BF16X3)
         again = ctx.project(funcs, lib.OPSEC_PREC_BF16X3)
+        got64 = ctx.project(funcs, lib.OPSEC_PREC_FP64)
+        got32 = ctx.project(funcs, lib.OPSEC_PREC_FP32)
     finally:
         ctx.set_projection(lib.OPSEC_PROJ_DENSE)
     assert np.array_equal(got, again)
     err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
     print("Toeplitz BF16x3 projection", dims, "max rel err per function", err)
     assert err.max() < 5e-5
+    # the same identity in FP64 (FP64 DFTs and MFMA) and in FP32 MFMA
+    err64 = np.abs(got64 - ref).max(axis=1) / np.abs(ref).max(axis=1)
+    err32 = np.abs(got32 - ref).max(axis=1) / np.abs(ref).max(axis=1)
+    print("Toeplitz FP64 / FP32 projection", dims, "max rel err", err64.max(), err32.max())
+    assert err64.max() < 1e-12
+    assert err32.max() < 2e-5
 
 
 def test_project_toeplitz_rejects_shared_grid_position(ctx):
@@ -291,8 +299,9 @@ def test_project_toeplitz_rejects_shared_grid_position(ctx):
     ctx.modes_upload(np.linalg.qr(np.random.default_rng(0).normal(size=(len(dup), 8)))[0].T.copy())
     try:
         ctx.set_projection(lib.OPSEC_PROJ_TOEPLITZ)
-        with pytest.raises(lib.OpsecError, match="share a grid position"):
-            ctx.project([1], lib.OPSEC_PREC_BF16X3)
+        for prec in (lib.OPSEC_PREC_BF16X3, lib.OPSEC_PREC_FP64):
+            with pytest.raises(lib.OpsecError, match="share a grid position"):
+                ctx.project([1], prec)
     finally:
         ctx.set_projection(lib.OPSEC_PROJ_DENSE)
 

@@ -181,6 +181,17 @@ def test_pipeline_end_to_end_vs_oracle(tmp_path):
         O.oracle().orc_project(N, O.ptr(mats[p + 1]), M, O.ptr(np.ascontiguousarray(B)), O.ptr(ref))
         np.testing.assert_allclose(D[p], ref, rtol=1e-10, atol=1e-11 * np.abs(ref).max())
 
+    # the FP64 Toeplitz projection (DFT along x, FP64 MFMA) against the oracle
+    r = run("comma", "-c", str(cfile), f"Nmodes={M}", f"modefile={modes_split}", f"covfile={tmp_path}/cov_t64.abn",
+            "hip.precision=fp64", "hip.projection=toeplitz")
+    assert r.returncode == 0, r.stderr
+    _, (Dt64,) = formats.read_doubles(tmp_path / "cov_t64.abn")
+    Dt64 = Dt64.reshape(nb, K)
+    for p in range(nb):
+        ref = np.zeros(K)
+        O.oracle().orc_project(N, O.ptr(mats[p + 1]), M, O.ptr(np.ascontiguousarray(B)), O.ptr(ref))
+        np.testing.assert_allclose(Dt64[p], ref, rtol=1e-10, atol=1e-11 * np.abs(ref).max())
+
     # the mixed-precision Toeplitz projection of the same bands
     r = run("comma", "-c", str(cfile), f"Nmodes={M}", f"modefile={modes_split}", f"covfile={tmp_path}/cov_t.abn",
             "hip.precision=bf16x3", "hip.projection=toeplitz")

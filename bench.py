@@ -399,8 +399,12 @@ def main():
                     help="also attribute the Fisher error to the surrogate and to the mixed precision (untimed)")
     ap.add_argument("--nt-tile", type=int, default=2, choices=[0, 1, 2],
                     help="split-BF16x3 GEMM tile: 0 = 128 x 128, 1 = 256 x 128, 2 = 256 x 256 double-buffered")
-    ap.add_argument("--kl", default="fourier", choices=["fourier", "lanczos"],
-                    help="KL modes (setup, untimed): the periodic box's analytic Fourier modes, or the klt solver")
+    ap.add_argument("--kl", default="lanczos", choices=["fourier", "lanczos"],
+                    help="KL modes (setup, untimed): the klt solver (block Lanczos on the device-resident S, "
+                         "checked against the circulant's spectrum) or the box's analytic Fourier modes")
+    ap.add_argument("--aux-steps", type=int, default=3,
+                    help="timed steps of the comparison paths (FP64 dense and Toeplitz, quadrature-only mixed, "
+                         "dense mixed); the headline path runs --steps")
     ap.add_argument("--nt-gen", type=int, default=0, choices=[0, 1],
                     help="stage 1 generates S_p from the pair table (1) or reads S_p planes (0)")
     ap.add_argument("--projection", default="toeplitz", choices=["dense", "toeplitz"],
@@ -455,13 +459,21 @@ def main():
     if args.kl == "lanczos":
         # the klt path: block Lanczos on the device-resident S (8.6 GB at 32^3),
         # checked against the circulant's spectrum (real DFT of the generator)
+        import ctypes as C
         t1 = time.perf_counter()
         ev, V, nconv = ctx.klt(M, tol=1e-8)
         t_klt = time.perf_counter() - t1
         err = float(np.max(np.abs(ev - lam)) / lam[0])
-        klt_info = {"s": t_klt, "nconv": int(nconv), "eigenvalue_max_err_over_lambda_max_vs_dft": err}
+        ph = (C.c_double * 9)()
+        ctx.lib.opsec_diag_klt_times(ph, 9)
+        names = ("block_matvecs", "orthogonalisation", "ritz_tridiagonal_reduction_gpu", "ritz_tridiagonal_eigen_host",
+                 "ritz_back_transformation", "ritz_vectors")
+        klt_info = {"s": t_klt, "nconv": int(nconv), "eigenvalue_max_err_over_lambda_max_vs_dft": err,
+                    "phases_ms": {nm: float(ph[i]) for i, nm in enumerate(names)},
+                    "krylov_dim": int(ph[6]), "rayleigh_ritz_steps": int(ph[7]), "block_steps": int(ph[8]),
+                    "tol": 1e-8}
         log(f"setup: klt (block Lanczos) {M} modes in {t_klt:.1f} s, nconv {nconv}, "
-            f"max |lambda - DFT| / lambda_max {err:.2e}")
+            f"max |lambda - DFT| / lambda_max {err:.2e}; phases {klt_info['phases_ms']}, K {int(ph[6])}")
         lam, B = ev, V
     ctx.modes_upload(B)
     prior = ctx.params()
@@ -492,12 +504,26 @@ def main():
     log(f"surrogate trained on {opts.nsamples} exact integrals in {t_train:.1f} s, final loss {loss:.3e}")
 
     results = {}
+    aux_steps, aux_warm = min(args.steps, args.aux_steps), min(args.warmup, 1)
     if args.path in ("both", "fp64"):
+        # the reference's algorithm in FP64: dense projection (comma's two pdgemm per band)
+        ctx.set_projection(L.OPSEC_PROJ_DENSE)
         ms, est64, k64 = run_path(ctx, L, funcs, q64, L.OPSEC_PREC_FP64, L.OPSEC_PREC_FP64, lam, y, prior,
-                                  args.steps, args.warmup, "fp64")
+                                  aux_steps, aux_warm, "fp64")
         st = ctx.pair_stats()
         results["fp64"] = dict(ms=ms, est=est64, kms=k64, stats=st)
         tables64 = [ctx.pair_table(f) for f in funcs]   # the quadrature tables of S and every C_{,p}
+        # the same Toeplitz algorithm as the headline path, in FP64
+        ctx.set_projection(L.OPSEC_PROJ_TOEPLITZ)
+        ms_t, est_t, k_t = run_path(ctx, L, funcs, q64, L.OPSEC_PREC_FP64, L.OPSEC_PREC_FP64, lam, y, prior,
+                                    aux_steps, aux_warm, "fp64 toeplitz")
+        results["fp64_toeplitz"] = dict(ms=ms_t, est=est_t, kms=k_t)
+        ctx.set_projection(L.OPSEC_PROJ_TOEPLITZ if args.projection == "toeplitz" else L.OPSEC_PROJ_DENSE)
+        if args.path == "both":
+            # the mixed path on the quadrature tables: what the surrogate alone saves
+            ms_q, est_q, k_q = run_path(ctx, L, funcs, q64, L.OPSEC_PREC_BF16X3, L.OPSEC_PREC_FP32, lam, y, prior,
+                                        aux_steps, aux_warm, "mixed, quadrature tables")
+            results["mixed_quadrature"] = dict(ms=ms_q, est=est_q, kms=k_q)
     if args.path in ("both", "surrogate"):
         ms, ests, ks = run_path(ctx, L, funcs, qs, L.OPSEC_PREC_BF16X3, L.OPSEC_PREC_FP32, lam, y, prior,
                                 args.steps, args.warmup, "surrogate")
@@ -520,7 +546,7 @@ def main():
             # the same mixed path with the dense projection, for comparison
             ctx.set_projection(L.OPSEC_PROJ_DENSE)
             ms_d, est_d, ks_d = run_path(ctx, L, funcs, qs, L.OPSEC_PREC_BF16X3, L.OPSEC_PREC_FP32, lam, y, prior,
-                                         args.steps, 1, "surrogate, dense projection")
+                                         aux_steps, aux_warm, "surrogate, dense projection")
             ctx.set_projection(L.OPSEC_PROJ_TOEPLITZ)
             results["surrogate_dense"] = dict(ms=ms_d, est=est_d, kms=ks_d)
 
@@ -545,12 +571,26 @@ def main():
                                f"quadrature", "path": main_path,
                    "projection": args.projection if main_path == "surrogate" else "dense"},
     }
+    # distinct (pair, function) integrals behind one step: the value counts the
+    # N^2 (1 + nb) matrix entries the step delivers, the Toeplitz path never
+    # forms an entry of S, and a step integrates each distinct key once
+    nkeys = ctx.nkeys()
+    out["distinct_integrals_per_step"] = nkeys * (nb + 1)
+    out["distinct_integrals_per_s"] = nkeys * (nb + 1) / (R["ms"] / 1e3)
     if "fp64" in results:
         k64 = results["fp64"]["kms"]
-        out["fp64_path"] = {"ms_per_step": results["fp64"]["ms"],
+        out["fp64_path"] = {"ms_per_step": results["fp64"]["ms"], "steps": aux_steps,
                             "value": pairs / (results["fp64"]["ms"] / 1e3),
+                            "projection": "dense (the reference's two GEMMs per band)",
                             "projection_ms": [float(k64[0]), float(k64[1])],
                             "stage1_tflops": 2.0 * N * N * M * nb / (float(k64[0]) / 1e3) / 1e12}
+        kt = results["fp64_toeplitz"]["kms"]
+        out["fp64_toeplitz_path"] = {"ms_per_step": results["fp64_toeplitz"]["ms"], "steps": aux_steps,
+                                     "value": pairs / (results["fp64_toeplitz"]["ms"] / 1e3),
+                                     "projection_ms": [float(kt[0]), float(kt[1])],
+                                     "stage1_tflops": 2.0 * (2 * n) * n ** 4 * M * nb / (float(kt[3]) / 1e3) / 1e12,
+                                     "stage2_tflops": float(N) * M * M * nb / (float(kt[4]) / 1e3) / 1e12}
+        out["fp64_toeplitz_path"].update(compare_estimates(results["fp64"]["est"], results["fp64_toeplitz"]["est"]))
     # pair quadrature (K1): distinct integrals and node evaluations per second
     # against the device's measured FP64 sincos rate (band nodes need two pairs)
     import ctypes as C
@@ -581,6 +621,17 @@ def main():
                                                                   results["surrogate_dense"]["est"]))
     if "fp64" in results and "surrogate" in results:
         out["speedup_vs_fp64"] = results["fp64"]["ms"] / results["surrogate"]["ms"]
+        out["speedup_vs_fp64_same_algorithm"] = results["fp64_toeplitz"]["ms"] / results["surrogate"]["ms"]
+        if "mixed_quadrature" in results:
+            mq = results["mixed_quadrature"]
+            out["mixed_quadrature_path"] = {"ms_per_step": mq["ms"], "steps": aux_steps,
+                                            "pair_tables_ms": float(mq["kms"][6])}
+            out["mixed_quadrature_path"].update(compare_estimates(results["fp64"]["est"], mq["est"]))
+            out["surrogate_share"] = {
+                "step_ms_saved": mq["ms"] - results["surrogate"]["ms"],
+                "fraction_of_step": (mq["ms"] - results["surrogate"]["ms"]) / mq["ms"],
+                "pair_tables_ms_quadrature": float(mq["kms"][6]),
+                "pair_tables_ms_surrogate": float(results["surrogate"]["kms"][6])}
         out.update(compare_estimates(results["fp64"]["est"], results["surrogate"]["est"]))
         st = results["surrogate"]["stats"]
         out["surrogate_stats"] = {k2: int(v) for k2, v in st.items()}

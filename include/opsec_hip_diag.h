@@ -28,6 +28,14 @@ int opsec_diag_sincos_rate(opsec_ctx* ctx, double* pairs_per_s);
  * solver's host QL + inverse iteration; Z is n x nev column-major. */
 int opsec_diag_tridiag_top(int n, const double* d, const double* e, int nev, double* evals, double* Z);
 
+/* Phases of the last Lanczos solve (opsec_klt / opsec_lanczos_dense; the
+ * last pass when retries ran): out[0..5] wall-clock ms of the block matvecs,
+ * the orthogonalisation, the Rayleigh-Ritz tridiagonal reduction (GPU), the
+ * host tridiagonal eigenpairs, the back-transformation and the Ritz vectors;
+ * out[6] the final Krylov dimension, out[7] the Rayleigh-Ritz steps, out[8]
+ * the block steps.  Returns the number of entries (9) or OPSEC_EINVAL. */
+int opsec_diag_klt_times(double* out, int n);
+
 /* Timings of the last opsec_project call (ms): stage 1 and stage 2 totals;
  * for the split-BF16x3 paths also plane generation / DFTs, the stage-1 and
  * stage-2 GEMMs and the launch counts. */

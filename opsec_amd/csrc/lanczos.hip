@@ -19,6 +19,7 @@
 //   * convergence: |B_j z_last| <= tol |theta_max| for each wanted Ritz pair.
 // Output: eigenvalues descending, eigenvectors mode-major (modefile layout).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -127,6 +128,23 @@ __global__ void fill_kernel(double* p, size_t n, double v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
+
+// Wall-clock phases of the last solve (ms): block matvecs, orthogonalisation,
+// Rayleigh-Ritz tridiagonal reduction on the GPU, host tridiagonal eigenpairs,
+// back-transformation, Ritz vectors; then the Krylov dimension, the number of
+// Rayleigh-Ritz steps and of block steps.
+enum { KT_MATVEC, KT_ORTH, KT_TRIRED, KT_TRIEIG, KT_BACK, KT_RITZVEC, KT_KDIM, KT_NRR, KT_NSTEPS, KT_N };
+static double g_klt_times[KT_N];
+
+struct PhaseTimer {
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double lap() {
+        auto t1 = std::chrono::steady_clock::now();
+        double ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        t0 = t1;
+        return ms;
+    }
+};
 
 struct Dev {
     opsec_ctx* ctx;
@@ -523,6 +541,7 @@ static int dense_eigen_top(Dev& D, double* dA, int K, int nev, std::vector<doubl
     OPSEC_HIP_TRY(ctx, tmp.alloc(&de, (size_t)K * 8 + 8));
     OPSEC_HIP_TRY(ctx, tmp.alloc(&dp, (size_t)std::max(K, nev) * 8 + 8));
     OPSEC_HIP_TRY(ctx, tmp.alloc(&dd, (size_t)K * 8 + 8));
+    PhaseTimer pt;
     for (int k = 0; k + 2 < K; k++) {
         const int m = K - k - 1;
         hipLaunchKernelGGL(hh_vector_kernel, dim3(1), dim3(1024), 0, D.s, dA, K, k, dtau, de);
@@ -537,16 +556,21 @@ static int dense_eigen_top(Dev& D, double* dA, int K, int nev, std::vector<doubl
     if (rc) return rc;
     OPSEC_HIP_TRY(ctx, stream_copy(ctx, d.data(), dd, (size_t)K * 8, hipMemcpyDeviceToHost));
     if (K >= 2) OPSEC_HIP_TRY(ctx, stream_copy(ctx, e.data(), de, (size_t)(K - 1) * 8, hipMemcpyDeviceToHost));
+    g_klt_times[KT_TRIRED] += pt.lap();
     std::vector<double> Z;
     tridiag_top(K, d, e, nev, evals, Z);
     OPSEC_HIP_TRY(ctx, stream_copy(ctx, dZ, Z.data(), (size_t)K * nev * 8, hipMemcpyHostToDevice));
+    g_klt_times[KT_TRIEIG] += pt.lap();
     for (int k = K - 3; k >= 0; k--) {
         const int m = K - k - 1;
         hipLaunchKernelGGL(hh_apply_dot_kernel, dim3(nev), dim3(256), 0, D.s, dA, K, k, dZ, nev, dp);
         hipLaunchKernelGGL(hh_apply_upd_kernel, dim3((m + 255) / 256, nev), dim3(256), 0, D.s, dA, K, k, dtau, dZ,
                            nev, dp);
     }
-    return D.sync();
+    int rc2 = D.sync();
+    g_klt_times[KT_BACK] += pt.lap();
+    g_klt_times[KT_NRR] += 1;
+    return rc2;
 }
 
 // ---- block orthonormalisation (SVQB) -------------------------------------------
@@ -672,6 +696,7 @@ static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double
                          double* evecs, int* nconv) {
     Dev D(ctx, ctx->stream);
     OPSEC_HIP_TRY(ctx, D.init());
+    for (double& t : g_klt_times) t = 0.0;
     nev = std::min(nev, N);
     if (b <= 0) b = std::min(128, std::max(1, N / 4));
     b = std::max(1, std::min(b, N));
@@ -731,7 +756,11 @@ static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double
         double* Vj = L.V + (size_t)j * b * N;
         const int K = (j + 1) * b;              // basis size including block j
         double* Xn = L.V + (size_t)K * N;       // next block slot (ncols = Kmax + b)
+        PhaseTimer pt;
         D.apply(S, N, Vj, b, Xn);               // Xn = S V_j
+        if ((rc = D.sync())) break;
+        g_klt_times[KT_MATVEC] += pt.lap();
+        g_klt_times[KT_NSTEPS] += 1;
         D.gemm_tn(Vj, N, Xn, N, N, b, b, L.small);   // A_j = V_j^T S V_j
         if ((rc = D.sync())) break;
         std::vector<double> Aj((size_t)b * b);
@@ -743,6 +772,7 @@ static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double
         std::vector<double> Bm((size_t)b * b, 0.0);   // coupling: S V_j = ... + V_{j+1} Bm
         if (K < N) {
             if ((rc = orthonormalise_block(D, L, N, b, K, Xn, Bm, rng))) break;
+            g_klt_times[KT_ORTH] += pt.lap();
             if (!last)
                 for (int r = 0; r < b; r++)
                     for (int c = 0; c < b; c++) {
@@ -809,8 +839,11 @@ static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double
             if (ymem.alloc(&dY, (size_t)N * want * 8) != hipSuccess) {
                 rc = set_error(ctx, OPSEC_ENOMEM, "Ritz vector output");
             } else {
+                PhaseTimer pt;
                 D.gemm_nn(0, dY, N, L.V, N, dZ, Kr, N, Kr, want);
                 rc = D.sync();
+                g_klt_times[KT_RITZVEC] += pt.lap();
+                g_klt_times[KT_KDIM] = Kr;
                 if (!rc) {
                     hipError_t e = stream_copy(ctx, evecs, dY, (size_t)N * want * 8, hipMemcpyDeviceToHost);
                     if (e != hipSuccess) rc = set_error(ctx, OPSEC_EHIP, "copy eigenvectors: %s", hipGetErrorString(e));
@@ -887,6 +920,12 @@ int opsec_klt(opsec_ctx* ctx, int nev, double tol, int block, int maxdim, double
     int rc = opsec_assemble_device(ctx, 0, dS, N);
     if (!rc) rc = lanczos_auto(ctx, dS, N, nev, tol, block, maxdim, evals, evecs, nconv);
     return rc;
+}
+
+int opsec_diag_klt_times(double* out, int n) {
+    if (!out || n < 1) return OPSEC_EINVAL;
+    for (int i = 0; i < n && i < KT_N; i++) out[i] = g_klt_times[i];
+    return KT_N;
 }
 
 // Host tridiagonal eigen-solver, exposed for CPU unit tests (not in the public header).

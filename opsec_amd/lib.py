@@ -138,6 +138,7 @@ _DIAG_SIGS = {
     "opsec_diag_copy": ([_P, _P, _P, C.c_size_t, _F], _I),
     "opsec_diag_sincos_rate": ([_P, C.POINTER(_D)], _I),
     "opsec_diag_tridiag_top": ([_I, _P, _P, _I, _P, _P], _I),
+    "opsec_diag_klt_times": ([_P, _I], _I),
     "opsec_diag_project_times": ([_P, _F, _F], _I),
     "opsec_diag_project_kernel_ms": ([_P, _F, _F, _F, C.POINTER(_I)], _I),
     "opsec_diag_project_launches": ([_P, C.POINTER(_I), C.POINTER(_I)], _I),

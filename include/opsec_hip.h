@@ -101,12 +101,26 @@ int opsec_model_real(opsec_ctx* ctx, const double* pk_k, const double* pk_P, int
  * Two parameters; derivative functions follow the reference's definitions. */
 int opsec_model_test(opsec_ctx* ctx, int form, double A, double b);
 
+/* A model given by its tables: nfuncs functions (f = 0 the prior xi, f = 1+p
+ * the derivative for parameter p), each as its values at the 768 QSpline
+ * nodes r_i (i < 512: r = 200 i / 512; then 256 nodes log-spaced over
+ * [200, 20000], QSpline::MakeArray, src/QSpline.cpp:6-17; the nodes are what
+ * opsec_model_table returns in r768).  xi768 holds nfuncs x 768 values;
+ * params the nfuncs - 1 prior parameter values p_n (NULL: zeros).  A RealModel
+ * XiFunc (src/RealModel.cpp:19-30, src/XiFunc.h:22-44) is exactly such a
+ * QSpline, so a caller holding an XiFunc -- ComputeSignalMatrixBlock(...,
+ * const XiFunc& xi, ...), src/sig.h:20-25 -- uploads xi at these nodes and
+ * passes its index as f: the device interpolates as QSpline::operator()
+ * does (src/QSpline.cpp:61-70) and gives the same values. */
+int opsec_model_tables(opsec_ctx* ctx, int nfuncs, const double* xi768, const double* params);
+
 int opsec_model_nparams(opsec_ctx* ctx, int* nparams);
 /* Prior parameter values p_n (RealModel::GetParam, src/RealModel.cpp:85-96). */
 int opsec_model_params(opsec_ctx* ctx, double* params);
 /* Evaluate function f at separations r[0..n-1] on the device. */
 int opsec_model_xi(opsec_ctx* ctx, int f, int n, const double* r, double* xi);
-/* The 768-node QSpline table of function f (for table-mode functions). */
+/* The 768-node QSpline table of function f (for table-mode functions of a
+ * RealModel or an uploaded table model) and its node separations. */
 int opsec_model_table(opsec_ctx* ctx, int f, double* r768, double* xi768);
 
 /* ---- pair quadrature (K1) and assembly (K2) ----------------------------- */

@@ -95,7 +95,7 @@ struct opsec_ctx {
     std::string arch;
 
     // model
-    int model_kind = -1;     // 0 real, 1 test
+    int model_kind = -1;     // 0 real, 1 test, 2 uploaded tables
     int nparams = 0;
     std::vector<opsec::XiDesc> funcs;     // nparams + 1
     std::vector<double> params;           // prior p_n

@@ -648,6 +648,42 @@ int opsec_model_test(opsec_ctx* ctx, int form, double A, double b) {
     return OPSEC_OK;
 }
 
+int opsec_model_tables(opsec_ctx* ctx, int nfuncs, const double* xi768, const double* params) {
+    if (!ctx) return OPSEC_EINVAL;
+    OPSEC_CHECK_ARG(ctx, nfuncs >= 1 && xi768, "need at least one table");
+    for (size_t i = 0; i < (size_t)nfuncs * QS_N; i++)
+        OPSEC_CHECK_ARG(ctx, std::isfinite(xi768[i]), "table %d node %d is not finite", (int)(i / QS_N),
+                        (int)(i % QS_N));
+    OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    free_model(ctx);
+    ctx->model_kind = 2;
+    ctx->nparams = nfuncs - 1;
+    ctx->params.assign(params ? params : xi768, params ? params + (nfuncs - 1) : xi768);
+    if (!params) ctx->params.assign(nfuncs - 1, 0.0);
+    ctx->funcs.resize(nfuncs);
+    for (int f = 0; f < nfuncs; f++) ctx->funcs[f] = XiDesc{XI_TABLE, f, 0.0, 0.0};
+    ctx->ntables = nfuncs;
+    // the QSpline nodes of a RealModel table (QSpline::MakeArray, src/QSpline.cpp:6-17)
+    ctx->h_xgrid.resize(QS_N);
+    ctx->h_ugrid.resize(QS_N2);
+    for (int i = 0; i < QS_N1; i++) ctx->h_xgrid[i] = i * QS_X1 / QS_N1;
+    for (int j = 0; j < QS_N2; j++) ctx->h_xgrid[QS_N1 + j] = QS_X1 * std::exp(j * std::log(QS_X2 / QS_X1) / (QS_N2 - 1));
+    for (int j = 0; j < QS_N2; j++) ctx->h_ugrid[j] = std::log(ctx->h_xgrid[QS_N1 + j]);
+    size_t cap;
+    int rc;
+    cap = 0; if ((rc = dev_realloc(ctx, (void**)&ctx->d_tables, &cap, (size_t)nfuncs * QS_N * 8))) return rc;
+    cap = 0; if ((rc = dev_realloc(ctx, (void**)&ctx->d_xgrid, &cap, QS_N * 8))) return rc;
+    cap = 0; if ((rc = dev_realloc(ctx, (void**)&ctx->d_ugrid, &cap, QS_N2 * 8))) return rc;
+    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_tables, xi768, (size_t)nfuncs * QS_N * 8, hipMemcpyHostToDevice,
+                                      ctx->stream));
+    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_xgrid, ctx->h_xgrid.data(), QS_N * 8, hipMemcpyHostToDevice, ctx->stream));
+    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_ugrid, ctx->h_ugrid.data(), QS_N2 * 8, hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = upload_desc(ctx))) return rc;
+    ctx->Q_valid.assign(ctx->funcs.size(), 0);
+    OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return OPSEC_OK;
+}
+
 int opsec_model_nparams(opsec_ctx* ctx, int* nparams) {
     if (!ctx || !nparams) return OPSEC_EINVAL;
     if (ctx->model_kind < 0) return set_error(ctx, OPSEC_ESTATE, "no model loaded");
@@ -684,7 +720,7 @@ int opsec_model_xi(opsec_ctx* ctx, int f, int n, const double* r, double* xi) {
 
 int opsec_model_table(opsec_ctx* ctx, int f, double* r768, double* xi768) {
     if (!ctx) return OPSEC_EINVAL;
-    if (ctx->model_kind != 0) return set_error(ctx, OPSEC_ESTATE, "no RealModel loaded");
+    if (ctx->model_kind != 0 && ctx->model_kind != 2) return set_error(ctx, OPSEC_ESTATE, "no tabulated model loaded");
     OPSEC_CHECK_ARG(ctx, f >= 0 && f < (int)ctx->funcs.size(), "function index %d out of range", f);
     const XiDesc& d = ctx->funcs[f];
     if (d.kind != XI_TABLE) return set_error(ctx, OPSEC_EINVAL, "function %d is not tabulated", f);

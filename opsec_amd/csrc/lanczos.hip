@@ -19,11 +19,13 @@
 //   * convergence: |B_j z_last| <= tol |theta_max| for each wanted Ritz pair.
 // Output: eigenvalues descending, eigenvectors mode-major (modefile layout).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <numeric>
 #include <random>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -197,6 +199,19 @@ struct Dev {
             hipLaunchKernelGGL((lgemm_kernel<LA_ColMajor, LA_KCol, LPut<1>>), g, dim3(GT_THREADS), 0, s, A, B,
                                LPut<1>{Y, ldy}, N, q, p, nullptr);
     }
+    // Y (N x q) op= X (N x p) Xt^T, Xt: q x p column-major (ld ldt); op: 1 subtract
+    void gemm_nt_sub(double* Y, size_t ldy, const double* X, size_t ldx, const double* Xt, size_t ldt, int N, int p,
+                     int q) {
+        LA_ColMajor A;
+        A.p = X;
+        A.ld = ldx;
+        LA_ColMajor B;   // B[k][n] = Xt(n, k) = Xt[k ldt + n]
+        B.p = Xt;
+        B.ld = ldt;
+        dim3 g((N + GT_M - 1) / GT_M, (q + GT_N - 1) / GT_N, 1);
+        hipLaunchKernelGGL((lgemm_kernel<LA_ColMajor, LA_ColMajor, LPut<1>>), g, dim3(GT_THREADS), 0, s, A, B,
+                           LPut<1>{Y, ldy}, N, q, p, nullptr);
+    }
     // Y (N x q) = S X, S dense row-major N x N (symmetric)
     void apply(const double* S, int N, const double* X, int q, double* Y) {
         LA_RowMajor A;
@@ -301,44 +316,73 @@ __global__ __launch_bounds__(1024) void hh_w_kernel(const double* A, int K, int 
     for (int i = threadIdx.x; i < m; i += blockDim.x) p[i] = t * p[i] - kap * v[i];
 }
 
-// A22 -= v w^T + w v^T
-__global__ __launch_bounds__(256) void hh_rank2_kernel(double* A, int K, int k, const double* w) {
-    const int m = K - k - 1;
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    const int j = blockIdx.y;
-    if (i >= m || j >= m) return;
-    const double* v = A + (size_t)k * K + (k + 1);
-    double* a = A + (size_t)(k + 1 + j) * K + (k + 1) + i;
-    *a -= v[i] * w[j] + w[i] * v[j];
+
+// ---- blocked reduction (panels of columns, deferred two-sided updates) -------
+// Within a panel the rank-2 updates of its reflectors are kept as the columns
+// of V (reflectors) and W (update vectors), K x nb column-major, zero above
+// their first row; the trailing matrix is updated once per panel.
+
+// Column k (rows k .. K-1) gets the panel's deferred updates from its first i
+// reflectors: A(r, k) -= sum_j V(r, j) W(k, j) + W(r, j) V(k, j).
+__global__ __launch_bounds__(256) void hh_colupd_kernel(double* A, int K, int k, int i, const double* __restrict__ V,
+                                                        const double* __restrict__ W) {
+    const int r = k + blockIdx.x * 256 + threadIdx.x;
+    if (r >= K) return;
+    double acc = 0.0;
+    for (int j = 0; j < i; j++)
+        acc += V[(size_t)j * K + r] * W[(size_t)j * K + k] + W[(size_t)j * K + r] * V[(size_t)j * K + k];
+    A[(size_t)k * K + r] -= acc;
 }
 
-// Back-transformation: Z[k+1:, :] -= tau_k v_k (v_k^T Z[k+1:, :])
-__global__ __launch_bounds__(256) void hh_apply_dot_kernel(const double* A, int K, int k, const double* Z, int nz,
-                                                           double* r) {
+// ab[j] = W(:, j) . v and ab[i + j] = V(:, j) . v over rows k+1 .. K-1 (v in
+// column k of A), j < i; one workgroup per dot, fixed-order reduction.
+__global__ __launch_bounds__(256) void hh_dots_kernel(const double* A, int K, int k, int i, const double* __restrict__ V,
+                                                      const double* __restrict__ W, double* __restrict__ ab) {
     __shared__ double red[256];
-    const int m = K - k - 1;
-    const int c = blockIdx.x;
-    const double* v = A + (size_t)k * K + (k + 1);
-    const double* z = Z + (size_t)c * K + (k + 1);
-    double s = 0;
-    for (int i = threadIdx.x; i < m; i += 256) s += v[i] * z[i];
-    red[threadIdx.x] = s;
+    const int j = blockIdx.x;
+    const double* x = (j < i ? W + (size_t)j * K : V + (size_t)(j - i) * K);
+    const double* v = A + (size_t)k * K;
+    double acc = 0.0;
+    for (int r = k + 1 + threadIdx.x; r < K; r += 256) acc += x[r] * v[r];
+    red[threadIdx.x] = acc;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) r[c] = red[0];
+    if (threadIdx.x == 0) ab[j] = red[0];
 }
 
-__global__ __launch_bounds__(256) void hh_apply_upd_kernel(const double* A, int K, int k, const double* tau, double* Z,
-                                                           int nz, const double* r) {
-    const int m = K - k - 1;
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    const int c = blockIdx.y;
-    if (i >= m) return;
-    const double* v = A + (size_t)k * K + (k + 1);
-    Z[(size_t)c * K + (k + 1) + i] -= tau[k] * r[c] * v[i];
+// p -= V' (W'^T v) + W' (V'^T v): the deferred updates' share of A22 v, p
+// indexed from row k+1.
+__global__ __launch_bounds__(256) void hh_corr_kernel(double* p, int K, int k, int i, const double* __restrict__ V,
+                                                      const double* __restrict__ W, const double* __restrict__ ab) {
+    const int r = k + 1 + blockIdx.x * 256 + threadIdx.x;
+    if (r >= K) return;
+    double acc = 0.0;
+    for (int j = 0; j < i; j++) acc += V[(size_t)j * K + r] * ab[j] + W[(size_t)j * K + r] * ab[i + j];
+    p[r - k - 1] -= acc;
+}
+
+// Column i of the panel: V(:, i) = v_k (rows k+1 .. K-1, from A), W(:, i) = w
+// (from p); zero above.
+__global__ __launch_bounds__(256) void hh_store_kernel(const double* A, int K, int k, int i, const double* __restrict__ w,
+                                                       double* __restrict__ V, double* __restrict__ W) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= K) return;
+    const bool in = r >= k + 1;
+    V[(size_t)i * K + r] = in ? A[(size_t)k * K + r] : 0.0;
+    W[(size_t)i * K + r] = in ? w[r - k - 1] : 0.0;
+}
+
+// Panel of reflectors k0 .. k0 + nb - 1 as a dense K x nb column-major
+// matrix: column j is v_(k0+j), nonzero in rows k0 + j + 1 .. K - 1 (its first
+// entry, 1, is stored in A by hh_vector_kernel), zero above.
+__global__ void hh_panel_kernel(const double* __restrict__ A, int K, int k0, int nb, double* __restrict__ Vp) {
+    const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+    if (i >= K || j >= nb) return;
+    const int k = k0 + j;
+    Vp[(size_t)j * K + i] = i >= k + 1 ? A[(size_t)k * K + i] : 0.0;
 }
 
 __global__ void diag_kernel(const double* A, int K, double* d, double* elast) {
@@ -349,90 +393,152 @@ __global__ void diag_kernel(const double* A, int K, double* d, double* elast) {
 
 // ---- host: symmetric tridiagonal eigenpairs --------------------------------------
 
-// Eigenvalues of the unreduced tridiagonal (d, e) by implicit QL with
-// Wilkinson-type shifts (no vectors).  e[i] couples i and i+1.
-static void tql_values(int n, std::vector<double>& d, std::vector<double> e) {
-    if (n <= 1) return;
-    e.push_back(0.0);
-    for (int l = 0; l < n; l++) {
-        int iter = 0, m;
-        do {
-            for (m = l; m < n - 1; m++) {
-                double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
-                if (std::fabs(e[m]) <= 1e-300 + 2.2e-16 * dd) break;
-            }
-            if (m != l) {
-                if (iter++ == 200) break;
-                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-                double r = std::hypot(g, 1.0);
-                g = d[m] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
-                double s = 1.0, c = 1.0, p = 0.0;
-                int i;
-                for (i = m - 1; i >= l; i--) {
-                    double f = s * e[i], b = c * e[i];
-                    e[i + 1] = (r = std::hypot(f, g));
-                    if (r == 0.0) {
-                        d[i + 1] -= p;
-                        e[m] = 0.0;
-                        break;
-                    }
-                    s = f / r;
-                    c = g / r;
-                    g = d[i + 1] - p;
-                    r = (d[i] - g) * s + 2.0 * c * b;
-                    d[i + 1] = g + (p = s * r);
-                    g = c * r - b;
-                }
-                if (r == 0.0 && i >= l) continue;
-                d[l] -= p;
-                e[l] = g;
-                e[m] = 0.0;
-            }
-        } while (m != l);
-    }
-}
-
-// Solve (T - lam I) x = b for the symmetric tridiagonal T (n, d, e) by
-// Gaussian elimination with partial pivoting (row interchanges as in LAPACK's
-// dgttrf/dgttrs); zero pivots are replaced by `tiny`.
-static void tri_solve(int n, const double* d, const double* e, double lam, double tiny, double* x) {
-    if (n == 1) {
-        double p = d[0] - lam;
-        x[0] /= (p == 0.0 ? tiny : p);
+// f(i) for i in [0, n) on up to 16 host threads, items taken in chunks from a
+// shared counter (the results must not depend on which thread runs an item).
+template <typename F>
+static void parallel_for(int n, int chunk, F f) {
+    const int nt = (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+    if (n <= chunk || nt == 1) {
+        for (int i = 0; i < n; i++) f(i);
         return;
     }
-    std::vector<double> dl(e, e + n - 1), dd(n), du(e, e + n - 1), du2(std::max(0, n - 2), 0.0);
-    std::vector<char> swp(n, 0);
-    for (int i = 0; i < n; i++) dd[i] = d[i] - lam;
-    for (int i = 0; i < n - 1; i++) {
-        if (std::fabs(dd[i]) >= std::fabs(dl[i])) {
-            if (dd[i] == 0.0) dd[i] = tiny;
-            double fact = dl[i] / dd[i];
-            dl[i] = fact;
-            dd[i + 1] -= fact * du[i];
-        } else {
-            double fact = dd[i] / dl[i];
-            dd[i] = dl[i];
-            dl[i] = fact;
-            double temp = dd[i + 1];
-            dd[i + 1] = du[i] - fact * temp;
-            if (i < n - 2) {
-                du2[i] = du[i + 1];
-                du[i + 1] = -fact * du2[i];
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (;;) {
+            const int i0 = next.fetch_add(chunk);
+            if (i0 >= n) break;
+            for (int i = i0; i < std::min(n, i0 + chunk); i++) f(i);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < std::min(nt, (n + chunk - 1) / chunk); t++) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+}
+
+// Number of eigenvalues of the symmetric tridiagonal (d, e2 = e^2) below x,
+// from the signs of the pivots of the LDL^T factorisation of T - x I (Sturm
+// count); a zero pivot is replaced by -pivmin.
+static int sturm_below(int n, const double* d, const double* e2, double x, double pivmin) {
+    int cnt = 0;
+    double q = d[0] - x;
+    if (std::fabs(q) < pivmin) q = -pivmin;
+    cnt += q < 0;
+    for (int i = 1; i < n; i++) {
+        q = d[i] - x - e2[i - 1] / q;
+        if (std::fabs(q) < pivmin) q = -pivmin;
+        cnt += q < 0;
+    }
+    return cnt;
+}
+
+// Eigenvalues of indices j[0..NB) (ascending, 0-based) by bisection on
+// [lo, hi] (Gershgorin bounds) to full precision; the NB Sturm sequences of a
+// pass run interleaved (independent dependency chains).
+template <int NB>
+static void bisect_batch(int n, const double* d, const double* e2, const int* j, double lo0, double hi0,
+                         double pivmin, double* out) {
+    const double eps = 2.2204460492503131e-16;
+    double lo[NB], hi[NB];
+    for (int b = 0; b < NB; b++) lo[b] = lo0, hi[b] = hi0;
+    for (int it = 0; it < 200; it++) {
+        double x[NB], q[NB];
+        int cnt[NB];
+        bool any = false;
+        for (int b = 0; b < NB; b++) {
+            x[b] = 0.5 * (lo[b] + hi[b]);
+            const bool done = hi[b] - lo[b] <= 2 * eps * std::max(std::fabs(lo[b]), std::fabs(hi[b])) + pivmin ||
+                              x[b] <= lo[b] || x[b] >= hi[b];
+            any |= !done;
+            q[b] = d[0] - x[b];
+            if (std::fabs(q[b]) < pivmin) q[b] = -pivmin;
+            cnt[b] = q[b] < 0;
+        }
+        if (!any) break;
+        for (int i = 1; i < n; i++) {
+            const double di = d[i], ei = e2[i - 1];
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                double v = di - x[b] - ei / q[b];
+                if (std::fabs(v) < pivmin) v = -pivmin;
+                q[b] = v;
+                cnt[b] += v < 0;
             }
-            du[i] = temp;
-            swp[i] = 1;
+        }
+        for (int b = 0; b < NB; b++) {
+            if (hi[b] - lo[b] <= 2 * eps * std::max(std::fabs(lo[b]), std::fabs(hi[b])) + pivmin ||
+                x[b] <= lo[b] || x[b] >= hi[b])
+                continue;
+            if (cnt[b] > j[b]) hi[b] = x[b];
+            else lo[b] = x[b];
         }
     }
-    if (dd[n - 1] == 0.0) dd[n - 1] = tiny;
-    for (int i = 0; i < n - 1; i++) {
-        if (swp[i]) std::swap(x[i], x[i + 1]);
-        x[i + 1] -= dl[i] * x[i];
-    }
-    x[n - 1] /= dd[n - 1];
-    x[n - 2] = (x[n - 2] - du[n - 2] * x[n - 1]) / dd[n - 2];
-    for (int i = n - 3; i >= 0; i--) x[i] = (x[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) / dd[i];
+    for (int b = 0; b < NB; b++) out[b] = 0.5 * (lo[b] + hi[b]);
 }
+
+// The eigenvalue of index j (ascending, 0-based) of the unreduced tridiagonal
+// by bisection on [lo, hi] (Gershgorin bounds) to full precision.
+static double bisect_eigenvalue(int n, const double* d, const double* e2, int j, double lo, double hi,
+                                double pivmin) {
+    const double eps = 2.2204460492503131e-16;
+    for (int it = 0; it < 200; it++) {
+        const double mid = 0.5 * (lo + hi);
+        if (hi - lo <= 2 * eps * std::max(std::fabs(lo), std::fabs(hi)) + pivmin || mid <= lo || mid >= hi) break;
+        if (sturm_below(n, d, e2, mid, pivmin) > j) hi = mid;
+        else lo = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+// LU factors (partial pivoting, as LAPACK's dgttrf) of T - lam I for the
+// symmetric tridiagonal (d, e); zero pivots are replaced by `tiny`.
+struct TriLU {
+    std::vector<double> dl, dd, du, du2;
+    std::vector<char> swp;
+    void factor(int n, const double* d, const double* e, double lam, double tiny) {
+        dl.assign(e, e + std::max(0, n - 1));
+        du.assign(e, e + std::max(0, n - 1));
+        dd.resize(n);
+        du2.assign(std::max(0, n - 2), 0.0);
+        swp.assign(n, 0);
+        for (int i = 0; i < n; i++) dd[i] = d[i] - lam;
+        for (int i = 0; i < n - 1; i++) {
+            if (std::fabs(dd[i]) >= std::fabs(dl[i])) {
+                if (dd[i] == 0.0) dd[i] = tiny;
+                double fact = dl[i] / dd[i];
+                dl[i] = fact;
+                dd[i + 1] -= fact * du[i];
+            } else {
+                double fact = dd[i] / dl[i];
+                dd[i] = dl[i];
+                dl[i] = fact;
+                double temp = dd[i + 1];
+                dd[i + 1] = du[i] - fact * temp;
+                if (i < n - 2) {
+                    du2[i] = du[i + 1];
+                    du[i + 1] = -fact * du2[i];
+                }
+                du[i] = temp;
+                swp[i] = 1;
+            }
+        }
+        if (dd[n - 1] == 0.0) dd[n - 1] = tiny;
+    }
+    // x <- (T - lam I)^-1 x
+    void solve(int n, double* x) const {
+        if (n == 1) {
+            x[0] /= dd[0];
+            return;
+        }
+        for (int i = 0; i < n - 1; i++) {
+            if (swp[i]) std::swap(x[i], x[i + 1]);
+            x[i + 1] -= dl[i] * x[i];
+        }
+        x[n - 1] /= dd[n - 1];
+        x[n - 2] = (x[n - 2] - du[n - 2] * x[n - 1]) / dd[n - 2];
+        for (int i = n - 3; i >= 0; i--) x[i] = (x[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) / dd[i];
+    }
+};
 
 struct TriBlock {
     int off, n;
@@ -440,78 +546,141 @@ struct TriBlock {
 
 // Top-nev eigenpairs of the symmetric tridiagonal (d, e), vectors as columns of
 // Z (n x nev, column-major), eigenvalues descending.
+//   * unreduced blocks split at negligible off-diagonals;
+//   * eigenvalues by bisection (Sturm counts), only the wanted ones: the
+//     nev-th largest of the whole matrix first, then every eigenvalue of each
+//     block above it, in parallel over eigenvalues;
+//   * eigenvectors by inverse iteration (LAPACK dstein's scheme): clusters of
+//     eigenvalues closer than 1e-7 |T| are orthogonalised against each other,
+//     coincident shifts separated by 10 eps |T|; the clusters run in parallel,
+//     each from its own seed, so the result does not depend on the threads.
 static void tridiag_top(int n, const std::vector<double>& d, const std::vector<double>& e, int nev,
                         std::vector<double>& evals, std::vector<double>& Z) {
-    double tnorm = 0;
-    for (int i = 0; i < n; i++)
-        tnorm = std::max(tnorm, std::fabs(d[i]) + (i > 0 ? std::fabs(e[i - 1]) : 0) + (i < n - 1 ? std::fabs(e[i]) : 0));
+    const double eps = 2.2204460492503131e-16;
+    double tnorm = 0, glo = 0, ghi = 0;
+    for (int i = 0; i < n; i++) {
+        const double r = (i > 0 ? std::fabs(e[i - 1]) : 0) + (i < n - 1 ? std::fabs(e[i]) : 0);
+        tnorm = std::max(tnorm, std::fabs(d[i]) + r);
+        glo = i ? std::min(glo, d[i] - r) : d[i] - r;
+        ghi = i ? std::max(ghi, d[i] + r) : d[i] + r;
+    }
     if (tnorm == 0) tnorm = 1;
+    glo -= 2 * eps * tnorm + 1e-300;
+    ghi += 2 * eps * tnorm + 1e-300;
+    std::vector<double> e2(std::max(1, n - 1), 0.0);
+    double emax2 = 0;
+    for (int i = 0; i + 1 < n; i++) {
+        e2[i] = e[i] * e[i];
+        emax2 = std::max(emax2, e2[i]);
+    }
+    const double pivmin = std::max(1e-300, 1e-300 * emax2);
     // split into unreduced blocks
     std::vector<TriBlock> blocks;
     int start = 0;
     for (int i = 0; i < n - 1; i++)
-        if (std::fabs(e[i]) <= 2.2e-16 * (std::fabs(d[i]) + std::fabs(d[i + 1])) + 1e-300) {
+        if (std::fabs(e[i]) <= eps * (std::fabs(d[i]) + std::fabs(d[i + 1])) + 1e-300) {
             blocks.push_back({start, i + 1 - start});
             start = i + 1;
         }
     blocks.push_back({start, n - start});
+    nev = std::min(nev, n);
+    // the nev-th largest eigenvalue of the whole matrix (split off-diagonals as zero)
+    std::vector<double> dsplit(d.begin(), d.end()), e2split(e2);
+    for (const TriBlock& B : blocks)
+        if (B.off > 0) e2split[B.off - 1] = 0.0;
+    const double tau = bisect_eigenvalue(n, dsplit.data(), e2split.data(), n - nev, glo, ghi, pivmin);
+    // every eigenvalue of each block at or above tau (with a margin for ties)
     struct EV {
         double v;
-        int blk;
+        int blk, idx;
     };
-    std::vector<EV> all;
-    all.reserve(n);
+    std::vector<EV> cand;
+    const double tcut = tau - 4 * eps * tnorm;
     for (size_t bi = 0; bi < blocks.size(); bi++) {
         const TriBlock& B = blocks[bi];
-        std::vector<double> dd(d.begin() + B.off, d.begin() + B.off + B.n);
-        std::vector<double> ee(e.begin() + B.off, e.begin() + B.off + std::max(0, B.n - 1));
-        tql_values(B.n, dd, ee);
-        for (double v : dd) all.push_back({v, (int)bi});
+        const int below = sturm_below(B.n, &d[B.off], &e2[B.off], tcut, pivmin);
+        for (int j = below; j < B.n; j++) cand.push_back({0.0, (int)bi, j});
     }
-    std::sort(all.begin(), all.end(), [](const EV& a, const EV& b) { return a.v > b.v; });
-    nev = std::min(nev, n);
+    // batches of 8 candidates of one block (each bisection converges alone)
+    constexpr int NB = 8;
+    std::vector<int> bstart;
+    for (size_t c = 0; c < cand.size();) {
+        bstart.push_back((int)c);
+        size_t e0 = c;
+        while (e0 < cand.size() && e0 - c < (size_t)NB && cand[e0].blk == cand[c].blk) e0++;
+        c = e0;
+    }
+    parallel_for((int)bstart.size(), 1, [&](int bi) {
+        const size_t c0 = bstart[bi];
+        const TriBlock& B = blocks[cand[c0].blk];
+        int js[NB];
+        double out[NB];
+        int nb = 0;
+        while (nb < NB && c0 + nb < cand.size() && cand[c0 + nb].blk == cand[c0].blk) {
+            js[nb] = cand[c0 + nb].idx;
+            nb++;
+        }
+        for (int t = nb; t < NB; t++) js[t] = js[nb - 1];
+        bisect_batch<NB>(B.n, &d[B.off], &e2[B.off], js, glo, ghi, pivmin, out);
+        for (int t = 0; t < nb; t++) cand[c0 + t].v = out[t];
+    });
+    std::sort(cand.begin(), cand.end(), [](const EV& a, const EV& b) { return a.v > b.v; });
+    cand.resize(std::min((size_t)nev, cand.size()));
+    nev = (int)cand.size();
     evals.resize(nev);
     Z.assign((size_t)n * nev, 0.0);
-    // group wanted eigenvalues by block, ascending order inside each block
+    // clusters: per block, the wanted eigenvalues in descending order, split at gaps > ortol
+    const double ortol = 1e-7 * tnorm;
     std::vector<std::vector<int>> per_block(blocks.size());
     for (int t = 0; t < nev; t++) {
-        evals[t] = all[t].v;
-        per_block[all[t].blk].push_back(t);
+        evals[t] = cand[t].v;
+        per_block[cand[t].blk].push_back(t);
     }
-    const double ortol = 1e-7 * tnorm, eps = 2.2e-16;
-#pragma omp parallel for schedule(dynamic)
-    for (int bi = 0; bi < (int)blocks.size(); bi++) {
-        const TriBlock& B = blocks[bi];
+    struct Cluster {
+        int blk, first, count;
+    };
+    std::vector<Cluster> clusters;
+    for (size_t bi = 0; bi < blocks.size(); bi++) {
         std::vector<int>& idx = per_block[bi];
-        if (idx.empty()) continue;
         std::sort(idx.begin(), idx.end(), [&](int a, int b) { return evals[a] > evals[b]; });
+        for (size_t t = 0; t < idx.size(); t++)
+            if (t == 0 || std::fabs(evals[idx[t - 1]] - evals[idx[t]]) > ortol)
+                clusters.push_back({(int)bi, (int)t, 1});
+            else
+                clusters.back().count++;
+    }
+    parallel_for((int)clusters.size(), 1, [&](int ci) {
+        const Cluster& cl = clusters[ci];
+        const TriBlock& B = blocks[cl.blk];
+        const std::vector<int>& idx = per_block[cl.blk];
         const double* db = &d[B.off];
         const double* eb = B.n > 1 ? &e[B.off] : nullptr;
         std::vector<double> x(B.n);
-        std::mt19937_64 rng(1234567 + bi);
+        TriLU lu;
+        std::mt19937_64 rng(1234567 + 1000003ULL * cl.blk + (uint64_t)cl.first);
         std::normal_distribution<double> nd;
         double prev = 0;
-        int cstart = 0;
-        for (size_t t = 0; t < idx.size(); t++) {
-            int col = idx[t];
+        for (int t = cl.first; t < cl.first + cl.count; t++) {
+            const int col = idx[t];
             double lam = evals[col];
-            if (t > 0 && std::fabs(prev - lam) > ortol) cstart = (int)t;
             // separate coincident shifts slightly (as LAPACK's dstein does)
-            if (t > 0 && std::fabs(prev - lam) < 10 * eps * tnorm) lam = prev - 10 * eps * tnorm;
+            if (t > cl.first && std::fabs(prev - lam) < 10 * eps * tnorm) lam = prev - 10 * eps * tnorm;
             prev = lam;
+            double* z = &Z[(size_t)col * n + B.off];
             if (B.n == 1) {
-                Z[(size_t)col * n + B.off] = 1.0;
+                z[0] = 1.0;
                 continue;
             }
+            lu.factor(B.n, db, eb, lam, eps * tnorm);
             for (int i = 0; i < B.n; i++) x[i] = nd(rng);
             for (int it = 0; it < 5; it++) {
-                tri_solve(B.n, db, eb, lam, eps * tnorm, x.data());
-                // orthogonalise against earlier vectors of the cluster
-                for (int s2 = cstart; s2 < (int)t; s2++) {
-                    const double* z = &Z[(size_t)idx[s2] * n + B.off];
+                lu.solve(B.n, x.data());
+                // orthogonalise against the earlier vectors of the cluster
+                for (int s2 = cl.first; s2 < t; s2++) {
+                    const double* zz = &Z[(size_t)idx[s2] * n + B.off];
                     double dot = 0;
-                    for (int i = 0; i < B.n; i++) dot += z[i] * x[i];
-                    for (int i = 0; i < B.n; i++) x[i] -= dot * z[i];
+                    for (int i = 0; i < B.n; i++) dot += zz[i] * x[i];
+                    for (int i = 0; i < B.n; i++) x[i] -= dot * zz[i];
                 }
                 double nrm = 0;
                 for (int i = 0; i < B.n; i++) nrm += x[i] * x[i];
@@ -522,10 +691,9 @@ static void tridiag_top(int n, const std::vector<double>& d, const std::vector<d
                 }
                 for (int i = 0; i < B.n; i++) x[i] /= nrm;
             }
-            double* z = &Z[(size_t)col * n + B.off];
             for (int i = 0; i < B.n; i++) z[i] = x[i];
         }
-    }
+    });
 }
 
 // Top-nev eigenpairs of the dense symmetric K x K device matrix dA (destroyed).
@@ -542,14 +710,41 @@ static int dense_eigen_top(Dev& D, double* dA, int K, int nev, std::vector<doubl
     OPSEC_HIP_TRY(ctx, tmp.alloc(&dp, (size_t)std::max(K, nev) * 8 + 8));
     OPSEC_HIP_TRY(ctx, tmp.alloc(&dd, (size_t)K * 8 + 8));
     PhaseTimer pt;
-    for (int k = 0; k + 2 < K; k++) {
-        const int m = K - k - 1;
-        hipLaunchKernelGGL(hh_vector_kernel, dim3(1), dim3(1024), 0, D.s, dA, K, k, dtau, de);
-        const int nc = (m + 255) / 256;
-        hipLaunchKernelGGL(hh_symv_kernel, dim3(nc, nc), dim3(256), 0, D.s, dA, K, k, dpart);
-        hipLaunchKernelGGL(hh_symv_sum_kernel, dim3(nc), dim3(256), 0, D.s, dpart, nc, m, dp);
-        hipLaunchKernelGGL(hh_w_kernel, dim3(1), dim3(1024), 0, D.s, dA, K, k, dtau, dp);
-        hipLaunchKernelGGL(hh_rank2_kernel, dim3((m + 255) / 256, m), dim3(256), 0, D.s, dA, K, k, dp);
+    // Householder tridiagonalisation in panels of TR_NB columns: per column
+    // the deferred updates of the panel's earlier reflectors are applied to
+    // the column, its reflector formed, and w = tau p - (tau^2/2)(p.v) v from
+    // p = A22 v corrected for the deferred updates; after the panel, the
+    // trailing matrix A22 -= V W^T + W V^T as two MFMA GEMMs.  The symmetric
+    // matrix-vector product is the only per-column pass over A22.
+    constexpr int TR_NB = 32;
+    double *dVr = nullptr, *dWr = nullptr, *dab = nullptr;
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dVr, (size_t)K * TR_NB * 8));
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dWr, (size_t)K * TR_NB * 8));
+    OPSEC_HIP_TRY(ctx, tmp.alloc(&dab, (size_t)2 * TR_NB * 8));
+    for (int k0 = 0; k0 + 2 < K; k0 += TR_NB) {
+        const int nbp = std::min(TR_NB, K - 2 - k0);
+        for (int i = 0; i < nbp; i++) {
+            const int k = k0 + i, m = K - k - 1;
+            if (i > 0)
+                hipLaunchKernelGGL(hh_colupd_kernel, dim3((K - k + 255) / 256), dim3(256), 0, D.s, dA, K, k, i, dVr,
+                                   dWr);
+            hipLaunchKernelGGL(hh_vector_kernel, dim3(1), dim3(1024), 0, D.s, dA, K, k, dtau, de);
+            const int nc = (m + 255) / 256;
+            hipLaunchKernelGGL(hh_symv_kernel, dim3(nc, nc), dim3(256), 0, D.s, dA, K, k, dpart);
+            hipLaunchKernelGGL(hh_symv_sum_kernel, dim3(nc), dim3(256), 0, D.s, dpart, nc, m, dp);
+            if (i > 0) {
+                hipLaunchKernelGGL(hh_dots_kernel, dim3(2 * i), dim3(256), 0, D.s, dA, K, k, i, dVr, dWr, dab);
+                hipLaunchKernelGGL(hh_corr_kernel, dim3(nc), dim3(256), 0, D.s, dp, K, k, i, dVr, dWr, dab);
+            }
+            hipLaunchKernelGGL(hh_w_kernel, dim3(1), dim3(1024), 0, D.s, dA, K, k, dtau, dp);
+            hipLaunchKernelGGL(hh_store_kernel, dim3((K + 255) / 256), dim3(256), 0, D.s, dA, K, k, i, dp, dVr, dWr);
+        }
+        const int r0 = k0 + nbp, m2 = K - r0;
+        if (m2 > 0) {
+            double* A22 = dA + (size_t)r0 * K + r0;
+            D.gemm_nt_sub(A22, K, dVr + r0, K, dWr + r0, K, m2, nbp, m2);
+            D.gemm_nt_sub(A22, K, dWr + r0, K, dVr + r0, K, m2, nbp, m2);
+        }
     }
     hipLaunchKernelGGL(diag_kernel, dim3((K + 255) / 256), dim3(256), 0, D.s, dA, K, dd, de + (K >= 2 ? K - 2 : 0));
     int rc = D.sync();
@@ -561,11 +756,43 @@ static int dense_eigen_top(Dev& D, double* dA, int K, int nev, std::vector<doubl
     tridiag_top(K, d, e, nev, evals, Z);
     OPSEC_HIP_TRY(ctx, stream_copy(ctx, dZ, Z.data(), (size_t)K * nev * 8, hipMemcpyHostToDevice));
     g_klt_times[KT_TRIEIG] += pt.lap();
-    for (int k = K - 3; k >= 0; k--) {
-        const int m = K - k - 1;
-        hipLaunchKernelGGL(hh_apply_dot_kernel, dim3(nev), dim3(256), 0, D.s, dA, K, k, dZ, nev, dp);
-        hipLaunchKernelGGL(hh_apply_upd_kernel, dim3((m + 255) / 256, nev), dim3(256), 0, D.s, dA, K, k, dtau, dZ,
-                           nev, dp);
+    // Back-transformation Z <- H_0 H_1 ... H_(K-3) Z in panels of HH_NB
+    // reflectors, last panel first: H_k0 ... H_(k0+nb-1) = I - V T V^T
+    // (compact WY, T upper triangular: T_jj = tau_j, T[0:j, j] = -tau_j
+    // T[0:j, 0:j] V[:, 0:j]^T v_j), applied as three GEMMs on rows k0+1..K-1.
+    const int nrefl = std::max(0, K - 2);
+    if (nrefl > 0) {
+        constexpr int HH_NB = 64;
+        std::vector<double> tau(nrefl);
+        OPSEC_HIP_TRY(ctx, stream_copy(ctx, tau.data(), dtau, (size_t)nrefl * 8, hipMemcpyDeviceToHost));
+        double *dVp = nullptr, *dS = nullptr, *dT = nullptr, *dW = nullptr, *dW2 = nullptr;
+        OPSEC_HIP_TRY(ctx, tmp.alloc(&dVp, (size_t)K * HH_NB * 8));
+        OPSEC_HIP_TRY(ctx, tmp.alloc(&dS, (size_t)HH_NB * HH_NB * 8));
+        OPSEC_HIP_TRY(ctx, tmp.alloc(&dT, (size_t)HH_NB * HH_NB * 8));
+        OPSEC_HIP_TRY(ctx, tmp.alloc(&dW, (size_t)HH_NB * nev * 8));
+        OPSEC_HIP_TRY(ctx, tmp.alloc(&dW2, (size_t)HH_NB * nev * 8));
+        std::vector<double> S((size_t)HH_NB * HH_NB), T((size_t)HH_NB * HH_NB);
+        for (int k0 = ((nrefl - 1) / HH_NB) * HH_NB; k0 >= 0; k0 -= HH_NB) {
+            const int nb = std::min(HH_NB, nrefl - k0), mp = K - k0 - 1;
+            hipLaunchKernelGGL(hh_panel_kernel, dim3((K + 255) / 256, nb), dim3(256), 0, D.s, dA, K, k0, nb, dVp);
+            D.gemm_tn(dVp + k0 + 1, K, dVp + k0 + 1, K, mp, nb, nb, dS);   // S = V^T V
+            if ((rc = D.sync())) return rc;
+            OPSEC_HIP_TRY(ctx, stream_copy(ctx, S.data(), dS, (size_t)nb * nb * 8, hipMemcpyDeviceToHost));
+            std::fill(T.begin(), T.end(), 0.0);
+            for (int j = 0; j < nb; j++) {
+                const double tj = tau[k0 + j];
+                T[(size_t)j * nb + j] = tj;
+                for (int i = 0; i < j; i++) {
+                    double acc = 0.0;   // (T[0:j, 0:j] V[:, 0:j]^T v_j)_i, T upper triangular
+                    for (int l = i; l < j; l++) acc += T[(size_t)l * nb + i] * S[(size_t)j * nb + l];
+                    T[(size_t)j * nb + i] = -tj * acc;
+                }
+            }
+            OPSEC_HIP_TRY(ctx, stream_copy(ctx, dT, T.data(), (size_t)nb * nb * 8, hipMemcpyHostToDevice));
+            D.gemm_tn(dVp + k0 + 1, K, dZ + k0 + 1, K, mp, nb, nev, dW);          // W = V^T Z
+            D.gemm_nn(0, dW2, nb, dT, nb, dW, nb, nb, nb, nev);                  // W2 = T W
+            D.gemm_nn(1, dZ + k0 + 1, K, dVp + k0 + 1, K, dW2, nb, mp, nb, nev);  // Z -= V W2
+        }
     }
     int rc2 = D.sync();
     g_klt_times[KT_BACK] += pt.lap();

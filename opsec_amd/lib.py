@@ -93,6 +93,7 @@ _SIGS = {
     "opsec_device_info": ([_P, C.POINTER(_I), C.c_char_p, _I], _I),
     "opsec_model_real": ([_P, _P, _P, _I, _I, _P, _P, _I], _I),
     "opsec_model_test": ([_P, _I, _D, _D], _I),
+    "opsec_model_tables": ([_P, _I, _P, _P], _I),
     "opsec_model_nparams": ([_P, C.POINTER(_I)], _I),
     "opsec_model_params": ([_P, _P], _I),
     "opsec_model_xi": ([_P, _I, _I, _P, _P], _I),
@@ -239,6 +240,13 @@ class Context:
         lo, hi = f64(band_lo), f64(band_hi)
         self.check(self.lib.opsec_model_real(self.h, _ptr(k), _ptr(P), len(k), len(lo), _ptr(lo), _ptr(hi),
                                              xi_mode), "model_real")
+
+    def model_tables(self, xi768, params=None):
+        """Upload a model given by its QSpline node values (nfuncs x 768)."""
+        t = np.ascontiguousarray(xi768, dtype=np.float64)
+        assert t.ndim == 2 and t.shape[1] == 768
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        self.check(self.lib.opsec_model_tables(self.h, t.shape[0], _ptr(t), _ptr(p)), "model_tables")
 
     def model_test(self, form, A, b):
         self.check(self.lib.opsec_model_test(self.h, form, A, b), "model_test")

@@ -187,6 +187,39 @@ REF_API int ref_read_cells(const char* path, void* out, int maxcells) {
 
 REF_API int ref_sizeof_cell(void) { return (int)sizeof(Cell); }
 
+/* Read a modefile with the reference's ReadModes (src/sig.cpp:646-697):
+ * returns Nmodes, or -1 when ReadModesHeader rejects the file; *ncells gets
+ * Ncells; up to maxvals doubles are copied to out. */
+REF_API int ref_read_modes(const char* path, int* ncells, double* out, long maxvals) {
+    int Nmodes = -1, Ncells = -1;
+    real* m = ReadModes(path, &Nmodes, &Ncells);
+    if (ncells) *ncells = Ncells;
+    if (!m) return -1;
+    const long n = (long)Nmodes * Ncells;
+    if (out) memcpy(out, m, sizeof(double) * (size_t)(n < maxvals ? n : maxvals));
+    free(m);
+    return Nmodes;
+}
+
+/* The first data block of an .abn file read by the reference's abn_read
+ * (src/abn.c): returns the element count n, or -1 on a read error or an
+ * element size other than 8; up to maxvals doubles are copied to out. */
+REF_API long ref_read_abn_doubles(const char* path, double* out, long maxvals) {
+    FILE* fp = fopen(path, "rb");
+    if (!fp) return -1;
+    void* data = NULL;
+    size_t n = 0, size = 0;
+    int err = abn_read(fp, &data, &n, &size, NULL, NULL, NULL);
+    fclose(fp);
+    if (err || size != sizeof(double)) {
+        free(data);
+        return -1;
+    }
+    if (out) memcpy(out, data, sizeof(double) * (n < (size_t)maxvals ? n : (size_t)maxvals));
+    free(data);
+    return (long)n;
+}
+
 /* Survey::GetGridIndex of the box survey (src/Survey.cpp:93-101) at n points
  * given as float triples (galfile coordinates). */
 REF_API void ref_grid_index(const char* survey_cfgtext, int n, const float* xyz, int* out) {

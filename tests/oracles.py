@@ -82,6 +82,8 @@ _REF = {
     "ref_sig_block": ([_P, _I, C.c_char_p, _P, _I, _P, _I, _P, _I, _P, _I, _D, _D], _I),
     "ref_read_cells": ([C.c_char_p, _P, _I], _I),
     "ref_sizeof_cell": ([], _I),
+    "ref_read_modes": ([C.c_char_p, C.POINTER(_I), _P, C.c_long], _I),
+    "ref_read_abn_doubles": ([C.c_char_p, _P, C.c_long], C.c_long),
     "ref_grid_index": ([C.c_char_p, _I, _P, _P], None),
     "ref_read_galaxies": ([C.c_char_p, _P, _I], _I),
     "ref_cell_integral": ([_I, _D, _I, _P, _P, _P, _P, _D, _D, C.POINTER(C.c_int64)], _D),

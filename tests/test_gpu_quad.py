@@ -73,6 +73,35 @@ def test_sig_block_gm_vs_reference(c1):
         np.testing.assert_allclose(s, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max(), err_msg=f"f={f}")
 
 
+def test_uploaded_xifunc_tables_bind_sig_block(c1):
+    """A caller's XiFunc given as its 768 QSpline node values
+    (opsec_model_tables): the RealModel's own tables uploaded that way give the
+    reference's xi and ComputeSignalMatrixBlock blocks, and the same bits as
+    the RealModel path."""
+    tabs = np.array([c1.table(f)[1] for f in range(21)])
+    params = c1.params()
+    cells = synth.box_cells(12)
+    sv = lib.survey_box(12)
+    q = c1.quad(lib.OPSEC_QUAD_GM)
+    rows, cols = GOLD["blk_rows"], np.arange(1728)
+    native = [c1.sig_block(cells, rows, cols, int(f), sv, q) for f in GOLD["blk_funcs"]]
+    c1.model_tables(tabs, params)
+    assert c1.nparams() == 20
+    np.testing.assert_array_equal(c1.params(), params)
+    for f in range(21):
+        ref = GOLD["c1_xi"][f]
+        np.testing.assert_allclose(c1.xi(f, GOLD["xi_r"]), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    for t, f in enumerate(GOLD["blk_funcs"]):
+        s = c1.sig_block(cells, rows, cols, int(f), sv, q)
+        np.testing.assert_array_equal(s, native[t])
+        ref = GOLD["blk_S"][t]
+        np.testing.assert_allclose(s, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max(), err_msg=f"f={f}")
+    with pytest.raises(lib.OpsecError, match="not finite"):
+        bad = tabs.copy()
+        bad[3, 7] = np.nan
+        c1.model_tables(bad)
+
+
 def test_sig_block_stride_and_subset(c1):
     cells = synth.box_cells(6, 0.0, 120.0)
     sv = lib.survey_box(6, 0.0, 120.0, L=1e9)

@@ -76,6 +76,40 @@ def test_cellfile_roundtrip_with_reference_reader(tmp_path):
     assert n == 64 and out.tobytes() == cells.tobytes()
 
 
+def _klt_headers(path_modes, path_evals, cellfile, Ncells, Nmodes, nconv, evals, modes):
+    """The modefile and evalfile exactly as the klt program writes them
+    (opsec_amd/host/klt.cpp): comment lines, opts Ncells, Nmodes, cellfile,
+    nconv, all Nmodes vectors."""
+    opts = {"Ncells": Ncells, "Nmodes": Nmodes, "cellfile": cellfile, "nconv": nconv}
+    formats.write(path_evals, evals, "d", opts, comments=("Signal matrix eigenvalues",))
+    formats.write(path_modes, modes.ravel(), "d", opts,
+                  comments=("Signal matrix eigenmodes", "Each mode is stored consecutively, as below:",
+                            "  v_1,1   v_1,2   ... v_1,n", "  ...", "  v_m,1   v_m,2   ... v_m,n"))
+
+
+def test_modefile_read_by_reference_reader(tmp_path):
+    """The reference's ReadModes (src/sig.cpp:621-697) and abn_read read the
+    klt output layout; with fewer converged modes the reference's own header
+    (n = nconv Ncells, Nmodes from the config, src/klt.cpp:210-225) is
+    rejected by its own reader, which is why klt writes all Nmodes vectors."""
+    R = O.ref()
+    N, M = 50, 7
+    rng = np.random.default_rng(3)
+    modes, evals = rng.normal(size=(M, N)), np.sort(rng.uniform(0, 3, M))[::-1].copy()
+    mf, ef = tmp_path / "modes.abn", tmp_path / "evals.abn"
+    _klt_headers(mf, ef, "cells.abn", N, M, M - 2, evals, modes)
+    out = np.zeros(M * N)
+    nc = C.c_int()
+    assert R.ref_read_modes(str(mf).encode(), C.byref(nc), O.ptr(out), out.size) == M
+    assert nc.value == N and np.array_equal(out.reshape(M, N), modes)
+    ev = np.zeros(M)
+    assert R.ref_read_abn_doubles(str(ef).encode(), O.ptr(ev), M) == M and np.array_equal(ev, evals)
+    # the reference's inconsistent header when nconv < Nmodes
+    bad = tmp_path / "bad.abn"
+    formats.write(bad, modes[:M - 2].ravel(), "d", {"Ncells": N, "Nmodes": M, "nconv": M - 2})
+    assert R.ref_read_modes(str(bad).encode(), C.byref(nc), None, 0) == -1
+
+
 def _write_inputs(tmp_path, n=6, nb=3, lo=0.0, hi=150.0):
     cells = synth.box_cells(n, lo, hi)
     cf = tmp_path / "cells.abn"
@@ -154,6 +188,14 @@ def test_pipeline_end_to_end_vs_oracle(tmp_path):
     assert eopts["Nmodes"] == str(M) and eopts["Ncells"] == str(N) and int(eopts["nconv"]) == M
     _, (modes,) = formats.read_doubles(tmp_path / "modes.abn")
     B = modes.reshape(M, N)
+    # the reference's own readers accept klt's files (src/sig.cpp:621-697, src/abn.c)
+    got = np.zeros(M * N)
+    nc = C.c_int()
+    assert O.ref().ref_read_modes(str(tmp_path / "modes.abn").encode(), C.byref(nc), O.ptr(got), got.size) == M
+    assert nc.value == N and np.array_equal(got, modes)
+    gl = np.zeros(M)
+    assert O.ref().ref_read_abn_doubles(str(tmp_path / "evals.abn").encode(), O.ptr(gl), M) == M
+    assert np.array_equal(gl, lam)
     # oracle S and its spectrum
     lo, hi = synth.linear_bands(nb, 0.02, 0.14)
     r768, tabs = O.oracle_tables(kP[:, 0], kP[:, 1], lo, hi)

@@ -328,6 +328,22 @@ int opsec_estimate_run(opsec_ctx* ctx, int nmodes, int nparams, const double* la
 int opsec_estimate_device(opsec_ctx* ctx, const double* lambda, const double* y,
                           const double* prior, int mixing, int prec, opsec_estimate* out);
 
+/* The Gram step of opsec_estimate_run -- F (nparams x nparams row-major), f
+ * and q (nparams each) -- from DEVICE inputs (d_lambda, d_y: nmodes; d_D:
+ * nparams packed matrices in the covfile layout; d_prior: nparams) into
+ * DEVICE outputs, allocating nothing: no hipMalloc, hipFree or memset, only
+ * kernel launches on the context stream (asynchronous: opsec_ctx_sync waits).
+ * Same kernels, same workgroup partition and the same fixed-order sums as
+ * opsec_estimate_run, so the same bits.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops).  Its partial sums use a fixed scratch array of 16 MiB per
+ * device, held until the process ends and shared by every call on that
+ * device: calls that overlap on one device race.  1 <= nparams <= 62, any
+ * nmodes >= 1. */
+int opsec_fisher_gram_noalloc(opsec_ctx* ctx, int nmodes, int nparams, const double* d_lambda,
+                              const double* d_D, const double* d_y, const double* d_prior, int prec,
+                              double* d_F, double* d_f, double* d_q);
+
 /* ---- learned pair-integral surrogate (K6) ------------------------------- */
 typedef struct opsec_mlp_train_opts {
     int32_t nsamples;         /* exact training integrals (2e5)               */

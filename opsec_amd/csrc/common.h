@@ -209,6 +209,9 @@ void free_proj(opsec_ctx* ctx);
 void free_mlp(opsec_ctx* ctx);
 void free_dot(opsec_ctx* ctx);
 void invalidate_modes(opsec_ctx* ctx);
+// Device address of the Gram scratch array on ctx's device (fisher.hip), once
+// per device, with that device current.
+int fisher_scratch_setup(opsec_ctx* ctx);
 
 // Rows of the packed upper triangle: column j starts at j(j+1)/2.
 inline int64_t packed_size(int64_t m) { return m * (m + 1) / 2; }

@@ -361,6 +361,12 @@ int opsec_ctx_create(int device, opsec_ctx** out) {
         delete ctx;
         return OPSEC_EHIP;
     }
+    // device address of the allocation-free Gram entry point's scratch
+    if (fisher_scratch_setup(ctx) != OPSEC_OK) {
+        hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return OPSEC_EHIP;
+    }
     *out = ctx;
     return OPSEC_OK;
 }

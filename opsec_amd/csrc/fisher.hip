@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -161,6 +162,40 @@ __global__ __launch_bounds__(256) void fisher_gram_kernel(const double* D, long 
     for (int e = threadIdx.x; e < FG * FG; e += blockDim.x) out[e] = red[e];
 }
 
+// Scratch of the allocation-free Gram entry point: FISHER_MAX_PARTS partial
+// Gram matrices of FG_MAX x FG_MAX doubles, then the summed FG_MAX x FG_MAX
+// matrix -- 16.03 MiB per device, held until the process ends.  Host code
+// never uses this name as an address: it uses the device address that
+// hipGetSymbolAddress returns for the current device (fisher_scratch_setup).
+constexpr size_t FISHER_SCRATCH_DOUBLES = (size_t)(FISHER_MAX_PARTS + 1) * FG_MAX * FG_MAX;
+static __device__ __align__(256) double scratch[FISHER_SCRATCH_DOUBLES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[SCRATCH_MAX_DEVICES];
+static bool g_scratch_done[SCRATCH_MAX_DEVICES];
+
+// Once per device, with that device current: the device address of `scratch`.
+int fisher_scratch_setup(opsec_ctx* ctx) {
+    if (ctx->device < 0 || ctx->device >= SCRATCH_MAX_DEVICES)
+        return set_error(ctx, OPSEC_EINVAL, "device %d beyond %d", ctx->device, SCRATCH_MAX_DEVICES);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_done[ctx->device]) return OPSEC_OK;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return set_error(ctx, OPSEC_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[ctx->device] = (double*)p;
+    g_scratch_done[ctx->device] = true;
+    return OPSEC_OK;
+}
+
+static double* fisher_scratch_ptr(int device) {
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_done[device] ? g_scratch_ptr[device] : nullptr;
+}
+
 // G[e] = sum over the partials in workgroup order (FP64, fixed order).
 __global__ void fisher_sum_parts_kernel(const double* __restrict__ Gpart, int nparts, int FG, double* __restrict__ G) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -168,6 +203,22 @@ __global__ void fisher_sum_parts_kernel(const double* __restrict__ Gpart, int np
     double acc = 0.0;
     for (int s = 0; s < nparts; s++) acc += Gpart[(size_t)s * FG * FG + e];
     G[e] = acc;
+}
+
+// The Gram outputs on the device: F (nb x nb row-major, symmetrised as
+// 0.5 (G_mn + G_nm), the same operation the host post-processing does),
+// f = G row nb, q = G row nb + 1.
+__global__ void fisher_outputs_kernel(const double* __restrict__ G, int FG, int nb, double* __restrict__ F,
+                                      double* __restrict__ f, double* __restrict__ q) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nb * nb) {
+        const int m = e / nb, n = e - m * nb;
+        F[e] = 0.5 * (G[m * FG + n] + G[n * FG + m]);
+    }
+    if (e < nb) {
+        f[e] = G[nb * FG + e];
+        q[e] = G[(nb + 1) * FG + e];
+    }
 }
 
 
@@ -400,6 +451,31 @@ int opsec_estimate_run(opsec_ctx* ctx, int nmodes, int nparams, const double* la
     int rc = run_fisher(ctx, dD, nmodes, nparams, lambda, y, prior, mixing, prec, out);
     hipStreamSynchronize(ctx->stream);
     return rc;
+}
+
+int opsec_fisher_gram_noalloc(opsec_ctx* ctx, int nmodes, int nparams, const double* d_lambda, const double* d_D,
+                              const double* d_y, const double* d_prior, int prec, double* d_F, double* d_f,
+                              double* d_q) {
+    if (!ctx) return OPSEC_EINVAL;
+    int rc = check_gram_args(ctx, nmodes, nparams, prec);
+    if (rc) return rc;
+    OPSEC_CHECK_ARG(ctx, d_lambda && d_D && d_y && d_prior && d_F && d_f && d_q, "NULL device buffer");
+    double* base = fisher_scratch_ptr(ctx->device);
+    if (!base) return set_error(ctx, OPSEC_ESTATE, "Gram scratch of device %d not set up", ctx->device);
+    const long long K = (long long)nmodes * (nmodes + 1) / 2;
+    int FG, KT;
+    gram_shape(nparams, prec, &FG, &KT);
+    const int nparts = fisher_parts(ctx, (K + KT - 1) / KT);
+    OPSEC_CHECK_ARG(ctx, nparts <= FISHER_MAX_PARTS && FG <= FG_MAX, "Gram scratch too small");
+    OPSEC_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    // every partial the sum reads is written by its workgroup, and G is
+    // written whole by the sum: nothing needs clearing
+    double* d_G = base + (size_t)FISHER_MAX_PARTS * FG_MAX * FG_MAX;
+    launch_gram(ctx, d_D, nmodes, nparams, d_lambda, d_y, d_prior, prec, nparts, base, d_G);
+    hipLaunchKernelGGL(fisher_outputs_kernel, dim3((nparams * nparams + 255) / 256), dim3(256), 0, ctx->stream, d_G,
+                       FG, nparams, d_F, d_f, d_q);
+    OPSEC_HIP_TRY(ctx, hipGetLastError());
+    return OPSEC_OK;
 }
 
 int opsec_estimate_device(opsec_ctx* ctx, const double* lambda, const double* y, const double* prior, int mixing,

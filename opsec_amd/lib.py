@@ -124,6 +124,7 @@ _SIGS = {
     "opsec_healpix_pixels": ([_P, C.c_int64, _I, C.c_int64, _P, _P, _P], _I),
     "opsec_estimate_run": ([_P, _I, _I, _P, _P, _P, _P, _I, _I, C.POINTER(Estimate)], _I),
     "opsec_estimate_device": ([_P, _P, _P, _P, _I, _I, C.POINTER(Estimate)], _I),
+    "opsec_fisher_gram_noalloc": ([_P, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P], _I),
     "opsec_mlp_defaults": ([C.POINTER(MlpTrainOpts)], _I),
     "opsec_mlp_train": ([_P, C.POINTER(MlpTrainOpts), C.POINTER(Quad), C.POINTER(_D)], _I),
     "opsec_mlp_nweights": ([_P, C.POINTER(_I)], _I),

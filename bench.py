@@ -336,10 +336,17 @@ def spherical_main(args):
     ctx.pair_tables([0], q)
     t_s = time.perf_counter() - t0
     t0 = time.perf_counter()
+    os.environ["OPSEC_KLT_PROGRESS"] = "1"    # one line per Rayleigh-Ritz step (long solves stay visible)
     # the shell's spectrum converges more slowly than the box's: a Krylov space of 6 M
     lam, B, nconv = ctx.klt(M, tol=1e-8, maxdim=min(N, 6 * M + 512))
     t_klt = time.perf_counter() - t0
-    log(f"setup: prior arc-pair table {t_s:.2f} s, klt {M} modes {t_klt:.1f} s (nconv {nconv})")
+    ph = (Cc.c_double * 9)()
+    ctx.lib.opsec_diag_klt_times(ph, 9)
+    klt_phases = {nm: float(ph[i]) for i, nm in enumerate(
+        ("block_matvecs", "orthogonalisation", "ritz_tridiagonal_reduction_gpu", "ritz_tridiagonal_eigen_host",
+         "ritz_back_transformation", "ritz_vectors"))}
+    log(f"setup: prior arc-pair table {t_s:.2f} s, klt {M} modes {t_klt:.1f} s (nconv {nconv}); phases "
+        f"{klt_phases}, Krylov dimension {int(ph[6])}, {int(ph[7])} Rayleigh-Ritz steps")
     ctx.modes_upload(B)
     prior = ctx.params()
     y = np.sqrt(1 + lam) * np.random.Generator(np.random.Philox(42)).normal(size=M)
@@ -365,7 +372,8 @@ def spherical_main(args):
                        "integrals_per_s": st64["integrals"] / (qm / 1e3),
                        "node_evaluations_per_s": st64["nodes"] / (qm / 1e3)},
         "basis": {"s": t_basis, "cells": N, "integrand_evaluations": int(nev), "evaluations_per_s": nev / t_basis},
-        "klt": {"s": t_klt, "nconv": int(nconv), "prior_table_s": t_s},
+        "klt": {"s": t_klt, "nconv": int(nconv), "prior_table_s": t_s, "tol": 1e-8, "phases_ms": klt_phases,
+                "krylov_dim": int(ph[6]), "rayleigh_ritz_steps": int(ph[7]), "block_steps": int(ph[8])},
     }
     out.update(compare_estimates(r64[1], rmx[1]))
     if not args.no_cpu_baseline:
@@ -414,6 +422,8 @@ def main():
     ap.add_argument("--nt-group", type=int, default=8, help="split-BF16x3 GEMM tile-group edge (stage 1)")
     ap.add_argument("--s2-group", type=int, default=2, help="split-BF16x3 stage-2 tile-group edge")
     ap.add_argument("--nt-remap", type=int, default=0, choices=[0, 1], help="deal tile groups to XCDs")
+    ap.add_argument("--g3-dl", type=int, default=1, choices=[0, 1],
+                    help="256 x 256 split-BF16x3 GEMM: operand tiles through VGPRs (0) or by DMA loads into LDS (1)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also time a plain device copy of this many GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -443,6 +453,7 @@ def main():
     ctx.lib.opsec_diag_set_nt_gen(args.nt_gen)
     ctx.lib.opsec_diag_set_stage2_split(args.s2_split)
     ctx.lib.opsec_diag_set_stage2_group(args.s2_group)
+    ctx.lib.opsec_diag_set_g3_dl(args.g3_dl)
     ctx.set_projection(L.OPSEC_PROJ_TOEPLITZ if args.projection == "toeplitz" else L.OPSEC_PROJ_DENSE)
     ncu, arch = ctx.device_info()
     log(f"device {arch} with {ncu} CUs; workload {args.config}: {N} cells, {nb} bands, {M} modes")

@@ -197,7 +197,9 @@ int opsec_d2h(opsec_ctx* ctx, void* dst, const void* src, size_t bytes);
  * while some of the nev modes have not converged; a retry's result replaces
  * the earlier one only if it converged more modes).  A Krylov space that
  * would span all but one block of the space is replaced by the dense
- * symmetric eigen-solver on S itself (same answer, all modes converged). */
+ * symmetric eigen-solver on S itself (same answer, all modes converged).
+ * With the environment variable OPSEC_KLT_PROGRESS set, each Rayleigh-Ritz
+ * step prints one progress line to stderr. */
 int opsec_klt(opsec_ctx* ctx, int nev, double tol, int block, int maxdim,
               double* evals, double* evecs, int* nconv);
 /* Same solver (and the same automatic Krylov dimension) on an arbitrary

@@ -8,20 +8,24 @@
 //     matrix (dense FP64, streamed from HBM by the MFMA GEMM);
 //   * full re-orthogonalisation against every previous block, twice (CGS2),
 //     as FP64 MFMA GEMMs;
-//   * block orthonormalisation by SVQB (eigen-decomposition of the b x b Gram
-//     matrix), which replaces numerically dependent directions by fresh random
-//     ones -- the periodic box's signal matrix has cubic-symmetry multiplets
-//     that single-vector Lanczos cannot resolve;
-//   * Rayleigh-Ritz on the block-tridiagonal T: GPU Householder
-//     tridiagonalisation (symmetric rank-2 updates), host implicit QL for the
-//     eigenvalues, host inverse iteration (with re-orthogonalisation inside
-//     clusters) for the wanted eigenvectors, GPU back-transformation;
+//   * block orthonormalisation by Cholesky QR, or, for ill-conditioned
+//     blocks, by SVQB (eigen-decomposition of the b x b Gram matrix), which
+//     replaces numerically dependent directions by fresh random ones -- the
+//     periodic box's signal matrix has cubic-symmetry multiplets that
+//     single-vector Lanczos cannot resolve;
+//   * Rayleigh-Ritz on the block-tridiagonal T: blocked GPU Householder
+//     tridiagonalisation (panels of deferred two-sided updates), host
+//     bisection for the wanted eigenvalues and inverse iteration (with
+//     re-orthogonalisation inside clusters) for their eigenvectors on host
+//     threads, compact-WY GPU back-transformation;
 //   * convergence: |B_j z_last| <= tol |theta_max| for each wanted Ritz pair.
 // Output: eigenvalues descending, eigenvectors mode-major (modefile layout).
 #include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <random>
@@ -841,6 +845,45 @@ static void host_sym_eigen(int n, std::vector<double> A, std::vector<double>& w,
     for (int i = 0; i < n; i++) w[i] = A[i * n + i];
 }
 
+// Cholesky QR of a block with Gram matrix G (b x b, symmetric): G = R^T R,
+// X R^-1 orthonormal.  Mh = R^-1 (column-major), Bnew = R (row-major).
+// Returns false -- and the caller uses the eigen-decomposition instead -- when
+// the block is too ill-conditioned for it (smallest pivot^2 below 1e-10 of
+// the largest diagonal entry, i.e. cond(X) above about 1e5).
+static bool host_cholesky_qr(int b, const std::vector<double>& G, std::vector<double>& Mh, std::vector<double>& Bnew) {
+    std::vector<double> R((size_t)b * b, 0.0);   // row-major upper triangle
+    double gmax = 0;
+    for (int i = 0; i < b; i++) gmax = std::max(gmax, G[(size_t)i * b + i]);
+    if (!(gmax > 0)) return false;
+    for (int j = 0; j < b; j++) {
+        double djj = G[(size_t)j * b + j];
+        for (int l = 0; l < j; l++) djj -= R[(size_t)l * b + j] * R[(size_t)l * b + j];
+        if (!(djj > 1e-10 * gmax)) return false;
+        const double rjj = std::sqrt(djj);
+        R[(size_t)j * b + j] = rjj;
+        for (int i = j + 1; i < b; i++) {
+            double v = G[(size_t)i * b + j];   // G symmetric: G(j, i)
+            for (int l = 0; l < j; l++) v -= R[(size_t)l * b + j] * R[(size_t)l * b + i];
+            R[(size_t)j * b + i] = v / rjj;
+        }
+    }
+    // R^-1 (upper triangular), column by column
+    std::vector<double> Ri((size_t)b * b, 0.0);   // row-major
+    for (int c = 0; c < b; c++) {
+        Ri[(size_t)c * b + c] = 1.0 / R[(size_t)c * b + c];
+        for (int r = c - 1; r >= 0; r--) {
+            double v = 0.0;
+            for (int l = r + 1; l <= c; l++) v += R[(size_t)r * b + l] * Ri[(size_t)l * b + c];
+            Ri[(size_t)r * b + c] = -v / R[(size_t)r * b + r];
+        }
+    }
+    Mh.assign((size_t)b * b, 0.0);
+    for (int i = 0; i < b; i++)
+        for (int k = 0; k < b; k++) Mh[(size_t)i * b + k] = Ri[(size_t)k * b + i];   // (k, i) column-major
+    Bnew = R;
+    return true;
+}
+
 struct LanczosWork {
     double* V = nullptr;      // N x (Kmax + b)
     double* W = nullptr;      // N x b
@@ -850,7 +893,7 @@ struct LanczosWork {
 };
 
 // Orthonormalise the N x b block X against V[:, 0:kprev] (twice) and itself
-// (SVQB, twice).  On return X = X_orth * Bmat (b x b, row-major host) for the
+// (Cholesky QR or SVQB, twice).  On return X = X_orth * Bmat (b x b, row-major host) for the
 // part of X inside the new directions; dependent directions are replaced by
 // random orthonormal ones with zero coefficient.
 static int orthonormalise_block(Dev& D, LanczosWork& L, int N, int b, int kprev, double* X, std::vector<double>& Bmat,
@@ -876,19 +919,26 @@ static int orthonormalise_block(Dev& D, LanczosWork& L, int N, int b, int kprev,
             OPSEC_HIP_TRY(ctx, stream_copy(ctx, G.data(), L.small, (size_t)b * b * 8, hipMemcpyDeviceToHost));
             for (int i = 0; i < b; i++)
                 for (int j = 0; j < i; j++) G[i * b + j] = G[j * b + i] = 0.5 * (G[i * b + j] + G[j * b + i]);
-            host_sym_eigen(b, G, w, U);   // G = U diag(w) U^T, U column i = eigvec i (U[k*b+i])
-            double wmax = *std::max_element(w.begin(), w.end());
-            const double thr = std::max(wmax, 1e-300) * 1e-20;
-            // X <- X U diag(1/sqrt(w)); coefficients Bnew = diag(sqrt(w)) U^T
-            std::vector<double> Mh((size_t)b * b, 0.0), Bnew((size_t)b * b, 0.0);
+            std::vector<double> Mh, Bnew;
             std::vector<int> bad;
-            for (int i = 0; i < b; i++) {
-                bool ok = w[i] > thr;
-                if (!ok) bad.push_back(i);
-                double sw = ok ? std::sqrt(w[i]) : 0.0;
-                for (int k = 0; k < b; k++) {
-                    Mh[(size_t)i * b + k] = ok ? U[k * b + i] / sw : 0.0;   // column-major b x b: (k, i)
-                    Bnew[i * b + k] = sw * U[k * b + i];
+            // well-conditioned blocks (the usual case): Cholesky QR; otherwise
+            // SVQB through the eigen-decomposition, which finds and replaces
+            // the dependent directions
+            if (!host_cholesky_qr(b, G, Mh, Bnew)) {
+                host_sym_eigen(b, G, w, U);   // G = U diag(w) U^T, U column i = eigvec i (U[k*b+i])
+                double wmax = *std::max_element(w.begin(), w.end());
+                const double thr = std::max(wmax, 1e-300) * 1e-20;
+                // X <- X U diag(1/sqrt(w)); coefficients Bnew = diag(sqrt(w)) U^T
+                Mh.assign((size_t)b * b, 0.0);
+                Bnew.assign((size_t)b * b, 0.0);
+                for (int i = 0; i < b; i++) {
+                    bool ok = w[i] > thr;
+                    if (!ok) bad.push_back(i);
+                    double sw = ok ? std::sqrt(w[i]) : 0.0;
+                    for (int k = 0; k < b; k++) {
+                        Mh[(size_t)i * b + k] = ok ? U[k * b + i] / sw : 0.0;   // column-major b x b: (k, i)
+                        Bnew[i * b + k] = sw * U[k * b + i];
+                    }
                 }
             }
             OPSEC_HIP_TRY(ctx, stream_copy(ctx, L.small, Mh.data(), (size_t)b * b * 8, hipMemcpyHostToDevice));
@@ -924,6 +974,7 @@ static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double
     Dev D(ctx, ctx->stream);
     OPSEC_HIP_TRY(ctx, D.init());
     for (double& t : g_klt_times) t = 0.0;
+    const auto t_solve = std::chrono::steady_clock::now();
     nev = std::min(nev, N);
     if (b <= 0) b = std::min(128, std::max(1, N / 4));
     b = std::max(1, std::min(b, N));
@@ -1052,6 +1103,10 @@ static int block_lanczos(opsec_ctx* ctx, const double* S, int N, int nev, double
             if (std::sqrt(s2) <= tol * tmax) converged++;
             else break;   // the leading run of converged pairs
         }
+        if (std::getenv("OPSEC_KLT_PROGRESS"))
+            std::fprintf(stderr, "klt: Krylov dimension %d of %d, %d of %d Ritz pairs converged (%.1f s)\n", K, Kmax,
+                         converged, nev, std::chrono::duration<double>(
+                                             std::chrono::steady_clock::now() - t_solve).count());
         if (converged >= nev || last) break;
         check_at = std::min(Kmax, ((K * 5 / 4 + b - 1) / b) * b);
     }

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "common.h"
 #include "gemm.h"
@@ -739,7 +740,13 @@ struct ASrcTable {
 // Batched form: workgroup id = (batch * nmb * nnb + tile) * ksplit + split;
 // batch b multiplies A at offset ((b + a_add) >> a_shift) * a_bstride by B at
 // offset b * b_bstride into the epilogue's batch b.
-template <typename ASrc, typename Epi>
+// DL (A from plane pairs only): the operand tiles go from global memory to
+// LDS by the DMA loads of gfx950 (global_load_lds_dwordx4), with no VGPR
+// staging and no LDS store instructions.  A wave-instruction fills 1 KB of LDS,
+// 16 B per lane in lane order: 16 tile rows of one plane; lane L takes row
+// 16 rb + L/4 and the chunk that the swizzle puts in its slot L % 4.  The 64
+// instructions of a step are spread over the 8 waves.
+template <typename ASrc, typename Epi, bool DL = false>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void gemm3_kernel(
     ASrc A0, const __bf16* __restrict__ Bh0, const __bf16* __restrict__ Bl0, size_t ldb, int K, int nmb, int nnb, int G,
     int remap, int ksplit, Epi E0, size_t a_bstride, size_t b_bstride, int a_add, int a_shift) {
@@ -779,6 +786,32 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void g
         *(bf8_t*)&sl[lds_slot(br + 128, c)] = rb[3];
     };
     auto stage = [&](int s, int plane) { return g3_lds + (size_t)(4 * s + plane) * G3_STAGE; };
+    const int kq = lane >> 4, fr = lane & 15;
+    // DL: the 8 wave-instructions of this wave (u = 8 wave + q): operand u / 32
+    // (A, B), plane (u / 16) % 2, rows 16 (u % 16) .. + 15; this lane's source
+    const __bf16* dl_src[8];
+    int dl_off[8];
+    if constexpr (DL) {
+        const int lr = lane >> 2, slot = lane & 3;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int u = wave * 8 + q, opnd = u >> 5, plane = (u >> 4) & 1, rb = u & 15;
+            const int r = rb * 16 + lr;
+            const int c = slot ^ (((r >> 1) ^ (r >> 2)) & 3);
+            const __bf16* base = opnd == 0 ? (plane == 0 ? A.hi : A.lo) : (plane == 0 ? Bh : Bl);
+            const size_t ldx = opnd == 0 ? A.ld : ldb;
+            dl_src[q] = base + (size_t)((opnd == 0 ? m0 : n0) + r) * ldx + c * 8;
+            dl_off[q] = (2 * opnd + plane) * G3_STAGE + rb * 16 * PK;
+        }
+    }
+    auto dl_issue = [&](int s, int k) {
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            __builtin_amdgcn_global_load_lds(dl_src[q] + k,
+                                             (__attribute__((address_space(3))) void*)(g3_lds + (size_t)s * 4 * G3_STAGE +
+                                                                                      dl_off[q]),
+                                             16, 0, 0);
+    };
     f4_t acc[8][4];
 #pragma unroll
     for (int i = 0; i < 8; i++)
@@ -788,18 +821,26 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void g
     // ksplit <= K / PK)
     const int nkall = K / PK;
     const int kt0 = (int)((long long)split * nkall / ksplit), nk = (int)((long long)(split + 1) * nkall / ksplit);
-    A.fetch(m0, kt0 * PK, ra);
-    fetchB(kt0 * PK);
-    A.store(stage(0, 0), stage(0, 1), ra);
-    storeB(stage(0, 2), stage(0, 3));
+    if constexpr (DL) {
+        dl_issue(0, kt0 * PK);
+        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    } else {
+        A.fetch(m0, kt0 * PK, ra);
+        fetchB(kt0 * PK);
+        A.store(stage(0, 0), stage(0, 1), ra);
+        storeB(stage(0, 2), stage(0, 3));
+    }
     __syncthreads();
-    const int kq = lane >> 4, fr = lane & 15;
     for (int kt = kt0; kt < nk; kt++) {
         const int cur = (kt - kt0) & 1;
         const bool more = kt + 1 < nk;
         if (more) {
-            A.fetch(m0, (kt + 1) * PK, ra);
-            fetchB((kt + 1) * PK);
+            if constexpr (DL) {
+                dl_issue(cur ^ 1, (kt + 1) * PK);
+            } else {
+                A.fetch(m0, (kt + 1) * PK, ra);
+                fetchB((kt + 1) * PK);
+            }
         }
         const __bf16* sAh = stage(cur, 0);
         const __bf16* sAl = stage(cur, 1);
@@ -816,7 +857,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void g
         for (int i = 0; i < 8; i++) {
             // the next step's LDS stores go out while the last two row
             // fragments' MFMAs run (measured 2-3 % faster than after them)
-            if (i == 6 && more) {
+            if (!DL && i == 6 && more) {
                 A.store(stage(cur ^ 1, 0), stage(cur ^ 1, 1), ra);
                 storeB(stage(cur ^ 1, 2), stage(cur ^ 1, 3));
             }
@@ -832,6 +873,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void g
 #pragma unroll
             for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
         }
+        // DL: this step's DMA loads into the other stage have landed
+        if constexpr (DL) __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
     }
 #pragma unroll
@@ -1218,12 +1261,26 @@ static int g_nt_gen = 0;
 static int g_s2_split = 8;
 static int g_s2_group = 2;
 
+// Operand tiles of the 256 x 256 GEMM (A from plane pairs) into LDS by DMA
+// loads (1, measured 4-6 % faster at 32^3 x 4000 modes) or through VGPRs (0).
+static int g_g3_dl = 1;
+
+template <typename ASrc, typename Epi, bool DL>
+static void set_g3_attr() {
+    (void)hipFuncSetAttribute((const void*)gemm3_kernel<ASrc, Epi, DL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)G3_LDS_BYTES);
+}
+
 template <typename ASrc, typename Epi>
 static void launch_g3(opsec_ctx* ctx, const ASrc& A, int arows, const __bf16* B, int brows, size_t ld, int K,
                       const Epi& E, int ksplit = 1, int nbat = 1, size_t a_bstride = 0, size_t b_bstride = 0,
                       int a_add = 0, int a_shift = 0, size_t b_lo_offset = 0, int remap = -1, int group = -1) {
     static bool attr = false;
     if (!attr) {   // 128 KB of dynamic LDS per workgroup
+        set_g3_attr<ASrcPlanes, EpiPlanes, true>();
+        set_g3_attr<ASrcPlanes, EpiPacked, true>();
+        set_g3_attr<ASrcPlanes, EpiYhat, true>();
+        set_g3_attr<ASrcPlanes, EpiPackedParts, true>();
         (void)hipFuncSetAttribute((const void*)gemm3_kernel<ASrcPlanes, EpiPlanes>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)G3_LDS_BYTES);
         (void)hipFuncSetAttribute((const void*)gemm3_kernel<ASrcPlanes, EpiPacked>,
@@ -1240,9 +1297,18 @@ static void launch_g3(opsec_ctx* ctx, const ASrc& A, int arows, const __bf16* B,
     ksplit = std::max(1, std::min(ksplit, K / PK));
     // the lo plane follows the hi plane (brows x ld) unless an offset is given
     const size_t blo = b_lo_offset ? b_lo_offset : (size_t)brows * ld;
-    hipLaunchKernelGGL((gemm3_kernel<ASrc, Epi>), dim3(nmb * nnb * ksplit * nbat), dim3(512), G3_LDS_BYTES, ctx->stream,
-                       A, B, B + blo, ld, K, nmb, nnb, group < 0 ? g_nt_group : group, remap < 0 ? g_nt_remap : remap,
-                       ksplit, E, a_bstride, b_bstride, a_add, a_shift);
+    if constexpr (std::is_same<ASrc, ASrcPlanes>::value) {
+        if (g_g3_dl) {
+            hipLaunchKernelGGL((gemm3_kernel<ASrcPlanes, Epi, true>), dim3(nmb * nnb * ksplit * nbat), dim3(512),
+                               G3_LDS_BYTES, ctx->stream, A, B, B + blo, ld, K, nmb, nnb,
+                               group < 0 ? g_nt_group : group, remap < 0 ? g_nt_remap : remap, ksplit, E, a_bstride,
+                               b_bstride, a_add, a_shift);
+            return;
+        }
+    }
+        hipLaunchKernelGGL((gemm3_kernel<ASrc, Epi, false>), dim3(nmb * nnb * ksplit * nbat), dim3(512), G3_LDS_BYTES,
+                           ctx->stream, A, B, B + blo, ld, K, nmb, nnb, group < 0 ? g_nt_group : group,
+                           remap < 0 ? g_nt_remap : remap, ksplit, E, a_bstride, b_bstride, a_add, a_shift);
 }
 
 // C = A B^T (stage 1, epilogue to split planes) or P = A B^T (stage 2, packed).
@@ -1667,6 +1733,13 @@ static int run_project_toeplitz64(opsec_ctx* ctx, int nb, const int* funcs, int 
 }
 
 // Stage 1 + stage 2 for nb functions (device pair tables), results to d_P.
+// Dense FP64 / FP32 projection of the bands funcs[0..nb) in groups: stage 1
+// (all bands of a group in one launch, grid.z = band) keeps the group's
+// products C_p = T_p B' (N x M each), so a group holds at most
+// DENSE_GROUP_BYTES of them (all 40 bands of the 32^3 box fit; the 50 bands
+// of the 65536-cell shell at 6000 modes take seven groups).
+constexpr size_t DENSE_GROUP_BYTES = (size_t)48 << 30;
+
 static int run_project(opsec_ctx* ctx, int nb, const int* d_funcs, const int* funcs, int prec) {
     if (prec == OPSEC_PREC_BF16X3 && ctx->projection == OPSEC_PROJ_TOEPLITZ) return run_project_toeplitz(ctx, nb, funcs);
     if (ctx->projection == OPSEC_PROJ_TOEPLITZ) return run_project_toeplitz64(ctx, nb, funcs, prec);
@@ -1675,8 +1748,10 @@ static int run_project(opsec_ctx* ctx, int nb, const int* d_funcs, const int* fu
     const int N = ctx->ncells, M = ctx->nmodes;
     const size_t esize = (prec == OPSEC_PREC_FP64) ? 8 : 4;
     const size_t ldc = ps->ldw;   // stage-1 products use the same padded pitch as B'
+    const size_t per_band = ldc * M * esize;
+    const int grp = (int)std::max<size_t>(1, std::min<size_t>(nb, DENSE_GROUP_BYTES / per_band));
     int rc;
-    if ((rc = dev_realloc(ctx, &ps->d_C, &ps->cap_C, (size_t)nb * ldc * M * esize))) return rc;
+    if ((rc = dev_realloc(ctx, &ps->d_C, &ps->cap_C, (size_t)grp * per_band))) return rc;
     if ((rc = dev_realloc(ctx, &ps->d_P, &ps->cap_P, (size_t)nb * packed_size(M) * 8))) return rc;
     const bool sph = ctx->sv.coordsys == OPSEC_COORD_SPHERICAL;
     GA_PairTable A;
@@ -1686,7 +1761,6 @@ static int run_project(opsec_ctx* ctx, int nb, const int* d_funcs, const int* fu
     A.Nz = ctx->sv.n3;
     A.narcs = ctx->sv.n1 * ctx->sv.n2;
     A.qstride = ctx->nkeys;
-    A.funcs = d_funcs;
     GA_PairTableSph As;
     As.Q = A.Q;
     As.cellidx = A.cellidx;
@@ -1694,54 +1768,67 @@ static int run_project(opsec_ctx* ctx, int nb, const int* d_funcs, const int* fu
     As.Nz = A.Nz;
     As.narcs = A.narcs;
     As.qstride = A.qstride;
-    As.funcs = A.funcs;
     GB_ColMajor Bop;
     Bop.p = ps->d_Bw;
     Bop.ld = ps->ldw;
-    hipEvent_t e0, e1, e2;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventCreate(&e2);
-    hipEventRecord(e0, ctx->stream);
-    dim3 g1 = grid_for(N, M, nb), g2 = grid_for(M, M, nb);
     GA_ColMajorT A2;
     A2.p = ps->d_Bw;
     A2.ld = ps->ldw;
-    PutPacked outP{(double*)ps->d_P, (size_t)packed_size(M)};
-    if (prec == OPSEC_PREC_FP64) {
-        PutColMajor<double> outC{(double*)ps->d_C, ldc, ldc * M};
-        if (sph)
-            hipLaunchKernelGGL((gemm_kernel<double, GA_PairTableSph, GB_ColMajor, PutColMajor<double>, false>), g1,
-                               dim3(GT_THREADS), 0, ctx->stream, As, 0, Bop, 0, outC, N, M, N);
-        else
-            hipLaunchKernelGGL((gemm_kernel<double, GA_PairTable, GB_ColMajor, PutColMajor<double>, false>), g1,
-                               dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, 0, outC, N, M, N);
-        hipEventRecord(e1, ctx->stream);
-        GB_ColMajorT<double> B2{(const double*)ps->d_C, ldc};
-        hipLaunchKernelGGL((gemm_kernel<double, GA_ColMajorT, GB_ColMajorT<double>, PutPacked, true>), g2,
-                           dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, ldc * M, outP, M, M, N);
-    } else if (prec == OPSEC_PREC_FP32) {
-        PutColMajor<float> outC{(float*)ps->d_C, ldc, ldc * M};
-        if (sph)
-            hipLaunchKernelGGL((gemm_kernel<float, GA_PairTableSph, GB_ColMajor, PutColMajor<float>, false>), g1,
-                               dim3(GT_THREADS), 0, ctx->stream, As, 0, Bop, 0, outC, N, M, N);
-        else
-            hipLaunchKernelGGL((gemm_kernel<float, GA_PairTable, GB_ColMajor, PutColMajor<float>, false>), g1,
-                               dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, 0, outC, N, M, N);
-        hipEventRecord(e1, ctx->stream);
-        GB_ColMajorT<float> B2{(const float*)ps->d_C, ldc};
-        hipLaunchKernelGGL((gemm_kernel<float, GA_ColMajorT, GB_ColMajorT<float>, PutPacked, true>), g2,
-                           dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, ldc * M, outP, M, M, N);
+    const size_t np = (size_t)packed_size(M);
+    const int ngroups = (nb + grp - 1) / grp;
+    std::vector<hipEvent_t> ev(2 * (size_t)ngroups + 1);
+    for (auto& e : ev) OPSEC_HIP_TRY(ctx, hipEventCreate(&e));
+    hipEventRecord(ev[0], ctx->stream);
+    for (int gi = 0; gi < ngroups; gi++) {
+        const int z0 = gi * grp, g = std::min(grp, nb - z0);
+        A.funcs = d_funcs + z0;
+        As.funcs = d_funcs + z0;
+        dim3 g1 = grid_for(N, M, g), g2 = grid_for(M, M, g);
+        PutPacked outP{(double*)ps->d_P + (size_t)z0 * np, np};
+        if (prec == OPSEC_PREC_FP64) {
+            PutColMajor<double> outC{(double*)ps->d_C, ldc, ldc * M};
+            if (sph)
+                hipLaunchKernelGGL((gemm_kernel<double, GA_PairTableSph, GB_ColMajor, PutColMajor<double>, false>), g1,
+                                   dim3(GT_THREADS), 0, ctx->stream, As, 0, Bop, 0, outC, N, M, N);
+            else
+                hipLaunchKernelGGL((gemm_kernel<double, GA_PairTable, GB_ColMajor, PutColMajor<double>, false>), g1,
+                                   dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, 0, outC, N, M, N);
+            hipEventRecord(ev[2 * gi + 1], ctx->stream);
+            GB_ColMajorT<double> B2{(const double*)ps->d_C, ldc};
+            hipLaunchKernelGGL((gemm_kernel<double, GA_ColMajorT, GB_ColMajorT<double>, PutPacked, true>), g2,
+                               dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, ldc * M, outP, M, M, N);
+        } else {
+            PutColMajor<float> outC{(float*)ps->d_C, ldc, ldc * M};
+            if (sph)
+                hipLaunchKernelGGL((gemm_kernel<float, GA_PairTableSph, GB_ColMajor, PutColMajor<float>, false>), g1,
+                                   dim3(GT_THREADS), 0, ctx->stream, As, 0, Bop, 0, outC, N, M, N);
+            else
+                hipLaunchKernelGGL((gemm_kernel<float, GA_PairTable, GB_ColMajor, PutColMajor<float>, false>), g1,
+                                   dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, 0, outC, N, M, N);
+            hipEventRecord(ev[2 * gi + 1], ctx->stream);
+            GB_ColMajorT<float> B2{(const float*)ps->d_C, ldc};
+            hipLaunchKernelGGL((gemm_kernel<float, GA_ColMajorT, GB_ColMajorT<float>, PutPacked, true>), g2,
+                               dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, ldc * M, outP, M, M, N);
+        }
+        hipEventRecord(ev[2 * gi + 2], ctx->stream);
     }
-    hipEventRecord(e2, ctx->stream);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return set_error(ctx, OPSEC_EHIP, "projection launch: %s", hipGetErrorString(e));
-    OPSEC_HIP_TRY(ctx, hipEventSynchronize(e2));
-    hipEventElapsedTime(&ps->last_ms_stage1, e0, e1);
-    hipEventElapsedTime(&ps->last_ms_stage2, e1, e2);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    hipEventDestroy(e2);
+    if (e != hipSuccess) rc = set_error(ctx, OPSEC_EHIP, "projection launch: %s", hipGetErrorString(e));
+    hipError_t es = hipEventSynchronize(ev[2 * ngroups]);
+    if (!rc && es != hipSuccess) rc = set_error(ctx, OPSEC_EHIP, "projection: %s", hipGetErrorString(es));
+    float t1 = 0, t2 = 0;
+    if (!rc)
+        for (int gi = 0; gi < ngroups; gi++) {
+            float a, b;
+            hipEventElapsedTime(&a, ev[2 * gi], ev[2 * gi + 1]);
+            hipEventElapsedTime(&b, ev[2 * gi + 1], ev[2 * gi + 2]);
+            t1 += a;
+            t2 += b;
+        }
+    for (auto& x : ev) hipEventDestroy(x);
+    if (rc) return rc;
+    ps->last_ms_stage1 = t1;
+    ps->last_ms_stage2 = t2;
     ps->nb = nb;
     ps->prec = prec;
     return OPSEC_OK;
@@ -1894,6 +1981,14 @@ int opsec_set_projection(opsec_ctx* ctx, int method) {
 int opsec_diag_set_nt_gen(int v) {
     if (v != 0 && v != 1) return OPSEC_EINVAL;
     g_nt_gen = v;
+    return OPSEC_OK;
+}
+
+// Operand tiles of the 256 x 256 split-BF16x3 GEMM: through VGPRs (0) or by
+// global-to-LDS DMA loads (1), for tuning.
+int opsec_diag_set_g3_dl(int v) {
+    if (v != 0 && v != 1) return OPSEC_EINVAL;
+    g_g3_dl = v;
     return OPSEC_OK;
 }
 

@@ -206,11 +206,13 @@ def test_project_bf16x3_generated_layouts(ctx, n, shuffle):
     assert err.max() < 5e-5
 
 
-@pytest.mark.parametrize("tile,gen", [(1, 0), (2, 0), (2, 1)])
-def test_project_bf16x3_gemm_variants(ctx, tile, gen):
+@pytest.mark.parametrize("tile,gen,dl", [(1, 0, 0), (2, 0, 0), (2, 1, 0), (2, 0, 1), (2, 1, 1)])
+def test_project_bf16x3_gemm_variants(ctx, tile, gen, dl):
     """The split-BF16x3 GEMM variants (256 x 128; 256 x 256 double-buffered,
-    with S_p planes or S_p generated from the pair table), with split-K in
-    stage 2, against the FP64 projection of the same pair tables."""
+    with S_p planes or S_p generated from the pair table, tiles staged through
+    VGPRs or by DMA loads into LDS), with split-K in stage 2, against the FP64
+    projection of the same pair tables; the Toeplitz path too with B in
+    registers."""
     n = 10
     cells = synth.box_cells(n, 0.0, 250.0)
     sv = lib.survey_box(n, 0.0, 250.0)
@@ -228,14 +230,20 @@ def test_project_bf16x3_gemm_variants(ctx, tile, gen):
         assert ctx.lib.opsec_diag_set_nt_tile(tile) == 0
         assert ctx.lib.opsec_diag_set_nt_gen(gen) == 0
         assert ctx.lib.opsec_diag_set_stage2_split(3) == 0
+        assert ctx.lib.opsec_diag_set_g3_dl(dl) == 0
         got = ctx.project(funcs, lib.OPSEC_PREC_BF16X3)
+        ctx.set_projection(lib.OPSEC_PROJ_TOEPLITZ)
+        got_t = ctx.project(funcs, lib.OPSEC_PREC_BF16X3)
     finally:
+        ctx.set_projection(lib.OPSEC_PROJ_DENSE)
         ctx.lib.opsec_diag_set_nt_tile(2)
         ctx.lib.opsec_diag_set_nt_gen(0)
         ctx.lib.opsec_diag_set_stage2_split(8)
-    err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
-    print("tile", tile, "gen", gen, "max rel err per function", err)
-    assert err.max() < 5e-5
+        ctx.lib.opsec_diag_set_g3_dl(1)
+    for name, g in (("dense", got), ("toeplitz", got_t)):
+        err = np.abs(g - ref).max(axis=1) / np.abs(ref).max(axis=1)
+        print(name, "tile", tile, "gen", gen, "dl", dl, "max rel err per function", err)
+        assert err.max() < 5e-5
 
 
 @pytest.mark.parametrize("dims,drop,shuffle", [((8, 8, 8), 0.0, False), ((6, 5, 4), 0.2, True), ((3, 7, 9), 0.0, True),

@@ -164,6 +164,45 @@ def diagnose(ctx, L, funcs, q64, qs, lam, y, prior, e64):
     return out
 
 
+def library_ceiling(shapes):
+    """Plain BF16 GEMM rate of the vendor library (torch.matmul / torch.bmm on
+    ROCm: hipBLASLt, FP32 accumulation) on the projection GEMM shapes: the MFMA
+    rate this device sustains in practice, against which the split-BF16x3
+    kernels' raw rate (3 MFMA products per useful flop) is read.  shapes:
+    name -> (batch, m, n, k).  Runs in a child process: torch brings its own
+    HIP runtime, which does not share a process with the library's."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--library-ceiling-child", json.dumps(shapes)],
+                       capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        return {"error": r.stderr.strip().splitlines()[-1] if r.stderr.strip() else f"exit {r.returncode}"}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def _library_ceiling_child(shapes, reps=10):
+    import torch
+    dev = torch.device("cuda:0")
+    out = {}
+    for name, (bt, m, n, k) in shapes.items():
+        a = torch.randn((bt, m, k) if bt > 1 else (m, k), device=dev, dtype=torch.bfloat16)
+        b = torch.randn((bt, k, n) if bt > 1 else (k, n), device=dev, dtype=torch.bfloat16)
+        f = (lambda: torch.bmm(a, b)) if bt > 1 else (lambda: a @ b)
+        for _ in range(3):
+            c = f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            c = f()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = {"shape": [bt, m, n, k], "ms": ms, "tflops": 2.0 * bt * m * n * k / ms / 1e9}
+        del a, b, c
+        torch.cuda.empty_cache()
+    return out
+
+
 def _blas_rate(N, B, R=1024):
     """FP64 GEMM rate of the host BLAS (numpy's OpenBLAS) on an R x N by N x M
     block of the projection's first product, S_p B (flop/s)."""
@@ -390,6 +429,9 @@ def spherical_main(args):
 
 
 def main():
+    if len(sys.argv) == 3 and sys.argv[1] == "--library-ceiling-child":
+        print(json.dumps(_library_ceiling_child({k: tuple(v) for k, v in json.loads(sys.argv[2]).items()})))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
@@ -422,8 +464,11 @@ def main():
     ap.add_argument("--nt-group", type=int, default=8, help="split-BF16x3 GEMM tile-group edge (stage 1)")
     ap.add_argument("--s2-group", type=int, default=2, help="split-BF16x3 stage-2 tile-group edge")
     ap.add_argument("--nt-remap", type=int, default=0, choices=[0, 1], help="deal tile groups to XCDs")
-    ap.add_argument("--g3-dl", type=int, default=1, choices=[0, 1],
-                    help="256 x 256 split-BF16x3 GEMM: operand tiles through VGPRs (0) or by DMA loads into LDS (1)")
+    ap.add_argument("--g3-dl", type=int, default=2, choices=[0, 1, 2],
+                    help="256 x 256 split-BF16x3 GEMM: operand tiles through VGPRs (0) or by DMA loads into LDS "
+                         "(1; 2: issued in two halves per step)")
+    ap.add_argument("--library-ceiling", action="store_true",
+                    help="also time the vendor library's plain BF16 GEMM on the projection shapes (imports torch)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also time a plain device copy of this many GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -717,6 +762,17 @@ def main():
         ctx.dev_free(dst)
     if klt_info is not None:
         out["klt"] = klt_info
+    if args.library_ceiling and main_path == "surrogate" and args.projection == "toeplitz":
+        Mp = (M + 255) // 256 * 256
+        lc = library_ceiling({"toeplitz_stage1": (2 * n, n * n, Mp, n * n), "toeplitz_stage2": (1, Mp, Mp, N),
+                              "dense_stage1": (1, N, Mp, N)})
+        rf = out["roofline"]
+        if "error" not in lc:
+            lc["kernel_raw_mfma_tflops"] = {"toeplitz_stage1": 3 * rf["stage1_tflops"],
+                                            "toeplitz_stage2": 3 * rf["stage2_tflops"]}
+            lc["kernel_over_library"] = {k2: lc["kernel_raw_mfma_tflops"][k2] / lc[k2]["tflops"]
+                                         for k2 in ("toeplitz_stage1", "toeplitz_stage2")}
+        out["library_ceiling"] = lc
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(cfg, cells, k, P, lo, hi, B)

@@ -54,7 +54,9 @@ int opsec_diag_set_stage2_split(int v);
 int opsec_diag_set_stage2_group(int v);
 int opsec_diag_set_nt_order(int group, int remap);
 /* Operand tiles of the 256 x 256 split-BF16x3 GEMM (A from plane pairs):
- * through VGPRs and LDS stores (0) or by global-to-LDS DMA loads (1, default). */
+ * through VGPRs and LDS stores (0) or by global-to-LDS DMA loads, all of a
+ * step's loads issued at its start (1) or in two halves, at its start and
+ * mid-step (2, the default). */
 int opsec_diag_set_g3_dl(int v);
 
 #ifdef __cplusplus

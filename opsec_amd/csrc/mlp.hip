@@ -322,6 +322,24 @@ __global__ void sum_f_parts_kernel(const float* __restrict__ parts, int nparts, 
     out[i] = (float)acc;
 }
 
+// out[i] = sum_z parts[z n + i] for many parts: block i, thread t sums parts
+// t, t + 256, ... in order, then a fixed tree over the 256 threads (FP64)
+__global__ __launch_bounds__(256) void sum_f_cols_kernel(const float* __restrict__ parts, int nparts, int n,
+                                                         float* __restrict__ out) {
+    __shared__ double red[256];
+    const int i = blockIdx.x, t = threadIdx.x;
+    double acc = 0.0;
+    for (int z = t; z < nparts; z += 256) acc += (double)parts[(size_t)z * n + i];
+    red[t] = acc;
+    __syncthreads();
+#pragma unroll
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) out[i] = (float)red[0];
+}
+
 // *loss += sum of the per-block partial losses, in block order
 __global__ void sum_loss_kernel(const double* __restrict__ parts, int nparts, double* loss) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -481,7 +499,7 @@ static void loss_grad_device(opsec_ctx* ctx, const float* dx, const float* dt, i
     // last layer -> delta for layer 3
     hipLaunchKernelGGL(mlp_last_back_kernel, dim3(nrb), dim3(MLP_H), 0, st, s->out, s->a + (size_t)4 * B * MLP_H,
                        s->z + (size_t)3 * B * MLP_H, s->w, n, s->gparts, s->delta);
-    hipLaunchKernelGGL(sum_f_parts_kernel, dim3(1), dim3(MLP_H + 1), 0, st, s->gparts, nrb, MLP_H + 1,
+    hipLaunchKernelGGL(sum_f_cols_kernel, dim3(MLP_H + 1), dim3(256), 0, st, s->gparts, nrb, MLP_H + 1,
                        s->g + mlp_w_off(4));
     float* d = s->delta;
     float* dn = s->delta2;
@@ -496,7 +514,7 @@ static void loss_grad_device(opsec_ctx* ctx, const float* dx, const float* dt, i
         hipLaunchKernelGGL(sum_f_parts_kernel, dim3((nw + 255) / 256), dim3(256), 0, st, s->gparts, splits, nw,
                            s->g + mlp_w_off(l));
         hipLaunchKernelGGL(mlp_bias_grad_kernel, dim3(nrb), dim3(MLP_H), 0, st, d, n, s->gparts);
-        hipLaunchKernelGGL(sum_f_parts_kernel, dim3(1), dim3(MLP_H), 0, st, s->gparts, nrb, MLP_H,
+        hipLaunchKernelGGL(sum_f_cols_kernel, dim3(MLP_H), dim3(256), 0, st, s->gparts, nrb, MLP_H,
                            s->g + mlp_w_off(l) + din * MLP_H);
         if (l > 0) {
             // dprev (n x 128) = d (n x 128) . W_l (128 x 128)

@@ -745,8 +745,10 @@ struct ASrcTable {
 // staging and no LDS store instructions.  A wave-instruction fills 1 KB of LDS,
 // 16 B per lane in lane order: 16 tile rows of one plane; lane L takes row
 // 16 rb + L/4 and the chunk that the swizzle puts in its slot L % 4.  The 64
-// instructions of a step are spread over the 8 waves.
-template <typename ASrc, typename Epi, bool DL = false>
+// instructions of a step are spread over the 8 waves.  DL = 2 issues half of a
+// wave's DMA loads at the start of the step and the other half after its
+// fourth row fragment, which spreads the LDS writes over the step.
+template <typename ASrc, typename Epi, int DL = 0>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void gemm3_kernel(
     ASrc A0, const __bf16* __restrict__ Bh0, const __bf16* __restrict__ Bl0, size_t ldb, int K, int nmb, int nnb, int G,
     int remap, int ksplit, Epi E0, size_t a_bstride, size_t b_bstride, int a_add, int a_shift) {
@@ -804,9 +806,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void g
             dl_off[q] = (2 * opnd + plane) * G3_STAGE + rb * 16 * PK;
         }
     }
-    auto dl_issue = [&](int s, int k) {
+    auto dl_issue = [&](int s, int k, int q0, int q1) {
 #pragma unroll
-        for (int q = 0; q < 8; q++)
+        for (int q = q0; q < q1; q++)
             __builtin_amdgcn_global_load_lds(dl_src[q] + k,
                                              (__attribute__((address_space(3))) void*)(g3_lds + (size_t)s * 4 * G3_STAGE +
                                                                                       dl_off[q]),
@@ -822,7 +824,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void g
     const int nkall = K / PK;
     const int kt0 = (int)((long long)split * nkall / ksplit), nk = (int)((long long)(split + 1) * nkall / ksplit);
     if constexpr (DL) {
-        dl_issue(0, kt0 * PK);
+        dl_issue(0, kt0 * PK, 0, 8);
         __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
     } else {
         A.fetch(m0, kt0 * PK, ra);
@@ -836,7 +838,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void g
         const bool more = kt + 1 < nk;
         if (more) {
             if constexpr (DL) {
-                dl_issue(cur ^ 1, (kt + 1) * PK);
+                dl_issue(cur ^ 1, (kt + 1) * PK, 0, DL == 2 ? 4 : 8);
             } else {
                 A.fetch(m0, (kt + 1) * PK, ra);
                 fetchB((kt + 1) * PK);
@@ -861,6 +863,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void g
                 A.store(stage(cur ^ 1, 0), stage(cur ^ 1, 1), ra);
                 storeB(stage(cur ^ 1, 2), stage(cur ^ 1, 3));
             }
+            if (DL == 2 && i == 4 && more) dl_issue(cur ^ 1, (kt + 1) * PK, 4, 8);
             const int o = lds_slot(wm + i * 16 + fr, kq);
             const bf8_t ah = *(const bf8_t*)&sAh[o];
             const bf8_t al = *(const bf8_t*)&sAl[o];
@@ -874,7 +877,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void g
             for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
         }
         // DL: this step's DMA loads into the other stage have landed
-        if constexpr (DL) __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        if constexpr (DL != 0) __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
     }
 #pragma unroll
@@ -1262,10 +1265,11 @@ static int g_s2_split = 8;
 static int g_s2_group = 2;
 
 // Operand tiles of the 256 x 256 GEMM (A from plane pairs) into LDS by DMA
-// loads (1, measured 4-6 % faster at 32^3 x 4000 modes) or through VGPRs (0).
-static int g_g3_dl = 1;
+// loads (1, measured 4-6 % faster at 32^3 x 4000 modes than through VGPRs, 0),
+// issued in two halves per step (2, another 1-3 %).
+static int g_g3_dl = 2;
 
-template <typename ASrc, typename Epi, bool DL>
+template <typename ASrc, typename Epi, int DL>
 static void set_g3_attr() {
     (void)hipFuncSetAttribute((const void*)gemm3_kernel<ASrc, Epi, DL>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)G3_LDS_BYTES);
@@ -1277,10 +1281,14 @@ static void launch_g3(opsec_ctx* ctx, const ASrc& A, int arows, const __bf16* B,
                       int a_add = 0, int a_shift = 0, size_t b_lo_offset = 0, int remap = -1, int group = -1) {
     static bool attr = false;
     if (!attr) {   // 128 KB of dynamic LDS per workgroup
-        set_g3_attr<ASrcPlanes, EpiPlanes, true>();
-        set_g3_attr<ASrcPlanes, EpiPacked, true>();
-        set_g3_attr<ASrcPlanes, EpiYhat, true>();
-        set_g3_attr<ASrcPlanes, EpiPackedParts, true>();
+        set_g3_attr<ASrcPlanes, EpiPlanes, 1>();
+        set_g3_attr<ASrcPlanes, EpiPacked, 1>();
+        set_g3_attr<ASrcPlanes, EpiYhat, 1>();
+        set_g3_attr<ASrcPlanes, EpiPackedParts, 1>();
+        set_g3_attr<ASrcPlanes, EpiPlanes, 2>();
+        set_g3_attr<ASrcPlanes, EpiPacked, 2>();
+        set_g3_attr<ASrcPlanes, EpiYhat, 2>();
+        set_g3_attr<ASrcPlanes, EpiPackedParts, 2>();
         (void)hipFuncSetAttribute((const void*)gemm3_kernel<ASrcPlanes, EpiPlanes>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)G3_LDS_BYTES);
         (void)hipFuncSetAttribute((const void*)gemm3_kernel<ASrcPlanes, EpiPacked>,
@@ -1298,15 +1306,22 @@ static void launch_g3(opsec_ctx* ctx, const ASrc& A, int arows, const __bf16* B,
     // the lo plane follows the hi plane (brows x ld) unless an offset is given
     const size_t blo = b_lo_offset ? b_lo_offset : (size_t)brows * ld;
     if constexpr (std::is_same<ASrc, ASrcPlanes>::value) {
-        if (g_g3_dl) {
-            hipLaunchKernelGGL((gemm3_kernel<ASrcPlanes, Epi, true>), dim3(nmb * nnb * ksplit * nbat), dim3(512),
+        if (g_g3_dl == 1) {
+            hipLaunchKernelGGL((gemm3_kernel<ASrcPlanes, Epi, 1>), dim3(nmb * nnb * ksplit * nbat), dim3(512),
+                               G3_LDS_BYTES, ctx->stream, A, B, B + blo, ld, K, nmb, nnb,
+                               group < 0 ? g_nt_group : group, remap < 0 ? g_nt_remap : remap, ksplit, E, a_bstride,
+                               b_bstride, a_add, a_shift);
+            return;
+        }
+        if (g_g3_dl == 2) {
+            hipLaunchKernelGGL((gemm3_kernel<ASrcPlanes, Epi, 2>), dim3(nmb * nnb * ksplit * nbat), dim3(512),
                                G3_LDS_BYTES, ctx->stream, A, B, B + blo, ld, K, nmb, nnb,
                                group < 0 ? g_nt_group : group, remap < 0 ? g_nt_remap : remap, ksplit, E, a_bstride,
                                b_bstride, a_add, a_shift);
             return;
         }
     }
-        hipLaunchKernelGGL((gemm3_kernel<ASrc, Epi, false>), dim3(nmb * nnb * ksplit * nbat), dim3(512), G3_LDS_BYTES,
+        hipLaunchKernelGGL((gemm3_kernel<ASrc, Epi, 0>), dim3(nmb * nnb * ksplit * nbat), dim3(512), G3_LDS_BYTES,
                            ctx->stream, A, B, B + blo, ld, K, nmb, nnb, group < 0 ? g_nt_group : group,
                            remap < 0 ? g_nt_remap : remap, ksplit, E, a_bstride, b_bstride, a_add, a_shift);
 }
@@ -1987,7 +2002,7 @@ int opsec_diag_set_nt_gen(int v) {
 // Operand tiles of the 256 x 256 split-BF16x3 GEMM: through VGPRs (0) or by
 // global-to-LDS DMA loads (1), for tuning.
 int opsec_diag_set_g3_dl(int v) {
-    if (v != 0 && v != 1) return OPSEC_EINVAL;
+    if (v < 0 || v > 2) return OPSEC_EINVAL;
     g_g3_dl = v;
     return OPSEC_OK;
 }

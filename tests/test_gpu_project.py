@@ -206,7 +206,7 @@ def test_project_bf16x3_generated_layouts(ctx, n, shuffle):
     assert err.max() < 5e-5
 
 
-@pytest.mark.parametrize("tile,gen,dl", [(1, 0, 0), (2, 0, 0), (2, 1, 0), (2, 0, 1), (2, 1, 1)])
+@pytest.mark.parametrize("tile,gen,dl", [(1, 0, 0), (2, 0, 0), (2, 1, 0), (2, 0, 1), (2, 1, 1), (2, 0, 2)])
 def test_project_bf16x3_gemm_variants(ctx, tile, gen, dl):
     """The split-BF16x3 GEMM variants (256 x 128; 256 x 256 double-buffered,
     with S_p planes or S_p generated from the pair table, tiles staged through
@@ -239,7 +239,7 @@ def test_project_bf16x3_gemm_variants(ctx, tile, gen, dl):
         ctx.lib.opsec_diag_set_nt_tile(2)
         ctx.lib.opsec_diag_set_nt_gen(0)
         ctx.lib.opsec_diag_set_stage2_split(8)
-        ctx.lib.opsec_diag_set_g3_dl(1)
+        ctx.lib.opsec_diag_set_g3_dl(2)
     for name, g in (("dense", got), ("toeplitz", got_t)):
         err = np.abs(g - ref).max(axis=1) / np.abs(ref).max(axis=1)
         print(name, "tile", tile, "gen", gen, "dl", dl, "max rel err per function", err)

@@ -599,11 +599,17 @@ static void tridiag_top(int n, const std::vector<double>& d, const std::vector<d
         int blk, idx;
     };
     std::vector<EV> cand;
-    const double tcut = tau - 4 * eps * tnorm;
-    for (size_t bi = 0; bi < blocks.size(); bi++) {
-        const TriBlock& B = blocks[bi];
-        const int below = sturm_below(B.n, &d[B.off], &e2[B.off], tcut, pivmin);
-        for (int j = below; j < B.n; j++) cand.push_back({0.0, (int)bi, j});
+    // floating-point Sturm counts are not exactly monotone, so inside a tight
+    // cluster the blocks' counts at tcut can leave fewer than nev candidates:
+    // widen the margin until there are enough
+    for (double margin = 4 * eps * tnorm; cand.size() < (size_t)nev && margin < 2 * tnorm; margin *= 16) {
+        cand.clear();
+        const double tcut = tau - margin;
+        for (size_t bi = 0; bi < blocks.size(); bi++) {
+            const TriBlock& B = blocks[bi];
+            const int below = sturm_below(B.n, &d[B.off], &e2[B.off], tcut, pivmin);
+            for (int j = below; j < B.n; j++) cand.push_back({0.0, (int)bi, j});
+        }
     }
     // batches of 8 candidates of one block (each bisection converges alone)
     constexpr int NB = 8;
@@ -758,6 +764,8 @@ static int dense_eigen_top(Dev& D, double* dA, int K, int nev, std::vector<doubl
     g_klt_times[KT_TRIRED] += pt.lap();
     std::vector<double> Z;
     tridiag_top(K, d, e, nev, evals, Z);
+    if ((int)evals.size() < nev || Z.size() < (size_t)K * nev)
+        return set_error(ctx, OPSEC_ECONV, "tridiagonal eigen-solver: %d of %d eigenpairs", (int)evals.size(), nev);
     OPSEC_HIP_TRY(ctx, stream_copy(ctx, dZ, Z.data(), (size_t)K * nev * 8, hipMemcpyHostToDevice));
     g_klt_times[KT_TRIEIG] += pt.lap();
     // Back-transformation Z <- H_0 H_1 ... H_(K-3) Z in panels of HH_NB
@@ -1216,6 +1224,7 @@ int opsec_diag_tridiag_top(int n, const double* d, const double* e, int nev, dou
     std::vector<double> dd(d, d + n), ee(e, e + std::max(0, n - 1)), ev, Zv;
     if (ee.empty()) ee.push_back(0.0);
     tridiag_top(n, dd, ee, nev, ev, Zv);
+    if ((int)ev.size() < nev) return OPSEC_ECONV;
     std::copy(ev.begin(), ev.end(), evals);
     std::copy(Zv.begin(), Zv.end(), Z);
     return OPSEC_OK;

@@ -151,6 +151,106 @@ __device__ __forceinline__ void gm_node(int p, const double* c, const double* hw
     }
 }
 
+// ---- spherical integrand with per-region tables ------------------------------
+// Along each axis the 149 nodes of a region take 7 coordinates: value index 0
+// the centre c, 1/2 c -/+ hw L2, 3/4 c -/+ hw L4, 5/6 c -/+ hw L5 (the same
+// c +- hw lambda sums gm_node forms).  The transcendentals of the spherical
+// integrand depend on one axis (cbrt of r^3) or on an axis pair (the mu and phi
+// pairs) and take at most 17 value-index pairs there, so they are computed once
+// per region: 14 cbrt, 17 sqrt and 17 cos instead of 2, 2 and 1 per node.  Each
+// table entry is the expression the per-node integrand forms, evaluated the same
+// way, so the node values are bit-identical.
+
+// Value index of node p along each axis (the node order of gm_node).
+__device__ __forceinline__ void gm_node_vi(int p, int* vi) {
+    for (int d = 0; d < 6; d++) vi[d] = 0;
+    if (p == 0) return;
+    if (p <= 24) {
+        const int q = p - 1;
+        vi[q >> 2] = 1 + (q & 3);
+        return;
+    }
+    if (p <= 84) {
+        int q = p - 25, pair = q >> 2, s = q & 3;
+        int i = 0, j = 1, k = pair;
+        for (i = 0; i < 5; i++) {
+            int cnt = 5 - i;
+            if (k < cnt) { j = i + 1 + k; break; }
+            k -= cnt;
+        }
+        vi[i] = (s == 1 || s == 2) ? 4 : 3;
+        vi[j] = (s >= 2) ? 4 : 3;
+        return;
+    }
+    const int g = p - 85, signs = g ^ (g >> 1);
+    for (int d = 0; d < 6; d++) vi[d] = ((signs >> d) & 1) ? 5 : 6;
+}
+
+// The 17 value-index pairs of an axis pair: (0,0), (1..4,0), (0,1..4),
+// (3|4,3|4), (5|6,5|6).
+constexpr int SPH_NCMB = 17;
+__device__ __forceinline__ int sph_cmb(int a, int b) {
+    if (b == 0) return a;
+    if (a == 0) return 4 + b;
+    if (a <= 4) return 9 + 2 * (a - 3) + (b - 3);
+    return 13 + 2 * (a - 5) + (b - 5);
+}
+__device__ __forceinline__ void sph_cmb_pair(int k, int& a, int& b) {
+    if (k <= 4) { a = k; b = 0; return; }
+    if (k <= 8) { a = 0; b = k - 4; return; }
+    if (k <= 12) { a = 3 + ((k - 9) >> 1); b = 3 + ((k - 9) & 1); return; }
+    a = 5 + ((k - 13) >> 1);
+    b = 5 + ((k - 13) & 1);
+}
+
+// Coordinate of value index v on an axis with centre c and half-width hw.
+__device__ __forceinline__ double gm_axis_value(double c, double hw, int v) {
+    if (v == 0) return c;
+    const double r = hw * (v <= 2 ? GM_L2 : v <= 4 ? GM_L4 : GM_L5);
+    return (v & 1) ? c - r : c + r;
+}
+
+// Per-region tables, one layout for registers (one thread per region) and LDS
+// (a wavefront's regions): 7 values of r1, r2, mu1, mu2, then the 17
+// sqrt((1 - mu1^2)(1 - mu2^2)) and cos(phi1 - phi2) of the pairs.
+constexpr int SPH_TAB = 4 * 7 + 2 * SPH_NCMB;
+struct SphIdx {
+    enum { R1 = 0, R2 = 7, M1 = 14, M2 = 21, SQ = 28, CS = 28 + SPH_NCMB };
+};
+
+// Table entries of region (c, hw) of job J (src/sig.cpp:436-462 mapping:
+// lo/hi = {r1^3, r2^3, mu1, mu2, phi1, phi2}); entry e of SPH_TAB, with the
+// phi values it needs formed on the spot.
+__device__ __forceinline__ double sph_entry(const GMJob& J, const double* c, const double* hw, int e) {
+    auto w = [&](int d, int v) {
+        const double u = gm_axis_value(c[d], hw[d], v);
+        return (1 - u) * J.lo[d] + u * J.hi[d];
+    };
+    if (e < SphIdx::SQ) {
+        const int q = e / 7, v = e - 7 * q;
+        return q < 2 ? cbrt(w(q, v)) : w(q, v);
+    }
+    int a, b;
+    if (e < SphIdx::CS) {
+        sph_cmb_pair(e - SphIdx::SQ, a, b);
+        const double m1 = w(2, a), m2 = w(3, b);
+        return sqrt((1 - m1 * m1) * (1 - m2 * m2));
+    }
+    sph_cmb_pair(e - SphIdx::CS, a, b);
+    return cos(w(4, a) - w(5, b));
+}
+
+// xi at node p from a region's table T (the expression of sph_r).
+template <typename Tab>
+__device__ __forceinline__ double sph_node_value(const XiModelDev& m, const XiDesc& xd, const Tab& T, int p) {
+    int vi[6];
+    gm_node_vi(p, vi);
+    const double r1 = T[SphIdx::R1 + vi[0]], r2 = T[SphIdx::R2 + vi[1]];
+    const double cosg = T[SphIdx::SQ + sph_cmb(vi[2], vi[3])] * T[SphIdx::CS + sph_cmb(vi[4], vi[5])] +
+                        T[SphIdx::M1 + vi[2]] * T[SphIdx::M2 + vi[3]];
+    return xi_eval(m, xd, sqrt(fmax(r1 * r1 + r2 * r2 - 2 * r1 * r2 * cosg, 0.0)));
+}
+
 // The degree-7 value, the degree-5 error and the split axis of one region from
 // its 149 node values, summed in node order (src/cubature.c:445-613).
 __device__ __forceinline__ void gm_region_sum(const double* vals, const double* hw, double& result, double& err,
@@ -192,20 +292,40 @@ constexpr int GM_SUB = 16;   // regions evaluated together by one wavefront
 // Evaluate the regions list[0..n): GM_SUB at a time, the lanes share out their
 // GM_SUB x 149 nodes, then lane t forms the rule of region t.
 __device__ void gm_eval_regions(const XiModelDev& m, const XiDesc& xd, const GMJob& J, const GMParams& P,
-                                GMSlot& S, int R, const int* list, int n, double (*vals)[GM_NPTS]) {
+                                GMSlot& S, int R, const int* list, int n, double (*vals)[GM_NPTS],
+                                double (*tab)[SPH_TAB]) {
     const int lane = threadIdx.x;
+    const bool sph = P.coordsys != OPSEC_COORD_CARTESIAN;
     for (int t0 = 0; t0 < n; t0 += GM_SUB) {
         const int nt = min(GM_SUB, n - t0);
-        for (int i = lane; i < nt * GM_NPTS; i += blockDim.x) {
-            const int t = i / GM_NPTS, p = i - t * GM_NPTS;
-            const int ri = list[t0 + t];
-            double c[6], hw[6], u[6];
-            for (int d = 0; d < 6; d++) {
-                c[d] = S.c[d * R + ri];
-                hw[d] = S.hw[d * R + ri];
+        if (sph) {   // the regions' tables, then the nodes from them
+            for (int i = lane; i < nt * SPH_TAB; i += blockDim.x) {
+                const int t = i / SPH_TAB, e = i - t * SPH_TAB;
+                const int ri = list[t0 + t];
+                double c[6], hw[6];
+                for (int d = 0; d < 6; d++) {
+                    c[d] = S.c[d * R + ri];
+                    hw[d] = S.hw[d * R + ri];
+                }
+                tab[t][e] = sph_entry(J, c, hw, e);
             }
-            gm_node(p, c, hw, u);
-            vals[t][p] = gm_integrand(m, xd, J, P, u);
+            __syncthreads();
+            for (int i = lane; i < nt * GM_NPTS; i += blockDim.x) {
+                const int t = i / GM_NPTS, p = i - t * GM_NPTS;
+                vals[t][p] = sph_node_value(m, xd, tab[t], p);
+            }
+        } else {
+            for (int i = lane; i < nt * GM_NPTS; i += blockDim.x) {
+                const int t = i / GM_NPTS, p = i - t * GM_NPTS;
+                const int ri = list[t0 + t];
+                double c[6], hw[6], u[6];
+                for (int d = 0; d < 6; d++) {
+                    c[d] = S.c[d * R + ri];
+                    hw[d] = S.hw[d * R + ri];
+                }
+                gm_node(p, c, hw, u);
+                vals[t][p] = gm_integrand(m, xd, J, P, u);
+            }
         }
         __syncthreads();
         if (lane < nt) {
@@ -307,6 +427,7 @@ __global__ __launch_bounds__(NT) void gm_kernel(XiModelDev m, const GMJob* jobs,
                                                 size_t qstride, double* Qerr, int* jobfail,
                                                 unsigned long long* nevals) {
     __shared__ double vals[GM_SUB][GM_NPTS];
+    __shared__ double sphtab[GM_SUB][SPH_TAB];
     __shared__ double lkey[NL > 0 ? NL : 1];
     __shared__ int lidx[NL > 0 ? NL : 1];
     __shared__ int sh_job, sh_nR, sh_done;
@@ -330,7 +451,7 @@ __global__ __launch_bounds__(NT) void gm_kernel(XiModelDev m, const GMJob* jobs,
         }
         if (lane == 0) S.list[0] = 0;
         __syncthreads();
-        gm_eval_regions(m, xd, J, P, S, R, S.list, 1, vals);
+        gm_eval_regions(m, xd, J, P, S, R, S.list, 1, vals, sphtab);
         // lane-0 state (heap size, totals, evaluation count)
         int nheap = 0, nregions = 1;
         double tv = 0, te = 0;
@@ -387,7 +508,7 @@ __global__ __launch_bounds__(NT) void gm_kernel(XiModelDev m, const GMJob* jobs,
                 break;
             }
             __threadfence_block();
-            gm_eval_regions(m, xd, J, P, S, R, S.list, nR, vals);
+            gm_eval_regions(m, xd, J, P, S, R, S.list, nR, vals, sphtab);
             if (lane == 0)
                 for (int t = 0; t < nR; t++) heap_push(S, H, nheap, S.list[t], tv, te);
         }
@@ -450,6 +571,17 @@ __global__ __launch_bounds__(256) void gm_first_kernel(XiModelDev m, const GMJob
             c[d] = 0.5;
             hw[d] = 0.5;
         }
+        const bool sph = P.coordsys != OPSEC_COORD_CARTESIAN;
+        double T[SPH_TAB];
+        if (sph) {
+#pragma unroll
+            for (int e = 0; e < SPH_TAB; e++) T[e] = sph_entry(J, c, hw, e);
+        }
+        auto node = [&](int p) {
+            if (sph) return sph_node_value(m, xd, T, p);
+            gm_node(p, c, hw, u);
+            return gm_integrand(m, xd, J, P, u);
+        };
         const double ratio = (GM_L2 * GM_L2) / (GM_L4 * GM_L4);
         const double w1 = (double)(12824 - 9120 * 6 + 400 * 36) / 19683.0;
         const double w2 = 980. / 6561.;
@@ -460,26 +592,16 @@ __global__ __launch_bounds__(256) void gm_first_kernel(XiModelDev m, const GMJob
         const double e2 = 245. / 486.;
         const double e3 = (double)(265 - 100 * 6) / 1458.0;
         const double e4 = 25. / 729.;
-        gm_node(0, c, hw, u);
-        const double val0 = gm_integrand(m, xd, J, P, u);
+        const double val0 = node(0);
         double sum2 = 0, sum3 = 0, sum4 = 0, sum5 = 0;
         for (int k = 0; k < 6; k++) {
             double v[4];
-            for (int e = 0; e < 4; e++) {
-                gm_node(1 + 4 * k + e, c, hw, u);
-                v[e] = gm_integrand(m, xd, J, P, u);
-            }
+            for (int e = 0; e < 4; e++) v[e] = node(1 + 4 * k + e);
             sum2 += v[0] + v[1];
             sum3 += v[2] + v[3];
         }
-        for (int p = 25; p < 85; p++) {
-            gm_node(p, c, hw, u);
-            sum4 += gm_integrand(m, xd, J, P, u);
-        }
-        for (int p = 85; p < GM_NPTS; p++) {
-            gm_node(p, c, hw, u);
-            sum5 += gm_integrand(m, xd, J, P, u);
-        }
+        for (int p = 25; p < 85; p++) sum4 += node(p);
+        for (int p = 85; p < GM_NPTS; p++) sum5 += node(p);
         double vol = 1;
         for (int d = 0; d < 6; d++) vol *= 2 * hw[d];
         const double result = vol * (w1 * val0 + w2 * sum2 + w3 * sum3 + w4 * sum4 + w5 * sum5);
@@ -510,6 +632,17 @@ __device__ void gm_thread_region(const XiModelDev& m, const XiDesc& xd, const GM
         c[d] = S.c[d * R + ri];
         hw[d] = S.hw[d * R + ri];
     }
+    const bool sph = P.coordsys != OPSEC_COORD_CARTESIAN;
+    double T[SPH_TAB];
+    if (sph) {
+#pragma unroll
+        for (int e = 0; e < SPH_TAB; e++) T[e] = sph_entry(J, c, hw, e);
+    }
+    auto node = [&](int p) {
+        if (sph) return sph_node_value(m, xd, T, p);
+        gm_node(p, c, hw, u);
+        return gm_integrand(m, xd, J, P, u);
+    };
     const double ratio = (GM_L2 * GM_L2) / (GM_L4 * GM_L4);
     const double w1 = (double)(12824 - 9120 * 6 + 400 * 36) / 19683.0;
     const double w2 = 980. / 6561.;
@@ -520,16 +653,12 @@ __device__ void gm_thread_region(const XiModelDev& m, const XiDesc& xd, const GM
     const double e2 = 245. / 486.;
     const double e3 = (double)(265 - 100 * 6) / 1458.0;
     const double e4 = 25. / 729.;
-    gm_node(0, c, hw, u);
-    const double val0 = gm_integrand(m, xd, J, P, u);
+    const double val0 = node(0);
     double sum2 = 0, sum3 = 0, sum4 = 0, sum5 = 0, maxdiff = 0;
     int split = 0;
     for (int k = 0; k < 6; k++) {
         double v[4];
-        for (int e = 0; e < 4; e++) {
-            gm_node(1 + 4 * k + e, c, hw, u);
-            v[e] = gm_integrand(m, xd, J, P, u);
-        }
+        for (int e = 0; e < 4; e++) v[e] = node(1 + 4 * k + e);
         sum2 += v[0] + v[1];
         sum3 += v[2] + v[3];
         const double diff = 0.0 + fabs(v[0] + v[1] - 2 * val0 - ratio * (v[2] + v[3] - 2 * val0));
@@ -538,14 +667,8 @@ __device__ void gm_thread_region(const XiModelDev& m, const XiDesc& xd, const GM
             split = k;
         }
     }
-    for (int p = 25; p < 85; p++) {
-        gm_node(p, c, hw, u);
-        sum4 += gm_integrand(m, xd, J, P, u);
-    }
-    for (int p = 85; p < GM_NPTS; p++) {
-        gm_node(p, c, hw, u);
-        sum5 += gm_integrand(m, xd, J, P, u);
-    }
+    for (int p = 25; p < 85; p++) sum4 += node(p);
+    for (int p = 85; p < GM_NPTS; p++) sum5 += node(p);
     double vol = 1;
     for (int d = 0; d < 6; d++) vol *= 2 * hw[d];
     const double result = vol * (w1 * val0 + w2 * sum2 + w3 * sum3 + w4 * sum4 + w5 * sum5);

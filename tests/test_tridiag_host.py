@@ -50,3 +50,33 @@ def test_split_and_degenerate_tridiagonal():
     T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
     np.testing.assert_allclose(Z @ Z.T, np.eye(nev), atol=1e-9)
     np.testing.assert_allclose(Z @ T @ Z.T, np.diag(ev), atol=1e-9)
+
+
+@pytest.mark.parametrize("nev", [1, 7, 30])
+def test_wilkinson_pairs(nev):
+    # W_101^+: the largest eigenvalues come in pairs equal to far below
+    # rounding, so nev = 7 splits a pair
+    m = 50
+    d = np.abs(np.arange(2 * m + 1) - m).astype(float)
+    e = np.ones(2 * m)
+    ev, Z = tridiag_top(d, e, nev)
+    w = sl.eigh_tridiagonal(d, e, eigvals_only=True)[::-1]
+    np.testing.assert_allclose(ev, w[:nev], rtol=1e-13, atol=1e-12)
+    T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    np.testing.assert_allclose(Z @ Z.T, np.eye(nev), atol=1e-9)
+    np.testing.assert_allclose(Z @ T @ Z.T, np.diag(ev), atol=1e-9)
+
+
+def test_tight_cluster_returns_every_pair():
+    # 60 eigenvalues within a few ulps of 1: the Sturm counts inside the
+    # cluster are not monotone, yet exactly nev orthonormal pairs come back
+    rng = np.random.default_rng(5)
+    n, nev = 60, 17
+    d = 1.0 + 1e-15 * rng.normal(size=n)
+    e = 1e-15 * (1.0 + rng.random(n - 1))
+    ev, Z = tridiag_top(d, e, nev)
+    assert ev.shape == (nev,) and np.all(np.diff(ev) <= 0)
+    np.testing.assert_allclose(ev, 1.0, atol=1e-13)
+    T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    np.testing.assert_allclose(Z @ Z.T, np.eye(nev), atol=1e-9)
+    assert np.abs(T @ Z.T - Z.T * ev).max() < 1e-13

@@ -38,9 +38,9 @@ PEAK_TF = {"fp64": 78.6, "fp32": 157.3, "bf16": 2516.6}
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB from two rocprofv3 --pmc passes, the factors being what
 # the same counters read for copy16_kernel on a 2 GiB copy (profiles/projection_c2_hbm_counters.txt).
 # Keys: (config, projection, kernel).
-TRAFFIC_BYTES = {("C2", "dense", "stage1"): (18956888 * 2.0 + 885416 * 1.0) * 1024,
-                 ("C2", "toeplitz", "stage1"): (1319407 * 2.0 + 1067077 * 1.0) * 1024,
-                 ("C2", "toeplitz", "stage2"): (4888851 * 2.0 + 1153599 * 1.0) * 1024}
+TRAFFIC_BYTES = {("C2", "dense", "stage1"): (18874466 * 2.0 + 883503 * 1.0) * 1024,
+                 ("C2", "toeplitz", "stage1"): (1321387 * 2.0 + 1059234 * 1.0) * 1024,
+                 ("C2", "toeplitz", "stage2"): (4895735 * 2.0 + 1169949 * 1.0) * 1024}
 
 
 def log(*a):

@@ -19,6 +19,10 @@
 //                   D as f32.
 #pragma once
 
+#include <cstdint>
+#include <type_traits>
+#include <utility>
+
 #include "common.h"
 
 namespace opsec {
@@ -43,11 +47,27 @@ __device__ __forceinline__ int acc_row<double>(int lane, int r) {
 // Each source exposes  T get(m_or_n, k)  with bounds handled by the caller,
 // and a flag telling which index is contiguous in memory.
 
+// Eight consecutive doubles at q by four 16-byte loads; false (nothing loaded)
+// when q is not 16-byte aligned.
+__device__ __forceinline__ bool load8_f64(const double* q, double (&r)[8]) {
+    if ((reinterpret_cast<uintptr_t>(q) & 15) != 0) return false;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const double2 v = reinterpret_cast<const double2*>(q)[e];
+        r[2 * e] = v.x;
+        r[2 * e + 1] = v.y;
+    }
+    return true;
+}
+
 struct SrcDenseRowMajor {      // X[i][k] = p[i*ld + k]   (k contiguous)
     const double* p;
     size_t ld;
     static constexpr bool k_contig = true;
     __device__ __forceinline__ double get(int i, int k) const { return p[(size_t)i * ld + k]; }
+    __device__ __forceinline__ bool load8(int i, int k, double (&r)[8]) const {
+        return load8_f64(p + (size_t)i * ld + k, r);
+    }
 };
 
 struct SrcDenseColMajor {      // X[i][k] = p[k*ld + i]   (i contiguous)
@@ -62,7 +82,17 @@ struct SrcKColMajor {          // X[i][k] = p[i*ld + k], i.e. a column-major K x
     size_t ld;
     static constexpr bool k_contig = true;
     __device__ __forceinline__ double get(int i, int k) const { return p[(size_t)i * ld + k]; }
+    __device__ __forceinline__ bool load8(int i, int k, double (&r)[8]) const {
+        return load8_f64(p + (size_t)i * ld + k, r);
+    }
 };
+
+// Whether a k-contiguous source can load eight consecutive k at once (load8).
+template <typename S, typename = void>
+struct HasLoad8 : std::false_type {};
+template <typename S>
+struct HasLoad8<S, std::void_t<decltype(std::declval<const S&>().load8(0, 0, std::declval<double (&)[8]>()))>>
+    : std::true_type {};
 
 // Signal matrix generated from the pair table: box key (SPH false) or the
 // arc-pair key of a spherical survey (SPH true, narcs = Nr Nmu).

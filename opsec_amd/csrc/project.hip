@@ -127,6 +127,9 @@ __device__ __forceinline__ void tile_fetch(const Src& s, int i0, int k0, int nI,
     const int t = threadIdx.x;
     if (Src::k_contig) {
         int i = i0 + (t >> 1), kb = k0 + (t & 1) * 8;
+        // whole, aligned runs of 8 by 16-byte loads
+        if constexpr (HasLoad8<Src>::value)
+            if (i < nI && kb + 8 <= nK && s.load8(i, kb, r)) return;
 #pragma unroll
         for (int e = 0; e < 8; e++) r[e] = (i < nI && kb + e < nK) ? s.get(i, kb + e) : 0.0;
     } else {
@@ -264,6 +267,23 @@ struct GB_ColMajorT {
     size_t ld;
     static constexpr bool k_contig = true;
     __device__ __forceinline__ double get(int n, int k) const { return (double)p[(size_t)n * ld + k]; }
+    __device__ __forceinline__ bool load8(int n, int k, double (&r)[8]) const {
+        const T* q = p + (size_t)n * ld + k;
+        if constexpr (std::is_same<T, double>::value) {
+            return load8_f64(q, r);
+        } else {
+            if ((reinterpret_cast<uintptr_t>(q) & 15) != 0) return false;
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const float4 v = reinterpret_cast<const float4*>(q)[e];
+                r[4 * e] = v.x;
+                r[4 * e + 1] = v.y;
+                r[4 * e + 2] = v.z;
+                r[4 * e + 3] = v.w;
+            }
+            return true;
+        }
+    }
     __device__ __forceinline__ void p_shift(int z, size_t s) { p += (size_t)z * s; }
 };
 

@@ -473,8 +473,8 @@ def main():
                     help="also time a plain device copy of this many GiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU band sharding "
-              "is a design, not part of this build)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (sharding the bands over "
+              "several GPUs is designed but not implemented)", file=sys.stderr)
         sys.exit(2)
     if args.config == "C5":
         spherical_main(args)
@@ -533,7 +533,7 @@ def main():
         lam, B = ev, V
     ctx.modes_upload(B)
     prior = ctx.params()
-    # K2 assembly of the dense S_0 (what klt's matvecs read; the projection never
+    # Assembly of the dense S_0 (what klt's matvecs read; the projection never
     # stores S_p): 8 N^2 bytes written, timed once in the setup
     dS = ctx.dev_alloc(N * N * 8)
     ctx.sync()
@@ -647,7 +647,7 @@ def main():
                                      "stage1_tflops": 2.0 * (2 * n) * n ** 4 * M * nb / (float(kt[3]) / 1e3) / 1e12,
                                      "stage2_tflops": float(N) * M * M * nb / (float(kt[4]) / 1e3) / 1e12}
         out["fp64_toeplitz_path"].update(compare_estimates(results["fp64"]["est"], results["fp64_toeplitz"]["est"]))
-    # pair quadrature (K1): distinct integrals and node evaluations per second
+    # pair quadrature: distinct integrals and node evaluations per second
     # against the device's measured FP64 sincos rate (band nodes need two pairs)
     import ctypes as C
     sc = C.c_double()

@@ -123,7 +123,7 @@ int opsec_model_xi(opsec_ctx* ctx, int f, int n, const double* r, double* xi);
  * RealModel or an uploaded table model) and its node separations. */
 int opsec_model_table(opsec_ctx* ctx, int f, double* r768, double* xi768);
 
-/* ---- pair quadrature (K1) and assembly (K2) ----------------------------- */
+/* ---- pair quadrature and assembly ------------------------------------------ */
 enum {
     OPSEC_QUAD_GM = 0,        /* reference: adaptive Genz-Malik 7/5 on [0,1]^6
                                  with the reference's argument order
@@ -163,7 +163,7 @@ int opsec_pair_integral(opsec_ctx* ctx, const opsec_cell* c1, const opsec_cell* 
 /* Device-resident survey: upload cells + geometry once. */
 int opsec_cells_upload(opsec_ctx* ctx, const opsec_cell* cells, int ncells,
                        const opsec_survey* sv);
-/* K1: pair-integral tables Q_f[key] for functions funcs[0..nf-1] on the device
+/* Pair-integral tables Q_f[key] for functions funcs[0..nf-1] on the device
  * survey.  Box: key = (|dx| Ny + |dy|) Nz + |dz| (src/sig.cpp:109-112), every
  * method.  Spherical: the arc cache of src/sig.cpp:150-190 for the block of all
  * rows and columns, key = (h_a narcs + h_b) Nphi + |dphi_a - dphi_b| with arcs
@@ -176,7 +176,7 @@ int opsec_pair_table_get(opsec_ctx* ctx, int f, double* Q);
  * evaluations, surrogate-served pairs, spot checks, spot failures. */
 int opsec_pair_stats(opsec_ctx* ctx, int64_t* integrals, int64_t* nodes,
                      int64_t* surrogate, int64_t* spot, int64_t* spot_fail);
-/* K2: assemble S_f (Ncells x Ncells, row-major, leading dimension ld) into a
+/* Assemble S_f (Ncells x Ncells, row-major, leading dimension ld) into a
  * DEVICE buffer dS. */
 int opsec_assemble_device(opsec_ctx* ctx, int f, double* dS, int ld);
 
@@ -186,7 +186,7 @@ int opsec_dev_free(opsec_ctx* ctx, void* dptr);
 int opsec_h2d(opsec_ctx* ctx, void* dst, const void* src, size_t bytes);
 int opsec_d2h(opsec_ctx* ctx, void* dst, const void* src, size_t bytes);
 
-/* ---- KL eigenmodes (K5) -------------------------------------------------- */
+/* ---- KL eigenmodes --------------------------------------------------------- */
 /* Largest-eigenvalue modes of S_0 (the prior signal matrix on the device
  * survey) by block Lanczos over the device-resident S with full FP64
  * re-orthogonalisation.  evals: nev values in descending order; evecs: nev x
@@ -208,7 +208,7 @@ int opsec_klt(opsec_ctx* ctx, int nev, double tol, int block, int maxdim,
 int opsec_lanczos_dense(opsec_ctx* ctx, int n, const double* A, int nev, double tol,
                         int block, int maxdim, double* evals, double* evecs, int* nconv);
 
-/* ---- projection (K3) ----------------------------------------------------- */
+/* ---- projection ------------------------------------------------------------ */
 enum {
     OPSEC_PREC_FP64 = 0,      /* FP64 MFMA, FP64 accumulation                  */
     OPSEC_PREC_FP32 = 1,      /* FP32 operands and accumulation                */
@@ -305,7 +305,7 @@ int opsec_basis(opsec_ctx* ctx, const opsec_survey* sv, const opsec_selection* s
 int opsec_healpix_pixels(opsec_ctx* ctx, int64_t nside, int nest, int64_t n, const double* theta,
                          const double* phi, int64_t* pix);
 
-/* ---- Fisher matrix and band powers (K4) --------------------------------- */
+/* ---- Fisher matrix and band powers ----------------------------------------- */
 enum { OPSEC_MIX_INVERSE = 0, OPSEC_MIX_INVERSE_SQRT = 1, OPSEC_MIX_IDENTITY = 2 };
 
 typedef struct opsec_estimate {
@@ -346,7 +346,7 @@ int opsec_fisher_gram_noalloc(opsec_ctx* ctx, int nmodes, int nparams, const dou
                               const double* d_D, const double* d_y, const double* d_prior, int prec,
                               double* d_F, double* d_f, double* d_q);
 
-/* ---- learned pair-integral surrogate (K6) ------------------------------- */
+/* ---- learned pair-integral surrogate --------------------------------------- */
 typedef struct opsec_mlp_train_opts {
     int32_t nsamples;         /* exact training integrals (2e5)               */
     int32_t epochs;

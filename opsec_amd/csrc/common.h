@@ -187,7 +187,7 @@ void gauss_jacobi10(int n, double* t, double* w);
 int survey_from_abi(opsec_ctx* ctx, const opsec_survey* in, SurveyDev* out);
 
 // Pair-table builder shared by sig_block and pair_tables: fills d_Q rows for
-// the functions in funcs (K1).
+// the functions in funcs.
 int build_pair_tables(opsec_ctx* ctx, int nf, const int* funcs, const opsec_quad* q);
 
 // Bounds of the two cells of a pair integral (cell 1 then cell 2).

@@ -1,4 +1,4 @@
-// Context, device buffers and the xi model (K0: xi tables on the device).
+// Context, device buffers and the xi model (the xi tables on the device).
 //
 // The RealModel tables follow src/RealModel.cpp:98-127 and src/Model.cpp:84-123:
 //   xi_0^2(r) = (1/2pi^2) int_{kmin}^{kmax} k^2 P(k) j0(kr) e^{-(k/kdamp)^2} dk

@@ -1,4 +1,4 @@
-// K4: Fisher matrix, bias and quadratic estimates from the packed projected
+// Fisher matrix, bias and quadratic estimates from the packed projected
 // matrices, then the estimate post-processing (src/estimate-mpi.cpp:480-654).
 //
 // Over the packed upper triangle k = j(j+1)/2 + i (i <= j), with

@@ -1,4 +1,4 @@
-// K5: KL eigenmodes by block Lanczos (replaces ParpackSolver,
+// KL eigenmodes by block Lanczos (replaces ParpackSolver,
 // src/ParpackSolver.cpp:68-227, and the klt solve, src/klt.cpp:160-248).
 //
 // The reference runs PARPACK's implicitly restarted Lanczos (which = LA,

@@ -1,4 +1,4 @@
-// K6: learned pair-integral surrogate (no counterpart in the reference).
+// Learned pair-integral surrogate (no counterpart in the reference).
 //
 // Network: 8 -> 128 -> 128 -> 128 -> 128 -> 1, SiLU on hidden layers.
 // Weights: FP32 master copy (Adam state in FP32) and BF16 copies used by the

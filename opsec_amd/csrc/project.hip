@@ -1,4 +1,4 @@
-// K3: projection of the band matrices onto the KL modes, P_p = B^T C_{,p} B.
+// Projection of the band matrices onto the KL modes, P_p = B^T C_{,p} B.
 //
 // Replaces the comma band loop (src/comma-scalapack.cpp:228-266): per band
 // p, fill S_p, C = S_p B (pdgemm), P = B^T C (pdgemm), write the packed upper

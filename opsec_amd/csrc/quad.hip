@@ -1,4 +1,4 @@
-// K1 pair quadrature and K2 signal-matrix assembly.
+// Pair quadrature and signal-matrix assembly.
 //
 // Pair integrals Q_ab = mean of xi(|x - y|) over x in cell a, y in cell b are
 // cached by the reference's translation key (src/sig.cpp:94-123):
@@ -894,7 +894,7 @@ __global__ void fill_u64_kernel(unsigned long long* p, size_t n, unsigned long l
     if (i < n) p[i] = v;
 }
 
-// K2: s[i*ld + j] = sqrt(Nbar_a Nbar_b) Q[key(a,b)] (src/sig.cpp:120, 203).
+// Assembly: s[i*ld + j] = sqrt(Nbar_a Nbar_b) Q[key(a,b)] (src/sig.cpp:120, 203).
 // Tile of 8 rows x 512 columns per 256-thread block; column geometry in LDS.
 __global__ __launch_bounds__(256) void assemble_kernel(const int* cellidx, const double* nbar, const int* rows,
                                                        int nrows, const int* cols, int ncols, PairKey K,

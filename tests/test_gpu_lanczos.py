@@ -1,4 +1,4 @@
-"""GPU block Lanczos (K5): the reference's eigen known answer, numpy eigh on
+"""GPU block Lanczos: the reference's eigen known answer, numpy eigh on
 random matrices, and the KL modes of a periodic box (degenerate multiplets)."""
 import ctypes as C
 import math

@@ -1,4 +1,4 @@
-"""Learned pair-integral surrogate (K6): BF16-MFMA forward and backward
+"""Learned pair-integral surrogate: BF16-MFMA forward and backward
 kernels against the FP64 oracle MLP, training, and the ablation against the
 quadrature-only method."""
 import os

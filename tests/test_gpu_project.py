@@ -1,4 +1,4 @@
-"""GPU parity of the projection (K3) and the Fisher / estimate step (K4)
+"""GPU parity of the projection and the Fisher / estimate step
 against the FP64 oracle."""
 import ctypes as C
 import os

@@ -1,4 +1,4 @@
-"""GPU parity of the xi model (K0), pair quadrature (K1) and assembly (K2)
+"""GPU parity of the xi model, pair quadrature and assembly
 against the reference golden vectors and the FP64 oracle."""
 import ctypes as C
 import os
@@ -156,7 +156,7 @@ def test_gauss_sig_block_vs_oracle(c1):
 
 
 def test_pair_tables_and_assembly_vs_reference(c1):
-    """Device-resident K1 + K2 for the full 12^3 matrix vs the reference rows."""
+    """Device-resident pair tables and assembly for the full 12^3 matrix vs the reference rows."""
     cells = synth.box_cells(12)
     sv = lib.survey_box(12)
     c1.cells_upload(cells, sv)

@@ -168,9 +168,11 @@ def library_ceiling(shapes):
     """Plain BF16 GEMM rate of the vendor library (torch.matmul / torch.bmm on
     ROCm: hipBLASLt, FP32 accumulation) on the projection GEMM shapes: the MFMA
     rate this device sustains in practice, against which the split-BF16x3
-    kernels' raw rate (3 MFMA products per useful flop) is read.  shapes:
-    name -> (batch, m, n, k).  Runs in a child process: torch brings its own
-    HIP runtime, which does not share a process with the library's."""
+    kernels' raw rate (3 MFMA products per useful flop) is read; shapes named
+    *_fp64 run in FP64 (rocBLAS / hipBLASLt dgemm), the ceiling of the FP64
+    path.  shapes: name -> (batch, m, n, k).  Runs in a child process: torch
+    brings its own HIP runtime, which does not share a process with the
+    library's."""
     import subprocess
     r = subprocess.run([sys.executable, os.path.abspath(__file__), "--library-ceiling-child", json.dumps(shapes)],
                        capture_output=True, text=True, timeout=600)
@@ -184,8 +186,9 @@ def _library_ceiling_child(shapes, reps=10):
     dev = torch.device("cuda:0")
     out = {}
     for name, (bt, m, n, k) in shapes.items():
-        a = torch.randn((bt, m, k) if bt > 1 else (m, k), device=dev, dtype=torch.bfloat16)
-        b = torch.randn((bt, k, n) if bt > 1 else (k, n), device=dev, dtype=torch.bfloat16)
+        dt = torch.float64 if name.endswith("_fp64") else torch.bfloat16
+        a = torch.randn((bt, m, k) if bt > 1 else (m, k), device=dev, dtype=dt)
+        b = torch.randn((bt, k, n) if bt > 1 else (k, n), device=dev, dtype=dt)
         f = (lambda: torch.bmm(a, b)) if bt > 1 else (lambda: a @ b)
         for _ in range(3):
             c = f()
@@ -765,7 +768,8 @@ def main():
     if args.library_ceiling and main_path == "surrogate" and args.projection == "toeplitz":
         Mp = (M + 255) // 256 * 256
         lc = library_ceiling({"toeplitz_stage1": (2 * n, n * n, Mp, n * n), "toeplitz_stage2": (1, Mp, Mp, N),
-                              "dense_stage1": (1, N, Mp, N)})
+                              "dense_stage1": (1, N, Mp, N), "toeplitz_stage1_fp64": (2 * n, n * n, M, n * n),
+                              "stage2_fp64": (1, M, M, N), "dense_stage1_fp64": (1, N // 4, M, N)})
         rf = out["roofline"]
         if "error" not in lc:
             lc["kernel_raw_mfma_tflops"] = {"toeplitz_stage1": 3 * rf["stage1_tflops"],

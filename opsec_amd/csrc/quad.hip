@@ -1131,15 +1131,15 @@ static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::v
                                        (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                        nkeys, dQerr, (int*)sf.p, d_nev);
                 else if (nrun <= 2 * ctx->num_cus)   // few deep integrals: 8 waves each
-                    hipLaunchKernelGGL((gm_kernel<512, 2048>), dim3(std::min(nslots, nrun)), dim3(512), 0, ctx->stream, mv,
+                    hipLaunchKernelGGL((gm_kernel<512, 8192>), dim3(std::min(nslots, nrun)), dim3(512), 0, ctx->stream, mv,
                                        (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                        nkeys, dQerr, (int*)sf.p, d_nev);
                 else if (nrun <= 4 * ctx->num_cus)
                     hipLaunchKernelGGL((gm_kernel<256, 1024>), dim3(std::min(nslots, nrun)), dim3(256), 0, ctx->stream, mv,
                                        (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                        nkeys, dQerr, (int*)sf.p, d_nev);
-                else   // many: one wave each, all resident at once
-                    hipLaunchKernelGGL((gm_kernel<64, 0>), dim3(std::min(nslots, nrun)), dim3(64), 0, ctx->stream, mv,
+                else   // many: one wave each, all resident at once, the heap's top 10 levels in LDS
+                    hipLaunchKernelGGL((gm_kernel<64, 1024>), dim3(std::min(nslots, nrun)), dim3(64), 0, ctx->stream, mv,
                                        (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                        nkeys, dQerr, (int*)sf.p, d_nev);
                 OPSEC_HIP_TRY(ctx, hipGetLastError());

@@ -541,6 +541,18 @@ __global__ void expand_gm_jobs_kernel(const GMJob* __restrict__ tmpl, int npairs
     jobs[j] = J;
 }
 
+// The rerun's jobs: job j (template j % npairs, function funcs[j / npairs]) of
+// the sorted list failed[0..n).
+__global__ void gather_gm_jobs_kernel(const GMJob* __restrict__ tmpl, long long npairs, const int* __restrict__ funcs,
+                                      const int* __restrict__ failed, int n, GMJob* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long j = failed[i];
+    GMJob J = tmpl[j % npairs];
+    J.f = funcs[j / npairs];
+    out[i] = J;
+}
+
 // Indices of the jobs (idx[0..n) or 0..n) whose region pool ran out.
 __global__ void list_failed_kernel(const int* __restrict__ fail, const int* __restrict__ idx, int n,
                                    int* __restrict__ out, int* __restrict__ nout) {
@@ -1007,6 +1019,165 @@ PairGeo grid_pair_geo(const SurveyDev& sv, int kx, int ky, int kz) {
 
 // Run method q->method on the given key pairs for functions funcs; results to
 // dQ[f * nkeys + key].  keys[i] with geometry geo[i].
+// Genz-Malik integrals of the functions funcs[0..nf) over the npairs device
+// templates d_tmpl (f unset): the nf x npairs jobs, function-major (the prior,
+// with the most adaptive work, first), are expanded on the device.
+// Device buffers of run_gm_device, kept across the calls of one pair-table
+// sweep (its chunks) so the multi-GB job lists are allocated once.
+struct GMScratch {
+    Scratch sj, sd, si, sc, sf, sp, st, sx, sa;
+};
+
+static int run_gm_device(opsec_ctx* ctx, int nf, const int* funcs, const GMJob* d_tmpl, size_t npairs,
+                         const opsec_quad* q, double* dQ, size_t nkeys, double* dQerr, GMScratch* gs = nullptr) {
+    XiModelDev mv = model_view(ctx);
+    if (npairs == 0 || nf == 0) return OPSEC_OK;
+    GMScratch local;
+    GMScratch& S = gs ? *gs : local;
+    Scratch& sa = S.sa;
+    Scratch& sj = S.sj;
+    Scratch& sd = S.sd;
+    Scratch& si = S.si;
+    Scratch& sc = S.sc;
+    Scratch& sf = S.sf;
+    Scratch& sp = S.sp;
+    Scratch& st = S.st;
+    Scratch& sx = S.sx;
+    GMParams P;
+    P.absTol = q->epsrel;   // reference argument order, src/sig.cpp:427-430
+    P.relTol = q->epsabs;
+    P.maxEval = (unsigned)q->maxeval;
+    P.coordsys = ctx->sv.coordsys;
+    P.L = ctx->sv.L;
+    // Pass 0: every job's first region (one thread per job), then the jobs
+    // that need splits with a 4096-region pool per wavefront slot.  Pass 1:
+    // jobs that exhausted it, with pools that hold every region the
+    // evaluation cap allows (maxeval / 149 + 2), on fewer slots.
+    const size_t njobs_all = npairs * (size_t)nf;
+    if (njobs_all > (size_t)INT32_MAX) return set_error(ctx, OPSEC_EINVAL, "too many pair integrals in one call");
+    unsigned long long nev_total = 0;
+    int nagain = 0;             // pass-1 jobs, gathered on the device into sa
+    for (int pass = 0; pass < 2; pass++) {
+        const int njobs = pass == 0 ? (int)njobs_all : nagain;
+        if (njobs == 0) break;
+        long long cap_regions = P.maxEval ? (long long)P.maxEval / GM_NPTS + 2 : (1LL << 20);
+        P.rmax = pass == 0 ? 4096 : (int)std::min<long long>(cap_regions, 1LL << 22);
+        int nslots = pass == 0 ? std::min(njobs, ctx->num_cus * 8) : std::min(njobs, 64);
+        if (pass == 1) {
+            // as many wavefronts as half the free device memory holds pools for
+            size_t fre = 0, tot = 0;
+            if (hipMemGetInfo(&fre, &tot) != hipSuccess) fre = (size_t)16 << 30;
+            const size_t per_slot = (size_t)P.rmax * (15 * 8 + 3 * 4);
+            const size_t budget = std::max<size_t>((size_t)1 << 30, fre / 2);
+            nslots = std::min(njobs, ctx->num_cus * 8);
+            nslots = (int)std::max<size_t>(1, std::min<size_t>(nslots, budget / per_slot));
+        }
+        int rc;
+        if ((rc = dev_realloc(ctx, &sj.p, &sj.cap, sizeof(GMJob) * njobs))) return rc;
+        if ((rc = dev_realloc(ctx, &sd.p, &sd.cap, sizeof(double) * 15 * (size_t)P.rmax * nslots))) return rc;
+        if ((rc = dev_realloc(ctx, &si.p, &si.cap, sizeof(int) * 3 * (size_t)P.rmax * nslots))) return rc;
+        if ((rc = dev_realloc(ctx, &sc.p, &sc.cap, 64))) return rc;
+        if ((rc = dev_realloc(ctx, &sf.p, &sf.cap, sizeof(int) * njobs))) return rc;
+        if ((rc = dev_realloc(ctx, &sp.p, &sp.cap, sizeof(int) * njobs))) return rc;
+        if ((rc = dev_realloc(ctx, &sx.p, &sx.cap, sizeof(int) * njobs))) return rc;
+        OPSEC_HIP_TRY(ctx, hipMemsetAsync(sc.p, 0, 64, ctx->stream));
+        int* d_counter = (int*)sc.p;
+        unsigned long long* d_nev = (unsigned long long*)((char*)sc.p + 8);
+        int* d_npend = (int*)((char*)sc.p + 16);
+        int* d_nfail = (int*)((char*)sc.p + 20);
+        int* d_ndeep = (int*)((char*)sc.p + 24);
+        const int* d_idx = nullptr;
+        int nrun = njobs;
+        if (pass == 0) {
+            if ((rc = dev_realloc(ctx, &st.p, &st.cap, sizeof(int) * nf))) return rc;
+            int* dfun = (int*)st.p;
+            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dfun, funcs, sizeof(int) * nf, hipMemcpyHostToDevice, ctx->stream));
+            hipLaunchKernelGGL(expand_gm_jobs_kernel, dim3((unsigned)((njobs + 255) / 256)), dim3(256), 0,
+                               ctx->stream, d_tmpl, (int)npairs, dfun, nf, (GMJob*)sj.p);
+            OPSEC_HIP_TRY(ctx, hipGetLastError());
+            {
+                // first regions, then the shallow integrals one thread each
+                hipLaunchKernelGGL(gm_first_kernel, dim3((unsigned)((njobs + 255) / 256)), dim3(256), 0,
+                                   ctx->stream, mv, (const GMJob*)sj.p, njobs, P, dQ, nkeys, dQerr, (int*)sf.p,
+                                   (int*)sx.p, d_npend, d_nev);
+                OPSEC_HIP_TRY(ctx, hipGetLastError());
+                int npend = 0;
+                OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&npend, d_npend, 4, hipMemcpyDeviceToHost, ctx->stream));
+                OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+                nrun = 0;
+                if (npend > 0) {
+                    hipLaunchKernelGGL(gm_thread_kernel, dim3((unsigned)((npend + 255) / 256)), dim3(256), 0,
+                                       ctx->stream, mv, (const GMJob*)sj.p, (const int*)sx.p, npend, P, dQ, nkeys,
+                                       dQerr, (int*)sf.p, (int*)sp.p, d_ndeep, d_nev);
+                    OPSEC_HIP_TRY(ctx, hipGetLastError());
+                    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nrun, d_ndeep, 4, hipMemcpyDeviceToHost, ctx->stream));
+                    OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+                }
+                d_idx = (const int*)sp.p;
+            }
+        } else {
+            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(sj.p, sa.p, sizeof(GMJob) * njobs, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        if (nrun > 0) {
+            if (pass == 0)
+                hipLaunchKernelGGL((gm_kernel<64, 0>), dim3(std::min(nslots, nrun)), dim3(64), 0, ctx->stream, mv,
+                                   (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
+                                   nkeys, dQerr, (int*)sf.p, d_nev);
+            else if (nrun <= 2 * ctx->num_cus)   // few deep integrals: 8 waves each
+                hipLaunchKernelGGL((gm_kernel<512, 8192>), dim3(std::min(nslots, nrun)), dim3(512), 0, ctx->stream, mv,
+                                   (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
+                                   nkeys, dQerr, (int*)sf.p, d_nev);
+            else if (nrun <= 4 * ctx->num_cus)
+                hipLaunchKernelGGL((gm_kernel<256, 1024>), dim3(std::min(nslots, nrun)), dim3(256), 0, ctx->stream, mv,
+                                   (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
+                                   nkeys, dQerr, (int*)sf.p, d_nev);
+            else   // many: one wave each, all resident at once, the heap's top 10 levels in LDS
+                hipLaunchKernelGGL((gm_kernel<64, 1024>), dim3(std::min(nslots, nrun)), dim3(64), 0, ctx->stream, mv,
+                                   (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
+                                   nkeys, dQerr, (int*)sf.p, d_nev);
+            OPSEC_HIP_TRY(ctx, hipGetLastError());
+        }
+        // jobs whose region pool ran out, listed on the device
+        int nfail = 0;
+        if (nrun > 0) {
+            hipLaunchKernelGGL(list_failed_kernel, dim3((unsigned)((nrun + 255) / 256)), dim3(256), 0, ctx->stream,
+                               (const int*)sf.p, d_idx, nrun, (int*)sx.p, d_nfail);
+            OPSEC_HIP_TRY(ctx, hipGetLastError());
+        }
+        unsigned long long nev = 0;
+        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nev, d_nev, 8, hipMemcpyDeviceToHost, ctx->stream));
+        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nfail, d_nfail, 4, hipMemcpyDeviceToHost, ctx->stream));
+        OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        nev_total += nev;
+        if (nfail > 0 && pass == 1)
+            return set_error(ctx, OPSEC_ECONV, "adaptive cubature exceeded %d regions", P.rmax);
+        std::vector<int> failed(nfail);
+        if (nfail > 0) {
+            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(failed.data(), sx.p, sizeof(int) * nfail, hipMemcpyDeviceToHost,
+                                              ctx->stream));
+            OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        }
+        std::sort(failed.begin(), failed.end());
+        nagain = nfail;
+        if (nfail > 0) {
+            // the rerun's jobs: template j % npairs with function funcs[j / npairs]
+            if ((rc = dev_realloc(ctx, &sa.p, &sa.cap, sizeof(GMJob) * nfail + sizeof(int) * (nfail + nf)))) return rc;
+            int* d_failed = (int*)((GMJob*)sa.p + nfail);
+            int* d_fun = d_failed + nfail;
+            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_failed, failed.data(), sizeof(int) * nfail, hipMemcpyHostToDevice,
+                                              ctx->stream));
+            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_fun, funcs, sizeof(int) * nf, hipMemcpyHostToDevice, ctx->stream));
+            hipLaunchKernelGGL(gather_gm_jobs_kernel, dim3((unsigned)((nfail + 255) / 256)), dim3(256), 0, ctx->stream,
+                               d_tmpl, (long long)npairs, d_fun, d_failed, nfail, (GMJob*)sa.p);
+            OPSEC_HIP_TRY(ctx, hipGetLastError());
+            OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        }
+    }
+    ctx->st_integrals += (int64_t)njobs_all;
+    ctx->st_nodes += (int64_t)nev_total;
+    return OPSEC_OK;
+}
+
 static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::vector<int>& keys,
                           const std::vector<PairGeo>& geo, const std::vector<int>& nearflag, const opsec_quad* q,
                           double* dQ, size_t nkeys, double* dQerr = nullptr) {
@@ -1046,135 +1217,12 @@ static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::v
                 J.hi[5] = g.hi2[2];
             }
         }
-        GMParams P;
-        P.absTol = q->epsrel;   // reference argument order, src/sig.cpp:427-430
-        P.relTol = q->epsabs;
-        P.maxEval = (unsigned)q->maxeval;
-        P.coordsys = ctx->sv.coordsys;
-        P.L = ctx->sv.L;
-        // Pass 0: every job's first region (one thread per job), then the jobs
-        // that need splits with a 4096-region pool per wavefront slot.  Pass 1:
-        // jobs that exhausted it, with pools that hold every region the
-        // evaluation cap allows (maxeval / 149 + 2), on fewer slots.
-        const size_t njobs_all = npairs * (size_t)nf;
-        if (njobs_all > (size_t)INT32_MAX) return set_error(ctx, OPSEC_EINVAL, "too many pair integrals in one call");
-        unsigned long long nev_total = 0;
-        std::vector<GMJob> again;   // pass-1 jobs
-        for (int pass = 0; pass < 2; pass++) {
-            const int njobs = pass == 0 ? (int)njobs_all : (int)again.size();
-            if (njobs == 0) break;
-            long long cap_regions = P.maxEval ? (long long)P.maxEval / GM_NPTS + 2 : (1LL << 20);
-            P.rmax = pass == 0 ? 4096 : (int)std::min<long long>(cap_regions, 1LL << 22);
-            int nslots = pass == 0 ? std::min(njobs, ctx->num_cus * 8) : std::min(njobs, 64);
-            if (pass == 1) {
-                // as many wavefronts as half the free device memory holds pools for
-                size_t fre = 0, tot = 0;
-                if (hipMemGetInfo(&fre, &tot) != hipSuccess) fre = (size_t)16 << 30;
-                const size_t per_slot = (size_t)P.rmax * (15 * 8 + 3 * 4);
-                const size_t budget = std::max<size_t>((size_t)1 << 30, fre / 2);
-                nslots = std::min(njobs, ctx->num_cus * 8);
-                nslots = (int)std::max<size_t>(1, std::min<size_t>(nslots, budget / per_slot));
-            }
-            Scratch sj, sd, si, sc, sf, sp, st, sx;
-            int rc;
-            if ((rc = dev_realloc(ctx, &sj.p, &sj.cap, sizeof(GMJob) * njobs))) return rc;
-            if ((rc = dev_realloc(ctx, &sd.p, &sd.cap, sizeof(double) * 15 * (size_t)P.rmax * nslots))) return rc;
-            if ((rc = dev_realloc(ctx, &si.p, &si.cap, sizeof(int) * 3 * (size_t)P.rmax * nslots))) return rc;
-            if ((rc = dev_realloc(ctx, &sc.p, &sc.cap, 64))) return rc;
-            if ((rc = dev_realloc(ctx, &sf.p, &sf.cap, sizeof(int) * njobs))) return rc;
-            if ((rc = dev_realloc(ctx, &sp.p, &sp.cap, sizeof(int) * njobs))) return rc;
-            if ((rc = dev_realloc(ctx, &sx.p, &sx.cap, sizeof(int) * njobs))) return rc;
-            OPSEC_HIP_TRY(ctx, hipMemsetAsync(sc.p, 0, 64, ctx->stream));
-            int* d_counter = (int*)sc.p;
-            unsigned long long* d_nev = (unsigned long long*)((char*)sc.p + 8);
-            int* d_npend = (int*)((char*)sc.p + 16);
-            int* d_nfail = (int*)((char*)sc.p + 20);
-            int* d_ndeep = (int*)((char*)sc.p + 24);
-            const int* d_idx = nullptr;
-            int nrun = njobs;
-            if (pass == 0) {
-                if ((rc = dev_realloc(ctx, &st.p, &st.cap, sizeof(GMJob) * npairs + sizeof(int) * nf))) return rc;
-                int* dfun = (int*)((GMJob*)st.p + npairs);
-                OPSEC_HIP_TRY(ctx, hipMemcpyAsync(st.p, tmpl.data(), sizeof(GMJob) * npairs, hipMemcpyHostToDevice,
-                                                  ctx->stream));
-                OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dfun, funcs, sizeof(int) * nf, hipMemcpyHostToDevice, ctx->stream));
-                hipLaunchKernelGGL(expand_gm_jobs_kernel, dim3((unsigned)((njobs + 255) / 256)), dim3(256), 0,
-                                   ctx->stream, (const GMJob*)st.p, (int)npairs, dfun, nf, (GMJob*)sj.p);
-                OPSEC_HIP_TRY(ctx, hipGetLastError());
-                {
-                    // first regions, then the shallow integrals one thread each
-                    hipLaunchKernelGGL(gm_first_kernel, dim3((unsigned)((njobs + 255) / 256)), dim3(256), 0,
-                                       ctx->stream, mv, (const GMJob*)sj.p, njobs, P, dQ, nkeys, dQerr, (int*)sf.p,
-                                       (int*)sx.p, d_npend, d_nev);
-                    OPSEC_HIP_TRY(ctx, hipGetLastError());
-                    int npend = 0;
-                    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&npend, d_npend, 4, hipMemcpyDeviceToHost, ctx->stream));
-                    OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-                    nrun = 0;
-                    if (npend > 0) {
-                        hipLaunchKernelGGL(gm_thread_kernel, dim3((unsigned)((npend + 255) / 256)), dim3(256), 0,
-                                           ctx->stream, mv, (const GMJob*)sj.p, (const int*)sx.p, npend, P, dQ, nkeys,
-                                           dQerr, (int*)sf.p, (int*)sp.p, d_ndeep, d_nev);
-                        OPSEC_HIP_TRY(ctx, hipGetLastError());
-                        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nrun, d_ndeep, 4, hipMemcpyDeviceToHost, ctx->stream));
-                        OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-                    }
-                    d_idx = (const int*)sp.p;
-                }
-            } else {
-                OPSEC_HIP_TRY(ctx, hipMemcpyAsync(sj.p, again.data(), sizeof(GMJob) * njobs, hipMemcpyHostToDevice,
-                                                  ctx->stream));
-            }
-            if (nrun > 0) {
-                if (pass == 0)
-                    hipLaunchKernelGGL((gm_kernel<64, 0>), dim3(std::min(nslots, nrun)), dim3(64), 0, ctx->stream, mv,
-                                       (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
-                                       nkeys, dQerr, (int*)sf.p, d_nev);
-                else if (nrun <= 2 * ctx->num_cus)   // few deep integrals: 8 waves each
-                    hipLaunchKernelGGL((gm_kernel<512, 8192>), dim3(std::min(nslots, nrun)), dim3(512), 0, ctx->stream, mv,
-                                       (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
-                                       nkeys, dQerr, (int*)sf.p, d_nev);
-                else if (nrun <= 4 * ctx->num_cus)
-                    hipLaunchKernelGGL((gm_kernel<256, 1024>), dim3(std::min(nslots, nrun)), dim3(256), 0, ctx->stream, mv,
-                                       (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
-                                       nkeys, dQerr, (int*)sf.p, d_nev);
-                else   // many: one wave each, all resident at once, the heap's top 10 levels in LDS
-                    hipLaunchKernelGGL((gm_kernel<64, 1024>), dim3(std::min(nslots, nrun)), dim3(64), 0, ctx->stream, mv,
-                                       (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
-                                       nkeys, dQerr, (int*)sf.p, d_nev);
-                OPSEC_HIP_TRY(ctx, hipGetLastError());
-            }
-            // jobs whose region pool ran out, listed on the device
-            int nfail = 0;
-            if (nrun > 0) {
-                hipLaunchKernelGGL(list_failed_kernel, dim3((unsigned)((nrun + 255) / 256)), dim3(256), 0, ctx->stream,
-                                   (const int*)sf.p, d_idx, nrun, (int*)sx.p, d_nfail);
-                OPSEC_HIP_TRY(ctx, hipGetLastError());
-            }
-            unsigned long long nev = 0;
-            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nev, d_nev, 8, hipMemcpyDeviceToHost, ctx->stream));
-            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nfail, d_nfail, 4, hipMemcpyDeviceToHost, ctx->stream));
-            OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-            nev_total += nev;
-            if (nfail > 0 && pass == 1)
-                return set_error(ctx, OPSEC_ECONV, "adaptive cubature exceeded %d regions", P.rmax);
-            std::vector<int> failed(nfail);
-            if (nfail > 0) {
-                OPSEC_HIP_TRY(ctx, hipMemcpyAsync(failed.data(), sx.p, sizeof(int) * nfail, hipMemcpyDeviceToHost,
-                                                  ctx->stream));
-                OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-            }
-            std::sort(failed.begin(), failed.end());
-            again.clear();
-            for (int j : failed) {
-                GMJob J = tmpl[(size_t)j % npairs];
-                J.f = funcs[(size_t)j / npairs];
-                again.push_back(J);
-            }
-        }
-        ctx->st_integrals += (int64_t)njobs_all;
-        ctx->st_nodes += (int64_t)nev_total;
-        return OPSEC_OK;
+        Scratch st;
+        int rc;
+        if ((rc = dev_realloc(ctx, &st.p, &st.cap, sizeof(GMJob) * npairs))) return rc;
+        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(st.p, tmpl.data(), sizeof(GMJob) * npairs, hipMemcpyHostToDevice,
+                                          ctx->stream));
+        return run_gm_device(ctx, nf, funcs, (const GMJob*)st.p, npairs, q, dQ, nkeys, dQerr);
     }
     if (method == OPSEC_QUAD_GAUSS) {
         OPSEC_CHECK_ARG(ctx, ctx->sv.coordsys == OPSEC_COORD_CARTESIAN, "GAUSS quadrature needs a box survey");
@@ -1277,6 +1325,58 @@ static int near_of(const opsec_cell& a, const opsec_cell& b, int near) {
     return m < near ? 1 : 0;
 }
 
+// First cell pairs of the arc pairs (ha, hb), ha in [ha0, ha0 + nha), hb in
+// [0, narcs): for each k = |dphi_a - dphi_b| the first pair met in cell order
+// (row cell of arc ha, then column cell of arc hb), as a job template of the
+// pair with the lower cell index first.  Arc members are listed in cell order
+// (arc_start / arc_mem), cb holds each cell's {min, max} of r, mu and phi.
+// The templates are appended in no particular order (each integral is
+// independent; count receives their number).  W 32-bit words hold the k seen.
+template <int W>
+__global__ __launch_bounds__(256) void sph_templates_kernel(const int* __restrict__ arc_start,
+                                                            const int* __restrict__ arc_mem,
+                                                            const int* __restrict__ cd3, const double* __restrict__ cb,
+                                                            int narcs, int Nphi, int ha0, int nha,
+                                                            GMJob* __restrict__ out,
+                                                            unsigned long long* __restrict__ count) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nha * narcs) return;
+    const int ha = ha0 + (int)(t / narcs), hb = (int)(t % narcs);
+    const int a0 = arc_start[ha], a1 = arc_start[ha + 1], b0 = arc_start[hb], b1 = arc_start[hb + 1];
+    unsigned seen[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) seen[w] = 0u;
+    int left = Nphi;
+    for (int ia = a0; ia < a1 && left > 0; ia++) {
+        const int a = arc_mem[ia], da = cd3[a];
+        for (int ib = b0; ib < b1; ib++) {
+            const int b = arc_mem[ib];
+            const int k = abs(da - cd3[b]);
+            if ((seen[k >> 5] >> (k & 31)) & 1u) continue;
+            seen[k >> 5] |= 1u << (k & 31);
+            const double* c1 = cb + 6 * (size_t)min(a, b);
+            const double* c2 = cb + 6 * (size_t)max(a, b);
+            GMJob J;
+            J.f = -1;
+            J.key = (ha * narcs + hb) * Nphi + k;
+            J.lo[0] = c1[0] * c1[0] * c1[0];
+            J.hi[0] = c1[1] * c1[1] * c1[1];
+            J.lo[1] = c2[0] * c2[0] * c2[0];
+            J.hi[1] = c2[1] * c2[1] * c2[1];
+            J.lo[2] = c1[2];
+            J.hi[2] = c1[3];
+            J.lo[3] = c2[2];
+            J.hi[3] = c2[3];
+            J.lo[4] = c1[4];
+            J.hi[4] = c1[5];
+            J.lo[5] = c2[4];
+            J.hi[5] = c2[5];
+            out[atomicAdd(count, 1ull)] = J;
+            if (--left == 0) break;
+        }
+    }
+}
+
 // Arc-pair tables of a spherical survey (src/sig.cpp:125-213) for the block of
 // all rows and all columns: for every ordered arc pair (h_a, h_b) and each
 // k = |dphi_a - dphi_b| the integral of the first cell pair met in cell order
@@ -1303,6 +1403,65 @@ static int build_pair_tables_sph(opsec_ctx* ctx, int nf, const int* funcs, const
     for (int t = 0; t < nf; t++) {
         const XiDesc& d = ctx->funcs[funcs[t]];
         (d.b > d.a && d.b > 0 ? shallow : deep).push_back(funcs[t]);
+    }
+    if (Nphi <= 1024) {
+        // templates generated on the device, a range of row arcs at a time
+        std::vector<int> arc_start(narcs + 1, 0), arc_mem, cd3(N);
+        std::vector<double> cb(6 * (size_t)N);
+        for (int h = 0; h < narcs; h++) {
+            arc_start[h + 1] = arc_start[h] + (int)arcs[h].size();
+            arc_mem.insert(arc_mem.end(), arcs[h].begin(), arcs[h].end());
+        }
+        for (int a = 0; a < N; a++) {
+            const opsec_cell& c = cells[a];
+            cd3[a] = c.d3;
+            const double v[6] = {c.min1, c.max1, c.min2, c.max2, c.min3, c.max3};
+            std::copy(v, v + 6, cb.begin() + 6 * (size_t)a);
+        }
+        Scratch sg, st;
+        int rc;
+        const size_t geo_bytes = 4 * (size_t)(narcs + 1) + 8 * (size_t)N + 48 * (size_t)N + 64;
+        if ((rc = dev_realloc(ctx, &sg.p, &sg.cap, geo_bytes))) return rc;
+        int* d_start = (int*)sg.p;
+        int* d_mem = d_start + (narcs + 1);
+        int* d_d3 = d_mem + N;
+        double* d_cb = (double*)(((uintptr_t)(d_d3 + N) + 15) & ~(uintptr_t)15);
+        unsigned long long* d_count = (unsigned long long*)(d_cb + 6 * (size_t)N);
+        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_start, arc_start.data(), 4 * (size_t)(narcs + 1), hipMemcpyHostToDevice,
+                                          ctx->stream));
+        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_mem, arc_mem.data(), 4 * (size_t)N, hipMemcpyHostToDevice, ctx->stream));
+        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_d3, cd3.data(), 4 * (size_t)N, hipMemcpyHostToDevice, ctx->stream));
+        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_cb, cb.data(), 48 * (size_t)N, hipMemcpyHostToDevice, ctx->stream));
+        const size_t per_row = (size_t)narcs * Nphi;   // keys of one row arc at most
+        for (const std::vector<int>* group : {&deep, &shallow}) {
+            if (group->empty()) continue;
+            GMScratch gs;   // one sweep's job buffers, freed before the next sweep
+            const int ng = (int)group->size();
+            const size_t chunk = std::max<size_t>(4096, ((size_t)1 << 28) / (size_t)ng);
+            const int nha = (int)std::max<size_t>(1, std::min<size_t>(narcs, chunk / per_row));
+            if ((rc = dev_realloc(ctx, &st.p, &st.cap, sizeof(GMJob) * (size_t)nha * per_row))) return rc;
+            for (int ha0 = 0; ha0 < narcs; ha0 += nha) {
+                const int nh = std::min(nha, narcs - ha0);
+                OPSEC_HIP_TRY(ctx, hipMemsetAsync(d_count, 0, 8, ctx->stream));
+                const long long nt = (long long)nh * narcs;
+                if (Nphi <= 64)
+                    hipLaunchKernelGGL(sph_templates_kernel<2>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0,
+                                       ctx->stream, d_start, d_mem, d_d3, d_cb, narcs, Nphi, ha0, nh, (GMJob*)st.p,
+                                       d_count);
+                else
+                    hipLaunchKernelGGL(sph_templates_kernel<32>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0,
+                                       ctx->stream, d_start, d_mem, d_d3, d_cb, narcs, Nphi, ha0, nh, (GMJob*)st.p,
+                                       d_count);
+                OPSEC_HIP_TRY(ctx, hipGetLastError());
+                unsigned long long npairs = 0;
+                OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&npairs, d_count, 8, hipMemcpyDeviceToHost, ctx->stream));
+                OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+                if ((rc = run_gm_device(ctx, ng, group->data(), (const GMJob*)st.p, (size_t)npairs, q, ctx->d_Q,
+                                        (size_t)ctx->nkeys, nullptr, &gs)))
+                    return rc;
+            }
+        }
+        return OPSEC_OK;
     }
     std::vector<int> keys, nearflag, tab(Nphi);
     std::vector<PairGeo> geo;

@@ -348,8 +348,10 @@ def spherical_main(args):
     as a radial table, 50 log-spaced bands over [0.003, 0.3] h/Mpc, 6000 / scale^3 KL modes.
     Setup (untimed): basis cells on the GPU, the prior's arc-pair tables and klt.  One step:
     arc-pair tables of the prior and all bands (adaptive Genz-Malik pairs; spherical pairs
-    have no surrogate), the projection and the estimate -- FP64, and split-BF16x3 + FP32
-    traces."""
+    have no surrogate), the projection and the estimate -- FP64 with the dense projection
+    (the reference's algorithm), and split-BF16x3 + FP32 traces with the --projection method
+    (default: Toeplitz, a length-2 Nphi DFT along phi); with Toeplitz also the FP64 path
+    with the same DFT."""
     import ctypes as Cc
     from opsec_amd import lib as L
     from opsec_amd import synth
@@ -395,6 +397,11 @@ def spherical_main(args):
     r64 = run_path(ctx, L, funcs, q, L.OPSEC_PREC_FP64, L.OPSEC_PREC_FP64, lam, y, prior, args.steps, args.warmup,
                    "spherical fp64")
     st64 = ctx.pair_stats()
+    toep = args.projection == "toeplitz"
+    if toep:
+        ctx.set_projection(L.OPSEC_PROJ_TOEPLITZ)
+        r64t = run_path(ctx, L, funcs, q, L.OPSEC_PREC_FP64, L.OPSEC_PREC_FP64, lam, y, prior, args.steps,
+                        args.warmup, "spherical fp64, Toeplitz projection")
     rmx = run_path(ctx, L, funcs, q, L.OPSEC_PREC_BF16X3, L.OPSEC_PREC_FP32, lam, y, prior, args.steps, args.warmup,
                    "spherical mixed")
     pairs = float(N) * N * (nb + 1)
@@ -406,10 +413,11 @@ def spherical_main(args):
         "config": {"workload": (f"C5/{s}: spherical shell r 200-1000 Mpc/h, mu >= 0.5 (60 deg cap), {nr} x {nmu} x "
                                 f"{nphi} = {N} cells, nbar(r) = 5e-4 exp(-(r/600)^2), {nb} log bands "
                                 f"[0.003, 0.3] h/Mpc, {M} KL modes, adaptive Genz-Malik pairs"),
-                   "path": "mixed", "projection": "dense"},
-        "fp64_path": {"ms_per_step": r64[0], "value": pairs / (r64[0] / 1e3),
+                   "path": "mixed", "projection": args.projection},
+        "fp64_path": {"ms_per_step": r64[0], "value": pairs / (r64[0] / 1e3), "projection": "dense",
                       "projection_ms": [float(r64[2][0]), float(r64[2][1])]},
         "speedup_vs_fp64": r64[0] / rmx[0],
+        "projection_ms": [float(rmx[2][0]), float(rmx[2][1])],
         "quadrature": {"ms_per_step": qm, "integrals": int(st64["integrals"]), "node_evaluations": int(st64["nodes"]),
                        "integrals_per_s": st64["integrals"] / (qm / 1e3),
                        "node_evaluations_per_s": st64["nodes"] / (qm / 1e3)},
@@ -418,14 +426,39 @@ def spherical_main(args):
                 "krylov_dim": int(ph[6]), "rayleigh_ritz_steps": int(ph[7]), "block_steps": int(ph[8])},
     }
     out.update(compare_estimates(r64[1], rmx[1]))
+    if toep:
+        out["fp64_toeplitz_path"] = {"ms_per_step": r64t[0], "value": pairs / (r64t[0] / 1e3),
+                                     "projection_ms": [float(r64t[2][0]), float(r64t[2][1])]}
+        out["fp64_toeplitz_path"].update(compare_estimates(r64[1], r64t[1]))
+        out["speedup_vs_fp64_same_algorithm"] = r64t[0] / rmx[0]
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline_sph(cells, (nr, nmu, nphi), k, P, lo, hi, B)
         log(f"cpu baseline sample took {time.perf_counter() - t0:.1f} s")
-    s1 = float(rmx[2][3]) / max(1.0, float(rmx[2][5]))
-    out["roofline"] = {"bound": "mfma", "achieved": 2.0 * N * N * M / (s1 / 1e3) / 1e12, "peak": PEAK_TF["bf16"] / 3,
-                       "unit": "TFLOP/s", "traffic": None, "launch_ms": s1,
-                       "kernel": "gemm3_kernel<ASrcPlanes,EpiPlanes> (stage 1, S_p planes, one launch per band)"}
+    kg_ms, k1_ms, k2_ms = (float(x) for x in rmx[2][2:5])
+    l1, l2 = max(1, int(round(float(rmx[2][7])))), max(1, int(round(float(rmx[2][8]))))
+    if toep:
+        # stage 1 = 2 Nphi real GEMMs of G x G by G x M per band (G = Nr Nmu arcs), one launch
+        # per band; stage 2 = N M^2 per band, one launch per group of bands
+        G = nr * nmu
+        s1_flops, s2_flops = 2.0 * (2 * nphi) * G * G * M * nb, float(N) * M * M * nb
+        dom = "stage1" if k1_ms >= k2_ms else "stage2"
+        out["roofline"] = {
+            "bound": "mfma", "peak": PEAK_TF["bf16"] / 3, "unit": "TFLOP/s", "traffic": None,
+            "achieved": (s1_flops / k1_ms if dom == "stage1" else s2_flops / k2_ms) * 1e3 / 1e12,
+            "launch_ms": k1_ms / l1 if dom == "stage1" else k2_ms / l2,
+            "kernel": ("gemm3_kernel<ASrcPlanes,EpiYhat> (Toeplitz stage 1, 2 Nphi arc GEMMs per launch, one "
+                       "launch per band)" if dom == "stage1" else
+                       f"gemm3_kernel<ASrcPlanes,EpiPackedParts> (stage 2, split-K, {nb // l2} bands per launch)"),
+            "note": ("useful flops per band: stage 1 2 (2 Nphi) G^2 M, stage 2 N M^2; "
+                     "peak = dense bf16 2516.6 TF / 3 products"),
+            "per_step_ms": {"that_inverse_dft_and_split_k_sum": kg_ms, "stage1_gemm": k1_ms, "stage2_gemm": k2_ms},
+            "stage1_tflops": s1_flops / k1_ms * 1e3 / 1e12, "stage2_tflops": s2_flops / k2_ms * 1e3 / 1e12}
+    else:
+        s1 = k1_ms / l1
+        out["roofline"] = {"bound": "mfma", "achieved": 2.0 * N * N * M / (s1 / 1e3) / 1e12,
+                           "peak": PEAK_TF["bf16"] / 3, "unit": "TFLOP/s", "traffic": None, "launch_ms": s1,
+                           "kernel": "gemm3_kernel<ASrcPlanes,EpiPlanes> (stage 1, S_p planes, one launch per band)"}
     out["roofline"]["frac"] = out["roofline"]["achieved"] / out["roofline"]["peak"]
     ctx.close()
     print(json.dumps(out))
@@ -461,8 +494,9 @@ def main():
     ap.add_argument("--nt-gen", type=int, default=0, choices=[0, 1],
                     help="stage 1 generates S_p from the pair table (1) or reads S_p planes (0)")
     ap.add_argument("--projection", default="toeplitz", choices=["dense", "toeplitz"],
-                    help="split-BF16x3 projection: dense S_p (2 N^2 M stage 1) or Toeplitz (DFT along x, "
-                         "2 Nx slab GEMMs); the FP64 path is always dense")
+                    help="split-BF16x3 projection: dense S_p (2 N^2 M stage 1) or Toeplitz (DFT along x of the "
+                         "box, 2 Nx slab GEMMs; along phi of the shell, 2 Nphi arc GEMMs); the FP64 path is "
+                         "dense (the reference's algorithm), timed with Toeplitz too as a comparison")
     ap.add_argument("--s2-split", type=int, default=8, help="split-K factor of the split-BF16x3 stage 2")
     ap.add_argument("--nt-group", type=int, default=8, help="split-BF16x3 GEMM tile-group edge (stage 1)")
     ap.add_argument("--s2-group", type=int, default=2, help="split-BF16x3 stage-2 tile-group edge")

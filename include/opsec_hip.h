@@ -227,12 +227,15 @@ int opsec_project(opsec_ctx* ctx, int nf, const int* funcs, int prec, double* P_
 /* Projection method (no counterpart in the reference, whose comma always
  * forms the dense product, src/comma-scalapack.cpp:228-266).
  * DENSE: stage 1 multiplies S_p (N x N, generated from the pair table) by B'.
- * TOEPLITZ (box surveys, Nx <= 64, one cell per grid position): S_p[a][b]
- * depends on |dx|, |dy|, |dz| only, so a length-2Nx DFT along x turns stage 1
- * into 2 Nx products of Ny Nz x Ny Nz slab matrices with the transformed
- * modes -- the same result up to rounding, Nx/2 times fewer flops.  Applies
- * to every precision: FP64 (FP64 DFTs and MFMA), FP32 (FP32 MFMA, FP64 DFTs)
- * and split-BF16x3. */
+ * TOEPLITZ (one cell per grid position, at most 64 grid planes along the
+ * transformed axis): on a box S_p[a][b] depends on |dx|, |dy|, |dz| only, so a
+ * length-2Nx DFT along x turns stage 1 into 2 Nx products of Ny Nz x Ny Nz
+ * slab matrices with the transformed modes; on a spherical shell S_p[a][b]
+ * depends on the two cells' (r, mu) arcs and |dphi| only (the arc cache of
+ * src/sig.cpp:151-199), so a length-2Nphi DFT along phi gives 2 Nphi products
+ * of (Nr Nmu) x (Nr Nmu) arc matrices.  The same result up to rounding, Nx/2
+ * (Nphi/2) times fewer flops.  Applies to every precision: FP64 (FP64 DFTs
+ * and MFMA), FP32 (FP32 MFMA, FP64 DFTs) and split-BF16x3. */
 enum { OPSEC_PROJ_DENSE = 0, OPSEC_PROJ_TOEPLITZ = 1 };
 int opsec_set_projection(opsec_ctx* ctx, int method);
 /* Projection of an explicit host matrix S (N x N row-major) -- used to pin the

@@ -60,7 +60,7 @@ struct ProjState {
     float last_ms_gen = 0, last_ms_gemm1 = 0, last_ms_gemm2 = 0;
     int last_launches = 0;       // stage-1 launches (one per band)
     int last_launches2 = 0;      // stage-2 launches (one per band, or per group of bands)
-    // Toeplitz projection of box surveys (grid order, DFT along x)
+    // Toeplitz projection (grid order, DFT along x of a box or phi of a shell)
     bool toep_valid = false;     // grid planes of B' and its DFT match the uploaded modes
     int* d_gmap = nullptr;       // grid index -> cell number, -1 where the grid has no cell
     size_t cap_gmap = 0;
@@ -70,7 +70,7 @@ struct ProjState {
     size_t cap_Bhat = 0;
     __bf16* d_That = nullptr;    // one band's That_k, k = 0..n1: Gr x Gp each (hi), then (lo)
     size_t cap_That = 0;
-    double* d_Qhat = nullptr;    // one band's Qhat_k(|dy|, |dz|): (n1 + 1) x G
+    double* d_Qhat = nullptr;    // one band's Qhat_k(|dy|, |dz|) of a box: (n1 + 1) x G
     size_t cap_Qhat = 0;
     float* d_Yhat = nullptr;     // stage-1 products in frequency: M x nbat x G
     size_t cap_Yhat = 0;
@@ -98,6 +98,8 @@ struct ProjState {
     size_t cap_Yhat64 = 0;
     double* d_Y64 = nullptr;     // one band's Y = S_p B' in grid order: M x Ng (pitch ldg)
     size_t cap_Y64 = 0;
+    double* d_That64 = nullptr;  // one band's That_k of a shell: (Nphi + 1) x G x G
+    size_t cap_That64 = 0;
 };
 
 // ---- FP64 / FP32 MFMA GEMM -----------------------------------------------------
@@ -1253,6 +1255,78 @@ __global__ __launch_bounds__(256) void idft64_kernel(const double* __restrict__ 
         if (x < n1) Y[(size_t)m * ldg + (size_t)x * G + r] = acc[x];
 }
 
+// ---- Toeplitz projection of spherical shells (DFT along phi) -----------------------
+//
+// The arc cache of src/sig.cpp:151-199 gives S_p[a][b] = sqrt(Nbar_a Nbar_b) Q_p(h_a, h_b,
+// |dphi_a - dphi_b|), h = dr Nmu + dmu the arc of a cell: along phi, S_p is a symmetric
+// block-Toeplitz matrix of blocks T_d[h][h'] = Q_p(h, h', d) over the G = Nr Nmu arcs.  In
+// the grid order g = dphi G + h the box identity above holds with x -> phi, and B' in grid
+// order, its DFT, the batched slab GEMMs, the inverse DFT and stage 2 are the same kernels.
+// Only That_k differs: a shell's arcs are not translation invariant, so That_k is a dense
+// G x G matrix, That_k[h][h'] = Q_p(h, h', 0) + 2 sum_{d=1}^{Nphi-1} cos(2 pi d k / L)
+// Q_p(h, h', d) (L = 2 Nphi, k = 0..Nphi), formed from the Nphi contiguous table entries
+// of each arc pair.  Stage 1 costs 2 Nphi G^2 M flops instead of 2 N^2 M (Nphi / 2 times
+// fewer).  Table keys that no cell pair carries hold zero, so every That_k is finite; their
+// terms meet B' = 0 (a missing column cell) or are dropped with the rows of missing cells.
+
+// One block: arc row h (blockIdx.y), 256 arc columns.  The row's table slice (256 x Nphi
+// contiguous doubles) is staged through LDS by coalesced reads; each thread then forms the
+// Nphi + 1 transformed values of its column.  SPLIT: hi / lo BF16 planes [k][Gr][Gp], zero
+// outside G x G (the grid covers Gr x Gp); else FP64 T[k][G][G] (the grid covers G x G).
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void that_sph_kernel(const double* __restrict__ Q, int G, int nph,
+                                                       const double* __restrict__ trig, __bf16* __restrict__ hi,
+                                                       __bf16* __restrict__ lo, size_t Gr, size_t Gp,
+                                                       double* __restrict__ T) {
+    extern __shared__ double ts_q[];   // 256 x (nph + 1) table entries, then 2 cos(2 pi j / L), j < L
+    const int L = 2 * nph, ldq = nph + 1;
+    double* s2c = ts_q + 256 * ldq;
+    const int h = blockIdx.y, c0 = blockIdx.x * 256, t = threadIdx.x;
+    const int ncol = (h < G && c0 < G) ? min(256, G - c0) : 0;
+    for (int i = t; i < L; i += 256) s2c[i] = 2.0 * trig[i];
+    if (ncol > 0) {
+        const double* src = Q + ((size_t)h * G + c0) * nph;
+        for (int i = t; i < ncol * nph; i += 256) {
+            const int c = i / nph;
+            ts_q[c * ldq + (i - c * nph)] = src[i];
+        }
+    }
+    __syncthreads();
+    const bool live = t < ncol;
+    double q[TOEP_MAXX];
+#pragma unroll
+    for (int d = 0; d < TOEP_MAXX; d++) q[d] = (live && d < nph) ? ts_q[t * ldq + d] : 0.0;
+    const int c = c0 + t;
+    for (int k = 0; k <= nph; k++) {
+        double acc = q[0];
+        int j = 0;   // d k mod L
+#pragma unroll
+        for (int d = 1; d < TOEP_MAXX; d++) {
+            if (d >= nph) break;
+            j += k;
+            if (j >= L) j -= L;
+            acc += s2c[j] * q[d];
+        }
+        if constexpr (SPLIT) {
+            if ((size_t)c < Gp) {
+                __bf16 x, y;
+                split2((float)acc, x, y);
+                const size_t o = ((size_t)k * Gr + h) * Gp + c;
+                hi[o] = x;
+                lo[o] = y;
+            }
+        } else if (live) {
+            T[((size_t)k * G + h) * G + c] = acc;
+        }
+    }
+}
+
+// A operand of the FP64 / FP32 Toeplitz stage 1 of a shell: batch b reads the dense
+// That_((b + 1) / 2), row-major G x G.
+struct GA_ThatDense : SrcDenseRowMajor {
+    __device__ __forceinline__ void p_shift(int z, size_t s) { p += (size_t)((z + 1) >> 1) * s; }
+};
+
 // ---- host --------------------------------------------------------------------------
 
 static dim3 grid_for(int M, int N, int nb) {
@@ -1484,23 +1558,59 @@ static void trig_table(int L, std::vector<double>& t) {
     }
 }
 
+// Geometry of the Toeplitz projection: the transformed axis (x of a box, phi of a
+// shell) of n1 grid planes, the slab of G positions ((y, z) of a box, the Nr Nmu arcs
+// of a shell) and the grid index g = x G + slab position of a cell.
+struct ToepGrid {
+    bool sph;
+    int n1, G;
+};
+static ToepGrid toep_grid(const SurveyDev& sv) {
+    if (sv.coordsys == OPSEC_COORD_SPHERICAL) return {true, sv.n3, sv.n1 * sv.n2};
+    return {false, sv.n1, sv.n2 * sv.n3};
+}
+static size_t toep_index(const SurveyDev& sv, const opsec_cell& c) {
+    if (sv.coordsys == OPSEC_COORD_SPHERICAL)
+        return (size_t)c.d3 * sv.n1 * sv.n2 + (size_t)c.d1 * sv.n2 + c.d2;
+    return ((size_t)c.d1 * sv.n2 + c.d2) * sv.n3 + c.d3;
+}
+// grid index -> cell number, -1 where the grid has no cell (cells must occupy
+// distinct grid positions)
+static int toep_gmap(opsec_ctx* ctx, size_t Ng, std::vector<int>& gmap) {
+    gmap.assign(Ng, -1);
+    for (int a = 0; a < ctx->ncells; a++) {
+        const size_t g = toep_index(ctx->sv, ctx->h_cells[a]);
+        if (gmap[g] >= 0) return set_error(ctx, OPSEC_EINVAL, "cells %d and %d share a grid position", gmap[g], a);
+        gmap[g] = a;
+    }
+    return OPSEC_OK;
+}
+static int toep_check(opsec_ctx* ctx, const ToepGrid& tg) {
+    if (tg.n1 > TOEP_MAXX)
+        return set_error(ctx, OPSEC_EINVAL, "Toeplitz projection supports at most %d grid planes along %s (%d)",
+                         TOEP_MAXX, tg.sph ? "phi" : "x", tg.n1);
+    return OPSEC_OK;
+}
+static size_t that_sph_lds(int n1) { return ((size_t)256 * (n1 + 1) + 2 * (size_t)n1) * 8; }
+
 static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
     ProjState* ps = ctx->proj;
     const SurveyDev& sv = ctx->sv;
-    const int N = ctx->ncells, M = ctx->nmodes;
-    const int n1 = sv.n1, G = sv.n2 * sv.n3, Ng = n1 * G, L = 2 * n1, nbat = 2 * n1;
-    if (sv.coordsys != OPSEC_COORD_CARTESIAN) return set_error(ctx, OPSEC_EINVAL, "Toeplitz projection needs a box survey");
-    if (n1 > 64) return set_error(ctx, OPSEC_EINVAL, "Toeplitz projection supports Nx <= 64 (Nx = %d)", n1);
+    const int M = ctx->nmodes;
+    const ToepGrid tgrid = toep_grid(sv);
+    const int n1 = tgrid.n1, G = tgrid.G, Ng = n1 * G, L = 2 * n1, nbat = 2 * n1;
+    int rc;
+    if ((rc = toep_check(ctx, tgrid))) return rc;
     const size_t Mp = round_up(M, 256), Gr = round_up(G, 256), Gk = round_up(G, 32), Gp = plane_pitch(Gk);
     const size_t Kg = round_up(Ng, 256), Pg = plane_pitch(Kg);
     const size_t plane_b = Mp * Gp, plane_t = Gr * Gp;
-    int rc;
     if ((rc = dev_realloc(ctx, (void**)&ps->d_Bg, &ps->cap_Bg, 2 * Mp * Pg * sizeof(__bf16)))) return rc;
     if ((rc = dev_realloc(ctx, (void**)&ps->d_Bhat, &ps->cap_Bhat, 2 * (size_t)nbat * plane_b * sizeof(__bf16))))
         return rc;
     if ((rc = dev_realloc(ctx, (void**)&ps->d_That, &ps->cap_That, 2 * (size_t)(n1 + 1) * plane_t * sizeof(__bf16))))
         return rc;
-    if ((rc = dev_realloc(ctx, (void**)&ps->d_Qhat, &ps->cap_Qhat, (size_t)(n1 + 1) * G * 8))) return rc;
+    if (!tgrid.sph && (rc = dev_realloc(ctx, (void**)&ps->d_Qhat, &ps->cap_Qhat, (size_t)(n1 + 1) * G * 8)))
+        return rc;
     if ((rc = dev_realloc(ctx, (void**)&ps->d_Yhat, &ps->cap_Yhat, (size_t)M * nbat * G * 4))) return rc;
     if ((rc = dev_realloc(ctx, &ps->d_P, &ps->cap_P, (size_t)nb * packed_size(M) * 8))) return rc;
     const int s2 = std::max(1, std::min(g_s2_split, (int)(Kg / PK)));
@@ -1511,15 +1621,8 @@ static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
         return rc;
     const size_t cg_pair = 2 * Mp * Pg;   // one band's Y plane pair (bf16 elements)
     if (!ps->toep_valid) {
-        // grid map (cells must occupy distinct grid positions)
-        std::vector<int> gmap((size_t)Ng, -1);
-        for (int a = 0; a < N; a++) {
-            const opsec_cell& c = ctx->h_cells[a];
-            const size_t g = ((size_t)c.d1 * sv.n2 + c.d2) * sv.n3 + c.d3;
-            if (gmap[g] >= 0)
-                return set_error(ctx, OPSEC_EINVAL, "cells %d and %d share a grid position", gmap[g], a);
-            gmap[g] = a;
-        }
+        std::vector<int> gmap;
+        if ((rc = toep_gmap(ctx, Ng, gmap))) return rc;
         if ((rc = dev_realloc(ctx, (void**)&ps->d_gmap, &ps->cap_gmap, (size_t)Ng * 4))) return rc;
         std::vector<double> trig;
         trig_table(L, trig);
@@ -1556,6 +1659,9 @@ static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
     const size_t id_lds = ((size_t)nbat * 128 + (size_t)((n1 + 3) & ~3) * nbat) * 4;
     OPSEC_HIP_TRY(ctx, hipFuncSetAttribute((const void*)idft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)id_lds));
+    if (tgrid.sph)
+        OPSEC_HIP_TRY(ctx, hipFuncSetAttribute((const void*)that_sph_kernel<true>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)that_sph_lds(n1)));
     __bf16* Th = ps->d_That;
     __bf16* Tl = ps->d_That + (size_t)(n1 + 1) * plane_t;
     EpiYhat toY{ps->d_Yhat, M, G, nbat};
@@ -1575,10 +1681,16 @@ static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
             __bf16* Cz = ps->d_Cg + (size_t)q * cg_pair;
             const double* Qf = ctx->d_Q + (size_t)funcs[z] * ctx->nkeys;
             hipEventRecord(evb[4 * z], ctx->stream);
-            hipLaunchKernelGGL(qhat_kernel, dim3((unsigned)(((n1 + 1) * G + 255) / 256)), dim3(256), 0, ctx->stream,
-                               Qf, n1, G, ps->d_trig, ps->d_Qhat);
-            hipLaunchKernelGGL(that_kernel, dim3((unsigned)((Gp / 8 + 255) / 256), (unsigned)Gr, (unsigned)(n1 + 1)),
-                               dim3(256), 0, ctx->stream, ps->d_Qhat, G, sv.n3, Th, Tl, Gr, Gp);
+            if (tgrid.sph) {
+                hipLaunchKernelGGL(that_sph_kernel<true>, dim3((unsigned)((Gp + 255) / 256), (unsigned)Gr), dim3(256),
+                                   that_sph_lds(n1), ctx->stream, Qf, G, n1, ps->d_trig, Th, Tl, Gr, Gp, nullptr);
+            } else {
+                hipLaunchKernelGGL(qhat_kernel, dim3((unsigned)(((n1 + 1) * G + 255) / 256)), dim3(256), 0,
+                                   ctx->stream, Qf, n1, G, ps->d_trig, ps->d_Qhat);
+                hipLaunchKernelGGL(that_kernel,
+                                   dim3((unsigned)((Gp / 8 + 255) / 256), (unsigned)Gr, (unsigned)(n1 + 1)), dim3(256),
+                                   0, ctx->stream, ps->d_Qhat, G, sv.n3, Th, Tl, Gr, Gp);
+            }
             hipEventRecord(evb[4 * z + 1], ctx->stream);
             // XCD-aware tile order: the four row tiles that share a transformed-mode
             // block run on one XCD and meet in its L2 (measured 7 % faster here;
@@ -1643,28 +1755,29 @@ static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
 static int run_project_toeplitz64(opsec_ctx* ctx, int nb, const int* funcs, int prec) {
     ProjState* ps = ctx->proj;
     const SurveyDev& sv = ctx->sv;
-    const int N = ctx->ncells, M = ctx->nmodes;
-    const int n1 = sv.n1, G = sv.n2 * sv.n3, Ng = n1 * G, L = 2 * n1, nbat = 2 * n1;
-    if (sv.coordsys != OPSEC_COORD_CARTESIAN) return set_error(ctx, OPSEC_EINVAL, "Toeplitz projection needs a box survey");
-    if (n1 > TOEP_MAXX) return set_error(ctx, OPSEC_EINVAL, "Toeplitz projection supports Nx <= %d (Nx = %d)", TOEP_MAXX, n1);
-    if (sv.n2 >= 65536 || sv.n3 >= 65536) return set_error(ctx, OPSEC_EINVAL, "Toeplitz projection: Ny, Nz < 65536");
-    const size_t ldg = (size_t)Ng + 32;   // pitch of the grid-order operands (not a power of two)
+    const int M = ctx->nmodes;
+    const ToepGrid tgrid = toep_grid(sv);
+    const int n1 = tgrid.n1, G = tgrid.G, Ng = n1 * G, L = 2 * n1, nbat = 2 * n1;
     int rc;
+    if ((rc = toep_check(ctx, tgrid))) return rc;
+    if (!tgrid.sph && (sv.n2 >= 65536 || sv.n3 >= 65536))
+        return set_error(ctx, OPSEC_EINVAL, "Toeplitz projection: Ny, Nz < 65536");
+    const size_t ldg = (size_t)Ng + 32;   // pitch of the grid-order operands (not a power of two)
     if ((rc = dev_realloc(ctx, (void**)&ps->d_Yhat64, &ps->cap_Yhat64, (size_t)nbat * M * G * 8))) return rc;
     if ((rc = dev_realloc(ctx, (void**)&ps->d_Y64, &ps->cap_Y64, (size_t)M * ldg * 8))) return rc;
-    if ((rc = dev_realloc(ctx, (void**)&ps->d_Qhat, &ps->cap_Qhat, (size_t)(n1 + 1) * G * 8))) return rc;
+    if (tgrid.sph) {
+        if ((rc = dev_realloc(ctx, (void**)&ps->d_That64, &ps->cap_That64, (size_t)(n1 + 1) * G * G * 8))) return rc;
+    } else if ((rc = dev_realloc(ctx, (void**)&ps->d_Qhat, &ps->cap_Qhat, (size_t)(n1 + 1) * G * 8))) {
+        return rc;
+    }
     if ((rc = dev_realloc(ctx, &ps->d_P, &ps->cap_P, (size_t)nb * packed_size(M) * 8))) return rc;
     if (!ps->toep64_valid) {
-        std::vector<int> gmap((size_t)Ng, -1);
-        for (int a = 0; a < N; a++) {
-            const opsec_cell& c = ctx->h_cells[a];
-            const size_t g = ((size_t)c.d1 * sv.n2 + c.d2) * sv.n3 + c.d3;
-            if (gmap[g] >= 0)
-                return set_error(ctx, OPSEC_EINVAL, "cells %d and %d share a grid position", gmap[g], a);
-            gmap[g] = a;
-        }
-        std::vector<int> slab((size_t)G);
-        for (int r = 0; r < G; r++) slab[r] = ((r / sv.n3) << 16) | (r % sv.n3);
+        std::vector<int> gmap;
+        if ((rc = toep_gmap(ctx, Ng, gmap))) return rc;
+        // slab position -> (y << 16) | z (box only)
+        std::vector<int> slab((size_t)G, 0);
+        if (!tgrid.sph)
+            for (int r = 0; r < G; r++) slab[r] = ((r / sv.n3) << 16) | (r % sv.n3);
         std::vector<double> trig;
         trig_table(L, trig);
         // inverse weights W[b][x]: (c_k / L) cos and -(c_k / L) sin, c_0 = c_n1 = 1, else 2
@@ -1697,6 +1810,9 @@ static int run_project_toeplitz64(opsec_ctx* ctx, int nb, const int* funcs, int 
     const size_t id_lds = (size_t)nbat * n1 * 8;
     OPSEC_HIP_TRY(ctx, hipFuncSetAttribute((const void*)idft64_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)id_lds));
+    if (tgrid.sph)
+        OPSEC_HIP_TRY(ctx, hipFuncSetAttribute((const void*)that_sph_kernel<false>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)that_sph_lds(n1)));
     std::vector<hipEvent_t> ev(4 * (size_t)nb);
     for (auto& e : ev) OPSEC_HIP_TRY(ctx, hipEventCreate(&e));
     const size_t np = (size_t)packed_size(M);
@@ -1705,6 +1821,9 @@ static int run_project_toeplitz64(opsec_ctx* ctx, int nb, const int* funcs, int 
     A.slab = ps->d_slab;
     A.G = G;
     A.n3 = sv.n3;
+    GA_ThatDense As;
+    As.p = ps->d_That64;
+    As.ld = G;
     GB_ColMajor Bop;   // batch b: Bhat[b] (B[k = r][n = m] = p[m G + r])
     Bop.p = ps->d_Bhat64;
     Bop.ld = G;
@@ -1718,10 +1837,22 @@ static int run_project_toeplitz64(opsec_ctx* ctx, int nb, const int* funcs, int 
         const double* Qf = ctx->d_Q + (size_t)funcs[z] * ctx->nkeys;
         PutPacked outP{(double*)ps->d_P + (size_t)z * np, 0};
         hipEventRecord(ev[4 * z], ctx->stream);
-        hipLaunchKernelGGL(qhat_kernel, dim3((unsigned)(((n1 + 1) * G + 255) / 256)), dim3(256), 0, ctx->stream, Qf, n1,
-                           G, ps->d_trig64, ps->d_Qhat);
+        if (tgrid.sph)
+            hipLaunchKernelGGL(that_sph_kernel<false>, dim3((unsigned)((G + 255) / 256), (unsigned)G), dim3(256),
+                               that_sph_lds(n1), ctx->stream, Qf, G, n1, ps->d_trig64, nullptr, nullptr, 0, 0,
+                               ps->d_That64);
+        else
+            hipLaunchKernelGGL(qhat_kernel, dim3((unsigned)(((n1 + 1) * G + 255) / 256)), dim3(256), 0, ctx->stream,
+                               Qf, n1, G, ps->d_trig64, ps->d_Qhat);
         hipEventRecord(ev[4 * z + 1], ctx->stream);
-        if (prec == OPSEC_PREC_FP64)
+        const size_t tplane = (size_t)G * G;
+        if (tgrid.sph && prec == OPSEC_PREC_FP64)
+            hipLaunchKernelGGL((gemm_kernel<double, GA_ThatDense, GB_ColMajor, PutColMajor<double>, false>), g1,
+                               dim3(GT_THREADS), 0, ctx->stream, As, tplane, Bop, (size_t)M * G, toYhat, G, M, G);
+        else if (tgrid.sph)
+            hipLaunchKernelGGL((gemm_kernel<float, GA_ThatDense, GB_ColMajor, PutColMajor<double>, false>), g1,
+                               dim3(GT_THREADS), 0, ctx->stream, As, tplane, Bop, (size_t)M * G, toYhat, G, M, G);
+        else if (prec == OPSEC_PREC_FP64)
             hipLaunchKernelGGL((gemm_kernel<double, GA_That, GB_ColMajor, PutColMajor<double>, false>), g1,
                                dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, (size_t)M * G, toYhat, G, M, G);
         else
@@ -1907,6 +2038,7 @@ void free_proj(opsec_ctx* ctx) {
     dev_free((void**)&ctx->proj->d_Bhat64);
     dev_free((void**)&ctx->proj->d_Yhat64);
     dev_free((void**)&ctx->proj->d_Y64);
+    dev_free((void**)&ctx->proj->d_That64);
     delete ctx->proj;
     ctx->proj = nullptr;
 }

@@ -60,7 +60,8 @@ bool load_survey(const Config& cfg, opsec_survey* sv);
 //   hip.near      = near-pair threshold in cells
 //   hip.xi        = table | closed           (default table: the reference's tables)
 //   hip.precision = fp64 | fp32 | bf16x3     (default fp64)
-//   hip.projection = dense | toeplitz        (split-BF16x3 projection method; default dense)
+//   hip.projection = dense | toeplitz        (projection method, every precision: dense S_p, or the DFT along
+//                                             x of a box / phi of a shell; default dense)
 //   hip.device    = device index             (default 0)
 //   hip.tol, hip.block, hip.maxdim           (klt eigen-solver)
 opsec_quad quad_from_config(const Config& cfg, double epsrel, double epsabs);

@@ -421,7 +421,7 @@ class Context:
         return out
 
     def set_projection(self, method):
-        """OPSEC_PROJ_DENSE or OPSEC_PROJ_TOEPLITZ (split-BF16x3 projections)."""
+        """OPSEC_PROJ_DENSE or OPSEC_PROJ_TOEPLITZ (DFT along x of a box or phi of a shell; every precision)."""
         self.check(self.lib.opsec_set_projection(self.h, method), "set_projection")
 
     def project(self, funcs, prec=OPSEC_PREC_FP64, keep_on_device=False):

@@ -96,12 +96,50 @@ def test_spherical_projection_all_precisions(ctx):
         err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
         print("spherical projection", prec, "max rel err per band", err)
         assert err.max() < tol
+    # the Toeplitz projection (DFT along phi) against the same brute-force products
     ctx.set_projection(lib.OPSEC_PROJ_TOEPLITZ)
     try:
-        with pytest.raises(lib.OpsecError, match="box survey"):
-            ctx.project(funcs, lib.OPSEC_PREC_BF16X3)
+        got = ctx.project(funcs, lib.OPSEC_PREC_FP64)
+        np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+        for prec, tol in ((lib.OPSEC_PREC_FP32, 2e-5), (lib.OPSEC_PREC_BF16X3, 5e-5)):
+            got = ctx.project(funcs, prec)
+            err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
+            print("spherical Toeplitz projection", prec, "max rel err per band", err)
+            assert err.max() < tol
     finally:
         ctx.set_projection(lib.OPSEC_PROJ_DENSE)
+
+
+@pytest.mark.parametrize("dims,drop,shuffle", [((17, 16, 4), 0.0, False), ((3, 5, 7), 0.3, True), ((4, 4, 1), 0.0, True),
+                                               ((2, 3, 2), 0.2, False)])
+def test_spherical_toeplitz_vs_dense(ctx, dims, drop, shuffle):
+    """Toeplitz projection of a shell (length-2 Nphi DFT along phi, dense arc x arc
+    That_k) against the dense FP64 projection of the same arc-pair tables: 272
+    arcs (two column tiles of the That kernel, a padded 512-row GEMM tile), an
+    odd Nphi with a third of the cells missing in shuffled order (table keys that
+    no pair carries), a single phi plane, and two planes."""
+    cells, _ = _shell(ctx, dims, drop, nb=2, shuffle=shuffle)
+    funcs = [0, 1, 2]
+    ctx.pair_tables(funcs, ctx.quad(lib.OPSEC_QUAD_GM))
+    N = len(cells)
+    m = min(N - 1, 70)
+    B = np.linalg.qr(np.random.default_rng(sum(dims)).normal(size=(N, m)))[0].T.copy()
+    ctx.modes_upload(B)
+    ref = ctx.project(funcs, lib.OPSEC_PREC_FP64)
+    try:
+        ctx.set_projection(lib.OPSEC_PROJ_TOEPLITZ)
+        got = ctx.project(funcs, lib.OPSEC_PREC_BF16X3)
+        again = ctx.project(funcs, lib.OPSEC_PREC_BF16X3)
+        got64 = ctx.project(funcs, lib.OPSEC_PREC_FP64)
+        got32 = ctx.project(funcs, lib.OPSEC_PREC_FP32)
+    finally:
+        ctx.set_projection(lib.OPSEC_PROJ_DENSE)
+    assert np.array_equal(got, again)
+    rel = lambda x: (np.abs(x - ref).max(axis=1) / np.abs(ref).max(axis=1)).max()
+    print("spherical Toeplitz", dims, "BF16x3", rel(got), "FP64", rel(got64), "FP32", rel(got32))
+    assert rel(got64) < 1e-12
+    assert rel(got32) < 2e-5
+    assert rel(got) < 5e-5
 
 
 def test_spherical_rejects_tent_rule(ctx):
@@ -127,8 +165,8 @@ def test_spherical_klt_vs_oracle(ctx):
 
 
 def test_spherical_programs_end_to_end(tmp_path):
-    """klt -> comma (FP64 and split-BF16x3) -> estimate on a spherical shell
-    given by a SphericalSurvey configuration."""
+    """klt -> comma (FP64 and split-BF16x3, dense and Toeplitz projections) ->
+    estimate on a spherical shell given by a SphericalSurvey configuration."""
     nr, nmu, nphi, nb, M = 3, 3, 8, 3, 30
     cells = synth.drop_cells(synth.shell_cells(nr, nmu, nphi, rmin=RMIN, rmax=RMAX), 0.1, seed=8)
     N = len(cells)
@@ -161,7 +199,8 @@ def test_spherical_programs_end_to_end(tmp_path):
     w = np.linalg.eigvalsh(0.5 * (mats[0] + mats[0].T))[::-1][:M]
     np.testing.assert_allclose(lam, w, atol=1e-9 * w[0])
     K = M * (M + 1) // 2
-    for extra, tol in (((), 1e-10), (("hip.precision=bf16x3",), 5e-5)):
+    for extra, tol in (((), 1e-10), (("hip.precision=bf16x3",), 5e-5), (("hip.projection=toeplitz",), 1e-10),
+                       (("hip.precision=bf16x3", "hip.projection=toeplitz"), 5e-5)):
         r = run("comma", "-c", str(cfile), f"Nmodes={M}", f"modefile={tmp_path}/modes.abn",
                 f"covfile={tmp_path}/cov.abn", *extra)
         assert r.returncode == 0, r.stderr

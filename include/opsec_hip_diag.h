@@ -58,6 +58,11 @@ int opsec_diag_set_nt_order(int group, int remap);
  * step's loads issued at its start (1) or in two halves, at its start and
  * mid-step (2, the default). */
 int opsec_diag_set_g3_dl(int v);
+/* Pair tables of a spherical survey: the sweep of the functions without band
+ * edges (the prior, whose near-diagonal integrals run to the evaluation cap)
+ * on a second stream from a second host thread, beside the band functions'
+ * sweep (1, the default), or after it (0).  The tables are the same bits. */
+int opsec_diag_set_gm_overlap(int v);
 
 #ifdef __cplusplus
 }

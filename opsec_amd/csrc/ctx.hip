@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "common.h"
 #include "xi_dev.h"
@@ -23,6 +24,8 @@ int set_error(opsec_ctx* ctx, int code, const char* fmt, ...) {
         va_start(ap, fmt);
         vsnprintf(buf, sizeof buf, fmt, ap);
         va_end(ap);
+        static std::mutex m;   // a shell's pair-table sweeps run on two host threads
+        std::lock_guard<std::mutex> lk(m);
         ctx->err = buf;
     }
     return code;

@@ -1285,10 +1285,22 @@ __global__ __launch_bounds__(256) void that_sph_kernel(const double* __restrict_
     const int ncol = (h < G && c0 < G) ? min(256, G - c0) : 0;
     for (int i = t; i < L; i += 256) s2c[i] = 2.0 * trig[i];
     if (ncol > 0) {
+        // all of a thread's loads (element t + 256 u, u < nph) in flight before the LDS stores
         const double* src = Q + ((size_t)h * G + c0) * nph;
-        for (int i = t; i < ncol * nph; i += 256) {
-            const int c = i / nph;
-            ts_q[c * ldq + (i - c * nph)] = src[i];
+        const int n = ncol * nph;
+        double v[TOEP_MAXX];
+#pragma unroll
+        for (int u = 0; u < TOEP_MAXX; u++) {
+            const int i = t + u * 256;
+            v[u] = (u < nph && i < n) ? src[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < TOEP_MAXX; u++) {
+            const int i = t + u * 256;
+            if (u < nph && i < n) {
+                const int c = i / nph;
+                ts_q[c * ldq + (i - c * nph)] = v[u];
+            }
         }
     }
     __syncthreads();
@@ -2152,7 +2164,7 @@ int opsec_diag_set_nt_gen(int v) {
 }
 
 // Operand tiles of the 256 x 256 split-BF16x3 GEMM: through VGPRs (0) or by
-// global-to-LDS DMA loads (1), for tuning.
+// global-to-LDS DMA loads, issued at a step's start (1) or in two halves (2).
 int opsec_diag_set_g3_dl(int v) {
     if (v < 0 || v > 2) return OPSEC_EINVAL;
     g_g3_dl = v;

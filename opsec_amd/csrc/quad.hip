@@ -28,6 +28,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <thread>
 
 #include "common.h"
 #include "xi_dev.h"
@@ -1029,9 +1031,15 @@ PairGeo grid_pair_geo(const SurveyDev& sv, int kx, int ky, int kz) {
 struct GMScratch {
     Scratch sj, sd, si, sc, sf, sp, st, sx, sa;
 };
+static std::mutex g_gm_stats_mutex;
+// Shell tables: the deep sweep beside the shallow one on a second stream (1, the
+// default) or after it (0).
+static int g_gm_overlap = 1;
 
 static int run_gm_device(opsec_ctx* ctx, int nf, const int* funcs, const GMJob* d_tmpl, size_t npairs,
-                         const opsec_quad* q, double* dQ, size_t nkeys, double* dQerr, GMScratch* gs = nullptr) {
+                         const opsec_quad* q, double* dQ, size_t nkeys, double* dQerr, GMScratch* gs = nullptr,
+                         hipStream_t stm = nullptr) {
+    if (!stm) stm = ctx->stream;
     XiModelDev mv = model_view(ctx);
     if (npairs == 0 || nf == 0) return OPSEC_OK;
     GMScratch local;
@@ -1082,7 +1090,7 @@ static int run_gm_device(opsec_ctx* ctx, int nf, const int* funcs, const GMJob* 
         if ((rc = dev_realloc(ctx, &sf.p, &sf.cap, sizeof(int) * njobs))) return rc;
         if ((rc = dev_realloc(ctx, &sp.p, &sp.cap, sizeof(int) * njobs))) return rc;
         if ((rc = dev_realloc(ctx, &sx.p, &sx.cap, sizeof(int) * njobs))) return rc;
-        OPSEC_HIP_TRY(ctx, hipMemsetAsync(sc.p, 0, 64, ctx->stream));
+        OPSEC_HIP_TRY(ctx, hipMemsetAsync(sc.p, 0, 64, stm));
         int* d_counter = (int*)sc.p;
         unsigned long long* d_nev = (unsigned long long*)((char*)sc.p + 8);
         int* d_npend = (int*)((char*)sc.p + 16);
@@ -1093,48 +1101,48 @@ static int run_gm_device(opsec_ctx* ctx, int nf, const int* funcs, const GMJob* 
         if (pass == 0) {
             if ((rc = dev_realloc(ctx, &st.p, &st.cap, sizeof(int) * nf))) return rc;
             int* dfun = (int*)st.p;
-            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dfun, funcs, sizeof(int) * nf, hipMemcpyHostToDevice, ctx->stream));
+            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dfun, funcs, sizeof(int) * nf, hipMemcpyHostToDevice, stm));
             hipLaunchKernelGGL(expand_gm_jobs_kernel, dim3((unsigned)((njobs + 255) / 256)), dim3(256), 0,
-                               ctx->stream, d_tmpl, (int)npairs, dfun, nf, (GMJob*)sj.p);
+                               stm, d_tmpl, (int)npairs, dfun, nf, (GMJob*)sj.p);
             OPSEC_HIP_TRY(ctx, hipGetLastError());
             {
                 // first regions, then the shallow integrals one thread each
                 hipLaunchKernelGGL(gm_first_kernel, dim3((unsigned)((njobs + 255) / 256)), dim3(256), 0,
-                                   ctx->stream, mv, (const GMJob*)sj.p, njobs, P, dQ, nkeys, dQerr, (int*)sf.p,
+                                   stm, mv, (const GMJob*)sj.p, njobs, P, dQ, nkeys, dQerr, (int*)sf.p,
                                    (int*)sx.p, d_npend, d_nev);
                 OPSEC_HIP_TRY(ctx, hipGetLastError());
                 int npend = 0;
-                OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&npend, d_npend, 4, hipMemcpyDeviceToHost, ctx->stream));
-                OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+                OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&npend, d_npend, 4, hipMemcpyDeviceToHost, stm));
+                OPSEC_HIP_TRY(ctx, hipStreamSynchronize(stm));
                 nrun = 0;
                 if (npend > 0) {
                     hipLaunchKernelGGL(gm_thread_kernel, dim3((unsigned)((npend + 255) / 256)), dim3(256), 0,
-                                       ctx->stream, mv, (const GMJob*)sj.p, (const int*)sx.p, npend, P, dQ, nkeys,
+                                       stm, mv, (const GMJob*)sj.p, (const int*)sx.p, npend, P, dQ, nkeys,
                                        dQerr, (int*)sf.p, (int*)sp.p, d_ndeep, d_nev);
                     OPSEC_HIP_TRY(ctx, hipGetLastError());
-                    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nrun, d_ndeep, 4, hipMemcpyDeviceToHost, ctx->stream));
-                    OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+                    OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nrun, d_ndeep, 4, hipMemcpyDeviceToHost, stm));
+                    OPSEC_HIP_TRY(ctx, hipStreamSynchronize(stm));
                 }
                 d_idx = (const int*)sp.p;
             }
         } else {
-            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(sj.p, sa.p, sizeof(GMJob) * njobs, hipMemcpyDeviceToDevice, ctx->stream));
+            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(sj.p, sa.p, sizeof(GMJob) * njobs, hipMemcpyDeviceToDevice, stm));
         }
         if (nrun > 0) {
             if (pass == 0)
-                hipLaunchKernelGGL((gm_kernel<64, 0>), dim3(std::min(nslots, nrun)), dim3(64), 0, ctx->stream, mv,
+                hipLaunchKernelGGL((gm_kernel<64, 0>), dim3(std::min(nslots, nrun)), dim3(64), 0, stm, mv,
                                    (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                    nkeys, dQerr, (int*)sf.p, d_nev);
             else if (nrun <= 2 * ctx->num_cus)   // few deep integrals: 8 waves each
-                hipLaunchKernelGGL((gm_kernel<512, 8192>), dim3(std::min(nslots, nrun)), dim3(512), 0, ctx->stream, mv,
+                hipLaunchKernelGGL((gm_kernel<512, 8192>), dim3(std::min(nslots, nrun)), dim3(512), 0, stm, mv,
                                    (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                    nkeys, dQerr, (int*)sf.p, d_nev);
             else if (nrun <= 4 * ctx->num_cus)
-                hipLaunchKernelGGL((gm_kernel<256, 1024>), dim3(std::min(nslots, nrun)), dim3(256), 0, ctx->stream, mv,
+                hipLaunchKernelGGL((gm_kernel<256, 1024>), dim3(std::min(nslots, nrun)), dim3(256), 0, stm, mv,
                                    (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                    nkeys, dQerr, (int*)sf.p, d_nev);
             else   // many: one wave each, all resident at once (two per SIMD)
-                hipLaunchKernelGGL((gm_kernel<64, 0>), dim3(std::min(nslots, nrun)), dim3(64), 0, ctx->stream, mv,
+                hipLaunchKernelGGL((gm_kernel<64, 0>), dim3(std::min(nslots, nrun)), dim3(64), 0, stm, mv,
                                    (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                    nkeys, dQerr, (int*)sf.p, d_nev);
             OPSEC_HIP_TRY(ctx, hipGetLastError());
@@ -1142,22 +1150,22 @@ static int run_gm_device(opsec_ctx* ctx, int nf, const int* funcs, const GMJob* 
         // jobs whose region pool ran out, listed on the device
         int nfail = 0;
         if (nrun > 0) {
-            hipLaunchKernelGGL(list_failed_kernel, dim3((unsigned)((nrun + 255) / 256)), dim3(256), 0, ctx->stream,
+            hipLaunchKernelGGL(list_failed_kernel, dim3((unsigned)((nrun + 255) / 256)), dim3(256), 0, stm,
                                (const int*)sf.p, d_idx, nrun, (int*)sx.p, d_nfail);
             OPSEC_HIP_TRY(ctx, hipGetLastError());
         }
         unsigned long long nev = 0;
-        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nev, d_nev, 8, hipMemcpyDeviceToHost, ctx->stream));
-        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nfail, d_nfail, 4, hipMemcpyDeviceToHost, ctx->stream));
-        OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nev, d_nev, 8, hipMemcpyDeviceToHost, stm));
+        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nfail, d_nfail, 4, hipMemcpyDeviceToHost, stm));
+        OPSEC_HIP_TRY(ctx, hipStreamSynchronize(stm));
         nev_total += nev;
         if (nfail > 0 && pass == 1)
             return set_error(ctx, OPSEC_ECONV, "adaptive cubature exceeded %d regions", P.rmax);
         std::vector<int> failed(nfail);
         if (nfail > 0) {
             OPSEC_HIP_TRY(ctx, hipMemcpyAsync(failed.data(), sx.p, sizeof(int) * nfail, hipMemcpyDeviceToHost,
-                                              ctx->stream));
-            OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+                                              stm));
+            OPSEC_HIP_TRY(ctx, hipStreamSynchronize(stm));
         }
         std::sort(failed.begin(), failed.end());
         nagain = nfail;
@@ -1167,16 +1175,19 @@ static int run_gm_device(opsec_ctx* ctx, int nf, const int* funcs, const GMJob* 
             int* d_failed = (int*)((GMJob*)sa.p + nfail);
             int* d_fun = d_failed + nfail;
             OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_failed, failed.data(), sizeof(int) * nfail, hipMemcpyHostToDevice,
-                                              ctx->stream));
-            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_fun, funcs, sizeof(int) * nf, hipMemcpyHostToDevice, ctx->stream));
-            hipLaunchKernelGGL(gather_gm_jobs_kernel, dim3((unsigned)((nfail + 255) / 256)), dim3(256), 0, ctx->stream,
+                                              stm));
+            OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_fun, funcs, sizeof(int) * nf, hipMemcpyHostToDevice, stm));
+            hipLaunchKernelGGL(gather_gm_jobs_kernel, dim3((unsigned)((nfail + 255) / 256)), dim3(256), 0, stm,
                                d_tmpl, (long long)npairs, d_fun, d_failed, nfail, (GMJob*)sa.p);
             OPSEC_HIP_TRY(ctx, hipGetLastError());
-            OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            OPSEC_HIP_TRY(ctx, hipStreamSynchronize(stm));
         }
     }
-    ctx->st_integrals += (int64_t)njobs_all;
-    ctx->st_nodes += (int64_t)nev_total;
+    {
+        std::lock_guard<std::mutex> lk(g_gm_stats_mutex);   // sweeps may run on two host threads
+        ctx->st_integrals += (int64_t)njobs_all;
+        ctx->st_nodes += (int64_t)nev_total;
+    }
     return OPSEC_OK;
 }
 
@@ -1428,41 +1439,78 @@ static int build_pair_tables_sph(opsec_ctx* ctx, int nf, const int* funcs, const
         int* d_mem = d_start + (narcs + 1);
         int* d_d3 = d_mem + N;
         double* d_cb = (double*)(((uintptr_t)(d_d3 + N) + 15) & ~(uintptr_t)15);
-        unsigned long long* d_count = (unsigned long long*)(d_cb + 6 * (size_t)N);
+        unsigned long long* d_count = (unsigned long long*)(d_cb + 6 * (size_t)N);   // two counters
         OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_start, arc_start.data(), 4 * (size_t)(narcs + 1), hipMemcpyHostToDevice,
                                           ctx->stream));
         OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_mem, arc_mem.data(), 4 * (size_t)N, hipMemcpyHostToDevice, ctx->stream));
         OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_d3, cd3.data(), 4 * (size_t)N, hipMemcpyHostToDevice, ctx->stream));
         OPSEC_HIP_TRY(ctx, hipMemcpyAsync(d_cb, cb.data(), 48 * (size_t)N, hipMemcpyHostToDevice, ctx->stream));
         const size_t per_row = (size_t)narcs * Nphi;   // keys of one row arc at most
-        for (const std::vector<int>* group : {&deep, &shallow}) {
-            if (group->empty()) continue;
-            GMScratch gs;   // one sweep's job buffers, freed before the next sweep
-            const int ng = (int)group->size();
+        // One sweep: a group's functions over every arc pair, a range of row arcs
+        // at a time, on stream s with its own template buffer tb and counter cnt.
+        auto sweep = [&](const std::vector<int>& group, hipStream_t s, Scratch& tb, unsigned long long* cnt) -> int {
+            GMScratch gs;   // the sweep's job buffers, kept across its chunks
+            const int ng = (int)group.size();
             const size_t chunk = std::max<size_t>(4096, ((size_t)1 << 28) / (size_t)ng);
             const int nha = (int)std::max<size_t>(1, std::min<size_t>(narcs, chunk / per_row));
-            if ((rc = dev_realloc(ctx, &st.p, &st.cap, sizeof(GMJob) * (size_t)nha * per_row))) return rc;
+            int rc2;
+            if ((rc2 = dev_realloc(ctx, &tb.p, &tb.cap, sizeof(GMJob) * (size_t)nha * per_row))) return rc2;
             for (int ha0 = 0; ha0 < narcs; ha0 += nha) {
                 const int nh = std::min(nha, narcs - ha0);
-                OPSEC_HIP_TRY(ctx, hipMemsetAsync(d_count, 0, 8, ctx->stream));
+                OPSEC_HIP_TRY(ctx, hipMemsetAsync(cnt, 0, 8, s));
                 const long long nt = (long long)nh * narcs;
                 if (Nphi <= 64)
-                    hipLaunchKernelGGL(sph_templates_kernel<2>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0,
-                                       ctx->stream, d_start, d_mem, d_d3, d_cb, narcs, Nphi, ha0, nh, (GMJob*)st.p,
-                                       d_count);
+                    hipLaunchKernelGGL(sph_templates_kernel<2>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s,
+                                       d_start, d_mem, d_d3, d_cb, narcs, Nphi, ha0, nh, (GMJob*)tb.p, cnt);
                 else
-                    hipLaunchKernelGGL(sph_templates_kernel<32>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0,
-                                       ctx->stream, d_start, d_mem, d_d3, d_cb, narcs, Nphi, ha0, nh, (GMJob*)st.p,
-                                       d_count);
+                    hipLaunchKernelGGL(sph_templates_kernel<32>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s,
+                                       d_start, d_mem, d_d3, d_cb, narcs, Nphi, ha0, nh, (GMJob*)tb.p, cnt);
                 OPSEC_HIP_TRY(ctx, hipGetLastError());
                 unsigned long long npairs = 0;
-                OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&npairs, d_count, 8, hipMemcpyDeviceToHost, ctx->stream));
-                OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-                if ((rc = run_gm_device(ctx, ng, group->data(), (const GMJob*)st.p, (size_t)npairs, q, ctx->d_Q,
-                                        (size_t)ctx->nkeys, nullptr, &gs)))
-                    return rc;
+                OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&npairs, cnt, 8, hipMemcpyDeviceToHost, s));
+                OPSEC_HIP_TRY(ctx, hipStreamSynchronize(s));
+                if ((rc2 = run_gm_device(ctx, ng, group.data(), (const GMJob*)tb.p, (size_t)npairs, q, ctx->d_Q,
+                                         (size_t)ctx->nkeys, nullptr, &gs, s)))
+                    return rc2;
             }
+            return OPSEC_OK;
+        };
+        if (deep.empty() || shallow.empty() || !g_gm_overlap) {
+            for (const std::vector<int>* group : {&deep, &shallow})
+                if (!group->empty() && (rc = sweep(*group, ctx->stream, st, d_count))) return rc;
+            return OPSEC_OK;
         }
+        // The deep sweep (few integrals, most of them run to the evaluation cap
+        // by latency-bound wavefronts that leave the CUs mostly idle) runs on a
+        // second stream from a second host thread, beside the shallow sweep's
+        // full-device passes.  The sweeps write different functions' tables and
+        // every integral is computed alone, so the tables are the same bits.
+        hipStream_t s2 = nullptr;
+        hipEvent_t ready = nullptr;
+        if (hipStreamCreateWithFlags(&s2, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&ready, hipEventDisableTiming) != hipSuccess ||
+            hipEventRecord(ready, ctx->stream) != hipSuccess || hipStreamWaitEvent(s2, ready, 0) != hipSuccess) {
+            const hipError_t e = hipGetLastError();
+            if (ready) hipEventDestroy(ready);
+            if (s2) hipStreamDestroy(s2);
+            return set_error(ctx, OPSEC_EHIP, "second quadrature stream: %s", hipGetErrorString(e));
+        }
+        Scratch st2;
+        int rc_deep = OPSEC_OK;
+        std::thread worker([&] {
+            if (hipSetDevice(ctx->device) != hipSuccess)
+                rc_deep = set_error(ctx, OPSEC_EHIP, "quadrature thread: hipSetDevice(%d) failed", ctx->device);
+            else
+                rc_deep = sweep(deep, s2, st2, d_count + 1);
+        });
+        rc = sweep(shallow, ctx->stream, st, d_count);
+        worker.join();
+        const hipError_t e2 = hipStreamSynchronize(s2);
+        hipEventDestroy(ready);
+        hipStreamDestroy(s2);
+        if (rc) return rc;
+        if (rc_deep) return rc_deep;
+        if (e2 != hipSuccess) return set_error(ctx, OPSEC_EHIP, "quadrature stream: %s", hipGetErrorString(e2));
         return OPSEC_OK;
     }
     std::vector<int> keys, nearflag, tab(Nphi);
@@ -1946,6 +1994,13 @@ int opsec_pair_integral(opsec_ctx* ctx, const opsec_cell* c1, const opsec_cell* 
     OPSEC_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     *Q = qe[0];
     if (err) *err = qe[1];
+    return OPSEC_OK;
+}
+
+// Shell pair tables: the deep sweep beside the shallow one (1) or after it (0).
+int opsec_diag_set_gm_overlap(int v) {
+    if (v != 0 && v != 1) return OPSEC_EINVAL;
+    g_gm_overlap = v;
     return OPSEC_OK;
 }
 

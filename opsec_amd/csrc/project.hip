@@ -1269,34 +1269,43 @@ __global__ __launch_bounds__(256) void idft64_kernel(const double* __restrict__ 
 // fewer).  Table keys that no cell pair carries hold zero, so every That_k is finite; their
 // terms meet B' = 0 (a missing column cell) or are dropped with the rows of missing cells.
 
-// One block: arc row h (blockIdx.y), 256 arc columns.  The row's table slice (256 x Nphi
-// contiguous doubles) is staged through LDS by coalesced reads; each thread then forms the
-// Nphi + 1 transformed values of its column.  SPLIT: hi / lo BF16 planes [k][Gr][Gp], zero
-// outside G x G (the grid covers Gr x Gp); else FP64 T[k][G][G] (the grid covers G x G).
-template <bool SPLIT>
+// One block: arc row h (blockIdx.y), CB arc columns (one per thread).  The row's table
+// slice (CB x Nphi contiguous doubles) is staged through LDS by coalesced reads, next to
+// the DFT weights
+// C[k][d] (1 at d = 0, else 2 cos(2 pi d k / L)); each thread then forms the Nphi + 1
+// transformed values of its column, That_k = sum_d C[k][d] q[d] in d order, from
+// wave-uniform LDS rows.  NMAX >= Nphi bounds the registers; CB = 128 for NMAX = 64 keeps
+// the block's LDS under 100 KB.  SPLIT: hi / lo BF16 planes
+// [k][Gr][Gp], zero outside G x G (the grid covers Gr x Gp); else FP64 T[k][G][G] (the
+// grid covers G x G).
+template <bool SPLIT, int NMAX>
 __global__ __launch_bounds__(256) void that_sph_kernel(const double* __restrict__ Q, int G, int nph,
                                                        const double* __restrict__ trig, __bf16* __restrict__ hi,
                                                        __bf16* __restrict__ lo, size_t Gr, size_t Gp,
                                                        double* __restrict__ T) {
-    extern __shared__ double ts_q[];   // 256 x (nph + 1) table entries, then 2 cos(2 pi j / L), j < L
+    constexpr int CB = NMAX <= 32 ? 256 : 128;
+    extern __shared__ double ts_q[];   // CB x (nph + 1) table entries, then C: (nph + 1) x NMAX
     const int L = 2 * nph, ldq = nph + 1;
-    double* s2c = ts_q + 256 * ldq;
-    const int h = blockIdx.y, c0 = blockIdx.x * 256, t = threadIdx.x;
-    const int ncol = (h < G && c0 < G) ? min(256, G - c0) : 0;
-    for (int i = t; i < L; i += 256) s2c[i] = 2.0 * trig[i];
+    double* sC = ts_q + CB * ldq;
+    const int h = blockIdx.y, c0 = blockIdx.x * CB, t = threadIdx.x;
+    const int ncol = (h < G && c0 < G) ? min(CB, G - c0) : 0;
+    for (int i = t; i < (nph + 1) * NMAX; i += CB) {
+        const int k = i / NMAX, d = i - k * NMAX;
+        sC[i] = d == 0 ? 1.0 : (d < nph ? 2.0 * trig[(d * k) % L] : 0.0);
+    }
     if (ncol > 0) {
-        // all of a thread's loads (element t + 256 u, u < nph) in flight before the LDS stores
+        // all of a thread's loads (element t + CB u, u < nph) in flight before the LDS stores
         const double* src = Q + ((size_t)h * G + c0) * nph;
         const int n = ncol * nph;
-        double v[TOEP_MAXX];
+        double v[NMAX];
 #pragma unroll
-        for (int u = 0; u < TOEP_MAXX; u++) {
-            const int i = t + u * 256;
+        for (int u = 0; u < NMAX; u++) {
+            const int i = t + u * CB;
             v[u] = (u < nph && i < n) ? src[i] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < TOEP_MAXX; u++) {
-            const int i = t + u * 256;
+        for (int u = 0; u < NMAX; u++) {
+            const int i = t + u * CB;
             if (u < nph && i < n) {
                 const int c = i / nph;
                 ts_q[c * ldq + (i - c * nph)] = v[u];
@@ -1305,20 +1314,16 @@ __global__ __launch_bounds__(256) void that_sph_kernel(const double* __restrict_
     }
     __syncthreads();
     const bool live = t < ncol;
-    double q[TOEP_MAXX];
+    double q[NMAX];
 #pragma unroll
-    for (int d = 0; d < TOEP_MAXX; d++) q[d] = (live && d < nph) ? ts_q[t * ldq + d] : 0.0;
+    for (int d = 0; d < NMAX; d++) q[d] = (live && d < nph) ? ts_q[t * ldq + d] : 0.0;
     const int c = c0 + t;
     for (int k = 0; k <= nph; k++) {
-        double acc = q[0];
-        int j = 0;   // d k mod L
+        const double* Ck = sC + k * NMAX;
+        double acc = 0.0;
 #pragma unroll
-        for (int d = 1; d < TOEP_MAXX; d++) {
-            if (d >= nph) break;
-            j += k;
-            if (j >= L) j -= L;
-            acc += s2c[j] * q[d];
-        }
+        for (int d = 0; d < NMAX; d++)
+            if (d < nph) acc += Ck[d] * q[d];
         if constexpr (SPLIT) {
             if ((size_t)c < Gp) {
                 __bf16 x, y;
@@ -1331,6 +1336,30 @@ __global__ __launch_bounds__(256) void that_sph_kernel(const double* __restrict_
             T[((size_t)k * G + h) * G + c] = acc;
         }
     }
+}
+
+// That_k of one band: the instantiation for Nphi <= 32 holds half the registers.
+static int that_sph_cb(int nph) { return nph <= 32 ? 256 : 128; }
+static size_t that_sph_lds(int nph) {
+    return ((size_t)that_sph_cb(nph) * (nph + 1) + (size_t)(nph + 1) * (nph <= 32 ? 32 : TOEP_MAXX)) * 8;
+}
+template <bool SPLIT>
+static int that_sph_attr(opsec_ctx* ctx, int nph) {
+    const void* fn = nph <= 32 ? (const void*)that_sph_kernel<SPLIT, 32> : (const void*)that_sph_kernel<SPLIT, TOEP_MAXX>;
+    OPSEC_HIP_TRY(ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)that_sph_lds(nph)));
+    return OPSEC_OK;
+}
+template <bool SPLIT>
+static void launch_that_sph(opsec_ctx* ctx, const double* Q, int G, int nph, const double* trig, __bf16* hi,
+                            __bf16* lo, size_t Gr, size_t Gp, double* T) {
+    const size_t cb = that_sph_cb(nph), cols = SPLIT ? Gp : (size_t)G;
+    const dim3 grid((unsigned)((cols + cb - 1) / cb), SPLIT ? (unsigned)Gr : (unsigned)G);
+    if (nph <= 32)
+        hipLaunchKernelGGL((that_sph_kernel<SPLIT, 32>), grid, dim3((unsigned)cb), that_sph_lds(nph), ctx->stream, Q, G,
+                           nph, trig, hi, lo, Gr, Gp, T);
+    else
+        hipLaunchKernelGGL((that_sph_kernel<SPLIT, TOEP_MAXX>), grid, dim3((unsigned)cb), that_sph_lds(nph), ctx->stream,
+                           Q, G, nph, trig, hi, lo, Gr, Gp, T);
 }
 
 // A operand of the FP64 / FP32 Toeplitz stage 1 of a shell: batch b reads the dense
@@ -1603,7 +1632,6 @@ static int toep_check(opsec_ctx* ctx, const ToepGrid& tg) {
                          TOEP_MAXX, tg.sph ? "phi" : "x", tg.n1);
     return OPSEC_OK;
 }
-static size_t that_sph_lds(int n1) { return ((size_t)256 * (n1 + 1) + 2 * (size_t)n1) * 8; }
 
 static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
     ProjState* ps = ctx->proj;
@@ -1671,9 +1699,7 @@ static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
     const size_t id_lds = ((size_t)nbat * 128 + (size_t)((n1 + 3) & ~3) * nbat) * 4;
     OPSEC_HIP_TRY(ctx, hipFuncSetAttribute((const void*)idft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)id_lds));
-    if (tgrid.sph)
-        OPSEC_HIP_TRY(ctx, hipFuncSetAttribute((const void*)that_sph_kernel<true>,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)that_sph_lds(n1)));
+    if (tgrid.sph && (rc = that_sph_attr<true>(ctx, n1))) return rc;
     __bf16* Th = ps->d_That;
     __bf16* Tl = ps->d_That + (size_t)(n1 + 1) * plane_t;
     EpiYhat toY{ps->d_Yhat, M, G, nbat};
@@ -1694,8 +1720,7 @@ static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
             const double* Qf = ctx->d_Q + (size_t)funcs[z] * ctx->nkeys;
             hipEventRecord(evb[4 * z], ctx->stream);
             if (tgrid.sph) {
-                hipLaunchKernelGGL(that_sph_kernel<true>, dim3((unsigned)((Gp + 255) / 256), (unsigned)Gr), dim3(256),
-                                   that_sph_lds(n1), ctx->stream, Qf, G, n1, ps->d_trig, Th, Tl, Gr, Gp, nullptr);
+                launch_that_sph<true>(ctx, Qf, G, n1, ps->d_trig, Th, Tl, Gr, Gp, nullptr);
             } else {
                 hipLaunchKernelGGL(qhat_kernel, dim3((unsigned)(((n1 + 1) * G + 255) / 256)), dim3(256), 0,
                                    ctx->stream, Qf, n1, G, ps->d_trig, ps->d_Qhat);
@@ -1822,9 +1847,7 @@ static int run_project_toeplitz64(opsec_ctx* ctx, int nb, const int* funcs, int 
     const size_t id_lds = (size_t)nbat * n1 * 8;
     OPSEC_HIP_TRY(ctx, hipFuncSetAttribute((const void*)idft64_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)id_lds));
-    if (tgrid.sph)
-        OPSEC_HIP_TRY(ctx, hipFuncSetAttribute((const void*)that_sph_kernel<false>,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)that_sph_lds(n1)));
+    if (tgrid.sph && (rc = that_sph_attr<false>(ctx, n1))) return rc;
     std::vector<hipEvent_t> ev(4 * (size_t)nb);
     for (auto& e : ev) OPSEC_HIP_TRY(ctx, hipEventCreate(&e));
     const size_t np = (size_t)packed_size(M);
@@ -1850,9 +1873,7 @@ static int run_project_toeplitz64(opsec_ctx* ctx, int nb, const int* funcs, int 
         PutPacked outP{(double*)ps->d_P + (size_t)z * np, 0};
         hipEventRecord(ev[4 * z], ctx->stream);
         if (tgrid.sph)
-            hipLaunchKernelGGL(that_sph_kernel<false>, dim3((unsigned)((G + 255) / 256), (unsigned)G), dim3(256),
-                               that_sph_lds(n1), ctx->stream, Qf, G, n1, ps->d_trig64, nullptr, nullptr, 0, 0,
-                               ps->d_That64);
+            launch_that_sph<false>(ctx, Qf, G, n1, ps->d_trig64, nullptr, nullptr, 0, 0, ps->d_That64);
         else
             hipLaunchKernelGGL(qhat_kernel, dim3((unsigned)(((n1 + 1) * G + 255) / 256)), dim3(256), 0, ctx->stream,
                                Qf, n1, G, ps->d_trig64, ps->d_Qhat);

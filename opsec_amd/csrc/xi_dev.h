@@ -61,10 +61,20 @@ __device__ __forceinline__ double tophat_kernel(double x) {
     return (s - x * c) / (x * x * x);
 }
 
-// Closed-form top-hat band: (1/2pi^2) int_{klo}^{khi} k^2 j0(kr) dk.
+// Closed-form top-hat band: (1/2pi^2) int_{klo}^{khi} k^2 j0(kr) dk
+// = [sin kr - kr cos kr]_{klo}^{khi} / (2 pi^2 r^3).  With klo r >= 0.5 both
+// terms take the closed form over one division by r^3; below, k^3 times
+// tophat_kernel (its series near 0).
 __device__ __forceinline__ double band_xi(double klo, double khi, double r) {
     const double inv2pi2 = 0.050660591821168885;   // 1/(2 pi^2)
-    return inv2pi2 * (khi * khi * khi * tophat_kernel(khi * r) - klo * klo * klo * tophat_kernel(klo * r));
+    const double xh = khi * r, xl = klo * r;
+    if (xl >= 0.5) {
+        double sh, ch, sl, cl;
+        sincos(xh, &sh, &ch);
+        sincos(xl, &sl, &cl);
+        return inv2pi2 * ((sh - xh * ch) - (sl - xl * cl)) / (r * r * r);
+    }
+    return inv2pi2 * (khi * khi * khi * tophat_kernel(xh) - klo * klo * klo * tophat_kernel(xl));
 }
 
 __device__ __forceinline__ double xi_eval(const XiModelDev& m, const XiDesc& d, double r) {

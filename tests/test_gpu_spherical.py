@@ -111,13 +111,14 @@ def test_spherical_projection_all_precisions(ctx):
 
 
 @pytest.mark.parametrize("dims,drop,shuffle", [((17, 16, 4), 0.0, False), ((3, 5, 7), 0.3, True), ((4, 4, 1), 0.0, True),
-                                               ((2, 3, 2), 0.2, False)])
+                                               ((2, 3, 2), 0.2, False), ((2, 3, 40), 0.1, True)])
 def test_spherical_toeplitz_vs_dense(ctx, dims, drop, shuffle):
     """Toeplitz projection of a shell (length-2 Nphi DFT along phi, dense arc x arc
     That_k) against the dense FP64 projection of the same arc-pair tables: 272
     arcs (two column tiles of the That kernel, a padded 512-row GEMM tile), an
     odd Nphi with a third of the cells missing in shuffled order (table keys that
-    no pair carries), a single phi plane, and two planes."""
+    no pair carries), a single phi plane, two planes, and 40 planes (the kernels
+    for 32 < Nphi <= 64)."""
     cells, _ = _shell(ctx, dims, drop, nb=2, shuffle=shuffle)
     funcs = [0, 1, 2]
     ctx.pair_tables(funcs, ctx.quad(lib.OPSEC_QUAD_GM))

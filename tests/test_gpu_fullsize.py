@@ -14,7 +14,7 @@ C3: the surrogate trained on 2 x 10^5 exact pair integrals of the 32^3 box,
 C5: the full spherical shell, 64 x 32 x 32 = 65536 cells of uneven volume,
     50 log bands: the arc-pair tables (xi = 1 known answer, sampled entries
     against the oracle's ComputeSignalMatrixBlockS restatement), klt, and the
-    FP64 against the split-BF16x3 projections.
+    FP64 against the split-BF16x3 projections, dense and Toeplitz along phi.
 """
 import ctypes as C
 
@@ -232,3 +232,21 @@ def test_c5_full_shell_tables_klt_and_projections(ctx, shell64):
     err = np.abs(Pm - P64).max(axis=1) / np.abs(P64).max(axis=1)
     print(f"C5 split-BF16x3 vs FP64 projection, 50 bands: max rel err {err.max():.2e}", flush=True)
     assert err.max() <= 5e-5
+    # the Toeplitz projection (length-64 DFT along phi, 2048 x 2048 arc GEMMs): the prior's
+    # B^T S B = diag(lambda), and the bands against the dense FP64 products
+    ctx.set_projection(lib.OPSEC_PROJ_TOEPLITZ)
+    try:
+        P0t = ctx.project([0], lib.OPSEC_PREC_FP64)[0]
+        Pt64 = ctx.project(bands, lib.OPSEC_PREC_FP64)
+        Ptm = ctx.project(bands, lib.OPSEC_PREC_BF16X3)
+    finally:
+        ctx.set_projection(lib.OPSEC_PROJ_DENSE)
+    off = P0t.copy()
+    off[d] = 0.0
+    assert np.abs(P0t[d] - lam).max() <= 1e-11 * lam[0]
+    assert np.abs(off).max() <= 1e-11 * lam[0]
+    e64 = (np.abs(Pt64 - P64).max(axis=1) / np.abs(P64).max(axis=1)).max()
+    em = (np.abs(Ptm - P64).max(axis=1) / np.abs(P64).max(axis=1)).max()
+    print(f"C5 Toeplitz projection vs dense FP64, 50 bands: FP64 {e64:.2e}, split-BF16x3 {em:.2e}", flush=True)
+    assert e64 <= 1e-12
+    assert em <= 5e-5

@@ -96,7 +96,7 @@ struct ProjState {
     size_t cap_Bhat64 = 0;
     double* d_Yhat64 = nullptr;  // one band's stage-1 products: nbat x M x G
     size_t cap_Yhat64 = 0;
-    double* d_Y64 = nullptr;     // one band's Y = S_p B' in grid order: M x Ng (pitch ldg)
+    double* d_Y64 = nullptr;     // Y = S_p B' in grid order, up to four bands: M x Ng each (pitch ldg)
     size_t cap_Y64 = 0;
     double* d_That64 = nullptr;  // one band's That_k of a shell: (Nphi + 1) x G x G
     size_t cap_That64 = 0;
@@ -1800,8 +1800,12 @@ static int run_project_toeplitz64(opsec_ctx* ctx, int nb, const int* funcs, int 
     if (!tgrid.sph && (sv.n2 >= 65536 || sv.n3 >= 65536))
         return set_error(ctx, OPSEC_EINVAL, "Toeplitz projection: Ny, Nz < 65536");
     const size_t ldg = (size_t)Ng + 32;   // pitch of the grid-order operands (not a power of two)
+    // stage 2 runs once per group of up to TOEP_GROUP bands (grid.z = band): the upper
+    // tiles of one band alone fill the CUs unevenly (528 at 4000 modes: 2.06 rounds of 256)
+    const int grp = std::min(nb, TOEP_GROUP);
+    const size_t ystride = (size_t)M * ldg;
     if ((rc = dev_realloc(ctx, (void**)&ps->d_Yhat64, &ps->cap_Yhat64, (size_t)nbat * M * G * 8))) return rc;
-    if ((rc = dev_realloc(ctx, (void**)&ps->d_Y64, &ps->cap_Y64, (size_t)M * ldg * 8))) return rc;
+    if ((rc = dev_realloc(ctx, (void**)&ps->d_Y64, &ps->cap_Y64, (size_t)grp * ystride * 8))) return rc;
     if (tgrid.sph) {
         if ((rc = dev_realloc(ctx, (void**)&ps->d_That64, &ps->cap_That64, (size_t)(n1 + 1) * G * G * 8))) return rc;
     } else if ((rc = dev_realloc(ctx, (void**)&ps->d_Qhat, &ps->cap_Qhat, (size_t)(n1 + 1) * G * 8))) {
@@ -1848,8 +1852,12 @@ static int run_project_toeplitz64(opsec_ctx* ctx, int nb, const int* funcs, int 
     OPSEC_HIP_TRY(ctx, hipFuncSetAttribute((const void*)idft64_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)id_lds));
     if (tgrid.sph && (rc = that_sph_attr<false>(ctx, n1))) return rc;
-    std::vector<hipEvent_t> ev(4 * (size_t)nb);
+    const int ngroups = (nb + grp - 1) / grp;
+    // events per band: start, That / Qhat generated, stage 1 and inverse DFT; per group: stage 2
+    std::vector<hipEvent_t> ev(3 * (size_t)nb + ngroups);
     for (auto& e : ev) OPSEC_HIP_TRY(ctx, hipEventCreate(&e));
+    hipEvent_t* evb = ev.data();
+    hipEvent_t* evg = ev.data() + 3 * (size_t)nb;
     const size_t np = (size_t)packed_size(M);
     GA_That A;
     A.qhat = ps->d_Qhat;
@@ -1867,63 +1875,75 @@ static int run_project_toeplitz64(opsec_ctx* ctx, int nb, const int* funcs, int 
     A2.p = ps->d_Bg64;
     A2.ld = ldg;
     GB_ColMajorT<double> B2{ps->d_Y64, ldg};
-    const dim3 g1 = grid_for(G, M, nbat), g2 = grid_for(M, M, 1);
-    for (int z = 0; z < nb; z++) {
-        const double* Qf = ctx->d_Q + (size_t)funcs[z] * ctx->nkeys;
-        PutPacked outP{(double*)ps->d_P + (size_t)z * np, 0};
-        hipEventRecord(ev[4 * z], ctx->stream);
-        if (tgrid.sph)
-            launch_that_sph<false>(ctx, Qf, G, n1, ps->d_trig64, nullptr, nullptr, 0, 0, ps->d_That64);
-        else
-            hipLaunchKernelGGL(qhat_kernel, dim3((unsigned)(((n1 + 1) * G + 255) / 256)), dim3(256), 0, ctx->stream,
-                               Qf, n1, G, ps->d_trig64, ps->d_Qhat);
-        hipEventRecord(ev[4 * z + 1], ctx->stream);
-        const size_t tplane = (size_t)G * G;
-        if (tgrid.sph && prec == OPSEC_PREC_FP64)
-            hipLaunchKernelGGL((gemm_kernel<double, GA_ThatDense, GB_ColMajor, PutColMajor<double>, false>), g1,
-                               dim3(GT_THREADS), 0, ctx->stream, As, tplane, Bop, (size_t)M * G, toYhat, G, M, G);
-        else if (tgrid.sph)
-            hipLaunchKernelGGL((gemm_kernel<float, GA_ThatDense, GB_ColMajor, PutColMajor<double>, false>), g1,
-                               dim3(GT_THREADS), 0, ctx->stream, As, tplane, Bop, (size_t)M * G, toYhat, G, M, G);
-        else if (prec == OPSEC_PREC_FP64)
-            hipLaunchKernelGGL((gemm_kernel<double, GA_That, GB_ColMajor, PutColMajor<double>, false>), g1,
-                               dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, (size_t)M * G, toYhat, G, M, G);
-        else
-            hipLaunchKernelGGL((gemm_kernel<float, GA_That, GB_ColMajor, PutColMajor<double>, false>), g1,
-                               dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, (size_t)M * G, toYhat, G, M, G);
-        hipLaunchKernelGGL(idft64_kernel, dim3((unsigned)((G + 255) / 256), (unsigned)M), dim3(256), id_lds,
-                           ctx->stream, ps->d_Yhat64, nbat, G, n1, M, ps->d_W64, ps->d_Y64, ldg);
-        hipEventRecord(ev[4 * z + 2], ctx->stream);
+    const dim3 g1 = grid_for(G, M, nbat);
+    for (int z0 = 0, gi = 0; z0 < nb; z0 += grp, gi++) {
+        const int g = std::min(grp, nb - z0);
+        for (int q = 0; q < g; q++) {
+            const int z = z0 + q;
+            const double* Qf = ctx->d_Q + (size_t)funcs[z] * ctx->nkeys;
+            hipEventRecord(evb[3 * z], ctx->stream);
+            if (tgrid.sph)
+                launch_that_sph<false>(ctx, Qf, G, n1, ps->d_trig64, nullptr, nullptr, 0, 0, ps->d_That64);
+            else
+                hipLaunchKernelGGL(qhat_kernel, dim3((unsigned)(((n1 + 1) * G + 255) / 256)), dim3(256), 0,
+                                   ctx->stream, Qf, n1, G, ps->d_trig64, ps->d_Qhat);
+            hipEventRecord(evb[3 * z + 1], ctx->stream);
+            const size_t tplane = (size_t)G * G;
+            if (tgrid.sph && prec == OPSEC_PREC_FP64)
+                hipLaunchKernelGGL((gemm_kernel<double, GA_ThatDense, GB_ColMajor, PutColMajor<double>, false>), g1,
+                                   dim3(GT_THREADS), 0, ctx->stream, As, tplane, Bop, (size_t)M * G, toYhat, G, M, G);
+            else if (tgrid.sph)
+                hipLaunchKernelGGL((gemm_kernel<float, GA_ThatDense, GB_ColMajor, PutColMajor<double>, false>), g1,
+                                   dim3(GT_THREADS), 0, ctx->stream, As, tplane, Bop, (size_t)M * G, toYhat, G, M, G);
+            else if (prec == OPSEC_PREC_FP64)
+                hipLaunchKernelGGL((gemm_kernel<double, GA_That, GB_ColMajor, PutColMajor<double>, false>), g1,
+                                   dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, (size_t)M * G, toYhat, G, M, G);
+            else
+                hipLaunchKernelGGL((gemm_kernel<float, GA_That, GB_ColMajor, PutColMajor<double>, false>), g1,
+                                   dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, (size_t)M * G, toYhat, G, M, G);
+            hipLaunchKernelGGL(idft64_kernel, dim3((unsigned)((G + 255) / 256), (unsigned)M), dim3(256), id_lds,
+                               ctx->stream, ps->d_Yhat64, nbat, G, n1, M, ps->d_W64, ps->d_Y64 + (size_t)q * ystride,
+                               ldg);
+            hipEventRecord(evb[3 * z + 2], ctx->stream);
+        }
+        // stage 2 of the group's bands in one launch: batch q = band z0 + q
+        PutPacked outP{(double*)ps->d_P + (size_t)z0 * np, np};
+        const dim3 g2 = grid_for(M, M, g);
         if (prec == OPSEC_PREC_FP64)
             hipLaunchKernelGGL((gemm_kernel<double, GA_ColMajorT, GB_ColMajorT<double>, PutPacked, true>), g2,
-                               dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, 0, outP, M, M, Ng);
+                               dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, ystride, outP, M, M, Ng);
         else
             hipLaunchKernelGGL((gemm_kernel<float, GA_ColMajorT, GB_ColMajorT<double>, PutPacked, true>), g2,
-                               dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, 0, outP, M, M, Ng);
-        hipEventRecord(ev[4 * z + 3], ctx->stream);
+                               dim3(GT_THREADS), 0, ctx->stream, A2, 0, B2, ystride, outP, M, M, Ng);
+        hipEventRecord(evg[gi], ctx->stream);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = set_error(ctx, OPSEC_EHIP, "projection launch: %s", hipGetErrorString(e));
-    hipError_t es = hipEventSynchronize(ev[4 * nb - 1]);
+    hipError_t es = hipEventSynchronize(evg[ngroups - 1]);
     if (!rc && es != hipSuccess) rc = set_error(ctx, OPSEC_EHIP, "projection: %s", hipGetErrorString(es));
     float tg = 0, t1 = 0, t2 = 0;
-    if (!rc)
+    if (!rc) {
         for (int z = 0; z < nb; z++) {
-            float a, b, c;
-            hipEventElapsedTime(&a, ev[4 * z], ev[4 * z + 1]);
-            hipEventElapsedTime(&b, ev[4 * z + 1], ev[4 * z + 2]);
-            hipEventElapsedTime(&c, ev[4 * z + 2], ev[4 * z + 3]);
+            float a, b;
+            hipEventElapsedTime(&a, evb[3 * z], evb[3 * z + 1]);
+            hipEventElapsedTime(&b, evb[3 * z + 1], evb[3 * z + 2]);
             tg += a;
             t1 += b;
+        }
+        for (int gi = 0; gi < ngroups; gi++) {
+            const int zl = std::min(nb, (gi + 1) * grp) - 1;   // last band of the group
+            float c;
+            hipEventElapsedTime(&c, evb[3 * zl + 2], evg[gi]);
             t2 += c;
         }
+    }
     for (auto& x : ev) hipEventDestroy(x);
     if (rc) return rc;
     ps->last_ms_gen = tg;
     ps->last_ms_gemm1 = t1;     // stage-1 GEMMs and the inverse DFT
     ps->last_ms_gemm2 = t2;
     ps->last_launches = nb;
-    ps->last_launches2 = nb;
+    ps->last_launches2 = ngroups;
     ps->last_ms_stage1 = tg + t1;
     ps->last_ms_stage2 = t2;
     ps->nb = nb;

@@ -143,6 +143,25 @@ def test_spherical_toeplitz_vs_dense(ctx, dims, drop, shuffle):
     assert rel(got) < 5e-5
 
 
+def test_overlapped_sweeps_give_the_same_tables(ctx):
+    """The prior's sweep on a second stream beside the bands' sweep (the default)
+    and after it give the same bits in every table."""
+    cells, _ = _shell(ctx, (4, 3, 8), 0.15, nb=3, shuffle=True)
+    q = ctx.quad(lib.OPSEC_QUAD_GM)
+    funcs = [0, 1, 2, 3]
+    ctx.pair_tables(funcs, q)
+    overlapped = [ctx.pair_table(f) for f in funcs]
+    try:
+        assert ctx.lib.opsec_diag_set_gm_overlap(0) == 0
+        ctx.pair_tables(funcs, q)
+        serial = [ctx.pair_table(f) for f in funcs]
+    finally:
+        ctx.lib.opsec_diag_set_gm_overlap(1)
+    for f, a, b in zip(funcs, overlapped, serial):
+        assert np.array_equal(a, b), f
+    assert ctx.lib.opsec_diag_set_gm_overlap(2) == -1   # OPSEC_EINVAL
+
+
 def test_spherical_rejects_tent_rule(ctx):
     _shell(ctx, (2, 2, 4))
     with pytest.raises(lib.OpsecError, match="adaptive"):

@@ -1231,8 +1231,10 @@ struct GA_That {
     }
 };
 
-// Y[m][x G + r] = sum_b W[b][x] Yhat[b][m][r] for x < n1 (FP64).  Block (r
-// tile, m); W (nbat x n1) in LDS, the n1 sums of a thread in registers.
+// Y[m][x G + r] = sum_b W[b][x] Yhat[b][m][r] for x < n1 (FP64), summed in b
+// order.  Block (r tile, m); W (nbat x n1) in LDS, the n1 <= NMAX sums of a
+// thread in registers; the Yhat values of eight batches are loaded together.
+template <int NMAX>
 __global__ __launch_bounds__(256) void idft64_kernel(const double* __restrict__ Yhat, int nbat, int G, int n1, int M,
                                                      const double* __restrict__ W, double* __restrict__ Y, size_t ldg) {
     extern __shared__ double id64_w[];   // nbat x n1
@@ -1240,18 +1242,24 @@ __global__ __launch_bounds__(256) void idft64_kernel(const double* __restrict__ 
     for (int i = threadIdx.x; i < nbat * n1; i += 256) id64_w[i] = W[i];
     __syncthreads();
     if (r >= G) return;
-    double acc[TOEP_MAXX];
+    double acc[NMAX];
 #pragma unroll
-    for (int x = 0; x < TOEP_MAXX; x++) acc[x] = 0.0;
-    for (int b = 0; b < nbat; b++) {
-        const double yh = Yhat[((size_t)b * M + m) * G + r];
-        const double* w = id64_w + b * n1;
+    for (int x = 0; x < NMAX; x++) acc[x] = 0.0;
+    for (int b0 = 0; b0 < nbat; b0 += 8) {
+        double yh[8];
 #pragma unroll
-        for (int x = 0; x < TOEP_MAXX; x++)
-            if (x < n1) acc[x] += w[x] * yh;
+        for (int u = 0; u < 8; u++) yh[u] = b0 + u < nbat ? Yhat[((size_t)(b0 + u) * M + m) * G + r] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (b0 + u >= nbat) break;
+            const double* w = id64_w + (b0 + u) * n1;
+#pragma unroll
+            for (int x = 0; x < NMAX; x++)
+                if (x < n1) acc[x] += w[x] * yh[u];
+        }
     }
 #pragma unroll
-    for (int x = 0; x < TOEP_MAXX; x++)
+    for (int x = 0; x < NMAX; x++)
         if (x < n1) Y[(size_t)m * ldg + (size_t)x * G + r] = acc[x];
 }
 
@@ -1849,8 +1857,8 @@ static int run_project_toeplitz64(opsec_ctx* ctx, int nb, const int* funcs, int 
         ps->toep64_valid = true;
     }
     const size_t id_lds = (size_t)nbat * n1 * 8;
-    OPSEC_HIP_TRY(ctx, hipFuncSetAttribute((const void*)idft64_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)id_lds));
+    OPSEC_HIP_TRY(ctx, hipFuncSetAttribute(n1 <= 32 ? (const void*)idft64_kernel<32> : (const void*)idft64_kernel<TOEP_MAXX>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)id_lds));
     if (tgrid.sph && (rc = that_sph_attr<false>(ctx, n1))) return rc;
     const int ngroups = (nb + grp - 1) / grp;
     // events per band: start, That / Qhat generated, stage 1 and inverse DFT; per group: stage 2
@@ -1901,9 +1909,14 @@ static int run_project_toeplitz64(opsec_ctx* ctx, int nb, const int* funcs, int 
             else
                 hipLaunchKernelGGL((gemm_kernel<float, GA_That, GB_ColMajor, PutColMajor<double>, false>), g1,
                                    dim3(GT_THREADS), 0, ctx->stream, A, 0, Bop, (size_t)M * G, toYhat, G, M, G);
-            hipLaunchKernelGGL(idft64_kernel, dim3((unsigned)((G + 255) / 256), (unsigned)M), dim3(256), id_lds,
-                               ctx->stream, ps->d_Yhat64, nbat, G, n1, M, ps->d_W64, ps->d_Y64 + (size_t)q * ystride,
-                               ldg);
+            if (n1 <= 32)
+                hipLaunchKernelGGL(idft64_kernel<32>, dim3((unsigned)((G + 255) / 256), (unsigned)M), dim3(256), id_lds,
+                                   ctx->stream, ps->d_Yhat64, nbat, G, n1, M, ps->d_W64,
+                                   ps->d_Y64 + (size_t)q * ystride, ldg);
+            else
+                hipLaunchKernelGGL(idft64_kernel<TOEP_MAXX>, dim3((unsigned)((G + 255) / 256), (unsigned)M), dim3(256),
+                                   id_lds, ctx->stream, ps->d_Yhat64, nbat, G, n1, M, ps->d_W64,
+                                   ps->d_Y64 + (size_t)q * ystride, ldg);
             hipEventRecord(evb[3 * z + 2], ctx->stream);
         }
         // stage 2 of the group's bands in one launch: batch q = band z0 + q

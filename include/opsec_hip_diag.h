@@ -63,6 +63,12 @@ int opsec_diag_set_g3_dl(int v);
  * on a second stream from a second host thread, beside the band functions'
  * sweep (1, the default), or after it (0).  The tables are the same bits. */
 int opsec_diag_set_gm_overlap(int v);
+/* Pair tables of a spherical survey, the sweep of closed-form bands: first
+ * regions of every band of an arc pair in one block, the pair's geometry and
+ * the band-edge terms sin kr - kr cos kr formed once and shared by adjacent
+ * bands (1, the default), or one thread per (pair, band) job (0).  The same
+ * bits either way. */
+int opsec_diag_set_gm_bands(int v);
 
 #ifdef __cplusplus
 }

@@ -639,6 +639,132 @@ __global__ __launch_bounds__(256) void gm_first_kernel(XiModelDev m, const GMJob
     if (threadIdx.x == 0 && blk) atomicAdd(nevals, blk);
 }
 
+// First regions of a shell's band sweep, GMB_PAIRS arc pairs per block for
+// every band of the sweep at once.  A pair's region table, its 149 node
+// separations r and r^3 are formed once instead of once per band, and the
+// band-edge term F_k(r) = sin kr - kr cos kr is shared by the two bands that
+// meet at edge k: one sincos per band and node instead of two when the sweep's
+// bands follow one another.  A node value is band_xi's, bit for bit (the same
+// operations; its series branch, klo r < 0.5, calls band_xi), and each band's
+// rule is summed by one thread in gm_first_kernel's node order, so the results,
+// the convergence test and the pending jobs (job t npairs + i for band t of pair
+// i) are those of gm_first_kernel.  Thread q GM_NPTS + n forms node n of pair
+// slot q; the bands go through LDS GMB_CHUNK at a time.
+constexpr int GMB_PAIRS = 3, GMB_CHUNK = 8;
+constexpr int GMB_THREADS = 448;   // >= GMB_PAIRS * GM_NPTS, whole wavefronts
+__global__ __launch_bounds__(GMB_THREADS) void gm_first_bands_kernel(
+    XiModelDev m, const GMJob* __restrict__ tmpl, int npairs, const int* __restrict__ funcs, int nf,
+    const double* __restrict__ edge, const int* __restrict__ fedge, GMParams P, double* Q, size_t qstride,
+    double* Qerr, int* jobfail, int* pending, int* npending, unsigned long long* nevals) {
+    __shared__ double tab[GMB_PAIRS][SPH_TAB];
+    __shared__ double vals[GMB_PAIRS][GMB_CHUNK][GM_NPTS];
+    __shared__ unsigned long long blk;
+    const int t = threadIdx.x;
+    const int q = t / GM_NPTS, n = t - q * GM_NPTS;
+    const int i0 = blockIdx.x * GMB_PAIRS;
+    const bool live = q < GMB_PAIRS && i0 + q < npairs;
+    if (t == 0) blk = 0;
+    for (int k = t; k < GMB_PAIRS * SPH_TAB; k += GMB_THREADS) {
+        const int qq = k / SPH_TAB, e = k - qq * SPH_TAB;
+        if (i0 + qq < npairs) {
+            const GMJob J = tmpl[i0 + qq];
+            double c[6], hw[6];
+            for (int d = 0; d < 6; d++) {
+                c[d] = 0.5;
+                hw[d] = 0.5;
+            }
+            tab[qq][e] = sph_entry(J, c, hw, e);
+        }
+    }
+    __syncthreads();
+    double r = 0.0, r3 = 0.0;
+    if (live) {   // the expression of sph_node_value
+        int vi[6];
+        gm_node_vi(n, vi);
+        const double* T = tab[q];
+        const double r1 = T[SphIdx::R1 + vi[0]], r2 = T[SphIdx::R2 + vi[1]];
+        const double cosg = T[SphIdx::SQ + sph_cmb(vi[2], vi[3])] * T[SphIdx::CS + sph_cmb(vi[4], vi[5])] +
+                            T[SphIdx::M1 + vi[2]] * T[SphIdx::M2 + vi[3]];
+        r = sqrt(fmax(r1 * r1 + r2 * r2 - 2 * r1 * r2 * cosg, 0.0));
+        r3 = r * r * r;
+    }
+    int ce = -1;        // edge of the last closed-form term of this node
+    double cF = 0.0;
+    for (int f0 = 0; f0 < nf; f0 += GMB_CHUNK) {
+        const int nc = min(GMB_CHUNK, nf - f0);
+        if (live)
+            for (int u = 0; u < nc; u++) {
+                const int ilo = fedge[2 * (f0 + u)], ihi = fedge[2 * (f0 + u) + 1];
+                const double klo = edge[ilo], khi = edge[ihi];
+                const double xl = klo * r;
+                double v;
+                if (xl >= 0.5) {   // band_xi's closed form
+                    double Fl;
+                    if (ce == ilo) {
+                        Fl = cF;
+                    } else {
+                        double sl, cl;
+                        sincos(xl, &sl, &cl);
+                        Fl = sl - xl * cl;
+                    }
+                    const double xh = khi * r;
+                    double sh, ch;
+                    sincos(xh, &sh, &ch);
+                    const double Fh = sh - xh * ch;
+                    v = 0.050660591821168885 * (Fh - Fl) / r3;
+                    ce = ihi;
+                    cF = Fh;
+                } else {
+                    v = band_xi(klo, khi, r);
+                    ce = -1;
+                }
+                vals[q][u][n] = v;
+            }
+        __syncthreads();
+        if (t < GMB_PAIRS * GMB_CHUNK) {
+            const int qq = t / GMB_CHUNK, u = t - qq * GMB_CHUNK, ii = i0 + qq;
+            if (u < nc && ii < npairs) {
+                const double* V = vals[qq][u];
+                const double w1 = (double)(12824 - 9120 * 6 + 400 * 36) / 19683.0;
+                const double w2 = 980. / 6561.;
+                const double w3 = (double)(1820 - 400 * 6) / 19683.0;
+                const double w4 = 200. / 19683.;
+                const double w5 = 6859.0 / 19683.0 / 64.0;
+                const double e1 = (double)(729 - 950 * 6 + 50 * 36) / 729.0;
+                const double e2 = 245. / 486.;
+                const double e3 = (double)(265 - 100 * 6) / 1458.0;
+                const double e4 = 25. / 729.;
+                const double val0 = V[0];
+                double sum2 = 0, sum3 = 0, sum4 = 0, sum5 = 0;
+                for (int k = 0; k < 6; k++) {
+                    sum2 += V[1 + 4 * k] + V[2 + 4 * k];
+                    sum3 += V[3 + 4 * k] + V[4 + 4 * k];
+                }
+                for (int p = 25; p < 85; p++) sum4 += V[p];
+                for (int p = 85; p < GM_NPTS; p++) sum5 += V[p];
+                double vol = 1;
+                for (int d = 0; d < 6; d++) vol *= 2 * 0.5;
+                const double result = vol * (w1 * val0 + w2 * sum2 + w3 * sum3 + w4 * sum4 + w5 * sum5);
+                const double res5th = vol * (e1 * val0 + e2 * sum2 + e3 * sum3 + e4 * sum4);
+                const double err = fabs(res5th - result);
+                const double tv = 0.0 + result, te = 0.0 + err;
+                const int jb = (f0 + u) * npairs + ii;
+                if (!((unsigned)GM_NPTS < P.maxEval || !P.maxEval) || gm_converged(tv, te, P)) {
+                    const size_t o = (size_t)funcs[f0 + u] * qstride + tmpl[ii].key;
+                    Q[o] = tv;
+                    if (Qerr) Qerr[o] = te;
+                    jobfail[jb] = 0;
+                    atomicAdd(&blk, (unsigned long long)GM_NPTS);
+                } else {
+                    pending[atomicAdd(npending, 1)] = jb;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0 && blk) atomicAdd(nevals, blk);
+}
+
 // The rule on region ri of S evaluated by one thread: node values summed on the
 // fly in the order of gm_region_sum.
 __device__ void gm_thread_region(const XiModelDev& m, const XiDesc& xd, const GMJob& J, const GMParams& P, GMSlot& S,
@@ -1029,12 +1155,15 @@ PairGeo grid_pair_geo(const SurveyDev& sv, int kx, int ky, int kz) {
 // Device buffers of run_gm_device, kept across the calls of one pair-table
 // sweep (its chunks) so the multi-GB job lists are allocated once.
 struct GMScratch {
-    Scratch sj, sd, si, sc, sf, sp, st, sx, sa;
+    Scratch sj, sd, si, sc, sf, sp, st, sx, sa, se;
 };
 static std::mutex g_gm_stats_mutex;
 // Shell tables: the deep sweep beside the shallow one on a second stream (1, the
 // default) or after it (0).
 static int g_gm_overlap = 1;
+// Shell band sweeps: first regions by gm_first_bands_kernel (1, the default) or
+// by gm_first_kernel (0); the same results.
+static int g_gm_bands = 1;
 
 static int run_gm_device(opsec_ctx* ctx, int nf, const int* funcs, const GMJob* d_tmpl, size_t npairs,
                          const opsec_quad* q, double* dQ, size_t nkeys, double* dQerr, GMScratch* gs = nullptr,
@@ -1053,6 +1182,7 @@ static int run_gm_device(opsec_ctx* ctx, int nf, const int* funcs, const GMJob* 
     Scratch& sp = S.sp;
     Scratch& st = S.st;
     Scratch& sx = S.sx;
+    Scratch& se = S.se;
     GMParams P;
     P.absTol = q->epsrel;   // reference argument order, src/sig.cpp:427-430
     P.relTol = q->epsabs;
@@ -1065,6 +1195,29 @@ static int run_gm_device(opsec_ctx* ctx, int nf, const int* funcs, const GMJob* 
     // evaluation cap allows (maxeval / 149 + 2), on fewer slots.
     const size_t njobs_all = npairs * (size_t)nf;
     if (njobs_all > (size_t)INT32_MAX) return set_error(ctx, OPSEC_EINVAL, "too many pair integrals in one call");
+    // A shell's sweep of closed-form bands takes its first regions from
+    // gm_first_bands_kernel: the distinct band edges, and each band's two edges
+    // as indices into them.
+    bool bands = g_gm_bands && ctx->sv.coordsys == OPSEC_COORD_SPHERICAL;
+    for (int t = 0; t < nf && bands; t++) bands = ctx->funcs[funcs[t]].kind == XI_BAND;
+    std::vector<double> edges;
+    std::vector<int> fedge;
+    if (bands) {
+        for (int t = 0; t < nf; t++) {
+            edges.push_back(ctx->funcs[funcs[t]].a);
+            edges.push_back(ctx->funcs[funcs[t]].b);
+        }
+        std::sort(edges.begin(), edges.end());
+        edges.erase(std::unique(edges.begin(), edges.end()), edges.end());
+        for (int t = 0; t < nf; t++)
+            for (double k : {ctx->funcs[funcs[t]].a, ctx->funcs[funcs[t]].b})
+                fedge.push_back((int)(std::lower_bound(edges.begin(), edges.end(), k) - edges.begin()));
+        int rc;
+        if ((rc = dev_realloc(ctx, &se.p, &se.cap, edges.size() * 8 + fedge.size() * 4))) return rc;
+        OPSEC_HIP_TRY(ctx, hipMemcpyAsync(se.p, edges.data(), edges.size() * 8, hipMemcpyHostToDevice, stm));
+        OPSEC_HIP_TRY(ctx, hipMemcpyAsync((double*)se.p + edges.size(), fedge.data(), fedge.size() * 4,
+                                          hipMemcpyHostToDevice, stm));
+    }
     unsigned long long nev_total = 0;
     int nagain = 0;             // pass-1 jobs, gathered on the device into sa
     for (int pass = 0; pass < 2; pass++) {
@@ -1107,9 +1260,16 @@ static int run_gm_device(opsec_ctx* ctx, int nf, const int* funcs, const GMJob* 
             OPSEC_HIP_TRY(ctx, hipGetLastError());
             {
                 // first regions, then the shallow integrals one thread each
-                hipLaunchKernelGGL(gm_first_kernel, dim3((unsigned)((njobs + 255) / 256)), dim3(256), 0,
-                                   stm, mv, (const GMJob*)sj.p, njobs, P, dQ, nkeys, dQerr, (int*)sf.p,
-                                   (int*)sx.p, d_npend, d_nev);
+                if (bands)
+                    hipLaunchKernelGGL(gm_first_bands_kernel,
+                                       dim3((unsigned)((npairs + GMB_PAIRS - 1) / GMB_PAIRS)), dim3(GMB_THREADS), 0,
+                                       stm, mv, d_tmpl, (int)npairs, (const int*)st.p, nf, (const double*)se.p,
+                                       (const int*)((const double*)se.p + edges.size()), P, dQ, nkeys, dQerr,
+                                       (int*)sf.p, (int*)sx.p, d_npend, d_nev);
+                else
+                    hipLaunchKernelGGL(gm_first_kernel, dim3((unsigned)((njobs + 255) / 256)), dim3(256), 0,
+                                       stm, mv, (const GMJob*)sj.p, njobs, P, dQ, nkeys, dQerr, (int*)sf.p,
+                                       (int*)sx.p, d_npend, d_nev);
                 OPSEC_HIP_TRY(ctx, hipGetLastError());
                 int npend = 0;
                 OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&npend, d_npend, 4, hipMemcpyDeviceToHost, stm));
@@ -2001,6 +2161,14 @@ int opsec_pair_integral(opsec_ctx* ctx, const opsec_cell* c1, const opsec_cell* 
 int opsec_diag_set_gm_overlap(int v) {
     if (v != 0 && v != 1) return OPSEC_EINVAL;
     g_gm_overlap = v;
+    return OPSEC_OK;
+}
+
+// Shell band sweeps: first regions with shared band-edge terms (1) or one
+// thread per (pair, band) job (0).
+int opsec_diag_set_gm_bands(int v) {
+    if (v != 0 && v != 1) return OPSEC_EINVAL;
+    g_gm_bands = v;
     return OPSEC_OK;
 }
 

@@ -151,6 +151,7 @@ _DIAG_SIGS = {
     "opsec_diag_set_nt_order": ([_I, _I], _I),
     "opsec_diag_set_g3_dl": ([_I], _I),
     "opsec_diag_set_gm_overlap": ([_I], _I),
+    "opsec_diag_set_gm_bands": ([_I], _I),
 }
 
 

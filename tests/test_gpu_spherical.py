@@ -162,6 +162,47 @@ def test_overlapped_sweeps_give_the_same_tables(ctx):
     assert ctx.lib.opsec_diag_set_gm_overlap(2) == -1   # OPSEC_EINVAL
 
 
+def test_band_first_regions_vs_per_job_kernel_and_oracle(ctx):
+    """Closed-form band tables of a shell: the first-region kernel that forms a
+    pair's geometry once and shares the band-edge terms between adjacent bands
+    gives the bits of the one-thread-per-job kernel, and agrees with the
+    oracle's arc-cache block to 1e-12 (1e-11 of the largest entry for the
+    entries that cancel; 12 log bands: two chunks of the kernel's band loop;
+    near and self pairs take band_xi's series branch)."""
+    nr, nmu, nphi = 3, 3, 6
+    cells = synth.drop_cells(synth.shell_cells(nr, nmu, nphi, rmin=RMIN, rmax=RMAX), 0.1, seed=11)
+    sv = lib.survey_sph(nr, nmu, nphi, RMIN, RMAX)
+    edges = np.logspace(np.log10(0.003), np.log10(0.3), 13)
+    blo, bhi = edges[:-1], edges[1:]
+    ctx.model_real(GOLD["pk_k"], GOLD["pk_P"], blo, bhi, lib.OPSEC_XI_CLOSED)
+    ctx.cells_upload(cells, sv)
+    q = ctx.quad(lib.OPSEC_QUAD_GM)
+    funcs = list(range(13))
+    ctx.pair_tables(funcs, q)
+    shared = [ctx.pair_table(f) for f in funcs]
+    N = len(cells)
+    S_gpu = {p: ctx.assemble(1 + p, N) for p in (0, 6, 11)}
+    try:
+        assert ctx.lib.opsec_diag_set_gm_bands(0) == 0
+        ctx.pair_tables(funcs, q)
+        per_job = [ctx.pair_table(f) for f in funcs]
+    finally:
+        ctx.lib.opsec_diag_set_gm_bands(1)
+    for f in funcs:
+        assert np.array_equal(shared[f], per_job[f]), f
+    assert ctx.lib.opsec_diag_set_gm_bands(2) == -1   # OPSEC_EINVAL
+    rows = np.arange(N, dtype=np.int32)
+    for p, got in S_gpu.items():
+        xi = O.XiHandle(1, a=blo[p], b=bhi[p])
+        S = np.zeros((N, N))
+        assert O.oracle().orc_sig_block_sph(C.byref(xi.s), cells.ctypes.data, N, nr, nmu, nphi, O.ptr(rows), N,
+                                            O.ptr(rows), N, O.ptr(S), N, q.epsrel, q.epsabs, q.maxeval) == 0
+        # band_xi forms the closed form over one division by r^3, the oracle over two
+        # (k^3 g(kr) per edge): near a node of an oscillating band the two differ by a
+        # few ulps of the edge terms, 2e-12 of the largest entry at most
+        np.testing.assert_allclose(got, S, rtol=1e-12, atol=1e-11 * np.abs(S).max(), err_msg=f"band {p}")
+
+
 def test_spherical_rejects_tent_rule(ctx):
     _shell(ctx, (2, 2, 4))
     with pytest.raises(lib.OpsecError, match="adaptive"):

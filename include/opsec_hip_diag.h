@@ -69,6 +69,11 @@ int opsec_diag_set_gm_overlap(int v);
  * bands (1, the default), or one thread per (pair, band) job (0).  The same
  * bits either way. */
 int opsec_diag_set_gm_bands(int v);
+/* Wavefront slots of the adaptive cubature's deep rerun (pools for the whole
+ * evaluation cap): 0 (the default) takes as many as half the free device memory
+ * holds, up to eight per CU (three per CU for a shell's prior sweep beside its
+ * band sweep); v > 0 caps them at v.  The tables are the same bits. */
+int opsec_diag_set_gm_deep_slots(int v);
 
 #ifdef __cplusplus
 }

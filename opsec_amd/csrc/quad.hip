@@ -1164,10 +1164,12 @@ static int g_gm_overlap = 1;
 // Shell band sweeps: first regions by gm_first_bands_kernel (1, the default) or
 // by gm_first_kernel (0); the same results.
 static int g_gm_bands = 1;
+// Wavefront slots of the rerun with whole-cap pools (0: as many as memory allows).
+static int g_gm_deep_slots = 0;
 
 static int run_gm_device(opsec_ctx* ctx, int nf, const int* funcs, const GMJob* d_tmpl, size_t npairs,
                          const opsec_quad* q, double* dQ, size_t nkeys, double* dQerr, GMScratch* gs = nullptr,
-                         hipStream_t stm = nullptr) {
+                         hipStream_t stm = nullptr, int deep_cap = 0) {
     if (!stm) stm = ctx->stream;
     XiModelDev mv = model_view(ctx);
     if (npairs == 0 || nf == 0) return OPSEC_OK;
@@ -1234,6 +1236,8 @@ static int run_gm_device(opsec_ctx* ctx, int nf, const int* funcs, const GMJob* 
             const size_t budget = std::max<size_t>((size_t)1 << 30, fre / 2);
             nslots = std::min(njobs, ctx->num_cus * 8);
             nslots = (int)std::max<size_t>(1, std::min<size_t>(nslots, budget / per_slot));
+            const int cap = g_gm_deep_slots > 0 ? g_gm_deep_slots : deep_cap;
+            if (cap > 0) nslots = std::min(nslots, cap);
         }
         int rc;
         if ((rc = dev_realloc(ctx, &sj.p, &sj.cap, sizeof(GMJob) * njobs))) return rc;
@@ -1608,7 +1612,8 @@ static int build_pair_tables_sph(opsec_ctx* ctx, int nf, const int* funcs, const
         const size_t per_row = (size_t)narcs * Nphi;   // keys of one row arc at most
         // One sweep: a group's functions over every arc pair, a range of row arcs
         // at a time, on stream s with its own template buffer tb and counter cnt.
-        auto sweep = [&](const std::vector<int>& group, hipStream_t s, Scratch& tb, unsigned long long* cnt) -> int {
+        auto sweep = [&](const std::vector<int>& group, hipStream_t s, Scratch& tb, unsigned long long* cnt,
+                         int deep_cap) -> int {
             GMScratch gs;   // the sweep's job buffers, kept across its chunks
             const int ng = (int)group.size();
             const size_t chunk = std::max<size_t>(4096, ((size_t)1 << 28) / (size_t)ng);
@@ -1630,21 +1635,24 @@ static int build_pair_tables_sph(opsec_ctx* ctx, int nf, const int* funcs, const
                 OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&npairs, cnt, 8, hipMemcpyDeviceToHost, s));
                 OPSEC_HIP_TRY(ctx, hipStreamSynchronize(s));
                 if ((rc2 = run_gm_device(ctx, ng, group.data(), (const GMJob*)tb.p, (size_t)npairs, q, ctx->d_Q,
-                                         (size_t)ctx->nkeys, nullptr, &gs, s)))
+                                         (size_t)ctx->nkeys, nullptr, &gs, s, deep_cap)))
                     return rc2;
             }
             return OPSEC_OK;
         };
         if (deep.empty() || shallow.empty() || !g_gm_overlap) {
             for (const std::vector<int>* group : {&deep, &shallow})
-                if (!group->empty() && (rc = sweep(*group, ctx->stream, st, d_count))) return rc;
+                if (!group->empty() && (rc = sweep(*group, ctx->stream, st, d_count, 0))) return rc;
             return OPSEC_OK;
         }
         // The deep sweep (few integrals, most of them run to the evaluation cap
         // by latency-bound wavefronts that leave the CUs mostly idle) runs on a
         // second stream from a second host thread, beside the shallow sweep's
         // full-device passes.  The sweeps write different functions' tables and
-        // every integral is computed alone, so the tables are the same bits.
+        // every integral is computed alone, so the tables are the same bits.  Its
+        // rerun takes at most three wavefronts per CU, so that the registers of
+        // its waiting waves leave room for the shallow sweep's (at the 65536-cell
+        // shell 8.3-8.5 s for both sweeps against 9.0 s with eight per CU).
         hipStream_t s2 = nullptr;
         hipEvent_t ready = nullptr;
         if (hipStreamCreateWithFlags(&s2, hipStreamNonBlocking) != hipSuccess ||
@@ -1661,9 +1669,9 @@ static int build_pair_tables_sph(opsec_ctx* ctx, int nf, const int* funcs, const
             if (hipSetDevice(ctx->device) != hipSuccess)
                 rc_deep = set_error(ctx, OPSEC_EHIP, "quadrature thread: hipSetDevice(%d) failed", ctx->device);
             else
-                rc_deep = sweep(deep, s2, st2, d_count + 1);
+                rc_deep = sweep(deep, s2, st2, d_count + 1, 3 * ctx->num_cus);
         });
-        rc = sweep(shallow, ctx->stream, st, d_count);
+        rc = sweep(shallow, ctx->stream, st, d_count, 0);
         worker.join();
         const hipError_t e2 = hipStreamSynchronize(s2);
         hipEventDestroy(ready);
@@ -2161,6 +2169,14 @@ int opsec_pair_integral(opsec_ctx* ctx, const opsec_cell* c1, const opsec_cell* 
 int opsec_diag_set_gm_overlap(int v) {
     if (v != 0 && v != 1) return OPSEC_EINVAL;
     g_gm_overlap = v;
+    return OPSEC_OK;
+}
+
+// Wavefront slots of the deep rerun (0: as many as memory allows, up to eight
+// per CU).
+int opsec_diag_set_gm_deep_slots(int v) {
+    if (v < 0) return OPSEC_EINVAL;
+    g_gm_deep_slots = v;
     return OPSEC_OK;
 }
 

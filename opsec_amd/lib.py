@@ -152,6 +152,7 @@ _DIAG_SIGS = {
     "opsec_diag_set_g3_dl": ([_I], _I),
     "opsec_diag_set_gm_overlap": ([_I], _I),
     "opsec_diag_set_gm_bands": ([_I], _I),
+    "opsec_diag_set_gm_deep_slots": ([_I], _I),
 }
 
 

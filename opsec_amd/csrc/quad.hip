@@ -49,6 +49,22 @@ struct GMJob {
     double lo[6], hi[6];   // integrand bounds in unit-cube coordinate order
 };
 
+// The jobs of a quadrature call: job j is template j % npairs with function
+// funcs[j / npairs] (funcs set: formed where it is read, so the nf x npairs
+// list is never written), or p[j] (funcs null: a gathered list).
+struct GMJobs {
+    const GMJob* p;
+    const int* funcs;
+    long long npairs;
+    __device__ __forceinline__ GMJob get(int j) const {
+        if (!funcs) return p[j];
+        const long long t = j / npairs;
+        GMJob J = p[j - t * npairs];
+        J.f = funcs[t];
+        return J;
+    }
+};
+
 struct GaussJob {
     int f;
     int key;
@@ -426,7 +442,7 @@ __device__ __forceinline__ bool gm_converged(double val, double err, const GMPar
 // integrals are latency-bound, so residency matters more than the few values
 // the register limit moves to scratch (11 % faster on the shell's prior).
 template <int NT, int NL>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void gm_kernel(XiModelDev m, const GMJob* jobs, const int* idx, int njobs,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void gm_kernel(XiModelDev m, GMJobs jobs, const int* idx, int njobs,
                                                 int* counter, double* dpool, int* ipool, GMParams P, double* Q,
                                                 size_t qstride, double* Qerr, int* jobfail,
                                                 unsigned long long* nevals) {
@@ -446,7 +462,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void gm
         __syncthreads();
         if (jl >= njobs) break;
         const int jb = idx ? idx[jl] : jl;
-        const GMJob J = jobs[jb];
+        const GMJob J = jobs.get(jb);
         const XiDesc xd = m.desc[J.f];
         // region 0 = [0,1]^6
         if (lane < 6) {
@@ -535,16 +551,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void gm
     }
 }
 
-// Job t * npairs + i = template i with function funcs[t].
-__global__ void expand_gm_jobs_kernel(const GMJob* __restrict__ tmpl, int npairs, const int* __restrict__ funcs,
-                                      int nf, GMJob* __restrict__ jobs) {
-    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= (long long)npairs * nf) return;
-    GMJob J = tmpl[j % npairs];
-    J.f = funcs[j / npairs];
-    jobs[j] = J;
-}
-
 // The rerun's jobs: job j (template j % npairs, function funcs[j / npairs]) of
 // the sorted list failed[0..n).
 __global__ void gather_gm_jobs_kernel(const GMJob* __restrict__ tmpl, long long npairs, const int* __restrict__ funcs,
@@ -572,7 +578,7 @@ __global__ void list_failed_kernel(const int* __restrict__ fail, const int* __re
 // with the value gm_kernel would give; the others are listed in `pending` for
 // gm_kernel.  Most band integrals end here: one region, no heap, no atomics
 // per job.
-__global__ __launch_bounds__(256) void gm_first_kernel(XiModelDev m, const GMJob* jobs, int njobs, GMParams P,
+__global__ __launch_bounds__(256) void gm_first_kernel(XiModelDev m, GMJobs jobs, int njobs, GMParams P,
                                                        double* Q, size_t qstride, double* Qerr, int* jobfail,
                                                        int* pending, int* npending, unsigned long long* nevals) {
     __shared__ unsigned long long blk;
@@ -580,7 +586,7 @@ __global__ __launch_bounds__(256) void gm_first_kernel(XiModelDev m, const GMJob
     __syncthreads();
     const int jb = blockIdx.x * blockDim.x + threadIdx.x;
     if (jb < njobs) {
-        const GMJob J = jobs[jb];
+        const GMJob J = jobs.get(jb);
         const XiDesc xd = m.desc[J.f];
         double c[6], hw[6], u[6];
         for (int d = 0; d < 6; d++) {
@@ -825,7 +831,7 @@ constexpr int GM_TR = 16;   // regions a thread keeps in private memory
 // The adaptive driver of gm_kernel's lane 0, one thread per integral, regions
 // in private memory: integrals that converge within GM_TR regions end here
 // with the same value; the others are listed in `deeper` for gm_kernel.
-__global__ __launch_bounds__(256) void gm_thread_kernel(XiModelDev m, const GMJob* jobs, const int* idx, int n,
+__global__ __launch_bounds__(256) void gm_thread_kernel(XiModelDev m, GMJobs jobs, const int* idx, int n,
                                                         GMParams P, double* Q, size_t qstride, double* Qerr,
                                                         int* jobfail, int* deeper, int* ndeeper,
                                                         unsigned long long* nevals) {
@@ -835,7 +841,7 @@ __global__ __launch_bounds__(256) void gm_thread_kernel(XiModelDev m, const GMJo
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) {
         const int jb = idx[t];
-        const GMJob J = jobs[jb];
+        const GMJob J = jobs.get(jb);
         const XiDesc xd = m.desc[J.f];
         double dreg[15 * GM_TR];
         int ireg[3 * GM_TR];
@@ -1151,9 +1157,10 @@ PairGeo grid_pair_geo(const SurveyDev& sv, int kx, int ky, int kz) {
 // dQ[f * nkeys + key].  keys[i] with geometry geo[i].
 // Genz-Malik integrals of the functions funcs[0..nf) over the npairs device
 // templates d_tmpl (f unset): the nf x npairs jobs, function-major (the prior,
-// with the most adaptive work, first), are expanded on the device.
+// with the most adaptive work, first), are formed from the templates where the
+// kernels read them (GMJobs).
 // Device buffers of run_gm_device, kept across the calls of one pair-table
-// sweep (its chunks) so the multi-GB job lists are allocated once.
+// sweep (its chunks) so the multi-GB pools and lists are allocated once.
 struct GMScratch {
     Scratch sj, sd, si, sc, sf, sp, st, sx, sa, se;
 };
@@ -1240,7 +1247,7 @@ static int run_gm_device(opsec_ctx* ctx, int nf, const int* funcs, const GMJob* 
             if (cap > 0) nslots = std::min(nslots, cap);
         }
         int rc;
-        if ((rc = dev_realloc(ctx, &sj.p, &sj.cap, sizeof(GMJob) * njobs))) return rc;
+        if (pass == 1 && (rc = dev_realloc(ctx, &sj.p, &sj.cap, sizeof(GMJob) * njobs))) return rc;
         if ((rc = dev_realloc(ctx, &sd.p, &sd.cap, sizeof(double) * 15 * (size_t)P.rmax * nslots))) return rc;
         if ((rc = dev_realloc(ctx, &si.p, &si.cap, sizeof(int) * 3 * (size_t)P.rmax * nslots))) return rc;
         if ((rc = dev_realloc(ctx, &sc.p, &sc.cap, 64))) return rc;
@@ -1255,13 +1262,12 @@ static int run_gm_device(opsec_ctx* ctx, int nf, const int* funcs, const GMJob* 
         int* d_ndeep = (int*)((char*)sc.p + 24);
         const int* d_idx = nullptr;
         int nrun = njobs;
+        GMJobs jobs{(const GMJob*)sj.p, nullptr, 0};   // the rerun's gathered jobs
         if (pass == 0) {
             if ((rc = dev_realloc(ctx, &st.p, &st.cap, sizeof(int) * nf))) return rc;
             int* dfun = (int*)st.p;
             OPSEC_HIP_TRY(ctx, hipMemcpyAsync(dfun, funcs, sizeof(int) * nf, hipMemcpyHostToDevice, stm));
-            hipLaunchKernelGGL(expand_gm_jobs_kernel, dim3((unsigned)((njobs + 255) / 256)), dim3(256), 0,
-                               stm, d_tmpl, (int)npairs, dfun, nf, (GMJob*)sj.p);
-            OPSEC_HIP_TRY(ctx, hipGetLastError());
+            jobs = GMJobs{d_tmpl, dfun, (long long)npairs};
             {
                 // first regions, then the shallow integrals one thread each
                 if (bands)
@@ -1272,8 +1278,8 @@ static int run_gm_device(opsec_ctx* ctx, int nf, const int* funcs, const GMJob* 
                                        (int*)sf.p, (int*)sx.p, d_npend, d_nev);
                 else
                     hipLaunchKernelGGL(gm_first_kernel, dim3((unsigned)((njobs + 255) / 256)), dim3(256), 0,
-                                       stm, mv, (const GMJob*)sj.p, njobs, P, dQ, nkeys, dQerr, (int*)sf.p,
-                                       (int*)sx.p, d_npend, d_nev);
+                                       stm, mv, jobs, njobs, P, dQ, nkeys, dQerr, (int*)sf.p, (int*)sx.p, d_npend,
+                                       d_nev);
                 OPSEC_HIP_TRY(ctx, hipGetLastError());
                 int npend = 0;
                 OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&npend, d_npend, 4, hipMemcpyDeviceToHost, stm));
@@ -1281,7 +1287,7 @@ static int run_gm_device(opsec_ctx* ctx, int nf, const int* funcs, const GMJob* 
                 nrun = 0;
                 if (npend > 0) {
                     hipLaunchKernelGGL(gm_thread_kernel, dim3((unsigned)((npend + 255) / 256)), dim3(256), 0,
-                                       stm, mv, (const GMJob*)sj.p, (const int*)sx.p, npend, P, dQ, nkeys,
+                                       stm, mv, jobs, (const int*)sx.p, npend, P, dQ, nkeys,
                                        dQerr, (int*)sf.p, (int*)sp.p, d_ndeep, d_nev);
                     OPSEC_HIP_TRY(ctx, hipGetLastError());
                     OPSEC_HIP_TRY(ctx, hipMemcpyAsync(&nrun, d_ndeep, 4, hipMemcpyDeviceToHost, stm));
@@ -1295,19 +1301,19 @@ static int run_gm_device(opsec_ctx* ctx, int nf, const int* funcs, const GMJob* 
         if (nrun > 0) {
             if (pass == 0)
                 hipLaunchKernelGGL((gm_kernel<64, 0>), dim3(std::min(nslots, nrun)), dim3(64), 0, stm, mv,
-                                   (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
+                                   jobs, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                    nkeys, dQerr, (int*)sf.p, d_nev);
             else if (nrun <= 2 * ctx->num_cus)   // few deep integrals: 8 waves each
                 hipLaunchKernelGGL((gm_kernel<512, 8192>), dim3(std::min(nslots, nrun)), dim3(512), 0, stm, mv,
-                                   (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
+                                   jobs, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                    nkeys, dQerr, (int*)sf.p, d_nev);
             else if (nrun <= 4 * ctx->num_cus)
                 hipLaunchKernelGGL((gm_kernel<256, 1024>), dim3(std::min(nslots, nrun)), dim3(256), 0, stm, mv,
-                                   (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
+                                   jobs, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                    nkeys, dQerr, (int*)sf.p, d_nev);
             else   // many: one wave each, all resident at once (two per SIMD)
                 hipLaunchKernelGGL((gm_kernel<64, 0>), dim3(std::min(nslots, nrun)), dim3(64), 0, stm, mv,
-                                   (const GMJob*)sj.p, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
+                                   jobs, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                    nkeys, dQerr, (int*)sf.p, d_nev);
             OPSEC_HIP_TRY(ctx, hipGetLastError());
         }
@@ -1364,8 +1370,8 @@ static int run_quadrature(opsec_ctx* ctx, int nf, const int* funcs, const std::v
     int method = q->method;
     if (method == OPSEC_QUAD_GM) {
         // one template per pair (f unset); the nf x npairs jobs, function-major
-        // (the prior, with the most adaptive work, first), are expanded on the
-        // device
+        // (the prior, with the most adaptive work, first), are formed from them on
+        // the device
         std::vector<GMJob> tmpl(npairs);
         for (size_t i = 0; i < npairs; i++) {
             GMJob& J = tmpl[i];

@@ -1671,14 +1671,21 @@ static int build_pair_tables_sph(opsec_ctx* ctx, int nf, const int* funcs, const
         }
         Scratch st2;
         int rc_deep = OPSEC_OK;
-        std::thread worker([&] {
-            if (hipSetDevice(ctx->device) != hipSuccess)
-                rc_deep = set_error(ctx, OPSEC_EHIP, "quadrature thread: hipSetDevice(%d) failed", ctx->device);
-            else
-                rc_deep = sweep(deep, s2, st2, d_count + 1, 3 * ctx->num_cus);
-        });
+        std::thread worker;
+        try {
+            worker = std::thread([&] {
+                if (hipSetDevice(ctx->device) != hipSuccess)
+                    rc_deep = set_error(ctx, OPSEC_EHIP, "quadrature thread: hipSetDevice(%d) failed", ctx->device);
+                else
+                    rc_deep = sweep(deep, s2, st2, d_count + 1, 3 * ctx->num_cus);
+            });
+        } catch (...) {   // no thread: the deep sweep runs after the shallow one, on s2
+        }
         rc = sweep(shallow, ctx->stream, st, d_count, 0);
-        worker.join();
+        if (worker.joinable())
+            worker.join();
+        else if (!rc)
+            rc_deep = sweep(deep, s2, st2, d_count + 1, 0);
         const hipError_t e2 = hipStreamSynchronize(s2);
         hipEventDestroy(ready);
         hipStreamDestroy(s2);

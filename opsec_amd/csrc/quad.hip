@@ -438,9 +438,9 @@ __device__ __forceinline__ bool gm_converged(double val, double err, const GMPar
 // integrals, whose split batches are short and follow one another, up to 512
 // when there are few of them, so the nodes of a batch are spread over more
 // lanes while all of them stay resident.
-// NL > 0: the first NL heap positions in LDS.  Two waves per SIMD: these
-// integrals are latency-bound, so residency matters more than the few values
-// the register limit moves to scratch (11 % faster on the shell's prior).
+// NL > 0: the first NL heap positions in LDS.  A register limit of 256 per
+// wave (two waves per SIMD by registers; the LDS of the 64-thread variant holds
+// five per CU): 11 % faster on the shell's prior than 262 registers.
 template <int NT, int NL>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void gm_kernel(XiModelDev m, GMJobs jobs, const int* idx, int njobs,
                                                 int* counter, double* dpool, int* ipool, GMParams P, double* Q,
@@ -1311,7 +1311,7 @@ static int run_gm_device(opsec_ctx* ctx, int nf, const int* funcs, const GMJob* 
                 hipLaunchKernelGGL((gm_kernel<256, 1024>), dim3(std::min(nslots, nrun)), dim3(256), 0, stm, mv,
                                    jobs, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                    nkeys, dQerr, (int*)sf.p, d_nev);
-            else   // many: one wave each, all resident at once (two per SIMD)
+            else   // many: one wave each
                 hipLaunchKernelGGL((gm_kernel<64, 0>), dim3(std::min(nslots, nrun)), dim3(64), 0, stm, mv,
                                    jobs, d_idx, nrun, d_counter, (double*)sd.p, (int*)si.p, P, dQ,
                                    nkeys, dQerr, (int*)sf.p, d_nev);

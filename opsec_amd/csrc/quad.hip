@@ -658,6 +658,9 @@ __global__ __launch_bounds__(256) void gm_first_kernel(XiModelDev m, GMJobs jobs
 // slot q; the bands go through LDS GMB_CHUNK at a time.
 constexpr int GMB_PAIRS = 3, GMB_CHUNK = 8;
 constexpr int GMB_THREADS = 448;   // >= GMB_PAIRS * GM_NPTS, whole wavefronts
+static_assert(GMB_THREADS >= GMB_PAIRS * GM_NPTS && GMB_THREADS % 64 == 0 &&
+                  GMB_PAIRS * GMB_CHUNK <= GMB_THREADS,
+              "a node thread per pair node, a summing thread per chunk band");
 __global__ __launch_bounds__(GMB_THREADS) void gm_first_bands_kernel(
     XiModelDev m, const GMJob* __restrict__ tmpl, int npairs, const int* __restrict__ funcs, int nf,
     const double* __restrict__ edge, const int* __restrict__ fedge, GMParams P, double* Q, size_t qstride,

@@ -40,7 +40,7 @@ PEAK_TF = {"fp64": 78.6, "fp32": 157.3, "bf16": 2516.6}
 # Keys: (config, projection, kernel).
 TRAFFIC_BYTES = {("C2", "dense", "stage1"): (18874466 * 2.0 + 883503 * 1.0) * 1024,
                  ("C2", "toeplitz", "stage1"): (1321387 * 2.0 + 1059234 * 1.0) * 1024,
-                 ("C2", "toeplitz", "stage2"): (4895735 * 2.0 + 1169949 * 1.0) * 1024}
+                 ("C2", "toeplitz", "stage2"): (12218166 * 2.0 + 1150502 * 1.0) * 1024}
 
 
 def log(*a):
@@ -498,6 +498,8 @@ def main():
                          "box, 2 Nx slab GEMMs; along phi of the shell, 2 Nphi arc GEMMs); the FP64 path is "
                          "dense (the reference's algorithm), timed with Toeplitz too as a comparison")
     ap.add_argument("--s2-split", type=int, default=8, help="split-K factor of the split-BF16x3 stage 2")
+    ap.add_argument("--s2-split-toep", type=int, default=4,
+                    help="split-K factor of the split-BF16x3 Toeplitz stage 2 (eight bands per launch)")
     ap.add_argument("--nt-group", type=int, default=8, help="split-BF16x3 GEMM tile-group edge (stage 1)")
     ap.add_argument("--s2-group", type=int, default=2, help="split-BF16x3 stage-2 tile-group edge")
     ap.add_argument("--nt-remap", type=int, default=0, choices=[0, 1], help="deal tile groups to XCDs")
@@ -534,6 +536,7 @@ def main():
     ctx.lib.opsec_diag_set_nt_order(args.nt_group, args.nt_remap)
     ctx.lib.opsec_diag_set_nt_gen(args.nt_gen)
     ctx.lib.opsec_diag_set_stage2_split(args.s2_split)
+    ctx.lib.opsec_diag_set_toeplitz_stage2_split(args.s2_split_toep)
     ctx.lib.opsec_diag_set_stage2_group(args.s2_group)
     ctx.lib.opsec_diag_set_g3_dl(args.g3_dl)
     ctx.set_projection(L.OPSEC_PROJ_TOEPLITZ if args.projection == "toeplitz" else L.OPSEC_PROJ_DENSE)

@@ -46,11 +46,12 @@ int opsec_diag_project_launches(opsec_ctx* ctx, int* stage1, int* stage2);
 /* Split-BF16x3 GEMM variants: tile (0: 128 x 128, 1: 256 x 128, 2: 256 x 256
  * double-buffered, the default), stage-1 A operand generated from the pair
  * table (1) or read from S_p planes (0, default), stage-2 split-K factor
- * (default 8) and tile-group edge (default 2), stage-1 tile order (group
- * edge, XCD remap). */
+ * (default 8; the Toeplitz path's, eight bands per launch, default 4) and
+ * tile-group edge (default 2), stage-1 tile order (group edge, XCD remap). */
 int opsec_diag_set_nt_tile(int v);
 int opsec_diag_set_nt_gen(int v);
 int opsec_diag_set_stage2_split(int v);
+int opsec_diag_set_toeplitz_stage2_split(int v);
 int opsec_diag_set_stage2_group(int v);
 int opsec_diag_set_nt_order(int group, int remap);
 /* Operand tiles of the 256 x 256 split-BF16x3 GEMM (A from plane pairs):

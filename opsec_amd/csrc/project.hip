@@ -1405,6 +1405,8 @@ static int g_nt_gen = 0;
 // Split-K factor of stage 2 on the 256 x 256 kernel, and its tile-group edge
 // (2 measured best for stage 2; 8 for the stage-1 launches).
 static int g_s2_split = 8;
+// split-K factor of the Toeplitz path's stage 2, which runs eight bands per launch
+static int g_s2_split_toep = 4;
 static int g_s2_group = 2;
 
 // Operand tiles of the 256 x 256 GEMM (A from plane pairs) into LDS by DMA
@@ -1590,7 +1592,8 @@ static int run_project_bf16x3(opsec_ctx* ctx, int nb, const int* funcs) {
     return OPSEC_OK;
 }
 
-constexpr int TOEP_GROUP = 4;   // bands per stage-2 launch of the Toeplitz path
+constexpr int TOEP_GROUP = 8;     // bands per stage-2 launch of the split-BF16x3 Toeplitz path
+constexpr int TOEP64_GROUP = 4;   // bands per stage-2 launch of the FP64 / FP32 Toeplitz path
 
 // cos / sin of 2 pi j / L, exact at the quarter points
 static void trig_table(int L, std::vector<double>& t) {
@@ -1661,9 +1664,10 @@ static int run_project_toeplitz(opsec_ctx* ctx, int nb, const int* funcs) {
         return rc;
     if ((rc = dev_realloc(ctx, (void**)&ps->d_Yhat, &ps->cap_Yhat, (size_t)M * nbat * G * 4))) return rc;
     if ((rc = dev_realloc(ctx, &ps->d_P, &ps->cap_P, (size_t)nb * packed_size(M) * 8))) return rc;
-    const int s2 = std::max(1, std::min(g_s2_split, (int)(Kg / PK)));
-    // stage 2 runs once per group of up to TOEP_GROUP bands: the 136 upper tiles x
-    // split-K parts of one band fill 4.25 rounds of the CUs, four bands 17
+    const int s2 = std::max(1, std::min(g_s2_split_toep, (int)(Kg / PK)));
+    // stage 2 runs once per group of up to TOEP_GROUP bands: the 136 upper tiles x 4
+    // split-K parts of one band fill 2.1 rounds of the CUs, eight bands 17 (against
+    // four bands x 8 parts: the same tiles, half the partial sums, 1.1 % faster steps)
     const int grp = std::min(nb, TOEP_GROUP);
     if ((rc = dev_realloc(ctx, (void**)&ps->d_Ppart, &ps->cap_Ppart, (size_t)grp * s2 * packed_size(M) * 4)))
         return rc;
@@ -1808,9 +1812,9 @@ static int run_project_toeplitz64(opsec_ctx* ctx, int nb, const int* funcs, int 
     if (!tgrid.sph && (sv.n2 >= 65536 || sv.n3 >= 65536))
         return set_error(ctx, OPSEC_EINVAL, "Toeplitz projection: Ny, Nz < 65536");
     const size_t ldg = (size_t)Ng + 32;   // pitch of the grid-order operands (not a power of two)
-    // stage 2 runs once per group of up to TOEP_GROUP bands (grid.z = band): the upper
+    // stage 2 runs once per group of up to TOEP64_GROUP bands (grid.z = band): the upper
     // tiles of one band alone fill the CUs unevenly (528 at 4000 modes: 2.06 rounds of 256)
-    const int grp = std::min(nb, TOEP_GROUP);
+    const int grp = std::min(nb, TOEP64_GROUP);
     const size_t ystride = (size_t)M * ldg;
     if ((rc = dev_realloc(ctx, (void**)&ps->d_Yhat64, &ps->cap_Yhat64, (size_t)nbat * M * G * 8))) return rc;
     if ((rc = dev_realloc(ctx, (void**)&ps->d_Y64, &ps->cap_Y64, (size_t)grp * ystride * 8))) return rc;
@@ -2191,6 +2195,14 @@ int opsec_diag_set_nt_tile(int v) {
 int opsec_diag_set_stage2_split(int v) {
     if (v < 1 || v > 16) return OPSEC_EINVAL;
     g_s2_split = v;
+    return OPSEC_OK;
+}
+
+// Split-K factor of the Toeplitz path's split-BF16x3 stage 2 (eight bands per
+// launch), for tuning.
+int opsec_diag_set_toeplitz_stage2_split(int v) {
+    if (v < 1 || v > 16) return OPSEC_EINVAL;
+    g_s2_split_toep = v;
     return OPSEC_OK;
 }
 

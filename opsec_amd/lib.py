@@ -147,6 +147,7 @@ _DIAG_SIGS = {
     "opsec_diag_set_nt_tile": ([_I], _I),
     "opsec_diag_set_nt_gen": ([_I], _I),
     "opsec_diag_set_stage2_split": ([_I], _I),
+    "opsec_diag_set_toeplitz_stage2_split": ([_I], _I),
     "opsec_diag_set_stage2_group": ([_I], _I),
     "opsec_diag_set_nt_order": ([_I, _I], _I),
     "opsec_diag_set_g3_dl": ([_I], _I),

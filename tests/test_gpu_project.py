@@ -230,6 +230,7 @@ def test_project_bf16x3_gemm_variants(ctx, tile, gen, dl):
         assert ctx.lib.opsec_diag_set_nt_tile(tile) == 0
         assert ctx.lib.opsec_diag_set_nt_gen(gen) == 0
         assert ctx.lib.opsec_diag_set_stage2_split(3) == 0
+        assert ctx.lib.opsec_diag_set_toeplitz_stage2_split(3) == 0
         assert ctx.lib.opsec_diag_set_g3_dl(dl) == 0
         got = ctx.project(funcs, lib.OPSEC_PREC_BF16X3)
         ctx.set_projection(lib.OPSEC_PROJ_TOEPLITZ)
@@ -239,6 +240,7 @@ def test_project_bf16x3_gemm_variants(ctx, tile, gen, dl):
         ctx.lib.opsec_diag_set_nt_tile(2)
         ctx.lib.opsec_diag_set_nt_gen(0)
         ctx.lib.opsec_diag_set_stage2_split(8)
+        ctx.lib.opsec_diag_set_toeplitz_stage2_split(4)
         ctx.lib.opsec_diag_set_g3_dl(2)
     for name, g in (("dense", got), ("toeplitz", got_t)):
         err = np.abs(g - ref).max(axis=1) / np.abs(ref).max(axis=1)
